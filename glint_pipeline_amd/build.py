@@ -1,0 +1,69 @@
+"""Build the native libraries in-tree.
+
+- ``libglint_gen.so``: the host-side seeded trace generator (gcc).
+- ``libglint.so``: the HIP kernels and the C-ABI declared in include/glint.h
+  (hipcc, --offload-arch=gfx950).
+
+Every HIP object is compiled with packed-FP32 instruction selection disabled and
+floating-point contraction off, so the FP64 distance tests round exactly as the
+reference's expressions are written.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+ROOT = PKG.parent
+CSRC = PKG / "csrc"
+INCLUDE = ROOT / "include"
+
+HIP_SOURCES = ["glint.hip", "walk.hip"]
+HIP_HEADERS = ["glint_device.h"]
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("GLINT_OFFLOAD_ARCH", "gfx950")
+
+
+def _run(cmd: list[str]) -> None:
+    print("+", " ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+
+
+def build_gen() -> Path:
+    out = PKG / "libglint_gen.so"
+    _run(["gcc", "-O3", "-fPIC", "-shared", "-std=c11", "-ffp-contract=off", "-fno-fast-math",
+          "-fopenmp", "-Wall", "-Wextra", f"-I{INCLUDE}", "-o", str(out), str(CSRC / "gen.c"), "-lm"])
+    return out
+
+
+def build_hip() -> Path:
+    out = PKG / "libglint.so"
+    tmp = PKG / "libglint.so.tmp"
+    bdir = PKG / "_build"
+    bdir.mkdir(exist_ok=True)
+    gen_obj = bdir / "gen.o"
+    _run(["gcc", "-O3", "-fPIC", "-c", "-std=c11", "-ffp-contract=off", "-fno-fast-math", "-fopenmp",
+          f"-I{INCLUDE}", "-o", str(gen_obj), str(CSRC / "gen.c")])
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
+           "-ffp-contract=off", "-fno-fast-math",
+           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+           "-Wall", "-Wno-unused-function", f"-I{INCLUDE}", f"-I{CSRC}",
+           "-o", str(tmp)]
+    cmd += [str(CSRC / s) for s in HIP_SOURCES] + ["-x", "none", str(gen_obj), "-fopenmp"]
+    _run(cmd)
+    os.replace(tmp, out)
+    return out
+
+
+def build_all() -> None:
+    build_gen()
+    build_hip()
+
+
+if __name__ == "__main__":
+    what = sys.argv[1:] or ["all"]
+    for w in what:
+        {"gen": build_gen, "hip": build_hip, "all": build_all}[w]()

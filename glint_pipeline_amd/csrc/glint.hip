@@ -1,0 +1,1519 @@
+// Contact-tracing path for MI355X (gfx950): partition -> refine -> evaluate
+// -> schema update, behind the C ABI of include/glint.h.
+//
+// Reference behaviour restated here (paths relative to the reference repo):
+//   descent and last_valid        src/tracing/filter.cpp:229-243, geometry.cu:234-249
+//   lookup rule                   src/tracing/filter.cpp:130-152, geometry.cu:350-386
+//   distance test                 src/geometry/geometry.h:40-48, refinement.cpp:30-33
+//   meeting rule                  src/tracing/meeting.cpp:134-155
+//   schema collect/split/merge    src/cuda/geometry.cu:1114-1234
+//   initial build                 src/index/QTree.h:157-214
+//
+// Compiled with -ffp-contract=off: every FP64 expression rounds as written,
+// so the descent, the box distances and the squared distances are bit-equal
+// to the reference's.  sqrt(s) <= r is evaluated as s <= t64, where t64 is
+// the largest double whose correctly rounded square root is <= r (computed
+// on the host), which is the same predicate without a device sqrt.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "glint.h"
+#include "glint_device.h"
+
+using namespace glint;
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kScanItems = 4;                       // per thread
+constexpr int kScanTile = kBlock * kScanItems;      // elements per scan block
+constexpr int kJoinTile = 1024;                     // points per LDS tile
+constexpr int kLookupStack = 128;
+constexpr uint32_t kInsideChunk = 4;                // leaves per work-queue grab
+
+// ------------------------------------------------------------ predicates
+
+__device__ __forceinline__ bool box_within(const Node &b, double px, double py, double t64)
+{
+    // box::distance (geometry.h:174-185) <= r, with contain() first.
+    if (px >= b.lx && px <= b.hx && py >= b.ly && py <= b.hy) return true;
+    double dx = fabs(px - (b.lx + b.hx) / 2) - (b.hx - b.lx) / 2;
+    double dy = fabs(py - (b.ly + b.hy) / 2) - (b.hy - b.ly) / 2;
+    dx = dx < 0.0 ? 0.0 : dx;
+    dy = dy < 0.0 ? 0.0 : dy;
+    double s = dx * dx + dy * dy;
+    return s <= t64;
+}
+
+__device__ __forceinline__ bool point_within64(double x1, double y1, double x2, double y2, double t64)
+{
+    // Point::distance (geometry.h:40-48) <= r
+    double dx = x1 - x2;
+    double dy = y1 - y2;
+    double s = dx * dx + dy * dy;
+    return s <= t64;
+}
+
+// ------------------------------------------------------------ small kernels
+
+__global__ void k_fill_u32(uint32_t *p, uint32_t v, uint64_t n)
+{
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+__global__ void k_fill_slots(Slot *p, uint64_t n)
+{
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        Slot s;
+        s.key = kEmptyKey;
+        s.start = 0;
+        s.seen = 0;
+        p[i] = s;
+    }
+}
+
+__global__ void k_set_tick(TickParams *tp, const double *xy, Slot *tab_old, Slot *tab_cur, uint32_t tick,
+                           uint32_t prev_tick)
+{
+    tp->xy = xy;
+    tp->tab_old = tab_old;
+    tp->tab_cur = tab_cur;
+    tp->tick = tick;
+    tp->prev_tick = prev_tick;
+}
+
+// Next tick of a device-resident trace: positions move one snapshot on and
+// the table generations swap (tab_cur of the last tick becomes tab_old).
+__global__ void k_advance_tick(TickParams *tp, uint64_t stride)
+{
+    uint32_t k = tp->trace_index + 1;
+    tp->trace_index = k;
+    tp->xy = tp->trace_base + stride * k;
+    tp->prev_tick = tp->tick;
+    tp->tick = tp->tick + 1;
+}
+
+__global__ void k_begin_trace(TickParams *tp, const double *base, Slot *tab_old, Slot *tab_cur, uint32_t tick,
+                              uint32_t prev_tick)
+{
+    tp->trace_base = base;
+    tp->trace_index = 0;
+    tp->xy = base;
+    tp->tab_old = tab_old;
+    tp->tab_cur = tab_cur;
+    tp->tick = tick;
+    tp->prev_tick = prev_tick;
+}
+
+__global__ void k_swap_tables(TickParams *tp)
+{
+    Slot *t = tp->tab_old;
+    tp->tab_old = tp->tab_cur;
+    tp->tab_cur = t;
+}
+
+__global__ void k_reset_tick(DevState s)
+{
+    Counters *c = s.ctr;
+    c->n_units = 0;
+    c->n_contacts = 0;
+    c->n_inserted = 0;
+    c->n_meet = 0;
+    c->n_rechecks = 0;
+    c->n_rec = 0;
+    c->work_border = 0;
+    c->n_split = 0;
+    c->n_merge = 0;
+    c->n_border = 0;
+    c->n_leaf_list = 0;
+    c->n_leaves = 0;
+    c->max_leaf = 0;
+    c->work_inside = 0;
+}
+
+// ------------------------------------------------------------ tree build
+
+__global__ void k_init_tree(DevState s)
+{
+    uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t k = gid; k < s.node_cap; k += stride) {
+        // free stack holds nodes 1..node_cap-1; pops return 1, 2, 3, ...
+        if (k + 1 < s.node_cap) s.free_stack[k] = (int32_t)(s.node_cap - 1 - k);
+        NodeMeta m;
+        m.type = kInvalid;
+        m.level = 0;
+        m.overflow = 0;
+        m.underflow = 0;
+        s.meta[k] = m;
+    }
+    if (gid == 0) {
+        Node root;
+        root.lx = s.lx;
+        root.ly = s.ly;
+        root.hx = s.hx;
+        root.hy = s.hy;
+        root.mx = (s.hx + s.lx) / 2;   // QTree.h:103-104
+        root.my = (s.hy + s.ly) / 2;
+        root.child[0] = kLeafTag;
+        root.child[1] = root.child[2] = root.child[3] = 0;
+        s.nodes[0] = root;
+        NodeMeta m;
+        m.type = kLeaf;
+        m.level = 0;
+        m.overflow = 0;
+        m.underflow = 0;
+        s.meta[0] = m;
+        s.ctr->free_top = (int32_t)s.node_cap - 1;
+        s.ctr->error = 0;
+    }
+}
+
+__device__ __forceinline__ uint32_t descend_to_leaf(const Node *nodes, double x, double y)
+{
+    uint32_t cur = 0;
+    Node nd = nodes[0];
+    while (nd.child[0] != kLeafTag) {
+        int c = (y > nd.my) * 2 + (x > nd.mx);
+        cur = nd.child[c];
+        nd = nodes[cur];
+    }
+    return cur;
+}
+
+__global__ void k_build_count(DevState s)
+{
+    const double *xy = s.tp->xy;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < s.n; i += gridDim.x * blockDim.x) {
+        double x = xy[2 * (size_t)i], y = xy[2 * (size_t)i + 1];
+        if (!(x >= s.lx && x <= s.hx && y >= s.ly && y <= s.hy)) {
+            atomicOr(&s.ctr->error, kErrBadInput);
+            continue;
+        }
+        atomicAdd(&s.count[descend_to_leaf(s.nodes, x, y)], 1u);
+    }
+}
+
+// QTNode::split (QTree.h:157-160): count >= cap && width/2 > r.
+__global__ void k_build_collect(DevState s)
+{
+    for (uint32_t n = blockIdx.x * blockDim.x + threadIdx.x; n < s.node_cap; n += gridDim.x * blockDim.x) {
+        bool want = false;
+        if (s.meta[n].type == kLeaf) {
+            Node nd = s.nodes[n];
+            want = s.count[n] >= s.cap && (nd.hx - nd.lx) / 2 > s.r;
+        }
+        uint32_t idx = wave_append_u32(&s.ctr->n_split, want);
+        if (want) s.split_list[idx] = n;
+    }
+}
+
+// Split the listed leaves one level.  build_boxes selects the child boxes of
+// the initial build, (lo, mid) and (mid, hi) (QTree.h:163-166); otherwise the
+// elastic update's lo + half and mid + half (geometry.cu:1151-1156).
+__global__ void k_split(DevState s, int build_boxes)
+{
+    uint32_t ns = s.ctr->n_split;
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < ns; k += gridDim.x * blockDim.x) {
+        uint32_t node = s.split_list[k];
+        int32_t top = atomicSub(&s.ctr->free_top, 4);
+        if (top < 4) {
+            atomicAdd(&s.ctr->free_top, 4);   // give the entries back: the pool never goes negative
+            atomicOr(&s.ctr->error, kErrNodeCap);
+            continue;
+        }
+        Node p = s.nodes[node];
+        NodeMeta pm = s.meta[node];
+        double xhalf = p.mx - p.lx;
+        double yhalf = p.my - p.ly;
+        Node np = p;
+        for (int i = 0; i < 4; i++) {
+            uint32_t c = (uint32_t)s.free_stack[top - 1 - i];
+            Node ch;
+            if (build_boxes) {
+                ch.lx = (i & 1) ? p.mx : p.lx;
+                ch.hx = (i & 1) ? p.hx : p.mx;
+                ch.ly = (i & 2) ? p.my : p.ly;
+                ch.hy = (i & 2) ? p.hy : p.my;
+            } else {
+                ch.lx = p.lx + (double)(i % 2 == 1) * xhalf;
+                ch.ly = p.ly + (double)(i / 2 == 1) * yhalf;
+                ch.hx = p.mx + (double)(i % 2 == 1) * xhalf;
+                ch.hy = p.my + (double)(i / 2 == 1) * yhalf;
+            }
+            ch.mx = (ch.lx + ch.hx) / 2;
+            ch.my = (ch.ly + ch.hy) / 2;
+            ch.child[0] = kLeafTag;
+            ch.child[1] = ch.child[2] = ch.child[3] = 0;
+            s.nodes[c] = ch;
+            NodeMeta m;
+            m.type = kLeaf;
+            m.level = pm.level + 1;
+            m.overflow = 0;
+            m.underflow = 0;
+            s.meta[c] = m;
+            np.child[i] = c;
+        }
+        s.nodes[node] = np;
+        pm.type = kBranch;
+        s.meta[node] = pm;
+    }
+}
+
+// geometry.cu:1166-1190: the four leaf children return to the pool.
+__global__ void k_merge(DevState s)
+{
+    uint32_t nm = s.ctr->n_merge;
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nm; k += gridDim.x * blockDim.x) {
+        uint32_t node = s.merge_list[k];
+        Node p = s.nodes[node];
+        int32_t top = atomicAdd(&s.ctr->free_top, 4);
+        if (top < 0 || top + 4 > (int32_t)s.node_cap) {
+            atomicOr(&s.ctr->error, kErrNodeCap);
+            continue;
+        }
+        for (int i = 0; i < 4; i++) {
+            uint32_t c = p.child[i];
+            s.meta[c].type = kInvalid;
+            s.free_stack[top + i] = (int32_t)c;
+        }
+        p.child[0] = kLeafTag;
+        p.child[1] = p.child[2] = p.child[3] = 0;
+        s.nodes[node] = p;
+        NodeMeta pm = s.meta[node];
+        pm.type = kLeaf;
+        pm.overflow = 0;
+        s.meta[node] = pm;
+    }
+}
+
+// geometry.cu:1193-1234: decisions from this tick's counts, before any change.
+__global__ void k_collect(DevState s)
+{
+    for (uint32_t n = blockIdx.x * blockDim.x + threadIdx.x; n < s.node_cap; n += gridDim.x * blockDim.x) {
+        NodeMeta m = s.meta[n];
+        bool split = false, merge = false;
+        if (m.type == kLeaf) {
+            Node nd = s.nodes[n];
+            if (nd.hy - nd.ly > 2 * s.r && s.count[n] > s.cap) {
+                if (++m.overflow >= s.delay) {
+                    split = true;
+                    m.overflow = 0;
+                }
+            } else {
+                m.overflow = 0;
+            }
+            s.meta[n] = m;
+        } else if (m.type == kBranch) {
+            Node nd = s.nodes[n];
+            int leafchild = 0;
+            int ncounter = 0;
+            for (int i = 0; i < 4; i++) {
+                uint32_t c = nd.child[i];
+                if (s.meta[c].type == kLeaf) {
+                    leafchild++;
+                    ncounter += (int)s.count[c];
+                }
+            }
+            if (leafchild == 4 && ncounter < (int)s.cap) {
+                if (++m.underflow >= s.delay) {
+                    merge = true;
+                    m.underflow = 0;
+                }
+            } else {
+                m.underflow = 0;
+            }
+            s.meta[n] = m;
+        }
+        uint32_t is = wave_append_u32(&s.ctr->n_split, split);
+        if (split) s.split_list[is] = n;
+        uint32_t im = wave_append_u32(&s.ctr->n_merge, merge);
+        if (merge) s.merge_list[im] = n;
+    }
+}
+
+__global__ void k_schema_done(DevState s)
+{
+    s.ctr->last_splits = s.ctr->n_split;
+    s.ctr->last_merges = s.ctr->n_merge;
+}
+
+// ------------------------------------------------------------ partition
+
+// Descent with last_valid (filter.cpp:229-243): last_valid is the deepest
+// node on the path, below the root, with x + r < high.x && y + r < high.y.
+__global__ __launch_bounds__(kBlock) void k_assign(DevState s)
+{
+    const double *xy = s.tp->xy;
+    const double r = s.r;
+    const bool root_leaf = s.nodes[0].child[0] == kLeafTag;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < s.n; base += gridDim.x * blockDim.x) {
+        uint32_t i = base + threadIdx.x;
+        bool active = i < s.n;
+        bool border = false;
+        uint32_t lv = 0, cur = 0;
+        if (active) {
+            double x = xy[2 * (size_t)i], y = xy[2 * (size_t)i + 1];
+            if (!(x >= s.lx && x <= s.hx && y >= s.ly && y <= s.hy)) {
+                atomicOr(&s.ctr->error, kErrBadInput);
+                s.leaf_of[i] = kNone;
+                active = false;
+            } else {
+                Node nd = s.nodes[0];
+                while (nd.child[0] != kLeafTag) {
+                    int c = (y > nd.my) * 2 + (x > nd.mx);
+                    cur = nd.child[c];
+                    nd = s.nodes[cur];
+                    if (x + r < nd.hx && y + r < nd.hy) lv = cur;
+                }
+                uint32_t rank = atomicAdd(&s.count[cur], 1u);
+                s.leaf_of[i] = cur;
+                s.rank_of[i] = rank;
+                border = s.complete ? !root_leaf : (lv != cur);
+                if (s.complete) lv = 0;
+            }
+        }
+        uint32_t idx = wave_append_u32(&s.ctr->n_border, border);
+        if (border) s.border[idx] = make_uint2(i, lv);
+    }
+}
+
+__device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t x)
+{
+    uint32_t lane = lane_id();
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t y = __shfl_up(x, d);
+        if (lane >= (uint32_t)d) x += y;
+    }
+    return x;
+}
+
+// Exclusive scan of a 256-thread block; returns the prefix of this thread.
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *total)
+{
+    __shared__ uint32_t wsum[kBlock / 64];
+    uint32_t w = threadIdx.x / 64;
+    uint32_t inc = wave_inclusive_scan(v);
+    if (lane_id() == 63) wsum[w] = inc;
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+    for (uint32_t k = 0; k < kBlock / 64; k++) {
+        if (k < w) before += wsum[k];
+        all += wsum[k];
+    }
+    __syncthreads();
+    *total = all;
+    return before + inc - v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_reduce(DevState s)
+{
+    uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    uint32_t v = 0;
+    for (int k = 0; k < kScanItems; k++)
+        if (base + k < s.node_cap) v += s.count[base + k];
+    uint32_t total;
+    block_exclusive_scan(v, &total);
+    if (threadIdx.x == 0) s.block_sums[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_top(DevState s, uint32_t nblocks)
+{
+    uint32_t carry = 0;
+    for (uint32_t b0 = 0; b0 < nblocks; b0 += kBlock) {
+        uint32_t b = b0 + threadIdx.x;
+        uint32_t v = b < nblocks ? s.block_sums[b] : 0;
+        uint32_t total;
+        uint32_t ex = block_exclusive_scan(v, &total);
+        if (b < nblocks) s.block_sums[b] = carry + ex;
+        carry += total;
+    }
+}
+
+// Leaf offsets in node-index order, plus the list of leaves with >= 2
+// objects (the inside-join work) and the leaf statistics.
+__global__ __launch_bounds__(kBlock) void k_scan_down(DevState s)
+{
+    uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    uint32_t c[kScanItems];
+    uint32_t v = 0;
+    for (int k = 0; k < kScanItems; k++) {
+        c[k] = base + k < s.node_cap ? s.count[base + k] : 0;
+        v += c[k];
+    }
+    uint32_t total;
+    uint32_t ex = block_exclusive_scan(v, &total) + s.block_sums[blockIdx.x];
+    uint32_t mx = 0, nleaf = 0;
+    for (int k = 0; k < kScanItems; k++) {
+        uint32_t n = base + k;
+        bool in = n < s.node_cap;
+        if (in) s.offset[n] = ex;
+        ex += c[k];
+        mx = max(mx, c[k]);
+        nleaf += (in && s.meta[n].type == kLeaf) ? 1u : 0u;
+        bool work = in && c[k] >= 2;
+        uint32_t idx = wave_append_u32(&s.ctr->n_leaf_list, work);
+        if (work) s.leaf_list[idx] = n;
+    }
+    // wave reductions, then one atomic per wave
+    for (int d = 32; d >= 1; d >>= 1) {
+        mx = max(mx, (uint32_t)__shfl_xor(mx, d));
+        nleaf += (uint32_t)__shfl_xor(nleaf, d);
+    }
+    if (lane_id() == 0) {
+        atomicMax(&s.ctr->max_leaf, mx);
+        atomicAdd(&s.ctr->n_leaves, nleaf);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_scatter(DevState s)
+{
+    const double *xy = s.tp->xy;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < s.n; i += gridDim.x * blockDim.x) {
+        uint32_t leaf = s.leaf_of[i];
+        if (leaf == kNone) continue;
+        uint32_t pos = s.offset[leaf] + s.rank_of[i];
+        double x = xy[2 * (size_t)i], y = xy[2 * (size_t)i + 1];
+        s.sx[pos] = x;
+        s.sy[pos] = y;
+        s.spid[pos] = i;
+        if (s.precision == GLINT_PRECISION_FP32_GUARD) {
+            Node nd = s.nodes[leaf];
+            s.fx[pos] = (float)(x - nd.lx);
+            s.fy[pos] = (float)(y - nd.ly);
+        }
+    }
+}
+
+// lookup_rec (filter.cpp:130-152) from last_valid for every border object.
+// In complete mode every leaf within r other than the object's own.
+__global__ __launch_bounds__(kBlock) void k_lookup(DevState s)
+{
+    const double *xy = s.tp->xy;
+    const uint32_t nb = s.ctr->n_border;
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nb; k += gridDim.x * blockDim.x) {
+        uint2 b = s.border[k];
+        uint32_t pid = b.x;
+        double x = xy[2 * (size_t)pid], y = xy[2 * (size_t)pid + 1];
+        uint32_t own = s.leaf_of[pid];
+        uint32_t stk[kLookupStack];
+        int sp = 0;
+        stk[sp++] = b.y;
+        while (sp > 0) {
+            uint32_t node = stk[--sp];
+            Node nd = s.nodes[node];
+            for (int i = 0; i < 4; i++) {
+                uint32_t c = nd.child[i];
+                Node ch = s.nodes[c];
+                if (!box_within(ch, x, y, s.t64)) continue;
+                if (ch.child[0] == kLeafTag) {
+                    bool take = s.complete ? (c != own)
+                                           : (y < ch.ly || (y < ch.hy && x < ch.lx));
+                    if (take) {
+                        unsigned long long u = atomicAdd(&s.ctr->n_units, 1ull);
+                        if (u < s.unit_cap)
+                            s.units[u] = make_uint2(pid, c);
+                        else
+                            atomicOr(&s.ctr->error, kErrUnitCap);
+                    }
+                } else {
+                    if (sp < kLookupStack)
+                        stk[sp++] = c;
+                    else
+                        atomicOr(&s.ctr->error, kErrStack);
+                }
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------ meeting table
+
+// Upsert of one contact: look the pair up in the previous generation (its
+// start tick if it was seen at prev_tick, flagged as continued), then insert
+// it into the current generation.  Linear probing, 64-bit CAS on the key.
+__device__ __forceinline__ void upsert(const DevState &s, Slot *tab_old, Slot *tab_cur, uint32_t tick,
+                                       unsigned long long key)
+{
+    const unsigned long long mask = s.tab_mask;
+    const unsigned long long h0 = mix64(key) & mask;
+    uint32_t start = tick;
+    unsigned long long h = h0;
+    for (unsigned long long probes = 0; probes <= mask; probes++) {
+        unsigned long long k = __builtin_nontemporal_load(&tab_old[h].key);
+        if (k == key) {
+            start = tab_old[h].start;
+            if (!tab_old[h].seen) tab_old[h].seen = 1;
+            break;
+        }
+        if (k == kEmptyKey) break;
+        h = (h + 1) & mask;
+    }
+    h = h0;
+    for (unsigned long long probes = 0;; probes++) {
+        if (probes > mask) {
+            atomicOr(&s.ctr->error, kErrTableFull);
+            return;
+        }
+        unsigned long long prev = atomicCAS(&tab_cur[h].key, kEmptyKey, key);
+        if (prev == kEmptyKey) {
+            tab_cur[h].start = start;
+            atomicAdd(&s.ctr->n_inserted, 1ull);
+            return;
+        }
+        if (prev == key) return;   // duplicate discovery in this tick: same start
+        h = (h + 1) & mask;
+    }
+}
+
+__device__ __forceinline__ void contact(const DevState &s, Slot *tab_old, Slot *tab_cur, uint32_t tick,
+                                        uint32_t a, uint32_t b)
+{
+    uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
+    unsigned long long key = ((unsigned long long)lo << 32) | hi;
+    if (s.record) {
+        unsigned long long idx = atomicAdd(&s.ctr->n_rec, 1ull);
+        if (idx < s.rec_cap)
+            s.rec[idx] = key;
+        else
+            atomicOr(&s.ctr->error, kErrContactCap);
+    }
+    upsert(s, tab_old, tab_cur, tick, key);
+}
+
+// ------------------------------------------------------------ refine: joins
+
+// All pairs inside one leaf (the "inside" check units of filter.cpp:107-109
+// with refinement.cpp:30: pid < q, i.e. every unordered pair once).
+__global__ __launch_bounds__(kBlock) void k_join_inside(DevState s)
+{
+    __shared__ double ax[kJoinTile], ay[kJoinTile], bx[kJoinTile], by[kJoinTile];
+    __shared__ uint32_t ap[kJoinTile], bp[kJoinTile];
+    __shared__ uint32_t item;
+    Slot *tab_old = s.tp->tab_old, *tab_cur = s.tp->tab_cur;
+    const uint32_t tick = s.tp->tick;
+    const uint32_t nl = s.ctr->n_leaf_list;
+    unsigned long long ncontacts = 0;
+    for (;;) {
+        if (threadIdx.x == 0) item = atomicAdd(&s.ctr->work_inside, kInsideChunk);
+        __syncthreads();
+        uint32_t w0 = item;
+        __syncthreads();
+        if (w0 >= nl) break;
+        for (uint32_t w = w0; w < min(w0 + kInsideChunk, nl); w++) {
+            uint32_t leaf = s.leaf_list[w];
+            uint32_t off = s.offset[leaf], cnt = s.count[leaf];
+            uint32_t ntile = (cnt + kJoinTile - 1) / kJoinTile;
+            for (uint32_t I = 0; I < ntile; I++) {
+                uint32_t ni = min((uint32_t)kJoinTile, cnt - I * kJoinTile);
+                for (uint32_t k = threadIdx.x; k < ni; k += kBlock) {
+                    ax[k] = s.sx[off + I * kJoinTile + k];
+                    ay[k] = s.sy[off + I * kJoinTile + k];
+                    ap[k] = s.spid[off + I * kJoinTile + k];
+                }
+                for (uint32_t J = I; J < ntile; J++) {
+                    uint32_t nj = min((uint32_t)kJoinTile, cnt - J * kJoinTile);
+                    const double *jx = ax, *jy = ay;
+                    const uint32_t *jp = ap;
+                    if (J != I) {
+                        for (uint32_t k = threadIdx.x; k < nj; k += kBlock) {
+                            bx[k] = s.sx[off + J * kJoinTile + k];
+                            by[k] = s.sy[off + J * kJoinTile + k];
+                            bp[k] = s.spid[off + J * kJoinTile + k];
+                        }
+                        jx = bx;
+                        jy = by;
+                        jp = bp;
+                    }
+                    __syncthreads();
+                    for (uint32_t i = threadIdx.x; i < ni; i += kBlock) {
+                        double xi = ax[i], yi = ay[i];
+                        uint32_t j0 = (I == J) ? i + 1 : 0;
+                        for (uint32_t j = j0; j < nj; j++) {
+                            if (point_within64(xi, yi, jx[j], jy[j], s.t64)) {
+                                ncontacts++;
+                                contact(s, tab_old, tab_cur, tick, ap[i], jp[j]);
+                            }
+                        }
+                    }
+                    __syncthreads();
+                }
+            }
+        }
+    }
+    if (ncontacts) atomicAdd(&s.ctr->n_contacts, ncontacts);
+}
+
+// Border check units (pid, leaf, inside=false): pid against every object of
+// the candidate leaf (refinement.cpp:30 with !inside).  One wave per unit.
+__global__ __launch_bounds__(kBlock) void k_join_border(DevState s)
+{
+    Slot *tab_old = s.tp->tab_old, *tab_cur = s.tp->tab_cur;
+    const uint32_t tick = s.tp->tick;
+    const double *xy = s.tp->xy;
+    unsigned long long nu = s.ctr->n_units;
+    if (nu > s.unit_cap) nu = s.unit_cap;
+    const uint32_t lane = lane_id();
+    const unsigned long long wave = (blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x) / 64;
+    const unsigned long long nwaves = (gridDim.x * (unsigned long long)blockDim.x) / 64;
+    unsigned long long ncontacts = 0;
+    for (unsigned long long u = wave; u < nu; u += nwaves) {
+        uint2 un = s.units[u];
+        uint32_t pid = un.x, leaf = un.y;
+        double x = xy[2 * (size_t)pid], y = xy[2 * (size_t)pid + 1];
+        uint32_t off = s.offset[leaf], cnt = s.count[leaf];
+        for (uint32_t k = lane; k < cnt; k += 64) {
+            if (point_within64(x, y, s.sx[off + k], s.sy[off + k], s.t64)) {
+                ncontacts++;
+                contact(s, tab_old, tab_cur, tick, pid, s.spid[off + k]);
+            }
+        }
+    }
+    if (ncontacts) atomicAdd(&s.ctr->n_contacts, ncontacts);
+}
+
+// ------------------------------------------------------------ evaluate
+
+// meeting.cpp:134-155 over the previous generation: a pair seen at
+// prev_tick and not again at this tick has ended; it is emitted iff
+// cur - start >= min + 1; every slot is cleared for reuse.
+__global__ __launch_bounds__(kBlock) void k_evaluate(DevState s)
+{
+    Slot *tab = s.tp->tab_old;
+    const uint32_t tick = s.tp->tick, last = s.tp->prev_tick;
+    const unsigned long long nslots = s.tab_mask + 1;
+    for (unsigned long long base = blockIdx.x * (unsigned long long)blockDim.x; base < nslots;
+         base += (unsigned long long)gridDim.x * blockDim.x) {
+        unsigned long long i = base + threadIdx.x;
+        bool emit = false;
+        Slot sl;
+        sl.key = kEmptyKey;
+        if (i < nslots) {
+            sl = tab[i];
+            if (sl.key != kEmptyKey) {
+                emit = !sl.seen && (tick - sl.start >= s.min_meet + 1);
+                Slot e;
+                e.key = kEmptyKey;
+                e.start = 0;
+                e.seen = 0;
+                tab[i] = e;
+            }
+        }
+        unsigned long long idx = wave_append_u64(&s.ctr->n_meet, emit);
+        if (emit) {
+            if (idx < s.meet_cap) {
+                glint_meeting m;
+                m.pid_a = (uint32_t)(sl.key >> 32);
+                m.pid_b = (uint32_t)sl.key;
+                m.start = sl.start;
+                m.end = last;
+                s.meet[idx] = m;
+            } else {
+                atomicOr(&s.ctr->error, kErrMeetCap);
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------ host side
+
+inline uint32_t grid_for(uint64_t n, uint32_t cap = 8192)
+{
+    uint64_t g = (n + kBlock - 1) / kBlock;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (uint32_t)g;
+}
+
+inline uint64_t next_pow2(uint64_t v)
+{
+    uint64_t p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
+
+// Largest double s with sqrt(s) <= r (correctly rounded host sqrt).
+double sqrt_threshold(double r)
+{
+    double s = r * r;
+    while (std::sqrt(s) > r) s = std::nextafter(s, 0.0);
+    for (;;) {
+        double n = std::nextafter(s, INFINITY);
+        if (std::sqrt(n) <= r)
+            s = n;
+        else
+            break;
+    }
+    return s;
+}
+
+}  // namespace
+
+struct glint_ctx {
+    glint_config cfg{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    DevState s{};
+    Slot *tab[2] = {nullptr, nullptr};
+    int cur_gen = 0;            // table generation filled at the current tick
+    bool built = false;
+    bool has_positions = false;
+    bool partitioned = false;
+    bool refined = false;
+    bool evaluated = false;
+    bool has_prev = false;
+    uint32_t tick = 0;
+    uint32_t prev_tick = 0;
+    uint32_t scan_blocks = 0;
+    double *own_xy = nullptr;
+    Counters *h_ctr = nullptr;  // pinned
+    std::vector<void *> allocs;
+    std::string err;
+    uint32_t inject = 0;
+    hipEvent_t ev[6] = {};
+    glint_tick_stats last{};
+};
+
+namespace {
+
+std::string g_create_error;
+
+enum InjectBits : uint32_t {
+    kInjPartition = 1,
+    kInjRefine = 2,
+    kInjEvaluate = 4,
+    kInjSchema = 8,
+    kInjAlloc = 16,
+};
+
+glint_status hip_fail(glint_ctx *c, hipError_t e, const char *what)
+{
+    if (c) c->err = std::string(what) + ": " + hipGetErrorString(e);
+    return GLINT_ERR_HIP;
+}
+
+#define HIP_TRY(c, call)                                      \
+    do {                                                      \
+        hipError_t e_ = (call);                               \
+        if (e_ != hipSuccess) return hip_fail((c), e_, #call); \
+    } while (0)
+
+template <typename T>
+glint_status dalloc(glint_ctx *c, T **p, uint64_t count)
+{
+    if (count == 0) count = 1;
+    void *v = nullptr;
+    hipError_t e = hipMalloc(&v, count * sizeof(T));
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        c->err = "hipMalloc of " + std::to_string(count * sizeof(T)) + " bytes failed";
+        return GLINT_ERR_OUT_OF_MEMORY;
+    }
+    c->allocs.push_back(v);
+    *p = (T *)v;
+    return GLINT_OK;
+}
+
+glint_status status_from_flags(glint_ctx *c, uint32_t f)
+{
+    if (!f) return GLINT_OK;
+    if (f & kErrBadInput) {
+        c->err = "a position is NaN or outside the root box";
+        return GLINT_ERR_BAD_INPUT;
+    }
+    if (f & kErrNodeCap) {
+        c->err = "quadtree node pool exhausted (raise node_capacity)";
+        return GLINT_ERR_NODE_CAPACITY;
+    }
+    if (f & kErrTableFull) {
+        c->err = "meeting table full (raise num_meeting_buckets)";
+        return GLINT_ERR_TABLE_FULL;
+    }
+    if (f & (kErrUnitCap | kErrStack)) {
+        c->err = "border check units exceed unit_capacity";
+        return GLINT_ERR_UNIT_CAPACITY;
+    }
+    c->err = "meeting or contact output exceeds its capacity";
+    return GLINT_ERR_OUTPUT_CAPACITY;
+}
+
+glint_status read_counters(glint_ctx *c)
+{
+    HIP_TRY(c, hipMemcpyAsync(c->h_ctr, c->s.ctr, sizeof(Counters), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return GLINT_OK;
+}
+
+void set_tick_params(glint_ctx *c, const double *xy)
+{
+    Slot *cur = c->tab[c->cur_gen], *old = c->tab[1 - c->cur_gen];
+    k_set_tick<<<1, 1, 0, c->stream>>>(c->s.tp, xy, old, cur, c->tick, c->has_prev ? c->prev_tick : c->tick);
+}
+
+glint_status build_tree(glint_ctx *c)
+{
+    DevState &s = c->s;
+    const uint32_t gn = grid_for(s.n), gnode = grid_for(s.node_cap);
+    k_init_tree<<<gnode, kBlock, 0, c->stream>>>(s);
+    for (int level = 0; level < 64; level++) {
+        k_fill_u32<<<gnode, kBlock, 0, c->stream>>>(s.count, 0u, s.node_cap);
+        k_reset_tick<<<1, 1, 0, c->stream>>>(s);
+        k_build_count<<<gn, kBlock, 0, c->stream>>>(s);
+        k_build_collect<<<gnode, kBlock, 0, c->stream>>>(s);
+        glint_status st = read_counters(c);
+        if (st) return st;
+        if (c->h_ctr->error) return status_from_flags(c, c->h_ctr->error);
+        if (c->h_ctr->n_split == 0) break;
+        k_split<<<grid_for(c->h_ctr->n_split), kBlock, 0, c->stream>>>(s, 1);
+    }
+    HIP_TRY(c, hipGetLastError());
+    c->built = true;
+    return GLINT_OK;
+}
+
+void enqueue_partition(glint_ctx *c)
+{
+    DevState &s = c->s;
+    const uint32_t gn = grid_for(s.n), gnode = grid_for(s.node_cap);
+    k_fill_u32<<<gnode, kBlock, 0, c->stream>>>(s.count, 0u, s.node_cap);
+    k_reset_tick<<<1, 1, 0, c->stream>>>(s);
+    k_assign<<<gn, kBlock, 0, c->stream>>>(s);
+    k_scan_reduce<<<c->scan_blocks, kBlock, 0, c->stream>>>(s);
+    k_scan_top<<<1, kBlock, 0, c->stream>>>(s, c->scan_blocks);
+    k_scan_down<<<c->scan_blocks, kBlock, 0, c->stream>>>(s);
+    k_scatter<<<gn, kBlock, 0, c->stream>>>(s);
+}
+
+void enqueue_lookup(glint_ctx *c)
+{
+    k_lookup<<<grid_for(c->s.n), kBlock, 0, c->stream>>>(c->s);
+}
+
+void enqueue_refine(glint_ctx *c)
+{
+    k_join_inside<<<2048, kBlock, 0, c->stream>>>(c->s);
+    k_join_border<<<4096, kBlock, 0, c->stream>>>(c->s);
+}
+
+void enqueue_evaluate(glint_ctx *c)
+{
+    k_evaluate<<<grid_for(c->s.tab_mask + 1, 16384), kBlock, 0, c->stream>>>(c->s);
+}
+
+void enqueue_schema(glint_ctx *c)
+{
+    DevState &s = c->s;
+    if (c->cfg.dynamic_schema) {
+        const uint32_t gnode = grid_for(s.node_cap);
+        k_collect<<<gnode, kBlock, 0, c->stream>>>(s);
+        k_split<<<gnode, kBlock, 0, c->stream>>>(s, 0);
+        k_merge<<<gnode, kBlock, 0, c->stream>>>(s);
+    }
+    k_schema_done<<<1, 1, 0, c->stream>>>(s);
+}
+
+void fill_stats(glint_ctx *c)
+{
+    const Counters &h = *c->h_ctr;
+    glint_tick_stats &t = c->last;
+    t.tick = c->tick;
+    t.num_leaves = h.n_leaves;
+    t.num_nodes = (uint32_t)(c->s.node_cap - (uint32_t)std::max(h.free_top, 0));
+    t.num_border = h.n_border;
+    t.max_leaf_count = h.max_leaf;
+    t.splits = h.last_splits;
+    t.merges = h.last_merges;
+    t.error_flags = h.error;
+    t.num_units = h.n_units;
+    t.num_contacts = h.n_contacts;
+    t.num_active = h.n_inserted;
+    t.num_meetings = h.n_meet;
+    t.num_rechecks = h.n_rechecks;
+    t.table_slots = c->s.tab_mask + 1;
+}
+
+}  // namespace
+
+// ====================================================================== C ABI
+
+extern "C" {
+
+const char *glint_status_string(glint_status s)
+{
+    switch (s) {
+    case GLINT_OK: return "ok";
+    case GLINT_ERR_INVALID_ARGUMENT: return "invalid argument";
+    case GLINT_ERR_OUT_OF_MEMORY: return "out of device memory";
+    case GLINT_ERR_HIP: return "HIP runtime error";
+    case GLINT_ERR_BAD_INPUT: return "position is NaN or outside the root box";
+    case GLINT_ERR_TABLE_FULL: return "meeting table full";
+    case GLINT_ERR_NODE_CAPACITY: return "quadtree node capacity exceeded";
+    case GLINT_ERR_UNIT_CAPACITY: return "check-unit capacity exceeded";
+    case GLINT_ERR_OUTPUT_CAPACITY: return "output capacity exceeded";
+    case GLINT_ERR_STATE: return "call out of order";
+    case GLINT_ERR_INJECTED: return "injected failure";
+    }
+    return "unknown status";
+}
+
+const char *glint_last_error_message(const glint_ctx *ctx)
+{
+    return ctx ? ctx->err.c_str() : g_create_error.c_str();
+}
+
+void glint_config_default(glint_config *cfg)
+{
+    if (!cfg) return;
+    std::memset(cfg, 0, sizeof(*cfg));
+    cfg->grid_capacity = 50;
+    cfg->min_meet_time = 5;
+    cfg->max_meet_time = 4096;
+    cfg->schema_update_delay = 1;
+    cfg->dynamic_schema = 1;
+    cfg->reach_distance = 2.0;
+    cfg->guard_epsilon = 1e-3;
+}
+
+glint_status glint_create(const glint_config *cfg, glint_ctx **out)
+{
+    if (!cfg || !out) return GLINT_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    g_create_error.clear();
+    if (cfg->num_objects == 0 || cfg->grid_capacity == 0 || !(cfg->reach_distance > 0) ||
+        !(cfg->high_x > cfg->low_x) || !(cfg->high_y > cfg->low_y) || cfg->candidate_mode > 1 ||
+        cfg->precision > 1 || cfg->num_objects >= 0xFFFFFFF0u) {
+        g_create_error = "invalid configuration";
+        return GLINT_ERR_INVALID_ARGUMENT;
+    }
+    if (cfg->precision == GLINT_PRECISION_FP32_GUARD && !(cfg->guard_epsilon > 0)) {
+        g_create_error = "guard_epsilon must be positive";
+        return GLINT_ERR_INVALID_ARGUMENT;
+    }
+    glint_ctx *c = new (std::nothrow) glint_ctx();
+    if (!c) return GLINT_ERR_OUT_OF_MEMORY;
+    c->cfg = *cfg;
+    if (c->cfg.schema_update_delay == 0) c->cfg.schema_update_delay = 1;
+    c->device = (int)cfg->device;
+    auto fail = [&](glint_status st) {
+        g_create_error = c->err;
+        glint_destroy(c);
+        return st;
+    };
+    hipError_t e = hipSetDevice(c->device);
+    if (e != hipSuccess) {
+        c->err = std::string("hipSetDevice: ") + hipGetErrorString(e);
+        return fail(GLINT_ERR_HIP);
+    }
+    e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        c->err = std::string("hipStreamCreate: ") + hipGetErrorString(e);
+        return fail(GLINT_ERR_HIP);
+    }
+    for (auto &ev : c->ev) {
+        if (hipEventCreate(&ev) != hipSuccess) {
+            c->err = "hipEventCreate failed";
+            return fail(GLINT_ERR_HIP);
+        }
+    }
+    if (hipHostMalloc((void **)&c->h_ctr, sizeof(Counters), hipHostMallocDefault) != hipSuccess) {
+        c->err = "hipHostMalloc failed";
+        return fail(GLINT_ERR_OUT_OF_MEMORY);
+    }
+    std::memset(c->h_ctr, 0, sizeof(Counters));
+
+    const uint64_t N = cfg->num_objects;
+    DevState &s = c->s;
+    s.n = (uint32_t)N;
+    s.cap = cfg->grid_capacity;
+    s.min_meet = cfg->min_meet_time;
+    s.complete = cfg->candidate_mode;
+    s.precision = cfg->precision;
+    s.delay = c->cfg.schema_update_delay;
+    s.record = cfg->record_contacts;
+    s.lx = cfg->low_x;
+    s.ly = cfg->low_y;
+    s.hx = cfg->high_x;
+    s.hy = cfg->high_y;
+    s.r = cfg->reach_distance;
+    s.t64 = sqrt_threshold(cfg->reach_distance);
+    {
+        double r = cfg->reach_distance, eps = cfg->guard_epsilon;
+        double lo = r > eps ? (r - eps) * (r - eps) : 0.0, hi = (r + eps) * (r + eps);
+        float fl = (float)lo, fh = (float)hi;
+        if ((double)fl > lo) fl = std::nextafter(fl, 0.0f);
+        if ((double)fh < hi) fh = std::nextafter(fh, INFINITY);
+        s.t32_in = fl;
+        s.t32_out = fh;
+    }
+    uint64_t node_cap = cfg->node_capacity ? cfg->node_capacity
+                                           : std::max<uint64_t>(65536, 16 * (N / cfg->grid_capacity + 1) + 1024);
+    if (node_cap > 0x7FFFFFF0ull) node_cap = 0x7FFFFFF0ull;
+    s.node_cap = (uint32_t)node_cap;
+    uint64_t slots = cfg->num_meeting_buckets ? next_pow2(cfg->num_meeting_buckets)
+                                              : next_pow2(std::max<uint64_t>(4 * N, 1024));
+    s.tab_mask = slots - 1;
+    s.unit_cap = cfg->unit_capacity ? cfg->unit_capacity : std::max<uint64_t>(4 * N, 1024);
+    s.meet_cap = cfg->meeting_capacity ? cfg->meeting_capacity : std::max<uint64_t>(2 * N, 1u << 20);
+    s.rec_cap = cfg->record_contacts ? (cfg->contact_capacity ? cfg->contact_capacity : 8 * N + 1024) : 0;
+    c->scan_blocks = (uint32_t)((node_cap + kScanTile - 1) / kScanTile);
+
+    glint_status st = GLINT_OK;
+#define ALLOC(p, n)                          \
+    do {                                     \
+        if ((st = dalloc(c, &(p), (n)))) return fail(st); \
+    } while (0)
+    ALLOC(s.tp, 1);
+    ALLOC(s.nodes, node_cap);
+    ALLOC(s.meta, node_cap);
+    ALLOC(s.free_stack, node_cap);
+    ALLOC(s.count, node_cap);
+    ALLOC(s.offset, node_cap);
+    ALLOC(s.block_sums, c->scan_blocks);
+    ALLOC(s.split_list, node_cap);
+    ALLOC(s.merge_list, node_cap);
+    ALLOC(s.leaf_list, node_cap);
+    ALLOC(s.leaf_of, N);
+    ALLOC(s.rank_of, N);
+    ALLOC(s.border, N);
+    ALLOC(s.units, s.unit_cap);
+    ALLOC(s.sx, N);
+    ALLOC(s.sy, N);
+    ALLOC(s.spid, N);
+    if (cfg->precision == GLINT_PRECISION_FP32_GUARD) {
+        ALLOC(s.fx, N);
+        ALLOC(s.fy, N);
+    }
+    ALLOC(s.meet, s.meet_cap);
+    if (s.rec_cap) ALLOC(s.rec, s.rec_cap);
+    ALLOC(s.ctr, 1);
+    ALLOC(c->tab[0], slots);
+    ALLOC(c->tab[1], slots);
+    ALLOC(c->own_xy, 2 * N);
+#undef ALLOC
+    e = hipMemsetAsync(s.ctr, 0, sizeof(Counters), c->stream);
+    if (e != hipSuccess) {
+        c->err = std::string("hipMemsetAsync: ") + hipGetErrorString(e);
+        return fail(GLINT_ERR_HIP);
+    }
+    k_fill_slots<<<grid_for(slots, 16384), kBlock, 0, c->stream>>>(c->tab[0], slots);
+    k_fill_slots<<<grid_for(slots, 16384), kBlock, 0, c->stream>>>(c->tab[1], slots);
+    e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+        c->err = std::string("initialisation: ") + hipGetErrorString(e);
+        return fail(GLINT_ERR_HIP);
+    }
+    c->tick = cfg->start_time;
+    *out = c;
+    return GLINT_OK;
+}
+
+glint_status glint_destroy(glint_ctx *c)
+{
+    if (!c) return GLINT_ERR_INVALID_ARGUMENT;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (void *p : c->allocs) (void)hipFree(p);
+    for (auto &ev : c->ev)
+        if (ev) (void)hipEventDestroy(ev);
+    if (c->h_ctr) (void)hipHostFree(c->h_ctr);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return GLINT_OK;
+}
+
+static glint_status begin_tick(glint_ctx *c, const double *d_xy, uint32_t tick)
+{
+    if (c->has_positions && !c->evaluated) {
+        c->err = "previous tick was not evaluated";
+        return GLINT_ERR_STATE;
+    }
+    if (c->has_prev && tick <= c->prev_tick) {
+        c->err = "ticks must increase";
+        return GLINT_ERR_STATE;
+    }
+    c->tick = tick;
+    set_tick_params(c, d_xy);
+    c->has_positions = true;
+    c->partitioned = c->refined = c->evaluated = false;
+    return GLINT_OK;
+}
+
+glint_status glint_set_positions(glint_ctx *c, const double *xy, uint32_t n, uint32_t tick)
+{
+    if (!c || !xy || n != c->s.n) return GLINT_ERR_INVALID_ARGUMENT;
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipMemcpyAsync(c->own_xy, xy, (size_t)n * 16, hipMemcpyHostToDevice, c->stream));
+    return begin_tick(c, c->own_xy, tick);
+}
+
+glint_status glint_set_positions_device(glint_ctx *c, const double *d_xy, uint32_t n, uint32_t tick)
+{
+    if (!c || !d_xy || n != c->s.n) return GLINT_ERR_INVALID_ARGUMENT;
+    HIP_TRY(c, hipSetDevice(c->device));
+    return begin_tick(c, d_xy, tick);
+}
+
+glint_status glint_partition(glint_ctx *c)
+{
+    if (!c) return GLINT_ERR_INVALID_ARGUMENT;
+    if (!c->has_positions || c->partitioned) {
+        c->err = "partition needs new positions";
+        return GLINT_ERR_STATE;
+    }
+    if (c->inject & kInjPartition) {
+        c->inject &= ~kInjPartition;
+        c->err = "injected partition failure";
+        return GLINT_ERR_INJECTED;
+    }
+    HIP_TRY(c, hipSetDevice(c->device));
+    if (!c->built) {
+        glint_status st = build_tree(c);
+        if (st) return st;
+    }
+    HIP_TRY(c, hipEventRecord(c->ev[0], c->stream));
+    enqueue_partition(c);
+    HIP_TRY(c, hipEventRecord(c->ev[1], c->stream));
+    enqueue_lookup(c);
+    HIP_TRY(c, hipEventRecord(c->ev[2], c->stream));
+    HIP_TRY(c, hipGetLastError());
+    c->partitioned = true;
+    return GLINT_OK;
+}
+
+glint_status glint_refine(glint_ctx *c)
+{
+    if (!c) return GLINT_ERR_INVALID_ARGUMENT;
+    if (!c->partitioned || c->refined) {
+        c->err = "refine needs a partitioned tick";
+        return GLINT_ERR_STATE;
+    }
+    if (c->inject & kInjRefine) {
+        c->inject &= ~kInjRefine;
+        c->err = "injected refine failure";
+        return GLINT_ERR_INJECTED;
+    }
+    HIP_TRY(c, hipSetDevice(c->device));
+    enqueue_refine(c);
+    HIP_TRY(c, hipEventRecord(c->ev[3], c->stream));
+    HIP_TRY(c, hipGetLastError());
+    c->refined = true;
+    return GLINT_OK;
+}
+
+glint_status glint_evaluate(glint_ctx *c, glint_meeting *out, uint64_t cap, uint64_t *n_out)
+{
+    if (!c) return GLINT_ERR_INVALID_ARGUMENT;
+    if (!c->refined || c->evaluated) {
+        c->err = "evaluate needs a refined tick";
+        return GLINT_ERR_STATE;
+    }
+    if (c->inject & kInjEvaluate) {
+        c->inject &= ~kInjEvaluate;
+        c->err = "injected evaluate failure";
+        return GLINT_ERR_INJECTED;
+    }
+    HIP_TRY(c, hipSetDevice(c->device));
+    enqueue_evaluate(c);
+    HIP_TRY(c, hipEventRecord(c->ev[4], c->stream));
+    HIP_TRY(c, hipGetLastError());
+    c->evaluated = true;
+    c->cur_gen = 1 - c->cur_gen;   // this tick's pairs become the previous generation
+    c->prev_tick = c->tick;
+    c->has_prev = true;
+    if (!out && !n_out) return GLINT_OK;
+    glint_status st = read_counters(c);
+    if (st) return st;
+    uint64_t n = c->h_ctr->n_meet;
+    if (n_out) *n_out = n;
+    if (c->h_ctr->error) return status_from_flags(c, c->h_ctr->error);
+    if (!out) return GLINT_OK;
+    if (n > cap) {
+        c->err = "meeting output buffer too small";
+        return GLINT_ERR_OUTPUT_CAPACITY;
+    }
+    if (n) {
+        HIP_TRY(c, hipMemcpy(out, c->s.meet, n * sizeof(glint_meeting), hipMemcpyDeviceToHost));
+        std::sort(out, out + n, [](const glint_meeting &a, const glint_meeting &b) {
+            if (a.pid_a != b.pid_a) return a.pid_a < b.pid_a;
+            if (a.pid_b != b.pid_b) return a.pid_b < b.pid_b;
+            return a.start < b.start;
+        });
+    }
+    return GLINT_OK;
+}
+
+glint_status glint_update_schema(glint_ctx *c)
+{
+    if (!c) return GLINT_ERR_INVALID_ARGUMENT;
+    if (!c->evaluated) {
+        c->err = "update_schema needs an evaluated tick";
+        return GLINT_ERR_STATE;
+    }
+    if (c->inject & kInjSchema) {
+        c->inject &= ~kInjSchema;
+        c->err = "injected schema failure";
+        return GLINT_ERR_INJECTED;
+    }
+    HIP_TRY(c, hipSetDevice(c->device));
+    enqueue_schema(c);
+    HIP_TRY(c, hipEventRecord(c->ev[5], c->stream));
+    HIP_TRY(c, hipGetLastError());
+    return GLINT_OK;
+}
+
+glint_status glint_process_tick(glint_ctx *c, const double *xy, uint32_t tick)
+{
+    glint_status st;
+    if ((st = glint_set_positions(c, xy, c ? c->s.n : 0, tick))) return st;
+    if ((st = glint_partition(c))) return st;
+    if ((st = glint_refine(c))) return st;
+    if ((st = glint_evaluate(c, nullptr, 0, nullptr))) return st;
+    return glint_update_schema(c);
+}
+
+glint_status glint_run_ticks(glint_ctx *c, const double *d_trace, uint32_t first_tick, uint32_t nticks,
+                             float *tick_ms)
+{
+    if (!c || !d_trace || nticks == 0) return GLINT_ERR_INVALID_ARGUMENT;
+    if (c->has_positions && !c->evaluated) {
+        c->err = "previous tick was not evaluated";
+        return GLINT_ERR_STATE;
+    }
+    if (c->has_prev && first_tick <= c->prev_tick) {
+        c->err = "ticks must increase";
+        return GLINT_ERR_STATE;
+    }
+    HIP_TRY(c, hipSetDevice(c->device));
+    const uint64_t stride = 2ull * c->s.n;
+    std::vector<hipEvent_t> evs;
+    if (tick_ms) {
+        evs.resize(nticks + 1);
+        for (auto &e : evs) HIP_TRY(c, hipEventCreate(&e));
+    }
+    glint_status st = GLINT_OK;
+    c->tick = first_tick;
+    k_begin_trace<<<1, 1, 0, c->stream>>>(c->s.tp, d_trace, c->tab[1 - c->cur_gen], c->tab[c->cur_gen],
+                                          first_tick, c->has_prev ? c->prev_tick : first_tick);
+    if (!c->built) {
+        st = build_tree(c);
+        if (st) return st;
+    }
+    if (tick_ms) HIP_TRY(c, hipEventRecord(evs[0], c->stream));
+    for (uint32_t k = 0; k < nticks; k++) {
+        if (k > 0) {
+            k_swap_tables<<<1, 1, 0, c->stream>>>(c->s.tp);
+            k_advance_tick<<<1, 1, 0, c->stream>>>(c->s.tp, stride);
+        }
+        enqueue_partition(c);
+        enqueue_lookup(c);
+        enqueue_refine(c);
+        enqueue_evaluate(c);
+        enqueue_schema(c);
+        if (tick_ms) HIP_TRY(c, hipEventRecord(evs[k + 1], c->stream));
+        c->cur_gen = 1 - c->cur_gen;
+        c->prev_tick = first_tick + k;
+        c->has_prev = true;
+    }
+    HIP_TRY(c, hipGetLastError());
+    c->tick = first_tick + nticks - 1;
+    c->has_positions = true;
+    c->partitioned = c->refined = c->evaluated = true;
+    st = read_counters(c);
+    if (st) return st;
+    if (tick_ms) {
+        for (uint32_t k = 0; k < nticks; k++) HIP_TRY(c, hipEventElapsedTime(&tick_ms[k], evs[k], evs[k + 1]));
+        for (auto &e : evs) (void)hipEventDestroy(e);
+    }
+    fill_stats(c);
+    return status_from_flags(c, c->h_ctr->error);
+}
+
+glint_status glint_synchronize(glint_ctx *c)
+{
+    if (!c) return GLINT_ERR_INVALID_ARGUMENT;
+    glint_status st = read_counters(c);
+    if (st) return st;
+    return status_from_flags(c, c->h_ctr->error);
+}
+
+glint_status glint_stats(glint_ctx *c, glint_tick_stats *out)
+{
+    if (!c || !out) return GLINT_ERR_INVALID_ARGUMENT;
+    glint_status st = read_counters(c);
+    if (st) return st;
+    fill_stats(c);
+    float ms;
+    glint_tick_stats &t = c->last;
+    t.ms_partition = t.ms_lookup = t.ms_refine = t.ms_evaluate = t.ms_schema = 0;
+    if (c->evaluated && c->partitioned) {
+        if (hipEventElapsedTime(&ms, c->ev[0], c->ev[1]) == hipSuccess) t.ms_partition = ms;
+        if (hipEventElapsedTime(&ms, c->ev[1], c->ev[2]) == hipSuccess) t.ms_lookup = ms;
+        if (hipEventElapsedTime(&ms, c->ev[2], c->ev[3]) == hipSuccess) t.ms_refine = ms;
+        if (hipEventElapsedTime(&ms, c->ev[3], c->ev[4]) == hipSuccess) t.ms_evaluate = ms;
+        if (hipEventElapsedTime(&ms, c->ev[4], c->ev[5]) == hipSuccess) t.ms_schema = ms;
+    }
+    (void)hipGetLastError();
+    *out = t;
+    return status_from_flags(c, c->h_ctr->error);
+}
+
+glint_status glint_contacts(glint_ctx *c, uint64_t *out, uint64_t cap, uint64_t *n_out)
+{
+    if (!c || !n_out) return GLINT_ERR_INVALID_ARGUMENT;
+    if (!c->cfg.record_contacts) {
+        c->err = "contacts are recorded only with record_contacts";
+        return GLINT_ERR_STATE;
+    }
+    glint_status st = read_counters(c);
+    if (st) return st;
+    uint64_t n = std::min<uint64_t>(c->h_ctr->n_rec, c->s.rec_cap);
+    std::vector<uint64_t> v(n);
+    if (n) HIP_TRY(c, hipMemcpy(v.data(), c->s.rec, n * 8, hipMemcpyDeviceToHost));
+    std::sort(v.begin(), v.end());
+    v.erase(std::unique(v.begin(), v.end()), v.end());
+    *n_out = v.size();
+    if (c->h_ctr->error) return status_from_flags(c, c->h_ctr->error);
+    if (!out) return GLINT_OK;
+    if (v.size() > cap) return GLINT_ERR_OUTPUT_CAPACITY;
+    std::copy(v.begin(), v.end(), out);
+    return GLINT_OK;
+}
+
+glint_status glint_leaves(glint_ctx *c, double *boxes, uint32_t *levels, uint32_t *counts, uint64_t cap,
+                          uint64_t *n_out)
+{
+    if (!c || !n_out) return GLINT_ERR_INVALID_ARGUMENT;
+    if (!c->built) {
+        *n_out = 0;
+        return GLINT_OK;
+    }
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    const uint32_t nc = c->s.node_cap;
+    std::vector<Node> nodes(nc);
+    std::vector<NodeMeta> meta(nc);
+    std::vector<uint32_t> cnt(nc);
+    HIP_TRY(c, hipMemcpy(nodes.data(), c->s.nodes, nc * sizeof(Node), hipMemcpyDeviceToHost));
+    HIP_TRY(c, hipMemcpy(meta.data(), c->s.meta, nc * sizeof(NodeMeta), hipMemcpyDeviceToHost));
+    HIP_TRY(c, hipMemcpy(cnt.data(), c->s.count, nc * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    std::vector<uint32_t> stack{0}, order;
+    while (!stack.empty()) {
+        uint32_t n = stack.back();
+        stack.pop_back();
+        if (nodes[n].child[0] == kLeafTag) {
+            order.push_back(n);
+        } else {
+            for (int i = 3; i >= 0; i--) stack.push_back(nodes[n].child[i]);
+        }
+    }
+    *n_out = order.size();
+    for (size_t k = 0; k < order.size() && k < cap; k++) {
+        const Node &nd = nodes[order[k]];
+        if (boxes) {
+            boxes[4 * k] = nd.lx;
+            boxes[4 * k + 1] = nd.ly;
+            boxes[4 * k + 2] = nd.hx;
+            boxes[4 * k + 3] = nd.hy;
+        }
+        if (levels) levels[k] = (uint32_t)meta[order[k]].level;
+        if (counts) counts[k] = cnt[order[k]];
+    }
+    return order.size() > cap && (boxes || levels || counts) ? GLINT_ERR_OUTPUT_CAPACITY : GLINT_OK;
+}
+
+glint_status glint_active_pairs(glint_ctx *c, uint64_t *keys, uint32_t *starts, uint64_t cap, uint64_t *n_out)
+{
+    if (!c || !n_out) return GLINT_ERR_INVALID_ARGUMENT;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    // after evaluate the current tick's pairs are the previous generation
+    const Slot *tab = c->tab[c->evaluated ? 1 - c->cur_gen : c->cur_gen];
+    const uint64_t slots = c->s.tab_mask + 1;
+    std::vector<Slot> h(slots);
+    HIP_TRY(c, hipMemcpy(h.data(), tab, slots * sizeof(Slot), hipMemcpyDeviceToHost));
+    std::vector<std::pair<uint64_t, uint32_t>> v;
+    for (const Slot &sl : h)
+        if (sl.key != kEmptyKey) v.emplace_back(sl.key, sl.start);
+    std::sort(v.begin(), v.end());
+    *n_out = v.size();
+    if (!keys && !starts) return GLINT_OK;
+    if (v.size() > cap) return GLINT_ERR_OUTPUT_CAPACITY;
+    for (size_t k = 0; k < v.size(); k++) {
+        if (keys) keys[k] = v[k].first;
+        if (starts) starts[k] = v[k].second;
+    }
+    return GLINT_OK;
+}
+
+glint_status glint_device_meetings(glint_ctx *c, const glint_meeting **d_out, uint64_t *n_out)
+{
+    if (!c || !d_out || !n_out) return GLINT_ERR_INVALID_ARGUMENT;
+    glint_status st = read_counters(c);
+    if (st) return st;
+    *d_out = c->s.meet;
+    *n_out = std::min<uint64_t>(c->h_ctr->n_meet, c->s.meet_cap);
+    return status_from_flags(c, c->h_ctr->error);
+}
+
+glint_status glint_debug_inject_failure(glint_ctx *c, const char *stage)
+{
+    if (!c || !stage) return GLINT_ERR_INVALID_ARGUMENT;
+    std::string s(stage);
+    if (s == "partition")
+        c->inject |= kInjPartition;
+    else if (s == "refine")
+        c->inject |= kInjRefine;
+    else if (s == "evaluate")
+        c->inject |= kInjEvaluate;
+    else if (s == "update_schema")
+        c->inject |= kInjSchema;
+    else
+        return GLINT_ERR_INVALID_ARGUMENT;
+    return GLINT_OK;
+}
+
+glint_status glint_device_malloc(void **ptr, uint64_t bytes, uint32_t device)
+{
+    if (!ptr) return GLINT_ERR_INVALID_ARGUMENT;
+    if (hipSetDevice((int)device) != hipSuccess) return GLINT_ERR_HIP;
+    if (hipMalloc(ptr, bytes ? bytes : 1) != hipSuccess) {
+        (void)hipGetLastError();
+        return GLINT_ERR_OUT_OF_MEMORY;
+    }
+    return GLINT_OK;
+}
+
+glint_status glint_device_free(void *ptr)
+{
+    return hipFree(ptr) == hipSuccess ? GLINT_OK : GLINT_ERR_HIP;
+}
+
+glint_status glint_memcpy_to_device(void *dst, const void *src, uint64_t bytes)
+{
+    return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess ? GLINT_OK : GLINT_ERR_HIP;
+}
+
+glint_status glint_memcpy_to_host(void *dst, const void *src, uint64_t bytes)
+{
+    return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost) == hipSuccess ? GLINT_OK : GLINT_ERR_HIP;
+}
+
+glint_status glint_device_count(int *count)
+{
+    if (!count) return GLINT_ERR_INVALID_ARGUMENT;
+    hipError_t e = hipGetDeviceCount(count);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        *count = 0;
+    }
+    return GLINT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,173 @@
+// Device-side data layout of the contact-tracing path on gfx950.
+//
+// HBM layout (one context):
+//   nodes[node_cap]      64 B quadtree nodes: descent midpoints, box, children
+//   meta[node_cap]       16 B: type, depth, split/merge hysteresis counters
+//   count/offset         u32 per node: leaf population and first slot of the
+//                        leaf in the sorted object arrays
+//   sx, sy, spid         objects sorted by leaf (FP64 x, FP64 y, object id)
+//   fx, fy               the same positions as FP32 offsets from the leaf's
+//                        low corner (guard-band mode)
+//   units                border check units (object id, candidate leaf)
+//   table[2][B]          16 B meeting slots, two generations
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "glint.h"
+
+namespace glint {
+
+constexpr uint32_t kLeafTag = 0xFFFFFFFFu;      // child[0] of a leaf
+constexpr uint32_t kNone = 0xFFFFFFFFu;
+constexpr unsigned long long kEmptyKey = ~0ull; // (a<<32)|b with a<b is never all ones
+
+enum NodeType : int32_t { kInvalid = 0, kBranch = 1, kLeaf = 2 };
+
+enum ErrorBits : uint32_t {
+    kErrBadInput = 1u << 0,
+    kErrTableFull = 1u << 1,
+    kErrNodeCap = 1u << 2,
+    kErrUnitCap = 1u << 3,
+    kErrMeetCap = 1u << 4,
+    kErrContactCap = 1u << 5,
+    kErrStack = 1u << 6,
+};
+
+struct alignas(64) Node {
+    double mx, my;          // midpoints used by the descent: child = (y>my)*2 + (x>mx)
+    double lx, ly, hx, hy;  // box
+    uint32_t child[4];      // child[0] == kLeafTag for a leaf
+};
+static_assert(sizeof(Node) == 64, "one node per 64-byte line");
+
+struct alignas(16) NodeMeta {
+    int32_t type;
+    int32_t level;
+    uint32_t overflow;
+    uint32_t underflow;
+};
+
+struct alignas(16) Slot {
+    unsigned long long key;   // (pid_a << 32) | pid_b, kEmptyKey when free
+    uint32_t start;           // first tick of the run of contacts
+    uint32_t seen;            // set when the pair is seen again in the next tick
+};
+static_assert(sizeof(Slot) == 16, "16-byte meeting slot");
+
+struct Counters {
+    unsigned long long n_units;
+    unsigned long long n_contacts;
+    unsigned long long n_inserted;
+    unsigned long long n_meet;
+    unsigned long long n_rechecks;
+    unsigned long long n_rec;
+    unsigned long long work_border;
+    int32_t free_top;
+    uint32_t n_split;
+    uint32_t n_merge;
+    uint32_t n_border;
+    uint32_t n_leaf_list;
+    uint32_t n_leaves;
+    uint32_t max_leaf;
+    uint32_t error;
+    uint32_t work_inside;
+    uint32_t n_nodes;
+    uint32_t last_splits;
+    uint32_t last_merges;
+};
+
+// Per-tick values, kept in device memory so a tick's launches can be
+// replayed from a graph; advanced on the device by k_advance_tick.
+struct TickParams {
+    const double *xy;
+    const double *trace_base;   // run_ticks: tick k of the trace at trace_base + 2*N*k
+    Slot *tab_old;              // pairs seen at prev_tick (read, flagged, then cleared)
+    Slot *tab_cur;              // pairs seen at tick (filled by refine)
+    uint32_t tick;
+    uint32_t prev_tick;
+    uint32_t trace_index;
+    uint32_t pad;
+};
+
+struct DevState {
+    uint32_t n;
+    uint32_t cap;
+    uint32_t min_meet;
+    uint32_t node_cap;
+    uint32_t complete;
+    uint32_t precision;
+    uint32_t delay;
+    uint32_t record;
+    double lx, ly, hx, hy;
+    double r;
+    double t64;         // largest s with sqrt(s) <= r (host-computed)
+    float t32_in;       // FP32 squared-distance bounds of the guard band
+    float t32_out;
+    unsigned long long tab_mask;
+    unsigned long long unit_cap;
+    unsigned long long meet_cap;
+    unsigned long long rec_cap;
+    TickParams *tp;
+    Node *nodes;
+    NodeMeta *meta;
+    int32_t *free_stack;
+    uint32_t *count;
+    uint32_t *offset;
+    uint32_t *block_sums;
+    uint32_t *split_list;
+    uint32_t *merge_list;
+    uint32_t *leaf_list;
+    uint32_t *leaf_of;
+    uint32_t *rank_of;
+    uint2 *border;
+    uint2 *units;
+    double *sx;
+    double *sy;
+    uint32_t *spid;
+    float *fx;
+    float *fy;
+    glint_meeting *meet;
+    unsigned long long *rec;
+    Counters *ctr;
+};
+
+// SplitMix64 finaliser (Steele, Lea, Flood 2014): slot hash of a pair key.
+__device__ __forceinline__ unsigned long long mix64(unsigned long long z)
+{
+    z += 0x9e3779b97f4a7c15ull;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+// Wave-aggregated append: one atomic per wave, lanes get consecutive slots.
+// Must be called by every active lane of the wave with its own predicate.
+__device__ __forceinline__ unsigned long long wave_append_u64(unsigned long long *counter, bool pred)
+{
+    unsigned long long mask = __ballot(pred);
+    if (mask == 0) return ~0ull;
+    uint32_t lane = lane_id();
+    uint32_t leader = __ffsll((long long)mask) - 1;
+    unsigned long long base = 0;
+    if (lane == leader) base = atomicAdd(counter, (unsigned long long)__popcll(mask));
+    base = __shfl(base, leader);
+    return base + __popcll(mask & ((1ull << lane) - 1ull));
+}
+
+__device__ __forceinline__ uint32_t wave_append_u32(uint32_t *counter, bool pred)
+{
+    unsigned long long mask = __ballot(pred);
+    if (mask == 0) return kNone;
+    uint32_t lane = lane_id();
+    uint32_t leader = __ffsll((long long)mask) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+    base = __shfl(base, leader);
+    return base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+}
+
+}  // namespace glint

@@ -1,0 +1,414 @@
+"""ctypes bindings of libglint.so (include/glint.h) and a Python mirror of the
+reference's stage interface.
+
+The reference drives one tick as (src/tracing/process.cpp:452-460, 608-612):
+
+    bench->filter();          // partition + border lookup
+    bench->reachability();    // distance join + meeting-table upsert
+    bench->update_meetings(); // emit ended meetings
+    bench->update_schema();   // one level of split/merge
+
+``Workbench`` exposes the same four methods over the HIP implementation, and
+``Tracer.process`` runs the tick loop of tracer::process (process.cpp:378-634).
+Nothing here falls back to a CPU path: if libglint.so is missing or no GPU is
+visible the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes
+from pathlib import Path
+
+import numpy as np
+
+from . import gen as _gen
+
+_PKG = Path(__file__).resolve().parent
+LIB_PATH = _PKG / "libglint.so"
+
+GLINT_OK = 0
+STATUS_NAMES = {
+    0: "GLINT_OK", 1: "GLINT_ERR_INVALID_ARGUMENT", 2: "GLINT_ERR_OUT_OF_MEMORY", 3: "GLINT_ERR_HIP",
+    4: "GLINT_ERR_BAD_INPUT", 5: "GLINT_ERR_TABLE_FULL", 6: "GLINT_ERR_NODE_CAPACITY",
+    7: "GLINT_ERR_UNIT_CAPACITY", 8: "GLINT_ERR_OUTPUT_CAPACITY", 9: "GLINT_ERR_STATE",
+    10: "GLINT_ERR_INJECTED",
+}
+CANDIDATES_REFERENCE, CANDIDATES_COMPLETE = 0, 1
+PRECISION_FP64, PRECISION_FP32_GUARD = 0, 1
+
+
+class GlintError(RuntimeError):
+    def __init__(self, status: int, where: str, detail: str = ""):
+        self.status = status
+        super().__init__(f"{where}: {STATUS_NAMES.get(status, status)}" + (f" ({detail})" if detail else ""))
+
+
+class Config(ctypes.Structure):
+    _fields_ = [
+        ("num_objects", ctypes.c_uint32),
+        ("grid_capacity", ctypes.c_uint32),
+        ("min_meet_time", ctypes.c_uint32),
+        ("max_meet_time", ctypes.c_uint32),
+        ("schema_update_delay", ctypes.c_uint32),
+        ("dynamic_schema", ctypes.c_uint32),
+        ("start_time", ctypes.c_uint32),
+        ("candidate_mode", ctypes.c_uint32),
+        ("precision", ctypes.c_uint32),
+        ("device", ctypes.c_uint32),
+        ("low_x", ctypes.c_double),
+        ("low_y", ctypes.c_double),
+        ("high_x", ctypes.c_double),
+        ("high_y", ctypes.c_double),
+        ("reach_distance", ctypes.c_double),
+        ("guard_epsilon", ctypes.c_double),
+        ("num_meeting_buckets", ctypes.c_uint64),
+        ("unit_capacity", ctypes.c_uint64),
+        ("meeting_capacity", ctypes.c_uint64),
+        ("contact_capacity", ctypes.c_uint64),
+        ("node_capacity", ctypes.c_uint32),
+        ("record_contacts", ctypes.c_uint32),
+    ]
+
+
+class Meeting(ctypes.Structure):
+    _fields_ = [("pid_a", ctypes.c_uint32), ("pid_b", ctypes.c_uint32), ("start", ctypes.c_uint32),
+                ("end", ctypes.c_uint32)]
+
+
+class TickStats(ctypes.Structure):
+    _fields_ = [
+        ("tick", ctypes.c_uint32), ("num_leaves", ctypes.c_uint32), ("num_nodes", ctypes.c_uint32),
+        ("num_border", ctypes.c_uint32), ("max_leaf_count", ctypes.c_uint32), ("splits", ctypes.c_uint32),
+        ("merges", ctypes.c_uint32), ("error_flags", ctypes.c_uint32),
+        ("num_units", ctypes.c_uint64), ("num_contacts", ctypes.c_uint64), ("num_active", ctypes.c_uint64),
+        ("num_meetings", ctypes.c_uint64), ("num_rechecks", ctypes.c_uint64), ("table_slots", ctypes.c_uint64),
+        ("ms_partition", ctypes.c_float), ("ms_lookup", ctypes.c_float), ("ms_refine", ctypes.c_float),
+        ("ms_evaluate", ctypes.c_float), ("ms_schema", ctypes.c_float),
+    ]
+
+
+# (name, restype, argtypes) of every entry point of include/glint.h
+_VP = ctypes.c_void_p
+_U32, _U64, _I = ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int
+SIGNATURES = {
+    "glint_status_string": (ctypes.c_char_p, [_I]),
+    "glint_last_error_message": (ctypes.c_char_p, [_VP]),
+    "glint_config_default": (None, [ctypes.POINTER(Config)]),
+    "glint_create": (_I, [ctypes.POINTER(Config), ctypes.POINTER(_VP)]),
+    "glint_destroy": (_I, [_VP]),
+    "glint_set_positions": (_I, [_VP, _VP, _U32, _U32]),
+    "glint_set_positions_device": (_I, [_VP, _VP, _U32, _U32]),
+    "glint_partition": (_I, [_VP]),
+    "glint_refine": (_I, [_VP]),
+    "glint_evaluate": (_I, [_VP, _VP, _U64, ctypes.POINTER(_U64)]),
+    "glint_update_schema": (_I, [_VP]),
+    "glint_process_tick": (_I, [_VP, _VP, _U32]),
+    "glint_run_ticks": (_I, [_VP, _VP, _U32, _U32, _VP]),
+    "glint_synchronize": (_I, [_VP]),
+    "glint_stats": (_I, [_VP, ctypes.POINTER(TickStats)]),
+    "glint_contacts": (_I, [_VP, _VP, _U64, ctypes.POINTER(_U64)]),
+    "glint_leaves": (_I, [_VP, _VP, _VP, _VP, _U64, ctypes.POINTER(_U64)]),
+    "glint_active_pairs": (_I, [_VP, _VP, _VP, _U64, ctypes.POINTER(_U64)]),
+    "glint_device_meetings": (_I, [_VP, ctypes.POINTER(_VP), ctypes.POINTER(_U64)]),
+    "glint_debug_inject_failure": (_I, [_VP, ctypes.c_char_p]),
+    "glint_device_malloc": (_I, [ctypes.POINTER(_VP), _U64, _U32]),
+    "glint_device_free": (_I, [_VP]),
+    "glint_memcpy_to_device": (_I, [_VP, _VP, _U64]),
+    "glint_memcpy_to_host": (_I, [_VP, _VP, _U64]),
+    "glint_device_count": (_I, [ctypes.POINTER(_I)]),
+    "glint_gen_walk_device": (_I, [ctypes.POINTER(_gen.GenConfig), ctypes.POINTER(_gen.Walkers), _U32, _U32,
+                                   _VP, _U32]),
+    "glint_gen_validate": (_I, [ctypes.POINTER(_gen.GenConfig)]),
+    "glint_gen_init": (_I, [ctypes.POINTER(_gen.GenConfig), ctypes.POINTER(_gen.Walkers), _U32, _U32]),
+    "glint_gen_step": (_I, [ctypes.POINTER(_gen.GenConfig), ctypes.POINTER(_gen.Walkers), _U32, _U32, _U32]),
+    "glint_gen_hash": (_U64, [_U64, _U64, _U64, _U64]),
+}
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """Load libglint.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        L = ctypes.CDLL(str(LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def device_count() -> int:
+    n = ctypes.c_int(0)
+    lib().glint_device_count(ctypes.byref(n))
+    return n.value
+
+
+def _check(st: int, where: str, ctx=None) -> None:
+    if st != GLINT_OK:
+        detail = lib().glint_last_error_message(ctx)
+        raise GlintError(st, where, detail.decode() if detail else "")
+
+
+class DeviceBuffer:
+    """Raw device allocation owned by Python (no framework needed)."""
+
+    def __init__(self, nbytes: int, device: int = 0):
+        self.ptr = ctypes.c_void_p()
+        self.nbytes = int(nbytes)
+        _check(lib().glint_device_malloc(ctypes.byref(self.ptr), self.nbytes, device), "glint_device_malloc")
+
+    def upload(self, arr: np.ndarray, offset: int = 0) -> None:
+        arr = np.ascontiguousarray(arr)
+        assert offset + arr.nbytes <= self.nbytes
+        _check(lib().glint_memcpy_to_device(ctypes.c_void_p(self.ptr.value + offset), arr.ctypes.data, arr.nbytes),
+               "glint_memcpy_to_device")
+
+    def download(self, arr: np.ndarray, offset: int = 0) -> np.ndarray:
+        assert arr.flags.c_contiguous and offset + arr.nbytes <= self.nbytes
+        _check(lib().glint_memcpy_to_host(arr.ctypes.data, ctypes.c_void_p(self.ptr.value + offset), arr.nbytes),
+               "glint_memcpy_to_host")
+        return arr
+
+    def address(self, offset: int = 0) -> int:
+        return self.ptr.value + offset
+
+    def free(self) -> None:
+        if self.ptr.value:
+            lib().glint_device_free(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def make_config(num_objects: int, box, *, reach_distance: float = 2.0, grid_capacity: int = 50,
+                min_meet_time: int = 5, schema_update_delay: int = 1, dynamic_schema: bool = True,
+                start_time: int = 0, complete: bool = False, precision: int = PRECISION_FP64,
+                guard_epsilon: float = 1e-3, num_meeting_buckets: int = 0, unit_capacity: int = 0,
+                meeting_capacity: int = 0, node_capacity: int = 0, record_contacts: bool = False,
+                contact_capacity: int = 0, device: int = 0, max_meet_time: int = 4096) -> Config:
+    c = Config()
+    lib().glint_config_default(ctypes.byref(c))
+    c.num_objects = num_objects
+    c.low_x, c.low_y, c.high_x, c.high_y = (float(v) for v in box)
+    c.reach_distance = reach_distance
+    c.grid_capacity = grid_capacity
+    c.min_meet_time = min_meet_time
+    c.max_meet_time = max_meet_time
+    c.schema_update_delay = schema_update_delay
+    c.dynamic_schema = int(dynamic_schema)
+    c.start_time = start_time
+    c.candidate_mode = CANDIDATES_COMPLETE if complete else CANDIDATES_REFERENCE
+    c.precision = precision
+    c.guard_epsilon = guard_epsilon
+    c.num_meeting_buckets = num_meeting_buckets
+    c.unit_capacity = unit_capacity
+    c.meeting_capacity = meeting_capacity
+    c.node_capacity = node_capacity
+    c.record_contacts = int(record_contacts)
+    c.contact_capacity = contact_capacity
+    c.device = device
+    return c
+
+
+class Workbench:
+    """One device context; the four stage methods of the reference's workbench."""
+
+    def __init__(self, cfg: Config):
+        self.cfg = cfg
+        self.n = cfg.num_objects
+        h = ctypes.c_void_p()
+        _check(lib().glint_create(ctypes.byref(cfg), ctypes.byref(h)), "glint_create")
+        self.h = h
+        self._meet_cap = max(1024, 2 * self.n)
+        self._meet_buf = (Meeting * self._meet_cap)()
+
+    def close(self) -> None:
+        if self.h and self.h.value:
+            lib().glint_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _chk(self, st, where):
+        _check(st, where, self.h)
+
+    # -- the reference's stage methods ------------------------------------
+    def set_points(self, xy: np.ndarray, tick: int) -> None:
+        """bench->points = trace + t*N; bench->cur_time = st + t (process.cpp:434-435)."""
+        xy = np.ascontiguousarray(xy, dtype=np.float64)
+        assert xy.size == 2 * self.n
+        self._xy_keep = xy
+        self._chk(lib().glint_set_positions(self.h, xy.ctypes.data, self.n, tick), "glint_set_positions")
+
+    def set_points_device(self, d_xy: int, tick: int) -> None:
+        self._chk(lib().glint_set_positions_device(self.h, ctypes.c_void_p(d_xy), self.n, tick),
+                  "glint_set_positions_device")
+
+    def filter(self) -> None:
+        """workbench::filter (filter.cpp:290-313)."""
+        self._chk(lib().glint_partition(self.h), "glint_partition")
+
+    def reachability(self) -> None:
+        """workbench::reachability (refinement.cpp:66-85)."""
+        self._chk(lib().glint_refine(self.h), "glint_refine")
+
+    def update_meetings(self, copy_out: bool = True) -> np.ndarray | None:
+        """workbench::update_meetings (meeting.cpp:166-189); returns the tick's
+        meetings as an (M, 4) uint32 array (pid_a, pid_b, start, end), sorted."""
+        if not copy_out:
+            self._chk(lib().glint_evaluate(self.h, None, 0, None), "glint_evaluate")
+            return None
+        n = ctypes.c_uint64(0)
+        st = lib().glint_evaluate(self.h, self._meet_buf, self._meet_cap, ctypes.byref(n))
+        if st == 8 and n.value > self._meet_cap:   # output capacity: grow and fetch
+            self._meet_cap = int(n.value) * 2
+            self._meet_buf = (Meeting * self._meet_cap)()
+            ptr = ctypes.c_void_p()
+            cnt = ctypes.c_uint64()
+            self._chk(lib().glint_device_meetings(self.h, ctypes.byref(ptr), ctypes.byref(cnt)),
+                      "glint_device_meetings")
+            arr = np.zeros((cnt.value, 4), dtype=np.uint32)
+            _check(lib().glint_memcpy_to_host(arr.ctypes.data, ptr, arr.nbytes), "glint_memcpy_to_host")
+            order = np.lexsort((arr[:, 2], arr[:, 1], arr[:, 0]))
+            return arr[order]
+        self._chk(st, "glint_evaluate")
+        arr = np.ctypeslib.as_array(self._meet_buf).view(np.uint32).reshape(-1, 4)[: n.value]
+        return arr.copy()
+
+    def update_schema(self) -> None:
+        """workbench::update_schema (filter.cpp:428-449)."""
+        self._chk(lib().glint_update_schema(self.h), "glint_update_schema")
+
+    # -- convenience --------------------------------------------------------
+    def tick(self, xy: np.ndarray, t: int) -> np.ndarray:
+        self.set_points(xy, t)
+        self.filter()
+        self.reachability()
+        m = self.update_meetings()
+        self.update_schema()
+        return m
+
+    def run_ticks(self, d_trace: int, first_tick: int, nticks: int, latencies: bool = True) -> np.ndarray | None:
+        ms = np.zeros(nticks, dtype=np.float32) if latencies else None
+        self._chk(lib().glint_run_ticks(self.h, ctypes.c_void_p(d_trace), first_tick, nticks,
+                                        ms.ctypes.data if latencies else None), "glint_run_ticks")
+        return ms
+
+    def synchronize(self) -> None:
+        self._chk(lib().glint_synchronize(self.h), "glint_synchronize")
+
+    def stats(self) -> dict:
+        s = TickStats()
+        self._chk(lib().glint_stats(self.h, ctypes.byref(s)), "glint_stats")
+        return {f: getattr(s, f) for f, _ in TickStats._fields_}
+
+    def contacts(self) -> np.ndarray:
+        n = ctypes.c_uint64()
+        self._chk(lib().glint_contacts(self.h, None, 0, ctypes.byref(n)), "glint_contacts")
+        out = np.zeros(max(1, n.value), dtype=np.uint64)
+        self._chk(lib().glint_contacts(self.h, out.ctypes.data, out.size, ctypes.byref(n)), "glint_contacts")
+        return out[: n.value]
+
+    def leaves(self):
+        n = ctypes.c_uint64()
+        self._chk(lib().glint_leaves(self.h, None, None, None, 0, ctypes.byref(n)), "glint_leaves")
+        boxes = np.zeros((max(1, n.value), 4), dtype=np.float64)
+        levels = np.zeros(max(1, n.value), dtype=np.uint32)
+        counts = np.zeros(max(1, n.value), dtype=np.uint32)
+        self._chk(lib().glint_leaves(self.h, boxes.ctypes.data, levels.ctypes.data, counts.ctypes.data,
+                                     n.value, ctypes.byref(n)), "glint_leaves")
+        k = n.value
+        return boxes[:k], levels[:k], counts[:k]
+
+    def active_pairs(self):
+        n = ctypes.c_uint64()
+        self._chk(lib().glint_active_pairs(self.h, None, None, 0, ctypes.byref(n)), "glint_active_pairs")
+        keys = np.zeros(max(1, n.value), dtype=np.uint64)
+        starts = np.zeros(max(1, n.value), dtype=np.uint32)
+        self._chk(lib().glint_active_pairs(self.h, keys.ctypes.data, starts.ctypes.data, keys.size,
+                                           ctypes.byref(n)), "glint_active_pairs")
+        return keys[: n.value], starts[: n.value]
+
+    def inject_failure(self, stage: str) -> None:
+        self._chk(lib().glint_debug_inject_failure(self.h, stage.encode()), "glint_debug_inject_failure")
+
+
+class DeviceWalkers:
+    """Walker state of the seeded generator held in device memory."""
+
+    def __init__(self, g: _gen.Generator, device: int = 0):
+        n = g.w.num_objects
+        self.cfg = g.cfg
+        self.n = n
+        self.device = device
+        self.bufs = {
+            "xy": DeviceBuffer(16 * n, device), "speed": DeviceBuffer(8 * n, device),
+            "tx": DeviceBuffer(4 * n, device), "ty": DeviceBuffer(4 * n, device),
+            "dir": DeviceBuffer(max(1, n), device),
+        }
+        self.bufs["xy"].upload(g.xy)
+        self.bufs["speed"].upload(g.speed)
+        self.bufs["tx"].upload(g.tx)
+        self.bufs["ty"].upload(g.ty)
+        self.bufs["dir"].upload(g.dir)
+        b = self.bufs
+        self.w = _gen.Walkers(ctypes.cast(ctypes.c_void_p(b["xy"].address()), ctypes.POINTER(ctypes.c_double)),
+                              ctypes.cast(ctypes.c_void_p(b["speed"].address()), ctypes.POINTER(ctypes.c_double)),
+                              ctypes.cast(ctypes.c_void_p(b["tx"].address()), ctypes.POINTER(ctypes.c_int32)),
+                              ctypes.cast(ctypes.c_void_p(b["ty"].address()), ctypes.POINTER(ctypes.c_int32)),
+                              ctypes.cast(ctypes.c_void_p(b["dir"].address()), ctypes.POINTER(ctypes.c_uint8)))
+        self.tick = g.tick
+
+    def walk(self, nticks: int, trace: DeviceBuffer | None = None, trace_offset: int = 0) -> None:
+        """Advance nticks ticks; tick k's positions go to trace at trace_offset + k*16N."""
+        ptr = ctypes.c_void_p(trace.address(trace_offset)) if trace is not None else None
+        _check(lib().glint_gen_walk_device(ctypes.byref(self.cfg), ctypes.byref(self.w), self.tick + 1, nticks,
+                                           ptr, self.device), "glint_gen_walk_device")
+        self.tick += nticks
+
+    def positions(self) -> np.ndarray:
+        out = np.zeros(2 * self.n, dtype=np.float64)
+        return self.bufs["xy"].download(out)
+
+
+class Tracer:
+    """tracer::process (process.cpp:378-634) over a host generator: per tick
+    set the positions, then filter, reachability, update_meetings and
+    update_schema; collects every emitted meeting."""
+
+    def __init__(self, workload: _gen.Workload, **cfg_kw):
+        self.workload = workload
+        kw = dict(reach_distance=workload.reach_distance, grid_capacity=workload.grid_capacity,
+                  min_meet_time=workload.min_meet_time)
+        kw.update(cfg_kw)
+        self.bench = Workbench(make_config(workload.num_objects, workload.box(), **kw))
+        self.generator = _gen.Generator(workload)
+
+    def process(self, ticks: int | None = None) -> np.ndarray:
+        T = self.workload.ticks if ticks is None else ticks
+        out = []
+        for t in range(T):
+            xy = self.generator.positions() if t == 0 else self.generator.step()
+            out.append(self.bench.tick(xy, t))
+        return np.concatenate(out) if out else np.zeros((0, 4), dtype=np.uint32)
+
+    def close(self):
+        self.bench.close()

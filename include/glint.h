@@ -1,0 +1,218 @@
+/*
+ * glint.h - C ABI of the MI355X (gfx950) contact-tracing path:
+ * partition -> refine -> evaluate -> schema update, one tick at a time.
+ *
+ * The reference has no plugin or FFI API for this path.  Its seam is two
+ * C++-linkage functions declared at src/tracing/process.cpp:373-376 and the
+ * CPU stage methods of class workbench (src/tracing/workbench.h:301-313):
+ *
+ *   workbench *cuda_create_device_bench(workbench*, gpu_info*)   geometry.cu:1395
+ *   void process_with_gpu(workbench*, workbench* d_bench, gpu_info*)  geometry.cu:1573
+ *   void workbench::filter()          filter.cpp:290   (partition + lookup)
+ *   void workbench::reachability()    refinement.cpp:66 (refine)
+ *   void workbench::update_meetings() meeting.cpp:166  (evaluate)
+ *   void workbench::update_schema()   filter.cpp:428   (schema update)
+ *
+ * Each entry point below names the reference function it replaces.  All
+ * functions return a glint_status instead of exit()/assert (the reference's
+ * CUDA_SAFE_CALL exits, cuda_util.cuh:19-27).  Pointers are plain host or
+ * device pointers; no C++ or framework types cross the boundary.
+ *
+ * Threading: one context owns one HIP stream; calls on one context must not
+ * overlap; different contexts may be used from different threads.
+ *
+ * Coordinates are planar metres (FP64), one (x, y) pair per object id, the
+ * layout of the reference's Point array (geometry.h:23-56, 16 bytes each).
+ */
+#ifndef GLINT_H
+#define GLINT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "glint_gen.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum glint_status {
+    GLINT_OK = 0,
+    GLINT_ERR_INVALID_ARGUMENT = 1,
+    GLINT_ERR_OUT_OF_MEMORY = 2,
+    GLINT_ERR_HIP = 3,             /* HIP runtime error (no device, launch failure) */
+    GLINT_ERR_BAD_INPUT = 4,       /* NaN or a position outside the root box */
+    GLINT_ERR_TABLE_FULL = 5,      /* meeting table probe sequence exhausted */
+    GLINT_ERR_NODE_CAPACITY = 6,   /* quadtree node pool exhausted */
+    GLINT_ERR_UNIT_CAPACITY = 7,   /* border check-unit buffer overflow */
+    GLINT_ERR_OUTPUT_CAPACITY = 8, /* meeting or contact buffer overflow */
+    GLINT_ERR_STATE = 9,           /* call out of order (e.g. refine before partition) */
+    GLINT_ERR_INJECTED = 10        /* failure forced by glint_debug_inject_failure */
+} glint_status;
+
+enum {
+    GLINT_CANDIDATES_REFERENCE = 0, /* the reference's top/right lookup rule (filter.cpp:139-141) */
+    GLINT_CANDIDATES_COMPLETE = 1   /* every pair within reach; a superset, for diagnostics */
+};
+
+enum {
+    GLINT_PRECISION_FP64 = 0,       /* FP64 distance test for every candidate pair */
+    GLINT_PRECISION_FP32_GUARD = 1  /* FP32 test with a guard band, FP64 recheck inside it */
+};
+
+/* Mirrors the hot-path fields of the reference's configuration
+ * (src/util/config.h:28-48, defaults at config.h:173-187). */
+typedef struct glint_config {
+    uint32_t num_objects;          /* config.h:21 */
+    uint32_t grid_capacity;        /* config.h:29: leaf split threshold */
+    uint32_t min_meet_time;        /* config.h:39 */
+    uint32_t max_meet_time;        /* config.h:40: longer meetings are flagged oversize */
+    uint32_t schema_update_delay;  /* config.h:38 */
+    uint32_t dynamic_schema;       /* config.h:35 */
+    uint32_t start_time;           /* config.h:28 */
+    uint32_t candidate_mode;       /* GLINT_CANDIDATES_* */
+    uint32_t precision;            /* GLINT_PRECISION_* */
+    uint32_t device;               /* HIP device ordinal */
+    double low_x, low_y, high_x, high_y;  /* root box (workbench::mbr) */
+    double reach_distance;         /* config.h:41, metres */
+    double guard_epsilon;          /* FP32 guard band half-width, metres */
+    uint64_t num_meeting_buckets;  /* slots per table generation; 0: automatic (power of 2) */
+    uint64_t unit_capacity;        /* border check units per tick; 0: automatic */
+    uint64_t meeting_capacity;     /* meetings per tick; 0: automatic */
+    uint64_t contact_capacity;     /* recorded contacts per tick (record_contacts) */
+    uint32_t node_capacity;        /* quadtree nodes; 0: automatic */
+    uint32_t record_contacts;      /* keep each tick's contact pairs for glint_contacts() */
+} glint_config;
+
+/* One emitted meeting: pid_a < pid_b, first and last tick of contact. */
+typedef struct glint_meeting {
+    uint32_t pid_a, pid_b, start, end;
+} glint_meeting;
+
+typedef struct glint_tick_stats {
+    uint32_t tick;
+    uint32_t num_leaves;           /* leaves after the partition of this tick */
+    uint32_t num_nodes;            /* allocated quadtree nodes */
+    uint32_t num_border;           /* objects with last_valid != leaf */
+    uint32_t max_leaf_count;
+    uint32_t splits, merges;       /* applied by the last schema update */
+    uint32_t error_flags;          /* device-side error bits (0 when healthy) */
+    uint64_t num_units;            /* border check units */
+    uint64_t num_contacts;         /* contact pairs found (with duplicates) */
+    uint64_t num_active;           /* distinct active pairs after this tick */
+    uint64_t num_meetings;         /* meetings emitted this tick */
+    uint64_t num_rechecks;         /* FP64 rechecks (FP32 guard mode) */
+    uint64_t table_slots;
+    float ms_partition, ms_lookup, ms_refine, ms_evaluate, ms_schema;  /* HIP event times */
+} glint_tick_stats;
+
+typedef struct glint_ctx glint_ctx;
+
+const char *glint_status_string(glint_status s);
+/* Detail of the last failure on this context (or the last create failure). */
+const char *glint_last_error_message(const glint_ctx *ctx);
+
+/* Defaults of the reference (config.h:29-41): capacity 50, min 5, r = 2 m,
+ * delay 1, dynamic schema on; the caller sets num_objects and the box. */
+void glint_config_default(glint_config *cfg);
+
+/* Replaces cuda_create_device_bench (geometry.cu:1395-1565): allocates the
+ * device state (quadtree pool, sorted object arrays, two meeting-table
+ * generations, output buffers) on cfg->device. */
+glint_status glint_create(const glint_config *cfg, glint_ctx **out);
+glint_status glint_destroy(glint_ctx *ctx);
+
+/* Positions of every object for tick `tick` (the reference's
+ * bench->points = trace + t*N and bench->cur_time = st + t,
+ * process.cpp:434-435).  The host variant copies N*16 bytes to the device
+ * (the per-tick H2D copy of geometry.cu:1625); the device variant uses the
+ * caller's device buffer in place, which must stay valid until the tick's
+ * glint_update_schema has returned. */
+glint_status glint_set_positions(glint_ctx *ctx, const double *xy, uint32_t n, uint32_t tick);
+glint_status glint_set_positions_device(glint_ctx *ctx, const double *d_xy, uint32_t n, uint32_t tick);
+
+/* Replaces the partition and filter half of workbench::filter()
+ * (filter.cpp:210-313) and cuda_partition + cuda_filtering
+ * (geometry.cu:221-392): on the first call builds the quadtree from these
+ * positions (QTree.h:157-214, partitioner::build_schema filter.cpp:27-93);
+ * then assigns every object to its leaf, sorts objects by leaf and generates
+ * the border check units with the reference's lookup rule. */
+glint_status glint_partition(glint_ctx *ctx);
+
+/* Replaces workbench::reachability() (refinement.cpp:66-85) and
+ * cuda_unroll + cuda_refinement (geometry.cu:401-497): the distance join of
+ * every check unit and the meeting-table upsert of every contact. */
+glint_status glint_refine(glint_ctx *ctx);
+
+/* Replaces workbench::update_meetings() (meeting.cpp:166-189) and
+ * cuda_identify_meetings (geometry.cu:595-737): emits the meetings that
+ * ended before this tick and lasted long enough, and retires every pair not
+ * seen this tick.  If `out` is not NULL the tick's meetings are copied to it,
+ * sorted by (pid_a, pid_b); *n_out receives the count (also when it exceeds
+ * cap, in which case GLINT_ERR_OUTPUT_CAPACITY is returned). */
+glint_status glint_evaluate(glint_ctx *ctx, glint_meeting *out, uint64_t cap, uint64_t *n_out);
+
+/* Replaces workbench::update_schema() (filter.cpp:428-449) and
+ * cuda_update_schema_collect/split/merge (geometry.cu:1114-1234,2160-2182):
+ * one level of split/merge decided from this tick's leaf counts. */
+glint_status glint_update_schema(glint_ctx *ctx);
+
+/* The four stages above for one tick (process_with_gpu's whole body). */
+glint_status glint_process_tick(glint_ctx *ctx, const double *xy, uint32_t tick);
+
+/* Runs `nticks` consecutive ticks from a device-resident trace laid out
+ * tick-major (tick k at d_trace + 2*N*k), with no host synchronisation
+ * between ticks; meetings stay on the device.  If tick_ms is not NULL it
+ * receives each tick's HIP-event latency in milliseconds.  The reference
+ * drives the same loop from the host, one tick per call (process.cpp:431). */
+glint_status glint_run_ticks(glint_ctx *ctx, const double *d_trace, uint32_t first_tick,
+                             uint32_t nticks, float *tick_ms);
+
+/* Waits for the context's stream and reports device-side errors. */
+glint_status glint_synchronize(glint_ctx *ctx);
+
+/* Statistics of the last completed tick (synchronises). */
+glint_status glint_stats(glint_ctx *ctx, glint_tick_stats *out);
+
+/* Contact pairs of the last tick ((a<<32)|b, a<b, sorted, unique); requires
+ * cfg.record_contacts. */
+glint_status glint_contacts(glint_ctx *ctx, uint64_t *out, uint64_t cap, uint64_t *n_out);
+
+/* Leaves of the current quadtree in depth-first (BL, BR, TL, TR) order:
+ * boxes as (lx, ly, hx, hy), depth, and the population of the last tick. */
+glint_status glint_leaves(glint_ctx *ctx, double *boxes, uint32_t *levels, uint32_t *counts,
+                          uint64_t cap, uint64_t *n_out);
+
+/* Active pairs after the last tick: keys (a<<32)|b and start ticks, sorted. */
+glint_status glint_active_pairs(glint_ctx *ctx, uint64_t *keys, uint32_t *starts, uint64_t cap,
+                                uint64_t *n_out);
+
+/* Device pointer to the last tick's meetings (unsorted) and their count;
+ * valid until the next glint_evaluate. */
+glint_status glint_device_meetings(glint_ctx *ctx, const glint_meeting **d_out, uint64_t *n_out);
+
+/* Forces the next call of the named stage ("partition", "refine",
+ * "evaluate", "update_schema", "alloc") to fail with GLINT_ERR_INJECTED, for
+ * failure-path tests. */
+glint_status glint_debug_inject_failure(glint_ctx *ctx, const char *stage);
+
+/* Device memory helpers for callers without a GPU framework. */
+glint_status glint_device_malloc(void **ptr, uint64_t bytes, uint32_t device);
+glint_status glint_device_free(void *ptr);
+glint_status glint_memcpy_to_device(void *dst, const void *src, uint64_t bytes);
+glint_status glint_memcpy_to_host(void *dst, const void *src, uint64_t bytes);
+glint_status glint_device_count(int *count);
+
+/* Device stepper of the seeded generator (glint_gen.h): advances walkers
+ * held in device memory from tick first_tick-1 and writes nticks snapshots of
+ * the positions, tick-major, to d_trace (which may be NULL to only advance).
+ * Bit-identical to glint_gen_step on the host. */
+glint_status glint_gen_walk_device(const glint_gen_config *cfg, glint_walkers *d_walkers,
+                                   uint32_t first_tick, uint32_t nticks, double *d_trace,
+                                   uint32_t device);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GLINT_H */

@@ -1,0 +1,116 @@
+/*
+ * oracle.h - CPU restatement of GLINT's per-tick contact-tracing path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load this library, and only as the checker
+ * or as the reported CPU baseline.  The product (libglint.so) never links,
+ * loads or calls it.
+ *
+ * What it restates (reference paths relative to the reference repository):
+ *   tree build      src/index/QTree.h:98-214 (QTNode ctor, split, insert)
+ *   filter          src/tracing/filter.cpp:210-288 (filter_unit, phased lookup)
+ *   lookup          src/tracing/filter.cpp:130-152 (lookup_rec, top/right rule)
+ *   refine          src/tracing/refinement.cpp:10-64 (reachability_unit)
+ *   evaluate        src/tracing/meeting.cpp:32-76 (update_meetings_unit)
+ *   schema update   src/cuda/geometry.cu:1114-1234 (collect, split, merge)
+ *   predicates      src/geometry/geometry.h:40-48,140-145,174-185
+ *   tick order      src/tracing/process.cpp:431-612
+ * with coordinates in planar metres (geography=false predicates) and the
+ * deterministic semantics the reference leaves undefined: no leaf-overflow
+ * drops, one table entry per unordered pair, unbounded probing, schema
+ * decisions taken from the counts of the tick before any split or merge.
+ */
+#ifndef GLINT_ORACLE_H
+#define GLINT_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct oracle_config {
+    uint32_t num_objects;
+    uint32_t grid_capacity;        /* config.h:29 */
+    uint32_t min_meet_time;        /* config.h:39 */
+    uint32_t schema_update_delay;  /* config.h:38 */
+    uint32_t dynamic_schema;       /* config.h:35 */
+    uint32_t complete_mode;        /* 1: every pair within r, no top/right rule */
+    uint32_t num_threads;          /* 0: OpenMP default */
+    uint32_t start_time;           /* first tick number, config.h:28 */
+    double low_x, low_y, high_x, high_y;   /* root box */
+    double reach_distance;         /* config.h:41 */
+} oracle_config;
+
+typedef struct oracle_stats {
+    uint64_t num_leaves;
+    uint64_t num_nodes;
+    uint64_t num_border;          /* objects with last_valid != leaf */
+    uint64_t num_units;           /* check units (one per object + border units) */
+    uint64_t num_tests;           /* distance tests performed */
+    uint64_t num_contacts;        /* unique contact pairs this tick */
+    uint64_t num_active;          /* active pairs after this tick */
+    uint64_t num_meetings;        /* meetings emitted this tick */
+    uint64_t max_leaf_count;      /* largest leaf population this tick */
+    uint64_t ref_drop_leaves;     /* leaves over 2*grid_capacity (reference would drop) */
+    uint64_t splits, merges;      /* schema changes applied after this tick */
+    double t_filter, t_refine, t_evaluate, t_schema;  /* seconds */
+} oracle_stats;
+
+void *oracle_create(const oracle_config *cfg);
+void oracle_destroy(void *h);
+
+/* One tick: xy is AoS (x0,y0,x1,y1,...) for all objects, tick is cur_time.
+ * Builds the tree on the first call (QTree.h build rule), then filter,
+ * refine, evaluate and (if dynamic) the schema update.  Returns 0 or <0. */
+int oracle_tick(void *h, const double *xy, uint32_t tick);
+
+/* Results of the last tick.  Contacts are (a<<32)|b with a<b, sorted,
+ * unique.  Meetings are 4 x u32 (pid_a, pid_b, start, end), sorted. */
+uint64_t oracle_num_contacts(void *h);
+const uint64_t *oracle_contacts(void *h);
+uint64_t oracle_num_meetings(void *h);
+const uint32_t *oracle_meetings(void *h);
+void oracle_get_stats(void *h, oracle_stats *out);
+
+/* Active pairs (key, start) after the last tick, sorted by key. */
+uint64_t oracle_num_active(void *h);
+void oracle_get_active(void *h, uint64_t *keys, uint32_t *starts);
+
+/* Emit every still-active pair as if it ended now (not a reference
+ * behaviour: the reference never emits meetings active at the last tick). */
+uint64_t oracle_flush(void *h, uint32_t *out4, uint64_t cap);
+
+/* Leaves of the current tree, in depth-first (BL, BR, TL, TR) order:
+ * boxes 4 doubles each (lx, ly, hx, hy), level, and the leaf population of
+ * the last tick (0 before the first tick). */
+uint64_t oracle_num_leaves(void *h);
+uint64_t oracle_get_leaves(void *h, double *boxes, uint32_t *levels, uint32_t *counts,
+                           uint64_t cap);
+
+/* Build the tree from xy without running a tick (tests). */
+int oracle_build(void *h, const double *xy);
+
+/* Descend one point: returns the depth-first index of its leaf, writes the
+ * range [lo, hi) of depth-first leaf indices under last_valid and the
+ * candidate leaves (depth-first leaf indices) of the lookup. */
+int64_t oracle_locate(void *h, double x, double y, int64_t *lv_span_lo, int64_t *lv_span_hi,
+                      uint32_t *cands, uint32_t cap, uint32_t *ncand);
+
+/* Feed a contact set directly to the evaluate stage (tests of the meeting
+ * rule): keys sorted or not, duplicates allowed. */
+int oracle_evaluate_contacts(void *h, const uint64_t *keys, uint64_t n, uint32_t tick);
+
+/* Apply one schema update from explicit per-leaf counts given in the
+ * depth-first leaf order (tests of split/merge). */
+int oracle_update_schema_counts(void *h, const uint32_t *counts, uint64_t nleaves);
+
+/* Predicates, exactly as the reference evaluates them in planar metres. */
+int oracle_point_within(double x1, double y1, double x2, double y2, double r);
+double oracle_box_distance(double lx, double ly, double hx, double hy, double px, double py);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
