@@ -1,0 +1,227 @@
+"""Benchmark of the contact-tracing hot path on one MI355X.
+
+A step is one tick of the path (partition -> refine -> evaluate -> schema
+update) over every object.  Default workload: the skewed-density scenario,
+10M objects in a 30 km box, 80% starting in 64 Gaussian hotspots (sigma 200 m,
+Zipf s=1.1 weights), 20% uniform, all walking a 50 m Manhattan grid at
+U[1,15) m/s; reach 2 m, minimum meeting 5 ticks, leaf capacity 200.  Positions
+for every tick are generated on the device before the timed region starts.
+
+Prints one JSON line (rank 0) with object-updates/s, p50/p99 tick latency,
+the roofline figure of the dominant kernel (HIP events around every launch
+on the context's stream) and the CPU oracle timed on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "object-updates/sec and p99 tick latency at 10M objects, 1 MI355X, exact contacts"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def workload_for(name: str, objects: int | None):
+    from glint_pipeline_amd import gen
+    if name == "skewed":
+        w = gen.skewed_density(objects or 10_000_000, 300)
+        desc = ("skewed density: {n} objects, 30 km box, 80% in 64 Gaussian hotspots (sigma 200 m, Zipf s=1.1), "
+                "20% uniform, Manhattan walkers U[1,15) m/s, r=2 m, min meeting 5, leaf capacity 200")
+    elif name == "plain_gpu":
+        w = gen.plain_gpu_form()
+        if objects:
+            w.num_objects = objects
+        desc = "plain GPU form: {n} objects, 30 km box, uniform Manhattan walkers, r=2 m, min 5, leaf capacity 200"
+    elif name == "plain_cpu":
+        w = gen.plain_cpu_reference()
+        if objects:
+            w.num_objects = objects
+        desc = "plain CPU reference: {n} objects, 10 km box, uniform Manhattan walkers, r=2 m, min 5, capacity 200"
+    else:
+        raise SystemExit(f"unknown workload {name}")
+    return w, desc.format(n=w.num_objects)
+
+
+def algorithmic_bytes(kernel: str, n: int, per_tick: dict, slots: int, precision: int) -> float:
+    """Algorithmic HBM bytes of one launch of `kernel` (one tick), from the
+    per-tick figures: N objects, C contacts (border share Cb), M meetings,
+    U border units, B table slots."""
+    C, Cb, M, U = per_tick["contacts"], per_tick["contacts_border"], per_tick["meetings"], per_tick["units"]
+    coord = 8 if precision else 16          # FP32 offsets or FP64 coordinates per object
+    if kernel == "k_join_inside":
+        return coord * n + 16.0 * (C - Cb)    # every sorted coordinate once, one 16 B upsert per contact
+    if kernel == "k_join_border":
+        return 8.0 * U + 16.0 * U + 16.0 * Cb  # unit, the probing object's xy, upserts
+    if kernel == "k_evaluate":
+        return 16.0 * slots + 16.0 * M        # table scan + meeting output
+    if kernel == "k_assign":
+        return 16.0 * n + 8.0 * n             # xy in, leaf and rank out
+    if kernel == "k_scatter":
+        return 16.0 * n + 8.0 * n + 20.0 * n  # xy, leaf/rank in; sorted xy + id out
+    if kernel == "k_lookup":
+        return 8.0 * per_tick["border_objects"] + 16.0 * per_tick["border_objects"] + 8.0 * U
+    return 0.0
+
+
+def cpu_baseline(w, ticks: int, threads: int) -> dict:
+    """The CPU oracle (C + OpenMP) on the first ticks of the same workload."""
+    from glint_pipeline_amd import gen
+    from oracle.oracle import Oracle
+    g = gen.Generator(w)
+    o = Oracle(w.num_objects, w.box(), w.reach_distance, w.grid_capacity, w.min_meet_time, num_threads=threads)
+    t_build0 = time.perf_counter()
+    o.tick(g.positions(), 0)          # tick 0 builds the tree (not timed, as on the GPU side)
+    t_build = time.perf_counter() - t_build0
+    tt = 0.0
+    for t in range(1, ticks + 1):
+        xy = g.step()
+        t0 = time.perf_counter()
+        o.tick(xy, t)
+        tt += time.perf_counter() - t0
+    st = o.stats()
+    o.close()
+    return {"seconds": tt, "ticks": ticks, "tick0_seconds": t_build, "contacts_last_tick": st["num_contacts"]}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="skewed", choices=["skewed", "plain_gpu", "plain_cpu"])
+    ap.add_argument("--objects", type=int, default=None)
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32guard"])
+    ap.add_argument("--buckets-log2", type=int, default=None)
+    ap.add_argument("--cpu-ticks", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-kernel-times", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
+                          "n_gpus": args.gpus}))
+        return 2
+
+    from glint_pipeline_amd import gen, runtime
+
+    if runtime.device_count() < 1:
+        raise SystemExit("no HIP device visible")
+    w, desc = workload_for(args.workload, args.objects)
+    N = w.num_objects
+    W, K = args.warmup, args.steps
+    precision = runtime.PRECISION_FP32_GUARD if args.precision == "fp32guard" else runtime.PRECISION_FP64
+    if args.buckets_log2 is not None:
+        slots = 1 << args.buckets_log2
+    else:
+        # the table holds one generation of distinct contacts at load <= ~0.5
+        est = {"skewed": 90.0, "plain_gpu": 0.2, "plain_cpu": 0.2}[args.workload]
+        slots = 1 << max(20, math.ceil(math.log2(max(1.0, 2.2 * est * N * (N / 1e7 if args.workload == "skewed" else 1)))))
+    log(f"[bench] workload: {desc}; warmup {W}, timed {K} ticks; table slots 2^{int(math.log2(slots))}")
+
+    # ---- inputs: host init of tick 0, device walk for the rest, all in HBM
+    t0 = time.perf_counter()
+    g = gen.Generator(w)
+    dw = runtime.DeviceWalkers(g)
+    total_ticks = W + K
+    trace = runtime.DeviceBuffer(16 * N * total_ticks)
+    trace.upload(g.positions(), 0)
+    if total_ticks > 1:
+        dw.walk(total_ticks - 1, trace, 16 * N)
+    log(f"[bench] generated {total_ticks} ticks of positions on the device in {time.perf_counter() - t0:.1f} s")
+
+    cfg = runtime.make_config(N, w.box(), reach_distance=w.reach_distance, grid_capacity=w.grid_capacity,
+                              min_meet_time=w.min_meet_time, precision=precision, num_meeting_buckets=slots,
+                              unit_capacity=8 * N, meeting_capacity=max(1 << 20, 6 * N))
+    wb = runtime.Workbench(cfg)
+    t0 = time.perf_counter()
+    if W:
+        wb.run_ticks(trace.address(0), 0, W, latencies=False)
+    wb.synchronize()
+    log(f"[bench] warmup ({W} ticks incl. tree build) {time.perf_counter() - t0:.2f} s; stats {wb.stats()}")
+    wb.totals(reset=True)
+    if not args.no_kernel_times:
+        wb.set_kernel_profiling(True)
+        wb.kernel_times(reset=True)
+
+    # ---- timed region
+    wb.synchronize()
+    t0 = time.perf_counter()
+    tick_ms = wb.run_ticks(trace.address(16 * N * W), W, K, latencies=True)
+    wb.synchronize()
+    elapsed = time.perf_counter() - t0
+    totals = wb.totals()
+    ktimes = wb.kernel_times() if not args.no_kernel_times else {}
+    stats = wb.stats()
+    log(f"[bench] timed {K} ticks in {elapsed:.3f} s; totals {totals}")
+    for name, (ms, cnt) in sorted(ktimes.items(), key=lambda kv: -kv[1][0]):
+        if cnt:
+            log(f"[bench]   {name:16s} {ms:10.2f} ms total  {cnt:6d} launches  {ms / cnt:9.3f} ms/launch")
+
+    value = N * K / elapsed
+    per_tick = {k: totals[k] / max(1, totals["ticks"]) for k in
+                ("contacts", "contacts_border", "meetings", "units", "border_objects", "distinct_pairs")}
+    roofline = None
+    if ktimes:
+        dom, (dms, dcnt) = max(((k, v) for k, v in ktimes.items() if v[1]), key=lambda kv: kv[1][0])
+        abytes = algorithmic_bytes(dom, N, per_tick, slots, precision)
+        avg_s = dms / dcnt / 1e3
+        achieved = abytes / avg_s / 1e9 if abytes and avg_s > 0 else 0.0
+        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                    "kernel_ms_per_launch": round(dms / dcnt, 4), "algorithmic_bytes_per_launch": abytes,
+                    "kernel_share_of_tick": round(dms / (elapsed * 1e3), 4)}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+        t0 = time.perf_counter()
+        cb = cpu_baseline(w, args.cpu_ticks, threads)
+        log(f"[bench] cpu baseline {cb} (wall {time.perf_counter() - t0:.1f} s)")
+        cpu = {"value": round(N * cb["ticks"] / cb["seconds"], 1), "unit": "object-updates/s", "cores": threads,
+               "kind": "port",
+               "sample": f"oracle (C, OpenMP) ticks 1..{cb['ticks']} of the same workload after its tick-0 tree "
+                         f"build; {cb['seconds']:.2f} s timed"}
+
+    lat = np.asarray(tick_ms, dtype=np.float64)
+    out = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "object-updates/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": round(elapsed * 1e3 / K, 3),
+        "p50_tick_ms": round(float(np.percentile(lat, 50)), 3),
+        "p99_tick_ms": round(float(np.percentile(lat, 99)), 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if precision == runtime.PRECISION_FP64 else "f32+f64",
+        "data": "synthetic (seeded generator, positions resident in HBM before timing)",
+        "config": {"workload": desc, "objects": N, "first_tick": W, "ticks": K,
+                   "table_slots": slots, "contacts_per_tick": round(per_tick["contacts"]),
+                   "meetings_per_tick": round(per_tick["meetings"], 1), "leaves": stats["num_leaves"]},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    wb.close()
+    trace.free()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstddef>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -34,8 +35,10 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kScanItems = 4;                       // per thread
 constexpr int kScanTile = kBlock * kScanItems;      // elements per scan block
-constexpr int kJoinTile = 1024;                     // points per LDS tile
+constexpr int kTile = 1024;                         // points per LDS tile of the join
+constexpr int kStage = 1024;                        // staged contacts per wave
 constexpr int kLookupStack = 128;
+constexpr int kMaxCand = 32;                        // candidate leaves per border object
 constexpr uint32_t kInsideChunk = 4;                // leaves per work-queue grab
 
 // ------------------------------------------------------------ predicates
@@ -127,6 +130,7 @@ __global__ void k_reset_tick(DevState s)
     Counters *c = s.ctr;
     c->n_units = 0;
     c->n_contacts = 0;
+    c->n_contacts_border = 0;
     c->n_inserted = 0;
     c->n_meet = 0;
     c->n_rechecks = 0;
@@ -342,10 +346,23 @@ __global__ void k_collect(DevState s)
     }
 }
 
+// End of a tick: schema change counts and the running totals.
 __global__ void k_schema_done(DevState s)
 {
-    s.ctr->last_splits = s.ctr->n_split;
-    s.ctr->last_merges = s.ctr->n_merge;
+    Counters *c = s.ctr;
+    c->last_splits = c->n_split;
+    c->last_merges = c->n_merge;
+    c->acc_ticks += 1;
+    c->acc_contacts += c->n_contacts;
+    c->acc_contacts_border += c->n_contacts_border;
+    c->acc_distinct += c->n_inserted;
+    c->acc_meet += c->n_meet;
+    c->acc_units += c->n_units;
+    c->acc_border += c->n_border;
+    c->acc_rechecks += c->n_rechecks;
+    if (c->n_contacts > c->max_contacts) c->max_contacts = c->n_contacts;
+    if (c->n_inserted > c->max_distinct) c->max_distinct = c->n_inserted;
+    if (c->n_meet > c->max_meet) c->max_meet = c->n_meet;
 }
 
 // ------------------------------------------------------------ partition
@@ -416,33 +433,33 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *t
     return before + inc - v;
 }
 
-__global__ __launch_bounds__(kBlock) void k_scan_reduce(DevState s)
+// Three-kernel exclusive scan over per-node counters (node_cap entries).
+__global__ __launch_bounds__(kBlock) void k_scan_reduce(const uint32_t *in, uint32_t n, uint32_t *block_sums)
 {
     uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
     uint32_t v = 0;
     for (int k = 0; k < kScanItems; k++)
-        if (base + k < s.node_cap) v += s.count[base + k];
+        if (base + k < n) v += in[base + k];
     uint32_t total;
     block_exclusive_scan(v, &total);
-    if (threadIdx.x == 0) s.block_sums[blockIdx.x] = total;
+    if (threadIdx.x == 0) block_sums[blockIdx.x] = total;
 }
 
-__global__ __launch_bounds__(kBlock) void k_scan_top(DevState s, uint32_t nblocks)
+__global__ __launch_bounds__(kBlock) void k_scan_top(uint32_t *block_sums, uint32_t nblocks)
 {
     uint32_t carry = 0;
     for (uint32_t b0 = 0; b0 < nblocks; b0 += kBlock) {
         uint32_t b = b0 + threadIdx.x;
-        uint32_t v = b < nblocks ? s.block_sums[b] : 0;
+        uint32_t v = b < nblocks ? block_sums[b] : 0;
         uint32_t total;
         uint32_t ex = block_exclusive_scan(v, &total);
-        if (b < nblocks) s.block_sums[b] = carry + ex;
+        if (b < nblocks) block_sums[b] = carry + ex;
         carry += total;
     }
 }
 
-// Leaf offsets in node-index order, plus the list of leaves with >= 2
-// objects (the inside-join work) and the leaf statistics.
-__global__ __launch_bounds__(kBlock) void k_scan_down(DevState s)
+// Object offsets of every leaf (node-index order) and the leaf statistics.
+__global__ __launch_bounds__(kBlock) void k_scan_objects(DevState s)
 {
     uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
     uint32_t c[kScanItems];
@@ -456,16 +473,13 @@ __global__ __launch_bounds__(kBlock) void k_scan_down(DevState s)
     uint32_t mx = 0, nleaf = 0;
     for (int k = 0; k < kScanItems; k++) {
         uint32_t n = base + k;
-        bool in = n < s.node_cap;
-        if (in) s.offset[n] = ex;
+        if (n < s.node_cap) {
+            s.offset[n] = ex;
+            nleaf += s.meta[n].type == kLeaf ? 1u : 0u;
+        }
         ex += c[k];
         mx = max(mx, c[k]);
-        nleaf += (in && s.meta[n].type == kLeaf) ? 1u : 0u;
-        bool work = in && c[k] >= 2;
-        uint32_t idx = wave_append_u32(&s.ctr->n_leaf_list, work);
-        if (work) s.leaf_list[idx] = n;
     }
-    // wave reductions, then one atomic per wave
     for (int d = 32; d >= 1; d >>= 1) {
         mx = max(mx, (uint32_t)__shfl_xor(mx, d));
         nleaf += (uint32_t)__shfl_xor(nleaf, d);
@@ -473,6 +487,30 @@ __global__ __launch_bounds__(kBlock) void k_scan_down(DevState s)
     if (lane_id() == 0) {
         atomicMax(&s.ctr->max_leaf, mx);
         atomicAdd(&s.ctr->n_leaves, nleaf);
+    }
+}
+
+// Unit offsets of every leaf, and the join work list: leaves with >= 2
+// objects (inside pairs) or with border units aimed at them.
+__global__ __launch_bounds__(kBlock) void k_scan_units(DevState s)
+{
+    uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    uint32_t c[kScanItems];
+    uint32_t v = 0;
+    for (int k = 0; k < kScanItems; k++) {
+        c[k] = base + k < s.node_cap ? s.ucount[base + k] : 0;
+        v += c[k];
+    }
+    uint32_t total;
+    uint32_t ex = block_exclusive_scan(v, &total) + s.block_sums[blockIdx.x];
+    for (int k = 0; k < kScanItems; k++) {
+        uint32_t n = base + k;
+        bool in = n < s.node_cap;
+        if (in) s.ustart[n] = ex;
+        ex += c[k];
+        bool work = in && (c[k] > 0 || s.count[n] >= 2);
+        uint32_t idx = wave_append_u32(&s.ctr->n_leaf_list, work);
+        if (work) s.leaf_list[idx] = n;
     }
 }
 
@@ -495,45 +533,85 @@ __global__ __launch_bounds__(kBlock) void k_scatter(DevState s)
     }
 }
 
+__device__ __forceinline__ uint32_t wave_exclusive_scan(uint32_t v, uint32_t *total)
+{
+    uint32_t inc = wave_inclusive_scan(v);
+    *total = __shfl(inc, 63);
+    return inc - v;
+}
+
 // lookup_rec (filter.cpp:130-152) from last_valid for every border object.
-// In complete mode every leaf within r other than the object's own.
+// In complete mode every leaf within r other than the object's own.  Each
+// unit (object, candidate leaf) takes a rank among the units of its leaf.
 __global__ __launch_bounds__(kBlock) void k_lookup(DevState s)
 {
     const double *xy = s.tp->xy;
     const uint32_t nb = s.ctr->n_border;
-    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nb; k += gridDim.x * blockDim.x) {
-        uint2 b = s.border[k];
-        uint32_t pid = b.x;
-        double x = xy[2 * (size_t)pid], y = xy[2 * (size_t)pid + 1];
-        uint32_t own = s.leaf_of[pid];
-        uint32_t stk[kLookupStack];
-        int sp = 0;
-        stk[sp++] = b.y;
-        while (sp > 0) {
-            uint32_t node = stk[--sp];
-            Node nd = s.nodes[node];
-            for (int i = 0; i < 4; i++) {
-                uint32_t c = nd.child[i];
-                Node ch = s.nodes[c];
-                if (!box_within(ch, x, y, s.t64)) continue;
-                if (ch.child[0] == kLeafTag) {
-                    bool take = s.complete ? (c != own)
-                                           : (y < ch.ly || (y < ch.hy && x < ch.lx));
-                    if (take) {
-                        unsigned long long u = atomicAdd(&s.ctr->n_units, 1ull);
-                        if (u < s.unit_cap)
-                            s.units[u] = make_uint2(pid, c);
+    for (uint32_t base = blockIdx.x * blockDim.x; base < nb; base += gridDim.x * blockDim.x) {
+        uint32_t k = base + threadIdx.x;
+        uint32_t cand[kMaxCand];
+        uint32_t nc = 0;
+        uint32_t pid = 0;
+        if (k < nb) {
+            uint2 b = s.border[k];
+            pid = b.x;
+            double x = xy[2 * (size_t)pid], y = xy[2 * (size_t)pid + 1];
+            uint32_t own = s.leaf_of[pid];
+            uint32_t stk[kLookupStack];
+            int sp = 0;
+            stk[sp++] = b.y;
+            while (sp > 0) {
+                uint32_t node = stk[--sp];
+                Node nd = s.nodes[node];
+                for (int i = 0; i < 4; i++) {
+                    uint32_t c = nd.child[i];
+                    Node ch = s.nodes[c];
+                    if (!box_within(ch, x, y, s.t64)) continue;
+                    if (ch.child[0] == kLeafTag) {
+                        bool take = s.complete ? (c != own) : (y < ch.ly || (y < ch.hy && x < ch.lx));
+                        if (take) {
+                            if (nc < kMaxCand)
+                                cand[nc++] = c;
+                            else
+                                atomicOr(&s.ctr->error, kErrStack);
+                        }
+                    } else {
+                        if (sp < kLookupStack)
+                            stk[sp++] = c;
                         else
-                            atomicOr(&s.ctr->error, kErrUnitCap);
+                            atomicOr(&s.ctr->error, kErrStack);
                     }
-                } else {
-                    if (sp < kLookupStack)
-                        stk[sp++] = c;
-                    else
-                        atomicOr(&s.ctr->error, kErrStack);
                 }
             }
         }
+        // one atomic per wave reserves the wave's units
+        uint32_t wtotal;
+        uint32_t wex = wave_exclusive_scan(nc, &wtotal);
+        unsigned long long wbase = 0;
+        if (lane_id() == 0 && wtotal) wbase = atomicAdd(&s.ctr->n_units, (unsigned long long)wtotal);
+        wbase = __shfl(wbase, 0);
+        for (uint32_t j = 0; j < nc; j++) {
+            unsigned long long u = wbase + wex + j;
+            uint32_t rank = atomicAdd(&s.ucount[cand[j]], 1u);
+            if (u < s.unit_cap) {
+                s.units[u] = make_uint2(pid, cand[j]);
+                s.urank[u] = rank;
+            } else {
+                atomicOr(&s.ctr->error, kErrUnitCap);
+            }
+        }
+    }
+}
+
+// Units grouped by target leaf: unit_pid[ustart[leaf] + rank] = object.
+__global__ __launch_bounds__(kBlock) void k_unit_scatter(DevState s)
+{
+    if (s.ctr->error & kErrUnitCap) return;   // units were dropped: the tick is void
+    const unsigned long long nu = s.ctr->n_units;
+    for (unsigned long long u = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x; u < nu;
+         u += (unsigned long long)gridDim.x * blockDim.x) {
+        uint2 un = s.units[u];
+        s.unit_pid[s.ustart[un.y] + s.urank[u]] = un.x;
     }
 }
 
@@ -542,15 +620,16 @@ __global__ __launch_bounds__(kBlock) void k_lookup(DevState s)
 // Upsert of one contact: look the pair up in the previous generation (its
 // start tick if it was seen at prev_tick, flagged as continued), then insert
 // it into the current generation.  Linear probing, 64-bit CAS on the key.
-__device__ __forceinline__ void upsert(const DevState &s, Slot *tab_old, Slot *tab_cur, uint32_t tick,
-                                       unsigned long long key)
+// Returns 1 when this call inserted the pair.
+__device__ __forceinline__ uint32_t upsert(const DevState &s, Slot *tab_old, Slot *tab_cur, uint32_t tick,
+                                           unsigned long long key)
 {
     const unsigned long long mask = s.tab_mask;
     const unsigned long long h0 = mix64(key) & mask;
     uint32_t start = tick;
     unsigned long long h = h0;
     for (unsigned long long probes = 0; probes <= mask; probes++) {
-        unsigned long long k = __builtin_nontemporal_load(&tab_old[h].key);
+        unsigned long long k = tab_old[h].key;
         if (k == key) {
             start = tab_old[h].start;
             if (!tab_old[h].seen) tab_old[h].seen = 1;
@@ -563,86 +642,189 @@ __device__ __forceinline__ void upsert(const DevState &s, Slot *tab_old, Slot *t
     for (unsigned long long probes = 0;; probes++) {
         if (probes > mask) {
             atomicOr(&s.ctr->error, kErrTableFull);
-            return;
+            return 0;
         }
         unsigned long long prev = atomicCAS(&tab_cur[h].key, kEmptyKey, key);
         if (prev == kEmptyKey) {
             tab_cur[h].start = start;
-            atomicAdd(&s.ctr->n_inserted, 1ull);
-            return;
+            return 1;
         }
-        if (prev == key) return;   // duplicate discovery in this tick: same start
+        if (prev == key) return 0;   // found twice this tick: same start
         h = (h + 1) & mask;
     }
 }
 
-__device__ __forceinline__ void contact(const DevState &s, Slot *tab_old, Slot *tab_cur, uint32_t tick,
-                                        uint32_t a, uint32_t b)
+// ------------------------------------------------------------ refine: join
+
+__device__ __forceinline__ void wave_sync()
 {
-    uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
-    unsigned long long key = ((unsigned long long)lo << 32) | hi;
-    if (s.record) {
-        unsigned long long idx = atomicAdd(&s.ctr->n_rec, 1ull);
-        if (idx < s.rec_cap)
-            s.rec[idx] = key;
-        else
-            atomicOr(&s.ctr->error, kErrContactCap);
-    }
-    upsert(s, tab_old, tab_cur, tick, key);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// ------------------------------------------------------------ refine: joins
-
-// All pairs inside one leaf (the "inside" check units of filter.cpp:107-109
-// with refinement.cpp:30: pid < q, i.e. every unordered pair once).
-__global__ __launch_bounds__(kBlock) void k_join_inside(DevState s)
+__device__ __forceinline__ unsigned long long pair_key(uint32_t a, uint32_t b)
 {
-    __shared__ double ax[kJoinTile], ay[kJoinTile], bx[kJoinTile], by[kJoinTile];
-    __shared__ uint32_t ap[kJoinTile], bp[kJoinTile];
-    __shared__ uint32_t item;
+    uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
+    return ((unsigned long long)lo << 32) | hi;
+}
+
+// Contacts found by one wave are staged in LDS and upserted lane-parallel
+// when the stage fills, so many independent table accesses are in flight.
+struct Stager {
+    unsigned long long *buf;
+    uint32_t n;           // uniform across the wave
+    uint32_t inserted;    // per lane
+};
+
+__device__ __forceinline__ void stage_push(Stager &st, bool hit, unsigned long long key)
+{
+    unsigned long long m = __ballot(hit);
+    if (hit) st.buf[st.n + __popcll(m & ((1ull << lane_id()) - 1ull))] = key;
+    st.n += (uint32_t)__popcll(m);
+}
+
+__device__ __forceinline__ void stage_flush(Stager &st, const DevState &s, Slot *tab_old, Slot *tab_cur,
+                                            uint32_t tick)
+{
+    if (st.n == 0) return;
+    wave_sync();
+    for (uint32_t k = lane_id(); k < st.n; k += 64) {
+        unsigned long long key = st.buf[k];
+        if (s.record) {
+            unsigned long long idx = atomicAdd(&s.ctr->n_rec, 1ull);
+            if (idx < s.rec_cap)
+                s.rec[idx] = key;
+            else
+                atomicOr(&s.ctr->error, kErrContactCap);
+        }
+        st.inserted += upsert(s, tab_old, tab_cur, tick, key);
+    }
+    wave_sync();
+    st.n = 0;
+}
+
+struct JoinLds {
+    double ax[kTile], ay[kTile];
+    double bx[kTile], by[kTile];
+    uint32_t ap[kTile], bp[kTile];
+    unsigned long long stage[kBlock / 64][kStage];
+    uint32_t item;
+};
+
+// The refine stage for one leaf L per block (work queue over the leaf list):
+//  - inside units (filter.cpp:107-109 with refinement.cpp:30, pid < q): every
+//    unordered pair of L's objects once;
+//  - border units (pid, L, inside=false) (refinement.cpp:30 with !inside):
+//    each object whose lookup selected L against every object of L.
+// Work is split into 64x64 wave tasks (64 row lanes x 64 broadcast columns).
+__global__ __launch_bounds__(kBlock) void k_join(DevState s)
+{
+    __shared__ JoinLds lds;
     Slot *tab_old = s.tp->tab_old, *tab_cur = s.tp->tab_cur;
     const uint32_t tick = s.tp->tick;
-    const uint32_t nl = s.ctr->n_leaf_list;
-    unsigned long long ncontacts = 0;
+    const double *xy = s.tp->xy;
+    const double t64 = s.t64;
+    const uint32_t nl = (s.ctr->error & kErrUnitCap) ? 0 : s.ctr->n_leaf_list;
+    const uint32_t wave = threadIdx.x / 64, lane = lane_id();
+    Stager st{lds.stage[wave], 0, 0};
+    uint32_t n_in = 0, n_bd = 0;   // contacts found by this lane
     for (;;) {
-        if (threadIdx.x == 0) item = atomicAdd(&s.ctr->work_inside, kInsideChunk);
+        if (threadIdx.x == 0) lds.item = atomicAdd(&s.ctr->work_inside, kInsideChunk);
         __syncthreads();
-        uint32_t w0 = item;
+        const uint32_t w0 = lds.item;
         __syncthreads();
         if (w0 >= nl) break;
         for (uint32_t w = w0; w < min(w0 + kInsideChunk, nl); w++) {
-            uint32_t leaf = s.leaf_list[w];
-            uint32_t off = s.offset[leaf], cnt = s.count[leaf];
-            uint32_t ntile = (cnt + kJoinTile - 1) / kJoinTile;
+            const uint32_t leaf = s.leaf_list[w];
+            const uint32_t off = s.offset[leaf], cnt = s.count[leaf];
+            const uint32_t uoff = s.ustart[leaf], ucnt = s.ucount[leaf];
+            const uint32_t ntile = (cnt + kTile - 1) / kTile;
             for (uint32_t I = 0; I < ntile; I++) {
-                uint32_t ni = min((uint32_t)kJoinTile, cnt - I * kJoinTile);
+                const uint32_t ni = min((uint32_t)kTile, cnt - I * kTile);
                 for (uint32_t k = threadIdx.x; k < ni; k += kBlock) {
-                    ax[k] = s.sx[off + I * kJoinTile + k];
-                    ay[k] = s.sy[off + I * kJoinTile + k];
-                    ap[k] = s.spid[off + I * kJoinTile + k];
+                    lds.ax[k] = s.sx[off + I * kTile + k];
+                    lds.ay[k] = s.sy[off + I * kTile + k];
+                    lds.ap[k] = s.spid[off + I * kTile + k];
                 }
                 for (uint32_t J = I; J < ntile; J++) {
-                    uint32_t nj = min((uint32_t)kJoinTile, cnt - J * kJoinTile);
-                    const double *jx = ax, *jy = ay;
-                    const uint32_t *jp = ap;
-                    if (J != I) {
+                    const uint32_t nj = min((uint32_t)kTile, cnt - J * kTile);
+                    const bool diag = (J == I);
+                    const double *bx = lds.ax, *by = lds.ay;
+                    const uint32_t *bp = lds.ap;
+                    if (!diag) {
                         for (uint32_t k = threadIdx.x; k < nj; k += kBlock) {
-                            bx[k] = s.sx[off + J * kJoinTile + k];
-                            by[k] = s.sy[off + J * kJoinTile + k];
-                            bp[k] = s.spid[off + J * kJoinTile + k];
+                            lds.bx[k] = s.sx[off + J * kTile + k];
+                            lds.by[k] = s.sy[off + J * kTile + k];
+                            lds.bp[k] = s.spid[off + J * kTile + k];
                         }
-                        jx = bx;
-                        jy = by;
-                        jp = bp;
+                        bx = lds.bx;
+                        by = lds.by;
+                        bp = lds.bp;
                     }
                     __syncthreads();
-                    for (uint32_t i = threadIdx.x; i < ni; i += kBlock) {
-                        double xi = ax[i], yi = ay[i];
-                        uint32_t j0 = (I == J) ? i + 1 : 0;
-                        for (uint32_t j = j0; j < nj; j++) {
-                            if (point_within64(xi, yi, jx[j], jy[j], s.t64)) {
-                                ncontacts++;
-                                contact(s, tab_old, tab_cur, tick, ap[i], jp[j]);
+                    // inside tasks: 64-row sub-tile r against 64-column sub-tile q
+                    const uint32_t nri = (ni + 63) / 64, nrj = (nj + 63) / 64;
+                    const uint32_t ntask_in = diag ? nri * (nri + 1) / 2 : nri * nrj;
+                    // border tasks (only once per tile): 64 units against the tile
+                    const uint32_t nuc = diag ? (ucnt + 63) / 64 : 0;
+                    const uint32_t ntask = ntask_in + nuc;
+                    for (uint32_t t = wave; t < ntask; t += kBlock / 64) {
+                        if (t < ntask_in) {
+                            uint32_t r, q;
+                            if (diag) {   // t -> (r <= q) in row-major upper triangle
+                                r = 0;
+                                uint32_t rem = t;
+                                while (rem >= nri - r) {
+                                    rem -= nri - r;
+                                    r++;
+                                }
+                                q = r + rem;
+                            } else {
+                                r = t / nrj;
+                                q = t % nrj;
+                            }
+                            const uint32_t i = r * 64 + lane;
+                            const bool vi = i < ni;
+                            const double xi = vi ? lds.ax[i] : 0.0, yi = vi ? lds.ay[i] : 0.0;
+                            const uint32_t pi = vi ? lds.ap[i] : 0;
+                            const uint32_t jend = min(64u, nj - q * 64);
+                            for (uint32_t jb = 0; jb < jend; jb += 8) {
+#pragma unroll
+                                for (uint32_t jj = 0; jj < 8; jj++) {
+                                    const uint32_t j = q * 64 + jb + jj;
+                                    if (jb + jj < jend) {
+                                        const double dx = xi - bx[j];
+                                        const double dy = yi - by[j];
+                                        const double d2 = dx * dx + dy * dy;
+                                        const bool hit = vi && (!diag || j > i) && d2 <= t64;
+                                        n_in += hit;
+                                        stage_push(st, hit, pair_key(pi, bp[j]));
+                                    }
+                                }
+                                if (st.n >= kStage - 512) stage_flush(st, s, tab_old, tab_cur, tick);
+                            }
+                        } else {
+                            const uint32_t uc = t - ntask_in;
+                            const uint32_t u = uc * 64 + lane;
+                            const bool vu = u < ucnt;
+                            const uint32_t p = vu ? s.unit_pid[uoff + u] : 0;
+                            const double xp = vu ? xy[2 * (size_t)p] : 0.0, yp = vu ? xy[2 * (size_t)p + 1] : 0.0;
+                            // against tile I of L (every tile of L is tile I once)
+                            for (uint32_t jb = 0; jb < ni; jb += 8) {
+#pragma unroll
+                                for (uint32_t jj = 0; jj < 8; jj++) {
+                                    const uint32_t j = jb + jj;
+                                    if (j < ni) {
+                                        const double dx = xp - lds.ax[j];
+                                        const double dy = yp - lds.ay[j];
+                                        const double d2 = dx * dx + dy * dy;
+                                        const bool hit = vu && d2 <= t64;
+                                        n_bd += hit;
+                                        stage_push(st, hit, pair_key(p, lds.ap[j]));
+                                    }
+                                }
+                                if (st.n >= kStage - 512) stage_flush(st, s, tab_old, tab_cur, tick);
                             }
                         }
                     }
@@ -651,35 +833,19 @@ __global__ __launch_bounds__(kBlock) void k_join_inside(DevState s)
             }
         }
     }
-    if (ncontacts) atomicAdd(&s.ctr->n_contacts, ncontacts);
-}
-
-// Border check units (pid, leaf, inside=false): pid against every object of
-// the candidate leaf (refinement.cpp:30 with !inside).  One wave per unit.
-__global__ __launch_bounds__(kBlock) void k_join_border(DevState s)
-{
-    Slot *tab_old = s.tp->tab_old, *tab_cur = s.tp->tab_cur;
-    const uint32_t tick = s.tp->tick;
-    const double *xy = s.tp->xy;
-    unsigned long long nu = s.ctr->n_units;
-    if (nu > s.unit_cap) nu = s.unit_cap;
-    const uint32_t lane = lane_id();
-    const unsigned long long wave = (blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x) / 64;
-    const unsigned long long nwaves = (gridDim.x * (unsigned long long)blockDim.x) / 64;
-    unsigned long long ncontacts = 0;
-    for (unsigned long long u = wave; u < nu; u += nwaves) {
-        uint2 un = s.units[u];
-        uint32_t pid = un.x, leaf = un.y;
-        double x = xy[2 * (size_t)pid], y = xy[2 * (size_t)pid + 1];
-        uint32_t off = s.offset[leaf], cnt = s.count[leaf];
-        for (uint32_t k = lane; k < cnt; k += 64) {
-            if (point_within64(x, y, s.sx[off + k], s.sy[off + k], s.t64)) {
-                ncontacts++;
-                contact(s, tab_old, tab_cur, tick, pid, s.spid[off + k]);
-            }
-        }
+    stage_flush(st, s, tab_old, tab_cur, tick);
+    // one atomic per wave for each counter
+    uint32_t ins = st.inserted, a = n_in, b = n_bd;
+    for (int d = 32; d >= 1; d >>= 1) {
+        ins += __shfl_xor(ins, d);
+        a += __shfl_xor(a, d);
+        b += __shfl_xor(b, d);
     }
-    if (ncontacts) atomicAdd(&s.ctr->n_contacts, ncontacts);
+    if (lane == 0) {
+        if (ins) atomicAdd(&s.ctr->n_inserted, (unsigned long long)ins);
+        if (a + b) atomicAdd(&s.ctr->n_contacts, (unsigned long long)(a + b));
+        if (b) atomicAdd(&s.ctr->n_contacts_border, (unsigned long long)b);
+    }
 }
 
 // ------------------------------------------------------------ evaluate
@@ -757,6 +923,60 @@ double sqrt_threshold(double r)
     return s;
 }
 
+enum KernelId {
+    K_FILL_COUNTS, K_RESET, K_ASSIGN, K_SCAN_REDUCE, K_SCAN_TOP, K_SCAN_OBJECTS, K_SCATTER, K_FILL_UCOUNT,
+    K_LOOKUP, K_SCAN_UNITS, K_UNIT_SCATTER, K_JOIN, K_EVALUATE, K_COLLECT, K_SPLIT, K_MERGE, K_SCHEMA_DONE,
+    K_TICK_PARAMS, K_COUNT
+};
+const char *const kKernelNames[K_COUNT] = {
+    "k_fill_u32", "k_reset_tick", "k_assign", "k_scan_reduce", "k_scan_top", "k_scan_objects", "k_scatter",
+    "k_fill_ucount", "k_lookup", "k_scan_units", "k_unit_scatter", "k_join", "k_evaluate", "k_collect",
+    "k_split", "k_merge", "k_schema_done", "k_advance_tick"};
+
+// Per-kernel HIP-event timing on the context's stream (optional).
+struct KernelProfiler {
+    bool on = false;
+    std::vector<hipEvent_t> pool;
+    struct Pending {
+        int id;
+        hipEvent_t a, b;
+    };
+    std::vector<Pending> pending;
+    double total_ms[K_COUNT] = {};
+    unsigned long long launches[K_COUNT] = {};
+    hipEvent_t get()
+    {
+        if (!pool.empty()) {
+            hipEvent_t e = pool.back();
+            pool.pop_back();
+            return e;
+        }
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        return e;
+    }
+    void resolve()
+    {
+        for (const Pending &p : pending) {
+            float ms = 0;
+            if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+                total_ms[p.id] += ms;
+                launches[p.id]++;
+            }
+            pool.push_back(p.a);
+            pool.push_back(p.b);
+        }
+        pending.clear();
+        (void)hipGetLastError();
+    }
+    void clear()
+    {
+        resolve();
+        for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+        pool.clear();
+    }
+};
+
 }  // namespace
 
 struct glint_ctx {
@@ -775,6 +995,7 @@ struct glint_ctx {
     uint32_t tick = 0;
     uint32_t prev_tick = 0;
     uint32_t scan_blocks = 0;
+    uint32_t join_blocks = 1024;
     double *own_xy = nullptr;
     Counters *h_ctr = nullptr;  // pinned
     std::vector<void *> allocs;
@@ -782,6 +1003,7 @@ struct glint_ctx {
     uint32_t inject = 0;
     hipEvent_t ev[6] = {};
     glint_tick_stats last{};
+    KernelProfiler prof;
 };
 
 namespace {
@@ -860,6 +1082,25 @@ void set_tick_params(glint_ctx *c, const double *xy)
     k_set_tick<<<1, 1, 0, c->stream>>>(c->s.tp, xy, old, cur, c->tick, c->has_prev ? c->prev_tick : c->tick);
 }
 
+// Launch through the profiler: HIP events around the kernel when enabled.
+template <typename F>
+inline void launch(glint_ctx *c, int id, F &&f)
+{
+    if (!c->prof.on) {
+        f();
+        return;
+    }
+    hipEvent_t a = c->prof.get(), b = c->prof.get();
+    if (!a || !b) {
+        f();
+        return;
+    }
+    (void)hipEventRecord(a, c->stream);
+    f();
+    (void)hipEventRecord(b, c->stream);
+    c->prof.pending.push_back({id, a, b});
+}
+
 glint_status build_tree(glint_ctx *c)
 {
     DevState &s = c->s;
@@ -884,42 +1125,52 @@ glint_status build_tree(glint_ctx *c)
 void enqueue_partition(glint_ctx *c)
 {
     DevState &s = c->s;
-    const uint32_t gn = grid_for(s.n), gnode = grid_for(s.node_cap);
-    k_fill_u32<<<gnode, kBlock, 0, c->stream>>>(s.count, 0u, s.node_cap);
-    k_reset_tick<<<1, 1, 0, c->stream>>>(s);
-    k_assign<<<gn, kBlock, 0, c->stream>>>(s);
-    k_scan_reduce<<<c->scan_blocks, kBlock, 0, c->stream>>>(s);
-    k_scan_top<<<1, kBlock, 0, c->stream>>>(s, c->scan_blocks);
-    k_scan_down<<<c->scan_blocks, kBlock, 0, c->stream>>>(s);
-    k_scatter<<<gn, kBlock, 0, c->stream>>>(s);
+    hipStream_t st = c->stream;
+    const uint32_t gn = grid_for(s.n), gnode = grid_for(s.node_cap), nb = c->scan_blocks;
+    launch(c, K_FILL_COUNTS, [&] { k_fill_u32<<<gnode, kBlock, 0, st>>>(s.count, 0u, s.node_cap); });
+    launch(c, K_RESET, [&] { k_reset_tick<<<1, 1, 0, st>>>(s); });
+    launch(c, K_ASSIGN, [&] { k_assign<<<gn, kBlock, 0, st>>>(s); });
+    launch(c, K_SCAN_REDUCE, [&] { k_scan_reduce<<<nb, kBlock, 0, st>>>(s.count, s.node_cap, s.block_sums); });
+    launch(c, K_SCAN_TOP, [&] { k_scan_top<<<1, kBlock, 0, st>>>(s.block_sums, nb); });
+    launch(c, K_SCAN_OBJECTS, [&] { k_scan_objects<<<nb, kBlock, 0, st>>>(s); });
+    launch(c, K_SCATTER, [&] { k_scatter<<<gn, kBlock, 0, st>>>(s); });
 }
 
 void enqueue_lookup(glint_ctx *c)
 {
-    k_lookup<<<grid_for(c->s.n), kBlock, 0, c->stream>>>(c->s);
+    DevState &s = c->s;
+    hipStream_t st = c->stream;
+    const uint32_t gnode = grid_for(s.node_cap), nb = c->scan_blocks;
+    launch(c, K_FILL_UCOUNT, [&] { k_fill_u32<<<gnode, kBlock, 0, st>>>(s.ucount, 0u, s.node_cap); });
+    launch(c, K_LOOKUP, [&] { k_lookup<<<grid_for(s.n), kBlock, 0, st>>>(s); });
+    launch(c, K_SCAN_REDUCE, [&] { k_scan_reduce<<<nb, kBlock, 0, st>>>(s.ucount, s.node_cap, s.block_sums); });
+    launch(c, K_SCAN_TOP, [&] { k_scan_top<<<1, kBlock, 0, st>>>(s.block_sums, nb); });
+    launch(c, K_SCAN_UNITS, [&] { k_scan_units<<<nb, kBlock, 0, st>>>(s); });
+    launch(c, K_UNIT_SCATTER, [&] { k_unit_scatter<<<grid_for(s.unit_cap), kBlock, 0, st>>>(s); });
 }
 
 void enqueue_refine(glint_ctx *c)
 {
-    k_join_inside<<<2048, kBlock, 0, c->stream>>>(c->s);
-    k_join_border<<<4096, kBlock, 0, c->stream>>>(c->s);
+    launch(c, K_JOIN, [&] { k_join<<<c->join_blocks, kBlock, 0, c->stream>>>(c->s); });
 }
 
 void enqueue_evaluate(glint_ctx *c)
 {
-    k_evaluate<<<grid_for(c->s.tab_mask + 1, 16384), kBlock, 0, c->stream>>>(c->s);
+    launch(c, K_EVALUATE,
+           [&] { k_evaluate<<<grid_for(c->s.tab_mask + 1, 16384), kBlock, 0, c->stream>>>(c->s); });
 }
 
 void enqueue_schema(glint_ctx *c)
 {
     DevState &s = c->s;
+    hipStream_t st = c->stream;
     if (c->cfg.dynamic_schema) {
         const uint32_t gnode = grid_for(s.node_cap);
-        k_collect<<<gnode, kBlock, 0, c->stream>>>(s);
-        k_split<<<gnode, kBlock, 0, c->stream>>>(s, 0);
-        k_merge<<<gnode, kBlock, 0, c->stream>>>(s);
+        launch(c, K_COLLECT, [&] { k_collect<<<gnode, kBlock, 0, st>>>(s); });
+        launch(c, K_SPLIT, [&] { k_split<<<gnode, kBlock, 0, st>>>(s, 0); });
+        launch(c, K_MERGE, [&] { k_merge<<<gnode, kBlock, 0, st>>>(s); });
     }
-    k_schema_done<<<1, 1, 0, c->stream>>>(s);
+    launch(c, K_SCHEMA_DONE, [&] { k_schema_done<<<1, 1, 0, st>>>(s); });
 }
 
 void fill_stats(glint_ctx *c)
@@ -1086,6 +1337,10 @@ glint_status glint_create(const glint_config *cfg, glint_ctx **out)
     ALLOC(s.rank_of, N);
     ALLOC(s.border, N);
     ALLOC(s.units, s.unit_cap);
+    ALLOC(s.urank, s.unit_cap);
+    ALLOC(s.unit_pid, s.unit_cap);
+    ALLOC(s.ucount, node_cap);
+    ALLOC(s.ustart, node_cap);
     ALLOC(s.sx, N);
     ALLOC(s.sy, N);
     ALLOC(s.spid, N);
@@ -1124,6 +1379,7 @@ glint_status glint_destroy(glint_ctx *c)
     for (void *p : c->allocs) (void)hipFree(p);
     for (auto &ev : c->ev)
         if (ev) (void)hipEventDestroy(ev);
+    c->prof.clear();
     if (c->h_ctr) (void)hipHostFree(c->h_ctr);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -1310,8 +1566,10 @@ glint_status glint_run_ticks(glint_ctx *c, const double *d_trace, uint32_t first
     if (tick_ms) HIP_TRY(c, hipEventRecord(evs[0], c->stream));
     for (uint32_t k = 0; k < nticks; k++) {
         if (k > 0) {
-            k_swap_tables<<<1, 1, 0, c->stream>>>(c->s.tp);
-            k_advance_tick<<<1, 1, 0, c->stream>>>(c->s.tp, stride);
+            launch(c, K_TICK_PARAMS, [&] {
+                k_swap_tables<<<1, 1, 0, c->stream>>>(c->s.tp);
+                k_advance_tick<<<1, 1, 0, c->stream>>>(c->s.tp, stride);
+            });
         }
         enqueue_partition(c);
         enqueue_lookup(c);
@@ -1460,6 +1718,61 @@ glint_status glint_device_meetings(glint_ctx *c, const glint_meeting **d_out, ui
     *d_out = c->s.meet;
     *n_out = std::min<uint64_t>(c->h_ctr->n_meet, c->s.meet_cap);
     return status_from_flags(c, c->h_ctr->error);
+}
+
+glint_status glint_set_kernel_profiling(glint_ctx *c, int enable)
+{
+    if (!c) return GLINT_ERR_INVALID_ARGUMENT;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->prof.resolve();
+    c->prof.on = enable != 0;
+    return GLINT_OK;
+}
+
+glint_status glint_kernel_times(glint_ctx *c, const char **names, double *total_ms, uint64_t *launches,
+                                uint32_t cap, uint32_t *n_out, int reset)
+{
+    if (!c || !n_out) return GLINT_ERR_INVALID_ARGUMENT;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->prof.resolve();
+    *n_out = K_COUNT;
+    for (uint32_t k = 0; k < K_COUNT && k < cap; k++) {
+        if (names) names[k] = kKernelNames[k];
+        if (total_ms) total_ms[k] = c->prof.total_ms[k];
+        if (launches) launches[k] = c->prof.launches[k];
+    }
+    if (reset) {
+        for (int k = 0; k < K_COUNT; k++) {
+            c->prof.total_ms[k] = 0;
+            c->prof.launches[k] = 0;
+        }
+    }
+    return GLINT_OK;
+}
+
+glint_status glint_totals_get(glint_ctx *c, glint_totals *out, int reset)
+{
+    if (!c || !out) return GLINT_ERR_INVALID_ARGUMENT;
+    glint_status st = read_counters(c);
+    if (st) return st;
+    const Counters &h = *c->h_ctr;
+    out->ticks = h.acc_ticks;
+    out->contacts = h.acc_contacts;
+    out->contacts_border = h.acc_contacts_border;
+    out->distinct_pairs = h.acc_distinct;
+    out->meetings = h.acc_meet;
+    out->units = h.acc_units;
+    out->border_objects = h.acc_border;
+    out->rechecks = h.acc_rechecks;
+    out->max_contacts_tick = h.max_contacts;
+    out->max_distinct_tick = h.max_distinct;
+    out->max_meetings_tick = h.max_meet;
+    if (reset) {
+        size_t off = offsetof(Counters, acc_ticks);
+        HIP_TRY(c, hipMemsetAsync((char *)c->s.ctr + off, 0, sizeof(Counters) - off, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+    }
+    return status_from_flags(c, h.error);
 }
 
 glint_status glint_debug_inject_failure(glint_ctx *c, const char *stage)

@@ -59,6 +59,7 @@ static_assert(sizeof(Slot) == 16, "16-byte meeting slot");
 struct Counters {
     unsigned long long n_units;
     unsigned long long n_contacts;
+    unsigned long long n_contacts_border;
     unsigned long long n_inserted;
     unsigned long long n_meet;
     unsigned long long n_rechecks;
@@ -76,6 +77,18 @@ struct Counters {
     uint32_t n_nodes;
     uint32_t last_splits;
     uint32_t last_merges;
+    // running totals over ticks (glint_totals)
+    unsigned long long acc_ticks;
+    unsigned long long acc_contacts;
+    unsigned long long acc_contacts_border;
+    unsigned long long acc_distinct;
+    unsigned long long acc_meet;
+    unsigned long long acc_units;
+    unsigned long long acc_border;
+    unsigned long long acc_rechecks;
+    unsigned long long max_contacts;
+    unsigned long long max_distinct;
+    unsigned long long max_meet;
 };
 
 // Per-tick values, kept in device memory so a tick's launches can be
@@ -122,7 +135,11 @@ struct DevState {
     uint32_t *leaf_of;
     uint32_t *rank_of;
     uint2 *border;
-    uint2 *units;
+    uint2 *units;           // (object, candidate leaf) in lookup order
+    uint32_t *urank;        // rank of the unit among the units of its leaf
+    uint32_t *unit_pid;     // units grouped by candidate leaf
+    uint32_t *ucount;       // units per leaf
+    uint32_t *ustart;       // first unit of each leaf in unit_pid
     double *sx;
     double *sy;
     uint32_t *spid;

@@ -86,6 +86,12 @@ class TickStats(ctypes.Structure):
     ]
 
 
+class Totals(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in (
+        "ticks", "contacts", "contacts_border", "distinct_pairs", "meetings", "units", "border_objects",
+        "rechecks", "max_contacts_tick", "max_distinct_tick", "max_meetings_tick")]
+
+
 # (name, restype, argtypes) of every entry point of include/glint.h
 _VP = ctypes.c_void_p
 _U32, _U64, _I = ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int
@@ -110,6 +116,10 @@ SIGNATURES = {
     "glint_active_pairs": (_I, [_VP, _VP, _VP, _U64, ctypes.POINTER(_U64)]),
     "glint_device_meetings": (_I, [_VP, ctypes.POINTER(_VP), ctypes.POINTER(_U64)]),
     "glint_debug_inject_failure": (_I, [_VP, ctypes.c_char_p]),
+    "glint_totals_get": (_I, [_VP, ctypes.POINTER(Totals), _I]),
+    "glint_set_kernel_profiling": (_I, [_VP, _I]),
+    "glint_kernel_times": (_I, [_VP, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_double),
+                                ctypes.POINTER(_U64), _U32, ctypes.POINTER(_U32), _I]),
     "glint_device_malloc": (_I, [ctypes.POINTER(_VP), _U64, _U32]),
     "glint_device_free": (_I, [_VP]),
     "glint_memcpy_to_device": (_I, [_VP, _VP, _U64]),
@@ -346,6 +356,25 @@ class Workbench:
         self._chk(lib().glint_active_pairs(self.h, keys.ctypes.data, starts.ctypes.data, keys.size,
                                            ctypes.byref(n)), "glint_active_pairs")
         return keys[: n.value], starts[: n.value]
+
+    def totals(self, reset: bool = False) -> dict:
+        t = Totals()
+        self._chk(lib().glint_totals_get(self.h, ctypes.byref(t), int(reset)), "glint_totals_get")
+        return {f: getattr(t, f) for f, _ in Totals._fields_}
+
+    def set_kernel_profiling(self, on: bool) -> None:
+        self._chk(lib().glint_set_kernel_profiling(self.h, int(on)), "glint_set_kernel_profiling")
+
+    def kernel_times(self, reset: bool = False) -> dict:
+        """{kernel name: (total ms, launches)} since the last reset."""
+        cap = 64
+        names = (ctypes.c_char_p * cap)()
+        ms = (ctypes.c_double * cap)()
+        cnt = (ctypes.c_uint64 * cap)()
+        n = ctypes.c_uint32()
+        self._chk(lib().glint_kernel_times(self.h, names, ms, cnt, cap, ctypes.byref(n), int(reset)),
+                  "glint_kernel_times")
+        return {names[k].decode(): (ms[k], cnt[k]) for k in range(min(n.value, cap))}
 
     def inject_failure(self, stage: str) -> None:
         self._chk(lib().glint_debug_inject_failure(self.h, stage.encode()), "glint_debug_inject_failure")

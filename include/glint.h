@@ -187,6 +187,30 @@ glint_status glint_leaves(glint_ctx *ctx, double *boxes, uint32_t *levels, uint3
 glint_status glint_active_pairs(glint_ctx *ctx, uint64_t *keys, uint32_t *starts, uint64_t cap,
                                 uint64_t *n_out);
 
+/* Running totals since creation or the last reset (consumed by benchmarks
+ * to price each kernel's algorithmic bytes). */
+typedef struct glint_totals {
+    uint64_t ticks;
+    uint64_t contacts;            /* contact pairs found, duplicates included */
+    uint64_t contacts_border;     /* of which from border check units */
+    uint64_t distinct_pairs;      /* distinct active pairs, summed over ticks */
+    uint64_t meetings;
+    uint64_t units;               /* border check units */
+    uint64_t border_objects;
+    uint64_t rechecks;            /* FP64 rechecks in FP32 guard mode */
+    uint64_t max_contacts_tick;
+    uint64_t max_distinct_tick;
+    uint64_t max_meetings_tick;
+} glint_totals;
+glint_status glint_totals_get(glint_ctx *ctx, glint_totals *out, int reset);
+
+/* Per-kernel HIP-event timing on the context's stream.  glint_kernel_times
+ * reports, for each kernel of the path, the summed milliseconds and launch
+ * count since the last reset (names are static strings). */
+glint_status glint_set_kernel_profiling(glint_ctx *ctx, int enable);
+glint_status glint_kernel_times(glint_ctx *ctx, const char **names, double *total_ms, uint64_t *launches,
+                                uint32_t cap, uint32_t *n_out, int reset);
+
 /* Device pointer to the last tick's meetings (unsorted) and their count;
  * valid until the next glint_evaluate. */
 glint_status glint_device_meetings(glint_ctx *ctx, const glint_meeting **d_out, uint64_t *n_out);
