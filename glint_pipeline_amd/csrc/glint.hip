@@ -37,6 +37,8 @@ constexpr int kScanItems = 4;                       // per thread
 constexpr int kScanTile = kBlock * kScanItems;      // elements per scan block
 constexpr int kTile = 1024;                         // points per LDS tile of the join
 constexpr int kStage = 1024;                        // staged contacts per wave
+constexpr int kMeetStage = 256;                     // staged meetings per wave (evaluate)
+constexpr int kBorderStage = 256;                   // staged border objects per wave (assign)
 constexpr int kLookupStack = 128;
 constexpr int kMaxCand = 32;                        // candidate leaves per border object
 constexpr uint32_t kInsideChunk = 4;                // leaves per work-queue grab
@@ -62,6 +64,14 @@ __device__ __forceinline__ bool point_within64(double x1, double y1, double x2, 
     double dy = y1 - y2;
     double s = dx * dx + dy * dy;
     return s <= t64;
+}
+
+// Orders LDS accesses of the lanes of one wave (staging buffers).
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // ------------------------------------------------------------ small kernels
@@ -371,6 +381,10 @@ __global__ void k_schema_done(DevState s)
 // node on the path, below the root, with x + r < high.x && y + r < high.y.
 __global__ __launch_bounds__(kBlock) void k_assign(DevState s)
 {
+    __shared__ uint2 stage[kBlock / 64][kBorderStage];
+    uint2 *st = stage[threadIdx.x / 64];
+    uint32_t nst = 0;
+    const uint32_t lane = lane_id();
     const double *xy = s.tp->xy;
     const double r = s.r;
     const bool root_leaf = s.nodes[0].child[0] == kLeafTag;
@@ -400,8 +414,26 @@ __global__ __launch_bounds__(kBlock) void k_assign(DevState s)
                 if (s.complete) lv = 0;
             }
         }
-        uint32_t idx = wave_append_u32(&s.ctr->n_border, border);
-        if (border) s.border[idx] = make_uint2(i, lv);
+        // border objects are staged per wave and appended one flush at a time
+        unsigned long long m = __ballot(border);
+        if (border) st[nst + __popcll(m & ((1ull << lane) - 1ull))] = make_uint2(i, lv);
+        nst += (uint32_t)__popcll(m);
+        if (nst > kBorderStage - 64) {
+            wave_sync();
+            uint32_t b0 = 0;
+            if (lane == 0) b0 = atomicAdd(&s.ctr->n_border, nst);
+            b0 = __shfl(b0, 0);
+            for (uint32_t k = lane; k < nst; k += 64) s.border[b0 + k] = st[k];
+            wave_sync();
+            nst = 0;
+        }
+    }
+    if (nst) {
+        wave_sync();
+        uint32_t b0 = 0;
+        if (lane == 0) b0 = atomicAdd(&s.ctr->n_border, nst);
+        b0 = __shfl(b0, 0);
+        for (uint32_t k = lane; k < nst; k += 64) s.border[b0 + k] = st[k];
     }
 }
 
@@ -617,51 +649,12 @@ __global__ __launch_bounds__(kBlock) void k_unit_scatter(DevState s)
 
 // ------------------------------------------------------------ meeting table
 
-// Upsert of one contact: look the pair up in the previous generation (its
-// start tick if it was seen at prev_tick, flagged as continued), then insert
-// it into the current generation.  Linear probing, 64-bit CAS on the key.
-// Returns 1 when this call inserted the pair.
-__device__ __forceinline__ uint32_t upsert(const DevState &s, Slot *tab_old, Slot *tab_cur, uint32_t tick,
-                                           unsigned long long key)
-{
-    const unsigned long long mask = s.tab_mask;
-    const unsigned long long h0 = mix64(key) & mask;
-    uint32_t start = tick;
-    unsigned long long h = h0;
-    for (unsigned long long probes = 0; probes <= mask; probes++) {
-        unsigned long long k = tab_old[h].key;
-        if (k == key) {
-            start = tab_old[h].start;
-            if (!tab_old[h].seen) tab_old[h].seen = 1;
-            break;
-        }
-        if (k == kEmptyKey) break;
-        h = (h + 1) & mask;
-    }
-    h = h0;
-    for (unsigned long long probes = 0;; probes++) {
-        if (probes > mask) {
-            atomicOr(&s.ctr->error, kErrTableFull);
-            return 0;
-        }
-        unsigned long long prev = atomicCAS(&tab_cur[h].key, kEmptyKey, key);
-        if (prev == kEmptyKey) {
-            tab_cur[h].start = start;
-            return 1;
-        }
-        if (prev == key) return 0;   // found twice this tick: same start
-        h = (h + 1) & mask;
-    }
-}
+// Upsert of a contact (upsert4 below): look the pair up in the previous
+// generation (its start tick if it was seen at prev_tick, flagged as
+// continued), then insert it into the current generation.  Linear probing,
+// 64-bit CAS on the key; a pair found twice in one tick keeps one entry.
 
 // ------------------------------------------------------------ refine: join
-
-__device__ __forceinline__ void wave_sync()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 __device__ __forceinline__ unsigned long long pair_key(uint32_t a, uint32_t b)
 {
@@ -684,21 +677,97 @@ __device__ __forceinline__ void stage_push(Stager &st, bool hit, unsigned long l
     st.n += (uint32_t)__popcll(m);
 }
 
+// Four independent upserts per lane: the first probes of all four are issued
+// before any result is needed, which keeps more random accesses in flight.
+__device__ __forceinline__ uint32_t upsert4(const DevState &s, Slot *tab_old, Slot *tab_cur, uint32_t tick,
+                                            const unsigned long long key[4], const bool valid[4])
+{
+    const unsigned long long mask = s.tab_mask;
+    unsigned long long h0[4];
+    Slot o[4];
+    uint32_t start[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) h0[q] = mix64(key[q]) & mask;
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+        if (valid[q]) o[q] = tab_old[h0[q]];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        if (!valid[q]) continue;
+        start[q] = tick;
+        if (o[q].key == key[q]) {
+            start[q] = o[q].start;
+            if (!o[q].seen) tab_old[h0[q]].seen = 1;
+        } else if (o[q].key != kEmptyKey) {
+            unsigned long long h = (h0[q] + 1) & mask;
+            for (unsigned long long probes = 1; probes <= mask; probes++) {
+                unsigned long long k = tab_old[h].key;
+                if (k == key[q]) {
+                    start[q] = tab_old[h].start;
+                    if (!tab_old[h].seen) tab_old[h].seen = 1;
+                    break;
+                }
+                if (k == kEmptyKey) break;
+                h = (h + 1) & mask;
+            }
+        }
+    }
+    unsigned long long prev[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+        if (valid[q]) prev[q] = atomicCAS(&tab_cur[h0[q]].key, kEmptyKey, key[q]);
+    uint32_t inserted = 0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        if (!valid[q]) continue;
+        if (prev[q] == kEmptyKey) {
+            tab_cur[h0[q]].start = start[q];
+            inserted++;
+        } else if (prev[q] != key[q]) {
+            unsigned long long h = (h0[q] + 1) & mask;
+            for (unsigned long long probes = 1;; probes++) {
+                if (probes > mask) {
+                    atomicOr(&s.ctr->error, kErrTableFull);
+                    break;
+                }
+                unsigned long long pv = atomicCAS(&tab_cur[h].key, kEmptyKey, key[q]);
+                if (pv == kEmptyKey) {
+                    tab_cur[h].start = start[q];
+                    inserted++;
+                    break;
+                }
+                if (pv == key[q]) break;
+                h = (h + 1) & mask;
+            }
+        }
+    }
+    return inserted;
+}
+
 __device__ __forceinline__ void stage_flush(Stager &st, const DevState &s, Slot *tab_old, Slot *tab_cur,
                                             uint32_t tick)
 {
     if (st.n == 0) return;
     wave_sync();
-    for (uint32_t k = lane_id(); k < st.n; k += 64) {
-        unsigned long long key = st.buf[k];
-        if (s.record) {
+    const uint32_t lane = lane_id();
+    if (s.record) {
+        for (uint32_t k = lane; k < st.n; k += 64) {
             unsigned long long idx = atomicAdd(&s.ctr->n_rec, 1ull);
             if (idx < s.rec_cap)
-                s.rec[idx] = key;
+                s.rec[idx] = st.buf[k];
             else
                 atomicOr(&s.ctr->error, kErrContactCap);
         }
-        st.inserted += upsert(s, tab_old, tab_cur, tick, key);
+    }
+    for (uint32_t k = lane; k < st.n; k += 256) {
+        unsigned long long key[4];
+        bool valid[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            valid[q] = k + 64 * q < st.n;
+            key[q] = valid[q] ? st.buf[k + 64 * q] : 0ull;
+        }
+        st.inserted += upsert4(s, tab_old, tab_cur, tick, key, valid);
     }
     wave_sync();
     st.n = 0;
@@ -855,9 +924,31 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
 // cur - start >= min + 1; every slot is cleared for reuse.
 __global__ __launch_bounds__(kBlock) void k_evaluate(DevState s)
 {
+    // Emitted meetings are staged per wave in LDS and appended with one
+    // atomic per flush: a single global counter hit once per wave-iteration
+    // would serialise the scan.
+    __shared__ glint_meeting stage[kBlock / 64][kMeetStage];
+    glint_meeting *st = stage[threadIdx.x / 64];
+    uint32_t n = 0;
+    const uint32_t lane = lane_id();
     Slot *tab = s.tp->tab_old;
     const uint32_t tick = s.tp->tick, last = s.tp->prev_tick;
     const unsigned long long nslots = s.tab_mask + 1;
+    auto flush = [&]() {
+        if (n == 0) return;
+        wave_sync();
+        unsigned long long base = 0;
+        if (lane == 0) base = atomicAdd(&s.ctr->n_meet, (unsigned long long)n);
+        base = __shfl(base, 0);
+        for (uint32_t k = lane; k < n; k += 64) {
+            if (base + k < s.meet_cap)
+                s.meet[base + k] = st[k];
+            else
+                atomicOr(&s.ctr->error, kErrMeetCap);
+        }
+        wave_sync();
+        n = 0;
+    };
     for (unsigned long long base = blockIdx.x * (unsigned long long)blockDim.x; base < nslots;
          base += (unsigned long long)gridDim.x * blockDim.x) {
         unsigned long long i = base + threadIdx.x;
@@ -875,20 +966,19 @@ __global__ __launch_bounds__(kBlock) void k_evaluate(DevState s)
                 tab[i] = e;
             }
         }
-        unsigned long long idx = wave_append_u64(&s.ctr->n_meet, emit);
+        unsigned long long m = __ballot(emit);
         if (emit) {
-            if (idx < s.meet_cap) {
-                glint_meeting m;
-                m.pid_a = (uint32_t)(sl.key >> 32);
-                m.pid_b = (uint32_t)sl.key;
-                m.start = sl.start;
-                m.end = last;
-                s.meet[idx] = m;
-            } else {
-                atomicOr(&s.ctr->error, kErrMeetCap);
-            }
+            glint_meeting mt;
+            mt.pid_a = (uint32_t)(sl.key >> 32);
+            mt.pid_b = (uint32_t)sl.key;
+            mt.start = sl.start;
+            mt.end = last;
+            st[n + __popcll(m & ((1ull << lane) - 1ull))] = mt;
         }
+        n += (uint32_t)__popcll(m);
+        if (n > kMeetStage - 64) flush();
     }
+    flush();
 }
 
 // ------------------------------------------------------------ host side
