@@ -61,10 +61,9 @@ def algorithmic_bytes(kernel: str, n: int, per_tick: dict, slots: int, precision
     U border units, B table slots."""
     C, Cb, M, U = per_tick["contacts"], per_tick["contacts_border"], per_tick["meetings"], per_tick["units"]
     coord = 8 if precision else 16          # FP32 offsets or FP64 coordinates per object
-    if kernel == "k_join_inside":
-        return coord * n + 16.0 * (C - Cb)    # every sorted coordinate once, one 16 B upsert per contact
-    if kernel == "k_join_border":
-        return 8.0 * U + 16.0 * U + 16.0 * Cb  # unit, the probing object's xy, upserts
+    if kernel == "k_join":
+        # every sorted coordinate once, each border unit (id + xy), one 16 B upsert per contact
+        return coord * n + 24.0 * U + 16.0 * C
     if kernel == "k_evaluate":
         return 16.0 * slots + 16.0 * M        # table scan + meeting output
     if kernel == "k_assign":

@@ -35,8 +35,9 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kScanItems = 4;                       // per thread
 constexpr int kScanTile = kBlock * kScanItems;      // elements per scan block
-constexpr int kTile = 1024;                         // points per LDS tile of the join
-constexpr int kStage = 1024;                        // staged contacts per wave
+constexpr int kTile = 512;                          // points per LDS tile of the join
+constexpr int kStage = 512;                         // staged contacts per wave
+constexpr int kFlushRound = 4;                      // join columns between stage checks
 constexpr int kMeetStage = 256;                     // staged meetings per wave (evaluate)
 constexpr int kBorderStage = 256;                   // staged border objects per wave (assign)
 constexpr int kLookupStack = 128;
@@ -95,18 +96,21 @@ __global__ void k_fill_slots(Slot *p, uint64_t n)
     }
 }
 
-__global__ void k_set_tick(TickParams *tp, const double *xy, Slot *tab_old, Slot *tab_cur, uint32_t tick,
-                           uint32_t prev_tick)
+__global__ void k_set_tick(TickParams *tp, const double *xy, Slot *tab_old, Slot *tab_cur, double *xy_prev,
+                           double *xy_save, uint32_t tick, uint32_t prev_tick, uint32_t has_prev)
 {
     tp->xy = xy;
     tp->tab_old = tab_old;
     tp->tab_cur = tab_cur;
+    tp->xy_prev = xy_prev;
+    tp->xy_save = xy_save;
     tp->tick = tick;
     tp->prev_tick = prev_tick;
+    tp->has_prev = has_prev;
 }
 
 // Next tick of a device-resident trace: positions move one snapshot on and
-// the table generations swap (tab_cur of the last tick becomes tab_old).
+// the generations swap (this tick's table and positions become the previous).
 __global__ void k_advance_tick(TickParams *tp, uint64_t stride)
 {
     uint32_t k = tp->trace_index + 1;
@@ -114,25 +118,28 @@ __global__ void k_advance_tick(TickParams *tp, uint64_t stride)
     tp->xy = tp->trace_base + stride * k;
     tp->prev_tick = tp->tick;
     tp->tick = tp->tick + 1;
+    tp->has_prev = 1;
+    Slot *t = tp->tab_old;
+    tp->tab_old = tp->tab_cur;
+    tp->tab_cur = t;
+    double *h = tp->xy_prev;
+    tp->xy_prev = tp->xy_save;
+    tp->xy_save = h;
 }
 
-__global__ void k_begin_trace(TickParams *tp, const double *base, Slot *tab_old, Slot *tab_cur, uint32_t tick,
-                              uint32_t prev_tick)
+__global__ void k_begin_trace(TickParams *tp, const double *base, Slot *tab_old, Slot *tab_cur, double *xy_prev,
+                              double *xy_save, uint32_t tick, uint32_t prev_tick, uint32_t has_prev)
 {
     tp->trace_base = base;
     tp->trace_index = 0;
     tp->xy = base;
     tp->tab_old = tab_old;
     tp->tab_cur = tab_cur;
+    tp->xy_prev = xy_prev;
+    tp->xy_save = xy_save;
     tp->tick = tick;
     tp->prev_tick = prev_tick;
-}
-
-__global__ void k_swap_tables(TickParams *tp)
-{
-    Slot *t = tp->tab_old;
-    tp->tab_old = tp->tab_cur;
-    tp->tab_cur = t;
+    tp->has_prev = has_prev;
 }
 
 __global__ void k_reset_tick(DevState s)
@@ -549,6 +556,9 @@ __global__ __launch_bounds__(kBlock) void k_scan_units(DevState s)
 __global__ __launch_bounds__(kBlock) void k_scatter(DevState s)
 {
     const double *xy = s.tp->xy;
+    const double *xy_prev = s.tp->xy_prev;
+    double *xy_save = s.tp->xy_save;
+    const bool has_prev = s.tp->has_prev != 0;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < s.n; i += gridDim.x * blockDim.x) {
         uint32_t leaf = s.leaf_of[i];
         if (leaf == kNone) continue;
@@ -557,6 +567,12 @@ __global__ __launch_bounds__(kBlock) void k_scatter(DevState s)
         s.sx[pos] = x;
         s.sy[pos] = y;
         s.spid[pos] = i;
+        if (has_prev) {
+            s.sxp[pos] = xy_prev[2 * (size_t)i];
+            s.syp[pos] = xy_prev[2 * (size_t)i + 1];
+        }
+        xy_save[2 * (size_t)i] = x;
+        xy_save[2 * (size_t)i + 1] = y;
         if (s.precision == GLINT_PRECISION_FP32_GUARD) {
             Node nd = s.nodes[leaf];
             s.fx[pos] = (float)(x - nd.lx);
@@ -679,22 +695,32 @@ __device__ __forceinline__ void stage_push(Stager &st, bool hit, unsigned long l
 
 // Four independent upserts per lane: the first probes of all four are issued
 // before any result is needed, which keeps more random accesses in flight.
+// A staged contact carries bit 63 when the pair was within reach at
+// prev_tick (the only case in which it can be in the previous generation);
+// for every other contact the lookup is skipped and start = tick.
 __device__ __forceinline__ uint32_t upsert4(const DevState &s, Slot *tab_old, Slot *tab_cur, uint32_t tick,
-                                            const unsigned long long key[4], const bool valid[4])
+                                            const unsigned long long tagged[4], const bool valid[4])
 {
     const unsigned long long mask = s.tab_mask;
     unsigned long long h0[4];
     Slot o[4];
     uint32_t start[4];
+    bool look[4];
+    unsigned long long key[4];
 #pragma unroll
-    for (int q = 0; q < 4; q++) h0[q] = mix64(key[q]) & mask;
+    for (int q = 0; q < 4; q++) {
+        look[q] = valid[q] && (tagged[q] >> 63);   // within reach at prev_tick
+        key[q] = tagged[q] & ~(1ull << 63);
+        h0[q] = mix64(key[q]) & mask;
+    }
 #pragma unroll
     for (int q = 0; q < 4; q++)
-        if (valid[q]) o[q] = tab_old[h0[q]];
+        if (look[q]) o[q] = tab_old[h0[q]];
 #pragma unroll
     for (int q = 0; q < 4; q++) {
         if (!valid[q]) continue;
         start[q] = tick;
+        if (!look[q]) continue;
         if (o[q].key == key[q]) {
             start[q] = o[q].start;
             if (!o[q].seen) tab_old[h0[q]].seen = 1;
@@ -754,7 +780,7 @@ __device__ __forceinline__ void stage_flush(Stager &st, const DevState &s, Slot 
         for (uint32_t k = lane; k < st.n; k += 64) {
             unsigned long long idx = atomicAdd(&s.ctr->n_rec, 1ull);
             if (idx < s.rec_cap)
-                s.rec[idx] = st.buf[k];
+                s.rec[idx] = st.buf[k] & ~(1ull << 63);
             else
                 atomicOr(&s.ctr->error, kErrContactCap);
         }
@@ -773,13 +799,65 @@ __device__ __forceinline__ void stage_flush(Stager &st, const DevState &s, Slot 
     st.n = 0;
 }
 
+struct JoinTile {
+    double x[kTile], y[kTile];     // positions at tick
+    double px[kTile], py[kTile];   // positions at prev_tick
+    float fx[kTile], fy[kTile];    // FP32 offsets from the leaf's low corner (guard mode)
+    uint32_t p[kTile];             // object ids
+};
+
 struct JoinLds {
-    double ax[kTile], ay[kTile];
-    double bx[kTile], by[kTile];
-    uint32_t ap[kTile], bp[kTile];
+    JoinTile a, b;
     unsigned long long stage[kBlock / 64][kStage];
     uint32_t item;
 };
+
+// Distance test of one candidate pair, bit-equal to sqrt(dx*dx+dy*dy) <= r
+// in FP64.  In guard mode the FP32 offsets decide every pair outside the
+// band (r-eps, r+eps]; pairs inside it are re-tested in FP64.
+struct PairTest {
+    double t64;
+    float t32_in, t32_out;
+    bool use32;
+};
+
+__device__ __forceinline__ bool pair_within(const PairTest &pt, double x1, double y1, double x2, double y2,
+                                            float f1x, float f1y, float f2x, float f2y, uint32_t &rechecks)
+{
+    if (pt.use32) {
+        const float dx = f1x - f2x;
+        const float dy = f1y - f2y;
+        const float s32 = dx * dx + dy * dy;
+        if (s32 <= pt.t32_in) return true;
+        if (s32 > pt.t32_out) return false;
+        rechecks++;
+    }
+    const double dx = x1 - x2;
+    const double dy = y1 - y2;
+    const double d2 = dx * dx + dy * dy;
+    return d2 <= pt.t64;
+}
+
+__device__ __forceinline__ void load_tile(JoinTile &t, const DevState &s, uint32_t from, uint32_t n, bool has_prev,
+                                          bool use32, double ox, double oy)
+{
+    for (uint32_t k = threadIdx.x; k < n; k += kBlock) {
+        const double x = s.sx[from + k], y = s.sy[from + k];
+        t.x[k] = x;
+        t.y[k] = y;
+        t.p[k] = s.spid[from + k];
+        if (has_prev) {
+            t.px[k] = s.sxp[from + k];
+            t.py[k] = s.syp[from + k];
+        }
+        if (use32) {
+            t.fx[k] = s.fx[from + k];
+            t.fy[k] = s.fy[from + k];
+        }
+    }
+    (void)ox;
+    (void)oy;
+}
 
 // The refine stage for one leaf L per block (work queue over the leaf list):
 //  - inside units (filter.cpp:107-109 with refinement.cpp:30, pid < q): every
@@ -787,17 +865,22 @@ struct JoinLds {
 //  - border units (pid, L, inside=false) (refinement.cpp:30 with !inside):
 //    each object whose lookup selected L against every object of L.
 // Work is split into 64x64 wave tasks (64 row lanes x 64 broadcast columns).
+// A contact is staged with bit 63 set when the pair was also within reach
+// at prev_tick, the only case that needs the previous table generation.
 __global__ __launch_bounds__(kBlock) void k_join(DevState s)
 {
     __shared__ JoinLds lds;
     Slot *tab_old = s.tp->tab_old, *tab_cur = s.tp->tab_cur;
     const uint32_t tick = s.tp->tick;
     const double *xy = s.tp->xy;
+    const double *xy_prev = s.tp->xy_prev;
+    const bool has_prev = s.tp->has_prev != 0;
     const double t64 = s.t64;
     const uint32_t nl = (s.ctr->error & kErrUnitCap) ? 0 : s.ctr->n_leaf_list;
     const uint32_t wave = threadIdx.x / 64, lane = lane_id();
     Stager st{lds.stage[wave], 0, 0};
-    uint32_t n_in = 0, n_bd = 0;   // contacts found by this lane
+    uint32_t n_in = 0, n_bd = 0, n_re = 0;   // per lane
+    PairTest pt{t64, s.t32_in, s.t32_out, false};
     for (;;) {
         if (threadIdx.x == 0) lds.item = atomicAdd(&s.ctr->work_inside, kInsideChunk);
         __syncthreads();
@@ -809,39 +892,33 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
             const uint32_t off = s.offset[leaf], cnt = s.count[leaf];
             const uint32_t uoff = s.ustart[leaf], ucnt = s.ucount[leaf];
             const uint32_t ntile = (cnt + kTile - 1) / kTile;
+            const Node nd = s.nodes[leaf];
+            pt.use32 = s.precision == GLINT_PRECISION_FP32_GUARD && (nd.hx - nd.lx) <= s.w32_max &&
+                       (nd.hy - nd.ly) <= s.w32_max;
             for (uint32_t I = 0; I < ntile; I++) {
                 const uint32_t ni = min((uint32_t)kTile, cnt - I * kTile);
-                for (uint32_t k = threadIdx.x; k < ni; k += kBlock) {
-                    lds.ax[k] = s.sx[off + I * kTile + k];
-                    lds.ay[k] = s.sy[off + I * kTile + k];
-                    lds.ap[k] = s.spid[off + I * kTile + k];
-                }
+                load_tile(lds.a, s, off + I * kTile, ni, has_prev, pt.use32, nd.lx, nd.ly);
                 for (uint32_t J = I; J < ntile; J++) {
                     const uint32_t nj = min((uint32_t)kTile, cnt - J * kTile);
                     const bool diag = (J == I);
-                    const double *bx = lds.ax, *by = lds.ay;
-                    const uint32_t *bp = lds.ap;
-                    if (!diag) {
-                        for (uint32_t k = threadIdx.x; k < nj; k += kBlock) {
-                            lds.bx[k] = s.sx[off + J * kTile + k];
-                            lds.by[k] = s.sy[off + J * kTile + k];
-                            lds.bp[k] = s.spid[off + J * kTile + k];
-                        }
-                        bx = lds.bx;
-                        by = lds.by;
-                        bp = lds.bp;
-                    }
+                    const JoinTile &B = diag ? lds.a : lds.b;
+                    if (!diag) load_tile(lds.b, s, off + J * kTile, nj, has_prev, pt.use32, nd.lx, nd.ly);
                     __syncthreads();
+                    const JoinTile &A = lds.a;
                     // inside tasks: 64-row sub-tile r against 64-column sub-tile q
                     const uint32_t nri = (ni + 63) / 64, nrj = (nj + 63) / 64;
                     const uint32_t ntask_in = diag ? nri * (nri + 1) / 2 : nri * nrj;
-                    // border tasks (only once per tile): 64 units against the tile
+                    // border tasks (once per tile of L): 64 units against the tile
                     const uint32_t nuc = diag ? (ucnt + 63) / 64 : 0;
                     const uint32_t ntask = ntask_in + nuc;
                     for (uint32_t t = wave; t < ntask; t += kBlock / 64) {
+                        double xi, yi, pxi = 0.0, pyi = 0.0;
+                        float fxi = 0.f, fyi = 0.f;
+                        uint32_t pi, jbeg, jend, row;
+                        bool vi, inside;
                         if (t < ntask_in) {
                             uint32_t r, q;
-                            if (diag) {   // t -> (r <= q) in row-major upper triangle
+                            if (diag) {   // t -> (r <= q), row-major upper triangle
                                 r = 0;
                                 uint32_t rem = t;
                                 while (rem >= nri - r) {
@@ -853,48 +930,64 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
                                 r = t / nrj;
                                 q = t % nrj;
                             }
-                            const uint32_t i = r * 64 + lane;
-                            const bool vi = i < ni;
-                            const double xi = vi ? lds.ax[i] : 0.0, yi = vi ? lds.ay[i] : 0.0;
-                            const uint32_t pi = vi ? lds.ap[i] : 0;
-                            const uint32_t jend = min(64u, nj - q * 64);
-                            for (uint32_t jb = 0; jb < jend; jb += 8) {
-#pragma unroll
-                                for (uint32_t jj = 0; jj < 8; jj++) {
-                                    const uint32_t j = q * 64 + jb + jj;
-                                    if (jb + jj < jend) {
-                                        const double dx = xi - bx[j];
-                                        const double dy = yi - by[j];
-                                        const double d2 = dx * dx + dy * dy;
-                                        const bool hit = vi && (!diag || j > i) && d2 <= t64;
-                                        n_in += hit;
-                                        stage_push(st, hit, pair_key(pi, bp[j]));
-                                    }
-                                }
-                                if (st.n >= kStage - 512) stage_flush(st, s, tab_old, tab_cur, tick);
+                            row = r * 64 + lane;
+                            vi = row < ni;
+                            xi = vi ? A.x[row] : 0.0;
+                            yi = vi ? A.y[row] : 0.0;
+                            if (has_prev && vi) {
+                                pxi = A.px[row];
+                                pyi = A.py[row];
                             }
+                            if (pt.use32 && vi) {
+                                fxi = A.fx[row];
+                                fyi = A.fy[row];
+                            }
+                            pi = vi ? A.p[row] : 0;
+                            jbeg = q * 64;
+                            jend = min(jbeg + 64, nj);
+                            inside = true;
                         } else {
-                            const uint32_t uc = t - ntask_in;
-                            const uint32_t u = uc * 64 + lane;
-                            const bool vu = u < ucnt;
-                            const uint32_t p = vu ? s.unit_pid[uoff + u] : 0;
-                            const double xp = vu ? xy[2 * (size_t)p] : 0.0, yp = vu ? xy[2 * (size_t)p + 1] : 0.0;
-                            // against tile I of L (every tile of L is tile I once)
-                            for (uint32_t jb = 0; jb < ni; jb += 8) {
-#pragma unroll
-                                for (uint32_t jj = 0; jj < 8; jj++) {
-                                    const uint32_t j = jb + jj;
-                                    if (j < ni) {
-                                        const double dx = xp - lds.ax[j];
-                                        const double dy = yp - lds.ay[j];
-                                        const double d2 = dx * dx + dy * dy;
-                                        const bool hit = vu && d2 <= t64;
-                                        n_bd += hit;
-                                        stage_push(st, hit, pair_key(p, lds.ap[j]));
-                                    }
-                                }
-                                if (st.n >= kStage - 512) stage_flush(st, s, tab_old, tab_cur, tick);
+                            const uint32_t u = (t - ntask_in) * 64 + lane;
+                            vi = u < ucnt;
+                            pi = vi ? s.unit_pid[uoff + u] : 0;
+                            xi = vi ? xy[2 * (size_t)pi] : 0.0;
+                            yi = vi ? xy[2 * (size_t)pi + 1] : 0.0;
+                            if (has_prev && vi) {
+                                pxi = xy_prev[2 * (size_t)pi];
+                                pyi = xy_prev[2 * (size_t)pi + 1];
                             }
+                            if (pt.use32) {
+                                fxi = (float)(xi - nd.lx);
+                                fyi = (float)(yi - nd.ly);
+                            }
+                            row = 0;
+                            jbeg = 0;
+                            jend = ni;
+                            inside = false;
+                        }
+                        const JoinTile &C = inside ? B : A;
+                        for (uint32_t jb = jbeg; jb < jend; jb += kFlushRound) {
+#pragma unroll
+                            for (uint32_t jj = 0; jj < kFlushRound; jj++) {
+                                const uint32_t j = jb + jj;
+                                if (j < jend) {
+                                    bool hit = vi && (!inside || !diag || j > row) &&
+                                               pair_within(pt, xi, yi, C.x[j], C.y[j], fxi, fyi, C.fx[j], C.fy[j],
+                                                           n_re);
+                                    unsigned long long key = pair_key(pi, C.p[j]);
+                                    if (hit && has_prev) {
+                                        const double dx = pxi - C.px[j];
+                                        const double dy = pyi - C.py[j];
+                                        if (dx * dx + dy * dy <= t64) key |= 1ull << 63;
+                                    }
+                                    if (inside)
+                                        n_in += hit;
+                                    else
+                                        n_bd += hit;
+                                    stage_push(st, hit, key);
+                                }
+                            }
+                            if (st.n > kStage - 64 * kFlushRound) stage_flush(st, s, tab_old, tab_cur, tick);
                         }
                     }
                     __syncthreads();
@@ -904,16 +997,18 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
     }
     stage_flush(st, s, tab_old, tab_cur, tick);
     // one atomic per wave for each counter
-    uint32_t ins = st.inserted, a = n_in, b = n_bd;
+    uint32_t ins = st.inserted, a = n_in, b = n_bd, re = n_re;
     for (int d = 32; d >= 1; d >>= 1) {
         ins += __shfl_xor(ins, d);
         a += __shfl_xor(a, d);
         b += __shfl_xor(b, d);
+        re += __shfl_xor(re, d);
     }
     if (lane == 0) {
         if (ins) atomicAdd(&s.ctr->n_inserted, (unsigned long long)ins);
         if (a + b) atomicAdd(&s.ctr->n_contacts, (unsigned long long)(a + b));
         if (b) atomicAdd(&s.ctr->n_contacts_border, (unsigned long long)b);
+        if (re) atomicAdd(&s.ctr->n_rechecks, (unsigned long long)re);
     }
 }
 
@@ -1075,6 +1170,7 @@ struct glint_ctx {
     hipStream_t stream = nullptr;
     DevState s{};
     Slot *tab[2] = {nullptr, nullptr};
+    double *hist[2] = {nullptr, nullptr};   // positions of the last two ticks (generation parity)
     int cur_gen = 0;            // table generation filled at the current tick
     bool built = false;
     bool has_positions = false;
@@ -1168,8 +1264,9 @@ glint_status read_counters(glint_ctx *c)
 
 void set_tick_params(glint_ctx *c, const double *xy)
 {
-    Slot *cur = c->tab[c->cur_gen], *old = c->tab[1 - c->cur_gen];
-    k_set_tick<<<1, 1, 0, c->stream>>>(c->s.tp, xy, old, cur, c->tick, c->has_prev ? c->prev_tick : c->tick);
+    const int g = c->cur_gen;
+    k_set_tick<<<1, 1, 0, c->stream>>>(c->s.tp, xy, c->tab[1 - g], c->tab[g], c->hist[1 - g], c->hist[g], c->tick,
+                                       c->has_prev ? c->prev_tick : c->tick, c->has_prev ? 1u : 0u);
 }
 
 // Launch through the profiler: HIP events around the kernel when enabled.
@@ -1395,6 +1492,13 @@ glint_status glint_create(const glint_config *cfg, glint_ctx **out)
         if ((double)fh < hi) fh = std::nextafter(fh, INFINITY);
         s.t32_in = fl;
         s.t32_out = fh;
+        // Largest leaf side whose FP32 offsets keep |d32 - d64| well inside
+        // the guard band: offsets carry <= u*(W + r) error each (u = 2^-24),
+        // the FP32 difference, squares and sum add <= (2+sqrt2)*u*(r+eps);
+        // the bound is 2*sqrt2*u*(W+r) + (2+sqrt2)*u*(r+eps) < eps/2.
+        const double u = std::ldexp(1.0, -24);
+        double w = (eps / 2 - (2 + std::sqrt(2.0)) * u * (r + eps)) / (2 * std::sqrt(2.0) * u * (1 + u)) - r;
+        s.w32_max = w > 0 ? w : 0.0;
     }
     uint64_t node_cap = cfg->node_capacity ? cfg->node_capacity
                                            : std::max<uint64_t>(65536, 16 * (N / cfg->grid_capacity + 1) + 1024);
@@ -1433,6 +1537,10 @@ glint_status glint_create(const glint_config *cfg, glint_ctx **out)
     ALLOC(s.ustart, node_cap);
     ALLOC(s.sx, N);
     ALLOC(s.sy, N);
+    ALLOC(s.sxp, N);
+    ALLOC(s.syp, N);
+    ALLOC(c->hist[0], 2 * N);
+    ALLOC(c->hist[1], 2 * N);
     ALLOC(s.spid, N);
     if (cfg->precision == GLINT_PRECISION_FP32_GUARD) {
         ALLOC(s.fx, N);
@@ -1647,8 +1755,10 @@ glint_status glint_run_ticks(glint_ctx *c, const double *d_trace, uint32_t first
     }
     glint_status st = GLINT_OK;
     c->tick = first_tick;
-    k_begin_trace<<<1, 1, 0, c->stream>>>(c->s.tp, d_trace, c->tab[1 - c->cur_gen], c->tab[c->cur_gen],
-                                          first_tick, c->has_prev ? c->prev_tick : first_tick);
+    const int g = c->cur_gen;
+    k_begin_trace<<<1, 1, 0, c->stream>>>(c->s.tp, d_trace, c->tab[1 - g], c->tab[g], c->hist[1 - g], c->hist[g],
+                                          first_tick, c->has_prev ? c->prev_tick : first_tick,
+                                          c->has_prev ? 1u : 0u);
     if (!c->built) {
         st = build_tree(c);
         if (st) return st;
@@ -1656,10 +1766,7 @@ glint_status glint_run_ticks(glint_ctx *c, const double *d_trace, uint32_t first
     if (tick_ms) HIP_TRY(c, hipEventRecord(evs[0], c->stream));
     for (uint32_t k = 0; k < nticks; k++) {
         if (k > 0) {
-            launch(c, K_TICK_PARAMS, [&] {
-                k_swap_tables<<<1, 1, 0, c->stream>>>(c->s.tp);
-                k_advance_tick<<<1, 1, 0, c->stream>>>(c->s.tp, stride);
-            });
+            launch(c, K_TICK_PARAMS, [&] { k_advance_tick<<<1, 1, 0, c->stream>>>(c->s.tp, stride); });
         }
         enqueue_partition(c);
         enqueue_lookup(c);
@@ -1891,6 +1998,25 @@ glint_status glint_device_malloc(void **ptr, uint64_t bytes, uint32_t device)
         return GLINT_ERR_OUT_OF_MEMORY;
     }
     return GLINT_OK;
+}
+
+namespace {
+__global__ void k_copy(const uint4 *src, uint4 *dst, uint64_t n16)
+{
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n16;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+}  // namespace
+
+glint_status glint_device_copy(void *dst, const void *src, uint64_t bytes)
+{
+    if (!dst || !src || (bytes % 16) || ((uintptr_t)dst % 16) || ((uintptr_t)src % 16))
+        return GLINT_ERR_INVALID_ARGUMENT;
+    uint64_t n16 = bytes / 16;
+    k_copy<<<grid_for(n16, 16384), kBlock>>>((const uint4 *)src, (uint4 *)dst, n16);
+    if (hipGetLastError() != hipSuccess) return GLINT_ERR_HIP;
+    return hipDeviceSynchronize() == hipSuccess ? GLINT_OK : GLINT_ERR_HIP;
 }
 
 glint_status glint_device_free(void *ptr)

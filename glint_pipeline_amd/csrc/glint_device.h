@@ -98,10 +98,12 @@ struct TickParams {
     const double *trace_base;   // run_ticks: tick k of the trace at trace_base + 2*N*k
     Slot *tab_old;              // pairs seen at prev_tick (read, flagged, then cleared)
     Slot *tab_cur;              // pairs seen at tick (filled by refine)
+    double *xy_prev;            // positions at prev_tick (pid order), valid if has_prev
+    double *xy_save;            // receives this tick's positions for the next tick
     uint32_t tick;
     uint32_t prev_tick;
     uint32_t trace_index;
-    uint32_t pad;
+    uint32_t has_prev;
 };
 
 struct DevState {
@@ -118,6 +120,7 @@ struct DevState {
     double t64;         // largest s with sqrt(s) <= r (host-computed)
     float t32_in;       // FP32 squared-distance bounds of the guard band
     float t32_out;
+    double w32_max;     // widest leaf decided in FP32 (error bound inside the band)
     unsigned long long tab_mask;
     unsigned long long unit_cap;
     unsigned long long meet_cap;
@@ -142,6 +145,8 @@ struct DevState {
     uint32_t *ustart;       // first unit of each leaf in unit_pid
     double *sx;
     double *sy;
+    double *sxp;            // positions at prev_tick, sorted like sx/sy
+    double *syp;
     uint32_t *spid;
     float *fx;
     float *fy;

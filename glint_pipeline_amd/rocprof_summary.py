@@ -1,0 +1,70 @@
+"""Summarise rocprofv3 CSV output (kernel stats, kernel trace, PMC counters)
+as the text tables kept under profiles/.
+
+    python -m glint_pipeline_amd.rocprof_summary <rocprofv3 output dir> [--kernel NAME]
+"""
+from __future__ import annotations
+
+import argparse
+import csv
+import glob
+import os
+from collections import defaultdict
+
+
+def _short(name: str) -> str:
+    n = name.split("(")[0]
+    for pre in ("void ", "(anonymous namespace)::"):
+        n = n.replace(pre, "")
+    return n.strip()
+
+
+def summarize(d: str, kernel: str | None = None) -> str:
+    out = []
+    stats = glob.glob(os.path.join(d, "**", "*kernel_stats.csv"), recursive=True)
+    for f in stats:
+        out.append(f"# kernel stats ({os.path.relpath(f, d)})")
+        out.append(f"{'kernel':28s} {'calls':>7s} {'total_ms':>11s} {'avg_ms':>10s} {'min_ms':>10s} {'max_ms':>10s} {'pct':>6s}")
+        with open(f) as fh:
+            rows = list(csv.DictReader(fh))
+        for r in rows:
+            name = _short(r.get("Name", r.get("KernelName", "?")))
+            if kernel and kernel not in name:
+                continue
+            calls = int(float(r.get("Calls", 0)))
+            tot = float(r.get("TotalDurationNs", 0)) / 1e6
+            avg = float(r.get("AverageNs", 0)) / 1e6
+            mn = float(r.get("MinNs", 0)) / 1e6
+            mx = float(r.get("MaxNs", 0)) / 1e6
+            pct = float(r.get("Percentage", 0))
+            out.append(f"{name:28s} {calls:7d} {tot:11.3f} {avg:10.4f} {mn:10.4f} {mx:10.4f} {pct:6.2f}")
+    pmc = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+    for f in pmc:
+        acc = defaultdict(lambda: defaultdict(list))
+        with open(f) as fh:
+            for r in csv.DictReader(fh):
+                name = _short(r.get("Kernel_Name", r.get("Kernel-Name", "?")))
+                if kernel and kernel not in name:
+                    continue
+                acc[name][r.get("Counter_Name", "?")].append(float(r.get("Counter_Value", 0)))
+        out.append(f"# counters per dispatch ({os.path.relpath(f, d)})")
+        out.append(f"{'kernel':28s} {'counter':14s} {'dispatches':>10s} {'avg':>16s} {'min':>16s} {'max':>16s}")
+        for name, cs in sorted(acc.items()):
+            for cn, vals in sorted(cs.items()):
+                out.append(f"{name:28s} {cn:14s} {len(vals):10d} {sum(vals) / len(vals):16.1f} {min(vals):16.1f} "
+                           f"{max(vals):16.1f}")
+    if not out:
+        out.append(f"no rocprofv3 CSV files under {d}")
+    return "\n".join(out)
+
+
+def main() -> None:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("dir")
+    ap.add_argument("--kernel", default=None)
+    a = ap.parse_args()
+    print(summarize(a.dir, a.kernel))
+
+
+if __name__ == "__main__":
+    main()

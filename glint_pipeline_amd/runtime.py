@@ -122,6 +122,7 @@ SIGNATURES = {
                                 ctypes.POINTER(_U64), _U32, ctypes.POINTER(_U32), _I]),
     "glint_device_malloc": (_I, [ctypes.POINTER(_VP), _U64, _U32]),
     "glint_device_free": (_I, [_VP]),
+    "glint_device_copy": (_I, [_VP, _VP, _U64]),
     "glint_memcpy_to_device": (_I, [_VP, _VP, _U64]),
     "glint_memcpy_to_host": (_I, [_VP, _VP, _U64]),
     "glint_device_count": (_I, [ctypes.POINTER(_I)]),

@@ -223,6 +223,9 @@ glint_status glint_debug_inject_failure(glint_ctx *ctx, const char *stage);
 /* Device memory helpers for callers without a GPU framework. */
 glint_status glint_device_malloc(void **ptr, uint64_t bytes, uint32_t device);
 glint_status glint_device_free(void *ptr);
+/* Device-to-device copy by a plain streaming kernel (16-byte aligned); used
+ * to calibrate HBM byte counters against a copy of known size. */
+glint_status glint_device_copy(void *dst, const void *src, uint64_t bytes);
 glint_status glint_memcpy_to_device(void *dst, const void *src, uint64_t bytes);
 glint_status glint_memcpy_to_host(void *dst, const void *src, uint64_t bytes);
 glint_status glint_device_count(int *count);

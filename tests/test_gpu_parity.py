@@ -56,3 +56,55 @@ def test_parity_plain_cpu_reference_config():
 def test_parity_complete_mode_is_all_pairs():
     w = gen.uniform_city(2500, 20, 400.0, seed=31, grid_capacity=16)
     run_parity(w, 20, complete=True)
+
+
+def test_parity_fp32_guard_band_equals_fp64():
+    # the guard-band form decides every pair outside (r-eps, r+eps] in FP32 and
+    # re-tests the rest in FP64: the contact and meeting sets stay bit-identical
+    w = gen.Workload("hot_guard", 30_000, 25, 3000.0, seed=41, num_hotspots=8, hotspot_fraction=0.8,
+                     hotspot_sigma=80.0, grid_capacity=64)
+    run_parity(w, 25, precision=runtime.PRECISION_FP32_GUARD)
+
+
+def test_guard_band_rechecks_only_inside_band():
+    w = gen.uniform_city(5000, 10, 600.0, seed=43, grid_capacity=32)
+    g = gen.Generator(w)
+    cfg = runtime.make_config(w.num_objects, w.box(), grid_capacity=w.grid_capacity,
+                              precision=runtime.PRECISION_FP32_GUARD, guard_epsilon=1e-3)
+    with runtime.Workbench(cfg) as wb:
+        for t in range(w.ticks):
+            xy = g.positions() if t == 0 else g.step()
+            wb.tick(xy, t)
+        tot = wb.totals()
+    # rechecks are pairs whose FP32 distance lies within 1 mm of the 2 m reach
+    assert tot["rechecks"] < tot["contacts"]
+
+
+def test_run_ticks_device_trace_matches_oracle():
+    """Many ticks from a device-resident trace in one call (no host sync per
+    tick) give the same active pairs and meeting totals as the oracle."""
+    from oracle.oracle import Oracle
+    w = gen.uniform_city(20_000, 40, 1000.0, seed=51, grid_capacity=32)
+    g = gen.Generator(w)
+    dw = runtime.DeviceWalkers(g)
+    trace = runtime.DeviceBuffer(16 * w.num_objects * w.ticks)
+    trace.upload(g.positions(), 0)
+    dw.walk(w.ticks - 1, trace, 16 * w.num_objects)
+    o = Oracle(w.num_objects, w.box(), w.reach_distance, w.grid_capacity, w.min_meet_time)
+    om = 0
+    for t in range(w.ticks):
+        o.tick(g.positions() if t == 0 else g.step(), t)
+        om += len(o.meetings())
+    cfg = runtime.make_config(w.num_objects, w.box(), grid_capacity=w.grid_capacity)
+    with runtime.Workbench(cfg) as wb:
+        ms = wb.run_ticks(trace.address(0), 0, 15)
+        ms2 = wb.run_ticks(trace.address(16 * w.num_objects * 15), 15, w.ticks - 15)
+        assert len(ms) == 15 and len(ms2) == w.ticks - 15 and (ms > 0).all()
+        tot = wb.totals()
+        assert tot["ticks"] == w.ticks and tot["meetings"] == om
+        k, s = wb.active_pairs()
+        ok, os_ = o.active()
+        assert np.array_equal(k, ok) and np.array_equal(s, os_)
+        b, lv, _ = wb.leaves()
+        ob, olv, _ = o.leaves()
+        assert np.array_equal(b, ob) and np.array_equal(lv, olv)
