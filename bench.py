@@ -75,6 +75,21 @@ def algorithmic_bytes(kernel: str, n: int, per_tick: dict, slots: int, precision
     return 0.0
 
 
+def profiled_traffic(kernel: str, workload: str, precision: int, objects: int) -> float | None:
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 summary
+    (FETCH_SIZE + WRITE_SIZE runs scaled by a plain copy's counters), when a
+    summary exists for this kernel, workload and precision."""
+    tag = f"{kernel}_{workload}_{objects // 1_000_000}m_{'fp32guard' if precision else 'fp64'}_traffic.txt"
+    f = ROOT / "profiles" / tag
+    if not f.exists():
+        return None
+    for line in f.read_text().splitlines():
+        parts = line.split()
+        if len(parts) == 3 and parts[0] == "TRAFFIC_BYTES_PER_LAUNCH" and parts[1] == kernel:
+            return float(parts[2])
+    return None
+
+
 def cpu_baseline(w, ticks: int, threads: int) -> dict:
     """The CPU oracle (C + OpenMP) on the first ticks of the same workload."""
     from glint_pipeline_amd import gen
@@ -178,8 +193,9 @@ def main() -> int:
         abytes = algorithmic_bytes(dom, N, per_tick, slots, precision)
         avg_s = dms / dcnt / 1e3
         achieved = abytes / avg_s / 1e9 if abytes and avg_s > 0 else 0.0
+        traffic = profiled_traffic(dom, args.workload, precision, N)
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom,
                     "kernel_ms_per_launch": round(dms / dcnt, 4), "algorithmic_bytes_per_launch": abytes,
                     "kernel_share_of_tick": round(dms / (elapsed * 1e3), 4)}
 

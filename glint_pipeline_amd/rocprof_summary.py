@@ -13,10 +13,8 @@ from collections import defaultdict
 
 
 def _short(name: str) -> str:
-    n = name.split("(")[0]
-    for pre in ("void ", "(anonymous namespace)::"):
-        n = n.replace(pre, "")
-    return n.strip()
+    n = name.replace("(anonymous namespace)::", "").replace("void ", "")
+    return n.split("(")[0].strip()
 
 
 def summarize(d: str, kernel: str | None = None) -> str:
