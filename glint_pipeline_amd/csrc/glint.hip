@@ -1409,6 +1409,11 @@ const char *glint_last_error_message(const glint_ctx *ctx)
     return ctx ? ctx->err.c_str() : g_create_error.c_str();
 }
 
+double glint_reach_threshold(double r)
+{
+    return r > 0 ? sqrt_threshold(r) : 0.0;
+}
+
 void glint_config_default(glint_config *cfg)
 {
     if (!cfg) return;

@@ -97,6 +97,7 @@ _VP = ctypes.c_void_p
 _U32, _U64, _I = ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int
 SIGNATURES = {
     "glint_status_string": (ctypes.c_char_p, [_I]),
+    "glint_reach_threshold": (ctypes.c_double, [ctypes.c_double]),
     "glint_last_error_message": (ctypes.c_char_p, [_VP]),
     "glint_config_default": (None, [ctypes.POINTER(Config)]),
     "glint_create": (_I, [ctypes.POINTER(Config), ctypes.POINTER(_VP)]),

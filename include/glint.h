@@ -112,6 +112,12 @@ const char *glint_status_string(glint_status s);
 /* Detail of the last failure on this context (or the last create failure). */
 const char *glint_last_error_message(const glint_ctx *ctx);
 
+/* Squared-distance threshold of the device predicates: the largest double
+ * s with sqrt(s) <= r under IEEE round-to-nearest, so that for every s,
+ * s <= glint_reach_threshold(r) exactly when sqrt(s) <= r (the reference's
+ * test, geometry.h:47 with refinement.cpp:33).  Host-only. */
+double glint_reach_threshold(double r);
+
 /* Defaults of the reference (config.h:29-41): capacity 50, min 5, r = 2 m,
  * delay 1, dynamic schema on; the caller sets num_objects and the box. */
 void glint_config_default(glint_config *cfg);
