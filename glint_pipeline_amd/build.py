@@ -58,12 +58,21 @@ def build_hip() -> Path:
     return out
 
 
+def build_host() -> Path:
+    """The C++ driver over the C ABI (links libglint.so)."""
+    out = PKG / "host" / "glint_pipeline"
+    _run(["g++", "-O2", "-std=c++17", "-Wall", f"-I{INCLUDE}", f"-I{PKG / 'host'}", "-o", str(out),
+          str(PKG / "host" / "glint_pipeline.cpp"), f"-L{PKG}", "-lglint", f"-Wl,-rpath,$ORIGIN/.."])
+    return out
+
+
 def build_all() -> None:
     build_gen()
     build_hip()
+    build_host()
 
 
 if __name__ == "__main__":
     what = sys.argv[1:] or ["all"]
     for w in what:
-        {"gen": build_gen, "hip": build_hip, "all": build_all}[w]()
+        {"gen": build_gen, "hip": build_hip, "host": build_host, "all": build_all}[w]()
