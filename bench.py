@@ -137,12 +137,16 @@ def main() -> int:
     N = w.num_objects
     W, K = args.warmup, args.steps
     precision = runtime.PRECISION_FP32_GUARD if args.precision == "fp32guard" else runtime.PRECISION_FP64
+    # contacts per tick grow with the square of the hotspot density: ~65 per
+    # object at 10M skewed (measured), ~0.06 per object for uniform walkers
+    per_obj = {"skewed": 70.0 * N / 1e7, "plain_gpu": 0.1, "plain_cpu": 0.1}[args.workload]
+    est_contacts = max(1 << 20, int(per_obj * N))
     if args.buckets_log2 is not None:
         slots = 1 << args.buckets_log2
     else:
-        # the table holds one generation of distinct contacts at load <= ~0.5
-        est = {"skewed": 90.0, "plain_gpu": 0.2, "plain_cpu": 0.2}[args.workload]
-        slots = 1 << max(20, math.ceil(math.log2(max(1.0, 2.2 * est * N * (N / 1e7 if args.workload == "skewed" else 1)))))
+        # one generation of distinct pairs at a load factor of at most ~0.75
+        # (probing stays inside 2048-slot regions held in LDS)
+        slots = 1 << max(20, math.ceil(math.log2(est_contacts / 0.75)))
     log(f"[bench] workload: {desc}; warmup {W}, timed {K} ticks; table slots 2^{int(math.log2(slots))}")
 
     # ---- inputs: host init of tick 0, device walk for the rest, all in HBM
@@ -158,7 +162,8 @@ def main() -> int:
 
     cfg = runtime.make_config(N, w.box(), reach_distance=w.reach_distance, grid_capacity=w.grid_capacity,
                               min_meet_time=w.min_meet_time, precision=precision, num_meeting_buckets=slots,
-                              unit_capacity=8 * N, meeting_capacity=max(1 << 20, 6 * N))
+                              unit_capacity=8 * N, meeting_capacity=max(1 << 20, 6 * N),
+                              contact_capacity=int(est_contacts * 1.4))
     wb = runtime.Workbench(cfg)
     t0 = time.perf_counter()
     if W:

@@ -38,7 +38,13 @@ constexpr int kScanTile = kBlock * kScanItems;      // elements per scan block
 constexpr int kTile = 512;                          // points per LDS tile of the join
 constexpr int kStage = 512;                         // staged contacts per wave
 constexpr int kFlushRound = 4;                      // join columns between stage checks
-constexpr int kMeetStage = 256;                     // staged meetings per wave (evaluate)
+constexpr int kMeetStage = 256;                     // staged meetings per wave (table update)
+constexpr uint32_t kChunk = 4096;                   // contact-list entries reserved per atomic
+constexpr int kRegion = 2048;                       // meeting-table slots per region (LDS-resident)
+constexpr int kBucketTile = 4096;                   // contacts per LDS tile of the coarse pass
+constexpr int kFineTile = 4096;                     // contacts per LDS tile of the fine pass
+constexpr uint32_t kMaxCoarse = 1024;               // coarse bins of pass 1
+constexpr uint32_t kMaxFine = 2048;                 // fine bins of pass 2
 constexpr int kBorderStage = 256;                   // staged border objects per wave (assign)
 constexpr int kLookupStack = 128;
 constexpr int kMaxCand = 32;                        // candidate leaves per border object
@@ -151,7 +157,7 @@ __global__ void k_reset_tick(DevState s)
     c->n_inserted = 0;
     c->n_meet = 0;
     c->n_rechecks = 0;
-    c->n_rec = 0;
+    c->n_clist = 0;
     c->work_border = 0;
     c->n_split = 0;
     c->n_merge = 0;
@@ -664,26 +670,27 @@ __global__ __launch_bounds__(kBlock) void k_unit_scatter(DevState s)
 }
 
 // ------------------------------------------------------------ meeting table
+//
+// The table is an open-addressed hash of the pair key (pid_lo << 32) | pid_hi
+// in two generations of B 16-byte slots.  h = mix64(key): the high bits pick
+// a region of kRegion slots, the low bits the home slot inside it, and linear
+// probing wraps inside the region.  A tick's contacts are bucketed by region
+// and every region is then updated by one workgroup in LDS: the region of the
+// previous generation is read once, the new region is written once.
 
-// Upsert of a contact (upsert4 below): look the pair up in the previous
-// generation (its start tick if it was seen at prev_tick, flagged as
-// continued), then insert it into the current generation.  Linear probing,
-// 64-bit CAS on the key; a pair found twice in one tick keeps one entry.
-
-// ------------------------------------------------------------ refine: join
-
-__device__ __forceinline__ unsigned long long pair_key(uint32_t a, uint32_t b)
+__device__ __forceinline__ uint32_t region_of(const DevState &s, unsigned long long key)
 {
-    uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
-    return ((unsigned long long)lo << 32) | hi;
+    // a one-region table has no region bits (a shift by 64 is undefined)
+    return s.region_bits ? (uint32_t)(mix64(key) >> (64 - s.region_bits)) : 0u;
 }
 
-// Contacts found by one wave are staged in LDS and upserted lane-parallel
-// when the stage fills, so many independent table accesses are in flight.
+// Contacts found by one wave are staged in LDS and appended to the tick's
+// contact list in 4096-entry chunks reserved with one atomic each.
 struct Stager {
     unsigned long long *buf;
-    uint32_t n;           // uniform across the wave
-    uint32_t inserted;    // per lane
+    uint32_t n;                 // staged entries (uniform across the wave)
+    unsigned long long chunk;   // current chunk in the contact list
+    uint32_t used;              // entries of the chunk written
 };
 
 __device__ __forceinline__ void stage_push(Stager &st, bool hit, unsigned long long key)
@@ -693,110 +700,46 @@ __device__ __forceinline__ void stage_push(Stager &st, bool hit, unsigned long l
     st.n += (uint32_t)__popcll(m);
 }
 
-// Four independent upserts per lane: the first probes of all four are issued
-// before any result is needed, which keeps more random accesses in flight.
-// A staged contact carries bit 63 when the pair was within reach at
-// prev_tick (the only case in which it can be in the previous generation);
-// for every other contact the lookup is skipped and start = tick.
-__device__ __forceinline__ uint32_t upsert4(const DevState &s, Slot *tab_old, Slot *tab_cur, uint32_t tick,
-                                            const unsigned long long tagged[4], const bool valid[4])
-{
-    const unsigned long long mask = s.tab_mask;
-    unsigned long long h0[4];
-    Slot o[4];
-    uint32_t start[4];
-    bool look[4];
-    unsigned long long key[4];
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-        look[q] = valid[q] && (tagged[q] >> 63);   // within reach at prev_tick
-        key[q] = tagged[q] & ~(1ull << 63);
-        h0[q] = mix64(key[q]) & mask;
-    }
-#pragma unroll
-    for (int q = 0; q < 4; q++)
-        if (look[q]) o[q] = tab_old[h0[q]];
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-        if (!valid[q]) continue;
-        start[q] = tick;
-        if (!look[q]) continue;
-        if (o[q].key == key[q]) {
-            start[q] = o[q].start;
-            if (!o[q].seen) tab_old[h0[q]].seen = 1;
-        } else if (o[q].key != kEmptyKey) {
-            unsigned long long h = (h0[q] + 1) & mask;
-            for (unsigned long long probes = 1; probes <= mask; probes++) {
-                unsigned long long k = tab_old[h].key;
-                if (k == key[q]) {
-                    start[q] = tab_old[h].start;
-                    if (!tab_old[h].seen) tab_old[h].seen = 1;
-                    break;
-                }
-                if (k == kEmptyKey) break;
-                h = (h + 1) & mask;
-            }
-        }
-    }
-    unsigned long long prev[4];
-#pragma unroll
-    for (int q = 0; q < 4; q++)
-        if (valid[q]) prev[q] = atomicCAS(&tab_cur[h0[q]].key, kEmptyKey, key[q]);
-    uint32_t inserted = 0;
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-        if (!valid[q]) continue;
-        if (prev[q] == kEmptyKey) {
-            tab_cur[h0[q]].start = start[q];
-            inserted++;
-        } else if (prev[q] != key[q]) {
-            unsigned long long h = (h0[q] + 1) & mask;
-            for (unsigned long long probes = 1;; probes++) {
-                if (probes > mask) {
-                    atomicOr(&s.ctr->error, kErrTableFull);
-                    break;
-                }
-                unsigned long long pv = atomicCAS(&tab_cur[h].key, kEmptyKey, key[q]);
-                if (pv == kEmptyKey) {
-                    tab_cur[h].start = start[q];
-                    inserted++;
-                    break;
-                }
-                if (pv == key[q]) break;
-                h = (h + 1) & mask;
-            }
-        }
-    }
-    return inserted;
-}
-
-__device__ __forceinline__ void stage_flush(Stager &st, const DevState &s, Slot *tab_old, Slot *tab_cur,
-                                            uint32_t tick)
+__device__ __forceinline__ void stage_flush(Stager &st, const DevState &s)
 {
     if (st.n == 0) return;
     wave_sync();
     const uint32_t lane = lane_id();
-    if (s.record) {
-        for (uint32_t k = lane; k < st.n; k += 64) {
-            unsigned long long idx = atomicAdd(&s.ctr->n_rec, 1ull);
-            if (idx < s.rec_cap)
-                s.rec[idx] = st.buf[k] & ~(1ull << 63);
-            else
-                atomicOr(&s.ctr->error, kErrContactCap);
+    uint32_t done = 0;
+    while (done < st.n) {
+        if (st.used == kChunk) {
+            unsigned long long c = 0;
+            if (lane == 0) c = atomicAdd(&s.ctr->n_clist, (unsigned long long)kChunk);
+            st.chunk = __shfl(c, 0);
+            st.used = 0;
         }
-    }
-    for (uint32_t k = lane; k < st.n; k += 256) {
-        unsigned long long key[4];
-        bool valid[4];
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            valid[q] = k + 64 * q < st.n;
-            key[q] = valid[q] ? st.buf[k + 64 * q] : 0ull;
+        const uint32_t take = min(st.n - done, kChunk - st.used);
+        if (st.chunk + kChunk <= s.clist_cap) {
+            for (uint32_t k = lane; k < take; k += 64) s.clist[st.chunk + st.used + k] = st.buf[done + k];
+        } else if (lane == 0) {
+            atomicOr(&s.ctr->error, kErrContactCap);
         }
-        st.inserted += upsert4(s, tab_old, tab_cur, tick, key, valid);
+        st.used += take;
+        done += take;
     }
     wave_sync();
     st.n = 0;
+}
+
+// The unused tail of a wave's last chunk holds kEmptyKey (skipped later).
+__device__ __forceinline__ void stage_close(Stager &st, const DevState &s)
+{
+    stage_flush(st, s);
+    if (st.used < kChunk && st.chunk + kChunk <= s.clist_cap)
+        for (uint32_t k = st.used + lane_id(); k < kChunk; k += 64) s.clist[st.chunk + k] = kEmptyKey;
+}
+
+// ------------------------------------------------------------ refine: join
+
+__device__ __forceinline__ unsigned long long pair_key(uint32_t a, uint32_t b)
+{
+    uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
+    return ((unsigned long long)lo << 32) | hi;
 }
 
 struct JoinTile {
@@ -870,15 +813,13 @@ __device__ __forceinline__ void load_tile(JoinTile &t, const DevState &s, uint32
 __global__ __launch_bounds__(kBlock) void k_join(DevState s)
 {
     __shared__ JoinLds lds;
-    Slot *tab_old = s.tp->tab_old, *tab_cur = s.tp->tab_cur;
-    const uint32_t tick = s.tp->tick;
     const double *xy = s.tp->xy;
     const double *xy_prev = s.tp->xy_prev;
     const bool has_prev = s.tp->has_prev != 0;
     const double t64 = s.t64;
     const uint32_t nl = (s.ctr->error & kErrUnitCap) ? 0 : s.ctr->n_leaf_list;
     const uint32_t wave = threadIdx.x / 64, lane = lane_id();
-    Stager st{lds.stage[wave], 0, 0};
+    Stager st{lds.stage[wave], 0, 0, kChunk};
     uint32_t n_in = 0, n_bd = 0, n_re = 0;   // per lane
     PairTest pt{t64, s.t32_in, s.t32_out, false};
     for (;;) {
@@ -987,7 +928,7 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
                                     stage_push(st, hit, key);
                                 }
                             }
-                            if (st.n > kStage - 64 * kFlushRound) stage_flush(st, s, tab_old, tab_cur, tick);
+                            if (st.n > kStage - 64 * kFlushRound) stage_flush(st, s);
                         }
                     }
                     __syncthreads();
@@ -995,85 +936,373 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
             }
         }
     }
-    stage_flush(st, s, tab_old, tab_cur, tick);
+    stage_close(st, s);
     // one atomic per wave for each counter
-    uint32_t ins = st.inserted, a = n_in, b = n_bd, re = n_re;
+    uint32_t a = n_in, b = n_bd, re = n_re;
     for (int d = 32; d >= 1; d >>= 1) {
-        ins += __shfl_xor(ins, d);
         a += __shfl_xor(a, d);
         b += __shfl_xor(b, d);
         re += __shfl_xor(re, d);
     }
     if (lane == 0) {
-        if (ins) atomicAdd(&s.ctr->n_inserted, (unsigned long long)ins);
         if (a + b) atomicAdd(&s.ctr->n_contacts, (unsigned long long)(a + b));
         if (b) atomicAdd(&s.ctr->n_contacts_border, (unsigned long long)b);
         if (re) atomicAdd(&s.ctr->n_rechecks, (unsigned long long)re);
     }
 }
 
-// ------------------------------------------------------------ evaluate
+// ------------------------------------------------------------ bucketing
+//
+// Two passes group the contact list by region.  Pass 1 (k_bucket_count, a
+// scan, k_bucket_scatter) splits it by the high bits of the region index into
+// at most 1024 coarse bins, sentinels going to one extra bin that is never
+// written out; each block sorts 8192-entry tiles in LDS so its writes form
+// runs.  Pass 2 (k_bucket_fine) splits every coarse bin by the low bits with
+// one workgroup per bin and records the first contact of every region.
 
-// meeting.cpp:134-155 over the previous generation: a pair seen at
-// prev_tick and not again at this tick has ended; it is emitted iff
-// cur - start >= min + 1; every slot is cleared for reuse.
-__global__ __launch_bounds__(kBlock) void k_evaluate(DevState s)
+__device__ __forceinline__ uint32_t coarse_bin(const DevState &s, unsigned long long tagged)
 {
-    // Emitted meetings are staged per wave in LDS and appended with one
-    // atomic per flush: a single global counter hit once per wave-iteration
-    // would serialise the scan.
-    __shared__ glint_meeting stage[kBlock / 64][kMeetStage];
-    glint_meeting *st = stage[threadIdx.x / 64];
-    uint32_t n = 0;
-    const uint32_t lane = lane_id();
-    Slot *tab = s.tp->tab_old;
+    if (tagged == kEmptyKey) return s.coarse_bins;
+    return region_of(s, tagged & ~(1ull << 63)) >> s.lo_bits;
+}
+
+__device__ __forceinline__ void block_range(const DevState &s, unsigned long long n, unsigned long long &b0,
+                                            unsigned long long &b1)
+{
+    const unsigned long long per = ((n + gridDim.x - 1) / gridDim.x + kBucketTile - 1) / kBucketTile * kBucketTile;
+    b0 = min(n, per * blockIdx.x);
+    b1 = min(n, b0 + per);
+}
+
+__global__ __launch_bounds__(kBlock) void k_bucket_count(DevState s)
+{
+    __shared__ uint32_t hist[kMaxCoarse + 1];
+    const unsigned long long n = min(s.ctr->n_clist, s.clist_cap);
+    for (uint32_t k = threadIdx.x; k <= s.coarse_bins; k += kBlock) hist[k] = 0;
+    __syncthreads();
+    unsigned long long b0, b1;
+    block_range(s, n, b0, b1);
+    for (unsigned long long i = b0 + threadIdx.x; i < b1; i += kBlock) atomicAdd(&hist[coarse_bin(s, s.clist[i])], 1u);
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k <= s.coarse_bins; k += kBlock) s.bcount[(size_t)k * gridDim.x + blockIdx.x] = hist[k];
+}
+
+// Exclusive offsets of a scanned counter array (third kernel of the scan).
+__global__ __launch_bounds__(kBlock) void k_scan_write(const uint32_t *in, uint32_t n, const uint32_t *block_sums,
+                                                       uint32_t *out)
+{
+    uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    uint32_t c[kScanItems];
+    uint32_t v = 0;
+    for (int k = 0; k < kScanItems; k++) {
+        c[k] = base + k < n ? in[base + k] : 0;
+        v += c[k];
+    }
+    uint32_t total;
+    uint32_t ex = block_exclusive_scan(v, &total) + block_sums[blockIdx.x];
+    for (int k = 0; k < kScanItems; k++) {
+        if (base + k < n) out[base + k] = ex;
+        ex += c[k];
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_bucket_scatter(DevState s)
+{
+    __shared__ unsigned long long tile[kBucketTile];
+    __shared__ uint32_t hist[kMaxCoarse + 1], start[kMaxCoarse + 1], cursor[kMaxCoarse + 1],
+        running[kMaxCoarse + 1];
+    __shared__ uint32_t wsum[kBlock / 64];
+    const unsigned long long n = min(s.ctr->n_clist, s.clist_cap);
+    const uint32_t nb = s.coarse_bins + 1;
+    unsigned long long b0, b1;
+    block_range(s, n, b0, b1);
+    for (uint32_t k = threadIdx.x; k < nb; k += kBlock) running[k] = s.boff[(size_t)k * gridDim.x + blockIdx.x];
+    for (unsigned long long t0 = b0; t0 < b1; t0 += kBucketTile) {
+        const uint32_t tn = (uint32_t)min((unsigned long long)kBucketTile, b1 - t0);
+        for (uint32_t k = threadIdx.x; k < nb; k += kBlock) hist[k] = 0;
+        __syncthreads();
+        unsigned long long keys[kBucketTile / kBlock];
+#pragma unroll
+        for (int q = 0; q < kBucketTile / kBlock; q++) {
+            uint32_t i = threadIdx.x + q * kBlock;
+            keys[q] = i < tn ? s.clist[t0 + i] : kEmptyKey;
+            if (i < tn) atomicAdd(&hist[coarse_bin(s, keys[q])], 1u);
+        }
+        __syncthreads();
+        // exclusive scan of the tile histogram (nb <= 1025 bins, 5 per thread)
+        {
+            uint32_t c[5], v = 0;
+            for (int k = 0; k < 5; k++) {
+                uint32_t b = threadIdx.x * 5 + k;
+                c[k] = b < nb ? hist[b] : 0;
+                v += c[k];
+            }
+            uint32_t inc = v;
+            for (int d = 1; d < 64; d <<= 1) {
+                uint32_t y = __shfl_up(inc, d);
+                if (lane_id() >= (uint32_t)d) inc += y;
+            }
+            if (lane_id() == 63) wsum[threadIdx.x / 64] = inc;
+            __syncthreads();
+            uint32_t ex = inc - v;
+            for (uint32_t w = 0; w < threadIdx.x / 64; w++) ex += wsum[w];
+            for (int k = 0; k < 5; k++) {
+                uint32_t b = threadIdx.x * 5 + k;
+                if (b < nb) {
+                    start[b] = ex;
+                    cursor[b] = ex;
+                }
+                ex += c[k];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < kBucketTile / kBlock; q++) {
+            uint32_t i = threadIdx.x + q * kBlock;
+            if (i < tn) tile[atomicAdd(&cursor[coarse_bin(s, keys[q])], 1u)] = keys[q];
+        }
+        __syncthreads();
+        for (uint32_t p = threadIdx.x; p < tn; p += kBlock) {
+            unsigned long long key = tile[p];
+            uint32_t b = coarse_bin(s, key);
+            if (b < s.coarse_bins) s.clist2[running[b] + (p - start[b])] = key;
+        }
+        __syncthreads();
+        for (uint32_t k = threadIdx.x; k < nb; k += kBlock) running[k] += hist[k];
+        __syncthreads();
+    }
+}
+
+// Pass 2: one workgroup per coarse bin c splits its segment of clist2 by the
+// low region bits into clist and writes the first contact of each region.
+// Tiles of the segment are sorted by fine bin in LDS so that the writes form
+// runs (a per-item scatter into 512-2048 streams would write partial lines).
+__device__ __forceinline__ uint32_t block_scan_bins(uint32_t *bins, uint32_t nbins, uint32_t *out_excl,
+                                                    uint32_t add, uint32_t *wsum)
+{
+    // exclusive scan of bins[0..nbins) (nbins <= 8 * kBlock) into out_excl (+add)
+    uint32_t cc[8], v = 0;
+    for (int k = 0; k < 8; k++) {
+        uint32_t b = threadIdx.x * 8 + k;
+        cc[k] = b < nbins ? bins[b] : 0;
+        v += cc[k];
+    }
+    uint32_t inc = wave_inclusive_scan(v);
+    if (lane_id() == 63) wsum[threadIdx.x / 64] = inc;
+    __syncthreads();
+    uint32_t ex = inc - v, total = 0;
+    for (uint32_t w = 0; w < kBlock / 64; w++) {
+        if (w < threadIdx.x / 64) ex += wsum[w];
+        total += wsum[w];
+    }
+    for (int k = 0; k < 8; k++) {
+        uint32_t b = threadIdx.x * 8 + k;
+        if (b < nbins) out_excl[b] = add + ex;
+        ex += cc[k];
+    }
+    __syncthreads();
+    return total;
+}
+
+__global__ __launch_bounds__(kBlock) void k_bucket_fine(DevState s)
+{
+    __shared__ unsigned long long tile[kFineTile];
+    __shared__ uint32_t thist[kMaxFine], tstart[kMaxFine], tcur[kMaxFine], gcur[kMaxFine];
+    __shared__ uint32_t wsum[kBlock / 64];
+    const uint32_t G1 = s.bucket_blocks;
+    const uint32_t nf = 1u << s.lo_bits;
+    for (uint32_t c = blockIdx.x; c < s.coarse_bins; c += gridDim.x) {
+        const uint32_t seg0 = s.boff[(size_t)c * G1], seg1 = s.boff[(size_t)(c + 1) * G1];
+        for (uint32_t k = threadIdx.x; k < nf; k += kBlock) thist[k] = 0;
+        __syncthreads();
+        for (uint32_t i = seg0 + threadIdx.x; i < seg1; i += kBlock)
+            atomicAdd(&thist[region_of(s, s.clist2[i] & ~(1ull << 63)) & (nf - 1)], 1u);
+        __syncthreads();
+        block_scan_bins(thist, nf, gcur, seg0, wsum);
+        for (uint32_t k = threadIdx.x; k < nf; k += kBlock) s.bstart[(size_t)c * nf + k] = gcur[k];
+        __syncthreads();
+        for (uint32_t t0 = seg0; t0 < seg1; t0 += kFineTile) {
+            const uint32_t tn = min((uint32_t)kFineTile, seg1 - t0);
+            for (uint32_t k = threadIdx.x; k < nf; k += kBlock) thist[k] = 0;
+            __syncthreads();
+            unsigned long long keys[kFineTile / kBlock];
+            uint32_t fb[kFineTile / kBlock];
+#pragma unroll
+            for (int q = 0; q < kFineTile / kBlock; q++) {
+                const uint32_t i = threadIdx.x + q * kBlock;
+                keys[q] = i < tn ? s.clist2[t0 + i] : 0ull;
+                fb[q] = region_of(s, keys[q] & ~(1ull << 63)) & (nf - 1);
+                if (i < tn) atomicAdd(&thist[fb[q]], 1u);
+            }
+            __syncthreads();
+            block_scan_bins(thist, nf, tstart, 0, wsum);
+            for (uint32_t k = threadIdx.x; k < nf; k += kBlock) tcur[k] = tstart[k];
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < kFineTile / kBlock; q++) {
+                const uint32_t i = threadIdx.x + q * kBlock;
+                if (i < tn) tile[atomicAdd(&tcur[fb[q]], 1u)] = keys[q];
+            }
+            __syncthreads();
+            for (uint32_t p = threadIdx.x; p < tn; p += kBlock) {
+                const unsigned long long key = tile[p];
+                const uint32_t f = region_of(s, key & ~(1ull << 63)) & (nf - 1);
+                s.clist[gcur[f] + (p - tstart[f])] = key;
+            }
+            __syncthreads();
+            for (uint32_t k = threadIdx.x; k < nf; k += kBlock) gcur[k] += thist[k];
+            __syncthreads();
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) s.bstart[(size_t)s.coarse_bins * nf] = s.boff[(size_t)s.coarse_bins * G1];
+}
+
+// Without a second pass the coarse bins are the regions.
+__global__ void k_bucket_single(DevState s)
+{
+    const uint32_t G1 = s.bucket_blocks;
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c <= s.coarse_bins; c += gridDim.x * blockDim.x)
+        s.bstart[c] = s.boff[(size_t)c * G1];
+}
+
+// ------------------------------------------------------------ table update
+//
+// One workgroup per region: the upsert of every contact of the tick
+// (refinement.cpp:36-55 with set semantics) and the evaluation of the
+// previous generation (meeting.cpp:134-155) in LDS.  A pair of the previous
+// generation keeps its start tick if it is seen again; a pair not seen again
+// has ended at prev_tick and is emitted iff tick - start >= min + 1.
+struct TableLds {
+    unsigned long long oldk[kRegion];
+    unsigned long long newk[kRegion];
+    uint32_t olds[kRegion];
+    uint32_t news[kRegion];
+    uint32_t seen[kRegion];
+    glint_meeting stage[kBlock / 64][kMeetStage];
+};
+
+__global__ __launch_bounds__(kBlock) void k_table(DevState s)
+{
+    __shared__ TableLds L;
+    const Slot *tab_old = s.tp->tab_old;
+    Slot *tab_cur = s.tp->tab_cur;
     const uint32_t tick = s.tp->tick, last = s.tp->prev_tick;
-    const unsigned long long nslots = s.tab_mask + 1;
-    auto flush = [&]() {
-        if (n == 0) return;
+    const unsigned long long *list = s.lo_bits ? s.clist : s.clist2;
+    const uint32_t nreg = 1u << s.region_bits;
+    const uint32_t lane = lane_id();
+    glint_meeting *st = L.stage[threadIdx.x / 64];
+    uint32_t nst = 0, inserted = 0;
+    const bool void_tick = (s.ctr->error & kErrContactCap) != 0;
+    // the previous generation's slots of the next region are loaded into
+    // registers while the current region is processed
+    constexpr int kPer = kRegion / kBlock;
+    Slot pre[kPer];
+    if (blockIdx.x < nreg) {
+#pragma unroll
+        for (int q = 0; q < kPer; q++) pre[q] = tab_old[(size_t)blockIdx.x * kRegion + threadIdx.x + q * kBlock];
+    }
+    for (uint32_t r = blockIdx.x; r < nreg; r += gridDim.x) {
+        const size_t base = (size_t)r * kRegion;
+#pragma unroll
+        for (int q = 0; q < kPer; q++) {
+            const uint32_t k = threadIdx.x + q * kBlock;
+            L.oldk[k] = pre[q].key;
+            L.olds[k] = pre[q].start;
+            L.seen[k] = 0;
+            L.newk[k] = kEmptyKey;
+        }
+        const uint32_t rn = r + gridDim.x;
+        if (rn < nreg) {
+#pragma unroll
+            for (int q = 0; q < kPer; q++) pre[q] = tab_old[(size_t)rn * kRegion + threadIdx.x + q * kBlock];
+        }
+        __syncthreads();
+        const uint32_t c0 = void_tick ? 0 : s.bstart[r], c1 = void_tick ? 0 : s.bstart[r + 1];
+        for (uint32_t c = c0 + threadIdx.x; c < c1; c += kBlock) {
+            const unsigned long long tagged = list[c];
+            const unsigned long long key = tagged & ~(1ull << 63);
+            const uint32_t h = (uint32_t)mix64(key) & (kRegion - 1);
+            uint32_t start = tick;
+            if (tagged >> 63) {   // within reach at prev_tick: it may continue a run
+                for (uint32_t p = 0; p < kRegion; p++) {
+                    const uint32_t q = (h + p) & (kRegion - 1);
+                    const unsigned long long k = L.oldk[q];
+                    if (k == key) {
+                        start = L.olds[q];
+                        L.seen[q] = 1;
+                        break;
+                    }
+                    if (k == kEmptyKey) break;
+                }
+            }
+            bool placed = false;
+            for (uint32_t p = 0; p < kRegion; p++) {
+                const uint32_t q = (h + p) & (kRegion - 1);
+                const unsigned long long prev = atomicCAS(&L.newk[q], kEmptyKey, key);
+                if (prev == kEmptyKey) {
+                    L.news[q] = start;
+                    inserted++;
+                    placed = true;
+                    break;
+                }
+                if (prev == key) {   // found twice in this tick: one entry
+                    placed = true;
+                    break;
+                }
+            }
+            if (!placed) atomicOr(&s.ctr->error, kErrTableFull);
+        }
+        __syncthreads();
+        for (uint32_t k0 = 0; k0 < kRegion; k0 += kBlock) {
+            const uint32_t k = k0 + threadIdx.x;
+            const unsigned long long ok = L.oldk[k];
+            const bool emit = ok != kEmptyKey && !L.seen[k] && (tick - L.olds[k] >= s.min_meet + 1);
+            unsigned long long m = __ballot(emit);
+            if (emit) {
+                glint_meeting mt;
+                mt.pid_a = (uint32_t)(ok >> 32);
+                mt.pid_b = (uint32_t)ok;
+                mt.start = L.olds[k];
+                mt.end = last;
+                st[nst + __popcll(m & ((1ull << lane) - 1ull))] = mt;
+            }
+            nst += (uint32_t)__popcll(m);
+            if (nst > kMeetStage - 64) {
+                wave_sync();
+                unsigned long long b = 0;
+                if (lane == 0) b = atomicAdd(&s.ctr->n_meet, (unsigned long long)nst);
+                b = __shfl(b, 0);
+                for (uint32_t j = lane; j < nst; j += 64) {
+                    if (b + j < s.meet_cap)
+                        s.meet[b + j] = st[j];
+                    else
+                        atomicOr(&s.ctr->error, kErrMeetCap);
+                }
+                wave_sync();
+                nst = 0;
+            }
+            Slot ns;
+            ns.key = L.newk[k];
+            ns.start = L.news[k];
+            ns.seen = 0;
+            tab_cur[base + k] = ns;
+        }
+        __syncthreads();
+    }
+    if (nst) {
         wave_sync();
-        unsigned long long base = 0;
-        if (lane == 0) base = atomicAdd(&s.ctr->n_meet, (unsigned long long)n);
-        base = __shfl(base, 0);
-        for (uint32_t k = lane; k < n; k += 64) {
-            if (base + k < s.meet_cap)
-                s.meet[base + k] = st[k];
+        unsigned long long b = 0;
+        if (lane == 0) b = atomicAdd(&s.ctr->n_meet, (unsigned long long)nst);
+        b = __shfl(b, 0);
+        for (uint32_t j = lane; j < nst; j += 64) {
+            if (b + j < s.meet_cap)
+                s.meet[b + j] = st[j];
             else
                 atomicOr(&s.ctr->error, kErrMeetCap);
         }
-        wave_sync();
-        n = 0;
-    };
-    for (unsigned long long base = blockIdx.x * (unsigned long long)blockDim.x; base < nslots;
-         base += (unsigned long long)gridDim.x * blockDim.x) {
-        unsigned long long i = base + threadIdx.x;
-        bool emit = false;
-        Slot sl;
-        sl.key = kEmptyKey;
-        if (i < nslots) {
-            sl = tab[i];
-            if (sl.key != kEmptyKey) {
-                emit = !sl.seen && (tick - sl.start >= s.min_meet + 1);
-                Slot e;
-                e.key = kEmptyKey;
-                e.start = 0;
-                e.seen = 0;
-                tab[i] = e;
-            }
-        }
-        unsigned long long m = __ballot(emit);
-        if (emit) {
-            glint_meeting mt;
-            mt.pid_a = (uint32_t)(sl.key >> 32);
-            mt.pid_b = (uint32_t)sl.key;
-            mt.start = sl.start;
-            mt.end = last;
-            st[n + __popcll(m & ((1ull << lane) - 1ull))] = mt;
-        }
-        n += (uint32_t)__popcll(m);
-        if (n > kMeetStage - 64) flush();
     }
-    flush();
+    for (int d = 32; d >= 1; d >>= 1) inserted += __shfl_xor(inserted, d);
+    if (lane == 0 && inserted) atomicAdd(&s.ctr->n_inserted, (unsigned long long)inserted);
 }
 
 // ------------------------------------------------------------ host side
@@ -1110,13 +1339,14 @@ double sqrt_threshold(double r)
 
 enum KernelId {
     K_FILL_COUNTS, K_RESET, K_ASSIGN, K_SCAN_REDUCE, K_SCAN_TOP, K_SCAN_OBJECTS, K_SCATTER, K_FILL_UCOUNT,
-    K_LOOKUP, K_SCAN_UNITS, K_UNIT_SCATTER, K_JOIN, K_EVALUATE, K_COLLECT, K_SPLIT, K_MERGE, K_SCHEMA_DONE,
-    K_TICK_PARAMS, K_COUNT
+    K_LOOKUP, K_SCAN_UNITS, K_UNIT_SCATTER, K_JOIN, K_BUCKET_COUNT, K_SCAN_WRITE, K_BUCKET_SCATTER, K_BUCKET_FINE,
+    K_TABLE, K_COLLECT, K_SPLIT, K_MERGE, K_SCHEMA_DONE, K_TICK_PARAMS, K_COUNT
 };
 const char *const kKernelNames[K_COUNT] = {
     "k_fill_u32", "k_reset_tick", "k_assign", "k_scan_reduce", "k_scan_top", "k_scan_objects", "k_scatter",
-    "k_fill_ucount", "k_lookup", "k_scan_units", "k_unit_scatter", "k_join", "k_evaluate", "k_collect",
-    "k_split", "k_merge", "k_schema_done", "k_advance_tick"};
+    "k_fill_ucount", "k_lookup", "k_scan_units", "k_unit_scatter", "k_join", "k_bucket_count", "k_scan_write",
+    "k_bucket_scatter", "k_bucket_fine", "k_table", "k_collect", "k_split", "k_merge", "k_schema_done",
+    "k_advance_tick"};
 
 // Per-kernel HIP-event timing on the context's stream (optional).
 struct KernelProfiler {
@@ -1338,13 +1568,27 @@ void enqueue_lookup(glint_ctx *c)
 
 void enqueue_refine(glint_ctx *c)
 {
-    launch(c, K_JOIN, [&] { k_join<<<c->join_blocks, kBlock, 0, c->stream>>>(c->s); });
+    DevState &s = c->s;
+    hipStream_t st = c->stream;
+    launch(c, K_JOIN, [&] { k_join<<<c->join_blocks, kBlock, 0, st>>>(s); });
+    // group the tick's contacts by table region
+    const uint32_t G1 = s.bucket_blocks;
+    const uint32_t nc = (s.coarse_bins + 1) * G1, nb = (nc + kScanTile - 1) / kScanTile;
+    launch(c, K_BUCKET_COUNT, [&] { k_bucket_count<<<G1, kBlock, 0, st>>>(s); });
+    launch(c, K_SCAN_REDUCE, [&] { k_scan_reduce<<<nb, kBlock, 0, st>>>(s.bcount, nc, s.block_sums); });
+    launch(c, K_SCAN_TOP, [&] { k_scan_top<<<1, kBlock, 0, st>>>(s.block_sums, nb); });
+    launch(c, K_SCAN_WRITE, [&] { k_scan_write<<<nb, kBlock, 0, st>>>(s.bcount, nc, s.block_sums, s.boff); });
+    launch(c, K_BUCKET_SCATTER, [&] { k_bucket_scatter<<<G1, kBlock, 0, st>>>(s); });
+    if (s.lo_bits)
+        launch(c, K_BUCKET_FINE, [&] { k_bucket_fine<<<s.coarse_bins, kBlock, 0, st>>>(s); });
+    else
+        launch(c, K_BUCKET_FINE, [&] { k_bucket_single<<<grid_for(s.coarse_bins + 1), kBlock, 0, st>>>(s); });
 }
 
 void enqueue_evaluate(glint_ctx *c)
 {
-    launch(c, K_EVALUATE,
-           [&] { k_evaluate<<<grid_for(c->s.tab_mask + 1, 16384), kBlock, 0, c->stream>>>(c->s); });
+    const uint32_t nreg = 1u << c->s.region_bits;
+    launch(c, K_TABLE, [&] { k_table<<<std::min<uint32_t>(nreg, 4096), kBlock, 0, c->stream>>>(c->s); });
 }
 
 void enqueue_schema(glint_ctx *c)
@@ -1434,7 +1678,7 @@ glint_status glint_create(const glint_config *cfg, glint_ctx **out)
     g_create_error.clear();
     if (cfg->num_objects == 0 || cfg->grid_capacity == 0 || !(cfg->reach_distance > 0) ||
         !(cfg->high_x > cfg->low_x) || !(cfg->high_y > cfg->low_y) || cfg->candidate_mode > 1 ||
-        cfg->precision > 1 || cfg->num_objects >= 0xFFFFFFF0u) {
+        cfg->precision > 1 || cfg->num_objects >= 0x80000000u) {
         g_create_error = "invalid configuration";
         return GLINT_ERR_INVALID_ARGUMENT;
     }
@@ -1482,7 +1726,6 @@ glint_status glint_create(const glint_config *cfg, glint_ctx **out)
     s.complete = cfg->candidate_mode;
     s.precision = cfg->precision;
     s.delay = c->cfg.schema_update_delay;
-    s.record = cfg->record_contacts;
     s.lx = cfg->low_x;
     s.ly = cfg->low_y;
     s.hx = cfg->high_x;
@@ -1511,11 +1754,28 @@ glint_status glint_create(const glint_config *cfg, glint_ctx **out)
     s.node_cap = (uint32_t)node_cap;
     uint64_t slots = cfg->num_meeting_buckets ? next_pow2(cfg->num_meeting_buckets)
                                               : next_pow2(std::max<uint64_t>(4 * N, 1024));
+    if (slots < (uint64_t)kRegion) slots = kRegion;
+    if (slots > (1ull << 32)) {
+        c->err = "num_meeting_buckets above 2^32";
+        return fail(GLINT_ERR_INVALID_ARGUMENT);
+    }
     s.tab_mask = slots - 1;
+    {
+        uint32_t bits = 0;
+        while ((1ull << bits) < slots / kRegion) bits++;
+        s.region_bits = bits;
+        s.lo_bits = bits > 10 ? bits - 10 : 0;
+        s.coarse_bins = 1u << (bits - s.lo_bits);
+        s.bucket_blocks = 1024;
+    }
     s.unit_cap = cfg->unit_capacity ? cfg->unit_capacity : std::max<uint64_t>(4 * N, 1024);
     s.meet_cap = cfg->meeting_capacity ? cfg->meeting_capacity : std::max<uint64_t>(2 * N, 1u << 20);
-    s.rec_cap = cfg->record_contacts ? (cfg->contact_capacity ? cfg->contact_capacity : 8 * N + 1024) : 0;
+    // contacts per tick plus the unused tails of the join's per-wave chunks
+    s.clist_cap = (cfg->contact_capacity ? cfg->contact_capacity : 16 * N + (1u << 20)) +
+                  (uint64_t)c->join_blocks * (kBlock / 64) * kChunk;
     c->scan_blocks = (uint32_t)((node_cap + kScanTile - 1) / kScanTile);
+    const uint64_t nbc = (uint64_t)(s.coarse_bins + 1) * s.bucket_blocks;
+    const uint32_t sum_blocks = std::max<uint32_t>(c->scan_blocks, (uint32_t)((nbc + kScanTile - 1) / kScanTile));
 
     glint_status st = GLINT_OK;
 #define ALLOC(p, n)                          \
@@ -1528,7 +1788,12 @@ glint_status glint_create(const glint_config *cfg, glint_ctx **out)
     ALLOC(s.free_stack, node_cap);
     ALLOC(s.count, node_cap);
     ALLOC(s.offset, node_cap);
-    ALLOC(s.block_sums, c->scan_blocks);
+    ALLOC(s.block_sums, sum_blocks);
+    ALLOC(s.bcount, nbc);
+    ALLOC(s.boff, nbc);
+    ALLOC(s.bstart, (1ull << s.region_bits) + 1);
+    ALLOC(s.clist, s.clist_cap);
+    ALLOC(s.clist2, s.clist_cap);
     ALLOC(s.split_list, node_cap);
     ALLOC(s.merge_list, node_cap);
     ALLOC(s.leaf_list, node_cap);
@@ -1552,7 +1817,6 @@ glint_status glint_create(const glint_config *cfg, glint_ctx **out)
         ALLOC(s.fy, N);
     }
     ALLOC(s.meet, s.meet_cap);
-    if (s.rec_cap) ALLOC(s.rec, s.rec_cap);
     ALLOC(s.ctr, 1);
     ALLOC(c->tab[0], slots);
     ALLOC(c->tab[1], slots);
@@ -1830,18 +2094,21 @@ glint_status glint_contacts(glint_ctx *c, uint64_t *out, uint64_t cap, uint64_t 
 {
     if (!c || !n_out) return GLINT_ERR_INVALID_ARGUMENT;
     if (!c->cfg.record_contacts) {
-        c->err = "contacts are recorded only with record_contacts";
+        c->err = "contacts are kept only with record_contacts";
         return GLINT_ERR_STATE;
     }
     glint_status st = read_counters(c);
     if (st) return st;
-    uint64_t n = std::min<uint64_t>(c->h_ctr->n_rec, c->s.rec_cap);
+    if (c->h_ctr->error) return status_from_flags(c, c->h_ctr->error);
+    // the tick's bucketed contact list holds exactly n_contacts entries
+    const uint64_t n = std::min<uint64_t>(c->h_ctr->n_contacts, c->s.clist_cap);
+    const unsigned long long *list = c->s.lo_bits ? c->s.clist : c->s.clist2;
     std::vector<uint64_t> v(n);
-    if (n) HIP_TRY(c, hipMemcpy(v.data(), c->s.rec, n * 8, hipMemcpyDeviceToHost));
+    if (n) HIP_TRY(c, hipMemcpy(v.data(), list, n * 8, hipMemcpyDeviceToHost));
+    for (auto &k : v) k &= ~(1ull << 63);
     std::sort(v.begin(), v.end());
     v.erase(std::unique(v.begin(), v.end()), v.end());
     *n_out = v.size();
-    if (c->h_ctr->error) return status_from_flags(c, c->h_ctr->error);
     if (!out) return GLINT_OK;
     if (v.size() > cap) return GLINT_ERR_OUTPUT_CAPACITY;
     std::copy(v.begin(), v.end(), out);

@@ -63,7 +63,7 @@ struct Counters {
     unsigned long long n_inserted;
     unsigned long long n_meet;
     unsigned long long n_rechecks;
-    unsigned long long n_rec;
+    unsigned long long n_clist;     // contact-list entries reserved this tick (chunks)
     unsigned long long work_border;
     int32_t free_top;
     uint32_t n_split;
@@ -114,7 +114,6 @@ struct DevState {
     uint32_t complete;
     uint32_t precision;
     uint32_t delay;
-    uint32_t record;
     double lx, ly, hx, hy;
     double r;
     double t64;         // largest s with sqrt(s) <= r (host-computed)
@@ -124,7 +123,7 @@ struct DevState {
     unsigned long long tab_mask;
     unsigned long long unit_cap;
     unsigned long long meet_cap;
-    unsigned long long rec_cap;
+    unsigned long long clist_cap;
     TickParams *tp;
     Node *nodes;
     NodeMeta *meta;
@@ -151,7 +150,15 @@ struct DevState {
     float *fx;
     float *fy;
     glint_meeting *meet;
-    unsigned long long *rec;
+    unsigned long long *clist;     // the tick's contacts (bit 63: within reach at prev_tick)
+    unsigned long long *clist2;    // bucketing buffer
+    uint32_t *bcount;              // coarse-bin counts per bucketing block
+    uint32_t *boff;                // their exclusive offsets
+    uint32_t *bstart;              // first contact of every table region
+    uint32_t region_bits;          // log2(regions)
+    uint32_t lo_bits;              // region bits split by the second pass
+    uint32_t coarse_bins;          // bins of the first pass
+    uint32_t bucket_blocks;        // blocks of the first pass
     Counters *ctr;
 };
 
