@@ -122,6 +122,11 @@ def main() -> int:
     ap.add_argument("--cpu-ticks", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-kernel-times", action="store_true")
+    ap.add_argument("--graph-ticks", type=int, default=0,
+                    help="ticks per captured hipGraph in the timed region (disables per-kernel events)")
+    ap.add_argument("--compare-forms", action="store_true",
+                    help="also time the plain per-tick FP64 form (host positions, one call sequence per tick) "
+                         "against the batched guard-band form on the same inputs")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -160,10 +165,15 @@ def main() -> int:
         dw.walk(total_ticks - 1, trace, 16 * N)
     log(f"[bench] generated {total_ticks} ticks of positions on the device in {time.perf_counter() - t0:.1f} s")
 
-    cfg = runtime.make_config(N, w.box(), reach_distance=w.reach_distance, grid_capacity=w.grid_capacity,
-                              min_meet_time=w.min_meet_time, precision=precision, num_meeting_buckets=slots,
-                              unit_capacity=8 * N, meeting_capacity=max(1 << 20, 6 * N),
-                              contact_capacity=int(est_contacts * 1.4))
+    def make_cfg(prec: int, graph: int):
+        return runtime.make_config(N, w.box(), reach_distance=w.reach_distance, grid_capacity=w.grid_capacity,
+                                   min_meet_time=w.min_meet_time, precision=prec, num_meeting_buckets=slots,
+                                   unit_capacity=8 * N, meeting_capacity=max(1 << 20, 6 * N),
+                                   contact_capacity=int(est_contacts * 1.4), graph_ticks=graph)
+
+    if args.graph_ticks:
+        args.no_kernel_times = True
+    cfg = make_cfg(precision, args.graph_ticks)
     wb = runtime.Workbench(cfg)
     t0 = time.perf_counter()
     if W:
@@ -204,6 +214,45 @@ def main() -> int:
                     "kernel_ms_per_launch": round(dms / dcnt, 4), "algorithmic_bytes_per_launch": abytes,
                     "kernel_share_of_tick": round(dms / (elapsed * 1e3), 4)}
 
+    forms = None
+    if args.compare_forms:
+        wb.close()
+        forms = {}
+        # plain form: positions copied from the host every tick, one call sequence per tick, FP64
+        host = np.zeros(2 * N, dtype=np.float64)
+        wp = runtime.Workbench(make_cfg(runtime.PRECISION_FP64, 0))
+        for t in range(W):
+            trace.download(host, 16 * N * t)
+            wp.tick(host, t)
+        wp.synchronize()
+        lat = []
+        t0 = time.perf_counter()
+        for t in range(W, W + K):
+            t1 = time.perf_counter()
+            trace.download(host, 16 * N * t)       # the caller's positions arrive from the host
+            wp.set_points(host, t)
+            wp.filter()
+            wp.reachability()
+            wp.update_meetings(copy_out=False)
+            wp.update_schema()
+            wp.synchronize()
+            lat.append((time.perf_counter() - t1) * 1e3)
+        forms["plain_fp64_per_tick_ms"] = round((time.perf_counter() - t0) * 1e3 / K, 3)
+        forms["plain_p99_ms"] = round(float(np.percentile(lat, 99)), 3)
+        wp.close()
+        # batched form: device-resident trace, 8 ticks per graph, FP32 guard band
+        wq = runtime.Workbench(make_cfg(runtime.PRECISION_FP32_GUARD, 8))
+        wq.run_ticks(trace.address(0), 0, W, latencies=False)
+        wq.synchronize()
+        t0 = time.perf_counter()
+        ms = wq.run_ticks(trace.address(16 * N * W), W, K, latencies=True)
+        wq.synchronize()
+        forms["batched_guard_graph8_ms"] = round((time.perf_counter() - t0) * 1e3 / K, 3)
+        forms["batched_p99_ms"] = round(float(np.percentile(ms, 99)), 3)
+        forms["speedup"] = round(forms["plain_fp64_per_tick_ms"] / forms["batched_guard_graph8_ms"], 3)
+        wq.close()
+        log(f"[bench] forms {forms}")
+
     cpu = None
     if not args.no_cpu_baseline:
         threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
@@ -237,8 +286,11 @@ def main() -> int:
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    if forms:
+        out["forms"] = forms
     print(json.dumps(out), flush=True)
-    wb.close()
+    if not args.compare_forms:
+        wb.close()
     trace.free()
     return 0
 

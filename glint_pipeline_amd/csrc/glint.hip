@@ -116,15 +116,19 @@ __global__ void k_set_tick(TickParams *tp, const double *xy, Slot *tab_old, Slot
 }
 
 // Next tick of a device-resident trace: positions move one snapshot on and
-// the generations swap (this tick's table and positions become the previous).
+// the generations swap (this tick's table and positions become the previous
+// ones).  Every tick of a run starts with it, so a run of ticks is a fixed
+// sequence of launches that can be captured in a graph and replayed.
 __global__ void k_advance_tick(TickParams *tp, uint64_t stride)
 {
     uint32_t k = tp->trace_index + 1;
     tp->trace_index = k;
     tp->xy = tp->trace_base + stride * k;
     tp->prev_tick = tp->tick;
-    tp->tick = tp->tick + 1;
-    tp->has_prev = 1;
+    tp->tick = tp->next_tick;
+    tp->next_tick = tp->next_tick + 1;
+    tp->has_prev = tp->has_prev_next;
+    tp->has_prev_next = 1;
     Slot *t = tp->tab_old;
     tp->tab_old = tp->tab_cur;
     tp->tab_cur = t;
@@ -133,19 +137,31 @@ __global__ void k_advance_tick(TickParams *tp, uint64_t stride)
     tp->xy_save = h;
 }
 
-__global__ void k_begin_trace(TickParams *tp, const double *base, Slot *tab_old, Slot *tab_cur, double *xy_prev,
-                              double *xy_save, uint32_t tick, uint32_t prev_tick, uint32_t has_prev)
+// State before the first tick of a run, one k_advance_tick short of it; the
+// positions already point at the first snapshot for the initial build.
+__global__ void k_begin_run(TickParams *tp, const double *base, Slot *tab_old, Slot *tab_cur, double *xy_prev,
+                            double *xy_save, uint32_t tick, uint32_t next_tick, uint32_t has_prev_next,
+                            unsigned long long *stamps, uint32_t nstamps)
 {
     tp->trace_base = base;
-    tp->trace_index = 0;
+    tp->trace_index = 0xFFFFFFFFu;
     tp->xy = base;
     tp->tab_old = tab_old;
     tp->tab_cur = tab_cur;
     tp->xy_prev = xy_prev;
     tp->xy_save = xy_save;
     tp->tick = tick;
-    tp->prev_tick = prev_tick;
-    tp->has_prev = has_prev;
+    tp->next_tick = next_tick;
+    tp->has_prev_next = has_prev_next;
+    tp->stamps = stamps;
+    tp->nstamps = nstamps;
+    if (stamps && nstamps) stamps[0] = wall_clock64();
+}
+
+__global__ void k_stamp(TickParams *tp)
+{
+    uint32_t k = tp->trace_index + 1;
+    if (tp->stamps && k < tp->nstamps) tp->stamps[k] = wall_clock64();
 }
 
 __global__ void k_reset_tick(DevState s)
@@ -1420,6 +1436,10 @@ struct glint_ctx {
     hipEvent_t ev[6] = {};
     glint_tick_stats last{};
     KernelProfiler prof;
+    hipGraphExec_t graph_exec = nullptr;
+    unsigned long long *stamps = nullptr;
+    uint32_t stamp_cap = 0;
+    double wall_khz = 0;
 };
 
 namespace {
@@ -1701,6 +1721,11 @@ glint_status glint_create(const glint_config *cfg, glint_ctx **out)
         c->err = std::string("hipSetDevice: ") + hipGetErrorString(e);
         return fail(GLINT_ERR_HIP);
     }
+    {
+        int khz = 0;
+        if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c->device) == hipSuccess && khz > 0)
+            c->wall_khz = khz;
+    }
     e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         c->err = std::string("hipStreamCreate: ") + hipGetErrorString(e);
@@ -1847,6 +1872,8 @@ glint_status glint_destroy(glint_ctx *c)
     for (auto &ev : c->ev)
         if (ev) (void)hipEventDestroy(ev);
     c->prof.clear();
+    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+    if (c->stamps) (void)hipFree(c->stamps);
     if (c->h_ctr) (void)hipHostFree(c->h_ctr);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -2003,6 +2030,17 @@ glint_status glint_process_tick(glint_ctx *c, const double *xy, uint32_t tick)
     return glint_update_schema(c);
 }
 
+static void enqueue_tick_body(glint_ctx *c, uint64_t stride)
+{
+    launch(c, K_TICK_PARAMS, [&] { k_advance_tick<<<1, 1, 0, c->stream>>>(c->s.tp, stride); });
+    enqueue_partition(c);
+    enqueue_lookup(c);
+    enqueue_refine(c);
+    enqueue_evaluate(c);
+    enqueue_schema(c);
+    k_stamp<<<1, 1, 0, c->stream>>>(c->s.tp);
+}
+
 glint_status glint_run_ticks(glint_ctx *c, const double *d_trace, uint32_t first_tick, uint32_t nticks,
                              float *tick_ms)
 {
@@ -2017,45 +2055,54 @@ glint_status glint_run_ticks(glint_ctx *c, const double *d_trace, uint32_t first
     }
     HIP_TRY(c, hipSetDevice(c->device));
     const uint64_t stride = 2ull * c->s.n;
-    std::vector<hipEvent_t> evs;
-    if (tick_ms) {
-        evs.resize(nticks + 1);
-        for (auto &e : evs) HIP_TRY(c, hipEventCreate(&e));
+    if (c->stamp_cap < nticks + 1) {
+        if (c->stamps) HIP_TRY(c, hipFree(c->stamps));
+        c->stamps = nullptr;
+        c->stamp_cap = 0;
+        HIP_TRY(c, hipMalloc((void **)&c->stamps, (size_t)(nticks + 1) * sizeof(unsigned long long)));
+        c->stamp_cap = nticks + 1;
     }
-    glint_status st = GLINT_OK;
     c->tick = first_tick;
     const int g = c->cur_gen;
-    k_begin_trace<<<1, 1, 0, c->stream>>>(c->s.tp, d_trace, c->tab[1 - g], c->tab[g], c->hist[1 - g], c->hist[g],
-                                          first_tick, c->has_prev ? c->prev_tick : first_tick,
-                                          c->has_prev ? 1u : 0u);
+    // one advance short of the first tick: tables and histories pre-swapped
+    k_begin_run<<<1, 1, 0, c->stream>>>(c->s.tp, d_trace, c->tab[g], c->tab[1 - g], c->hist[g], c->hist[1 - g],
+                                        c->prev_tick, first_tick, c->has_prev ? 1u : 0u, c->stamps, nticks + 1);
+    HIP_TRY(c, hipGetLastError());
     if (!c->built) {
-        st = build_tree(c);
+        glint_status st = build_tree(c);
         if (st) return st;
     }
-    if (tick_ms) HIP_TRY(c, hipEventRecord(evs[0], c->stream));
-    for (uint32_t k = 0; k < nticks; k++) {
-        if (k > 0) {
-            launch(c, K_TICK_PARAMS, [&] { k_advance_tick<<<1, 1, 0, c->stream>>>(c->s.tp, stride); });
+    const uint32_t G = c->cfg.graph_ticks;
+    uint32_t done = 0;
+    if (G && !c->prof.on && nticks >= G) {
+        if (!c->graph_exec) {
+            // G consecutive ticks captured once; every tick reads its parameters
+            // from device memory, so the graph is replayed unchanged
+            hipGraph_t graph = nullptr;
+            HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            for (uint32_t k = 0; k < G; k++) enqueue_tick_body(c, stride);
+            HIP_TRY(c, hipStreamEndCapture(c->stream, &graph));
+            hipError_t e = hipGraphInstantiate(&c->graph_exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (e != hipSuccess) return hip_fail(c, e, "hipGraphInstantiate");
         }
-        enqueue_partition(c);
-        enqueue_lookup(c);
-        enqueue_refine(c);
-        enqueue_evaluate(c);
-        enqueue_schema(c);
-        if (tick_ms) HIP_TRY(c, hipEventRecord(evs[k + 1], c->stream));
-        c->cur_gen = 1 - c->cur_gen;
-        c->prev_tick = first_tick + k;
-        c->has_prev = true;
+        for (; done + G <= nticks; done += G) HIP_TRY(c, hipGraphLaunch(c->graph_exec, c->stream));
     }
+    for (; done < nticks; done++) enqueue_tick_body(c, stride);
     HIP_TRY(c, hipGetLastError());
+    for (uint32_t k = 0; k < nticks; k++) c->cur_gen = 1 - c->cur_gen;
+    c->prev_tick = first_tick + nticks - 1;
+    c->has_prev = true;
     c->tick = first_tick + nticks - 1;
     c->has_positions = true;
     c->partitioned = c->refined = c->evaluated = true;
-    st = read_counters(c);
+    glint_status st = read_counters(c);
     if (st) return st;
     if (tick_ms) {
-        for (uint32_t k = 0; k < nticks; k++) HIP_TRY(c, hipEventElapsedTime(&tick_ms[k], evs[k], evs[k + 1]));
-        for (auto &e : evs) (void)hipEventDestroy(e);
+        std::vector<unsigned long long> h(nticks + 1);
+        HIP_TRY(c, hipMemcpy(h.data(), c->stamps, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+        const double khz = c->wall_khz > 0 ? c->wall_khz : 100000.0;
+        for (uint32_t k = 0; k < nticks; k++) tick_ms[k] = (float)((double)(h[k + 1] - h[k]) / khz);
     }
     fill_stats(c);
     return status_from_flags(c, c->h_ctr->error);

@@ -100,10 +100,14 @@ struct TickParams {
     Slot *tab_cur;              // pairs seen at tick (filled by refine)
     double *xy_prev;            // positions at prev_tick (pid order), valid if has_prev
     double *xy_save;            // receives this tick's positions for the next tick
+    unsigned long long *stamps; // run_ticks: device wall clock at the end of every tick
+    uint32_t nstamps;
     uint32_t tick;
     uint32_t prev_tick;
     uint32_t trace_index;
     uint32_t has_prev;
+    uint32_t next_tick;         // tick number given by the next k_advance_tick
+    uint32_t has_prev_next;
 };
 
 struct DevState {

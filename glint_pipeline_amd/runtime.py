@@ -66,6 +66,8 @@ class Config(ctypes.Structure):
         ("contact_capacity", ctypes.c_uint64),
         ("node_capacity", ctypes.c_uint32),
         ("record_contacts", ctypes.c_uint32),
+        ("graph_ticks", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
     ]
 
 
@@ -205,7 +207,8 @@ def make_config(num_objects: int, box, *, reach_distance: float = 2.0, grid_capa
                 start_time: int = 0, complete: bool = False, precision: int = PRECISION_FP64,
                 guard_epsilon: float = 1e-3, num_meeting_buckets: int = 0, unit_capacity: int = 0,
                 meeting_capacity: int = 0, node_capacity: int = 0, record_contacts: bool = False,
-                contact_capacity: int = 0, device: int = 0, max_meet_time: int = 4096) -> Config:
+                contact_capacity: int = 0, device: int = 0, max_meet_time: int = 4096,
+                graph_ticks: int = 0) -> Config:
     c = Config()
     lib().glint_config_default(ctypes.byref(c))
     c.num_objects = num_objects
@@ -227,6 +230,7 @@ def make_config(num_objects: int, box, *, reach_distance: float = 2.0, grid_capa
     c.record_contacts = int(record_contacts)
     c.contact_capacity = contact_capacity
     c.device = device
+    c.graph_ticks = graph_ticks
     return c
 
 

@@ -82,6 +82,8 @@ typedef struct glint_config {
     uint64_t contact_capacity;     /* recorded contacts per tick (record_contacts) */
     uint32_t node_capacity;        /* quadtree nodes; 0: automatic */
     uint32_t record_contacts;      /* keep each tick's contact pairs for glint_contacts() */
+    uint32_t graph_ticks;          /* glint_run_ticks: ticks per captured hipGraph (0: plain launches) */
+    uint32_t reserved;
 } glint_config;
 
 /* One emitted meeting: pid_a < pid_b, first and last tick of contact. */
@@ -168,9 +170,11 @@ glint_status glint_process_tick(glint_ctx *ctx, const double *xy, uint32_t tick)
 
 /* Runs `nticks` consecutive ticks from a device-resident trace laid out
  * tick-major (tick k at d_trace + 2*N*k), with no host synchronisation
- * between ticks; meetings stay on the device.  If tick_ms is not NULL it
- * receives each tick's HIP-event latency in milliseconds.  The reference
- * drives the same loop from the host, one tick per call (process.cpp:431). */
+ * between ticks; meetings stay on the device.  With cfg.graph_ticks = G the
+ * ticks run as replays of one captured hipGraph of G ticks.  If tick_ms is
+ * not NULL it receives each tick's latency in milliseconds from the device
+ * wall clock.  The reference drives the same loop from the host, one tick
+ * per call (process.cpp:431). */
 glint_status glint_run_ticks(glint_ctx *ctx, const double *d_trace, uint32_t first_tick,
                              uint32_t nticks, float *tick_ms);
 

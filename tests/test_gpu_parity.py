@@ -80,9 +80,11 @@ def test_guard_band_rechecks_only_inside_band():
     assert tot["rechecks"] < tot["contacts"]
 
 
-def test_run_ticks_device_trace_matches_oracle():
+@pytest.mark.parametrize("graph_ticks", [0, 4])
+def test_run_ticks_device_trace_matches_oracle(graph_ticks):
     """Many ticks from a device-resident trace in one call (no host sync per
-    tick) give the same active pairs and meeting totals as the oracle."""
+    tick; with graph_ticks, replays of one captured hipGraph) give the same
+    active pairs and meeting totals as the oracle."""
     from oracle.oracle import Oracle
     w = gen.uniform_city(20_000, 40, 1000.0, seed=51, grid_capacity=32)
     g = gen.Generator(w)
@@ -95,7 +97,7 @@ def test_run_ticks_device_trace_matches_oracle():
     for t in range(w.ticks):
         o.tick(g.positions() if t == 0 else g.step(), t)
         om += len(o.meetings())
-    cfg = runtime.make_config(w.num_objects, w.box(), grid_capacity=w.grid_capacity)
+    cfg = runtime.make_config(w.num_objects, w.box(), grid_capacity=w.grid_capacity, graph_ticks=graph_ticks)
     with runtime.Workbench(cfg) as wb:
         ms = wb.run_ticks(trace.address(0), 0, 15)
         ms2 = wb.run_ticks(trace.address(16 * w.num_objects * 15), 15, w.ticks - 15)
