@@ -20,8 +20,8 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 INCLUDE = ROOT / "include"
 
-HIP_SOURCES = ["glint.hip", "walk.hip"]
-HIP_HEADERS = ["glint_device.h"]
+HIP_SOURCES = ["glint.hip", "walk.hip", "kv.hip"]
+HIP_HEADERS = ["glint_device.h", "kv_engine.h"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("GLINT_OFFLOAD_ARCH", "gfx950")
@@ -48,7 +48,7 @@ def build_hip() -> Path:
     _run(["gcc", "-O3", "-fPIC", "-c", "-std=c11", "-ffp-contract=off", "-fno-fast-math", "-fopenmp",
           f"-I{INCLUDE}", "-o", str(gen_obj), str(CSRC / "gen.c")])
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
-           "-ffp-contract=off", "-fno-fast-math",
+           "-ffp-contract=off", "-fno-fast-math", "-fhip-fp32-correctly-rounded-divide-sqrt",
            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
            "-Wall", "-Wno-unused-function", f"-I{INCLUDE}", f"-I{CSRC}",
            "-o", str(tmp)]

@@ -23,10 +23,12 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "glint.h"
 #include "glint_device.h"
+#include "kv_engine.h"
 
 using namespace glint;
 
@@ -102,12 +104,15 @@ __global__ void k_fill_slots(Slot *p, uint64_t n)
     }
 }
 
-__global__ void k_set_tick(TickParams *tp, const double *xy, Slot *tab_old, Slot *tab_cur, double *xy_prev,
-                           double *xy_save, uint32_t tick, uint32_t prev_tick, uint32_t has_prev)
+__global__ void k_set_tick(TickParams *tp, const double *xy, Slot *tab_old, Slot *tab_cur, float4 *mbr_old,
+                           float4 *mbr_cur, double *xy_prev, double *xy_save, uint32_t tick, uint32_t prev_tick,
+                           uint32_t has_prev)
 {
     tp->xy = xy;
     tp->tab_old = tab_old;
     tp->tab_cur = tab_cur;
+    tp->mbr_old = mbr_old;
+    tp->mbr_cur = mbr_cur;
     tp->xy_prev = xy_prev;
     tp->xy_save = xy_save;
     tp->tick = tick;
@@ -132,6 +137,9 @@ __global__ void k_advance_tick(TickParams *tp, uint64_t stride)
     Slot *t = tp->tab_old;
     tp->tab_old = tp->tab_cur;
     tp->tab_cur = t;
+    float4 *m = tp->mbr_old;
+    tp->mbr_old = tp->mbr_cur;
+    tp->mbr_cur = m;
     double *h = tp->xy_prev;
     tp->xy_prev = tp->xy_save;
     tp->xy_save = h;
@@ -139,15 +147,17 @@ __global__ void k_advance_tick(TickParams *tp, uint64_t stride)
 
 // State before the first tick of a run, one k_advance_tick short of it; the
 // positions already point at the first snapshot for the initial build.
-__global__ void k_begin_run(TickParams *tp, const double *base, Slot *tab_old, Slot *tab_cur, double *xy_prev,
-                            double *xy_save, uint32_t tick, uint32_t next_tick, uint32_t has_prev_next,
-                            unsigned long long *stamps, uint32_t nstamps)
+__global__ void k_begin_run(TickParams *tp, const double *base, Slot *tab_old, Slot *tab_cur, float4 *mbr_old,
+                            float4 *mbr_cur, double *xy_prev, double *xy_save, uint32_t tick, uint32_t next_tick,
+                            uint32_t has_prev_next, unsigned long long *stamps, uint32_t nstamps)
 {
     tp->trace_base = base;
     tp->trace_index = 0xFFFFFFFFu;
     tp->xy = base;
     tp->tab_old = tab_old;
     tp->tab_cur = tab_cur;
+    tp->mbr_old = mbr_old;
+    tp->mbr_cur = mbr_cur;
     tp->xy_prev = xy_prev;
     tp->xy_save = xy_save;
     tp->tick = tick;
@@ -1197,9 +1207,20 @@ struct TableLds {
     glint_meeting stage[kBlock / 64][kMeetStage];
 };
 
+// With the key-value output on, every pair also carries the float box of
+// both objects over its run (geometry.cu:469-494): the box of a new pair is
+// its two points at this tick, a continuing pair adds them to its old box.
+// min/max of the rounded coordinates equals the rounded min/max, so the box
+// does not depend on the order of the updates.
+struct TableLdsKv : TableLds {
+    float4 oldm[kRegion];
+    float4 newm[kRegion];
+};
+
+template <bool KV>
 __global__ __launch_bounds__(kBlock) void k_table(DevState s)
 {
-    __shared__ TableLds L;
+    __shared__ typename std::conditional<KV, TableLdsKv, TableLds>::type L;
     const Slot *tab_old = s.tp->tab_old;
     Slot *tab_cur = s.tp->tab_cur;
     const uint32_t tick = s.tp->tick, last = s.tp->prev_tick;
@@ -1226,6 +1247,7 @@ __global__ __launch_bounds__(kBlock) void k_table(DevState s)
             L.olds[k] = pre[q].start;
             L.seen[k] = 0;
             L.newk[k] = kEmptyKey;
+            if constexpr (KV) L.oldm[k] = s.tp->mbr_old[base + k];
         }
         const uint32_t rn = r + gridDim.x;
         if (rn < nreg) {
@@ -1239,6 +1261,7 @@ __global__ __launch_bounds__(kBlock) void k_table(DevState s)
             const unsigned long long key = tagged & ~(1ull << 63);
             const uint32_t h = (uint32_t)mix64(key) & (kRegion - 1);
             uint32_t start = tick;
+            int32_t oq = -1;
             if (tagged >> 63) {   // within reach at prev_tick: it may continue a run
                 for (uint32_t p = 0; p < kRegion; p++) {
                     const uint32_t q = (h + p) & (kRegion - 1);
@@ -1246,6 +1269,7 @@ __global__ __launch_bounds__(kBlock) void k_table(DevState s)
                     if (k == key) {
                         start = L.olds[q];
                         L.seen[q] = 1;
+                        oq = (int32_t)q;
                         break;
                     }
                     if (k == kEmptyKey) break;
@@ -1257,6 +1281,18 @@ __global__ __launch_bounds__(kBlock) void k_table(DevState s)
                 const unsigned long long prev = atomicCAS(&L.newk[q], kEmptyKey, key);
                 if (prev == kEmptyKey) {
                     L.news[q] = start;
+                    if constexpr (KV) {
+                        const double *xy = s.tp->xy;
+                        const size_t a = (size_t)(key >> 32), b = (size_t)(uint32_t)key;
+                        const float xa = (float)xy[2 * a], ya = (float)xy[2 * a + 1];
+                        const float xb = (float)xy[2 * b], yb = (float)xy[2 * b + 1];
+                        float4 m = make_float4(fminf(xa, xb), fminf(ya, yb), fmaxf(xa, xb), fmaxf(ya, yb));
+                        if (oq >= 0) {
+                            const float4 o = L.oldm[oq];
+                            m = make_float4(fminf(m.x, o.x), fminf(m.y, o.y), fmaxf(m.z, o.z), fmaxf(m.w, o.w));
+                        }
+                        L.newm[q] = m;
+                    }
                     inserted++;
                     placed = true;
                     break;
@@ -1283,6 +1319,32 @@ __global__ __launch_bounds__(kBlock) void k_table(DevState s)
                 st[nst + __popcll(m & ((1ull << lane) - 1ull))] = mt;
             }
             nst += (uint32_t)__popcll(m);
+            if constexpr (KV) {
+                // meetings whose box lies in the map box are logged
+                // (geometry.cu:665-670, box_contain geometry.cu:53-58)
+                const float4 bx = L.oldm[k];
+                const bool put = emit && bx.x >= s.lx && bx.z <= s.hx && bx.y >= s.ly && bx.w <= s.hy;
+                const unsigned long long mk = __ballot(put);
+                if (mk) {
+                    unsigned long long b = 0;
+                    if (lane == 0) b = atomicAdd(&s.ctr->n_kvlog, (unsigned long long)__popcll(mk));
+                    b = __shfl(b, 0);
+                    if (put) {
+                        const unsigned long long j = b + __popcll(mk & ((1ull << lane) - 1ull));
+                        if (j < s.kvlog_cap) {
+                            KvLogEntry e;
+                            e.pid_a = (uint32_t)(ok >> 32);
+                            e.pid_b = (uint32_t)ok;
+                            e.start = L.olds[k];
+                            e.end = last;
+                            e.box = bx;
+                            s.kvlog[j] = e;
+                        } else {
+                            atomicOr(&s.ctr->error, kErrKvCap);
+                        }
+                    }
+                }
+            }
             if (nst > kMeetStage - 64) {
                 wave_sync();
                 unsigned long long b = 0;
@@ -1302,6 +1364,7 @@ __global__ __launch_bounds__(kBlock) void k_table(DevState s)
             ns.start = L.news[k];
             ns.seen = 0;
             tab_cur[base + k] = ns;
+            if constexpr (KV) s.tp->mbr_cur[base + k] = L.newm[k];
         }
         __syncthreads();
     }
@@ -1416,6 +1479,8 @@ struct glint_ctx {
     hipStream_t stream = nullptr;
     DevState s{};
     Slot *tab[2] = {nullptr, nullptr};
+    float4 *mbr[2] = {nullptr, nullptr};    // pair boxes of each generation (key-value output)
+    KvEngine *kv = nullptr;                 // sorted key-value output (null when off)
     double *hist[2] = {nullptr, nullptr};   // positions of the last two ticks (generation parity)
     int cur_gen = 0;            // table generation filled at the current tick
     bool built = false;
@@ -1501,6 +1566,10 @@ glint_status status_from_flags(glint_ctx *c, uint32_t f)
         c->err = "border check units exceed unit_capacity";
         return GLINT_ERR_UNIT_CAPACITY;
     }
+    if (f & kErrKvCap) {
+        c->err = "key-value log exceeds kv_capacity (flush more often or raise kv_capacity)";
+        return GLINT_ERR_OUTPUT_CAPACITY;
+    }
     c->err = "meeting or contact output exceeds its capacity";
     return GLINT_ERR_OUTPUT_CAPACITY;
 }
@@ -1515,8 +1584,9 @@ glint_status read_counters(glint_ctx *c)
 void set_tick_params(glint_ctx *c, const double *xy)
 {
     const int g = c->cur_gen;
-    k_set_tick<<<1, 1, 0, c->stream>>>(c->s.tp, xy, c->tab[1 - g], c->tab[g], c->hist[1 - g], c->hist[g], c->tick,
-                                       c->has_prev ? c->prev_tick : c->tick, c->has_prev ? 1u : 0u);
+    k_set_tick<<<1, 1, 0, c->stream>>>(c->s.tp, xy, c->tab[1 - g], c->tab[g], c->mbr[1 - g], c->mbr[g],
+                                       c->hist[1 - g], c->hist[g], c->tick, c->has_prev ? c->prev_tick : c->tick,
+                                       c->has_prev ? 1u : 0u);
 }
 
 // Launch through the profiler: HIP events around the kernel when enabled.
@@ -1608,7 +1678,11 @@ void enqueue_refine(glint_ctx *c)
 void enqueue_evaluate(glint_ctx *c)
 {
     const uint32_t nreg = 1u << c->s.region_bits;
-    launch(c, K_TABLE, [&] { k_table<<<std::min<uint32_t>(nreg, 4096), kBlock, 0, c->stream>>>(c->s); });
+    const uint32_t g = std::min<uint32_t>(nreg, 4096);
+    if (c->kv)
+        launch(c, K_TABLE, [&] { k_table<true><<<g, kBlock, 0, c->stream>>>(c->s); });
+    else
+        launch(c, K_TABLE, [&] { k_table<false><<<g, kBlock, 0, c->stream>>>(c->s); });
 }
 
 void enqueue_schema(glint_ctx *c)
@@ -1689,6 +1763,8 @@ void glint_config_default(glint_config *cfg)
     cfg->dynamic_schema = 1;
     cfg->reach_distance = 2.0;
     cfg->guard_epsilon = 1e-3;
+    cfg->kv_split_num = 10;        // config.h:60
+    cfg->kv_area_limit = 100.0;    // the "not cut" entry of process.cpp:408-410
 }
 
 glint_status glint_create(const glint_config *cfg, glint_ctx **out)
@@ -1846,6 +1922,24 @@ glint_status glint_create(const glint_config *cfg, glint_ctx **out)
     ALLOC(c->tab[0], slots);
     ALLOC(c->tab[1], slots);
     ALLOC(c->own_xy, 2 * N);
+    if (cfg->kv_capacity) {
+        ALLOC(c->mbr[0], slots);
+        ALLOC(c->mbr[1], slots);
+        KvParams kp{};
+        kp.num_objects = (uint32_t)N;
+        kp.split_num = cfg->kv_split_num;
+        kp.max_meet_time = cfg->max_meet_time;
+        kp.first_end_time_min = cfg->start_time + cfg->min_meet_time;   // process.cpp:404
+        kp.area_limit = cfg->kv_area_limit;
+        kp.map[0] = cfg->low_x;
+        kp.map[1] = cfg->low_y;
+        kp.map[2] = cfg->high_x;
+        kp.map[3] = cfg->high_y;
+        kp.key_capacity = cfg->kv_capacity;
+        if ((st = kv_create(kp, c->stream, &c->kv, c->err))) return fail(st);
+        s.kvlog = kv_log(c->kv);
+        s.kvlog_cap = kv_log_capacity(c->kv);
+    }
 #undef ALLOC
     e = hipMemsetAsync(s.ctr, 0, sizeof(Counters), c->stream);
     if (e != hipSuccess) {
@@ -1874,6 +1968,7 @@ glint_status glint_destroy(glint_ctx *c)
     c->prof.clear();
     if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
     if (c->stamps) (void)hipFree(c->stamps);
+    kv_destroy(c->kv);
     if (c->h_ctr) (void)hipHostFree(c->h_ctr);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -2065,7 +2160,8 @@ glint_status glint_run_ticks(glint_ctx *c, const double *d_trace, uint32_t first
     c->tick = first_tick;
     const int g = c->cur_gen;
     // one advance short of the first tick: tables and histories pre-swapped
-    k_begin_run<<<1, 1, 0, c->stream>>>(c->s.tp, d_trace, c->tab[g], c->tab[1 - g], c->hist[g], c->hist[1 - g],
+    k_begin_run<<<1, 1, 0, c->stream>>>(c->s.tp, d_trace, c->tab[g], c->tab[1 - g], c->mbr[g], c->mbr[1 - g],
+                                        c->hist[g], c->hist[1 - g],
                                         c->prev_tick, first_tick, c->has_prev ? 1u : 0u, c->stamps, nticks + 1);
     HIP_TRY(c, hipGetLastError());
     if (!c->built) {
@@ -2289,6 +2385,65 @@ glint_status glint_totals_get(glint_ctx *c, glint_totals *out, int reset)
         HIP_TRY(c, hipStreamSynchronize(c->stream));
     }
     return status_from_flags(c, h.error);
+}
+
+glint_status glint_kv_pending(glint_ctx *c, uint64_t *keys)
+{
+    if (!c || !keys) return GLINT_ERR_INVALID_ARGUMENT;
+    if (!c->kv) {
+        c->err = "key-value output is off (kv_capacity = 0)";
+        return GLINT_ERR_STATE;
+    }
+    HIP_TRY(c, hipSetDevice(c->device));
+    glint_status st = read_counters(c);
+    if (st) return st;
+    *keys = 2 * std::min<uint64_t>(c->h_ctr->n_kvlog, kv_log_capacity(c->kv));
+    return status_from_flags(c, c->h_ctr->error);
+}
+
+glint_status glint_kv_flush(glint_ctx *c, glint_kv_summary *out)
+{
+    if (!c) return GLINT_ERR_INVALID_ARGUMENT;
+    if (!c->kv) {
+        c->err = "key-value output is off (kv_capacity = 0)";
+        return GLINT_ERR_STATE;
+    }
+    if (!c->has_prev || (c->has_positions && !c->evaluated)) {
+        c->err = "flush needs an evaluated tick";
+        return GLINT_ERR_STATE;
+    }
+    HIP_TRY(c, hipSetDevice(c->device));
+    glint_status st = read_counters(c);
+    if (st) return st;
+    if (c->h_ctr->error) return status_from_flags(c, c->h_ctr->error);
+    st = kv_flush(c->kv, c->h_ctr->n_kvlog, c->prev_tick, out, c->err);
+    if (st) return st;
+    // the log starts over (the flush runs outside any captured graph)
+    HIP_TRY(c, hipMemsetAsync(&c->s.ctr->n_kvlog, 0, sizeof(unsigned long long), c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return GLINT_OK;
+}
+
+glint_status glint_kv_part_size(glint_ctx *c, uint32_t part, uint64_t *bytes)
+{
+    if (!c || !bytes || part > GLINT_KV_PACKED) return GLINT_ERR_INVALID_ARGUMENT;
+    if (!c->kv) {
+        c->err = "key-value output is off (kv_capacity = 0)";
+        return GLINT_ERR_STATE;
+    }
+    *bytes = kv_part_bytes(c->kv, part);
+    return GLINT_OK;
+}
+
+glint_status glint_kv_copy(glint_ctx *c, uint32_t part, void *dst, uint64_t bytes)
+{
+    if (!c) return GLINT_ERR_INVALID_ARGUMENT;
+    if (!c->kv) {
+        c->err = "key-value output is off (kv_capacity = 0)";
+        return GLINT_ERR_STATE;
+    }
+    HIP_TRY(c, hipSetDevice(c->device));
+    return kv_copy(c->kv, part, dst, bytes, c->err);
 }
 
 glint_status glint_debug_inject_failure(glint_ctx *c, const char *stage)

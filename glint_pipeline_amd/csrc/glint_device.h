@@ -33,6 +33,7 @@ enum ErrorBits : uint32_t {
     kErrMeetCap = 1u << 4,
     kErrContactCap = 1u << 5,
     kErrStack = 1u << 6,
+    kErrKvCap = 1u << 7,
 };
 
 struct alignas(64) Node {
@@ -89,16 +90,27 @@ struct Counters {
     unsigned long long max_contacts;
     unsigned long long max_distinct;
     unsigned long long max_meet;
+    unsigned long long n_kvlog;     // meetings logged for the key-value output since its last flush
 };
 
 // Per-tick values, kept in device memory so a tick's launches can be
 // replayed from a graph; advanced on the device by k_advance_tick.
+// One ended meeting kept for the sorted key-value output: the pair, its
+// first and last contact tick and the float box of both objects over the
+// run (low_x, low_y, high_x, high_y).
+struct KvLogEntry {
+    uint32_t pid_a, pid_b, start, end;
+    float4 box;
+};
+
 struct TickParams {
     const double *xy;
     const double *trace_base;   // run_ticks: tick k of the trace at trace_base + 2*N*k
     Slot *tab_old;              // pairs seen at prev_tick (read, flagged, then cleared)
     Slot *tab_cur;              // pairs seen at tick (filled by refine)
     double *xy_prev;            // positions at prev_tick (pid order), valid if has_prev
+    float4 *mbr_old;            // boxes of the pairs in tab_old (key-value output only)
+    float4 *mbr_cur;            // boxes of the pairs in tab_cur
     double *xy_save;            // receives this tick's positions for the next tick
     unsigned long long *stamps; // run_ticks: device wall clock at the end of every tick
     uint32_t nstamps;
@@ -163,6 +175,8 @@ struct DevState {
     uint32_t lo_bits;              // region bits split by the second pass
     uint32_t coarse_bins;          // bins of the first pass
     uint32_t bucket_blocks;        // blocks of the first pass
+    KvLogEntry *kvlog;             // key-value output log (null when off)
+    unsigned long long kvlog_cap;
     Counters *ctr;
 };
 

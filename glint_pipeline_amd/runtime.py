@@ -32,6 +32,7 @@ STATUS_NAMES = {
     7: "GLINT_ERR_UNIT_CAPACITY", 8: "GLINT_ERR_OUTPUT_CAPACITY", 9: "GLINT_ERR_STATE",
     10: "GLINT_ERR_INJECTED",
 }
+globals().update({name: code for code, name in STATUS_NAMES.items()})
 CANDIDATES_REFERENCE, CANDIDATES_COMPLETE = 0, 1
 PRECISION_FP64, PRECISION_FP32_GUARD = 0, 1
 
@@ -67,7 +68,9 @@ class Config(ctypes.Structure):
         ("node_capacity", ctypes.c_uint32),
         ("record_contacts", ctypes.c_uint32),
         ("graph_ticks", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32),
+        ("kv_split_num", ctypes.c_uint32),
+        ("kv_capacity", ctypes.c_uint64),
+        ("kv_area_limit", ctypes.c_double),
     ]
 
 
@@ -94,6 +97,22 @@ class Totals(ctypes.Structure):
         "rechecks", "max_contacts_tick", "max_distinct_tick", "max_meetings_tick")]
 
 
+class KvSummary(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in ("kept_keys", "oversize_keys", "packed_bytes", "logged_meetings")] + [
+        (n, ctypes.c_uint32) for n in ("ctf_count", "sid_count", "oversize_oid_count", "zero_one_count",
+                                       "end_time_min", "end_time_max", "o_start_time_min", "o_start_time_max")]
+
+
+# glint_ctf, one file of the sorted key-value output
+CTF_DTYPE = np.dtype([("mbr", "<f4", (4,)), ("key_offset", "<u8"), ("packed_offset", "<u8")] +
+                     [(n, "<u4") for n in ("kv_count", "start_time_min", "start_time_max", "end_time_min",
+                                           "end_time_max", "bitmap_bytes")] +
+                     [(n, "<u2") for n in ("key_bit", "id_bit", "duration_bit", "end_bit", "low_x_bit",
+                                           "low_y_bit", "edge_bit", "mbr_bit", "x_grid", "y_grid")] +
+                     [("reserved", "<u4")])
+KV_KEYS, KV_BOXES, KV_OVERSIZE_KEYS, KV_OVERSIZE_BOXES, KV_SIDS, KV_CTFS, KV_BITMAPS, KV_OVERSIZE_BITMAP, \
+    KV_PACKED = range(9)
+
 # (name, restype, argtypes) of every entry point of include/glint.h
 _VP = ctypes.c_void_p
 _U32, _U64, _I = ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int
@@ -119,6 +138,10 @@ SIGNATURES = {
     "glint_active_pairs": (_I, [_VP, _VP, _VP, _U64, ctypes.POINTER(_U64)]),
     "glint_device_meetings": (_I, [_VP, ctypes.POINTER(_VP), ctypes.POINTER(_U64)]),
     "glint_debug_inject_failure": (_I, [_VP, ctypes.c_char_p]),
+    "glint_kv_pending": (_I, [_VP, ctypes.POINTER(_U64)]),
+    "glint_kv_flush": (_I, [_VP, ctypes.POINTER(KvSummary)]),
+    "glint_kv_part_size": (_I, [_VP, _U32, ctypes.POINTER(_U64)]),
+    "glint_kv_copy": (_I, [_VP, _U32, _VP, _U64]),
     "glint_totals_get": (_I, [_VP, ctypes.POINTER(Totals), _I]),
     "glint_set_kernel_profiling": (_I, [_VP, _I]),
     "glint_kernel_times": (_I, [_VP, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_double),
@@ -208,7 +231,8 @@ def make_config(num_objects: int, box, *, reach_distance: float = 2.0, grid_capa
                 guard_epsilon: float = 1e-3, num_meeting_buckets: int = 0, unit_capacity: int = 0,
                 meeting_capacity: int = 0, node_capacity: int = 0, record_contacts: bool = False,
                 contact_capacity: int = 0, device: int = 0, max_meet_time: int = 4096,
-                graph_ticks: int = 0) -> Config:
+                graph_ticks: int = 0, kv_capacity: int = 0, kv_split_num: int = 10,
+                kv_area_limit: float = 100.0) -> Config:
     c = Config()
     lib().glint_config_default(ctypes.byref(c))
     c.num_objects = num_objects
@@ -231,6 +255,9 @@ def make_config(num_objects: int, box, *, reach_distance: float = 2.0, grid_capa
     c.contact_capacity = contact_capacity
     c.device = device
     c.graph_ticks = graph_ticks
+    c.kv_capacity = kv_capacity
+    c.kv_split_num = kv_split_num
+    c.kv_area_limit = kv_area_limit
     return c
 
 
@@ -381,6 +408,40 @@ class Workbench:
         self._chk(lib().glint_kernel_times(self.h, names, ms, cnt, cap, ctypes.byref(n), int(reset)),
                   "glint_kernel_times")
         return {names[k].decode(): (ms[k], cnt[k]) for k in range(min(n.value, cap))}
+
+    def kv_pending(self) -> int:
+        """Directed keys logged since the last key-value flush."""
+        n = ctypes.c_uint64()
+        _check(lib().glint_kv_pending(self.h, ctypes.byref(n)), "glint_kv_pending", self.h)
+        return int(n.value)
+
+    def kv_flush(self) -> dict:
+        """Flush the sorted key-value output after the last evaluated tick;
+        returns every part as numpy arrays (keys as (lo, hi) u64 pairs)."""
+        sm = KvSummary()
+        _check(lib().glint_kv_flush(self.h, ctypes.byref(sm)), "glint_kv_flush", self.h)
+        summary = {f: int(getattr(sm, f)) for f, _ in KvSummary._fields_}
+
+        def part(p, dtype, shape=None):
+            nb = ctypes.c_uint64()
+            _check(lib().glint_kv_part_size(self.h, p, ctypes.byref(nb)), "glint_kv_part_size", self.h)
+            out = np.zeros(nb.value // np.dtype(dtype).itemsize, dtype=dtype)
+            _check(lib().glint_kv_copy(self.h, p, out.ctypes.data if out.size else None, nb.value),
+                   "glint_kv_copy", self.h)
+            return out.reshape(shape) if shape else out
+
+        return {
+            "summary": summary,
+            "keys": part(KV_KEYS, np.uint64, (-1, 2)),
+            "boxes": part(KV_BOXES, np.float32, (-1, 4)),
+            "okeys": part(KV_OVERSIZE_KEYS, np.uint64, (-1, 2)),
+            "oboxes": part(KV_OVERSIZE_BOXES, np.float32, (-1, 4)),
+            "sids": part(KV_SIDS, np.uint16),
+            "ctfs": part(KV_CTFS, CTF_DTYPE),
+            "bitmaps": part(KV_BITMAPS, np.uint8),
+            "obitmap": part(KV_OVERSIZE_BITMAP, np.uint8),
+            "packed": part(KV_PACKED, np.uint8),
+        }
 
     def inject_failure(self, stage: str) -> None:
         self._chk(lib().glint_debug_inject_failure(self.h, stage.encode()), "glint_debug_inject_failure")

@@ -83,7 +83,9 @@ typedef struct glint_config {
     uint32_t node_capacity;        /* quadtree nodes; 0: automatic */
     uint32_t record_contacts;      /* keep each tick's contact pairs for glint_contacts() */
     uint32_t graph_ticks;          /* glint_run_ticks: ticks per captured hipGraph (0: plain launches) */
-    uint32_t reserved;
+    uint32_t kv_split_num;         /* config.h:60: sorted key-value output in kv_split_num^2 files */
+    uint64_t kv_capacity;          /* directed keys held between flushes (2 per meeting); 0: output off */
+    double kv_area_limit;          /* dProbFalse (process.cpp:408-411): object box area, squared input units */
 } glint_config;
 
 /* One emitted meeting: pid_a < pid_b, first and last tick of contact. */
@@ -229,6 +231,64 @@ glint_status glint_device_meetings(glint_ctx *ctx, const glint_meeting **d_out, 
  * "evaluate", "update_schema", "alloc") to fail with GLINT_ERR_INJECTED, for
  * failure-path tests. */
 glint_status glint_debug_inject_failure(glint_ctx *ctx, const char *stage);
+
+/* ---------------------------------------------------- sorted key-value output
+ * The flush branch of the reference's GPU path (geometry.cu:1752-2138).  With
+ * kv_capacity > 0 every active pair carries the float box of both objects
+ * over its run (geometry.cu:469-494), and every ended meeting whose box lies
+ * in the root box is logged (geometry.cu:665-735).  A flush turns the log into
+ * two directed keys per meeting
+ *     [sid:12 | oid:26 | target:26 | duration:32 | end - end_time_min:32]
+ * (util.h:47-52, geometry.cu:724-726), marks objects oversize (sid 1) by
+ * meeting duration and object box area, assigns the other objects STR file
+ * ids 2..split^2+1 (geometry.cu:740-796,1832-1895), sorts the keys of each
+ * part, and computes per-file limits, widths (step_merge.cpp:145-186),
+ * bitmaps and variable-width keys (geometry.cu:820-1107). */
+typedef struct glint_kv_summary {
+    uint64_t kept_keys;            /* keys in the files (h_keys) */
+    uint64_t oversize_keys;        /* keys of sid-1 objects (oversize buffer) */
+    uint64_t packed_bytes;         /* bytes of the variable-width key stream */
+    uint64_t logged_meetings;      /* meetings organised by this flush */
+    uint32_t ctf_count;            /* files: kv_split_num^2 */
+    uint32_t sid_count;            /* objects with at least one key */
+    uint32_t oversize_oid_count;   /* the reference's count: oversize meetings + oversize objects */
+    uint32_t zero_one_count;       /* STR position of the first object given a file */
+    uint32_t end_time_min, end_time_max;
+    uint32_t o_start_time_min, o_start_time_max;   /* oversize buffer start times */
+} glint_kv_summary;
+
+typedef struct glint_ctf {         /* CTF (step_merge.h:64-84): one file */
+    float mbr[4];                  /* low_x, low_y, high_x, high_y of its keys' boxes */
+    uint64_t key_offset;           /* first key in the sorted kept keys */
+    uint64_t packed_offset;        /* first byte in the packed key stream */
+    uint32_t kv_count, start_time_min, start_time_max, end_time_min, end_time_max, bitmap_bytes;
+    uint16_t key_bit, id_bit, duration_bit, end_bit, low_x_bit, low_y_bit, edge_bit, mbr_bit, x_grid, y_grid;
+    uint32_t reserved;
+} glint_ctf;
+
+typedef enum glint_kv_part {
+    GLINT_KV_KEYS = 0,             /* kept keys, sorted: 16 B each, little-endian 128-bit */
+    GLINT_KV_BOXES = 1,            /* their boxes: 4 floats each */
+    GLINT_KV_OVERSIZE_KEYS = 2,    /* oversize keys, sorted */
+    GLINT_KV_OVERSIZE_BOXES = 3,
+    GLINT_KV_SIDS = 4,             /* uint16 file id per object (0: no key, 1: oversize) */
+    GLINT_KV_CTFS = 5,             /* glint_ctf per file */
+    GLINT_KV_BITMAPS = 6,          /* 8192 bytes per file: bit x + y*x_grid */
+    GLINT_KV_OVERSIZE_BITMAP = 7,  /* 8192 bytes: 256x256 cells of the root box */
+    GLINT_KV_PACKED = 8            /* files' keys at key_bit/8 bytes each, little-endian */
+} glint_kv_part;
+
+/* Directed keys waiting for the next flush (2 per logged meeting). */
+glint_status glint_kv_pending(glint_ctx *ctx, uint64_t *keys);
+/* Organise everything logged since the last flush, after the last evaluated
+ * tick (the reference flushes when the count passes kv_restriction,
+ * geometry.cu:1752); the next run's end offsets count from that tick
+ * (process.cpp:549,271). */
+glint_status glint_kv_flush(glint_ctx *ctx, glint_kv_summary *out);
+/* Size in bytes of one output part of the last flush. */
+glint_status glint_kv_part_size(glint_ctx *ctx, uint32_t part, uint64_t *bytes);
+/* Copy one output part of the last flush to host memory (bytes = its size). */
+glint_status glint_kv_copy(glint_ctx *ctx, uint32_t part, void *dst, uint64_t bytes);
 
 /* Device memory helpers for callers without a GPU framework. */
 glint_status glint_device_malloc(void **ptr, uint64_t bytes, uint32_t device);

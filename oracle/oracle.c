@@ -30,6 +30,7 @@
  *    this tick's counts before any change.
  */
 #include "oracle.h"
+#include "oracle_kv_internal.h"
 
 #include <math.h>
 #include <omp.h>
@@ -65,7 +66,10 @@ typedef struct octx {
     ubuf contacts;
     uint64_t *akeys;     /* active pairs, sorted */
     uint32_t *astart, *alast;
+    float *abox;         /* 4 per active pair when key-value output is on */
     uint64_t nactive, cap_active;
+    okv *kv;
+    uint32_t last_tick;
     uint32_t *meet;      /* 4 u32 per meeting */
     uint64_t nmeet, cap_meet;
     oracle_stats st;
@@ -323,31 +327,66 @@ static void meet_push(octx *c, uint64_t key, uint32_t start, uint32_t end)
     m[3] = end;
 }
 
-/* keys: sorted unique contact pairs of this tick. */
-static void evaluate(octx *c, const uint64_t *keys, uint64_t nk, uint32_t tick)
+/* mbr_update_by_point (geometry.cu:82-97): float box, double point. */
+static void box_point(float *m, double px, double py)
+{
+    if (m[0] > px) m[0] = (float)px;
+    if (m[2] < px) m[2] = (float)px;
+    if (m[1] > py) m[1] = (float)py;
+    if (m[3] < py) m[3] = (float)py;
+}
+
+/* The two points of a contact grow the pair's box (geometry.cu:474-489). */
+static void box_contact(float *m, const double *xy, uint64_t key)
+{
+    const size_t a = (size_t)(key >> 32), b = (size_t)(uint32_t)key;
+    box_point(m, xy[2 * a], xy[2 * a + 1]);
+    box_point(m, xy[2 * b], xy[2 * b + 1]);
+}
+
+/* keys: sorted unique contact pairs of this tick; xy: this tick's
+ * positions (needed only for key-value output). */
+static void evaluate(octx *c, const uint64_t *keys, uint64_t nk, uint32_t tick, const double *xy)
 {
     uint64_t cap = c->nactive + nk;
     uint64_t *nkeys = (uint64_t *)malloc((cap ? cap : 1) * sizeof(uint64_t));
     uint32_t *nstart = (uint32_t *)malloc((cap ? cap : 1) * sizeof(uint32_t));
     uint32_t *nlast = (uint32_t *)malloc((cap ? cap : 1) * sizeof(uint32_t));
+    const int boxes = c->kv != NULL && xy != NULL;
+    float *nbox = boxes ? (float *)malloc((cap ? cap : 1) * 4 * sizeof(float)) : NULL;
     uint64_t i = 0, j = 0, k = 0;
     const uint32_t minm = c->cfg.min_meet_time;
     c->nmeet = 0;
     while (i < c->nactive || j < nk) {
         if (j >= nk || (i < c->nactive && c->akeys[i] < keys[j])) {
             /* not seen this tick: ended (meeting.cpp:147-154) */
-            if (tick - c->astart[i] >= minm + 1) meet_push(c, c->akeys[i], c->astart[i], c->alast[i]);
+            if (tick - c->astart[i] >= minm + 1) {
+                meet_push(c, c->akeys[i], c->astart[i], c->alast[i]);
+                if (boxes)
+                    okv_emit(c->kv, (uint32_t)(c->akeys[i] >> 32), (uint32_t)c->akeys[i], c->astart[i], c->alast[i],
+                             c->abox + 4 * i);
+            }
             i++;
         } else if (i >= c->nactive || keys[j] < c->akeys[i]) {
             nkeys[k] = keys[j];   /* new bucket: start = end = cur (refinement.cpp:47-50) */
             nstart[k] = tick;
             nlast[k] = tick;
+            if (boxes) {
+                float *m = nbox + 4 * k;
+                m[0] = m[1] = 100000.0f;
+                m[2] = m[3] = -100000.0f;
+                box_contact(m, xy, keys[j]);
+            }
             k++;
             j++;
         } else {
             nkeys[k] = keys[j];   /* still active: end = cur (refinement.cpp:40-41) */
             nstart[k] = c->astart[i];
             nlast[k] = tick;
+            if (boxes) {
+                memcpy(nbox + 4 * k, c->abox + 4 * i, 4 * sizeof(float));
+                box_contact(nbox + 4 * k, xy, keys[j]);
+            }
             k++;
             i++;
             j++;
@@ -356,9 +395,11 @@ static void evaluate(octx *c, const uint64_t *keys, uint64_t nk, uint32_t tick)
     free(c->akeys);
     free(c->astart);
     free(c->alast);
+    free(c->abox);
     c->akeys = nkeys;
     c->astart = nstart;
     c->alast = nlast;
+    c->abox = nbox;
     c->nactive = k;
     c->cap_active = cap;
 }
@@ -370,7 +411,7 @@ int oracle_evaluate_contacts(void *h, const uint64_t *keys, uint64_t n, uint32_t
     memcpy(tmp, keys, n * sizeof(uint64_t));
     radix_sort_u64(tmp, n);
     n = unique_u64(tmp, n);
-    evaluate(c, tmp, n, tick);
+    evaluate(c, tmp, n, tick, NULL);
     free(tmp);
     return 0;
 }
@@ -495,6 +536,8 @@ void oracle_destroy(void *h)
     free(c->akeys);
     free(c->astart);
     free(c->alast);
+    free(c->abox);
+    okv_destroy(c->kv);
     free(c->meet);
     free(c);
 }
@@ -613,7 +656,8 @@ int oracle_tick(void *h, const double *xy, uint32_t tick)
     double t2 = omp_get_wtime();
 
     /* ---- evaluate (meeting.cpp:134-155) */
-    evaluate(c, c->contacts.v, c->contacts.n, tick);
+    evaluate(c, c->contacts.v, c->contacts.n, tick, xy);
+    c->last_tick = tick;
     double t3 = omp_get_wtime();
 
     /* ---- stats, then schema update (process.cpp:608-612) */
@@ -763,3 +807,35 @@ int oracle_update_schema_counts(void *h, const uint32_t *counts, uint64_t nleave
     update_schema(c);
     return 0;
 }
+
+/* --------------------------------------------------- key-value output */
+
+int oracle_kv_enable(void *h, const oracle_kv_config *kc)
+{
+    octx *c = (octx *)h;
+    if (!kc || kc->split_num == 0 || kc->split_num * kc->split_num + 2 > 4096 || kc->max_meet_time == 0) return -1;
+    if (c->built || c->kv) return -1;
+    const double map[4] = {c->cfg.low_x, c->cfg.low_y, c->cfg.high_x, c->cfg.high_y};
+    /* first end_time_min = start_time + min_meet_time (process.cpp:404) */
+    c->kv = okv_create(c->cfg.num_objects, map, c->cfg.start_time + c->cfg.min_meet_time, kc);
+    return 0;
+}
+
+uint64_t oracle_kv_pending(void *h) { return ((octx *)h)->kv ? okv_pending(((octx *)h)->kv) : 0; }
+
+int oracle_kv_flush(void *h, oracle_kv_summary *out)
+{
+    octx *c = (octx *)h;
+    if (!c->kv) return -1;
+    int rc = okv_flush(c->kv, c->last_tick);
+    if (out) *out = *okv_summary(c->kv);
+    return rc;
+}
+
+const uint64_t *oracle_kv_keys(void *h, int part) { return okv_keys(((octx *)h)->kv, part); }
+const float *oracle_kv_boxes(void *h, int part) { return okv_boxes(((octx *)h)->kv, part); }
+const uint16_t *oracle_kv_sids(void *h) { return okv_sids(((octx *)h)->kv); }
+const oracle_ctf *oracle_kv_ctfs(void *h) { return okv_ctfs(((octx *)h)->kv); }
+const uint8_t *oracle_kv_bitmaps(void *h) { return okv_bitmaps(((octx *)h)->kv); }
+const uint8_t *oracle_kv_obitmap(void *h) { return okv_obitmap(((octx *)h)->kv); }
+const uint8_t *oracle_kv_packed(void *h) { return okv_packed(((octx *)h)->kv); }

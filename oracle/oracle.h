@@ -105,6 +105,58 @@ int oracle_evaluate_contacts(void *h, const uint64_t *keys, uint64_t n, uint32_t
  * depth-first leaf order (tests of split/merge). */
 int oracle_update_schema_counts(void *h, const uint32_t *counts, uint64_t nleaves);
 
+/* ------------------------------------------------ sorted key-value output
+ * The flush branch of the reference's GPU path (src/cuda/geometry.cu:665-806,
+ * 1752-2138; src/tracing/step_merge.cpp:140-310): every meeting carries the
+ * float box of both objects over its contact ticks; ended meetings become
+ * two directed 128-bit keys; a flush sorts them, assigns STR file ids and
+ * encodes per-file bitmaps and variable-width keys.  oracle_kv.c states the
+ * rules line by line. */
+typedef struct oracle_kv_config {
+    uint32_t split_num;        /* config.h:60: split_num^2 files per flush */
+    uint32_t max_meet_time;    /* config.h:40 */
+    double area_limit;         /* dProbFalse, process.cpp:408-411 (squared input units) */
+} oracle_kv_config;
+
+typedef struct oracle_kv_summary {
+    uint64_t kept_keys;        /* keys in files (sid >= 2 or unassigned) */
+    uint64_t oversize_keys;    /* keys of sid-1 objects */
+    uint64_t packed_bytes;
+    uint64_t logged_meetings;  /* meetings whose box lies in the map box */
+    uint32_t ctf_count;
+    uint32_t sid_count;        /* objects with at least one key */
+    uint32_t oversize_oid_count;
+    uint32_t zero_one_count;
+    uint32_t end_time_min, end_time_max;
+    uint32_t o_start_time_min, o_start_time_max;
+} oracle_kv_summary;
+
+typedef struct oracle_ctf {    /* CTF (step_merge.h:64-84), one per file */
+    float mbr[4];              /* low_x, low_y, high_x, high_y */
+    uint64_t key_offset;       /* first key in the sorted kept keys */
+    uint64_t packed_offset;    /* first byte in the packed key stream */
+    uint32_t kv_count, start_time_min, start_time_max, end_time_min, end_time_max, bitmap_bytes;
+    uint16_t key_bit, id_bit, duration_bit, end_bit, low_x_bit, low_y_bit, edge_bit, mbr_bit, x_grid, y_grid;
+    uint32_t reserved;
+} oracle_ctf;
+
+/* Before the first tick: start tracking meeting boxes and collecting keys. */
+int oracle_kv_enable(void *h, const oracle_kv_config *kc);
+/* Directed keys collected since the last flush. */
+uint64_t oracle_kv_pending(void *h);
+/* Organise the collected keys after the last tick (cur_time = that tick). */
+int oracle_kv_flush(void *h, oracle_kv_summary *out);
+/* Results of the last flush: part 0 kept keys, 1 oversize keys; keys are
+ * (lo, hi) u64 pairs, boxes 4 floats; sids one per object; bitmaps 8192
+ * bytes per file; the oversize bitmap 8192 bytes. */
+const uint64_t *oracle_kv_keys(void *h, int part);
+const float *oracle_kv_boxes(void *h, int part);
+const uint16_t *oracle_kv_sids(void *h);
+const oracle_ctf *oracle_kv_ctfs(void *h);
+const uint8_t *oracle_kv_bitmaps(void *h);
+const uint8_t *oracle_kv_obitmap(void *h);
+const uint8_t *oracle_kv_packed(void *h);
+
 /* Predicates, exactly as the reference evaluates them in planar metres. */
 int oracle_point_within(double x1, double y1, double x2, double y2, double r);
 double oracle_box_distance(double lx, double ly, double hx, double hy, double px, double py);

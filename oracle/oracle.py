@@ -39,6 +39,26 @@ class OracleStats(ctypes.Structure):
         (n, ctypes.c_double) for n in ("t_filter", "t_refine", "t_evaluate", "t_schema")]
 
 
+class OracleKvConfig(ctypes.Structure):
+    _fields_ = [("split_num", ctypes.c_uint32), ("max_meet_time", ctypes.c_uint32), ("area_limit", ctypes.c_double)]
+
+
+class OracleKvSummary(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in ("kept_keys", "oversize_keys", "packed_bytes", "logged_meetings")] + [
+        (n, ctypes.c_uint32) for n in ("ctf_count", "sid_count", "oversize_oid_count", "zero_one_count",
+                                       "end_time_min", "end_time_max", "o_start_time_min", "o_start_time_max")]
+
+
+# one file (CTF) record, oracle_ctf in oracle.h
+CTF_DTYPE = np.dtype([("mbr", "<f4", (4,)), ("key_offset", "<u8"), ("packed_offset", "<u8")] +
+                     [(n, "<u4") for n in ("kv_count", "start_time_min", "start_time_max", "end_time_min",
+                                           "end_time_max", "bitmap_bytes")] +
+                     [(n, "<u2") for n in ("key_bit", "id_bit", "duration_bit", "end_bit", "low_x_bit",
+                                           "low_y_bit", "edge_bit", "mbr_bit", "x_grid", "y_grid")] +
+                     [("reserved", "<u4")])
+assert CTF_DTYPE.itemsize == 80
+BITMAP_BYTES = 8192
+
 _lib = None
 
 
@@ -83,6 +103,16 @@ def lib() -> ctypes.CDLL:
         L.oracle_point_within.restype = ctypes.c_int
         L.oracle_box_distance.argtypes = [ctypes.c_double] * 6
         L.oracle_box_distance.restype = ctypes.c_double
+        L.oracle_kv_enable.argtypes = [vp, ctypes.POINTER(OracleKvConfig)]
+        L.oracle_kv_pending.argtypes = [vp]
+        L.oracle_kv_pending.restype = ctypes.c_uint64
+        L.oracle_kv_flush.argtypes = [vp, ctypes.POINTER(OracleKvSummary)]
+        for f in ("oracle_kv_keys", "oracle_kv_boxes"):
+            getattr(L, f).argtypes = [vp, ctypes.c_int]
+            getattr(L, f).restype = ctypes.c_void_p
+        for f in ("oracle_kv_sids", "oracle_kv_ctfs", "oracle_kv_bitmaps", "oracle_kv_obitmap", "oracle_kv_packed"):
+            getattr(L, f).argtypes = [vp]
+            getattr(L, f).restype = ctypes.c_void_p
         _lib = L
     return _lib
 
@@ -176,6 +206,44 @@ class Oracle:
     def evaluate_contacts(self, keys, t: int) -> None:
         keys = np.ascontiguousarray(keys, dtype=np.uint64)
         lib().oracle_evaluate_contacts(self.h, keys.ctypes.data, keys.size, t)
+
+    def kv_enable(self, split_num: int = 10, max_meet_time: int = 4096, area_limit: float = 100.0) -> None:
+        kc = OracleKvConfig(split_num, max_meet_time, area_limit)
+        if lib().oracle_kv_enable(self.h, ctypes.byref(kc)):
+            raise ValueError("invalid key-value configuration (or enabled after the first tick)")
+
+    def kv_pending(self) -> int:
+        return int(lib().oracle_kv_pending(self.h))
+
+    def kv_flush(self) -> dict:
+        """Organise the keys collected since the last flush; returns every
+        output of the flush as numpy arrays."""
+        sm = OracleKvSummary()
+        rc = lib().oracle_kv_flush(self.h, ctypes.byref(sm))
+        summary = {f: int(getattr(sm, f)) for f, _ in OracleKvSummary._fields_}
+        if rc:
+            raise ValueError(f"key-value flush failed: {summary}")
+        L = lib()
+
+        def arr(ptr, dtype, count):
+            if count == 0:
+                return np.zeros(0, dtype=dtype)
+            buf = (ctypes.c_uint8 * (count * np.dtype(dtype).itemsize)).from_address(ptr)
+            return np.frombuffer(buf, dtype=dtype).copy()
+
+        nk, no, C = summary["kept_keys"], summary["oversize_keys"], summary["ctf_count"]
+        return {
+            "summary": summary,
+            "keys": arr(L.oracle_kv_keys(self.h, 0), np.uint64, 2 * nk).reshape(-1, 2),
+            "boxes": arr(L.oracle_kv_boxes(self.h, 0), np.float32, 4 * nk).reshape(-1, 4),
+            "okeys": arr(L.oracle_kv_keys(self.h, 1), np.uint64, 2 * no).reshape(-1, 2),
+            "oboxes": arr(L.oracle_kv_boxes(self.h, 1), np.float32, 4 * no).reshape(-1, 4),
+            "sids": arr(L.oracle_kv_sids(self.h), np.uint16, self.n),
+            "ctfs": arr(L.oracle_kv_ctfs(self.h), CTF_DTYPE, C),
+            "bitmaps": arr(L.oracle_kv_bitmaps(self.h), np.uint8, C * BITMAP_BYTES),
+            "obitmap": arr(L.oracle_kv_obitmap(self.h), np.uint8, BITMAP_BYTES),
+            "packed": arr(L.oracle_kv_packed(self.h), np.uint8, summary["packed_bytes"]),
+        }
 
     def update_schema_counts(self, counts) -> None:
         counts = np.ascontiguousarray(counts, dtype=np.uint32)
