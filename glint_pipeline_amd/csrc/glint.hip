@@ -23,7 +23,6 @@
 #include <cstring>
 #include <new>
 #include <string>
-#include <type_traits>
 #include <vector>
 
 #include "glint.h"
@@ -1211,16 +1210,15 @@ struct TableLds {
 // both objects over its run (geometry.cu:469-494): the box of a new pair is
 // its two points at this tick, a continuing pair adds them to its old box.
 // min/max of the rounded coordinates equals the rounded min/max, so the box
-// does not depend on the order of the updates.
-struct TableLdsKv : TableLds {
-    float4 oldm[kRegion];
-    float4 newm[kRegion];
-};
-
+// does not depend on the order of the updates.  Boxes stay in HBM beside
+// their slots (read only for continuing and ended pairs, written by the
+// thread that claims the slot), so the workgroup keeps the plain variant's
+// LDS footprint; the points come from a float copy of the positions that
+// fits the last-level cache.
 template <bool KV>
 __global__ __launch_bounds__(kBlock) void k_table(DevState s)
 {
-    __shared__ typename std::conditional<KV, TableLdsKv, TableLds>::type L;
+    __shared__ TableLds L;
     const Slot *tab_old = s.tp->tab_old;
     Slot *tab_cur = s.tp->tab_cur;
     const uint32_t tick = s.tp->tick, last = s.tp->prev_tick;
@@ -1247,7 +1245,6 @@ __global__ __launch_bounds__(kBlock) void k_table(DevState s)
             L.olds[k] = pre[q].start;
             L.seen[k] = 0;
             L.newk[k] = kEmptyKey;
-            if constexpr (KV) L.oldm[k] = s.tp->mbr_old[base + k];
         }
         const uint32_t rn = r + gridDim.x;
         if (rn < nreg) {
@@ -1282,16 +1279,14 @@ __global__ __launch_bounds__(kBlock) void k_table(DevState s)
                 if (prev == kEmptyKey) {
                     L.news[q] = start;
                     if constexpr (KV) {
-                        const double *xy = s.tp->xy;
-                        const size_t a = (size_t)(key >> 32), b = (size_t)(uint32_t)key;
-                        const float xa = (float)xy[2 * a], ya = (float)xy[2 * a + 1];
-                        const float xb = (float)xy[2 * b], yb = (float)xy[2 * b + 1];
-                        float4 m = make_float4(fminf(xa, xb), fminf(ya, yb), fmaxf(xa, xb), fmaxf(ya, yb));
+                        const float2 pa = s.pos32[(uint32_t)(key >> 32)], pb = s.pos32[(uint32_t)key];
+                        float4 m = make_float4(fminf(pa.x, pb.x), fminf(pa.y, pb.y), fmaxf(pa.x, pb.x),
+                                               fmaxf(pa.y, pb.y));
                         if (oq >= 0) {
-                            const float4 o = L.oldm[oq];
+                            const float4 o = s.tp->mbr_old[base + oq];
                             m = make_float4(fminf(m.x, o.x), fminf(m.y, o.y), fmaxf(m.z, o.z), fmaxf(m.w, o.w));
                         }
-                        L.newm[q] = m;
+                        s.tp->mbr_cur[base + q] = m;
                     }
                     inserted++;
                     placed = true;
@@ -1322,7 +1317,8 @@ __global__ __launch_bounds__(kBlock) void k_table(DevState s)
             if constexpr (KV) {
                 // meetings whose box lies in the map box are logged
                 // (geometry.cu:665-670, box_contain geometry.cu:53-58)
-                const float4 bx = L.oldm[k];
+                float4 bx = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (emit) bx = s.tp->mbr_old[base + k];
                 const bool put = emit && bx.x >= s.lx && bx.z <= s.hx && bx.y >= s.ly && bx.w <= s.hy;
                 const unsigned long long mk = __ballot(put);
                 if (mk) {
@@ -1364,7 +1360,6 @@ __global__ __launch_bounds__(kBlock) void k_table(DevState s)
             ns.start = L.news[k];
             ns.seen = 0;
             tab_cur[base + k] = ns;
-            if constexpr (KV) s.tp->mbr_cur[base + k] = L.newm[k];
         }
         __syncthreads();
     }
@@ -1675,8 +1670,17 @@ void enqueue_refine(glint_ctx *c)
         launch(c, K_BUCKET_FINE, [&] { k_bucket_single<<<grid_for(s.coarse_bins + 1), kBlock, 0, st>>>(s); });
 }
 
+// Float copy of this tick's positions (pid order) for the pair boxes.
+__global__ __launch_bounds__(kBlock) void k_pos32(DevState s)
+{
+    const double *xy = s.tp->xy;
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < s.n; i += gridDim.x * kBlock)
+        s.pos32[i] = make_float2((float)xy[2 * (size_t)i], (float)xy[2 * (size_t)i + 1]);
+}
+
 void enqueue_evaluate(glint_ctx *c)
 {
+    if (c->kv) launch(c, K_TABLE, [&] { k_pos32<<<grid_for(c->s.n), kBlock, 0, c->stream>>>(c->s); });
     const uint32_t nreg = 1u << c->s.region_bits;
     const uint32_t g = std::min<uint32_t>(nreg, 4096);
     if (c->kv)
@@ -1925,6 +1929,7 @@ glint_status glint_create(const glint_config *cfg, glint_ctx **out)
     if (cfg->kv_capacity) {
         ALLOC(c->mbr[0], slots);
         ALLOC(c->mbr[1], slots);
+        ALLOC(s.pos32, N);
         KvParams kp{};
         kp.num_objects = (uint32_t)N;
         kp.split_num = cfg->kv_split_num;

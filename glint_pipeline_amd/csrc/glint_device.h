@@ -176,6 +176,7 @@ struct DevState {
     uint32_t coarse_bins;          // bins of the first pass
     uint32_t bucket_blocks;        // blocks of the first pass
     KvLogEntry *kvlog;             // key-value output log (null when off)
+    float2 *pos32;                 // float positions of the tick, pid order (key-value output)
     unsigned long long kvlog_cap;
     Counters *ctr;
 };

@@ -414,15 +414,21 @@ __global__ __launch_bounds__(kB) void k_gather_boxes(const uint32_t *val, const 
         out[i] = in[val[i]];
 }
 
-// get_CTF_capacity (geometry.cu:808-818)
+// get_CTF_capacity (geometry.cu:808-818): per-block histogram in LDS, then
+// one global add per file and block.
 __global__ __launch_bounds__(kB) void k_capacity(const uint32_t *oid, uint32_t n, uint32_t z, const uint32_t *sids,
-                                                 const uint32_t *cnt, uint32_t *cap)
+                                                 const uint32_t *cnt, uint32_t nctf, uint32_t *cap)
 {
-    for (uint32_t p = blockIdx.x * kB + threadIdx.x; p < n; p += gridDim.x * kB) {
-        if (p < z) continue;
+    __shared__ uint32_t h[4096];
+    for (uint32_t k = threadIdx.x; k < nctf; k += kB) h[k] = 0;
+    __syncthreads();
+    for (uint32_t p = z + blockIdx.x * kB + threadIdx.x; p < n; p += gridDim.x * kB) {
         const uint32_t o = oid[p];
-        atomicAdd(&cap[sids[o] - 2], cnt[o]);
+        atomicAdd(&h[sids[o] - 2], cnt[o]);
     }
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < nctf; k += kB)
+        if (h[k]) atomicAdd(&cap[k], h[k]);
 }
 
 __global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v)
@@ -510,17 +516,21 @@ __device__ __forceinline__ void key_range(const glint_ctf &f, uint64_t nk, uint6
     b1 = min((uint64_t)f.key_offset + f.kv_count, nk);
 }
 
-// cuda_write_bitmap (geometry.cu:992-1019) with cells clipped to the grid.
+// cuda_write_bitmap (geometry.cu:992-1019) with cells clipped to the grid;
+// gridDim.y slices of every file's keys, each OR-ed in from LDS.
 __global__ __launch_bounds__(kB) void k_bitmaps(const glint_ctf *ctfs, const float4 *boxes, uint64_t nk,
                                                 uint32_t *bitmaps)
 {
     __shared__ uint32_t bm[kBitmapBytes / 4];
     const glint_ctf f = ctfs[blockIdx.x];
+    if (!f.key_bit) return;
     for (uint32_t k = threadIdx.x; k < kBitmapBytes / 4; k += kB) bm[k] = 0;
     __syncthreads();
-    if (f.key_bit) {
-        uint64_t b0, b1;
-        key_range(f, nk, b0, b1);
+    {
+        uint64_t r0, r1;
+        key_range(f, nk, r0, r1);
+        const uint64_t per = (r1 - r0 + gridDim.y - 1) / gridDim.y;
+        const uint64_t b0 = min(r1, r0 + per * blockIdx.y), b1 = min(r1, b0 + per);
         const float w = f.mbr[2] - f.mbr[0], h = f.mbr[3] - f.mbr[1];
         for (uint64_t i = b0 + threadIdx.x; i < b1; i += kB) {
             const float4 b = boxes[i];
@@ -537,42 +547,56 @@ __global__ __launch_bounds__(kB) void k_bitmaps(const glint_ctf *ctfs, const flo
     }
     __syncthreads();
     uint32_t *out = bitmaps + (size_t)blockIdx.x * (kBitmapBytes / 4);
-    for (uint32_t k = threadIdx.x; k < kBitmapBytes / 4; k += kB) out[k] = bm[k];
+    for (uint32_t k = threadIdx.x; k < kBitmapBytes / 4; k += kB)
+        if (bm[k]) atomicOr(&out[k], bm[k]);
 }
 
-// write_key_and_mbr (geometry.cu:1076-1107), serialize_mbr (geometry.cu:162-172).
-__global__ __launch_bounds__(kB) void k_pack(const glint_ctf *ctfs, const uint64_t *hi, const uint64_t *lo,
-                                             const float4 *boxes, uint64_t nk, uint8_t *packed)
+__device__ __forceinline__ void pack_one(const glint_ctf &f, uint64_t i, const uint64_t *hi, const uint64_t *lo,
+                                         const float4 *boxes, uint8_t *packed)
 {
-    const glint_ctf f = ctfs[blockIdx.x];
-    if (!f.key_bit) return;
-    uint64_t b0, b1;
-    key_range(f, nk, b0, b1);
     const uint32_t kb = f.key_bit / 8u;
     const float w = f.mbr[2] - f.mbr[0], h = f.mbr[3] - f.mbr[1];
     const float longer = fmaxf(w, h);
     const float qx = (float)((1ull << f.low_x_bit) - 1), qy = (float)((1ull << f.low_y_bit) - 1);
     const float qe = (float)((1ull << f.edge_bit) - 1);
+    const float4 b = boxes[i];
+    const uint64_t low0 = (uint64_t)(ratio(b.x - f.mbr[0], w) * qx);
+    const uint64_t low1 = (uint64_t)(ratio(b.y - f.mbr[1], h) * qy);
+    const uint64_t ex = (uint64_t)(ratio(b.z - b.x, longer) * qe);
+    const uint64_t ey = (uint64_t)(ratio(b.w - b.y, longer) * qe);
+    const uint64_t vm = (low0 << (f.low_y_bit + 2 * f.edge_bit)) + (low1 << (2 * f.edge_bit)) + (ex << f.edge_bit) + ey;
     const uint32_t s_pid = f.id_bit + f.duration_bit + f.end_bit + f.mbr_bit;
     const uint32_t s_tgt = f.duration_bit + f.end_bit + f.mbr_bit;
     const uint32_t s_dur = f.end_bit + f.mbr_bit;
-    for (uint64_t i = b0 + threadIdx.x; i < b1; i += kB) {
-        const float4 b = boxes[i];
-        const uint64_t low0 = (uint64_t)(ratio(b.x - f.mbr[0], w) * qx);
-        const uint64_t low1 = (uint64_t)(ratio(b.y - f.mbr[1], h) * qy);
-        const uint64_t ex = (uint64_t)(ratio(b.z - b.x, longer) * qe);
-        const uint64_t ey = (uint64_t)(ratio(b.w - b.y, longer) * qe);
-        const uint64_t vm = (low0 << (f.low_y_bit + 2 * f.edge_bit)) + (low1 << (2 * f.edge_bit)) +
-                            (ex << f.edge_bit) + ey;
-        const uint64_t h_ = hi[i], l_ = lo[i];
-        const unsigned __int128 pid = (unsigned __int128)(h_ >> 26);   // sid bits stay above the oid
-        const unsigned __int128 tgt = h_ & kOidMask;
-        const unsigned __int128 dur = l_ >> 32;
-        const unsigned __int128 end = l_ & 0xFFFFFFFFull;
-        const unsigned __int128 v = (pid << s_pid) + (tgt << s_tgt) + (dur << s_dur) + (end << f.mbr_bit) +
-                                    (unsigned __int128)vm;
-        uint8_t *o = packed + f.packed_offset + (i - b0) * kb;
-        for (uint32_t j = 0; j < kb; j++) o[j] = (uint8_t)(v >> (8 * j));
+    const uint64_t h_ = hi[i], l_ = lo[i];
+    const unsigned __int128 pid = (unsigned __int128)(h_ >> 26);   // sid bits stay above the oid
+    const unsigned __int128 tgt = h_ & kOidMask;
+    const unsigned __int128 dur = l_ >> 32;
+    const unsigned __int128 end = l_ & 0xFFFFFFFFull;
+    const unsigned __int128 v =
+        (pid << s_pid) + (tgt << s_tgt) + (dur << s_dur) + (end << f.mbr_bit) + (unsigned __int128)vm;
+    uint8_t *o = packed + f.packed_offset + (i - f.key_offset) * kb;
+    for (uint32_t j = 0; j < kb; j++) o[j] = (uint8_t)(v >> (8 * j));
+}
+
+// write_key_and_mbr (geometry.cu:1076-1107), serialize_mbr (geometry.cu:162-172):
+// one thread per kept key; its file is the last one starting at or before it.
+__global__ __launch_bounds__(kB) void k_pack(const glint_ctf *ctfs, uint32_t nctf, const uint64_t *hi,
+                                             const uint64_t *lo, const float4 *boxes, uint64_t nk, uint8_t *packed)
+{
+    for (uint64_t i = blockIdx.x * (uint64_t)kB + threadIdx.x; i < nk; i += (uint64_t)gridDim.x * kB) {
+        uint32_t a = 0, b = nctf;   // first file with key_offset > i
+        while (a < b) {
+            const uint32_t m = (a + b) / 2;
+            if (ctfs[m].key_offset <= i)
+                a = m + 1;
+            else
+                b = m;
+        }
+        if (a == 0) continue;
+        const glint_ctf &f = ctfs[a - 1];
+        if (!f.key_bit || i >= f.key_offset + f.kv_count) continue;
+        pack_one(f, i, hi, lo, boxes, packed);
     }
 }
 
@@ -879,7 +903,7 @@ glint_status kv_flush(KvEngine *e, uint64_t nlog, uint32_t cur_time, glint_kv_su
 
     // files: capacities, limits; widths on the host
     k_fill_u32<<<grid_of(C), kB, 0, st>>>(e->cap, C, 0);
-    if (z < N) k_capacity<<<gN, kB, 0, st>>>(e->oa.val, N, z, e->sids, e->cnt, e->cap);
+    if (z < N) k_capacity<<<grid_of(N - z, 1024), kB, 0, st>>>(e->oa.val, N, z, e->sids, e->cnt, C, e->cap);
     k_limits<<<C, kB, 0, st>>>(e->cap, C, e->b.hi, e->b.lo, e->box_out, nk, e->end_time_min, cur_time, e->ctfs);
     KV_TRY(hipGetLastError());
     e->h_ctfs.assign(C, glint_ctf{});
@@ -905,8 +929,12 @@ glint_status kv_flush(KvEngine *e, uint64_t nlog, uint32_t cur_time, glint_kv_su
     KV_TRY(hipMemcpyAsync(e->ctfs, e->h_ctfs.data(), C * sizeof(glint_ctf), hipMemcpyHostToDevice, st));
     // bytes of keys past the kept keys (clipped ranges) stay zero
     k_fill_u32<<<grid_of((bytes + 3) / 4), kB, 0, st>>>((uint32_t *)e->packed, (bytes + 3) / 4, 0);
-    k_bitmaps<<<C, kB, 0, st>>>(e->ctfs, e->box_out, nk, e->bitmaps);
-    k_pack<<<C, kB, 0, st>>>(e->ctfs, e->b.hi, e->b.lo, e->box_out, nk, e->packed);
+    k_fill_u32<<<grid_of((uint64_t)C * (kBitmapBytes / 4)), kB, 0, st>>>(e->bitmaps, (uint64_t)C * (kBitmapBytes / 4), 0);
+    {
+        const uint32_t slices = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(64, nk / ((uint64_t)C * 4096) + 1));
+        k_bitmaps<<<dim3(C, slices), kB, 0, st>>>(e->ctfs, e->box_out, nk, e->bitmaps);
+    }
+    if (nk) k_pack<<<grid_of(nk), kB, 0, st>>>(e->ctfs, C, e->b.hi, e->b.lo, e->box_out, nk, e->packed);
 
     // oversize buffer: start times and map bitmap
     k_fill_u32<<<8, kB, 0, st>>>(e->obitmap, kBitmapBytes / 4, 0);

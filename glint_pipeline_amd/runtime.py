@@ -415,12 +415,15 @@ class Workbench:
         _check(lib().glint_kv_pending(self.h, ctypes.byref(n)), "glint_kv_pending", self.h)
         return int(n.value)
 
-    def kv_flush(self) -> dict:
+    def kv_flush(self, copy: bool = True) -> dict:
         """Flush the sorted key-value output after the last evaluated tick;
-        returns every part as numpy arrays (keys as (lo, hi) u64 pairs)."""
+        returns every part as numpy arrays (keys as (lo, hi) u64 pairs), or
+        only the summary when copy is False (the parts stay on the device)."""
         sm = KvSummary()
         _check(lib().glint_kv_flush(self.h, ctypes.byref(sm)), "glint_kv_flush", self.h)
         summary = {f: int(getattr(sm, f)) for f, _ in KvSummary._fields_}
+        if not copy:
+            return {"summary": summary}
 
         def part(p, dtype, shape=None):
             nb = ctypes.c_uint64()
