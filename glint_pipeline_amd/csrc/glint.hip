@@ -411,6 +411,17 @@ __global__ void k_schema_done(DevState s)
     if (c->n_contacts > c->max_contacts) c->max_contacts = c->n_contacts;
     if (c->n_inserted > c->max_distinct) c->max_distinct = c->n_inserted;
     if (c->n_meet > c->max_meet) c->max_meet = c->n_meet;
+    if (s.mlog) c->n_mlog += c->n_meet < s.meet_cap ? c->n_meet : s.meet_cap;
+}
+
+// The tick's meetings appended to the meeting log ring (k_schema_done
+// advances its total afterwards).
+__global__ __launch_bounds__(kBlock) void k_log_append(DevState s)
+{
+    const unsigned long long n = min(s.ctr->n_meet, s.meet_cap), base = s.ctr->n_mlog;
+    for (unsigned long long i = blockIdx.x * (unsigned long long)kBlock + threadIdx.x; i < n;
+         i += (unsigned long long)gridDim.x * kBlock)
+        s.mlog[(base + i) % s.mlog_cap] = s.meet[i];
 }
 
 // ------------------------------------------------------------ partition
@@ -1414,13 +1425,13 @@ double sqrt_threshold(double r)
 enum KernelId {
     K_FILL_COUNTS, K_RESET, K_ASSIGN, K_SCAN_REDUCE, K_SCAN_TOP, K_SCAN_OBJECTS, K_SCATTER, K_FILL_UCOUNT,
     K_LOOKUP, K_SCAN_UNITS, K_UNIT_SCATTER, K_JOIN, K_BUCKET_COUNT, K_SCAN_WRITE, K_BUCKET_SCATTER, K_BUCKET_FINE,
-    K_TABLE, K_COLLECT, K_SPLIT, K_MERGE, K_SCHEMA_DONE, K_TICK_PARAMS, K_COUNT
+    K_TABLE, K_COLLECT, K_SPLIT, K_MERGE, K_SCHEMA_DONE, K_TICK_PARAMS, K_POS32, K_LOG_APPEND, K_COUNT
 };
 const char *const kKernelNames[K_COUNT] = {
     "k_fill_u32", "k_reset_tick", "k_assign", "k_scan_reduce", "k_scan_top", "k_scan_objects", "k_scatter",
     "k_fill_ucount", "k_lookup", "k_scan_units", "k_unit_scatter", "k_join", "k_bucket_count", "k_scan_write",
     "k_bucket_scatter", "k_bucket_fine", "k_table", "k_collect", "k_split", "k_merge", "k_schema_done",
-    "k_advance_tick"};
+    "k_advance_tick", "k_pos32", "k_log_append"};
 
 // Per-kernel HIP-event timing on the context's stream (optional).
 struct KernelProfiler {
@@ -1500,6 +1511,19 @@ struct glint_ctx {
     unsigned long long *stamps = nullptr;
     uint32_t stamp_cap = 0;
     double wall_khz = 0;
+};
+
+// Independent scenarios ticked together: one context (and stream) each,
+// optionally one graph holding graph_ticks ticks of every scenario as
+// parallel branches forked from and joined into an origin stream.
+struct glint_batch {
+    std::vector<glint_ctx *> ctx;
+    hipStream_t origin = nullptr;
+    hipEvent_t fork = nullptr, after = nullptr;
+    std::vector<hipEvent_t> join, ready;
+    hipGraphExec_t exec = nullptr;
+    uint32_t graph_ticks = 0;
+    std::string err;
 };
 
 namespace {
@@ -1680,13 +1704,14 @@ __global__ __launch_bounds__(kBlock) void k_pos32(DevState s)
 
 void enqueue_evaluate(glint_ctx *c)
 {
-    if (c->kv) launch(c, K_TABLE, [&] { k_pos32<<<grid_for(c->s.n), kBlock, 0, c->stream>>>(c->s); });
+    if (c->kv) launch(c, K_POS32, [&] { k_pos32<<<grid_for(c->s.n), kBlock, 0, c->stream>>>(c->s); });
     const uint32_t nreg = 1u << c->s.region_bits;
     const uint32_t g = std::min<uint32_t>(nreg, 4096);
     if (c->kv)
         launch(c, K_TABLE, [&] { k_table<true><<<g, kBlock, 0, c->stream>>>(c->s); });
     else
         launch(c, K_TABLE, [&] { k_table<false><<<g, kBlock, 0, c->stream>>>(c->s); });
+    if (c->s.mlog) launch(c, K_LOG_APPEND, [&] { k_log_append<<<1024, kBlock, 0, c->stream>>>(c->s); });
 }
 
 void enqueue_schema(glint_ctx *c)
@@ -1926,6 +1951,10 @@ glint_status glint_create(const glint_config *cfg, glint_ctx **out)
     ALLOC(c->tab[0], slots);
     ALLOC(c->tab[1], slots);
     ALLOC(c->own_xy, 2 * N);
+    if (cfg->meeting_log_capacity) {
+        ALLOC(s.mlog, cfg->meeting_log_capacity);
+        s.mlog_cap = cfg->meeting_log_capacity;
+    }
     if (cfg->kv_capacity) {
         ALLOC(c->mbr[0], slots);
         ALLOC(c->mbr[1], slots);
@@ -2141,10 +2170,10 @@ static void enqueue_tick_body(glint_ctx *c, uint64_t stride)
     k_stamp<<<1, 1, 0, c->stream>>>(c->s.tp);
 }
 
-glint_status glint_run_ticks(glint_ctx *c, const double *d_trace, uint32_t first_tick, uint32_t nticks,
-                             float *tick_ms)
+// Checks, stamp buffer, the pre-run state (k_begin_run) and, on the first
+// run, the tree build from the first snapshot.
+static glint_status run_begin(glint_ctx *c, const double *d_trace, uint32_t first_tick, uint32_t nticks)
 {
-    if (!c || !d_trace || nticks == 0) return GLINT_ERR_INVALID_ARGUMENT;
     if (c->has_positions && !c->evaluated) {
         c->err = "previous tick was not evaluated";
         return GLINT_ERR_STATE;
@@ -2154,7 +2183,6 @@ glint_status glint_run_ticks(glint_ctx *c, const double *d_trace, uint32_t first
         return GLINT_ERR_STATE;
     }
     HIP_TRY(c, hipSetDevice(c->device));
-    const uint64_t stride = 2ull * c->s.n;
     if (c->stamp_cap < nticks + 1) {
         if (c->stamps) HIP_TRY(c, hipFree(c->stamps));
         c->stamps = nullptr;
@@ -2173,6 +2201,42 @@ glint_status glint_run_ticks(glint_ctx *c, const double *d_trace, uint32_t first
         glint_status st = build_tree(c);
         if (st) return st;
     }
+    return GLINT_OK;
+}
+
+// Host view of the state after nticks enqueued ticks.
+static void run_end_state(glint_ctx *c, uint32_t first_tick, uint32_t nticks)
+{
+    for (uint32_t k = 0; k < nticks; k++) c->cur_gen = 1 - c->cur_gen;
+    c->prev_tick = first_tick + nticks - 1;
+    c->has_prev = true;
+    c->tick = first_tick + nticks - 1;
+    c->has_positions = true;
+    c->partitioned = c->refined = c->evaluated = true;
+}
+
+// Counters, per-tick latencies from the device stamps and the status.
+static glint_status run_finish(glint_ctx *c, uint32_t nticks, float *tick_ms)
+{
+    glint_status st = read_counters(c);
+    if (st) return st;
+    if (tick_ms) {
+        std::vector<unsigned long long> h(nticks + 1);
+        HIP_TRY(c, hipMemcpy(h.data(), c->stamps, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+        const double khz = c->wall_khz > 0 ? c->wall_khz : 100000.0;
+        for (uint32_t k = 0; k < nticks; k++) tick_ms[k] = (float)((double)(h[k + 1] - h[k]) / khz);
+    }
+    fill_stats(c);
+    return status_from_flags(c, c->h_ctr->error);
+}
+
+glint_status glint_run_ticks(glint_ctx *c, const double *d_trace, uint32_t first_tick, uint32_t nticks,
+                             float *tick_ms)
+{
+    if (!c || !d_trace || nticks == 0) return GLINT_ERR_INVALID_ARGUMENT;
+    glint_status st = run_begin(c, d_trace, first_tick, nticks);
+    if (st) return st;
+    const uint64_t stride = 2ull * c->s.n;
     const uint32_t G = c->cfg.graph_ticks;
     uint32_t done = 0;
     if (G && !c->prof.on && nticks >= G) {
@@ -2191,22 +2255,8 @@ glint_status glint_run_ticks(glint_ctx *c, const double *d_trace, uint32_t first
     }
     for (; done < nticks; done++) enqueue_tick_body(c, stride);
     HIP_TRY(c, hipGetLastError());
-    for (uint32_t k = 0; k < nticks; k++) c->cur_gen = 1 - c->cur_gen;
-    c->prev_tick = first_tick + nticks - 1;
-    c->has_prev = true;
-    c->tick = first_tick + nticks - 1;
-    c->has_positions = true;
-    c->partitioned = c->refined = c->evaluated = true;
-    glint_status st = read_counters(c);
-    if (st) return st;
-    if (tick_ms) {
-        std::vector<unsigned long long> h(nticks + 1);
-        HIP_TRY(c, hipMemcpy(h.data(), c->stamps, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
-        const double khz = c->wall_khz > 0 ? c->wall_khz : 100000.0;
-        for (uint32_t k = 0; k < nticks; k++) tick_ms[k] = (float)((double)(h[k + 1] - h[k]) / khz);
-    }
-    fill_stats(c);
-    return status_from_flags(c, c->h_ctr->error);
+    run_end_state(c, first_tick, nticks);
+    return run_finish(c, nticks, tick_ms);
 }
 
 glint_status glint_synchronize(glint_ctx *c)
@@ -2390,6 +2440,194 @@ glint_status glint_totals_get(glint_ctx *c, glint_totals *out, int reset)
         HIP_TRY(c, hipStreamSynchronize(c->stream));
     }
     return status_from_flags(c, h.error);
+}
+
+glint_status glint_batch_destroy(glint_batch *b)
+{
+    if (!b) return GLINT_ERR_INVALID_ARGUMENT;
+    if (b->origin) (void)hipStreamSynchronize(b->origin);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    for (hipEvent_t e : b->join)
+        if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : b->ready)
+        if (e) (void)hipEventDestroy(e);
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    if (b->after) (void)hipEventDestroy(b->after);
+    for (glint_ctx *c : b->ctx) glint_destroy(c);
+    if (b->origin) (void)hipStreamDestroy(b->origin);
+    delete b;
+    return GLINT_OK;
+}
+
+glint_status glint_batch_create(const glint_config *cfgs, uint32_t n, uint32_t graph_ticks, glint_batch **out)
+{
+    if (!cfgs || !out || n == 0) return GLINT_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    glint_batch *b = new (std::nothrow) glint_batch();
+    if (!b) return GLINT_ERR_OUT_OF_MEMORY;
+    b->graph_ticks = graph_ticks;
+    auto fail = [&](glint_status st, const std::string &why) {
+        g_create_error = why;
+        glint_batch_destroy(b);
+        return st;
+    };
+    for (uint32_t i = 0; i < n; i++) {
+        if (cfgs[i].device != cfgs[0].device) return fail(GLINT_ERR_INVALID_ARGUMENT, "one device per batch");
+        glint_config c = cfgs[i];
+        c.graph_ticks = 0;   // the batch captures its own graph
+        glint_ctx *ctx = nullptr;
+        glint_status st = glint_create(&c, &ctx);
+        if (st) return fail(st, "scenario " + std::to_string(i) + ": " + g_create_error);
+        b->ctx.push_back(ctx);
+    }
+    if (hipSetDevice((int)cfgs[0].device) != hipSuccess ||
+        hipStreamCreateWithFlags(&b->origin, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&b->after, hipEventDisableTiming) != hipSuccess)
+        return fail(GLINT_ERR_HIP, "batch stream or event creation failed");
+    b->join.assign(n, nullptr);
+    b->ready.assign(n, nullptr);
+    for (uint32_t i = 0; i < n; i++)
+        if (hipEventCreateWithFlags(&b->join[i], hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&b->ready[i], hipEventDisableTiming) != hipSuccess)
+            return fail(GLINT_ERR_HIP, "batch event creation failed");
+    *out = b;
+    return GLINT_OK;
+}
+
+glint_status glint_batch_context(glint_batch *b, uint32_t i, glint_ctx **out)
+{
+    if (!b || !out || i >= b->ctx.size()) return GLINT_ERR_INVALID_ARGUMENT;
+    *out = b->ctx[i];
+    return GLINT_OK;
+}
+
+const char *glint_batch_last_error(const glint_batch *b) { return b ? b->err.c_str() : g_create_error.c_str(); }
+
+glint_status glint_batch_run_ticks(glint_batch *b, const double *const *d_traces, uint32_t first_tick,
+                                   uint32_t nticks, float *tick_ms)
+{
+    if (!b || !d_traces || nticks == 0) return GLINT_ERR_INVALID_ARGUMENT;
+    const uint32_t n = (uint32_t)b->ctx.size();
+    auto ctx_fail = [&](uint32_t i, glint_status st) {
+        b->err = "scenario " + std::to_string(i) + ": " + b->ctx[i]->err;
+        return st;
+    };
+#define BATCH_TRY(call)                                                        \
+    do {                                                                       \
+        hipError_t e_ = (call);                                                \
+        if (e_ != hipSuccess) {                                                \
+            b->err = std::string(#call) + ": " + hipGetErrorString(e_);        \
+            return GLINT_ERR_HIP;                                              \
+        }                                                                      \
+    } while (0)
+    for (uint32_t i = 0; i < n; i++) {
+        if (!d_traces[i]) return GLINT_ERR_INVALID_ARGUMENT;
+        if (b->ctx[i]->prof.on) b->ctx[i]->prof.on = false;
+        glint_status st = run_begin(b->ctx[i], d_traces[i], first_tick, nticks);
+        if (st) return ctx_fail(i, st);
+    }
+    BATCH_TRY(hipSetDevice(b->ctx[0]->device));
+    const uint32_t G = b->graph_ticks;
+    uint32_t done = 0;
+    if (G && nticks >= G) {
+        if (!b->exec) {
+            hipGraph_t graph = nullptr;
+            BATCH_TRY(hipStreamBeginCapture(b->origin, hipStreamCaptureModeThreadLocal));
+            BATCH_TRY(hipEventRecord(b->fork, b->origin));
+            for (uint32_t i = 0; i < n; i++) {
+                glint_ctx *c = b->ctx[i];
+                BATCH_TRY(hipStreamWaitEvent(c->stream, b->fork, 0));
+                for (uint32_t k = 0; k < G; k++) enqueue_tick_body(c, 2ull * c->s.n);
+                BATCH_TRY(hipEventRecord(b->join[i], c->stream));
+                BATCH_TRY(hipStreamWaitEvent(b->origin, b->join[i], 0));
+            }
+            BATCH_TRY(hipStreamEndCapture(b->origin, &graph));
+            hipError_t e = hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (e != hipSuccess) {
+                b->err = std::string("hipGraphInstantiate: ") + hipGetErrorString(e);
+                return GLINT_ERR_HIP;
+            }
+        }
+        // the graph runs on the origin stream after every scenario's pre-run
+        // launches, and the scenarios' streams continue after it
+        for (uint32_t i = 0; i < n; i++) {
+            BATCH_TRY(hipEventRecord(b->ready[i], b->ctx[i]->stream));
+            BATCH_TRY(hipStreamWaitEvent(b->origin, b->ready[i], 0));
+        }
+        for (; done + G <= nticks; done += G) BATCH_TRY(hipGraphLaunch(b->exec, b->origin));
+        BATCH_TRY(hipEventRecord(b->after, b->origin));
+        for (uint32_t i = 0; i < n; i++) BATCH_TRY(hipStreamWaitEvent(b->ctx[i]->stream, b->after, 0));
+    }
+    for (; done < nticks; done++)
+        for (uint32_t i = 0; i < n; i++) enqueue_tick_body(b->ctx[i], 2ull * b->ctx[i]->s.n);
+    BATCH_TRY(hipGetLastError());
+#undef BATCH_TRY
+    glint_status first_err = GLINT_OK;
+    std::vector<std::vector<unsigned long long>> stamps(n);
+    for (uint32_t i = 0; i < n; i++) {
+        glint_ctx *c = b->ctx[i];
+        run_end_state(c, first_tick, nticks);
+        glint_status st = run_finish(c, nticks, nullptr);
+        if (st && !first_err) first_err = ctx_fail(i, st);
+        if (tick_ms) {
+            stamps[i].resize(nticks + 1);
+            if (hipMemcpy(stamps[i].data(), c->stamps, (nticks + 1) * sizeof(unsigned long long),
+                          hipMemcpyDeviceToHost) != hipSuccess && !first_err) {
+                b->err = "stamp copy failed";
+                first_err = GLINT_ERR_HIP;
+            }
+        }
+    }
+    if (tick_ms && !first_err) {
+        // tick k of the batch ends when its last scenario ends tick k
+        const double khz = b->ctx[0]->wall_khz > 0 ? b->ctx[0]->wall_khz : 100000.0;
+        unsigned long long prev = 0;
+        for (uint32_t i = 0; i < n; i++) prev = std::max(prev, stamps[i][0]);
+        for (uint32_t k = 0; k < nticks; k++) {
+            unsigned long long end = 0;
+            for (uint32_t i = 0; i < n; i++) end = std::max(end, stamps[i][k + 1]);
+            tick_ms[k] = (float)((double)(end - prev) / khz);
+            prev = end;
+        }
+    }
+    return first_err;
+}
+
+glint_status glint_meeting_log_info(glint_ctx *c, uint64_t *total, uint64_t *first_retained)
+{
+    if (!c || !total || !first_retained) return GLINT_ERR_INVALID_ARGUMENT;
+    if (!c->s.mlog) {
+        c->err = "meeting log is off (meeting_log_capacity = 0)";
+        return GLINT_ERR_STATE;
+    }
+    HIP_TRY(c, hipSetDevice(c->device));
+    glint_status st = read_counters(c);
+    if (st) return st;
+    *total = c->h_ctr->n_mlog;
+    *first_retained = *total > c->s.mlog_cap ? *total - c->s.mlog_cap : 0;
+    return GLINT_OK;
+}
+
+glint_status glint_meeting_log_read(glint_ctx *c, uint64_t from, glint_meeting *out, uint64_t cap,
+                                    uint64_t *n_out)
+{
+    if (!c || !n_out || (cap && !out)) return GLINT_ERR_INVALID_ARGUMENT;
+    uint64_t total = 0, first = 0;
+    glint_status st = glint_meeting_log_info(c, &total, &first);
+    if (st) return st;
+    if (from < first || from > total) {
+        c->err = "meeting log range no longer retained (or past its end)";
+        return GLINT_ERR_INVALID_ARGUMENT;
+    }
+    const uint64_t n = std::min<uint64_t>(total - from, cap);
+    const uint64_t L = c->s.mlog_cap, a = from % L;
+    const uint64_t n1 = std::min<uint64_t>(n, L - a);
+    if (n1) HIP_TRY(c, hipMemcpy(out, c->s.mlog + a, n1 * sizeof(glint_meeting), hipMemcpyDeviceToHost));
+    if (n > n1) HIP_TRY(c, hipMemcpy(out + n1, c->s.mlog, (n - n1) * sizeof(glint_meeting), hipMemcpyDeviceToHost));
+    *n_out = n;
+    return GLINT_OK;
 }
 
 glint_status glint_kv_pending(glint_ctx *c, uint64_t *keys)

@@ -91,6 +91,7 @@ struct Counters {
     unsigned long long max_distinct;
     unsigned long long max_meet;
     unsigned long long n_kvlog;     // meetings logged for the key-value output since its last flush
+    unsigned long long n_mlog;      // meetings ever appended to the meeting log
 };
 
 // Per-tick values, kept in device memory so a tick's launches can be
@@ -177,6 +178,8 @@ struct DevState {
     uint32_t bucket_blocks;        // blocks of the first pass
     KvLogEntry *kvlog;             // key-value output log (null when off)
     float2 *pos32;                 // float positions of the tick, pid order (key-value output)
+    glint_meeting *mlog;           // meeting log ring (null when off)
+    unsigned long long mlog_cap;
     unsigned long long kvlog_cap;
     Counters *ctr;
 };

@@ -158,3 +158,19 @@ class Generator:
     def state(self) -> dict:
         return {"xy": self.xy.copy(), "speed": self.speed.copy(), "tx": self.tx.copy(),
                 "ty": self.ty.copy(), "dir": self.dir.copy()}
+
+
+def batched_cities(count: int = 32, ticks: int = 200, seed: int = 4, n_min: int = 50_000,
+                   n_max: int = 2_000_000, grid_capacity: int = 200) -> list[Workload]:
+    """Independent cities of uneven size: N log-uniform in [n_min, n_max],
+    side sqrt(N)*30 m (whole blocks), reach in {1.5, 2, 3} m, minimum
+    meeting in {3, 5, 10} ticks, one seed per city."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        n = int(math.floor(math.exp(rng.uniform(math.log(n_min), math.log(n_max)))))
+        r = float(rng.choice([1.5, 2.0, 3.0]))
+        m = int(rng.choice([3, 5, 10]))
+        out.append(Workload(f"city_{i}", n, ticks, city_side(n), seed=1000 * seed + i, reach_distance=r,
+                            min_meet_time=m, grid_capacity=grid_capacity))
+    return out

@@ -71,6 +71,7 @@ class Config(ctypes.Structure):
         ("kv_split_num", ctypes.c_uint32),
         ("kv_capacity", ctypes.c_uint64),
         ("kv_area_limit", ctypes.c_double),
+        ("meeting_log_capacity", ctypes.c_uint64),
     ]
 
 
@@ -138,6 +139,13 @@ SIGNATURES = {
     "glint_active_pairs": (_I, [_VP, _VP, _VP, _U64, ctypes.POINTER(_U64)]),
     "glint_device_meetings": (_I, [_VP, ctypes.POINTER(_VP), ctypes.POINTER(_U64)]),
     "glint_debug_inject_failure": (_I, [_VP, ctypes.c_char_p]),
+    "glint_batch_create": (_I, [ctypes.POINTER(Config), _U32, _U32, ctypes.POINTER(_VP)]),
+    "glint_batch_destroy": (_I, [_VP]),
+    "glint_batch_context": (_I, [_VP, _U32, ctypes.POINTER(_VP)]),
+    "glint_batch_last_error": (ctypes.c_char_p, [_VP]),
+    "glint_batch_run_ticks": (_I, [_VP, _VP, _U32, _U32, _VP]),
+    "glint_meeting_log_info": (_I, [_VP, ctypes.POINTER(_U64), ctypes.POINTER(_U64)]),
+    "glint_meeting_log_read": (_I, [_VP, _U64, _VP, _U64, ctypes.POINTER(_U64)]),
     "glint_kv_pending": (_I, [_VP, ctypes.POINTER(_U64)]),
     "glint_kv_flush": (_I, [_VP, ctypes.POINTER(KvSummary)]),
     "glint_kv_part_size": (_I, [_VP, _U32, ctypes.POINTER(_U64)]),
@@ -232,7 +240,7 @@ def make_config(num_objects: int, box, *, reach_distance: float = 2.0, grid_capa
                 meeting_capacity: int = 0, node_capacity: int = 0, record_contacts: bool = False,
                 contact_capacity: int = 0, device: int = 0, max_meet_time: int = 4096,
                 graph_ticks: int = 0, kv_capacity: int = 0, kv_split_num: int = 10,
-                kv_area_limit: float = 100.0) -> Config:
+                kv_area_limit: float = 100.0, meeting_log_capacity: int = 0) -> Config:
     c = Config()
     lib().glint_config_default(ctypes.byref(c))
     c.num_objects = num_objects
@@ -258,25 +266,28 @@ def make_config(num_objects: int, box, *, reach_distance: float = 2.0, grid_capa
     c.kv_capacity = kv_capacity
     c.kv_split_num = kv_split_num
     c.kv_area_limit = kv_area_limit
+    c.meeting_log_capacity = meeting_log_capacity
     return c
 
 
 class Workbench:
     """One device context; the four stage methods of the reference's workbench."""
 
-    def __init__(self, cfg: Config):
+    def __init__(self, cfg: Config, handle: ctypes.c_void_p | None = None):
         self.cfg = cfg
         self.n = cfg.num_objects
-        h = ctypes.c_void_p()
-        _check(lib().glint_create(ctypes.byref(cfg), ctypes.byref(h)), "glint_create")
-        self.h = h
+        self._owned = handle is None
+        if handle is None:
+            handle = ctypes.c_void_p()
+            _check(lib().glint_create(ctypes.byref(cfg), ctypes.byref(handle)), "glint_create")
+        self.h = handle
         self._meet_cap = max(1024, 2 * self.n)
         self._meet_buf = (Meeting * self._meet_cap)()
 
     def close(self) -> None:
-        if self.h and self.h.value:
+        if self._owned and self.h and self.h.value:
             lib().glint_destroy(self.h)
-            self.h = ctypes.c_void_p()
+        self.h = ctypes.c_void_p()
 
     def __del__(self):
         try:
@@ -409,6 +420,27 @@ class Workbench:
                   "glint_kernel_times")
         return {names[k].decode(): (ms[k], cnt[k]) for k in range(min(n.value, cap))}
 
+    def meeting_log_info(self) -> tuple[int, int]:
+        """(meetings ever logged, index of the oldest one retained)."""
+        tot, first = ctypes.c_uint64(), ctypes.c_uint64()
+        _check(lib().glint_meeting_log_info(self.h, ctypes.byref(tot), ctypes.byref(first)),
+               "glint_meeting_log_info", self.h)
+        return int(tot.value), int(first.value)
+
+    def meeting_log(self, start: int | None = None, sort: bool = True) -> np.ndarray:
+        """Retained log entries from `start` (default: the oldest retained) as
+        an (m, 4) array of (pid_a, pid_b, start, end), sorted if asked."""
+        tot, first = self.meeting_log_info()
+        start = first if start is None else start
+        out = np.zeros((max(tot - start, 1), 4), dtype=np.uint32)
+        n = ctypes.c_uint64()
+        _check(lib().glint_meeting_log_read(self.h, start, out.ctypes.data, tot - start, ctypes.byref(n)),
+               "glint_meeting_log_read", self.h)
+        out = out[:n.value]
+        if sort and len(out):
+            out = out[np.lexsort((out[:, 3], out[:, 2], out[:, 1], out[:, 0]))]
+        return out
+
     def kv_pending(self) -> int:
         """Directed keys logged since the last key-value flush."""
         n = ctypes.c_uint64()
@@ -511,3 +543,53 @@ class Tracer:
 
     def close(self):
         self.bench.close()
+
+
+class Batch:
+    """Independent scenarios ticked together on one device (glint_batch_*):
+    one context per scenario; with graph_ticks, that many ticks of every
+    scenario are replayed as one captured graph."""
+
+    def __init__(self, cfgs: list, graph_ticks: int = 0):
+        arr = (Config * len(cfgs))(*cfgs)
+        h = ctypes.c_void_p()
+        st = lib().glint_batch_create(arr, len(cfgs), graph_ticks, ctypes.byref(h))
+        if st != GLINT_OK:
+            msg = lib().glint_batch_last_error(None)
+            raise GlintError(st, "glint_batch_create", msg.decode() if msg else "")
+        self.h = h
+        self.cfgs = list(cfgs)
+        self.contexts = []
+        for i, c in enumerate(cfgs):
+            ch = ctypes.c_void_p()
+            _check(lib().glint_batch_context(self.h, i, ctypes.byref(ch)), "glint_batch_context")
+            self.contexts.append(Workbench(c, handle=ch))
+
+    def run_ticks(self, d_traces: list, first_tick: int, nticks: int, latencies: bool = True):
+        ptrs = (ctypes.c_void_p * len(d_traces))(*[ctypes.c_void_p(p) for p in d_traces])
+        ms = np.zeros(nticks, dtype=np.float32) if latencies else None
+        st = lib().glint_batch_run_ticks(self.h, ptrs, first_tick, nticks, ms.ctypes.data if latencies else None)
+        if st != GLINT_OK:
+            msg = lib().glint_batch_last_error(self.h)
+            raise GlintError(st, "glint_batch_run_ticks", msg.decode() if msg else "")
+        return ms
+
+    def close(self) -> None:
+        for c in self.contexts:
+            c.close()
+        self.contexts = []
+        if self.h and self.h.value:
+            lib().glint_batch_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -86,6 +86,7 @@ typedef struct glint_config {
     uint32_t kv_split_num;         /* config.h:60: sorted key-value output in kv_split_num^2 files */
     uint64_t kv_capacity;          /* directed keys held between flushes (2 per meeting); 0: output off */
     double kv_area_limit;          /* dProbFalse (process.cpp:408-411): object box area, squared input units */
+    uint64_t meeting_log_capacity; /* meetings kept in the device meeting log ring; 0: log off */
 } glint_config;
 
 /* One emitted meeting: pid_a < pid_b, first and last tick of contact. */
@@ -231,6 +232,39 @@ glint_status glint_device_meetings(glint_ctx *ctx, const glint_meeting **d_out, 
  * "evaluate", "update_schema", "alloc") to fail with GLINT_ERR_INJECTED, for
  * failure-path tests. */
 glint_status glint_debug_inject_failure(glint_ctx *ctx, const char *stage);
+
+/* ----------------------------------------------------------------- batches
+ * Many independent scenarios (cities of uneven size, each with its own
+ * configuration) ticked together on one device: each scenario is a context
+ * of its own, with its own stream and state, so its results are those of
+ * the same context run alone.  With graph_ticks > 0, graph_ticks ticks of
+ * every scenario are captured once in one hipGraph (the scenarios as
+ * parallel branches) and replayed.  The reference runs one scenario per
+ * process (process.cpp:378-634). */
+typedef struct glint_batch glint_batch;
+
+glint_status glint_batch_create(const glint_config *cfgs, uint32_t n, uint32_t graph_ticks, glint_batch **out);
+glint_status glint_batch_destroy(glint_batch *batch);
+/* The context of scenario i (owned by the batch): every per-context query
+ * (glint_stats, glint_active_pairs, glint_meeting_log_read, ...) works on it. */
+glint_status glint_batch_context(glint_batch *batch, uint32_t i, glint_ctx **out);
+const char *glint_batch_last_error(const glint_batch *batch);
+/* nticks ticks of every scenario: scenario i reads tick k at
+ * d_traces[i] + 2*N_i*k (device memory, FP64 AoS).  tick_ms[k] (optional):
+ * device wall time until the last scenario finished tick k. */
+glint_status glint_batch_run_ticks(glint_batch *batch, const double *const *d_traces, uint32_t first_tick,
+                                   uint32_t nticks, float *tick_ms);
+
+/* ------------------------------------------------------------ meeting log
+ * With meeting_log_capacity > 0 every emitted meeting is appended to a
+ * device-resident ring (the role of the reference's per-tick meeting buffer
+ * h_meetings_ps, meeting.cpp:20-22, kept across ticks); the newest
+ * meeting_log_capacity meetings are retained.  Entries are in emission order
+ * (ticks in order; within a tick unordered). */
+glint_status glint_meeting_log_info(glint_ctx *ctx, uint64_t *total, uint64_t *first_retained);
+/* Copy entries [from, total) (at most cap) to host memory; from must be retained. */
+glint_status glint_meeting_log_read(glint_ctx *ctx, uint64_t from, glint_meeting *out, uint64_t cap,
+                                    uint64_t *n_out);
 
 /* ---------------------------------------------------- sorted key-value output
  * The flush branch of the reference's GPU path (geometry.cu:1752-2138).  With
