@@ -4,8 +4,11 @@ A step is one tick of the path (partition -> refine -> evaluate -> schema
 update) over every object.  Default workload: the skewed-density scenario,
 10M objects in a 30 km box, 80% starting in 64 Gaussian hotspots (sigma 200 m,
 Zipf s=1.1 weights), 20% uniform, all walking a 50 m Manhattan grid at
-U[1,15) m/s; reach 2 m, minimum meeting 5 ticks, leaf capacity 200.  Positions
+U[1,15) m/s; reach 2 m, minimum meeting 5 ticks; FP32 guard-band test with
+FP64 recheck, 8 ticks per captured hipGraph, leaf capacity, join tile and
+table load factor from the autotuner (tuned.json, or --autotune).  Positions
 for every tick are generated on the device before the timed region starts.
+--workload batched times 32 independent cities of uneven size in one batch.
 
 Prints one JSON line (rank 0) with object-updates/s, p50/p99 tick latency,
 the roofline figure of the dominant kernel (HIP events around every launch
@@ -120,20 +123,41 @@ def cpu_baseline(w, ticks: int, threads: int) -> dict:
     return {"seconds": tt, "ticks": ticks, "tick0_seconds": t_build, "contacts_last_tick": st["num_contacts"]}
 
 
+WORKLOAD_SPEC = {
+    # config 3: FP32 guard band + FP64 recheck, 8 ticks per graph, tuned leaf capacity
+    "skewed": {"precision": "fp32guard", "graph_ticks": 8, "tuned": "skewed_10m"},
+    # config 2: FP64 throughout, one call sequence per tick
+    "plain_gpu": {"precision": "fp64", "graph_ticks": 0, "tuned": None},
+    "plain_cpu": {"precision": "fp64", "graph_ticks": 0, "tuned": None},
+}
+
+
+def distinct_estimate(workload: str, n: int) -> float:
+    # distinct pairs per tick grow with the square of the hotspot density:
+    # ~65 per object at 10M skewed (measured), ~0.06 per object for uniform walkers
+    return {"skewed": 70.0 * n / 1e7, "plain_gpu": 0.1, "plain_cpu": 0.1, "batched": 0.1}[workload] * n
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--steps", type=int, default=96)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="skewed", choices=["skewed", "plain_gpu", "plain_cpu"])
+    ap.add_argument("--workload", default="skewed", choices=["skewed", "plain_gpu", "plain_cpu", "batched"])
     ap.add_argument("--objects", type=int, default=None)
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32guard"])
-    ap.add_argument("--buckets-log2", type=int, default=None)
+    ap.add_argument("--precision", default=None, choices=["fp64", "fp32guard"],
+                    help="default: the workload's configuration (skewed: fp32guard; plain forms: fp64)")
+    ap.add_argument("--graph-ticks", type=int, default=None, help="ticks per captured hipGraph (default: per workload)")
+    ap.add_argument("--grid-capacity", type=int, default=None, help="leaf capacity (default: tuned or 200)")
+    ap.add_argument("--join-tile", type=int, default=None, choices=[256, 512])
+    ap.add_argument("--load-factor", type=float, default=None)
+    ap.add_argument("--autotune", action="store_true", help="tune leaf capacity, join tile, load factor and "
+                                                             "precision on this workload before timing")
+    ap.add_argument("--save-tuned", action="store_true", help="store the autotuned parameters in tuned.json")
+    ap.add_argument("--profile-ticks", type=int, default=3,
+                    help="ticks run after the timed region with per-kernel HIP events (roofline)")
     ap.add_argument("--cpu-ticks", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-kernel-times", action="store_true")
-    ap.add_argument("--graph-ticks", type=int, default=0,
-                    help="ticks per captured hipGraph in the timed region (disables per-kernel events)")
     ap.add_argument("--compare-forms", action="store_true",
                     help="also time the plain per-tick FP64 form (host positions, one call sequence per tick) "
                          "against the batched guard-band form on the same inputs")
@@ -147,46 +171,64 @@ def main() -> int:
                           "n_gpus": args.gpus}))
         return 2
 
-    from glint_pipeline_amd import gen, runtime
+    from glint_pipeline_amd import autotune, gen, runtime
 
     if runtime.device_count() < 1:
         raise SystemExit("no HIP device visible")
+    if args.workload == "batched":
+        return bench_batched(args)
     w, desc = workload_for(args.workload, args.objects)
     N = w.num_objects
-    W, K = args.warmup, args.steps
-    precision = runtime.PRECISION_FP32_GUARD if args.precision == "fp32guard" else runtime.PRECISION_FP64
-    # contacts per tick grow with the square of the hotspot density: ~65 per
-    # object at 10M skewed (measured), ~0.06 per object for uniform walkers
-    per_obj = {"skewed": 70.0 * N / 1e7, "plain_gpu": 0.1, "plain_cpu": 0.1}[args.workload]
-    est_contacts = max(1 << 20, int(per_obj * N))
-    if args.buckets_log2 is not None:
-        slots = 1 << args.buckets_log2
-    else:
-        # one generation of distinct pairs at a load factor of at most ~0.75
-        # (probing stays inside 2048-slot regions held in LDS)
-        slots = 1 << max(20, math.ceil(math.log2(est_contacts / 0.75)))
-    log(f"[bench] workload: {desc}; warmup {W}, timed {K} ticks; table slots 2^{int(math.log2(slots))}")
+    W, K, P = args.warmup, args.steps, max(0, args.profile_ticks)
+    spec = WORKLOAD_SPEC[args.workload]
+    params = autotune.params_for(spec["tuned"]) if spec["tuned"] else dict(autotune.DEFAULTS)
+    params["precision"] = spec["precision"]     # prescribed by the configuration, not tuned
+    if spec["tuned"] and "grid_capacity" not in params:
+        params["grid_capacity"] = w.grid_capacity
+    graph_ticks = spec["graph_ticks"] if args.graph_ticks is None else args.graph_ticks
+    distinct = distinct_estimate(args.workload, N)
 
     # ---- inputs: host init of tick 0, device walk for the rest, all in HBM
     t0 = time.perf_counter()
     g = gen.Generator(w)
     dw = runtime.DeviceWalkers(g)
-    total_ticks = W + K
+    total_ticks = W + K + P
     trace = runtime.DeviceBuffer(16 * N * total_ticks)
     trace.upload(g.positions(), 0)
     if total_ticks > 1:
         dw.walk(total_ticks - 1, trace, 16 * N)
     log(f"[bench] generated {total_ticks} ticks of positions on the device in {time.perf_counter() - t0:.1f} s")
 
-    def make_cfg(prec: int, graph: int):
-        return runtime.make_config(N, w.box(), reach_distance=w.reach_distance, grid_capacity=w.grid_capacity,
-                                   min_meet_time=w.min_meet_time, precision=prec, num_meeting_buckets=slots,
-                                   unit_capacity=8 * N, meeting_capacity=max(1 << 20, 6 * N),
-                                   contact_capacity=int(est_contacts * 1.4), graph_ticks=graph)
+    tuning = None
+    if args.autotune:
+        t0 = time.perf_counter()
+        res = autotune.tune(w, tune_capacity=bool(spec["tuned"]), warmup=min(3, W), ticks=4, trace=trace,
+                            distinct_hint=distinct, fixed={"precision": spec["precision"]}, log=log)
+        params.update(res.best)
+        tuning = {"best": res.best, "best_ms_per_tick": round(res.best_ms, 3), "trials": res.trials,
+                  "seconds": round(time.perf_counter() - t0, 1)}
+        if args.save_tuned and spec["tuned"]:
+            table = autotune.load_table()
+            table[spec["tuned"]] = {"params": res.best, "ms_per_tick": round(res.best_ms, 3),
+                                    "workload": desc, "trials": res.trials}
+            autotune.save_table(table)
+    for k, v in (("precision", args.precision), ("grid_capacity", args.grid_capacity), ("join_tile", args.join_tile),
+                 ("load_factor", args.load_factor)):
+        if v is not None:
+            params[k] = v
+    precision = runtime.PRECISION_FP32_GUARD if params["precision"] == "fp32guard" else runtime.PRECISION_FP64
+    w.grid_capacity = params.get("grid_capacity", w.grid_capacity)
+    desc = desc.replace("leaf capacity 200", f"leaf capacity {w.grid_capacity}")
 
-    if args.graph_ticks:
-        args.no_kernel_times = True
-    cfg = make_cfg(precision, args.graph_ticks)
+    def make_cfg(prec: int, graph: int, **extra):
+        p = dict(params)
+        p["precision"] = "fp32guard" if prec == runtime.PRECISION_FP32_GUARD else "fp64"
+        return autotune.config_for(w, p, distinct, graph_ticks=graph, **extra)
+
+    cfg = make_cfg(precision, graph_ticks)
+    slots = int(cfg.num_meeting_buckets)
+    log(f"[bench] workload: {desc}; params {params}; graph ticks {graph_ticks}; warmup {W}, timed {K}, "
+        f"profiled {P}; table slots 2^{int(math.log2(slots))}")
     wb = runtime.Workbench(cfg)
     t0 = time.perf_counter()
     if W:
@@ -194,9 +236,6 @@ def main() -> int:
     wb.synchronize()
     log(f"[bench] warmup ({W} ticks incl. tree build) {time.perf_counter() - t0:.2f} s; stats {wb.stats()}")
     wb.totals(reset=True)
-    if not args.no_kernel_times:
-        wb.set_kernel_profiling(True)
-        wb.kernel_times(reset=True)
 
     # ---- timed region
     wb.synchronize()
@@ -205,12 +244,23 @@ def main() -> int:
     wb.synchronize()
     elapsed = time.perf_counter() - t0
     totals = wb.totals()
-    ktimes = wb.kernel_times() if not args.no_kernel_times else {}
     stats = wb.stats()
     log(f"[bench] timed {K} ticks in {elapsed:.3f} s; totals {totals}")
-    for name, (ms, cnt) in sorted(ktimes.items(), key=lambda kv: -kv[1][0]):
-        if cnt:
-            log(f"[bench]   {name:16s} {ms:10.2f} ms total  {cnt:6d} launches  {ms / cnt:9.3f} ms/launch")
+
+    # ---- per-kernel times on a few more ticks (HIP events around every launch)
+    ktimes = {}
+    if P:
+        wb.set_kernel_profiling(True)
+        wb.kernel_times(reset=True)
+        wb.run_ticks(trace.address(16 * N * (W + K)), W + K, P, latencies=False)
+        wb.synchronize()
+        ktimes = wb.kernel_times()
+        wb.set_kernel_profiling(False)
+        ptick = sum(ms for ms, cnt in ktimes.values()) / P
+        for name, (ms, cnt) in sorted(ktimes.items(), key=lambda kv: -kv[1][0]):
+            if cnt:
+                log(f"[bench]   {name:16s} {ms / P:9.3f} ms/tick  {cnt // P:4d} launches/tick  {ms / cnt:9.3f} ms/launch")
+        log(f"[bench]   kernels sum {ptick:.3f} ms/tick (profiled, no graph)")
 
     value = N * K / elapsed
     per_tick = {k: totals[k] / max(1, totals["ticks"]) for k in
@@ -225,7 +275,7 @@ def main() -> int:
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom,
                     "kernel_ms_per_launch": round(dms / dcnt, 4), "algorithmic_bytes_per_launch": abytes,
-                    "kernel_share_of_tick": round(dms / (elapsed * 1e3), 4)}
+                    "kernel_share_of_tick": round(dms / P / (elapsed * 1e3 / K), 4)}
 
     forms = None
     if args.compare_forms:
@@ -270,10 +320,8 @@ def main() -> int:
         if not args.compare_forms:
             wb.close()
         restriction = 2 * 33554432                 # kv_restriction at G_bytes = 2 (config.h:53,80)
-        kcfg = make_cfg(precision, args.graph_ticks)
-        kcfg.kv_capacity = 2 * restriction
-        kcfg.kv_area_limit = 1e30                  # keep every object in the files
-        wk = runtime.Workbench(kcfg)
+        wk = runtime.Workbench(make_cfg(precision, graph_ticks, kv_capacity=2 * restriction,
+                                        kv_area_limit=1e30))   # keep every object in the files
         wk.run_ticks(trace.address(0), 0, W, latencies=False)
         wk.synchronize()
         flush_ms, keys = [], []
@@ -324,12 +372,14 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "f64" if precision == runtime.PRECISION_FP64 else "f32+f64",
         "data": "synthetic (seeded generator, positions resident in HBM before timing)",
-        "config": {"workload": desc, "objects": N, "first_tick": W, "ticks": K,
-                   "table_slots": slots, "contacts_per_tick": round(per_tick["contacts"]),
+        "config": {"workload": desc, "objects": N, "first_tick": W, "ticks": K, "graph_ticks": graph_ticks,
+                   "params": params, "table_slots": slots, "contacts_per_tick": round(per_tick["contacts"]),
                    "meetings_per_tick": round(per_tick["meetings"], 1), "leaves": stats["num_leaves"]},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    if tuning:
+        out["autotune"] = tuning
     if forms:
         out["forms"] = forms
     if kv:
@@ -338,6 +388,81 @@ def main() -> int:
     if not args.compare_forms and not args.kv:
         wb.close()
     trace.free()
+    return 0
+
+
+def bench_batched(args) -> int:
+    """Config 4: 32 independent cities of uneven size in one batch (one graph
+    replaying every city's ticks), against the same cities run one after the
+    other."""
+    from glint_pipeline_amd import autotune, gen, runtime
+    W, K = args.warmup, args.steps
+    G = 8 if args.graph_ticks is None else args.graph_ticks
+    ws = gen.batched_cities(32, ticks=W + K)
+    total = sum(w.num_objects for w in ws)
+    t0 = time.perf_counter()
+    if args.autotune:
+        # one representative (the largest) city per size class; capacity is part of the result, not tuned
+        table = autotune.load_table()
+        for cls in ("small", "medium", "large"):
+            members = [w for w in ws if autotune.size_class(w.num_objects) == cls]
+            if not members:
+                continue
+            rep = max(members, key=lambda w: w.num_objects)
+            res = autotune.tune(rep, tune_capacity=False, warmup=3, ticks=5, log=log)
+            table[cls] = {"params": res.best, "ms_per_tick": round(res.best_ms, 3),
+                          "workload": f"{rep.name}: {rep.num_objects} objects", "trials": res.trials}
+        if args.save_tuned:
+            autotune.save_table(table)
+    bufs, cfgs, classes = [], [], []
+    for w in ws:
+        g = gen.Generator(w)
+        b = runtime.DeviceBuffer(16 * w.num_objects * (W + K))
+        b.upload(g.positions(), 0)
+        runtime.DeviceWalkers(g).walk(W + K - 1, b, 16 * w.num_objects)
+        bufs.append(b)
+        cls = autotune.size_class(w.num_objects)
+        p = autotune.params_for(cls)
+        classes.append(cls)
+        cfgs.append(autotune.config_for(w, p, distinct_estimate("batched", w.num_objects), graph_ticks=G))
+    log(f"[bench] {len(ws)} cities, {total} objects, classes { {c: classes.count(c) for c in set(classes)} }; "
+        f"traces generated in {time.perf_counter() - t0:.1f} s")
+
+    # one batch: every city's ticks in one graph per G ticks
+    bt = runtime.Batch(cfgs, graph_ticks=G)
+    bt.run_ticks([b.address(0) for b in bufs], 0, W, latencies=False)
+    t0 = time.perf_counter()
+    ms = bt.run_ticks([b.address(16 * w.num_objects * W) for b, w in zip(bufs, ws)], W, K, latencies=True)
+    elapsed = time.perf_counter() - t0
+    meetings = sum(c.totals()["meetings"] for c in bt.contexts)
+    bt.close()
+    # the same cities one after the other, each with its own graph
+    seq = 0.0
+    for w, b, c in zip(ws, bufs, cfgs):
+        with runtime.Workbench(c) as wb:
+            wb.run_ticks(b.address(0), 0, W, latencies=False)
+            wb.synchronize()
+            t1 = time.perf_counter()
+            wb.run_ticks(b.address(16 * w.num_objects * W), W, K, latencies=False)
+            wb.synchronize()
+            seq += time.perf_counter() - t1
+    for b in bufs:
+        b.free()
+    lat = np.asarray(ms, dtype=np.float64).reshape(-1)   # every city's own ticks inside the batch
+    out = {
+        "metric": "object-updates/sec, 32 independent cities in one batch, 1 MI355X, exact contacts",
+        "value": round(total * K / elapsed, 1), "unit": "object-updates/s", "n_gpus": 1, "steps": K, "warmup": W,
+        "ms_per_step": round(elapsed * 1e3 / K, 3), "p50_tick_ms": round(float(np.percentile(lat, 50)), 3),
+        "p99_tick_ms": round(float(np.percentile(lat, 99)), 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic (seeded generator, positions resident in HBM)",
+        "config": {"workload": f"batched uneven cities: 32 cities, N log-uniform in [50k, 2M] ({total} objects), "
+                               f"side sqrt(N)*30 m, r in {{1.5,2,3}} m, min in {{3,5,10}}",
+                   "graph_ticks": G, "meetings": meetings,
+                   "sizes": [w.num_objects for w in ws]},
+        "sequential_ms_per_tick": round(seq * 1e3 / K, 3),
+        "batch_speedup": round(seq / elapsed, 3),
+    }
+    print(json.dumps(out), flush=True)
     return 0
 
 

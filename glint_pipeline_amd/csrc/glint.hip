@@ -36,7 +36,7 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kScanItems = 4;                       // per thread
 constexpr int kScanTile = kBlock * kScanItems;      // elements per scan block
-constexpr int kTile = 512;                          // points per LDS tile of the join
+constexpr int kTile = 512;                          // points per LDS tile of the join (default)
 constexpr int kStage = 512;                         // staged contacts per wave
 constexpr int kFlushRound = 4;                      // join columns between stage checks
 constexpr int kMeetStage = 256;                     // staged meetings per wave (table update)
@@ -778,15 +778,17 @@ __device__ __forceinline__ unsigned long long pair_key(uint32_t a, uint32_t b)
     return ((unsigned long long)lo << 32) | hi;
 }
 
+template <int TILE>
 struct JoinTile {
-    double x[kTile], y[kTile];     // positions at tick
-    double px[kTile], py[kTile];   // positions at prev_tick
-    float fx[kTile], fy[kTile];    // FP32 offsets from the leaf's low corner (guard mode)
-    uint32_t p[kTile];             // object ids
+    double x[TILE], y[TILE];       // positions at tick
+    double px[TILE], py[TILE];     // positions at prev_tick
+    float fx[TILE], fy[TILE];      // FP32 offsets from the leaf's low corner (guard mode)
+    uint32_t p[TILE];              // object ids
 };
 
+template <int TILE>
 struct JoinLds {
-    JoinTile a, b;
+    JoinTile<TILE> a, b;
     unsigned long long stage[kBlock / 64][kStage];
     uint32_t item;
 };
@@ -817,7 +819,8 @@ __device__ __forceinline__ bool pair_within(const PairTest &pt, double x1, doubl
     return d2 <= pt.t64;
 }
 
-__device__ __forceinline__ void load_tile(JoinTile &t, const DevState &s, uint32_t from, uint32_t n, bool has_prev,
+template <int TILE>
+__device__ __forceinline__ void load_tile(JoinTile<TILE> &t, const DevState &s, uint32_t from, uint32_t n, bool has_prev,
                                           bool use32, double ox, double oy)
 {
     for (uint32_t k = threadIdx.x; k < n; k += kBlock) {
@@ -846,9 +849,10 @@ __device__ __forceinline__ void load_tile(JoinTile &t, const DevState &s, uint32
 // Work is split into 64x64 wave tasks (64 row lanes x 64 broadcast columns).
 // A contact is staged with bit 63 set when the pair was also within reach
 // at prev_tick, the only case that needs the previous table generation.
+template <int TILE>
 __global__ __launch_bounds__(kBlock) void k_join(DevState s)
 {
-    __shared__ JoinLds lds;
+    __shared__ JoinLds<TILE> lds;
     const double *xy = s.tp->xy;
     const double *xy_prev = s.tp->xy_prev;
     const bool has_prev = s.tp->has_prev != 0;
@@ -868,20 +872,20 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
             const uint32_t leaf = s.leaf_list[w];
             const uint32_t off = s.offset[leaf], cnt = s.count[leaf];
             const uint32_t uoff = s.ustart[leaf], ucnt = s.ucount[leaf];
-            const uint32_t ntile = (cnt + kTile - 1) / kTile;
+            const uint32_t ntile = (cnt + TILE - 1) / TILE;
             const Node nd = s.nodes[leaf];
             pt.use32 = s.precision == GLINT_PRECISION_FP32_GUARD && (nd.hx - nd.lx) <= s.w32_max &&
                        (nd.hy - nd.ly) <= s.w32_max;
             for (uint32_t I = 0; I < ntile; I++) {
-                const uint32_t ni = min((uint32_t)kTile, cnt - I * kTile);
-                load_tile(lds.a, s, off + I * kTile, ni, has_prev, pt.use32, nd.lx, nd.ly);
+                const uint32_t ni = min((uint32_t)TILE, cnt - I * TILE);
+                load_tile(lds.a, s, off + I * TILE, ni, has_prev, pt.use32, nd.lx, nd.ly);
                 for (uint32_t J = I; J < ntile; J++) {
-                    const uint32_t nj = min((uint32_t)kTile, cnt - J * kTile);
+                    const uint32_t nj = min((uint32_t)TILE, cnt - J * TILE);
                     const bool diag = (J == I);
-                    const JoinTile &B = diag ? lds.a : lds.b;
-                    if (!diag) load_tile(lds.b, s, off + J * kTile, nj, has_prev, pt.use32, nd.lx, nd.ly);
+                    const JoinTile<TILE> &B = diag ? lds.a : lds.b;
+                    if (!diag) load_tile(lds.b, s, off + J * TILE, nj, has_prev, pt.use32, nd.lx, nd.ly);
                     __syncthreads();
-                    const JoinTile &A = lds.a;
+                    const JoinTile<TILE> &A = lds.a;
                     // inside tasks: 64-row sub-tile r against 64-column sub-tile q
                     const uint32_t nri = (ni + 63) / 64, nrj = (nj + 63) / 64;
                     const uint32_t ntask_in = diag ? nri * (nri + 1) / 2 : nri * nrj;
@@ -942,7 +946,7 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
                             jend = ni;
                             inside = false;
                         }
-                        const JoinTile &C = inside ? B : A;
+                        const JoinTile<TILE> &C = inside ? B : A;
                         for (uint32_t jb = jbeg; jb < jend; jb += kFlushRound) {
 #pragma unroll
                             for (uint32_t jj = 0; jj < kFlushRound; jj++) {
@@ -1679,7 +1683,10 @@ void enqueue_refine(glint_ctx *c)
 {
     DevState &s = c->s;
     hipStream_t st = c->stream;
-    launch(c, K_JOIN, [&] { k_join<<<c->join_blocks, kBlock, 0, st>>>(s); });
+    if (c->cfg.join_tile == 256)
+        launch(c, K_JOIN, [&] { k_join<256><<<c->join_blocks, kBlock, 0, st>>>(s); });
+    else
+        launch(c, K_JOIN, [&] { k_join<kTile><<<c->join_blocks, kBlock, 0, st>>>(s); });
     // group the tick's contacts by table region
     const uint32_t G1 = s.bucket_blocks;
     const uint32_t nc = (s.coarse_bins + 1) * G1, nb = (nc + kScanTile - 1) / kScanTile;
@@ -1803,7 +1810,8 @@ glint_status glint_create(const glint_config *cfg, glint_ctx **out)
     g_create_error.clear();
     if (cfg->num_objects == 0 || cfg->grid_capacity == 0 || !(cfg->reach_distance > 0) ||
         !(cfg->high_x > cfg->low_x) || !(cfg->high_y > cfg->low_y) || cfg->candidate_mode > 1 ||
-        cfg->precision > 1 || cfg->num_objects >= 0x80000000u) {
+        cfg->precision > 1 || cfg->num_objects >= 0x80000000u ||
+        (cfg->join_tile != 0 && cfg->join_tile != 256 && cfg->join_tile != 512)) {
         g_create_error = "invalid configuration";
         return GLINT_ERR_INVALID_ARGUMENT;
     }
@@ -2581,16 +2589,11 @@ glint_status glint_batch_run_ticks(glint_batch *b, const double *const *d_traces
         }
     }
     if (tick_ms && !first_err) {
-        // tick k of the batch ends when its last scenario ends tick k
+        // each scenario's own tick times (device wall clock between its stamps)
         const double khz = b->ctx[0]->wall_khz > 0 ? b->ctx[0]->wall_khz : 100000.0;
-        unsigned long long prev = 0;
-        for (uint32_t i = 0; i < n; i++) prev = std::max(prev, stamps[i][0]);
-        for (uint32_t k = 0; k < nticks; k++) {
-            unsigned long long end = 0;
-            for (uint32_t i = 0; i < n; i++) end = std::max(end, stamps[i][k + 1]);
-            tick_ms[k] = (float)((double)(end - prev) / khz);
-            prev = end;
-        }
+        for (uint32_t i = 0; i < n; i++)
+            for (uint32_t k = 0; k < nticks; k++)
+                tick_ms[(size_t)i * nticks + k] = (float)((double)(stamps[i][k + 1] - stamps[i][k]) / khz);
     }
     return first_err;
 }

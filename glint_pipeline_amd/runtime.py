@@ -72,6 +72,8 @@ class Config(ctypes.Structure):
         ("kv_capacity", ctypes.c_uint64),
         ("kv_area_limit", ctypes.c_double),
         ("meeting_log_capacity", ctypes.c_uint64),
+        ("join_tile", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
     ]
 
 
@@ -240,7 +242,7 @@ def make_config(num_objects: int, box, *, reach_distance: float = 2.0, grid_capa
                 meeting_capacity: int = 0, node_capacity: int = 0, record_contacts: bool = False,
                 contact_capacity: int = 0, device: int = 0, max_meet_time: int = 4096,
                 graph_ticks: int = 0, kv_capacity: int = 0, kv_split_num: int = 10,
-                kv_area_limit: float = 100.0, meeting_log_capacity: int = 0) -> Config:
+                kv_area_limit: float = 100.0, meeting_log_capacity: int = 0, join_tile: int = 0) -> Config:
     c = Config()
     lib().glint_config_default(ctypes.byref(c))
     c.num_objects = num_objects
@@ -267,6 +269,7 @@ def make_config(num_objects: int, box, *, reach_distance: float = 2.0, grid_capa
     c.kv_split_num = kv_split_num
     c.kv_area_limit = kv_area_limit
     c.meeting_log_capacity = meeting_log_capacity
+    c.join_tile = join_tile
     return c
 
 
@@ -567,7 +570,7 @@ class Batch:
 
     def run_ticks(self, d_traces: list, first_tick: int, nticks: int, latencies: bool = True):
         ptrs = (ctypes.c_void_p * len(d_traces))(*[ctypes.c_void_p(p) for p in d_traces])
-        ms = np.zeros(nticks, dtype=np.float32) if latencies else None
+        ms = np.zeros((len(d_traces), nticks), dtype=np.float32) if latencies else None
         st = lib().glint_batch_run_ticks(self.h, ptrs, first_tick, nticks, ms.ctypes.data if latencies else None)
         if st != GLINT_OK:
             msg = lib().glint_batch_last_error(self.h)

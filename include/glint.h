@@ -87,6 +87,8 @@ typedef struct glint_config {
     uint64_t kv_capacity;          /* directed keys held between flushes (2 per meeting); 0: output off */
     double kv_area_limit;          /* dProbFalse (process.cpp:408-411): object box area, squared input units */
     uint64_t meeting_log_capacity; /* meetings kept in the device meeting log ring; 0: log off */
+    uint32_t join_tile;            /* points per LDS tile of the distance join: 256 or 512 (0: 512) */
+    uint32_t reserved;
 } glint_config;
 
 /* One emitted meeting: pid_a < pid_b, first and last tick of contact. */
@@ -250,8 +252,9 @@ glint_status glint_batch_destroy(glint_batch *batch);
 glint_status glint_batch_context(glint_batch *batch, uint32_t i, glint_ctx **out);
 const char *glint_batch_last_error(const glint_batch *batch);
 /* nticks ticks of every scenario: scenario i reads tick k at
- * d_traces[i] + 2*N_i*k (device memory, FP64 AoS).  tick_ms[k] (optional):
- * device wall time until the last scenario finished tick k. */
+ * d_traces[i] + 2*N_i*k (device memory, FP64 AoS).  tick_ms (optional,
+ * n*nticks floats): tick_ms[i*nticks + k] is the device wall time of
+ * scenario i's tick k inside the batch. */
 glint_status glint_batch_run_ticks(glint_batch *batch, const double *const *d_traces, uint32_t first_tick,
                                    uint32_t nticks, float *tick_ms);
 

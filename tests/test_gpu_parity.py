@@ -46,6 +46,16 @@ def test_parity_hotspots_small():
     assert tot["contacts"] > 10_000
 
 
+@pytest.mark.parametrize("join_tile,capacity", [(256, 600), (512, 1024), (256, 64), (256, 256)])
+def test_parity_join_tiles_and_leaf_capacities(join_tile, capacity):
+    """Leaves larger than one LDS tile are joined tile by tile; every tile size
+    and leaf capacity gives the oracle's result at that capacity."""
+    w = gen.skewed_density(40_000, 12)
+    w.side = 3000.0
+    w.grid_capacity = capacity
+    run_parity(w, 12, join_tile=join_tile, precision=runtime.PRECISION_FP32_GUARD, check_leaves_every=6)
+
+
 def test_parity_plain_cpu_reference_config():
     # 100k objects, 10 km box, r=2, min=5, leaf capacity 200; 100 ticks
     w = gen.plain_cpu_reference()
