@@ -20,8 +20,8 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 INCLUDE = ROOT / "include"
 
-HIP_SOURCES = ["glint.hip", "walk.hip", "kv.hip"]
-HIP_HEADERS = ["glint_device.h", "kv_engine.h"]
+HIP_SOURCES = ["glint.hip", "walk.hip", "kv.hip", "serve.hip"]
+HIP_HEADERS = ["glint_device.h", "kv_engine.h", "ctx_view.h"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("GLINT_OFFLOAD_ARCH", "gfx950")

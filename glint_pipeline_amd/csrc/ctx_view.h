@@ -1,0 +1,25 @@
+// ctx_view.h - what the serving layer reads from a context: its stream, the
+// table generation holding the pairs active after the last evaluated tick,
+// and the meeting log ring.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "glint.h"
+
+struct CtxView {
+    hipStream_t stream;
+    int device;
+    uint32_t num_objects;
+    uint32_t tick;              // last evaluated tick
+    uint32_t min_meet;
+    bool evaluated;
+    const void *table;          // glint::Slot[slots]
+    uint64_t slots;
+    const glint_meeting *mlog;  // meeting log ring (null when off)
+    uint64_t mlog_cap;
+};
+
+extern "C" glint_status glint_ctx_view(glint_ctx *c, CtxView *v);

@@ -28,6 +28,7 @@
 #include "glint.h"
 #include "glint_device.h"
 #include "kv_engine.h"
+#include "ctx_view.h"
 
 using namespace glint;
 
@@ -1504,6 +1505,8 @@ struct glint_ctx {
     uint32_t scan_blocks = 0;
     uint32_t join_blocks = 1024;
     double *own_xy = nullptr;
+    const double *last_xy = nullptr;   // positions of the current tick (caller's or own_xy)
+    bool resident = false;             // own_xy holds the resident positions (update ingestion)
     Counters *h_ctr = nullptr;  // pinned
     std::vector<void *> allocs;
     std::string err;
@@ -1699,6 +1702,22 @@ void enqueue_refine(glint_ctx *c)
         launch(c, K_BUCKET_FINE, [&] { k_bucket_fine<<<s.coarse_bins, kBlock, 0, st>>>(s); });
     else
         launch(c, K_BUCKET_FINE, [&] { k_bucket_single<<<grid_for(s.coarse_bins + 1), kBlock, 0, st>>>(s); });
+}
+
+// Location updates scattered into the resident positions; pids out of
+// range flag bad input.
+__global__ __launch_bounds__(kBlock) void k_apply_updates(DevState s, double *xy, const uint32_t *pids,
+                                                          const double *upd, uint32_t k)
+{
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < k; i += gridDim.x * kBlock) {
+        const uint32_t p = pids[i];
+        if (p >= s.n) {
+            atomicOr(&s.ctr->error, kErrBadInput);
+            continue;
+        }
+        xy[2 * (size_t)p] = upd[2 * (size_t)i];
+        xy[2 * (size_t)p + 1] = upd[2 * (size_t)i + 1];
+    }
 }
 
 // Float copy of this tick's positions (pid order) for the pair boxes.
@@ -2029,6 +2048,8 @@ static glint_status begin_tick(glint_ctx *c, const double *d_xy, uint32_t tick)
     }
     c->tick = tick;
     set_tick_params(c, d_xy);
+    c->last_xy = d_xy;
+    c->resident = d_xy == c->own_xy && c->resident;
     c->has_positions = true;
     c->partitioned = c->refined = c->evaluated = false;
     return GLINT_OK;
@@ -2047,6 +2068,49 @@ glint_status glint_set_positions_device(glint_ctx *c, const double *d_xy, uint32
     if (!c || !d_xy || n != c->s.n) return GLINT_ERR_INVALID_ARGUMENT;
     HIP_TRY(c, hipSetDevice(c->device));
     return begin_tick(c, d_xy, tick);
+}
+
+glint_status glint_ingest_updates(glint_ctx *c, const uint32_t *d_pids, const double *d_xy, uint32_t k,
+                                  uint32_t tick)
+{
+    if (!c || (k && (!d_pids || !d_xy))) return GLINT_ERR_INVALID_ARGUMENT;
+    if (!c->last_xy) {
+        c->err = "ingestion needs a first tick with every position";
+        return GLINT_ERR_STATE;
+    }
+    if (c->has_positions && !c->evaluated) {
+        c->err = "previous tick was not evaluated";
+        return GLINT_ERR_STATE;
+    }
+    HIP_TRY(c, hipSetDevice(c->device));
+    if (!c->resident) {
+        // the last tick's positions become the resident copy
+        if (c->last_xy != c->own_xy)
+            HIP_TRY(c, hipMemcpyAsync(c->own_xy, c->last_xy, (size_t)c->s.n * 16, hipMemcpyDeviceToDevice,
+                                      c->stream));
+    }
+    if (k) k_apply_updates<<<grid_for(k), kBlock, 0, c->stream>>>(c->s, c->own_xy, d_pids, d_xy, k);
+    HIP_TRY(c, hipGetLastError());
+    glint_status st = begin_tick(c, c->own_xy, tick);
+    c->resident = true;
+    return st;
+}
+
+glint_status glint_ctx_view(glint_ctx *c, CtxView *v)
+{
+    if (!c || !v) return GLINT_ERR_INVALID_ARGUMENT;
+    v->stream = c->stream;
+    v->device = c->device;
+    v->num_objects = c->s.n;
+    v->tick = c->prev_tick;
+    v->min_meet = c->s.min_meet;
+    v->evaluated = c->evaluated && c->has_prev;
+    // after evaluate the current tick's pairs are the previous generation
+    v->table = c->tab[c->evaluated ? 1 - c->cur_gen : c->cur_gen];
+    v->slots = c->s.tab_mask + 1;
+    v->mlog = c->s.mlog;
+    v->mlog_cap = c->s.mlog_cap;
+    return GLINT_OK;
 }
 
 glint_status glint_partition(glint_ctx *c)

@@ -35,7 +35,7 @@ struct WalkParams {
 };
 
 __global__ void k_walk(WalkParams p, double *xy, const double *speed, int32_t *txa, int32_t *tya, uint8_t *dira,
-                       uint32_t tick, double *snapshot)
+                       uint32_t tick, double *snapshot, uint32_t *upd_pid, double *upd_xy, uint32_t *upd_count)
 {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < p.n; i += gridDim.x * blockDim.x) {
         double x = xy[2 * (size_t)i], y = xy[2 * (size_t)i + 1];
@@ -81,6 +81,19 @@ __global__ void k_walk(WalkParams p, double *xy, const double *speed, int32_t *t
             tya[i] = ty;
             dira[i] = dir;
         }
+        if (upd_count) {
+            // the objects that moved, as a location-update list (any order)
+            const unsigned long long m = __ballot(moves);
+            uint32_t base = 0;
+            if (__lane_id() == 0 && m) base = atomicAdd(upd_count, (uint32_t)__popcll(m));
+            base = __shfl(base, 0);
+            if (moves) {
+                const uint32_t j = base + (uint32_t)__popcll(m & ((1ull << __lane_id()) - 1ull));
+                upd_pid[j] = i;
+                upd_xy[2 * (size_t)j] = x;
+                upd_xy[2 * (size_t)j + 1] = y;
+            }
+        }
         if (snapshot) {
             snapshot[2 * (size_t)i] = x;
             snapshot[2 * (size_t)i + 1] = y;
@@ -106,8 +119,34 @@ extern "C" glint_status glint_gen_walk_device(const glint_gen_config *cfg, glint
     if (grid > 16384) grid = 16384;
     for (uint32_t k = 0; k < nticks; k++) {
         double *snap = d_trace ? d_trace + 2ull * p.n * k : nullptr;
-        k_walk<<<grid, 256>>>(p, w->xy, w->speed, w->tx, w->ty, w->dir, first_tick + k, snap);
+        k_walk<<<grid, 256>>>(p, w->xy, w->speed, w->tx, w->ty, w->dir, first_tick + k, snap, nullptr, nullptr,
+                              nullptr);
     }
+    if (hipGetLastError() != hipSuccess) return GLINT_ERR_HIP;
+    if (hipDeviceSynchronize() != hipSuccess) return GLINT_ERR_HIP;
+    return GLINT_OK;
+}
+
+// One tick of the walk that also writes the objects that moved as an update
+// list: d_pids[j], d_xy[2j..2j+1] for j < *d_count (device memory; capacity
+// num_objects).
+extern "C" glint_status glint_gen_walk_updates_device(const glint_gen_config *cfg, glint_walkers *w, uint32_t tick,
+                                                      uint32_t *d_pids, double *d_xy, uint32_t *d_count,
+                                                      uint32_t device)
+{
+    if (!cfg || !w || tick == 0 || !d_pids || !d_xy || !d_count) return GLINT_ERR_INVALID_ARGUMENT;
+    if (glint_gen_validate(cfg) != 0) return GLINT_ERR_INVALID_ARGUMENT;
+    if (hipSetDevice((int)device) != hipSuccess) return GLINT_ERR_HIP;
+    WalkParams p;
+    p.seed = cfg->seed;
+    p.n = cfg->num_objects;
+    p.ncell = (int32_t)(cfg->side / cfg->street_gap);
+    p.gap = cfg->street_gap;
+    p.move_fraction = cfg->move_fraction;
+    uint32_t grid = (p.n + 255) / 256;
+    if (grid > 16384) grid = 16384;
+    if (hipMemsetAsync(d_count, 0, sizeof(uint32_t), nullptr) != hipSuccess) return GLINT_ERR_HIP;
+    k_walk<<<grid, 256>>>(p, w->xy, w->speed, w->tx, w->ty, w->dir, tick, nullptr, d_pids, d_xy, d_count);
     if (hipGetLastError() != hipSuccess) return GLINT_ERR_HIP;
     if (hipDeviceSynchronize() != hipSuccess) return GLINT_ERR_HIP;
     return GLINT_OK;

@@ -100,6 +100,22 @@ class Totals(ctypes.Structure):
         "rechecks", "max_contacts_tick", "max_distinct_tick", "max_meetings_tick")]
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("arena_bytes", ctypes.c_uint64), ("max_running", ctypes.c_uint32),
+                ("estimate_percent", ctypes.c_uint32), ("min_hits", ctypes.c_uint64)]
+
+
+class QueryInfo(ctypes.Structure):
+    _fields_ = [("id", ctypes.c_uint64), ("state", ctypes.c_uint32), ("hops", ctypes.c_uint32),
+                ("hits1", ctypes.c_uint64), ("hits2", ctypes.c_uint64), ("footprint_bytes", ctypes.c_uint64),
+                ("submit_tick", ctypes.c_uint32), ("admitted_tick", ctypes.c_uint32), ("retries", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32), ("submit_ms", ctypes.c_double), ("admit_ms", ctypes.c_double),
+                ("done_ms", ctypes.c_double)]
+
+
+QUERY_PENDING, QUERY_RUNNING, QUERY_DONE, QUERY_FAILED = range(4)
+
+
 class KvSummary(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint64) for n in ("kept_keys", "oversize_keys", "packed_bytes", "logged_meetings")] + [
         (n, ctypes.c_uint32) for n in ("ctf_count", "sid_count", "oversize_oid_count", "zero_one_count",
@@ -141,6 +157,18 @@ SIGNATURES = {
     "glint_active_pairs": (_I, [_VP, _VP, _VP, _U64, ctypes.POINTER(_U64)]),
     "glint_device_meetings": (_I, [_VP, ctypes.POINTER(_VP), ctypes.POINTER(_U64)]),
     "glint_debug_inject_failure": (_I, [_VP, ctypes.c_char_p]),
+    "glint_ingest_updates": (_I, [_VP, _VP, _VP, _U32, _U32]),
+    "glint_server_create": (_I, [_VP, ctypes.POINTER(ServerConfig), ctypes.POINTER(_VP)]),
+    "glint_server_destroy": (_I, [_VP]),
+    "glint_server_last_error": (ctypes.c_char_p, [_VP]),
+    "glint_server_tick": (_I, [_VP, _VP, _VP, _VP, _U32, _U32]),
+    "glint_server_submit": (_I, [_VP, _VP, _U32, _U32, _U32, _U32, ctypes.POINTER(_U64)]),
+    "glint_server_poll": (_I, [_VP, ctypes.POINTER(_U32), ctypes.POINTER(_U32)]),
+    "glint_server_drain": (_I, [_VP]),
+    "glint_server_query_info": (_I, [_VP, _U64, ctypes.POINTER(QueryInfo)]),
+    "glint_server_result": (_I, [_VP, _U64, _VP, _U64, ctypes.POINTER(_U64)]),
+    "glint_gen_walk_updates_device": (_I, [ctypes.POINTER(_gen.GenConfig), ctypes.POINTER(_gen.Walkers), _U32, _VP,
+                                           _VP, _VP, _U32]),
     "glint_batch_create": (_I, [ctypes.POINTER(Config), _U32, _U32, ctypes.POINTER(_VP)]),
     "glint_batch_destroy": (_I, [_VP]),
     "glint_batch_context": (_I, [_VP, _U32, ctypes.POINTER(_VP)]),
@@ -444,6 +472,11 @@ class Workbench:
             out = out[np.lexsort((out[:, 3], out[:, 2], out[:, 1], out[:, 0]))]
         return out
 
+    def ingest_updates(self, d_pids: int, d_xy: int, k: int, tick: int) -> None:
+        """Location updates (device arrays) into the resident positions; starts tick."""
+        self._chk(lib().glint_ingest_updates(self.h, ctypes.c_void_p(d_pids), ctypes.c_void_p(d_xy), k, tick),
+                  "glint_ingest_updates")
+
     def kv_pending(self) -> int:
         """Directed keys logged since the last key-value flush."""
         n = ctypes.c_uint64()
@@ -522,6 +555,18 @@ class DeviceWalkers:
         out = np.zeros(2 * self.n, dtype=np.float64)
         return self.bufs["xy"].download(out)
 
+    def step_updates(self, pids: DeviceBuffer, xy: DeviceBuffer, count: DeviceBuffer) -> int:
+        """One tick; the objects that moved go to (pids, xy) as a location-update
+        list; returns its length."""
+        _check(lib().glint_gen_walk_updates_device(
+            ctypes.byref(self.cfg), ctypes.byref(self.w), self.tick + 1, ctypes.c_void_p(pids.address()),
+            ctypes.c_void_p(xy.address()), ctypes.c_void_p(count.address()), self.device),
+            "glint_gen_walk_updates_device")
+        self.tick += 1
+        k = np.zeros(1, dtype=np.uint32)
+        count.download(k)
+        return int(k[0])
+
 
 class Tracer:
     """tracer::process (process.cpp:378-634) over a host generator: per tick
@@ -583,6 +628,80 @@ class Batch:
         self.contexts = []
         if self.h and self.h.value:
             lib().glint_batch_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """The serving loop over one context (glint_server_*): ticks of location
+    updates, and tracing queries admitted by HBM footprint."""
+
+    def __init__(self, wb: Workbench, arena_bytes: int = 1 << 30, max_running: int = 8, min_hits: int = 4096,
+                 estimate_percent: int = 100):
+        self.wb = wb
+        sc = ServerConfig(arena_bytes, max_running, estimate_percent, min_hits)
+        h = ctypes.c_void_p()
+        st = lib().glint_server_create(wb.h, ctypes.byref(sc), ctypes.byref(h))
+        if st != GLINT_OK:
+            raise GlintError(st, "glint_server_create", "the context needs meeting_log_capacity > 0")
+        self.h = h
+
+    def _chk(self, st, where):
+        if st != GLINT_OK:
+            msg = lib().glint_server_last_error(self.h)
+            ctx = lib().glint_last_error_message(self.wb.h)
+            raise GlintError(st, where, " / ".join(m.decode() for m in (msg, ctx) if m))
+
+    def tick(self, tick: int, *, d_full_xy: int | None = None, d_pids: int = 0, d_xy: int = 0, k: int = 0) -> None:
+        self._chk(lib().glint_server_tick(self.h, ctypes.c_void_p(d_full_xy) if d_full_xy else None,
+                                          ctypes.c_void_p(d_pids) if d_pids else None,
+                                          ctypes.c_void_p(d_xy) if d_xy else None, k, tick), "glint_server_tick")
+
+    def submit(self, seeds, t_start: int, t_end: int, hops: int = 1) -> int:
+        seeds = np.ascontiguousarray(seeds, dtype=np.uint32)
+        qid = ctypes.c_uint64()
+        self._chk(lib().glint_server_submit(self.h, seeds.ctypes.data, seeds.size, t_start, t_end, hops,
+                                            ctypes.byref(qid)), "glint_server_submit")
+        return int(qid.value)
+
+    def poll(self) -> tuple[int, int]:
+        r, p = ctypes.c_uint32(), ctypes.c_uint32()
+        self._chk(lib().glint_server_poll(self.h, ctypes.byref(r), ctypes.byref(p)), "glint_server_poll")
+        return int(r.value), int(p.value)
+
+    def drain(self) -> None:
+        self._chk(lib().glint_server_drain(self.h), "glint_server_drain")
+
+    def info(self, qid: int) -> dict:
+        qi = QueryInfo()
+        self._chk(lib().glint_server_query_info(self.h, qid, ctypes.byref(qi)), "glint_server_query_info")
+        return {f: getattr(qi, f) for f, _ in QueryInfo._fields_}
+
+    def result(self, qid: int) -> tuple[np.ndarray, np.ndarray]:
+        """(hop-1 hits, hop-2 hits) as (m, 4) arrays of (source, target, start, end); releases the query."""
+        info = self.info(qid)
+        n = ctypes.c_uint64()
+        self._chk(lib().glint_server_result(self.h, qid, None, 0, ctypes.byref(n)), "glint_server_result")
+        out = np.zeros((max(1, n.value), 4), dtype=np.uint32)
+        self._chk(lib().glint_server_result(self.h, qid, out.ctypes.data, out.shape[0], ctypes.byref(n)),
+                  "glint_server_result")
+        out = out[:n.value]
+        return out[:info["hits1"]], out[info["hits1"]:]
+
+    def close(self) -> None:
+        if self.h and self.h.value:
+            lib().glint_server_destroy(self.h)
             self.h = ctypes.c_void_p()
 
     def __enter__(self):

@@ -258,6 +258,75 @@ const char *glint_batch_last_error(const glint_batch *batch);
 glint_status glint_batch_run_ticks(glint_batch *batch, const double *const *d_traces, uint32_t first_tick,
                                    uint32_t nticks, float *tick_ms);
 
+/* ------------------------------------------------------------------ serving
+ * Location updates ingested per tick into device-resident state, and tracing
+ * queries answered from the meeting log and the active table while ticks
+ * continue (the reference's interactive searches: search_single over active
+ * meetings, geometry.cu:605-662, whose targets seed the next search,
+ * process.cpp:443-495; id_search_in_disk over flushed meetings by time
+ * window, workbench.cpp:397-444). */
+
+/* Scatter k location updates (pid, x, y; device memory) into the resident
+ * positions and start tick `tick` on them (like glint_set_positions); the
+ * first tick of a context needs every position. */
+glint_status glint_ingest_updates(glint_ctx *ctx, const uint32_t *d_pids, const double *d_xy, uint32_t k,
+                                  uint32_t tick);
+
+typedef struct glint_server glint_server;
+
+typedef struct glint_server_config {
+    uint64_t arena_bytes;      /* device memory shared by admitted queries (the HBM admission budget) */
+    uint32_t max_running;      /* queries on the device at once (one stream each) */
+    uint32_t estimate_percent; /* scale of the hit-buffer estimates (0: 100); an overflow re-queues */
+    uint64_t min_hits;         /* smallest hit buffer per hop (entries; 0: 4096) */
+} glint_server_config;
+
+/* One contact found by a query: source (a seed at hop 1, a hop-1 contact at
+ * hop 2), target, first and last tick of the meeting. */
+typedef struct glint_hit {
+    uint32_t source, target, start, end;
+} glint_hit;
+
+enum { GLINT_QUERY_PENDING = 0, GLINT_QUERY_RUNNING = 1, GLINT_QUERY_DONE = 2, GLINT_QUERY_FAILED = 3 };
+
+typedef struct glint_query_info {
+    uint64_t id;
+    uint32_t state;            /* GLINT_QUERY_* */
+    uint32_t hops;
+    uint64_t hits1, hits2;
+    uint64_t footprint_bytes;
+    uint32_t submit_tick;      /* the state the answer describes */
+    uint32_t admitted_tick;
+    uint32_t retries;          /* re-admissions after a hit buffer overflow */
+    uint32_t reserved;
+    double submit_ms, admit_ms, done_ms;   /* host clock since the server was created */
+} glint_query_info;
+
+/* The context must have a meeting log (meeting_log_capacity > 0) sized for the
+ * longest window with an eighth of the ring to spare. */
+glint_status glint_server_create(glint_ctx *ctx, const glint_server_config *cfg, glint_server **out);
+glint_status glint_server_destroy(glint_server *server);
+const char *glint_server_last_error(const glint_server *server);
+/* One tick: every position (d_full_xy, device) or k location updates, then
+ * partition, refine, evaluate and schema update; then running queries
+ * advance and pending ones are admitted. */
+glint_status glint_server_tick(glint_server *server, const double *d_full_xy, const uint32_t *d_pids,
+                               const double *d_xy, uint32_t k, uint32_t tick);
+/* Queue a query: contacts of the seeds (host array) whose meetings overlap
+ * [t_start, t_end], with hops 1 or 2, answered on the state after the last
+ * tick.  It runs when a slot and arena memory are free. */
+glint_status glint_server_submit(glint_server *server, const uint32_t *seeds, uint32_t nseeds, uint32_t t_start,
+                                 uint32_t t_end, uint32_t hops, uint64_t *id);
+/* Advance finished work and admit pending queries; counts are optional. */
+glint_status glint_server_poll(glint_server *server, uint32_t *running, uint32_t *pending);
+/* Poll until no query is running or pending. */
+glint_status glint_server_drain(glint_server *server);
+glint_status glint_server_query_info(glint_server *server, uint64_t id, glint_query_info *out);
+/* Hits of a finished query (hop 1 then hop 2, each sorted by source, target,
+ * start); releases it.  With out = NULL only *n_out is set. */
+glint_status glint_server_result(glint_server *server, uint64_t id, glint_hit *out, uint64_t cap,
+                                 uint64_t *n_out);
+
 /* ------------------------------------------------------------ meeting log
  * With meeting_log_capacity > 0 every emitted meeting is appended to a
  * device-resident ring (the role of the reference's per-tick meeting buffer
@@ -344,6 +413,11 @@ glint_status glint_device_count(int *count);
 glint_status glint_gen_walk_device(const glint_gen_config *cfg, glint_walkers *d_walkers,
                                    uint32_t first_tick, uint32_t nticks, double *d_trace,
                                    uint32_t device);
+/* One tick of the device walk that also writes the objects that moved as a
+ * location-update list (d_pids, d_xy; capacity num_objects) and its length
+ * (*d_count, device memory). */
+glint_status glint_gen_walk_updates_device(const glint_gen_config *cfg, glint_walkers *d_walkers, uint32_t tick,
+                                           uint32_t *d_pids, double *d_xy, uint32_t *d_count, uint32_t device);
 
 #ifdef __cplusplus
 }
