@@ -143,7 +143,11 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=96)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="skewed", choices=["skewed", "plain_gpu", "plain_cpu", "batched"])
+    ap.add_argument("--workload", default="skewed", choices=["skewed", "plain_gpu", "plain_cpu", "batched", "serving"])
+    ap.add_argument("--history", type=int, default=300, help="serving: ticks of history before the timed ticks")
+    ap.add_argument("--no-pace", action="store_true",
+                    help="serving: run ticks back to back with 50 queries per tick (a stress test) instead of "
+                         "one tick per wall-clock second with Poisson arrivals at 50/s")
     ap.add_argument("--objects", type=int, default=None)
     ap.add_argument("--precision", default=None, choices=["fp64", "fp32guard"],
                     help="default: the workload's configuration (skewed: fp32guard; plain forms: fp64)")
@@ -177,6 +181,8 @@ def main() -> int:
         raise SystemExit("no HIP device visible")
     if args.workload == "batched":
         return bench_batched(args)
+    if args.workload == "serving":
+        return bench_serving(args)
     w, desc = workload_for(args.workload, args.objects)
     N = w.num_objects
     W, K, P = args.warmup, args.steps, max(0, args.profile_ticks)
@@ -388,6 +394,106 @@ def main() -> int:
     if not args.compare_forms and not args.kv:
         wb.close()
     trace.free()
+    return 0
+
+
+def bench_serving(args) -> int:
+    """Config 5: 20M resident objects, a seeded random 10% moving every tick
+    (1 Hz), location updates ingested from a device update list; tracing
+    queries arrive as a Poisson process at 50/s of simulated time (ticks are
+    the simulated seconds), each with 1..10^4 seed pids and a window of
+    60..3600 ticks (clipped to the history held by the run), 1 hop, or 2 hops
+    for a fifth of them; queries are admitted by HBM footprint and overlap
+    the ticks."""
+    from glint_pipeline_amd import autotune, gen, runtime
+    N = args.objects or 20_000_000
+    H, K = args.history, args.steps
+    w = gen.uniform_city(N, H + K + 1, gen.city_side(N), seed=7, grid_capacity=200)
+    w.move_fraction = 0.1
+    params = autotune.params_for(autotune.size_class(N))
+    cfg = autotune.config_for(w, params, distinct_estimate("batched", N), graph_ticks=0,
+                              meeting_log_capacity=1 << 27)
+    g = gen.Generator(w)
+    dw = runtime.DeviceWalkers(g)
+    pids, upd, cnt = runtime.DeviceBuffer(4 * N), runtime.DeviceBuffer(16 * N), runtime.DeviceBuffer(4)
+    full = runtime.DeviceBuffer(16 * N)
+    full.upload(g.positions())
+    rng = np.random.default_rng(11)
+    wb = runtime.Workbench(cfg)
+    sv = runtime.Server(wb, arena_bytes=8 << 30, max_running=16)
+    t0 = time.perf_counter()
+    moved = 0
+    for t in range(H):
+        if t == 0:
+            sv.tick(0, d_full_xy=full.address())
+        else:
+            k = dw.step_updates(pids, upd, cnt)
+            sv.tick(t, d_pids=pids.address(), d_xy=upd.address(), k=k)
+    log(f"[bench] serving: {N} objects, {H} ticks of history in {time.perf_counter() - t0:.1f} s; "
+        f"log {wb.meeting_log_info()}")
+    tick_ms, ids = [], []
+    pace = not args.no_pace
+
+    def submit(t):
+        seeds = rng.choice(N, size=int(rng.integers(1, 10_001)), replace=False)
+        window = int(rng.integers(60, 3601))
+        hops = 2 if rng.random() < 0.2 else 1
+        ids.append(sv.submit(seeds, max(0, t - window), t, hops))
+
+    t_start = time.perf_counter()
+    for t in range(H, H + K):
+        t1 = time.perf_counter()
+        k = dw.step_updates(pids, upd, cnt)
+        moved += k
+        sv.tick(t, d_pids=pids.address(), d_xy=upd.address(), k=k)
+        tick_ms.append((time.perf_counter() - t1) * 1e3)
+        arrivals = np.sort(rng.uniform(0.0, 1.0, int(rng.poisson(50))))
+        for a in arrivals:
+            if pace:   # submit each query at its arrival time within the tick's second
+                while time.perf_counter() < t1 + a:
+                    sv.poll()
+                    time.sleep(0.0005)
+            submit(t)
+            sv.poll()
+        while pace and time.perf_counter() < t1 + 1.0:
+            sv.poll()
+            time.sleep(0.0005)
+    elapsed_ticks = time.perf_counter() - t_start
+    sv.drain()
+    elapsed = time.perf_counter() - t_start
+    qlat, hits, waited, retries = [], 0, 0, 0
+    for qid in ids:
+        info = sv.info(qid)
+        qlat.append(info["done_ms"] - info["submit_ms"])
+        waited += info["admitted_tick"] > info["submit_tick"]
+        retries += info["retries"]
+        h1, h2 = sv.result(qid)
+        hits += len(h1) + len(h2)
+    sv.close()
+    wb.close()
+    for b in (pids, upd, cnt, full):
+        b.free()
+    lat = np.asarray(tick_ms)
+    ql = np.asarray(qlat)
+    out = {
+        "metric": "object-updates/sec (moved objects) with concurrent tracing queries, 20M resident, 1 MI355X",
+        # capacity: moved objects per second of tick processing (ingestion + full tick), queries running beside
+        "value": round(moved / (float(np.sum(tick_ms)) / 1e3), 1), "unit": "object-updates/s", "n_gpus": 1,
+        "steps": K, "warmup": H, "ms_per_step": round(float(np.mean(tick_ms)), 3),
+        "paced": pace, "wall_seconds": round(elapsed_ticks, 2),
+        "p50_tick_ms": round(float(np.percentile(lat, 50)), 3), "p99_tick_ms": round(float(np.percentile(lat, 99)), 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded walk of resident objects, 10% moving per tick; Poisson queries)",
+        "config": {"workload": f"serving: {N} resident objects, side {w.side:.0f} m, 10% moving per tick, "
+                               f"queries Poisson 50/tick, seeds U{{1..10^4}}, windows U[60,3600] ticks "
+                               f"(clipped to {H + K} ticks of history), 20% two-hop",
+                   "params": params, "arena_bytes": 8 << 30, "max_running": 16},
+        "queries": {"count": len(ids), "p50_ms": round(float(np.percentile(ql, 50)), 3) if len(ql) else None,
+                    "p99_ms": round(float(np.percentile(ql, 99)), 3) if len(ql) else None,
+                    "admitted_later": int(waited), "retries": int(retries), "hits": int(hits),
+                    "per_second": round(len(ids) / elapsed, 1)},
+    }
+    print(json.dumps(out), flush=True)
     return 0
 
 
