@@ -43,8 +43,9 @@ constexpr int kFlushRound = 4;                      // join columns between stag
 constexpr int kMeetStage = 256;                     // staged meetings per wave (table update)
 constexpr uint32_t kChunk = 4096;                   // contact-list entries reserved per atomic
 constexpr int kRegion = 2048;                       // meeting-table slots per region (LDS-resident)
-constexpr int kBucketTile = 4096;                   // contacts per LDS tile of the coarse pass
-constexpr int kFineTile = 4096;                     // contacts per LDS tile of the fine pass
+constexpr int kBucketThreads = 1024;                // workgroup of the two bucketing passes
+constexpr int kBucketTile = 16384;                  // contacts per LDS tile of the coarse pass (128 KB)
+constexpr int kFineTile = 12288;                    // contacts per LDS tile of the fine pass (96 KB)
 constexpr uint32_t kMaxCoarse = 1024;               // coarse bins of pass 1
 constexpr uint32_t kMaxFine = 2048;                 // fine bins of pass 2
 constexpr int kBorderStage = 256;                   // staged border objects per wave (assign)
@@ -1047,34 +1048,35 @@ __global__ __launch_bounds__(kBlock) void k_scan_write(const uint32_t *in, uint3
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_bucket_scatter(DevState s)
+__global__ __launch_bounds__(kBucketThreads) void k_bucket_scatter(DevState s)
 {
     __shared__ unsigned long long tile[kBucketTile];
     __shared__ uint32_t hist[kMaxCoarse + 1], start[kMaxCoarse + 1], cursor[kMaxCoarse + 1],
         running[kMaxCoarse + 1];
-    __shared__ uint32_t wsum[kBlock / 64];
+    __shared__ uint32_t wsum[kBucketThreads / 64];
     const unsigned long long n = min(s.ctr->n_clist, s.clist_cap);
     const uint32_t nb = s.coarse_bins + 1;
     unsigned long long b0, b1;
     block_range(s, n, b0, b1);
-    for (uint32_t k = threadIdx.x; k < nb; k += kBlock) running[k] = s.boff[(size_t)k * gridDim.x + blockIdx.x];
+    for (uint32_t k = threadIdx.x; k < nb; k += kBucketThreads) running[k] = s.boff[(size_t)k * gridDim.x + blockIdx.x];
     for (unsigned long long t0 = b0; t0 < b1; t0 += kBucketTile) {
         const uint32_t tn = (uint32_t)min((unsigned long long)kBucketTile, b1 - t0);
-        for (uint32_t k = threadIdx.x; k < nb; k += kBlock) hist[k] = 0;
+        for (uint32_t k = threadIdx.x; k < nb; k += kBucketThreads) hist[k] = 0;
         __syncthreads();
-        unsigned long long keys[kBucketTile / kBlock];
+        unsigned long long keys[kBucketTile / kBucketThreads];
 #pragma unroll
-        for (int q = 0; q < kBucketTile / kBlock; q++) {
-            uint32_t i = threadIdx.x + q * kBlock;
+        for (int q = 0; q < kBucketTile / kBucketThreads; q++) {
+            uint32_t i = threadIdx.x + q * kBucketThreads;
             keys[q] = i < tn ? s.clist[t0 + i] : kEmptyKey;
             if (i < tn) atomicAdd(&hist[coarse_bin(s, keys[q])], 1u);
         }
         __syncthreads();
-        // exclusive scan of the tile histogram (nb <= 1025 bins, 5 per thread)
+        // exclusive scan of the tile histogram (nb <= 1025 bins, kPer per thread)
         {
-            uint32_t c[5], v = 0;
-            for (int k = 0; k < 5; k++) {
-                uint32_t b = threadIdx.x * 5 + k;
+            constexpr int kPer = (kMaxCoarse + 1 + kBucketThreads - 1) / kBucketThreads;
+            uint32_t c[kPer], v = 0;
+            for (int k = 0; k < kPer; k++) {
+                uint32_t b = threadIdx.x * kPer + k;
                 c[k] = b < nb ? hist[b] : 0;
                 v += c[k];
             }
@@ -1087,8 +1089,8 @@ __global__ __launch_bounds__(kBlock) void k_bucket_scatter(DevState s)
             __syncthreads();
             uint32_t ex = inc - v;
             for (uint32_t w = 0; w < threadIdx.x / 64; w++) ex += wsum[w];
-            for (int k = 0; k < 5; k++) {
-                uint32_t b = threadIdx.x * 5 + k;
+            for (int k = 0; k < kPer; k++) {
+                uint32_t b = threadIdx.x * kPer + k;
                 if (b < nb) {
                     start[b] = ex;
                     cursor[b] = ex;
@@ -1098,18 +1100,18 @@ __global__ __launch_bounds__(kBlock) void k_bucket_scatter(DevState s)
         }
         __syncthreads();
 #pragma unroll
-        for (int q = 0; q < kBucketTile / kBlock; q++) {
-            uint32_t i = threadIdx.x + q * kBlock;
+        for (int q = 0; q < kBucketTile / kBucketThreads; q++) {
+            uint32_t i = threadIdx.x + q * kBucketThreads;
             if (i < tn) tile[atomicAdd(&cursor[coarse_bin(s, keys[q])], 1u)] = keys[q];
         }
         __syncthreads();
-        for (uint32_t p = threadIdx.x; p < tn; p += kBlock) {
+        for (uint32_t p = threadIdx.x; p < tn; p += kBucketThreads) {
             unsigned long long key = tile[p];
             uint32_t b = coarse_bin(s, key);
             if (b < s.coarse_bins) s.clist2[running[b] + (p - start[b])] = key;
         }
         __syncthreads();
-        for (uint32_t k = threadIdx.x; k < nb; k += kBlock) running[k] += hist[k];
+        for (uint32_t k = threadIdx.x; k < nb; k += kBucketThreads) running[k] += hist[k];
         __syncthreads();
     }
 }
@@ -1121,10 +1123,11 @@ __global__ __launch_bounds__(kBlock) void k_bucket_scatter(DevState s)
 __device__ __forceinline__ uint32_t block_scan_bins(uint32_t *bins, uint32_t nbins, uint32_t *out_excl,
                                                     uint32_t add, uint32_t *wsum)
 {
-    // exclusive scan of bins[0..nbins) (nbins <= 8 * kBlock) into out_excl (+add)
-    uint32_t cc[8], v = 0;
-    for (int k = 0; k < 8; k++) {
-        uint32_t b = threadIdx.x * 8 + k;
+    // exclusive scan of bins[0..nbins) (nbins <= kMaxFine) into out_excl (+add)
+    constexpr int kPer = kMaxFine / kBucketThreads;
+    uint32_t cc[kPer], v = 0;
+    for (int k = 0; k < kPer; k++) {
+        uint32_t b = threadIdx.x * kPer + k;
         cc[k] = b < nbins ? bins[b] : 0;
         v += cc[k];
     }
@@ -1132,12 +1135,12 @@ __device__ __forceinline__ uint32_t block_scan_bins(uint32_t *bins, uint32_t nbi
     if (lane_id() == 63) wsum[threadIdx.x / 64] = inc;
     __syncthreads();
     uint32_t ex = inc - v, total = 0;
-    for (uint32_t w = 0; w < kBlock / 64; w++) {
+    for (uint32_t w = 0; w < kBucketThreads / 64; w++) {
         if (w < threadIdx.x / 64) ex += wsum[w];
         total += wsum[w];
     }
-    for (int k = 0; k < 8; k++) {
-        uint32_t b = threadIdx.x * 8 + k;
+    for (int k = 0; k < kPer; k++) {
+        uint32_t b = threadIdx.x * kPer + k;
         if (b < nbins) out_excl[b] = add + ex;
         ex += cc[k];
     }
@@ -1145,53 +1148,53 @@ __device__ __forceinline__ uint32_t block_scan_bins(uint32_t *bins, uint32_t nbi
     return total;
 }
 
-__global__ __launch_bounds__(kBlock) void k_bucket_fine(DevState s)
+__global__ __launch_bounds__(kBucketThreads) void k_bucket_fine(DevState s)
 {
     __shared__ unsigned long long tile[kFineTile];
     __shared__ uint32_t thist[kMaxFine], tstart[kMaxFine], tcur[kMaxFine], gcur[kMaxFine];
-    __shared__ uint32_t wsum[kBlock / 64];
+    __shared__ uint32_t wsum[kBucketThreads / 64];
     const uint32_t G1 = s.bucket_blocks;
     const uint32_t nf = 1u << s.lo_bits;
     for (uint32_t c = blockIdx.x; c < s.coarse_bins; c += gridDim.x) {
         const uint32_t seg0 = s.boff[(size_t)c * G1], seg1 = s.boff[(size_t)(c + 1) * G1];
-        for (uint32_t k = threadIdx.x; k < nf; k += kBlock) thist[k] = 0;
+        for (uint32_t k = threadIdx.x; k < nf; k += kBucketThreads) thist[k] = 0;
         __syncthreads();
-        for (uint32_t i = seg0 + threadIdx.x; i < seg1; i += kBlock)
+        for (uint32_t i = seg0 + threadIdx.x; i < seg1; i += kBucketThreads)
             atomicAdd(&thist[region_of(s, s.clist2[i] & ~(1ull << 63)) & (nf - 1)], 1u);
         __syncthreads();
         block_scan_bins(thist, nf, gcur, seg0, wsum);
-        for (uint32_t k = threadIdx.x; k < nf; k += kBlock) s.bstart[(size_t)c * nf + k] = gcur[k];
+        for (uint32_t k = threadIdx.x; k < nf; k += kBucketThreads) s.bstart[(size_t)c * nf + k] = gcur[k];
         __syncthreads();
         for (uint32_t t0 = seg0; t0 < seg1; t0 += kFineTile) {
             const uint32_t tn = min((uint32_t)kFineTile, seg1 - t0);
-            for (uint32_t k = threadIdx.x; k < nf; k += kBlock) thist[k] = 0;
+            for (uint32_t k = threadIdx.x; k < nf; k += kBucketThreads) thist[k] = 0;
             __syncthreads();
-            unsigned long long keys[kFineTile / kBlock];
-            uint32_t fb[kFineTile / kBlock];
+            unsigned long long keys[kFineTile / kBucketThreads];
+            uint32_t fb[kFineTile / kBucketThreads];
 #pragma unroll
-            for (int q = 0; q < kFineTile / kBlock; q++) {
-                const uint32_t i = threadIdx.x + q * kBlock;
+            for (int q = 0; q < kFineTile / kBucketThreads; q++) {
+                const uint32_t i = threadIdx.x + q * kBucketThreads;
                 keys[q] = i < tn ? s.clist2[t0 + i] : 0ull;
                 fb[q] = region_of(s, keys[q] & ~(1ull << 63)) & (nf - 1);
                 if (i < tn) atomicAdd(&thist[fb[q]], 1u);
             }
             __syncthreads();
             block_scan_bins(thist, nf, tstart, 0, wsum);
-            for (uint32_t k = threadIdx.x; k < nf; k += kBlock) tcur[k] = tstart[k];
+            for (uint32_t k = threadIdx.x; k < nf; k += kBucketThreads) tcur[k] = tstart[k];
             __syncthreads();
 #pragma unroll
-            for (int q = 0; q < kFineTile / kBlock; q++) {
-                const uint32_t i = threadIdx.x + q * kBlock;
+            for (int q = 0; q < kFineTile / kBucketThreads; q++) {
+                const uint32_t i = threadIdx.x + q * kBucketThreads;
                 if (i < tn) tile[atomicAdd(&tcur[fb[q]], 1u)] = keys[q];
             }
             __syncthreads();
-            for (uint32_t p = threadIdx.x; p < tn; p += kBlock) {
+            for (uint32_t p = threadIdx.x; p < tn; p += kBucketThreads) {
                 const unsigned long long key = tile[p];
                 const uint32_t f = region_of(s, key & ~(1ull << 63)) & (nf - 1);
                 s.clist[gcur[f] + (p - tstart[f])] = key;
             }
             __syncthreads();
-            for (uint32_t k = threadIdx.x; k < nf; k += kBlock) gcur[k] += thist[k];
+            for (uint32_t k = threadIdx.x; k < nf; k += kBucketThreads) gcur[k] += thist[k];
             __syncthreads();
         }
     }
@@ -1697,9 +1700,9 @@ void enqueue_refine(glint_ctx *c)
     launch(c, K_SCAN_REDUCE, [&] { k_scan_reduce<<<nb, kBlock, 0, st>>>(s.bcount, nc, s.block_sums); });
     launch(c, K_SCAN_TOP, [&] { k_scan_top<<<1, kBlock, 0, st>>>(s.block_sums, nb); });
     launch(c, K_SCAN_WRITE, [&] { k_scan_write<<<nb, kBlock, 0, st>>>(s.bcount, nc, s.block_sums, s.boff); });
-    launch(c, K_BUCKET_SCATTER, [&] { k_bucket_scatter<<<G1, kBlock, 0, st>>>(s); });
+    launch(c, K_BUCKET_SCATTER, [&] { k_bucket_scatter<<<G1, kBucketThreads, 0, st>>>(s); });
     if (s.lo_bits)
-        launch(c, K_BUCKET_FINE, [&] { k_bucket_fine<<<s.coarse_bins, kBlock, 0, st>>>(s); });
+        launch(c, K_BUCKET_FINE, [&] { k_bucket_fine<<<s.coarse_bins, kBucketThreads, 0, st>>>(s); });
     else
         launch(c, K_BUCKET_FINE, [&] { k_bucket_single<<<grid_for(s.coarse_bins + 1), kBlock, 0, st>>>(s); });
 }
