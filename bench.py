@@ -68,9 +68,9 @@ def algorithmic_bytes(kernel: str, n: int, per_tick: dict, slots: int, precision
         # every sorted coordinate once, each border unit (id + xy), one 16 B upsert per contact
         return coord * n + 24.0 * U + 16.0 * C
     if kernel == "k_table":
-        # previous generation read and new generation written (16 B slots),
-        # every bucketed contact read once, meetings out
-        return 32.0 * slots + 8.0 * C + 16.0 * M
+        # previous generation read and new generation written (12 B slots:
+        # 8 B key + 4 B start), every bucketed contact read once, meetings out
+        return 24.0 * slots + 8.0 * C + 16.0 * M
     if kernel == "k_bucket_fine":
         return 24.0 * C                       # segment read twice (histogram, tiles), written once
     if kernel == "k_bucket_scatter":

@@ -16,7 +16,8 @@ struct CtxView {
     uint32_t tick;              // last evaluated tick
     uint32_t min_meet;
     bool evaluated;
-    const void *table;          // glint::Slot[slots]
+    const unsigned long long *keys;   // active pairs by table slot (~0 when free)
+    const uint32_t *starts;           // first tick of each pair's run
     uint64_t slots;
     const glint_meeting *mlog;  // meeting log ring (null when off)
     uint64_t mlog_cap;

@@ -93,19 +93,7 @@ __global__ void k_fill_u32(uint32_t *p, uint32_t v, uint64_t n)
         p[i] = v;
 }
 
-__global__ void k_fill_slots(Slot *p, uint64_t n)
-{
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
-         i += (uint64_t)gridDim.x * blockDim.x) {
-        Slot s;
-        s.key = kEmptyKey;
-        s.start = 0;
-        s.seen = 0;
-        p[i] = s;
-    }
-}
-
-__global__ void k_set_tick(TickParams *tp, const double *xy, Slot *tab_old, Slot *tab_cur, float4 *mbr_old,
+__global__ void k_set_tick(TickParams *tp, const double *xy, TableGen tab_old, TableGen tab_cur, float4 *mbr_old,
                            float4 *mbr_cur, double *xy_prev, double *xy_save, uint32_t tick, uint32_t prev_tick,
                            uint32_t has_prev)
 {
@@ -135,7 +123,7 @@ __global__ void k_advance_tick(TickParams *tp, uint64_t stride)
     tp->next_tick = tp->next_tick + 1;
     tp->has_prev = tp->has_prev_next;
     tp->has_prev_next = 1;
-    Slot *t = tp->tab_old;
+    const TableGen t = tp->tab_old;
     tp->tab_old = tp->tab_cur;
     tp->tab_cur = t;
     float4 *m = tp->mbr_old;
@@ -148,7 +136,7 @@ __global__ void k_advance_tick(TickParams *tp, uint64_t stride)
 
 // State before the first tick of a run, one k_advance_tick short of it; the
 // positions already point at the first snapshot for the initial build.
-__global__ void k_begin_run(TickParams *tp, const double *base, Slot *tab_old, Slot *tab_cur, float4 *mbr_old,
+__global__ void k_begin_run(TickParams *tp, const double *base, TableGen tab_old, TableGen tab_cur, float4 *mbr_old,
                             float4 *mbr_cur, double *xy_prev, double *xy_save, uint32_t tick, uint32_t next_tick,
                             uint32_t has_prev_next, unsigned long long *stamps, uint32_t nstamps)
 {
@@ -1238,8 +1226,7 @@ template <bool KV>
 __global__ __launch_bounds__(kBlock) void k_table(DevState s)
 {
     __shared__ TableLds L;
-    const Slot *tab_old = s.tp->tab_old;
-    Slot *tab_cur = s.tp->tab_cur;
+    const TableGen told = s.tp->tab_old, tcur = s.tp->tab_cur;
     const uint32_t tick = s.tp->tick, last = s.tp->prev_tick;
     const unsigned long long *list = s.lo_bits ? s.clist : s.clist2;
     const uint32_t nreg = 1u << s.region_bits;
@@ -1250,25 +1237,34 @@ __global__ __launch_bounds__(kBlock) void k_table(DevState s)
     // the previous generation's slots of the next region are loaded into
     // registers while the current region is processed
     constexpr int kPer = kRegion / kBlock;
-    Slot pre[kPer];
+    unsigned long long pk[kPer];
+    uint32_t ps[kPer];
     if (blockIdx.x < nreg) {
 #pragma unroll
-        for (int q = 0; q < kPer; q++) pre[q] = tab_old[(size_t)blockIdx.x * kRegion + threadIdx.x + q * kBlock];
+        for (int q = 0; q < kPer; q++) {
+            const size_t i = (size_t)blockIdx.x * kRegion + threadIdx.x + q * kBlock;
+            pk[q] = told.keys[i];
+            ps[q] = told.starts[i];
+        }
     }
     for (uint32_t r = blockIdx.x; r < nreg; r += gridDim.x) {
         const size_t base = (size_t)r * kRegion;
 #pragma unroll
         for (int q = 0; q < kPer; q++) {
             const uint32_t k = threadIdx.x + q * kBlock;
-            L.oldk[k] = pre[q].key;
-            L.olds[k] = pre[q].start;
+            L.oldk[k] = pk[q];
+            L.olds[k] = ps[q];
             L.seen[k] = 0;
             L.newk[k] = kEmptyKey;
         }
         const uint32_t rn = r + gridDim.x;
         if (rn < nreg) {
 #pragma unroll
-            for (int q = 0; q < kPer; q++) pre[q] = tab_old[(size_t)rn * kRegion + threadIdx.x + q * kBlock];
+            for (int q = 0; q < kPer; q++) {
+                const size_t i = (size_t)rn * kRegion + threadIdx.x + q * kBlock;
+                pk[q] = told.keys[i];
+                ps[q] = told.starts[i];
+            }
         }
         __syncthreads();
         const uint32_t c0 = void_tick ? 0 : s.bstart[r], c1 = void_tick ? 0 : s.bstart[r + 1];
@@ -1374,11 +1370,8 @@ __global__ __launch_bounds__(kBlock) void k_table(DevState s)
                 wave_sync();
                 nst = 0;
             }
-            Slot ns;
-            ns.key = L.newk[k];
-            ns.start = L.news[k];
-            ns.seen = 0;
-            tab_cur[base + k] = ns;
+            tcur.keys[base + k] = L.newk[k];
+            tcur.starts[base + k] = L.news[k];
         }
         __syncthreads();
     }
@@ -1396,6 +1389,12 @@ __global__ __launch_bounds__(kBlock) void k_table(DevState s)
     }
     for (int d = 32; d >= 1; d >>= 1) inserted += __shfl_xor(inserted, d);
     if (lane == 0 && inserted) atomicAdd(&s.ctr->n_inserted, (unsigned long long)inserted);
+}
+
+__global__ void k_fill_keys(unsigned long long *p, uint64_t n)
+{
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = kEmptyKey;
 }
 
 // ------------------------------------------------------------ host side
@@ -1492,7 +1491,7 @@ struct glint_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     DevState s{};
-    Slot *tab[2] = {nullptr, nullptr};
+    TableGen tab[2] = {};
     float4 *mbr[2] = {nullptr, nullptr};    // pair boxes of each generation (key-value output)
     KvEngine *kv = nullptr;                 // sorted key-value output (null when off)
     double *hist[2] = {nullptr, nullptr};   // positions of the last two ticks (generation parity)
@@ -1978,8 +1977,10 @@ glint_status glint_create(const glint_config *cfg, glint_ctx **out)
     }
     ALLOC(s.meet, s.meet_cap);
     ALLOC(s.ctr, 1);
-    ALLOC(c->tab[0], slots);
-    ALLOC(c->tab[1], slots);
+    for (int g = 0; g < 2; g++) {
+        ALLOC(c->tab[g].keys, slots);
+        ALLOC(c->tab[g].starts, slots);
+    }
     ALLOC(c->own_xy, 2 * N);
     if (cfg->meeting_log_capacity) {
         ALLOC(s.mlog, cfg->meeting_log_capacity);
@@ -2010,8 +2011,7 @@ glint_status glint_create(const glint_config *cfg, glint_ctx **out)
         c->err = std::string("hipMemsetAsync: ") + hipGetErrorString(e);
         return fail(GLINT_ERR_HIP);
     }
-    k_fill_slots<<<grid_for(slots, 16384), kBlock, 0, c->stream>>>(c->tab[0], slots);
-    k_fill_slots<<<grid_for(slots, 16384), kBlock, 0, c->stream>>>(c->tab[1], slots);
+    for (int g = 0; g < 2; g++) k_fill_keys<<<grid_for(slots, 16384), kBlock, 0, c->stream>>>(c->tab[g].keys, slots);
     e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) {
         c->err = std::string("initialisation: ") + hipGetErrorString(e);
@@ -2109,7 +2109,9 @@ glint_status glint_ctx_view(glint_ctx *c, CtxView *v)
     v->min_meet = c->s.min_meet;
     v->evaluated = c->evaluated && c->has_prev;
     // after evaluate the current tick's pairs are the previous generation
-    v->table = c->tab[c->evaluated ? 1 - c->cur_gen : c->cur_gen];
+    const TableGen &tab = c->tab[c->evaluated ? 1 - c->cur_gen : c->cur_gen];
+    v->keys = tab.keys;
+    v->starts = tab.starts;
     v->slots = c->s.tab_mask + 1;
     v->mlog = c->s.mlog;
     v->mlog_cap = c->s.mlog_cap;
@@ -2434,13 +2436,15 @@ glint_status glint_active_pairs(glint_ctx *c, uint64_t *keys, uint32_t *starts, 
     if (!c || !n_out) return GLINT_ERR_INVALID_ARGUMENT;
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     // after evaluate the current tick's pairs are the previous generation
-    const Slot *tab = c->tab[c->evaluated ? 1 - c->cur_gen : c->cur_gen];
+    const TableGen &tab = c->tab[c->evaluated ? 1 - c->cur_gen : c->cur_gen];
     const uint64_t slots = c->s.tab_mask + 1;
-    std::vector<Slot> h(slots);
-    HIP_TRY(c, hipMemcpy(h.data(), tab, slots * sizeof(Slot), hipMemcpyDeviceToHost));
+    std::vector<unsigned long long> hk(slots);
+    std::vector<uint32_t> hs(slots);
+    HIP_TRY(c, hipMemcpy(hk.data(), tab.keys, slots * 8, hipMemcpyDeviceToHost));
+    HIP_TRY(c, hipMemcpy(hs.data(), tab.starts, slots * 4, hipMemcpyDeviceToHost));
     std::vector<std::pair<uint64_t, uint32_t>> v;
-    for (const Slot &sl : h)
-        if (sl.key != kEmptyKey) v.emplace_back(sl.key, sl.start);
+    for (uint64_t i = 0; i < slots; i++)
+        if (hk[i] != kEmptyKey) v.emplace_back(hk[i], hs[i]);
     std::sort(v.begin(), v.end());
     *n_out = v.size();
     if (!keys && !starts) return GLINT_OK;

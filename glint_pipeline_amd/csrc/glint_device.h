@@ -50,13 +50,6 @@ struct alignas(16) NodeMeta {
     uint32_t underflow;
 };
 
-struct alignas(16) Slot {
-    unsigned long long key;   // (pid_a << 32) | pid_b, kEmptyKey when free
-    uint32_t start;           // first tick of the run of contacts
-    uint32_t seen;            // set when the pair is seen again in the next tick
-};
-static_assert(sizeof(Slot) == 16, "16-byte meeting slot");
-
 struct Counters {
     unsigned long long n_units;
     unsigned long long n_contacts;
@@ -104,13 +97,20 @@ struct KvLogEntry {
     float4 box;
 };
 
+// One generation of the meeting table, slot by slot: the pair's key
+// ((pid_a << 32) | pid_b, kEmptyKey when free) and the first tick of its run.
+struct TableGen {
+    unsigned long long *keys;
+    uint32_t *starts;
+};
+
 struct TickParams {
     const double *xy;
     const double *trace_base;   // run_ticks: tick k of the trace at trace_base + 2*N*k
-    Slot *tab_old;              // pairs seen at prev_tick (read, flagged, then cleared)
-    Slot *tab_cur;              // pairs seen at tick (filled by refine)
+    TableGen tab_old;           // pairs seen at prev_tick
+    TableGen tab_cur;           // pairs seen at tick (written by the table update)
     double *xy_prev;            // positions at prev_tick (pid order), valid if has_prev
-    float4 *mbr_old;            // boxes of the pairs in tab_old (key-value output only)
+    float4 *mbr_old;            // boxes of the pairs in tab_old, by slot (key-value output only)
     float4 *mbr_cur;            // boxes of the pairs in tab_cur
     double *xy_save;            // receives this tick's positions for the next tick
     unsigned long long *stamps; // run_ticks: device wall clock at the end of every tick

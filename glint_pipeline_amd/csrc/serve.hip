@@ -37,8 +37,6 @@
 #include "glint.h"
 #include "glint_device.h"
 
-using glint::Slot;
-
 namespace {
 
 constexpr uint32_t kB = 256;
@@ -79,21 +77,24 @@ __device__ __forceinline__ void append_hit(bool put, glint_hit h, glint_hit *out
 
 // Pairs of the table with a seed on either side that were in progress at
 // tick T (start <= T, T - start >= min + 1), clipped to end = T.
-__global__ __launch_bounds__(kB) void k_active_hits(const Slot *tab, uint64_t slots, uint32_t T, uint32_t min_meet,
-                                                    uint32_t t_start, uint32_t t_end, const uint32_t *seeds_bm,
-                                                    glint_hit *out, unsigned long long *count, uint64_t cap)
+__global__ __launch_bounds__(kB) void k_active_hits(const unsigned long long *keys, const uint32_t *starts,
+                                                    uint64_t slots, uint32_t T, uint32_t min_meet, uint32_t t_start,
+                                                    uint32_t t_end, const uint32_t *seeds_bm, glint_hit *out,
+                                                    unsigned long long *count, uint64_t cap)
 {
     for (uint64_t i0 = blockIdx.x * (uint64_t)kB; i0 < slots; i0 += (uint64_t)gridDim.x * kB) {
         const uint64_t i = i0 + threadIdx.x;
-        Slot sl;
-        sl.key = kEmpty;
-        if (i < slots) sl = tab[i];
-        const bool valid = sl.key != kEmpty && sl.start <= T && T - sl.start >= min_meet + 1 &&
-                           sl.start <= t_end && T >= t_start;
-        const uint32_t a = (uint32_t)(sl.key >> 32), b = (uint32_t)sl.key;
+        unsigned long long key = kEmpty;
+        uint32_t start = 0;
+        if (i < slots) {
+            key = keys[i];
+            start = starts[i];
+        }
+        const bool valid = key != kEmpty && start <= T && T - start >= min_meet + 1 && start <= t_end && T >= t_start;
+        const uint32_t a = (uint32_t)(key >> 32), b = (uint32_t)key;
         const bool ha = valid && has_bit(seeds_bm, a), hb = valid && has_bit(seeds_bm, b);
-        append_hit(ha, glint_hit{a, b, sl.start, T}, out, count, cap);
-        append_hit(hb, glint_hit{b, a, sl.start, T}, out, count, cap);
+        append_hit(ha, glint_hit{a, b, start, T}, out, count, cap);
+        append_hit(hb, glint_hit{b, a, start, T}, out, count, cap);
     }
 }
 
@@ -363,7 +364,7 @@ glint_status start_query(glint_server *s, Query &q, int slot)
     CtxView v{};
     glint_ctx_view(s->ctx, &v);
     SV_TRY(s, hipStreamWaitEvent(v.stream, sl.ready, 0));
-    k_active_hits<<<grid_of(v.slots, 8192), kB, 0, v.stream>>>((const Slot *)v.table, v.slots, q.T, v.min_meet,
+    k_active_hits<<<grid_of(v.slots, 8192), kB, 0, v.stream>>>(v.keys, v.starts, v.slots, q.T, v.min_meet,
                                                                q.t_start, q.t_end, q.bm1, q.h1, &q.ctr->hits1,
                                                                q.cap1);
     SV_TRY(s, hipEventRecord(sl.snap, v.stream));
