@@ -158,8 +158,8 @@ def main() -> int:
     ap.add_argument("--autotune", action="store_true", help="tune leaf capacity, join tile, load factor and "
                                                              "precision on this workload before timing")
     ap.add_argument("--save-tuned", action="store_true", help="store the autotuned parameters in tuned.json")
-    ap.add_argument("--profile-ticks", type=int, default=3,
-                    help="ticks run after the timed region with per-kernel HIP events (roofline)")
+    ap.add_argument("--no-profile", action="store_true",
+                    help="skip the second run that times every kernel with HIP events over the timed ticks")
     ap.add_argument("--cpu-ticks", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--compare-forms", action="store_true",
@@ -185,7 +185,7 @@ def main() -> int:
         return bench_serving(args)
     w, desc = workload_for(args.workload, args.objects)
     N = w.num_objects
-    W, K, P = args.warmup, args.steps, max(0, args.profile_ticks)
+    W, K = args.warmup, args.steps
     spec = WORKLOAD_SPEC[args.workload]
     params = autotune.params_for(spec["tuned"]) if spec["tuned"] else dict(autotune.DEFAULTS)
     params["precision"] = spec["precision"]     # prescribed by the configuration, not tuned
@@ -198,7 +198,7 @@ def main() -> int:
     t0 = time.perf_counter()
     g = gen.Generator(w)
     dw = runtime.DeviceWalkers(g)
-    total_ticks = W + K + P
+    total_ticks = W + K
     trace = runtime.DeviceBuffer(16 * N * total_ticks)
     trace.upload(g.positions(), 0)
     if total_ticks > 1:
@@ -233,8 +233,8 @@ def main() -> int:
 
     cfg = make_cfg(precision, graph_ticks)
     slots = int(cfg.num_meeting_buckets)
-    log(f"[bench] workload: {desc}; params {params}; graph ticks {graph_ticks}; warmup {W}, timed {K}, "
-        f"profiled {P}; table slots 2^{int(math.log2(slots))}")
+    log(f"[bench] workload: {desc}; params {params}; graph ticks {graph_ticks}; warmup {W}, timed {K}; "
+        f"table slots 2^{int(math.log2(slots))}")
     wb = runtime.Workbench(cfg)
     t0 = time.perf_counter()
     if W:
@@ -253,20 +253,26 @@ def main() -> int:
     stats = wb.stats()
     log(f"[bench] timed {K} ticks in {elapsed:.3f} s; totals {totals}")
 
-    # ---- per-kernel times on a few more ticks (HIP events around every launch)
+    # ---- per-kernel times over the same ticks: a second run of the same
+    # inputs without graph capture (events cannot sit inside a replayed graph),
+    # HIP events around every launch on the context's stream during ticks W..W+K
     ktimes = {}
-    if P:
+    if not args.no_profile:
+        wb.close()
+        wb = runtime.Workbench(make_cfg(precision, 0))
+        wb.run_ticks(trace.address(0), 0, W, latencies=False)
+        wb.synchronize()
         wb.set_kernel_profiling(True)
         wb.kernel_times(reset=True)
-        wb.run_ticks(trace.address(16 * N * (W + K)), W + K, P, latencies=False)
+        wb.run_ticks(trace.address(16 * N * W), W, K, latencies=False)
         wb.synchronize()
         ktimes = wb.kernel_times()
         wb.set_kernel_profiling(False)
-        ptick = sum(ms for ms, cnt in ktimes.values()) / P
+        ptick = sum(ms for ms, cnt in ktimes.values()) / K
         for name, (ms, cnt) in sorted(ktimes.items(), key=lambda kv: -kv[1][0]):
             if cnt:
-                log(f"[bench]   {name:16s} {ms / P:9.3f} ms/tick  {cnt // P:4d} launches/tick  {ms / cnt:9.3f} ms/launch")
-        log(f"[bench]   kernels sum {ptick:.3f} ms/tick (profiled, no graph)")
+                log(f"[bench]   {name:16s} {ms / K:9.3f} ms/tick  {cnt // K:4d} launches/tick  {ms / cnt:9.3f} ms/launch")
+        log(f"[bench]   kernels sum {ptick:.3f} ms/tick (events on every launch, no graph, ticks {W}..{W + K - 1})")
 
     value = N * K / elapsed
     per_tick = {k: totals[k] / max(1, totals["ticks"]) for k in
@@ -281,7 +287,9 @@ def main() -> int:
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom,
                     "kernel_ms_per_launch": round(dms / dcnt, 4), "algorithmic_bytes_per_launch": abytes,
-                    "kernel_share_of_tick": round(dms / P / (elapsed * 1e3 / K), 4)}
+                    "kernel_share_of_tick": round(dms / K / (elapsed * 1e3 / K), 4),
+                    "measured": f"HIP events on every launch over ticks {W}..{W + K - 1}, ungraphed run of the same "
+                                f"inputs"}
 
     forms = None
     if args.compare_forms:
