@@ -154,6 +154,8 @@ def main() -> int:
     ap.add_argument("--graph-ticks", type=int, default=None, help="ticks per captured hipGraph (default: per workload)")
     ap.add_argument("--grid-capacity", type=int, default=None, help="leaf capacity (default: tuned or 200)")
     ap.add_argument("--join-tile", type=int, default=None, choices=[256, 512])
+    ap.add_argument("--table-threads", type=int, default=None, choices=[256, 512, 1024],
+                    help="workgroup size of the table update (default: tuned or 256)")
     ap.add_argument("--load-factor", type=float, default=None)
     ap.add_argument("--autotune", action="store_true", help="tune leaf capacity, join tile, load factor and "
                                                              "precision on this workload before timing")
@@ -219,7 +221,7 @@ def main() -> int:
                                     "workload": desc, "trials": res.trials}
             autotune.save_table(table)
     for k, v in (("precision", args.precision), ("grid_capacity", args.grid_capacity), ("join_tile", args.join_tile),
-                 ("load_factor", args.load_factor)):
+                 ("load_factor", args.load_factor), ("table_threads", args.table_threads)):
         if v is not None:
             params[k] = v
     precision = runtime.PRECISION_FP32_GUARD if params["precision"] == "fp32guard" else runtime.PRECISION_FP64

@@ -4,6 +4,7 @@ path per workload size class by timing short device-resident runs.
 Knobs (none changes the emitted meetings except grid_capacity, which the
 reference's candidate rule makes part of the result):
   join_tile      points per LDS tile of the distance join (256, 512)
+  table_threads  workgroup size of the table update (256, 512, 1024)
   load_factor    table slots = next_pow2(distinct pairs per tick / load_factor)
   precision      FP64 test, or FP32 guard band with FP64 recheck (both exact)
   grid_capacity  leaf capacity (64..1024); tuned only when the caller allows
@@ -25,8 +26,9 @@ from . import gen, runtime
 
 TUNED_PATH = Path(__file__).resolve().parent / "tuned.json"
 
-DEFAULTS = {"join_tile": 512, "load_factor": 0.75, "precision": "fp32guard"}
-SPACE = {"join_tile": [256, 512], "load_factor": [0.375, 0.75], "precision": ["fp64", "fp32guard"]}
+DEFAULTS = {"join_tile": 512, "table_threads": 512, "load_factor": 0.75, "precision": "fp32guard"}
+SPACE = {"join_tile": [256, 512], "table_threads": [256, 512, 1024], "load_factor": [0.375, 0.75],
+         "precision": ["fp64", "fp32guard"]}
 LEAF_CAPACITIES = [64, 128, 256, 512, 1024]
 
 
@@ -49,7 +51,7 @@ def config_for(w: gen.Workload, params: dict, distinct: float, graph_ticks: int 
         num_meeting_buckets=table_slots(distinct, params.get("load_factor", 0.75)),
         unit_capacity=8 * n + 4096, meeting_capacity=max(1 << 20, 6 * n),
         contact_capacity=int(1.5 * distinct) + (1 << 20), graph_ticks=graph_ticks,
-        join_tile=params.get("join_tile", 512), **extra)
+        join_tile=params.get("join_tile", 512), table_threads=params.get("table_threads", 0), **extra)
 
 
 @dataclass

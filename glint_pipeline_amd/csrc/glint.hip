@@ -40,7 +40,7 @@ constexpr int kScanTile = kBlock * kScanItems;      // elements per scan block
 constexpr int kTile = 512;                          // points per LDS tile of the join (default)
 constexpr int kStage = 512;                         // staged contacts per wave
 constexpr int kFlushRound = 4;                      // join columns between stage checks
-constexpr int kMeetStage = 256;                     // staged meetings per wave (table update)
+constexpr int kMeetStage = 128;                     // staged meetings per wave (table update)
 constexpr uint32_t kChunk = 4096;                   // contact-list entries reserved per atomic
 constexpr int kRegion = 2048;                       // meeting-table slots per region (LDS-resident)
 constexpr int kBucketThreads = 1024;                // workgroup of the two bucketing passes
@@ -117,6 +117,7 @@ __global__ void k_advance_tick(TickParams *tp, uint64_t stride)
 {
     uint32_t k = tp->trace_index + 1;
     tp->trace_index = k;
+    if (tp->stamps && 2 * k < tp->nstamps) tp->stamps[2 * k] = wall_clock64();
     tp->xy = tp->trace_base + stride * k;
     tp->prev_tick = tp->tick;
     tp->tick = tp->next_tick;
@@ -154,13 +155,15 @@ __global__ void k_begin_run(TickParams *tp, const double *base, TableGen tab_old
     tp->has_prev_next = has_prev_next;
     tp->stamps = stamps;
     tp->nstamps = nstamps;
-    if (stamps && nstamps) stamps[0] = wall_clock64();
 }
 
+// Tick k of a run is stamped at its start (k_advance_tick, entry 2k) and at
+// its end (entry 2k+1), so a tick's latency excludes time spent waiting for
+// other work between ticks.
 __global__ void k_stamp(TickParams *tp)
 {
-    uint32_t k = tp->trace_index + 1;
-    if (tp->stamps && k < tp->nstamps) tp->stamps[k] = wall_clock64();
+    uint32_t k = tp->trace_index;
+    if (tp->stamps && 2 * k + 1 < tp->nstamps) tp->stamps[2 * k + 1] = wall_clock64();
 }
 
 __global__ void k_reset_tick(DevState s)
@@ -1204,13 +1207,14 @@ __global__ void k_bucket_single(DevState s)
 // previous generation (meeting.cpp:134-155) in LDS.  A pair of the previous
 // generation keeps its start tick if it is seen again; a pair not seen again
 // has ended at prev_tick and is emitted iff tick - start >= min + 1.
+template <int NT>
 struct TableLds {
     unsigned long long oldk[kRegion];
     unsigned long long newk[kRegion];
     uint32_t olds[kRegion];
     uint32_t news[kRegion];
-    uint32_t seen[kRegion];
-    glint_meeting stage[kBlock / 64][kMeetStage];
+    uint8_t seen[kRegion];
+    glint_meeting stage[NT / 64][kMeetStage];
 };
 
 // With the key-value output on, every pair also carries the float box of
@@ -1222,10 +1226,10 @@ struct TableLds {
 // thread that claims the slot), so the workgroup keeps the plain variant's
 // LDS footprint; the points come from a float copy of the positions that
 // fits the last-level cache.
-template <bool KV>
-__global__ __launch_bounds__(kBlock) void k_table(DevState s)
+template <bool KV, int NT>
+__global__ __launch_bounds__(NT) void k_table(DevState s)
 {
-    __shared__ TableLds L;
+    __shared__ TableLds<NT> L;
     const TableGen told = s.tp->tab_old, tcur = s.tp->tab_cur;
     const uint32_t tick = s.tp->tick, last = s.tp->prev_tick;
     const unsigned long long *list = s.lo_bits ? s.clist : s.clist2;
@@ -1236,13 +1240,13 @@ __global__ __launch_bounds__(kBlock) void k_table(DevState s)
     const bool void_tick = (s.ctr->error & kErrContactCap) != 0;
     // the previous generation's slots of the next region are loaded into
     // registers while the current region is processed
-    constexpr int kPer = kRegion / kBlock;
+    constexpr int kPer = kRegion / NT;
     unsigned long long pk[kPer];
     uint32_t ps[kPer];
     if (blockIdx.x < nreg) {
 #pragma unroll
         for (int q = 0; q < kPer; q++) {
-            const size_t i = (size_t)blockIdx.x * kRegion + threadIdx.x + q * kBlock;
+            const size_t i = (size_t)blockIdx.x * kRegion + threadIdx.x + q * NT;
             pk[q] = told.keys[i];
             ps[q] = told.starts[i];
         }
@@ -1251,7 +1255,7 @@ __global__ __launch_bounds__(kBlock) void k_table(DevState s)
         const size_t base = (size_t)r * kRegion;
 #pragma unroll
         for (int q = 0; q < kPer; q++) {
-            const uint32_t k = threadIdx.x + q * kBlock;
+            const uint32_t k = threadIdx.x + q * NT;
             L.oldk[k] = pk[q];
             L.olds[k] = ps[q];
             L.seen[k] = 0;
@@ -1261,14 +1265,14 @@ __global__ __launch_bounds__(kBlock) void k_table(DevState s)
         if (rn < nreg) {
 #pragma unroll
             for (int q = 0; q < kPer; q++) {
-                const size_t i = (size_t)rn * kRegion + threadIdx.x + q * kBlock;
+                const size_t i = (size_t)rn * kRegion + threadIdx.x + q * NT;
                 pk[q] = told.keys[i];
                 ps[q] = told.starts[i];
             }
         }
         __syncthreads();
         const uint32_t c0 = void_tick ? 0 : s.bstart[r], c1 = void_tick ? 0 : s.bstart[r + 1];
-        for (uint32_t c = c0 + threadIdx.x; c < c1; c += kBlock) {
+        for (uint32_t c = c0 + threadIdx.x; c < c1; c += NT) {
             const unsigned long long tagged = list[c];
             const unsigned long long key = tagged & ~(1ull << 63);
             const uint32_t h = (uint32_t)mix64(key) & (kRegion - 1);
@@ -1315,7 +1319,7 @@ __global__ __launch_bounds__(kBlock) void k_table(DevState s)
             if (!placed) atomicOr(&s.ctr->error, kErrTableFull);
         }
         __syncthreads();
-        for (uint32_t k0 = 0; k0 < kRegion; k0 += kBlock) {
+        for (uint32_t k0 = 0; k0 < kRegion; k0 += NT) {
             const uint32_t k = k0 + threadIdx.x;
             const unsigned long long ok = L.oldk[k];
             const bool emit = ok != kEmptyKey && !L.seen[k] && (tick - L.olds[k] >= s.min_meet + 1);
@@ -1735,10 +1739,24 @@ void enqueue_evaluate(glint_ctx *c)
     if (c->kv) launch(c, K_POS32, [&] { k_pos32<<<grid_for(c->s.n), kBlock, 0, c->stream>>>(c->s); });
     const uint32_t nreg = 1u << c->s.region_bits;
     const uint32_t g = std::min<uint32_t>(nreg, 4096);
-    if (c->kv)
-        launch(c, K_TABLE, [&] { k_table<true><<<g, kBlock, 0, c->stream>>>(c->s); });
-    else
-        launch(c, K_TABLE, [&] { k_table<false><<<g, kBlock, 0, c->stream>>>(c->s); });
+    const uint32_t nt = c->cfg.table_threads ? c->cfg.table_threads : 512;
+    launch(c, K_TABLE, [&] {
+        if (c->kv) {
+            if (nt == 1024)
+                k_table<true, 1024><<<g, 1024, 0, c->stream>>>(c->s);
+            else if (nt == 512)
+                k_table<true, 512><<<g, 512, 0, c->stream>>>(c->s);
+            else
+                k_table<true, 256><<<g, 256, 0, c->stream>>>(c->s);
+        } else {
+            if (nt == 1024)
+                k_table<false, 1024><<<g, 1024, 0, c->stream>>>(c->s);
+            else if (nt == 512)
+                k_table<false, 512><<<g, 512, 0, c->stream>>>(c->s);
+            else
+                k_table<false, 256><<<g, 256, 0, c->stream>>>(c->s);
+        }
+    });
     if (c->s.mlog) launch(c, K_LOG_APPEND, [&] { k_log_append<<<1024, kBlock, 0, c->stream>>>(c->s); });
 }
 
@@ -1832,7 +1850,9 @@ glint_status glint_create(const glint_config *cfg, glint_ctx **out)
     if (cfg->num_objects == 0 || cfg->grid_capacity == 0 || !(cfg->reach_distance > 0) ||
         !(cfg->high_x > cfg->low_x) || !(cfg->high_y > cfg->low_y) || cfg->candidate_mode > 1 ||
         cfg->precision > 1 || cfg->num_objects >= 0x80000000u ||
-        (cfg->join_tile != 0 && cfg->join_tile != 256 && cfg->join_tile != 512)) {
+        (cfg->join_tile != 0 && cfg->join_tile != 256 && cfg->join_tile != 512) ||
+        (cfg->table_threads != 0 && cfg->table_threads != 256 && cfg->table_threads != 512 &&
+         cfg->table_threads != 1024)) {
         g_create_error = "invalid configuration";
         return GLINT_ERR_INVALID_ARGUMENT;
     }
@@ -2260,19 +2280,19 @@ static glint_status run_begin(glint_ctx *c, const double *d_trace, uint32_t firs
         return GLINT_ERR_STATE;
     }
     HIP_TRY(c, hipSetDevice(c->device));
-    if (c->stamp_cap < nticks + 1) {
+    if (c->stamp_cap < 2 * nticks) {
         if (c->stamps) HIP_TRY(c, hipFree(c->stamps));
         c->stamps = nullptr;
         c->stamp_cap = 0;
-        HIP_TRY(c, hipMalloc((void **)&c->stamps, (size_t)(nticks + 1) * sizeof(unsigned long long)));
-        c->stamp_cap = nticks + 1;
+        HIP_TRY(c, hipMalloc((void **)&c->stamps, (size_t)2 * nticks * sizeof(unsigned long long)));
+        c->stamp_cap = 2 * nticks;
     }
     c->tick = first_tick;
     const int g = c->cur_gen;
     // one advance short of the first tick: tables and histories pre-swapped
     k_begin_run<<<1, 1, 0, c->stream>>>(c->s.tp, d_trace, c->tab[g], c->tab[1 - g], c->mbr[g], c->mbr[1 - g],
                                         c->hist[g], c->hist[1 - g],
-                                        c->prev_tick, first_tick, c->has_prev ? 1u : 0u, c->stamps, nticks + 1);
+                                        c->prev_tick, first_tick, c->has_prev ? 1u : 0u, c->stamps, 2 * nticks);
     HIP_TRY(c, hipGetLastError());
     if (!c->built) {
         glint_status st = build_tree(c);
@@ -2298,10 +2318,10 @@ static glint_status run_finish(glint_ctx *c, uint32_t nticks, float *tick_ms)
     glint_status st = read_counters(c);
     if (st) return st;
     if (tick_ms) {
-        std::vector<unsigned long long> h(nticks + 1);
+        std::vector<unsigned long long> h(2 * (size_t)nticks);
         HIP_TRY(c, hipMemcpy(h.data(), c->stamps, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
         const double khz = c->wall_khz > 0 ? c->wall_khz : 100000.0;
-        for (uint32_t k = 0; k < nticks; k++) tick_ms[k] = (float)((double)(h[k + 1] - h[k]) / khz);
+        for (uint32_t k = 0; k < nticks; k++) tick_ms[k] = (float)((double)(h[2 * k + 1] - h[2 * k]) / khz);
     }
     fill_stats(c);
     return status_from_flags(c, c->h_ctr->error);
@@ -2651,8 +2671,8 @@ glint_status glint_batch_run_ticks(glint_batch *b, const double *const *d_traces
         glint_status st = run_finish(c, nticks, nullptr);
         if (st && !first_err) first_err = ctx_fail(i, st);
         if (tick_ms) {
-            stamps[i].resize(nticks + 1);
-            if (hipMemcpy(stamps[i].data(), c->stamps, (nticks + 1) * sizeof(unsigned long long),
+            stamps[i].resize(2 * (size_t)nticks);
+            if (hipMemcpy(stamps[i].data(), c->stamps, 2 * (size_t)nticks * sizeof(unsigned long long),
                           hipMemcpyDeviceToHost) != hipSuccess && !first_err) {
                 b->err = "stamp copy failed";
                 first_err = GLINT_ERR_HIP;
@@ -2660,11 +2680,13 @@ glint_status glint_batch_run_ticks(glint_batch *b, const double *const *d_traces
         }
     }
     if (tick_ms && !first_err) {
-        // each scenario's own tick times (device wall clock between its stamps)
+        // each scenario's own tick times (device wall clock from its tick's
+        // start to its end, while the other scenarios share the device)
         const double khz = b->ctx[0]->wall_khz > 0 ? b->ctx[0]->wall_khz : 100000.0;
         for (uint32_t i = 0; i < n; i++)
             for (uint32_t k = 0; k < nticks; k++)
-                tick_ms[(size_t)i * nticks + k] = (float)((double)(stamps[i][k + 1] - stamps[i][k]) / khz);
+                tick_ms[(size_t)i * nticks + k] =
+                    (float)((double)(stamps[i][2 * k + 1] - stamps[i][2 * k]) / khz);
     }
     return first_err;
 }

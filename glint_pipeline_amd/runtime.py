@@ -73,7 +73,7 @@ class Config(ctypes.Structure):
         ("kv_area_limit", ctypes.c_double),
         ("meeting_log_capacity", ctypes.c_uint64),
         ("join_tile", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32),
+        ("table_threads", ctypes.c_uint32),
     ]
 
 
@@ -270,7 +270,8 @@ def make_config(num_objects: int, box, *, reach_distance: float = 2.0, grid_capa
                 meeting_capacity: int = 0, node_capacity: int = 0, record_contacts: bool = False,
                 contact_capacity: int = 0, device: int = 0, max_meet_time: int = 4096,
                 graph_ticks: int = 0, kv_capacity: int = 0, kv_split_num: int = 10,
-                kv_area_limit: float = 100.0, meeting_log_capacity: int = 0, join_tile: int = 0) -> Config:
+                kv_area_limit: float = 100.0, meeting_log_capacity: int = 0, join_tile: int = 0,
+                table_threads: int = 0) -> Config:
     c = Config()
     lib().glint_config_default(ctypes.byref(c))
     c.num_objects = num_objects
@@ -298,6 +299,7 @@ def make_config(num_objects: int, box, *, reach_distance: float = 2.0, grid_capa
     c.kv_area_limit = kv_area_limit
     c.meeting_log_capacity = meeting_log_capacity
     c.join_tile = join_tile
+    c.table_threads = table_threads
     return c
 
 

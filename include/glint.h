@@ -88,7 +88,7 @@ typedef struct glint_config {
     double kv_area_limit;          /* dProbFalse (process.cpp:408-411): object box area, squared input units */
     uint64_t meeting_log_capacity; /* meetings kept in the device meeting log ring; 0: log off */
     uint32_t join_tile;            /* points per LDS tile of the distance join: 256 or 512 (0: 512) */
-    uint32_t reserved;
+    uint32_t table_threads;        /* workgroup size of the table update: 256, 512 or 1024 (0: 512) */
 } glint_config;
 
 /* One emitted meeting: pid_a < pid_b, first and last tick of contact. */
@@ -178,7 +178,8 @@ glint_status glint_process_tick(glint_ctx *ctx, const double *xy, uint32_t tick)
  * between ticks; meetings stay on the device.  With cfg.graph_ticks = G the
  * ticks run as replays of one captured hipGraph of G ticks.  If tick_ms is
  * not NULL it receives each tick's latency in milliseconds from the device
- * wall clock.  The reference drives the same loop from the host, one tick
+ * wall clock (from the tick's first launch to the end of its last).  The
+ * reference drives the same loop from the host, one tick
  * per call (process.cpp:431). */
 glint_status glint_run_ticks(glint_ctx *ctx, const double *d_trace, uint32_t first_tick,
                              uint32_t nticks, float *tick_ms);

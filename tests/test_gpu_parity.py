@@ -46,14 +46,17 @@ def test_parity_hotspots_small():
     assert tot["contacts"] > 10_000
 
 
-@pytest.mark.parametrize("join_tile,capacity", [(256, 600), (512, 1024), (256, 64), (256, 256)])
-def test_parity_join_tiles_and_leaf_capacities(join_tile, capacity):
-    """Leaves larger than one LDS tile are joined tile by tile; every tile size
-    and leaf capacity gives the oracle's result at that capacity."""
+@pytest.mark.parametrize("join_tile,capacity,table_threads",
+                         [(256, 600, 512), (512, 1024, 1024), (256, 64, 256), (256, 256, 1024)])
+def test_parity_join_tiles_and_leaf_capacities(join_tile, capacity, table_threads):
+    """Leaves larger than one LDS tile are joined tile by tile; every tile size,
+    leaf capacity and table-update workgroup size gives the oracle's result at
+    that capacity."""
     w = gen.skewed_density(40_000, 12)
     w.side = 3000.0
     w.grid_capacity = capacity
-    run_parity(w, 12, join_tile=join_tile, precision=runtime.PRECISION_FP32_GUARD, check_leaves_every=6)
+    run_parity(w, 12, join_tile=join_tile, table_threads=table_threads, precision=runtime.PRECISION_FP32_GUARD,
+               check_leaves_every=6)
 
 
 def test_parity_plain_cpu_reference_config():
