@@ -58,27 +58,27 @@ def workload_for(name: str, objects: int | None):
     return w, desc.format(n=w.num_objects)
 
 
-def algorithmic_bytes(kernel: str, n: int, per_tick: dict, slots: int, precision: int) -> float:
+def algorithmic_bytes(kernel: str, n: int, per_tick: dict, precision: int) -> float:
     """Algorithmic HBM bytes of one launch of `kernel` (one tick), from the
     per-tick figures: N objects, C contacts (border share Cb), M meetings,
-    U border units, B table slots."""
+    U border units."""
     C, Cb, M, U = per_tick["contacts"], per_tick["contacts_border"], per_tick["meetings"], per_tick["units"]
     coord = 8 if precision else 16          # FP32 offsets or FP64 coordinates per object
     if kernel == "k_join":
-        # every sorted coordinate once, each border unit (id + xy), one 16 B upsert per contact
-        return coord * n + 24.0 * U + 16.0 * C
+        # every sorted object once (coordinates, previous FP64 position, id),
+        # each border unit (id + xy), one 8 B key per contact out
+        return (coord + 16.0 + 4.0) * n + 24.0 * U + 8.0 * C
     if kernel == "k_table":
-        # previous generation read and new generation written (12 B slots:
-        # 8 B key + 4 B start), every bucketed contact read once, meetings out
-        return 24.0 * slots + 8.0 * C + 16.0 * M
+        # the previous generation's entries (8 B key + 4 B start; as many as
+        # the tick before had contacts, taken equal to C), every contact's key
+        # read and its start written, meetings out
+        return 12.0 * C + 12.0 * C + 16.0 * M
     if kernel == "k_bucket_fine":
         return 24.0 * C                       # segment read twice (histogram, tiles), written once
     if kernel == "k_bucket_scatter":
         return 16.0 * C
     if kernel == "k_bucket_count":
         return 8.0 * C
-    if kernel == "k_evaluate":
-        return 16.0 * slots + 16.0 * M        # table scan + meeting output
     if kernel == "k_assign":
         return 16.0 * n + 8.0 * n             # xy in, leaf and rank out
     if kernel == "k_scatter":
@@ -282,7 +282,7 @@ def main() -> int:
     roofline = None
     if ktimes:
         dom, (dms, dcnt) = max(((k, v) for k, v in ktimes.items() if v[1]), key=lambda kv: kv[1][0])
-        abytes = algorithmic_bytes(dom, N, per_tick, slots, precision)
+        abytes = algorithmic_bytes(dom, N, per_tick, precision)
         avg_s = dms / dcnt / 1e3
         achieved = abytes / avg_s / 1e9 if abytes and avg_s > 0 else 0.0
         traffic = profiled_traffic(dom, args.workload, precision, N)

@@ -16,9 +16,10 @@ struct CtxView {
     uint32_t tick;              // last evaluated tick
     uint32_t min_meet;
     bool evaluated;
-    const unsigned long long *keys;   // active pairs by table slot (~0 when free)
-    const uint32_t *starts;           // first tick of each pair's run
-    uint64_t slots;
+    const unsigned long long *keys;   // the tick's pairs (bit 63: a tag, masked off by readers)
+    const uint32_t *starts;           // first tick of each pair's run (~0: a second copy, skipped)
+    const uint32_t *count;            // device word holding the number of entries
+    uint64_t cap;                     // entries the list can hold
     const glint_meeting *mlog;  // meeting log ring (null when off)
     uint64_t mlog_cap;
 };

@@ -707,6 +707,15 @@ __global__ __launch_bounds__(kBlock) void k_unit_scatter(DevState s)
 // and every region is then updated by one workgroup in LDS: the region of the
 // previous generation is read once, the new region is written once.
 
+// The join writes the tick's contacts where the first bucketing pass reads
+// them: into the current generation's list when there are two passes (the
+// second pass writes the grouped list back there), else into the bucketing
+// buffer (the one pass writes the grouped list into the generation).
+__device__ __forceinline__ unsigned long long *contact_out(const DevState &s)
+{
+    return s.lo_bits ? s.tp->tab_cur.keys : s.clist2;
+}
+
 __device__ __forceinline__ uint32_t region_of(const DevState &s, unsigned long long key)
 {
     // a one-region table has no region bits (a shift by 64 is undefined)
@@ -716,6 +725,7 @@ __device__ __forceinline__ uint32_t region_of(const DevState &s, unsigned long l
 // Contacts found by one wave are staged in LDS and appended to the tick's
 // contact list in 4096-entry chunks reserved with one atomic each.
 struct Stager {
+    unsigned long long *out;    // the tick's contact list
     unsigned long long *buf;
     uint32_t n;                 // staged entries (uniform across the wave)
     unsigned long long chunk;   // current chunk in the contact list
@@ -744,7 +754,7 @@ __device__ __forceinline__ void stage_flush(Stager &st, const DevState &s)
         }
         const uint32_t take = min(st.n - done, kChunk - st.used);
         if (st.chunk + kChunk <= s.clist_cap) {
-            for (uint32_t k = lane; k < take; k += 64) s.clist[st.chunk + st.used + k] = st.buf[done + k];
+            for (uint32_t k = lane; k < take; k += 64) st.out[st.chunk + st.used + k] = st.buf[done + k];
         } else if (lane == 0) {
             atomicOr(&s.ctr->error, kErrContactCap);
         }
@@ -760,7 +770,7 @@ __device__ __forceinline__ void stage_close(Stager &st, const DevState &s)
 {
     stage_flush(st, s);
     if (st.used < kChunk && st.chunk + kChunk <= s.clist_cap)
-        for (uint32_t k = st.used + lane_id(); k < kChunk; k += 64) s.clist[st.chunk + k] = kEmptyKey;
+        for (uint32_t k = st.used + lane_id(); k < kChunk; k += 64) st.out[st.chunk + k] = kEmptyKey;
 }
 
 // ------------------------------------------------------------ refine: join
@@ -852,7 +862,7 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
     const double t64 = s.t64;
     const uint32_t nl = (s.ctr->error & kErrUnitCap) ? 0 : s.ctr->n_leaf_list;
     const uint32_t wave = threadIdx.x / 64, lane = lane_id();
-    Stager st{lds.stage[wave], 0, 0, kChunk};
+    Stager st{contact_out(s), lds.stage[wave], 0, 0, kChunk};
     uint32_t n_in = 0, n_bd = 0, n_re = 0;   // per lane
     PairTest pt{t64, s.t32_in, s.t32_out, false};
     for (;;) {
@@ -952,8 +962,9 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
                                     if (hit && has_prev) {
                                         const double dx = pxi - C.px[j];
                                         const double dy = pyi - C.py[j];
-                                        if (dx * dx + dy * dy <= t64) key |= 1ull << 63;
+                                        if (dx * dx + dy * dy <= t64) key |= kTagPrev;
                                     }
+                                    if (!inside) key |= kTagBorder;
                                     if (inside)
                                         n_in += hit;
                                     else
@@ -996,7 +1007,7 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
 __device__ __forceinline__ uint32_t coarse_bin(const DevState &s, unsigned long long tagged)
 {
     if (tagged == kEmptyKey) return s.coarse_bins;
-    return region_of(s, tagged & ~(1ull << 63)) >> s.lo_bits;
+    return region_of(s, tagged & kKeyBits) >> s.lo_bits;
 }
 
 __device__ __forceinline__ void block_range(const DevState &s, unsigned long long n, unsigned long long &b0,
@@ -1015,7 +1026,8 @@ __global__ __launch_bounds__(kBlock) void k_bucket_count(DevState s)
     __syncthreads();
     unsigned long long b0, b1;
     block_range(s, n, b0, b1);
-    for (unsigned long long i = b0 + threadIdx.x; i < b1; i += kBlock) atomicAdd(&hist[coarse_bin(s, s.clist[i])], 1u);
+    const unsigned long long *in = contact_out(s);
+    for (unsigned long long i = b0 + threadIdx.x; i < b1; i += kBlock) atomicAdd(&hist[coarse_bin(s, in[i])], 1u);
     __syncthreads();
     for (uint32_t k = threadIdx.x; k <= s.coarse_bins; k += kBlock) s.bcount[(size_t)k * gridDim.x + blockIdx.x] = hist[k];
 }
@@ -1047,6 +1059,8 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_scatter(DevState s)
     __shared__ uint32_t wsum[kBucketThreads / 64];
     const unsigned long long n = min(s.ctr->n_clist, s.clist_cap);
     const uint32_t nb = s.coarse_bins + 1;
+    const unsigned long long *in = contact_out(s);
+    unsigned long long *out = s.lo_bits ? s.clist2 : s.tp->tab_cur.keys;
     unsigned long long b0, b1;
     block_range(s, n, b0, b1);
     for (uint32_t k = threadIdx.x; k < nb; k += kBucketThreads) running[k] = s.boff[(size_t)k * gridDim.x + blockIdx.x];
@@ -1058,7 +1072,7 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_scatter(DevState s)
 #pragma unroll
         for (int q = 0; q < kBucketTile / kBucketThreads; q++) {
             uint32_t i = threadIdx.x + q * kBucketThreads;
-            keys[q] = i < tn ? s.clist[t0 + i] : kEmptyKey;
+            keys[q] = i < tn ? in[t0 + i] : kEmptyKey;
             if (i < tn) atomicAdd(&hist[coarse_bin(s, keys[q])], 1u);
         }
         __syncthreads();
@@ -1099,7 +1113,7 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_scatter(DevState s)
         for (uint32_t p = threadIdx.x; p < tn; p += kBucketThreads) {
             unsigned long long key = tile[p];
             uint32_t b = coarse_bin(s, key);
-            if (b < s.coarse_bins) s.clist2[running[b] + (p - start[b])] = key;
+            if (b < s.coarse_bins) out[running[b] + (p - start[b])] = key;
         }
         __syncthreads();
         for (uint32_t k = threadIdx.x; k < nb; k += kBucketThreads) running[k] += hist[k];
@@ -1108,7 +1122,8 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_scatter(DevState s)
 }
 
 // Pass 2: one workgroup per coarse bin c splits its segment of clist2 by the
-// low region bits into clist and writes the first contact of each region.
+// low region bits into the generation's list and writes the first contact of
+// each region.
 // Tiles of the segment are sorted by fine bin in LDS so that the writes form
 // runs (a per-item scatter into 512-2048 streams would write partial lines).
 __device__ __forceinline__ uint32_t block_scan_bins(uint32_t *bins, uint32_t nbins, uint32_t *out_excl,
@@ -1146,15 +1161,17 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_fine(DevState s)
     __shared__ uint32_t wsum[kBucketThreads / 64];
     const uint32_t G1 = s.bucket_blocks;
     const uint32_t nf = 1u << s.lo_bits;
+    unsigned long long *out = s.tp->tab_cur.keys;
+    uint32_t *rstart = s.tp->tab_cur.rstart;
     for (uint32_t c = blockIdx.x; c < s.coarse_bins; c += gridDim.x) {
         const uint32_t seg0 = s.boff[(size_t)c * G1], seg1 = s.boff[(size_t)(c + 1) * G1];
         for (uint32_t k = threadIdx.x; k < nf; k += kBucketThreads) thist[k] = 0;
         __syncthreads();
         for (uint32_t i = seg0 + threadIdx.x; i < seg1; i += kBucketThreads)
-            atomicAdd(&thist[region_of(s, s.clist2[i] & ~(1ull << 63)) & (nf - 1)], 1u);
+            atomicAdd(&thist[region_of(s, s.clist2[i] & kKeyBits) & (nf - 1)], 1u);
         __syncthreads();
         block_scan_bins(thist, nf, gcur, seg0, wsum);
-        for (uint32_t k = threadIdx.x; k < nf; k += kBucketThreads) s.bstart[(size_t)c * nf + k] = gcur[k];
+        for (uint32_t k = threadIdx.x; k < nf; k += kBucketThreads) rstart[(size_t)c * nf + k] = gcur[k];
         __syncthreads();
         for (uint32_t t0 = seg0; t0 < seg1; t0 += kFineTile) {
             const uint32_t tn = min((uint32_t)kFineTile, seg1 - t0);
@@ -1166,7 +1183,7 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_fine(DevState s)
             for (int q = 0; q < kFineTile / kBucketThreads; q++) {
                 const uint32_t i = threadIdx.x + q * kBucketThreads;
                 keys[q] = i < tn ? s.clist2[t0 + i] : 0ull;
-                fb[q] = region_of(s, keys[q] & ~(1ull << 63)) & (nf - 1);
+                fb[q] = region_of(s, keys[q] & kKeyBits) & (nf - 1);
                 if (i < tn) atomicAdd(&thist[fb[q]], 1u);
             }
             __syncthreads();
@@ -1181,23 +1198,24 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_fine(DevState s)
             __syncthreads();
             for (uint32_t p = threadIdx.x; p < tn; p += kBucketThreads) {
                 const unsigned long long key = tile[p];
-                const uint32_t f = region_of(s, key & ~(1ull << 63)) & (nf - 1);
-                s.clist[gcur[f] + (p - tstart[f])] = key;
+                const uint32_t f = region_of(s, key & kKeyBits) & (nf - 1);
+                out[gcur[f] + (p - tstart[f])] = key;
             }
             __syncthreads();
             for (uint32_t k = threadIdx.x; k < nf; k += kBucketThreads) gcur[k] += thist[k];
             __syncthreads();
         }
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) s.bstart[(size_t)s.coarse_bins * nf] = s.boff[(size_t)s.coarse_bins * G1];
+    if (blockIdx.x == 0 && threadIdx.x == 0) rstart[(size_t)s.coarse_bins * nf] = s.boff[(size_t)s.coarse_bins * G1];
 }
 
 // Without a second pass the coarse bins are the regions.
 __global__ void k_bucket_single(DevState s)
 {
     const uint32_t G1 = s.bucket_blocks;
+    uint32_t *rstart = s.tp->tab_cur.rstart;
     for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c <= s.coarse_bins; c += gridDim.x * blockDim.x)
-        s.bstart[c] = s.boff[(size_t)c * G1];
+        rstart[c] = s.boff[(size_t)c * G1];
 }
 
 // ------------------------------------------------------------ table update
@@ -1207,198 +1225,266 @@ __global__ void k_bucket_single(DevState s)
 // previous generation (meeting.cpp:134-155) in LDS.  A pair of the previous
 // generation keeps its start tick if it is seen again; a pair not seen again
 // has ended at prev_tick and is emitted iff tick - start >= min + 1.
-template <int NT>
+//
+// A generation is not a slot table in HBM but the tick's bucketed contact
+// list itself, with one start tick per entry.  Only a contact tagged as
+// within reach at prev_tick can continue a run, and only one found through
+// a border unit can be a second copy of a pair (an inside pair is found once,
+// by one leaf).  The workgroup therefore hashes just those contacts of its
+// region into LDS (removing second copies), streams the previous
+// generation's entries of the region past the hash (a hit continues a run,
+// a miss has ended), and writes every contact's start beside it.  HBM
+// traffic is 12 bytes per entry of each generation plus the key read; no
+// slot is ever empty.
+constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
+
+template <bool KV, int NT>
 struct TableLds {
-    unsigned long long oldk[kRegion];
-    unsigned long long newk[kRegion];
-    uint32_t olds[kRegion];
-    uint32_t news[kRegion];
-    uint8_t seen[kRegion];
+    unsigned long long hk[kRegion];   // hashed contacts of the tick (kEmptyKey: free)
+    uint32_t hc[kRegion];             // their list index
+    uint32_t hs[kRegion];             // their start tick
+    uint32_t ho[KV ? kRegion : 1];    // list index of the old entry they continue (kNoSlot: none)
     glint_meeting stage[NT / 64][kMeetStage];
 };
+
+template <bool KV, int NT>
+__device__ __forceinline__ void table_flush_stage(const DevState &s, glint_meeting *st, uint32_t &nst)
+{
+    wave_sync();
+    unsigned long long b = 0;
+    if (lane_id() == 0) b = atomicAdd(&s.ctr->n_meet, (unsigned long long)nst);
+    b = __shfl(b, 0);
+    for (uint32_t j = lane_id(); j < nst; j += 64) {
+        if (b + j < s.meet_cap)
+            s.meet[b + j] = st[j];
+        else
+            atomicOr(&s.ctr->error, kErrMeetCap);
+    }
+    wave_sync();
+    nst = 0;
+}
+
+// Hashes contact c (tagged or border) of the tick; returns its slot, or
+// kNoSlot for a second copy of a pair (its start is marked) or a full region.
+template <bool KV, int NT>
+__device__ __forceinline__ uint32_t table_put(TableLds<KV, NT> &L, const DevState &s, const TableGen &tcur,
+                                              unsigned long long key, uint32_t c, uint32_t tick)
+{
+    const uint32_t h = (uint32_t)mix64(key) & (kRegion - 1);
+    for (uint32_t p = 0; p < kRegion; p++) {
+        const uint32_t q = (h + p) & (kRegion - 1);
+        const unsigned long long prev = atomicCAS(&L.hk[q], kEmptyKey, key);
+        if (prev == kEmptyKey) {
+            L.hc[q] = c;
+            L.hs[q] = tick;
+            if constexpr (KV) L.ho[q] = kNoSlot;
+            return q;
+        }
+        if (prev == key) {
+            tcur.starts[c] = kDupStart;
+            return kNoSlot;
+        }
+    }
+    atomicOr(&s.ctr->error, kErrTableFull);
+    return kNoSlot;
+}
+
+template <bool KV, int NT>
+__device__ __forceinline__ uint32_t table_find(const TableLds<KV, NT> &L, unsigned long long key)
+{
+    const uint32_t h = (uint32_t)mix64(key) & (kRegion - 1);
+    for (uint32_t p = 0; p < kRegion; p++) {
+        const uint32_t q = (h + p) & (kRegion - 1);
+        const unsigned long long k = L.hk[q];
+        if (k == key) return q;
+        if (k == kEmptyKey) return kNoSlot;
+    }
+    return kNoSlot;
+}
+
+// One entry of the previous generation: continues the run of the tick's
+// contact of the same pair, or has ended (emitted if long enough).
+template <bool KV, int NT>
+__device__ __forceinline__ void table_old(TableLds<KV, NT> &L, const DevState &s, bool valid,
+                                          unsigned long long tagged, uint32_t start, uint32_t i, uint32_t tick,
+                                          uint32_t last, glint_meeting *st, uint32_t &nst)
+{
+    valid = valid && start != kDupStart;
+    const unsigned long long key = tagged & kKeyBits;
+    bool emit = false;
+    if (valid) {
+        const uint32_t q = table_find(L, key);
+        if (q != kNoSlot) {
+            L.hs[q] = start;
+            if constexpr (KV) L.ho[q] = i;
+        } else {
+            emit = tick - start >= s.min_meet + 1;
+        }
+    }
+    const uint32_t lane = lane_id();
+    const unsigned long long m = __ballot(emit);
+    if (emit) {
+        glint_meeting mt;
+        mt.pid_a = (uint32_t)(key >> 32);
+        mt.pid_b = (uint32_t)key;
+        mt.start = start;
+        mt.end = last;
+        st[nst + __popcll(m & ((1ull << lane) - 1ull))] = mt;
+    }
+    nst += (uint32_t)__popcll(m);
+    if constexpr (KV) {
+        // meetings whose box lies in the map box are logged
+        // (geometry.cu:665-670, box_contain geometry.cu:53-58)
+        float4 bx = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (emit) bx = s.tp->mbr_old[i];
+        const bool put = emit && bx.x >= s.lx && bx.z <= s.hx && bx.y >= s.ly && bx.w <= s.hy;
+        const unsigned long long mk = __ballot(put);
+        if (mk) {
+            unsigned long long b = 0;
+            if (lane == 0) b = atomicAdd(&s.ctr->n_kvlog, (unsigned long long)__popcll(mk));
+            b = __shfl(b, 0);
+            if (put) {
+                const unsigned long long j = b + __popcll(mk & ((1ull << lane) - 1ull));
+                if (j < s.kvlog_cap) {
+                    KvLogEntry e;
+                    e.pid_a = (uint32_t)(key >> 32);
+                    e.pid_b = (uint32_t)key;
+                    e.start = start;
+                    e.end = last;
+                    e.box = bx;
+                    s.kvlog[j] = e;
+                } else {
+                    atomicOr(&s.ctr->error, kErrKvCap);
+                }
+            }
+        }
+    }
+    if (nst > kMeetStage - 64) table_flush_stage<KV, NT>(s, st, nst);
+}
+
+// Start (and, with the key-value output, box) of contact c, whose hash slot
+// is q (kNoSlot: a contact that was not hashed, new at this tick).
+template <bool KV, int NT>
+__device__ __forceinline__ void table_write(const TableLds<KV, NT> &L, const DevState &s, const TableGen &tcur,
+                                            unsigned long long key, uint32_t c, uint32_t q, uint32_t tick)
+{
+    tcur.starts[c] = q == kNoSlot ? tick : L.hs[q];
+    if constexpr (KV) {
+        const float2 pa = s.pos32[(uint32_t)(key >> 32)], pb = s.pos32[(uint32_t)key];
+        float4 m = make_float4(fminf(pa.x, pb.x), fminf(pa.y, pb.y), fmaxf(pa.x, pb.x), fmaxf(pa.y, pb.y));
+        if (q != kNoSlot && L.ho[q] != kNoSlot) {
+            const float4 o = s.tp->mbr_old[L.ho[q]];
+            m = make_float4(fminf(m.x, o.x), fminf(m.y, o.y), fmaxf(m.z, o.z), fmaxf(m.w, o.w));
+        }
+        s.tp->mbr_cur[c] = m;
+    }
+}
 
 // With the key-value output on, every pair also carries the float box of
 // both objects over its run (geometry.cu:469-494): the box of a new pair is
 // its two points at this tick, a continuing pair adds them to its old box.
 // min/max of the rounded coordinates equals the rounded min/max, so the box
-// does not depend on the order of the updates.  Boxes stay in HBM beside
-// their slots (read only for continuing and ended pairs, written by the
-// thread that claims the slot), so the workgroup keeps the plain variant's
-// LDS footprint; the points come from a float copy of the positions that
+// does not depend on the order of the updates.  Boxes stay in HBM beside the
+// list entries; the points come from a float copy of the positions that
 // fits the last-level cache.
 template <bool KV, int NT>
 __global__ __launch_bounds__(NT) void k_table(DevState s)
 {
-    __shared__ TableLds<NT> L;
+    __shared__ TableLds<KV, NT> L;
     const TableGen told = s.tp->tab_old, tcur = s.tp->tab_cur;
     const uint32_t tick = s.tp->tick, last = s.tp->prev_tick;
-    const unsigned long long *list = s.lo_bits ? s.clist : s.clist2;
     const uint32_t nreg = 1u << s.region_bits;
-    const uint32_t lane = lane_id();
     glint_meeting *st = L.stage[threadIdx.x / 64];
-    uint32_t nst = 0, inserted = 0;
+    uint32_t nst = 0, dups = 0;
+    unsigned long long total = 0;
     const bool void_tick = (s.ctr->error & kErrContactCap) != 0;
-    // the previous generation's slots of the next region are loaded into
+    // the previous generation's entries of the next region are loaded into
     // registers while the current region is processed
     constexpr int kPer = kRegion / NT;
-    unsigned long long pk[kPer];
-    uint32_t ps[kPer];
+    unsigned long long pk[kPer], nk[kPer];
+    uint32_t ps[kPer], ns[kPer];
+    uint32_t o0 = 0, o1 = 0;
     if (blockIdx.x < nreg) {
+        o0 = told.rstart[blockIdx.x];
+        o1 = told.rstart[blockIdx.x + 1];
 #pragma unroll
         for (int q = 0; q < kPer; q++) {
-            const size_t i = (size_t)blockIdx.x * kRegion + threadIdx.x + q * NT;
-            pk[q] = told.keys[i];
-            ps[q] = told.starts[i];
+            const uint32_t i = o0 + threadIdx.x + q * NT;
+            pk[q] = i < o1 ? told.keys[i] : kEmptyKey;
+            ps[q] = i < o1 ? told.starts[i] : kDupStart;
         }
     }
     for (uint32_t r = blockIdx.x; r < nreg; r += gridDim.x) {
-        const size_t base = (size_t)r * kRegion;
+        for (uint32_t k = threadIdx.x; k < kRegion; k += NT) L.hk[k] = kEmptyKey;
+        const uint32_t c0 = void_tick ? 0 : tcur.rstart[r], c1 = void_tick ? 0 : tcur.rstart[r + 1];
+        total += (threadIdx.x == 0) ? c1 - c0 : 0;
 #pragma unroll
         for (int q = 0; q < kPer; q++) {
-            const uint32_t k = threadIdx.x + q * NT;
-            L.oldk[k] = pk[q];
-            L.olds[k] = ps[q];
-            L.seen[k] = 0;
-            L.newk[k] = kEmptyKey;
+            const uint32_t c = c0 + threadIdx.x + q * NT;
+            nk[q] = c < c1 ? tcur.keys[c] : 0ull;
+        }
+        __syncthreads();
+        // the tick's contacts that can continue a run or be a second copy
+#pragma unroll
+        for (int q = 0; q < kPer; q++) {
+            const uint32_t c = c0 + threadIdx.x + q * NT;
+            ns[q] = kNoSlot;
+            if (c < c1 && (nk[q] & (kTagPrev | kTagBorder))) {
+                ns[q] = table_put(L, s, tcur, nk[q] & kKeyBits, c, tick);
+                dups += ns[q] == kNoSlot;
+                if (ns[q] == kNoSlot) ns[q] = kNoSlot - 1;   // nothing left to write
+            }
+        }
+        for (uint32_t c = c0 + kRegion + threadIdx.x; c < c1; c += NT) {   // regions of more than kRegion contacts
+            const unsigned long long k = tcur.keys[c];
+            if (k & (kTagPrev | kTagBorder)) dups += table_put(L, s, tcur, k & kKeyBits, c, tick) == kNoSlot;
+        }
+        __syncthreads();
+        // the previous generation's entries of the region
+#pragma unroll
+        for (int q = 0; q < kPer; q++) {
+            const uint32_t i = o0 + threadIdx.x + q * NT;
+            table_old(L, s, i < o1, pk[q], ps[q], i, tick, last, st, nst);
+        }
+        for (uint32_t b = o0 + kRegion; b < o1; b += NT) {   // regions of more than kRegion entries
+            const uint32_t i = b + threadIdx.x;
+            const bool v = i < o1;
+            table_old(L, s, v, v ? told.keys[i] : 0ull, v ? told.starts[i] : kDupStart, i, tick, last, st, nst);
         }
         const uint32_t rn = r + gridDim.x;
         if (rn < nreg) {
+            o0 = told.rstart[rn];
+            o1 = told.rstart[rn + 1];
 #pragma unroll
             for (int q = 0; q < kPer; q++) {
-                const size_t i = (size_t)rn * kRegion + threadIdx.x + q * NT;
-                pk[q] = told.keys[i];
-                ps[q] = told.starts[i];
+                const uint32_t i = o0 + threadIdx.x + q * NT;
+                pk[q] = i < o1 ? told.keys[i] : kEmptyKey;
+                ps[q] = i < o1 ? told.starts[i] : kDupStart;
             }
         }
         __syncthreads();
-        const uint32_t c0 = void_tick ? 0 : s.bstart[r], c1 = void_tick ? 0 : s.bstart[r + 1];
-        for (uint32_t c = c0 + threadIdx.x; c < c1; c += NT) {
-            const unsigned long long tagged = list[c];
-            const unsigned long long key = tagged & ~(1ull << 63);
-            const uint32_t h = (uint32_t)mix64(key) & (kRegion - 1);
-            uint32_t start = tick;
-            int32_t oq = -1;
-            if (tagged >> 63) {   // within reach at prev_tick: it may continue a run
-                for (uint32_t p = 0; p < kRegion; p++) {
-                    const uint32_t q = (h + p) & (kRegion - 1);
-                    const unsigned long long k = L.oldk[q];
-                    if (k == key) {
-                        start = L.olds[q];
-                        L.seen[q] = 1;
-                        oq = (int32_t)q;
-                        break;
-                    }
-                    if (k == kEmptyKey) break;
-                }
-            }
-            bool placed = false;
-            for (uint32_t p = 0; p < kRegion; p++) {
-                const uint32_t q = (h + p) & (kRegion - 1);
-                const unsigned long long prev = atomicCAS(&L.newk[q], kEmptyKey, key);
-                if (prev == kEmptyKey) {
-                    L.news[q] = start;
-                    if constexpr (KV) {
-                        const float2 pa = s.pos32[(uint32_t)(key >> 32)], pb = s.pos32[(uint32_t)key];
-                        float4 m = make_float4(fminf(pa.x, pb.x), fminf(pa.y, pb.y), fmaxf(pa.x, pb.x),
-                                               fmaxf(pa.y, pb.y));
-                        if (oq >= 0) {
-                            const float4 o = s.tp->mbr_old[base + oq];
-                            m = make_float4(fminf(m.x, o.x), fminf(m.y, o.y), fmaxf(m.z, o.z), fmaxf(m.w, o.w));
-                        }
-                        s.tp->mbr_cur[base + q] = m;
-                    }
-                    inserted++;
-                    placed = true;
-                    break;
-                }
-                if (prev == key) {   // found twice in this tick: one entry
-                    placed = true;
-                    break;
-                }
-            }
-            if (!placed) atomicOr(&s.ctr->error, kErrTableFull);
+        // starts (and boxes) of every contact of the region
+#pragma unroll
+        for (int q = 0; q < kPer; q++) {
+            const uint32_t c = c0 + threadIdx.x + q * NT;
+            if (c < c1 && ns[q] != kNoSlot - 1) table_write(L, s, tcur, nk[q] & kKeyBits, c, ns[q], tick);
         }
-        __syncthreads();
-        for (uint32_t k0 = 0; k0 < kRegion; k0 += NT) {
-            const uint32_t k = k0 + threadIdx.x;
-            const unsigned long long ok = L.oldk[k];
-            const bool emit = ok != kEmptyKey && !L.seen[k] && (tick - L.olds[k] >= s.min_meet + 1);
-            unsigned long long m = __ballot(emit);
-            if (emit) {
-                glint_meeting mt;
-                mt.pid_a = (uint32_t)(ok >> 32);
-                mt.pid_b = (uint32_t)ok;
-                mt.start = L.olds[k];
-                mt.end = last;
-                st[nst + __popcll(m & ((1ull << lane) - 1ull))] = mt;
+        for (uint32_t c = c0 + kRegion + threadIdx.x; c < c1; c += NT) {
+            const unsigned long long k = tcur.keys[c];
+            uint32_t q = kNoSlot;
+            if (k & (kTagPrev | kTagBorder)) {
+                q = table_find(L, k & kKeyBits);
+                if (q == kNoSlot || L.hc[q] != c) continue;   // a second copy (already marked)
             }
-            nst += (uint32_t)__popcll(m);
-            if constexpr (KV) {
-                // meetings whose box lies in the map box are logged
-                // (geometry.cu:665-670, box_contain geometry.cu:53-58)
-                float4 bx = make_float4(0.f, 0.f, 0.f, 0.f);
-                if (emit) bx = s.tp->mbr_old[base + k];
-                const bool put = emit && bx.x >= s.lx && bx.z <= s.hx && bx.y >= s.ly && bx.w <= s.hy;
-                const unsigned long long mk = __ballot(put);
-                if (mk) {
-                    unsigned long long b = 0;
-                    if (lane == 0) b = atomicAdd(&s.ctr->n_kvlog, (unsigned long long)__popcll(mk));
-                    b = __shfl(b, 0);
-                    if (put) {
-                        const unsigned long long j = b + __popcll(mk & ((1ull << lane) - 1ull));
-                        if (j < s.kvlog_cap) {
-                            KvLogEntry e;
-                            e.pid_a = (uint32_t)(ok >> 32);
-                            e.pid_b = (uint32_t)ok;
-                            e.start = L.olds[k];
-                            e.end = last;
-                            e.box = bx;
-                            s.kvlog[j] = e;
-                        } else {
-                            atomicOr(&s.ctr->error, kErrKvCap);
-                        }
-                    }
-                }
-            }
-            if (nst > kMeetStage - 64) {
-                wave_sync();
-                unsigned long long b = 0;
-                if (lane == 0) b = atomicAdd(&s.ctr->n_meet, (unsigned long long)nst);
-                b = __shfl(b, 0);
-                for (uint32_t j = lane; j < nst; j += 64) {
-                    if (b + j < s.meet_cap)
-                        s.meet[b + j] = st[j];
-                    else
-                        atomicOr(&s.ctr->error, kErrMeetCap);
-                }
-                wave_sync();
-                nst = 0;
-            }
-            tcur.keys[base + k] = L.newk[k];
-            tcur.starts[base + k] = L.news[k];
+            table_write(L, s, tcur, k & kKeyBits, c, q, tick);
         }
         __syncthreads();
     }
-    if (nst) {
-        wave_sync();
-        unsigned long long b = 0;
-        if (lane == 0) b = atomicAdd(&s.ctr->n_meet, (unsigned long long)nst);
-        b = __shfl(b, 0);
-        for (uint32_t j = lane; j < nst; j += 64) {
-            if (b + j < s.meet_cap)
-                s.meet[b + j] = st[j];
-            else
-                atomicOr(&s.ctr->error, kErrMeetCap);
-        }
-    }
-    for (int d = 32; d >= 1; d >>= 1) inserted += __shfl_xor(inserted, d);
-    if (lane == 0 && inserted) atomicAdd(&s.ctr->n_inserted, (unsigned long long)inserted);
-}
-
-__global__ void k_fill_keys(unsigned long long *p, uint64_t n)
-{
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-        p[i] = kEmptyKey;
+    if (nst) table_flush_stage<KV, NT>(s, st, nst);
+    // distinct pairs of the tick: contacts less second copies
+    unsigned long long v = total - dups;
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d);
+    if (lane_id() == 0 && v) atomicAdd(&s.ctr->n_inserted, v);
 }
 
 // ------------------------------------------------------------ host side
@@ -1849,7 +1935,7 @@ glint_status glint_create(const glint_config *cfg, glint_ctx **out)
     g_create_error.clear();
     if (cfg->num_objects == 0 || cfg->grid_capacity == 0 || !(cfg->reach_distance > 0) ||
         !(cfg->high_x > cfg->low_x) || !(cfg->high_y > cfg->low_y) || cfg->candidate_mode > 1 ||
-        cfg->precision > 1 || cfg->num_objects >= 0x80000000u ||
+        cfg->precision > 1 || cfg->num_objects >= 0x40000000u ||
         (cfg->join_tile != 0 && cfg->join_tile != 256 && cfg->join_tile != 512) ||
         (cfg->table_threads != 0 && cfg->table_threads != 256 && cfg->table_threads != 512 &&
          cfg->table_threads != 1024)) {
@@ -1970,8 +2056,6 @@ glint_status glint_create(const glint_config *cfg, glint_ctx **out)
     ALLOC(s.block_sums, sum_blocks);
     ALLOC(s.bcount, nbc);
     ALLOC(s.boff, nbc);
-    ALLOC(s.bstart, (1ull << s.region_bits) + 1);
-    ALLOC(s.clist, s.clist_cap);
     ALLOC(s.clist2, s.clist_cap);
     ALLOC(s.split_list, node_cap);
     ALLOC(s.merge_list, node_cap);
@@ -1998,8 +2082,9 @@ glint_status glint_create(const glint_config *cfg, glint_ctx **out)
     ALLOC(s.meet, s.meet_cap);
     ALLOC(s.ctr, 1);
     for (int g = 0; g < 2; g++) {
-        ALLOC(c->tab[g].keys, slots);
-        ALLOC(c->tab[g].starts, slots);
+        ALLOC(c->tab[g].keys, s.clist_cap);
+        ALLOC(c->tab[g].starts, s.clist_cap);
+        ALLOC(c->tab[g].rstart, (1ull << s.region_bits) + 1);
     }
     ALLOC(c->own_xy, 2 * N);
     if (cfg->meeting_log_capacity) {
@@ -2007,8 +2092,8 @@ glint_status glint_create(const glint_config *cfg, glint_ctx **out)
         s.mlog_cap = cfg->meeting_log_capacity;
     }
     if (cfg->kv_capacity) {
-        ALLOC(c->mbr[0], slots);
-        ALLOC(c->mbr[1], slots);
+        ALLOC(c->mbr[0], s.clist_cap);
+        ALLOC(c->mbr[1], s.clist_cap);
         ALLOC(s.pos32, N);
         KvParams kp{};
         kp.num_objects = (uint32_t)N;
@@ -2031,7 +2116,10 @@ glint_status glint_create(const glint_config *cfg, glint_ctx **out)
         c->err = std::string("hipMemsetAsync: ") + hipGetErrorString(e);
         return fail(GLINT_ERR_HIP);
     }
-    for (int g = 0; g < 2; g++) k_fill_keys<<<grid_for(slots, 16384), kBlock, 0, c->stream>>>(c->tab[g].keys, slots);
+    // both generations start empty
+    for (int g = 0; g < 2; g++)
+        k_fill_u32<<<grid_for((1ull << s.region_bits) + 1), kBlock, 0, c->stream>>>(c->tab[g].rstart, 0u,
+                                                                                 (1ull << s.region_bits) + 1);
     e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) {
         c->err = std::string("initialisation: ") + hipGetErrorString(e);
@@ -2132,7 +2220,8 @@ glint_status glint_ctx_view(glint_ctx *c, CtxView *v)
     const TableGen &tab = c->tab[c->evaluated ? 1 - c->cur_gen : c->cur_gen];
     v->keys = tab.keys;
     v->starts = tab.starts;
-    v->slots = c->s.tab_mask + 1;
+    v->count = tab.rstart + (1ull << c->s.region_bits);
+    v->cap = c->s.clist_cap;
     v->mlog = c->s.mlog;
     v->mlog_cap = c->s.mlog_cap;
     return GLINT_OK;
@@ -2397,10 +2486,11 @@ glint_status glint_contacts(glint_ctx *c, uint64_t *out, uint64_t cap, uint64_t 
     if (c->h_ctr->error) return status_from_flags(c, c->h_ctr->error);
     // the tick's bucketed contact list holds exactly n_contacts entries
     const uint64_t n = std::min<uint64_t>(c->h_ctr->n_contacts, c->s.clist_cap);
-    const unsigned long long *list = c->s.lo_bits ? c->s.clist : c->s.clist2;
+    // after evaluate the tick's list is the previous generation
+    const unsigned long long *list = c->tab[c->evaluated ? 1 - c->cur_gen : c->cur_gen].keys;
     std::vector<uint64_t> v(n);
     if (n) HIP_TRY(c, hipMemcpy(v.data(), list, n * 8, hipMemcpyDeviceToHost));
-    for (auto &k : v) k &= ~(1ull << 63);
+    for (auto &k : v) k &= kKeyBits;
     std::sort(v.begin(), v.end());
     v.erase(std::unique(v.begin(), v.end()), v.end());
     *n_out = v.size();
@@ -2457,14 +2547,17 @@ glint_status glint_active_pairs(glint_ctx *c, uint64_t *keys, uint32_t *starts, 
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     // after evaluate the current tick's pairs are the previous generation
     const TableGen &tab = c->tab[c->evaluated ? 1 - c->cur_gen : c->cur_gen];
-    const uint64_t slots = c->s.tab_mask + 1;
-    std::vector<unsigned long long> hk(slots);
-    std::vector<uint32_t> hs(slots);
-    HIP_TRY(c, hipMemcpy(hk.data(), tab.keys, slots * 8, hipMemcpyDeviceToHost));
-    HIP_TRY(c, hipMemcpy(hs.data(), tab.starts, slots * 4, hipMemcpyDeviceToHost));
+    uint32_t n = 0;
+    HIP_TRY(c, hipMemcpy(&n, tab.rstart + (1ull << c->s.region_bits), 4, hipMemcpyDeviceToHost));
+    std::vector<unsigned long long> hk(n);
+    std::vector<uint32_t> hs(n);
+    if (n) {
+        HIP_TRY(c, hipMemcpy(hk.data(), tab.keys, (size_t)n * 8, hipMemcpyDeviceToHost));
+        HIP_TRY(c, hipMemcpy(hs.data(), tab.starts, (size_t)n * 4, hipMemcpyDeviceToHost));
+    }
     std::vector<std::pair<uint64_t, uint32_t>> v;
-    for (uint64_t i = 0; i < slots; i++)
-        if (hk[i] != kEmptyKey) v.emplace_back(hk[i], hs[i]);
+    for (uint32_t i = 0; i < n; i++)
+        if (hs[i] != kDupStart) v.emplace_back(hk[i] & kKeyBits, hs[i]);
     std::sort(v.begin(), v.end());
     *n_out = v.size();
     if (!keys && !starts) return GLINT_OK;

@@ -9,7 +9,9 @@
 //   fx, fy               the same positions as FP32 offsets from the leaf's
 //                        low corner (guard-band mode)
 //   units                border check units (object id, candidate leaf)
-//   table[2][B]          16 B meeting slots, two generations
+//   gen[2]               the pairs of the last two ticks: each tick's contact
+//                        list grouped by table region (8 B keys) with a start
+//                        tick per entry (4 B) and the first entry per region
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -22,6 +24,11 @@ namespace glint {
 constexpr uint32_t kLeafTag = 0xFFFFFFFFu;      // child[0] of a leaf
 constexpr uint32_t kNone = 0xFFFFFFFFu;
 constexpr unsigned long long kEmptyKey = ~0ull; // (a<<32)|b with a<b is never all ones
+constexpr uint32_t kDupStart = 0xFFFFFFFFu;     // start of the second copy of a pair found twice
+// Flags carried in the high bits of a contact's key (pids are below 2^30):
+constexpr unsigned long long kTagPrev = 1ull << 63;     // the pair was within reach at the tick before
+constexpr unsigned long long kTagBorder = 1ull << 62;   // found through a border unit (may be found twice)
+constexpr unsigned long long kKeyBits = kTagBorder - 1;
 
 enum NodeType : int32_t { kInvalid = 0, kBranch = 1, kLeaf = 2 };
 
@@ -97,11 +104,14 @@ struct KvLogEntry {
     float4 box;
 };
 
-// One generation of the meeting table, slot by slot: the pair's key
-// ((pid_a << 32) | pid_b, kEmptyKey when free) and the first tick of its run.
+// One generation of the meeting table: the contacts of one tick grouped by
+// table region, keys[i] = (pid_a << 32) | pid_b with the kTag flags;
+// starts[i] = first tick of the pair's run (kDupStart: a second copy,
+// ignored); region r holds entries rstart[r] .. rstart[r+1].
 struct TableGen {
     unsigned long long *keys;
     uint32_t *starts;
+    uint32_t *rstart;
 };
 
 struct TickParams {
@@ -110,7 +120,7 @@ struct TickParams {
     TableGen tab_old;           // pairs seen at prev_tick
     TableGen tab_cur;           // pairs seen at tick (written by the table update)
     double *xy_prev;            // positions at prev_tick (pid order), valid if has_prev
-    float4 *mbr_old;            // boxes of the pairs in tab_old, by slot (key-value output only)
+    float4 *mbr_old;            // boxes of the pairs in tab_old, by entry (key-value output only)
     float4 *mbr_cur;            // boxes of the pairs in tab_cur
     double *xy_save;            // receives this tick's positions for the next tick
     unsigned long long *stamps; // run_ticks: device wall clock at the end of every tick
@@ -167,11 +177,9 @@ struct DevState {
     float *fx;
     float *fy;
     glint_meeting *meet;
-    unsigned long long *clist;     // the tick's contacts (bit 63: within reach at prev_tick)
-    unsigned long long *clist2;    // bucketing buffer
+    unsigned long long *clist2;    // bucketing buffer (the join's output when there is one pass)
     uint32_t *bcount;              // coarse-bin counts per bucketing block
     uint32_t *boff;                // their exclusive offsets
-    uint32_t *bstart;              // first contact of every table region
     uint32_t region_bits;          // log2(regions)
     uint32_t lo_bits;              // region bits split by the second pass
     uint32_t coarse_bins;          // bins of the first pass

@@ -42,6 +42,8 @@ namespace {
 constexpr uint32_t kB = 256;
 constexpr uint64_t kAlign = 256;
 constexpr unsigned long long kEmpty = ~0ull;
+constexpr uint32_t kDup = 0xFFFFFFFFu;      // start of a second copy of a pair in a generation
+constexpr unsigned long long kPairBits = (1ull << 62) - 1;   // a list key without its two flag bits
 
 struct QCounters {
     unsigned long long hits1, hits2;
@@ -78,23 +80,26 @@ __device__ __forceinline__ void append_hit(bool put, glint_hit h, glint_hit *out
 // Pairs of the table with a seed on either side that were in progress at
 // tick T (start <= T, T - start >= min + 1), clipped to end = T.
 __global__ __launch_bounds__(kB) void k_active_hits(const unsigned long long *keys, const uint32_t *starts,
-                                                    uint64_t slots, uint32_t T, uint32_t min_meet, uint32_t t_start,
-                                                    uint32_t t_end, const uint32_t *seeds_bm, glint_hit *out,
-                                                    unsigned long long *count, uint64_t cap)
+                                                    const uint32_t *count, uint32_t num_objects, uint32_t T,
+                                                    uint32_t min_meet, uint32_t t_start, uint32_t t_end,
+                                                    const uint32_t *seeds_bm, glint_hit *out,
+                                                    unsigned long long *hits, uint64_t cap)
 {
-    for (uint64_t i0 = blockIdx.x * (uint64_t)kB; i0 < slots; i0 += (uint64_t)gridDim.x * kB) {
+    const uint64_t n = *count;
+    for (uint64_t i0 = blockIdx.x * (uint64_t)kB; i0 < n; i0 += (uint64_t)gridDim.x * kB) {
         const uint64_t i = i0 + threadIdx.x;
         unsigned long long key = kEmpty;
-        uint32_t start = 0;
-        if (i < slots) {
-            key = keys[i];
+        uint32_t start = kDup;
+        if (i < n) {
+            key = keys[i] & kPairBits;
             start = starts[i];
         }
-        const bool valid = key != kEmpty && start <= T && T - start >= min_meet + 1 && start <= t_end && T >= t_start;
         const uint32_t a = (uint32_t)(key >> 32), b = (uint32_t)key;
+        const bool valid = start != kDup && a < num_objects && b < num_objects && start <= T &&
+                           T - start >= min_meet + 1 && start <= t_end && T >= t_start;
         const bool ha = valid && has_bit(seeds_bm, a), hb = valid && has_bit(seeds_bm, b);
-        append_hit(ha, glint_hit{a, b, start, T}, out, count, cap);
-        append_hit(hb, glint_hit{b, a, start, T}, out, count, cap);
+        append_hit(ha, glint_hit{a, b, start, T}, out, hits, cap);
+        append_hit(hb, glint_hit{b, a, start, T}, out, hits, cap);
     }
 }
 
@@ -364,9 +369,9 @@ glint_status start_query(glint_server *s, Query &q, int slot)
     CtxView v{};
     glint_ctx_view(s->ctx, &v);
     SV_TRY(s, hipStreamWaitEvent(v.stream, sl.ready, 0));
-    k_active_hits<<<grid_of(v.slots, 8192), kB, 0, v.stream>>>(v.keys, v.starts, v.slots, q.T, v.min_meet,
-                                                               q.t_start, q.t_end, q.bm1, q.h1, &q.ctr->hits1,
-                                                               q.cap1);
+    k_active_hits<<<grid_of(v.cap, 8192), kB, 0, v.stream>>>(v.keys, v.starts, v.count, v.num_objects, q.T,
+                                                             v.min_meet, q.t_start, q.t_end, q.bm1, q.h1,
+                                                             &q.ctr->hits1, q.cap1);
     SV_TRY(s, hipEventRecord(sl.snap, v.stream));
     // log hits on the query's stream
     uint64_t total = 0, first = 0;

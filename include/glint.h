@@ -63,7 +63,7 @@ enum {
 /* Mirrors the hot-path fields of the reference's configuration
  * (src/util/config.h:28-48, defaults at config.h:173-187). */
 typedef struct glint_config {
-    uint32_t num_objects;          /* config.h:21 */
+    uint32_t num_objects;          /* config.h:21; below 2^30 */
     uint32_t grid_capacity;        /* config.h:29: leaf split threshold */
     uint32_t min_meet_time;        /* config.h:39 */
     uint32_t max_meet_time;        /* config.h:40: longer meetings are flagged oversize */

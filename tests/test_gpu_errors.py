@@ -57,12 +57,15 @@ def test_injected_failures(stage):
 
 
 def test_table_too_small_reports_table_full():
-    w = gen.uniform_city(3000, 3, 300.0, seed=62, grid_capacity=16)
+    # one region of 2048 slots; from tick 1 on, the contacts that may continue
+    # a run (within reach at the tick before) outnumber its slots
+    w = gen.uniform_city(6000, 4, 200.0, seed=62, grid_capacity=16)
     g = gen.Generator(w)
     cfg = runtime.make_config(w.num_objects, w.box(), grid_capacity=16, num_meeting_buckets=64)
     with runtime.Workbench(cfg) as wb:
         with pytest.raises(runtime.GlintError) as e:
-            wb.tick(g.positions(), 0)
+            for t in range(w.ticks):
+                wb.tick(g.positions() if t == 0 else g.step(), t)
         assert e.value.status == 5
 
 
