@@ -60,25 +60,26 @@ def workload_for(name: str, objects: int | None):
 
 def algorithmic_bytes(kernel: str, n: int, per_tick: dict, precision: int) -> float:
     """Algorithmic HBM bytes of one launch of `kernel` (one tick), from the
-    per-tick figures: N objects, C contacts (border share Cb), M meetings,
-    U border units."""
-    C, Cb, M, U = per_tick["contacts"], per_tick["contacts_border"], per_tick["meetings"], per_tick["units"]
+    per-tick figures: N objects, S contacts kept in the meeting table (those
+    within reach at the tick before or found through a border unit), M
+    meetings, U border units."""
+    S, M, U = per_tick["stored"], per_tick["meetings"], per_tick["units"]
     coord = 8 if precision else 16          # FP32 offsets or FP64 coordinates per object
     if kernel == "k_join":
         # every sorted object once (coordinates, previous FP64 position, id),
-        # each border unit (id + xy), one 8 B key per contact out
-        return (coord + 16.0 + 4.0) * n + 24.0 * U + 8.0 * C
+        # each border unit (id + xy), one 8 B key per kept contact out
+        return (coord + 16.0 + 4.0) * n + 24.0 * U + 8.0 * S
     if kernel == "k_table":
         # the previous generation's entries (8 B key + 4 B start; as many as
-        # the tick before had contacts, taken equal to C), every contact's key
+        # the tick before kept, taken equal to S), every kept contact's key
         # read and its start written, meetings out
-        return 12.0 * C + 12.0 * C + 16.0 * M
+        return 12.0 * S + 12.0 * S + 16.0 * M
     if kernel == "k_bucket_fine":
-        return 24.0 * C                       # segment read twice (histogram, tiles), written once
+        return 24.0 * S                       # segment read twice (histogram, tiles), written once
     if kernel == "k_bucket_scatter":
-        return 16.0 * C
+        return 16.0 * S
     if kernel == "k_bucket_count":
-        return 8.0 * C
+        return 8.0 * S
     if kernel == "k_assign":
         return 16.0 * n + 8.0 * n             # xy in, leaf and rank out
     if kernel == "k_scatter":
@@ -278,7 +279,7 @@ def main() -> int:
 
     value = N * K / elapsed
     per_tick = {k: totals[k] / max(1, totals["ticks"]) for k in
-                ("contacts", "contacts_border", "meetings", "units", "border_objects", "distinct_pairs")}
+                ("contacts", "contacts_border", "meetings", "units", "border_objects", "distinct_pairs", "stored")}
     roofline = None
     if ktimes:
         dom, (dms, dcnt) = max(((k, v) for k, v in ktimes.items() if v[1]), key=lambda kv: kv[1][0])
@@ -390,7 +391,7 @@ def main() -> int:
         "data": "synthetic (seeded generator, positions resident in HBM before timing)",
         "config": {"workload": desc, "objects": N, "first_tick": W, "ticks": K, "graph_ticks": graph_ticks,
                    "params": params, "table_slots": slots, "contacts_per_tick": round(per_tick["contacts"]),
-                   "meetings_per_tick": round(per_tick["meetings"], 1), "leaves": stats["num_leaves"]},
+                   "stored_per_tick": round(per_tick["stored"]), "meetings_per_tick": round(per_tick["meetings"], 1), "leaves": stats["num_leaves"]},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

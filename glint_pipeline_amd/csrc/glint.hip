@@ -172,7 +172,8 @@ __global__ void k_reset_tick(DevState s)
     c->n_units = 0;
     c->n_contacts = 0;
     c->n_contacts_border = 0;
-    c->n_inserted = 0;
+    c->n_dup = 0;
+    c->n_stored = 0;
     c->n_meet = 0;
     c->n_rechecks = 0;
     c->n_clist = 0;
@@ -396,13 +397,16 @@ __global__ void k_schema_done(DevState s)
     c->acc_ticks += 1;
     c->acc_contacts += c->n_contacts;
     c->acc_contacts_border += c->n_contacts_border;
-    c->acc_distinct += c->n_inserted;
+    const unsigned long long distinct = c->n_contacts - c->n_dup;
+    c->acc_distinct += distinct;
     c->acc_meet += c->n_meet;
     c->acc_units += c->n_units;
     c->acc_border += c->n_border;
     c->acc_rechecks += c->n_rechecks;
     if (c->n_contacts > c->max_contacts) c->max_contacts = c->n_contacts;
-    if (c->n_inserted > c->max_distinct) c->max_distinct = c->n_inserted;
+    if (distinct > c->max_distinct) c->max_distinct = distinct;
+    c->acc_stored += c->n_stored;
+    if (c->n_stored > c->max_stored) c->max_stored = c->n_stored;
     if (c->n_meet > c->max_meet) c->max_meet = c->n_meet;
     if (s.mlog) c->n_mlog += c->n_meet < s.meet_cap ? c->n_meet : s.meet_cap;
 }
@@ -437,6 +441,7 @@ __global__ __launch_bounds__(kBlock) void k_assign(DevState s)
         uint32_t lv = 0, cur = 0;
         if (active) {
             double x = xy[2 * (size_t)i], y = xy[2 * (size_t)i + 1];
+            s.prev_leaf[i] = s.leaf_of[i];
             if (!(x >= s.lx && x <= s.hx && y >= s.ly && y <= s.hy)) {
                 atomicOr(&s.ctr->error, kErrBadInput);
                 s.leaf_of[i] = kNone;
@@ -722,44 +727,59 @@ __device__ __forceinline__ uint32_t region_of(const DevState &s, unsigned long l
     return s.region_bits ? (uint32_t)(mix64(key) >> (64 - s.region_bits)) : 0u;
 }
 
-// Contacts found by one wave are staged in LDS and appended to the tick's
-// contact list in 4096-entry chunks reserved with one atomic each.
+// Contacts found by one wave are staged in LDS as (row lane, column) index
+// pairs and appended to the tick's contact list in 4096-entry chunks
+// reserved with one atomic each; the pair key (and its flags) is formed when
+// the stage is written out, one lane per contact.
 struct Stager {
     unsigned long long *out;    // the tick's contact list
-    unsigned long long *buf;
+    uint32_t *buf;
     uint32_t n;                 // staged entries (uniform across the wave)
     unsigned long long chunk;   // current chunk in the contact list
     uint32_t used;              // entries of the chunk written
+    uint32_t kept;              // entries written in all (uniform)
 };
 
-__device__ __forceinline__ void stage_push(Stager &st, bool hit, unsigned long long key)
+// Lanes without a hit write to their spare slot past the stage (no branch).
+__device__ __forceinline__ void stage_push(Stager &st, bool hit, uint32_t e)
 {
-    unsigned long long m = __ballot(hit);
-    if (hit) st.buf[st.n + __popcll(m & ((1ull << lane_id()) - 1ull))] = key;
+    const unsigned long long m = __ballot(hit);
+    const uint32_t lane = lane_id();
+    st.buf[hit ? st.n + __popcll(m & ((1ull << lane) - 1ull)) : kStage + lane] = e;
     st.n += (uint32_t)__popcll(m);
 }
 
-__device__ __forceinline__ void stage_flush(Stager &st, const DevState &s)
+// Writes the staged entries out: key(e) gives the contact-list key of entry
+// e, or kEmptyKey for an entry that is not kept; kept keys are packed.
+template <class Key>
+__device__ __forceinline__ void stage_flush(Stager &st, const DevState &s, Key key)
 {
     if (st.n == 0) return;
     wave_sync();
     const uint32_t lane = lane_id();
-    uint32_t done = 0;
-    while (done < st.n) {
-        if (st.used == kChunk) {
+    for (uint32_t done = 0; done < st.n; done += 64) {
+        const uint32_t i = done + lane;
+        const unsigned long long k = i < st.n ? key(st.buf[i]) : kEmptyKey;
+        const bool keep = k != kEmptyKey;
+        const unsigned long long m = __ballot(keep);
+        const uint32_t cnt = (uint32_t)__popcll(m);
+        if (cnt == 0) continue;
+        st.kept += cnt;
+        if (st.used + cnt > kChunk) {
+            // close the open chunk (its tail holds kEmptyKey) and reserve the next
+            if (st.chunk + kChunk <= s.clist_cap)
+                for (uint32_t t = st.used + lane; t < kChunk; t += 64) st.out[st.chunk + t] = kEmptyKey;
             unsigned long long c = 0;
             if (lane == 0) c = atomicAdd(&s.ctr->n_clist, (unsigned long long)kChunk);
             st.chunk = __shfl(c, 0);
             st.used = 0;
         }
-        const uint32_t take = min(st.n - done, kChunk - st.used);
         if (st.chunk + kChunk <= s.clist_cap) {
-            for (uint32_t k = lane; k < take; k += 64) st.out[st.chunk + st.used + k] = st.buf[done + k];
+            if (keep) st.out[st.chunk + st.used + __popcll(m & ((1ull << lane) - 1ull))] = k;
         } else if (lane == 0) {
             atomicOr(&s.ctr->error, kErrContactCap);
         }
-        st.used += take;
-        done += take;
+        st.used += cnt;
     }
     wave_sync();
     st.n = 0;
@@ -768,7 +788,6 @@ __device__ __forceinline__ void stage_flush(Stager &st, const DevState &s)
 // The unused tail of a wave's last chunk holds kEmptyKey (skipped later).
 __device__ __forceinline__ void stage_close(Stager &st, const DevState &s)
 {
-    stage_flush(st, s);
     if (st.used < kChunk && st.chunk + kChunk <= s.clist_cap)
         for (uint32_t k = st.used + lane_id(); k < kChunk; k += 64) st.out[st.chunk + k] = kEmptyKey;
 }
@@ -783,65 +802,118 @@ __device__ __forceinline__ unsigned long long pair_key(uint32_t a, uint32_t b)
 
 template <int TILE>
 struct JoinTile {
-    double x[TILE], y[TILE];       // positions at tick
-    double px[TILE], py[TILE];     // positions at prev_tick
-    float fx[TILE], fy[TILE];      // FP32 offsets from the leaf's low corner (guard mode)
+    double2 xy[TILE];              // positions at tick
+    double2 pxy[TILE];             // positions at prev_tick
+    float2 f[TILE];                // FP32 offsets from the leaf's low corner (guard mode)
     uint32_t p[TILE];              // object ids
+};
+
+// The row objects of a wave's current task (read when its contacts are
+// written out).
+struct JoinRows {
+    double2 pxy[64];
+    uint32_t p[64];
 };
 
 template <int TILE>
 struct JoinLds {
     JoinTile<TILE> a, b;
-    unsigned long long stage[kBlock / 64][kStage];
+    JoinRows rows[kBlock / 64];
+    uint32_t stage[kBlock / 64][kStage + 64];
     uint32_t item;
 };
 
-// Distance test of one candidate pair, bit-equal to sqrt(dx*dx+dy*dy) <= r
-// in FP64.  In guard mode the FP32 offsets decide every pair outside the
-// band (r-eps, r+eps]; pairs inside it are re-tested in FP64.
-struct PairTest {
-    double t64;
-    float t32_in, t32_out;
-    bool use32;
-};
-
-__device__ __forceinline__ bool pair_within(const PairTest &pt, double x1, double y1, double x2, double y2,
-                                            float f1x, float f1y, float f2x, float f2y, uint32_t &rechecks)
-{
-    if (pt.use32) {
-        const float dx = f1x - f2x;
-        const float dy = f1y - f2y;
-        const float s32 = dx * dx + dy * dy;
-        if (s32 <= pt.t32_in) return true;
-        if (s32 > pt.t32_out) return false;
-        rechecks++;
-    }
-    const double dx = x1 - x2;
-    const double dy = y1 - y2;
-    const double d2 = dx * dx + dy * dy;
-    return d2 <= pt.t64;
-}
-
 template <int TILE>
 __device__ __forceinline__ void load_tile(JoinTile<TILE> &t, const DevState &s, uint32_t from, uint32_t n, bool has_prev,
-                                          bool use32, double ox, double oy)
+                                          bool use32)
 {
     for (uint32_t k = threadIdx.x; k < n; k += kBlock) {
-        const double x = s.sx[from + k], y = s.sy[from + k];
-        t.x[k] = x;
-        t.y[k] = y;
+        t.xy[k] = make_double2(s.sx[from + k], s.sy[from + k]);
+        if (has_prev) t.pxy[k] = make_double2(s.sxp[from + k], s.syp[from + k]);
+        t.f[k] = use32 ? make_float2(s.fx[from + k], s.fy[from + k]) : make_float2(0.f, 0.f);
         t.p[k] = s.spid[from + k];
-        if (has_prev) {
-            t.px[k] = s.sxp[from + k];
-            t.py[k] = s.syp[from + k];
+    }
+}
+
+// Stage entries: row lane << 16 | band << 15 | column.  A band entry is a
+// pair whose FP32 distance lies inside the guard band; it is re-tested in
+// FP64 when the stage is written out.
+constexpr uint32_t kBandBit = 1u << 15;
+
+// One wave task's column loop: the row lane's object against columns
+// [jbeg, jend) of tile C, from column jmin on (jmin = ~0 for an idle lane).
+// In guard mode (USE32) pairs are decided on the FP32 offsets: inside
+// r - eps a hit, past r + eps a miss, in between a band entry.
+template <bool USE32, int TILE>
+__device__ __forceinline__ void join_columns(Stager &st, const DevState &s, const JoinTile<TILE> &C, double xi,
+                                             double yi, float fxi, float fyi, uint32_t jmin, uint32_t jbeg,
+                                             uint32_t jend, float t32_in, float t32_out, double t64,
+                                             uint32_t &n_staged, uint32_t &rej, uint32_t &band,
+                                             const JoinRows &rows, unsigned long long flag, bool has_prev)
+{
+    const uint32_t tag = lane_id() << 16;
+    // the stage's contacts as keys (kEmptyKey: a band entry that is not
+    // within reach in FP64, or a contact that is not stored)
+    auto key_of = [&](uint32_t e) {
+        const uint32_t rl = e >> 16, j = e & 0x7FFFu;
+        if (USE32 && (e & kBandBit)) {
+            band++;
+            const uint32_t pa = rows.p[rl];
+            const double2 b = C.xy[j];
+            const double ex = s.tp->xy[2 * (size_t)pa] - b.x;
+            const double ey = s.tp->xy[2 * (size_t)pa + 1] - b.y;
+            if (!(ex * ex + ey * ey <= t64)) {
+                rej++;
+                return kEmptyKey;
+            }
         }
-        if (use32) {
-            t.fx[k] = s.fx[from + k];
-            t.fy[k] = s.fy[from + k];
+        unsigned long long k = pair_key(rows.p[rl], C.p[j]) | flag;
+        if (has_prev) {
+            const double2 a = rows.pxy[rl], b = C.pxy[j];
+            const double dx = a.x - b.x;
+            const double dy = a.y - b.y;
+            if (dx * dx + dy * dy <= t64) k |= kTagPrev;
+        }
+        // a contact neither tagged nor from a border unit is not stored
+        return (s.store_all || (k & (kTagPrev | kTagBorder))) ? k : kEmptyKey;
+    };
+    for (uint32_t jb = jbeg; jb < jend; jb += kFlushRound) {
+        float2 cf[kFlushRound];
+        double2 cx[kFlushRound];
+#pragma unroll
+        for (uint32_t jj = 0; jj < kFlushRound; jj++) {
+            const uint32_t jc = min(jb + jj, jend - 1);
+            if constexpr (USE32)
+                cf[jj] = C.f[jc];
+            else
+                cx[jj] = C.xy[jc];
+        }
+#pragma unroll
+        for (uint32_t jj = 0; jj < kFlushRound; jj++) {
+            const uint32_t j = jb + jj;
+            const bool cand = j >= jmin && j < jend;
+            bool hit;
+            uint32_t e = tag | j;
+            if constexpr (USE32) {
+                const float dx = fxi - cf[jj].x;
+                const float dy = fyi - cf[jj].y;
+                const float s32 = dx * dx + dy * dy;
+                hit = s32 <= t32_out;
+                e |= s32 <= t32_in ? 0u : kBandBit;
+            } else {
+                const double ex = xi - cx[jj].x;
+                const double ey = yi - cx[jj].y;
+                hit = ex * ex + ey * ey <= t64;
+            }
+            stage_push(st, hit && cand, e);
+        }
+        if (st.n > kStage - 64 * kFlushRound) {
+            n_staged += st.n;
+            stage_flush(st, s, key_of);
         }
     }
-    (void)ox;
-    (void)oy;
+    n_staged += st.n;
+    stage_flush(st, s, key_of);
 }
 
 // The refine stage for one leaf L per block (work queue over the leaf list):
@@ -850,8 +922,11 @@ __device__ __forceinline__ void load_tile(JoinTile<TILE> &t, const DevState &s, 
 //  - border units (pid, L, inside=false) (refinement.cpp:30 with !inside):
 //    each object whose lookup selected L against every object of L.
 // Work is split into 64x64 wave tasks (64 row lanes x 64 broadcast columns).
-// A contact is staged with bit 63 set when the pair was also within reach
-// at prev_tick, the only case that needs the previous table generation.
+// The column loop only tests distances and stages (row lane, column) of each
+// hit; when a task's stage is written out, one lane per contact forms the
+// pair key, adds kTagPrev when the pair was also within reach at prev_tick
+// (the only case that needs the previous table generation) and kTagBorder
+// when it came from a border unit (the only case that can find a pair twice).
 template <int TILE>
 __global__ __launch_bounds__(kBlock) void k_join(DevState s)
 {
@@ -862,9 +937,10 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
     const double t64 = s.t64;
     const uint32_t nl = (s.ctr->error & kErrUnitCap) ? 0 : s.ctr->n_leaf_list;
     const uint32_t wave = threadIdx.x / 64, lane = lane_id();
-    Stager st{contact_out(s), lds.stage[wave], 0, 0, kChunk};
-    uint32_t n_in = 0, n_bd = 0, n_re = 0;   // per lane
-    PairTest pt{t64, s.t32_in, s.t32_out, false};
+    Stager st{contact_out(s), lds.stage[wave], 0, 0, kChunk, 0};
+    JoinRows &rows = lds.rows[wave];
+    uint32_t n_in = 0, n_bd = 0;                 // staged entries (wave-uniform)
+    uint32_t rej_in = 0, rej_bd = 0, band = 0;   // per lane
     for (;;) {
         if (threadIdx.x == 0) lds.item = atomicAdd(&s.ctr->work_inside, kInsideChunk);
         __syncthreads();
@@ -877,16 +953,16 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
             const uint32_t uoff = s.ustart[leaf], ucnt = s.ucount[leaf];
             const uint32_t ntile = (cnt + TILE - 1) / TILE;
             const Node nd = s.nodes[leaf];
-            pt.use32 = s.precision == GLINT_PRECISION_FP32_GUARD && (nd.hx - nd.lx) <= s.w32_max &&
-                       (nd.hy - nd.ly) <= s.w32_max;
+            const bool use32 = s.precision == GLINT_PRECISION_FP32_GUARD && (nd.hx - nd.lx) <= s.w32_max &&
+                               (nd.hy - nd.ly) <= s.w32_max;
             for (uint32_t I = 0; I < ntile; I++) {
                 const uint32_t ni = min((uint32_t)TILE, cnt - I * TILE);
-                load_tile(lds.a, s, off + I * TILE, ni, has_prev, pt.use32, nd.lx, nd.ly);
+                load_tile(lds.a, s, off + I * TILE, ni, has_prev, use32);
                 for (uint32_t J = I; J < ntile; J++) {
                     const uint32_t nj = min((uint32_t)TILE, cnt - J * TILE);
                     const bool diag = (J == I);
                     const JoinTile<TILE> &B = diag ? lds.a : lds.b;
-                    if (!diag) load_tile(lds.b, s, off + J * TILE, nj, has_prev, pt.use32, nd.lx, nd.ly);
+                    if (!diag) load_tile(lds.b, s, off + J * TILE, nj, has_prev, use32);
                     __syncthreads();
                     const JoinTile<TILE> &A = lds.a;
                     // inside tasks: 64-row sub-tile r against 64-column sub-tile q
@@ -896,10 +972,10 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
                     const uint32_t nuc = diag ? (ucnt + 63) / 64 : 0;
                     const uint32_t ntask = ntask_in + nuc;
                     for (uint32_t t = wave; t < ntask; t += kBlock / 64) {
-                        double xi, yi, pxi = 0.0, pyi = 0.0;
+                        double xi = 0.0, yi = 0.0, pxi = 0.0, pyi = 0.0;
                         float fxi = 0.f, fyi = 0.f;
-                        uint32_t pi, jbeg, jend, row;
-                        bool vi, inside;
+                        uint32_t pi, jbeg, jend, jmin;
+                        bool inside;
                         if (t < ntask_in) {
                             uint32_t r, q;
                             if (diag) {   // t -> (r <= q), row-major upper triangle
@@ -914,66 +990,59 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
                                 r = t / nrj;
                                 q = t % nrj;
                             }
-                            row = r * 64 + lane;
-                            vi = row < ni;
-                            xi = vi ? A.x[row] : 0.0;
-                            yi = vi ? A.y[row] : 0.0;
-                            if (has_prev && vi) {
-                                pxi = A.px[row];
-                                pyi = A.py[row];
+                            const uint32_t row = r * 64 + lane;
+                            const bool vi = row < ni;
+                            const uint32_t rr = vi ? row : 0;
+                            const double2 a = A.xy[rr];
+                            const float2 af = A.f[rr];
+                            xi = a.x;
+                            yi = a.y;
+                            if (has_prev) {
+                                const double2 ap = A.pxy[rr];
+                                pxi = ap.x;
+                                pyi = ap.y;
                             }
-                            if (pt.use32 && vi) {
-                                fxi = A.fx[row];
-                                fyi = A.fy[row];
-                            }
-                            pi = vi ? A.p[row] : 0;
+                            fxi = af.x;
+                            fyi = af.y;
+                            pi = A.p[rr];
                             jbeg = q * 64;
                             jend = min(jbeg + 64, nj);
+                            // only j > row in a diagonal task; ~0: an idle lane
+                            jmin = !vi ? 0xFFFFFFFFu : (diag ? row + 1 : 0u);
                             inside = true;
                         } else {
                             const uint32_t u = (t - ntask_in) * 64 + lane;
-                            vi = u < ucnt;
+                            const bool vi = u < ucnt;
                             pi = vi ? s.unit_pid[uoff + u] : 0;
-                            xi = vi ? xy[2 * (size_t)pi] : 0.0;
-                            yi = vi ? xy[2 * (size_t)pi + 1] : 0.0;
-                            if (has_prev && vi) {
-                                pxi = xy_prev[2 * (size_t)pi];
-                                pyi = xy_prev[2 * (size_t)pi + 1];
+                            if (vi) {
+                                xi = xy[2 * (size_t)pi];
+                                yi = xy[2 * (size_t)pi + 1];
+                                if (has_prev) {
+                                    pxi = xy_prev[2 * (size_t)pi];
+                                    pyi = xy_prev[2 * (size_t)pi + 1];
+                                }
                             }
-                            if (pt.use32) {
+                            if (use32) {
                                 fxi = (float)(xi - nd.lx);
                                 fyi = (float)(yi - nd.ly);
                             }
-                            row = 0;
                             jbeg = 0;
                             jend = ni;
+                            jmin = vi ? 0u : 0xFFFFFFFFu;
                             inside = false;
                         }
+                        rows.p[lane] = pi;
+                        rows.pxy[lane] = make_double2(pxi, pyi);
                         const JoinTile<TILE> &C = inside ? B : A;
-                        for (uint32_t jb = jbeg; jb < jend; jb += kFlushRound) {
-#pragma unroll
-                            for (uint32_t jj = 0; jj < kFlushRound; jj++) {
-                                const uint32_t j = jb + jj;
-                                if (j < jend) {
-                                    bool hit = vi && (!inside || !diag || j > row) &&
-                                               pair_within(pt, xi, yi, C.x[j], C.y[j], fxi, fyi, C.fx[j], C.fy[j],
-                                                           n_re);
-                                    unsigned long long key = pair_key(pi, C.p[j]);
-                                    if (hit && has_prev) {
-                                        const double dx = pxi - C.px[j];
-                                        const double dy = pyi - C.py[j];
-                                        if (dx * dx + dy * dy <= t64) key |= kTagPrev;
-                                    }
-                                    if (!inside) key |= kTagBorder;
-                                    if (inside)
-                                        n_in += hit;
-                                    else
-                                        n_bd += hit;
-                                    stage_push(st, hit, key);
-                                }
-                            }
-                            if (st.n > kStage - 64 * kFlushRound) stage_flush(st, s);
-                        }
+                        const unsigned long long flag = inside ? 0ull : kTagBorder;
+                        uint32_t &n_staged = inside ? n_in : n_bd;
+                        uint32_t &rej = inside ? rej_in : rej_bd;
+                        if (use32)
+                            join_columns<true>(st, s, C, xi, yi, fxi, fyi, jmin, jbeg, jend, s.t32_in, s.t32_out,
+                                               t64, n_staged, rej, band, rows, flag, has_prev);
+                        else
+                            join_columns<false>(st, s, C, xi, yi, fxi, fyi, jmin, jbeg, jend, s.t32_in, s.t32_out,
+                                                t64, n_staged, rej, band, rows, flag, has_prev);
                     }
                     __syncthreads();
                 }
@@ -982,16 +1051,18 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
     }
     stage_close(st, s);
     // one atomic per wave for each counter
-    uint32_t a = n_in, b = n_bd, re = n_re;
+    uint32_t ri = rej_in, rb = rej_bd, bd = band;
     for (int d = 32; d >= 1; d >>= 1) {
-        a += __shfl_xor(a, d);
-        b += __shfl_xor(b, d);
-        re += __shfl_xor(re, d);
+        ri += __shfl_xor(ri, d);
+        rb += __shfl_xor(rb, d);
+        bd += __shfl_xor(bd, d);
     }
     if (lane == 0) {
-        if (a + b) atomicAdd(&s.ctr->n_contacts, (unsigned long long)(a + b));
-        if (b) atomicAdd(&s.ctr->n_contacts_border, (unsigned long long)b);
-        if (re) atomicAdd(&s.ctr->n_rechecks, (unsigned long long)re);
+        const uint32_t ci = n_in - ri, cb = n_bd - rb;
+        if (ci + cb) atomicAdd(&s.ctr->n_contacts, (unsigned long long)(ci + cb));
+        if (cb) atomicAdd(&s.ctr->n_contacts_border, (unsigned long long)cb);
+        if (bd) atomicAdd(&s.ctr->n_rechecks, (unsigned long long)bd);
+        if (st.kept) atomicAdd(&s.ctr->n_stored, (unsigned long long)st.kept);
     }
 }
 
@@ -1363,18 +1434,36 @@ __device__ __forceinline__ void table_old(TableLds<KV, NT> &L, const DevState &s
 }
 
 // Start (and, with the key-value output, box) of contact c, whose hash slot
-// is q (kNoSlot: a contact that was not hashed, new at this tick).
+// is q (kNoSlot: a contact that was not hashed, new at this tick).  A tagged
+// contact with no entry in the previous generation either is new or was a
+// contact of the tick before that was not stored (found by the inside join
+// of its leaf, untagged); the objects' leaves at the tick before tell which.
 template <bool KV, int NT>
 __device__ __forceinline__ void table_write(const TableLds<KV, NT> &L, const DevState &s, const TableGen &tcur,
-                                            unsigned long long key, uint32_t c, uint32_t q, uint32_t tick)
+                                            unsigned long long tagged, uint32_t c, uint32_t q, uint32_t tick,
+                                            uint32_t last)
 {
-    tcur.starts[c] = q == kNoSlot ? tick : L.hs[q];
+    const unsigned long long key = tagged & kKeyBits;
+    const uint32_t a = (uint32_t)(key >> 32), b = (uint32_t)key;
+    uint32_t start = q == kNoSlot ? tick : L.hs[q];
+    bool was_unstored = false;
+    if (start == tick && (tagged & kTagPrev) && !s.store_all && s.prev_leaf[a] == s.prev_leaf[b]) {
+        start = last;
+        was_unstored = true;
+    }
+    tcur.starts[c] = start;
     if constexpr (KV) {
-        const float2 pa = s.pos32[(uint32_t)(key >> 32)], pb = s.pos32[(uint32_t)key];
+        const float2 pa = s.pos32[a], pb = s.pos32[b];
         float4 m = make_float4(fminf(pa.x, pb.x), fminf(pa.y, pb.y), fmaxf(pa.x, pb.x), fmaxf(pa.y, pb.y));
         if (q != kNoSlot && L.ho[q] != kNoSlot) {
             const float4 o = s.tp->mbr_old[L.ho[q]];
             m = make_float4(fminf(m.x, o.x), fminf(m.y, o.y), fmaxf(m.z, o.z), fmaxf(m.w, o.w));
+        } else if (was_unstored) {   // the run's box so far: both points at the tick before
+            const double *xp = s.tp->xy_prev;
+            const float ax = (float)xp[2 * (size_t)a], ay = (float)xp[2 * (size_t)a + 1];
+            const float bx = (float)xp[2 * (size_t)b], by = (float)xp[2 * (size_t)b + 1];
+            m = make_float4(fminf(m.x, fminf(ax, bx)), fminf(m.y, fminf(ay, by)), fmaxf(m.z, fmaxf(ax, bx)),
+                            fmaxf(m.w, fmaxf(ay, by)));
         }
         s.tp->mbr_cur[c] = m;
     }
@@ -1396,7 +1485,6 @@ __global__ __launch_bounds__(NT) void k_table(DevState s)
     const uint32_t nreg = 1u << s.region_bits;
     glint_meeting *st = L.stage[threadIdx.x / 64];
     uint32_t nst = 0, dups = 0;
-    unsigned long long total = 0;
     const bool void_tick = (s.ctr->error & kErrContactCap) != 0;
     // the previous generation's entries of the next region are loaded into
     // registers while the current region is processed
@@ -1417,7 +1505,6 @@ __global__ __launch_bounds__(NT) void k_table(DevState s)
     for (uint32_t r = blockIdx.x; r < nreg; r += gridDim.x) {
         for (uint32_t k = threadIdx.x; k < kRegion; k += NT) L.hk[k] = kEmptyKey;
         const uint32_t c0 = void_tick ? 0 : tcur.rstart[r], c1 = void_tick ? 0 : tcur.rstart[r + 1];
-        total += (threadIdx.x == 0) ? c1 - c0 : 0;
 #pragma unroll
         for (int q = 0; q < kPer; q++) {
             const uint32_t c = c0 + threadIdx.x + q * NT;
@@ -1467,7 +1554,7 @@ __global__ __launch_bounds__(NT) void k_table(DevState s)
 #pragma unroll
         for (int q = 0; q < kPer; q++) {
             const uint32_t c = c0 + threadIdx.x + q * NT;
-            if (c < c1 && ns[q] != kNoSlot - 1) table_write(L, s, tcur, nk[q] & kKeyBits, c, ns[q], tick);
+            if (c < c1 && ns[q] != kNoSlot - 1) table_write(L, s, tcur, nk[q], c, ns[q], tick, last);
         }
         for (uint32_t c = c0 + kRegion + threadIdx.x; c < c1; c += NT) {
             const unsigned long long k = tcur.keys[c];
@@ -1476,15 +1563,14 @@ __global__ __launch_bounds__(NT) void k_table(DevState s)
                 q = table_find(L, k & kKeyBits);
                 if (q == kNoSlot || L.hc[q] != c) continue;   // a second copy (already marked)
             }
-            table_write(L, s, tcur, k & kKeyBits, c, q, tick);
+            table_write(L, s, tcur, k, c, q, tick, last);
         }
         __syncthreads();
     }
     if (nst) table_flush_stage<KV, NT>(s, st, nst);
     // distinct pairs of the tick: contacts less second copies
-    unsigned long long v = total - dups;
-    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d);
-    if (lane_id() == 0 && v) atomicAdd(&s.ctr->n_inserted, v);
+    for (int d = 32; d >= 1; d >>= 1) dups += __shfl_xor(dups, d);
+    if (lane_id() == 0 && dups) atomicAdd(&s.ctr->n_dup, (unsigned long long)dups);
 }
 
 // ------------------------------------------------------------ host side
@@ -1873,7 +1959,8 @@ void fill_stats(glint_ctx *c)
     t.error_flags = h.error;
     t.num_units = h.n_units;
     t.num_contacts = h.n_contacts;
-    t.num_active = h.n_inserted;
+    t.num_active = h.n_contacts - h.n_dup;
+    t.num_stored = h.n_stored;
     t.num_meetings = h.n_meet;
     t.num_rechecks = h.n_rechecks;
     t.table_slots = c->s.tab_mask + 1;
@@ -1990,6 +2077,11 @@ glint_status glint_create(const glint_config *cfg, glint_ctx **out)
     s.min_meet = cfg->min_meet_time;
     s.complete = cfg->candidate_mode;
     s.precision = cfg->precision;
+    // a contact first seen by the inside join of its leaf and not within reach
+    // at the tick before needs no entry: it can only continue a run (found
+    // again from the leaves of the tick before) or end unemitted, unless
+    // min_meet_time is 0 or every contact is kept for glint_contacts
+    s.store_all = (cfg->record_contacts || cfg->min_meet_time == 0) ? 1u : 0u;
     s.delay = c->cfg.schema_update_delay;
     s.lx = cfg->low_x;
     s.ly = cfg->low_y;
@@ -2061,6 +2153,7 @@ glint_status glint_create(const glint_config *cfg, glint_ctx **out)
     ALLOC(s.merge_list, node_cap);
     ALLOC(s.leaf_list, node_cap);
     ALLOC(s.leaf_of, N);
+    ALLOC(s.prev_leaf, N);
     ALLOC(s.rank_of, N);
     ALLOC(s.border, N);
     ALLOC(s.units, s.unit_cap);
@@ -2555,9 +2648,12 @@ glint_status glint_active_pairs(glint_ctx *c, uint64_t *keys, uint32_t *starts, 
         HIP_TRY(c, hipMemcpy(hk.data(), tab.keys, (size_t)n * 8, hipMemcpyDeviceToHost));
         HIP_TRY(c, hipMemcpy(hs.data(), tab.starts, (size_t)n * 4, hipMemcpyDeviceToHost));
     }
+    // without store_all the generation also holds some pairs first seen at
+    // the last tick (border and tagged ones); they are listed only with it
+    const uint32_t last = c->prev_tick;
     std::vector<std::pair<uint64_t, uint32_t>> v;
     for (uint32_t i = 0; i < n; i++)
-        if (hs[i] != kDupStart) v.emplace_back(hk[i] & kKeyBits, hs[i]);
+        if (hs[i] != kDupStart && (c->s.store_all || hs[i] != last)) v.emplace_back(hk[i] & kKeyBits, hs[i]);
     std::sort(v.begin(), v.end());
     *n_out = v.size();
     if (!keys && !starts) return GLINT_OK;
@@ -2625,6 +2721,8 @@ glint_status glint_totals_get(glint_ctx *c, glint_totals *out, int reset)
     out->rechecks = h.acc_rechecks;
     out->max_contacts_tick = h.max_contacts;
     out->max_distinct_tick = h.max_distinct;
+    out->stored = h.acc_stored;
+    out->max_stored_tick = h.max_stored;
     out->max_meetings_tick = h.max_meet;
     if (reset) {
         size_t off = offsetof(Counters, acc_ticks);

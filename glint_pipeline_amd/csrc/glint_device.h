@@ -61,7 +61,8 @@ struct Counters {
     unsigned long long n_units;
     unsigned long long n_contacts;
     unsigned long long n_contacts_border;
-    unsigned long long n_inserted;
+    unsigned long long n_dup;       // second copies of pairs found twice this tick
+    unsigned long long n_stored;    // contacts kept in the tick's generation
     unsigned long long n_meet;
     unsigned long long n_rechecks;
     unsigned long long n_clist;     // contact-list entries reserved this tick (chunks)
@@ -90,6 +91,8 @@ struct Counters {
     unsigned long long max_contacts;
     unsigned long long max_distinct;
     unsigned long long max_meet;
+    unsigned long long acc_stored;
+    unsigned long long max_stored;
     unsigned long long n_kvlog;     // meetings logged for the key-value output since its last flush
     unsigned long long n_mlog;      // meetings ever appended to the meeting log
 };
@@ -140,6 +143,7 @@ struct DevState {
     uint32_t node_cap;
     uint32_t complete;
     uint32_t precision;
+    uint32_t store_all;     // keep every contact in the generation (else only those flagged kTagPrev/kTagBorder)
     uint32_t delay;
     double lx, ly, hx, hy;
     double r;
@@ -162,6 +166,7 @@ struct DevState {
     uint32_t *merge_list;
     uint32_t *leaf_list;
     uint32_t *leaf_of;
+    uint32_t *prev_leaf;    // leaf of every object at the tick before (pid order)
     uint32_t *rank_of;
     uint2 *border;
     uint2 *units;           // (object, candidate leaf) in lookup order

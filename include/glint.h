@@ -111,6 +111,7 @@ typedef struct glint_tick_stats {
     uint64_t num_rechecks;         /* FP64 rechecks (FP32 guard mode) */
     uint64_t table_slots;
     float ms_partition, ms_lookup, ms_refine, ms_evaluate, ms_schema;  /* HIP event times */
+    uint64_t num_stored;           /* contacts kept in the meeting table's generation */
 } glint_tick_stats;
 
 typedef struct glint_ctx glint_ctx;
@@ -199,7 +200,12 @@ glint_status glint_contacts(glint_ctx *ctx, uint64_t *out, uint64_t cap, uint64_
 glint_status glint_leaves(glint_ctx *ctx, double *boxes, uint32_t *levels, uint32_t *counts,
                           uint64_t cap, uint64_t *n_out);
 
-/* Active pairs after the last tick: keys (a<<32)|b and start ticks, sorted. */
+/* Active pairs after the last tick: keys (a<<32)|b and start ticks, sorted.
+ * With record_contacts or min_meet_time 0 every active pair is listed;
+ * otherwise a pair first seen at the last tick is not kept (it can only
+ * continue, when it is found again, or end unemitted) and only the pairs
+ * whose run began earlier are listed.  glint_tick_stats.num_active counts
+ * all of them. */
 glint_status glint_active_pairs(glint_ctx *ctx, uint64_t *keys, uint32_t *starts, uint64_t cap,
                                 uint64_t *n_out);
 
@@ -217,6 +223,8 @@ typedef struct glint_totals {
     uint64_t max_contacts_tick;
     uint64_t max_distinct_tick;
     uint64_t max_meetings_tick;
+    uint64_t stored;              /* contacts kept in the meeting table, summed over ticks */
+    uint64_t max_stored_tick;
 } glint_totals;
 glint_status glint_totals_get(glint_ctx *ctx, glint_totals *out, int reset);
 

@@ -79,6 +79,20 @@ def test_parity_fp32_guard_band_equals_fp64():
     run_parity(w, 25, precision=runtime.PRECISION_FP32_GUARD)
 
 
+@pytest.mark.parametrize("min_meet", [0, 3])
+def test_parity_without_recorded_contacts(min_meet):
+    """Without record_contacts a contact first seen by the inside join of its
+    leaf, and not within reach at the tick before, is not stored: it is
+    recognised when it continues from the objects' leaves at the tick before.
+    Meetings, distinct-pair counts and the listed pairs still equal the
+    oracle's (min_meet_time 0 keeps every contact)."""
+    w = gen.Workload("hot_pruned", 30_000, 30, 3000.0, seed=45, num_hotspots=8, hotspot_fraction=0.8,
+                     hotspot_sigma=80.0, grid_capacity=64)
+    w.min_meet_time = min_meet
+    tot = run_parity(w, 30, check_contacts=False, precision=runtime.PRECISION_FP32_GUARD)
+    assert tot["meetings"] > 100
+
+
 def test_guard_band_rechecks_only_inside_band():
     w = gen.uniform_city(5000, 10, 600.0, seed=43, grid_capacity=32)
     g = gen.Generator(w)
@@ -119,7 +133,9 @@ def test_run_ticks_device_trace_matches_oracle(graph_ticks):
         assert tot["ticks"] == w.ticks and tot["meetings"] == om
         k, s = wb.active_pairs()
         ok, os_ = o.active()
-        assert np.array_equal(k, ok) and np.array_equal(s, os_)
+        keep = os_ != w.ticks - 1   # pairs first seen at the last tick are not kept
+        assert len(k) > 100 and np.array_equal(k, ok[keep]) and np.array_equal(s, os_[keep])
+        assert wb.stats()["num_active"] == len(ok)
         b, lv, _ = wb.leaves()
         ob, olv, _ = o.leaves()
         assert np.array_equal(b, ob) and np.array_equal(lv, olv)
