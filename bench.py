@@ -104,8 +104,10 @@ def profiled_traffic(kernel: str, workload: str, precision: int, objects: int) -
     return None
 
 
-def cpu_baseline(w, ticks: int, threads: int) -> dict:
-    """The CPU oracle (C + OpenMP) on the first ticks of the same workload."""
+def cpu_baseline(w, ticks: int, threads: int, min_seconds: float = 10.0, max_ticks: int = 5000) -> dict:
+    """The CPU oracle (C + OpenMP) on the first ticks of the same workload:
+    at least `ticks` ticks, and more (the walk continues) until about
+    `min_seconds` of CPU work is timed."""
     from glint_pipeline_amd import gen
     from oracle.oracle import Oracle
     g = gen.Generator(w)
@@ -114,14 +116,16 @@ def cpu_baseline(w, ticks: int, threads: int) -> dict:
     o.tick(g.positions(), 0)          # tick 0 builds the tree (not timed, as on the GPU side)
     t_build = time.perf_counter() - t_build0
     tt = 0.0
-    for t in range(1, ticks + 1):
+    t = 0
+    while t < max_ticks and (t < ticks or tt < min_seconds):
+        t += 1
         xy = g.step()
         t0 = time.perf_counter()
         o.tick(xy, t)
         tt += time.perf_counter() - t0
     st = o.stats()
     o.close()
-    return {"seconds": tt, "ticks": ticks, "tick0_seconds": t_build, "contacts_last_tick": st["num_contacts"]}
+    return {"seconds": tt, "ticks": t, "tick0_seconds": t_build, "contacts_last_tick": st["num_contacts"]}
 
 
 WORKLOAD_SPEC = {

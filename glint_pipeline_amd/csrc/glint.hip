@@ -38,7 +38,7 @@ constexpr int kBlock = 256;
 constexpr int kScanItems = 4;                       // per thread
 constexpr int kScanTile = kBlock * kScanItems;      // elements per scan block
 constexpr int kTile = 512;                          // points per LDS tile of the join (default)
-constexpr int kStage = 512;                         // staged contacts per wave
+constexpr int kStage = 384;                         // staged contacts per wave
 constexpr int kFlushRound = 4;                      // join columns between stage checks
 constexpr int kMeetStage = 128;                     // staged meetings per wave (table update)
 constexpr uint32_t kChunk = 4096;                   // contact-list entries reserved per atomic
@@ -225,14 +225,35 @@ __global__ void k_init_tree(DevState s)
     }
 }
 
+// Node fields read as 16-byte vectors.  A descent picks child c from the
+// four ids with selects: indexing a private array by a runtime value would
+// place it in scratch memory.
+__device__ __forceinline__ double2 node_mid(const Node *nodes, uint32_t n)
+{
+    return reinterpret_cast<const double2 *>(nodes + n)[0];
+}
+__device__ __forceinline__ double2 node_high(const Node *nodes, uint32_t n)
+{
+    return reinterpret_cast<const double2 *>(nodes + n)[2];
+}
+__device__ __forceinline__ uint4 node_children(const Node *nodes, uint32_t n)
+{
+    return reinterpret_cast<const uint4 *>(nodes + n)[3];
+}
+__device__ __forceinline__ uint32_t pick_child(uint4 ch, int c)
+{
+    return c < 2 ? (c == 0 ? ch.x : ch.y) : (c == 2 ? ch.z : ch.w);
+}
+
 __device__ __forceinline__ uint32_t descend_to_leaf(const Node *nodes, double x, double y)
 {
     uint32_t cur = 0;
-    Node nd = nodes[0];
-    while (nd.child[0] != kLeafTag) {
-        int c = (y > nd.my) * 2 + (x > nd.mx);
-        cur = nd.child[c];
-        nd = nodes[cur];
+    double2 mid = node_mid(nodes, 0);
+    uint4 ch = node_children(nodes, 0);
+    while (ch.x != kLeafTag) {
+        cur = pick_child(ch, (y > mid.y) * 2 + (x > mid.x));
+        mid = node_mid(nodes, cur);
+        ch = node_children(nodes, cur);
     }
     return cur;
 }
@@ -425,6 +446,10 @@ __global__ __launch_bounds__(kBlock) void k_log_append(DevState s)
 
 // Descent with last_valid (filter.cpp:229-243): last_valid is the deepest
 // node on the path, below the root, with x + r < high.x && y + r < high.y.
+// Objects are taken in the leaf order of the tick before (spid), so the
+// lanes of a wave mostly descend the same path (one node line per wave and
+// level) and share a leaf, whose count one atomic per leaf and wave
+// advances.  The order used is kept for the scatter.
 __global__ __launch_bounds__(kBlock) void k_assign(DevState s)
 {
     __shared__ uint2 stage[kBlock / 64][kBorderStage];
@@ -434,11 +459,17 @@ __global__ __launch_bounds__(kBlock) void k_assign(DevState s)
     const double *xy = s.tp->xy;
     const double r = s.r;
     const bool root_leaf = s.nodes[0].child[0] == kLeafTag;
+    const bool ordered = s.ctr->sorted_valid != 0;
     for (uint32_t base = blockIdx.x * blockDim.x; base < s.n; base += gridDim.x * blockDim.x) {
-        uint32_t i = base + threadIdx.x;
-        bool active = i < s.n;
+        const uint32_t k = base + threadIdx.x;
+        bool active = k < s.n;
+        uint32_t i = k;
+        if (active && ordered) {
+            i = s.spid[k];
+            if (i >= s.n) i = k;   // not a permutation: only after an earlier error
+        }
         bool border = false;
-        uint32_t lv = 0, cur = 0;
+        uint32_t lv = 0, cur = kNone;
         if (active) {
             double x = xy[2 * (size_t)i], y = xy[2 * (size_t)i + 1];
             s.prev_leaf[i] = s.leaf_of[i];
@@ -447,19 +478,43 @@ __global__ __launch_bounds__(kBlock) void k_assign(DevState s)
                 s.leaf_of[i] = kNone;
                 active = false;
             } else {
-                Node nd = s.nodes[0];
-                while (nd.child[0] != kLeafTag) {
-                    int c = (y > nd.my) * 2 + (x > nd.mx);
-                    cur = nd.child[c];
-                    nd = s.nodes[cur];
-                    if (x + r < nd.hx && y + r < nd.hy) lv = cur;
+                cur = 0;
+                double2 mid = node_mid(s.nodes, 0);
+                uint4 ch = node_children(s.nodes, 0);
+                while (ch.x != kLeafTag) {
+                    cur = pick_child(ch, (y > mid.y) * 2 + (x > mid.x));
+                    mid = node_mid(s.nodes, cur);
+                    const double2 hi = node_high(s.nodes, cur);
+                    ch = node_children(s.nodes, cur);
+                    if (x + r < hi.x && y + r < hi.y) lv = cur;
                 }
-                uint32_t rank = atomicAdd(&s.count[cur], 1u);
-                s.leaf_of[i] = cur;
-                s.rank_of[i] = rank;
                 border = s.complete ? !root_leaf : (lv != cur);
                 if (s.complete) lv = 0;
             }
+        }
+        // ranks: the lanes that share a leaf take consecutive ranks with one atomic
+        uint32_t rank = 0;
+        bool need = active;
+        for (;;) {
+            const unsigned long long pend = __ballot(need);
+            if (!pend) break;
+            const uint32_t leader = __ffsll((long long)pend) - 1;
+            const uint32_t L = __shfl(cur, leader);
+            const bool mine = need && cur == L;
+            const unsigned long long mm = __ballot(mine);
+            uint32_t b = 0;
+            if (lane == leader) b = atomicAdd(&s.count[L], (uint32_t)__popcll(mm));
+            b = __shfl(b, leader);
+            if (mine) {
+                rank = b + (uint32_t)__popcll(mm & ((1ull << lane) - 1ull));
+                need = false;
+            }
+        }
+        if (active) s.leaf_of[i] = cur;
+        if (k < s.n) {
+            s.order[k] = i;
+            s.ord_leaf[k] = active ? cur : kNone;
+            s.ord_rank[k] = rank;
         }
         // border objects are staged per wave and appended one flush at a time
         unsigned long long m = __ballot(border);
@@ -470,7 +525,7 @@ __global__ __launch_bounds__(kBlock) void k_assign(DevState s)
             uint32_t b0 = 0;
             if (lane == 0) b0 = atomicAdd(&s.ctr->n_border, nst);
             b0 = __shfl(b0, 0);
-            for (uint32_t k = lane; k < nst; k += 64) s.border[b0 + k] = st[k];
+            for (uint32_t q = lane; q < nst; q += 64) s.border[b0 + q] = st[q];
             wave_sync();
             nst = 0;
         }
@@ -480,7 +535,7 @@ __global__ __launch_bounds__(kBlock) void k_assign(DevState s)
         uint32_t b0 = 0;
         if (lane == 0) b0 = atomicAdd(&s.ctr->n_border, nst);
         b0 = __shfl(b0, 0);
-        for (uint32_t k = lane; k < nst; k += 64) s.border[b0 + k] = st[k];
+        for (uint32_t q = lane; q < nst; q += 64) s.border[b0 + q] = st[q];
     }
 }
 
@@ -593,16 +648,20 @@ __global__ __launch_bounds__(kBlock) void k_scan_units(DevState s)
     }
 }
 
+// Counting sort by leaf, in the order of the assignment: objects that share
+// a leaf were given consecutive ranks there, so the sorted arrays are
+// written in runs.
 __global__ __launch_bounds__(kBlock) void k_scatter(DevState s)
 {
     const double *xy = s.tp->xy;
     const double *xy_prev = s.tp->xy_prev;
     double *xy_save = s.tp->xy_save;
     const bool has_prev = s.tp->has_prev != 0;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < s.n; i += gridDim.x * blockDim.x) {
-        uint32_t leaf = s.leaf_of[i];
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < s.n; k += gridDim.x * blockDim.x) {
+        const uint32_t leaf = s.ord_leaf[k];
         if (leaf == kNone) continue;
-        uint32_t pos = s.offset[leaf] + s.rank_of[i];
+        const uint32_t i = s.order[k];
+        const uint32_t pos = s.offset[leaf] + s.ord_rank[k];
         double x = xy[2 * (size_t)i], y = xy[2 * (size_t)i + 1];
         s.sx[pos] = x;
         s.sy[pos] = y;
@@ -617,8 +676,14 @@ __global__ __launch_bounds__(kBlock) void k_scatter(DevState s)
             Node nd = s.nodes[leaf];
             s.fx[pos] = (float)(x - nd.lx);
             s.fy[pos] = (float)(y - nd.ly);
+            if (has_prev) {
+                const double ox = xy_prev[2 * (size_t)i] - nd.lx, oy = xy_prev[2 * (size_t)i + 1] - nd.ly;
+                s.pfx[pos] = fabs(ox) <= s.pf_lim ? (float)ox : __builtin_nanf("");
+                s.pfy[pos] = fabs(oy) <= s.pf_lim ? (float)oy : __builtin_nanf("");
+            }
         }
     }
+    if (blockIdx.x == 0 && threadIdx.x == 0) s.ctr->sorted_valid = 1;
 }
 
 __device__ __forceinline__ uint32_t wave_exclusive_scan(uint32_t v, uint32_t *total)
@@ -759,7 +824,7 @@ __device__ __forceinline__ void stage_flush(Stager &st, const DevState &s, Key k
     const uint32_t lane = lane_id();
     for (uint32_t done = 0; done < st.n; done += 64) {
         const uint32_t i = done + lane;
-        const unsigned long long k = i < st.n ? key(st.buf[i]) : kEmptyKey;
+        const unsigned long long k = i < st.n ? key(st.buf[i], true) : kEmptyKey;
         const bool keep = k != kEmptyKey;
         const unsigned long long m = __ballot(keep);
         const uint32_t cnt = (uint32_t)__popcll(m);
@@ -805,6 +870,7 @@ struct JoinTile {
     double2 xy[TILE];              // positions at tick
     double2 pxy[TILE];             // positions at prev_tick
     float2 f[TILE];                // FP32 offsets from the leaf's low corner (guard mode)
+    float2 pf[TILE];               // the same for prev_tick (NaN: not bounded)
     uint32_t p[TILE];              // object ids
 };
 
@@ -831,6 +897,7 @@ __device__ __forceinline__ void load_tile(JoinTile<TILE> &t, const DevState &s, 
         t.xy[k] = make_double2(s.sx[from + k], s.sy[from + k]);
         if (has_prev) t.pxy[k] = make_double2(s.sxp[from + k], s.syp[from + k]);
         t.f[k] = use32 ? make_float2(s.fx[from + k], s.fy[from + k]) : make_float2(0.f, 0.f);
+        if (use32 && has_prev) t.pf[k] = make_float2(s.pfx[from + k], s.pfy[from + k]);
         t.p[k] = s.spid[from + k];
     }
 }
@@ -843,29 +910,38 @@ constexpr uint32_t kBandBit = 1u << 15;
 // One wave task's column loop: the row lane's object against columns
 // [jbeg, jend) of tile C, from column jmin on (jmin = ~0 for an idle lane).
 // In guard mode (USE32) pairs are decided on the FP32 offsets: inside
-// r - eps a hit, past r + eps a miss, in between a band entry.
+// r - eps a hit, past r + eps a miss, in between a band entry.  Sure hits
+// are counted here; a sure inside hit is staged only if it may be stored:
+// when `keep_all`, or when the FP32 offsets at prev_tick do not place the
+// pair surely out of reach then (the exact test follows at write-out).
+struct JoinCounts {
+    uint32_t sure;   // sure contacts (wave-uniform)
+    uint32_t pass;   // band entries within reach in FP64 (per lane)
+    uint32_t band;   // band entries re-tested (per lane)
+};
+
 template <bool USE32, int TILE>
-__device__ __forceinline__ void join_columns(Stager &st, const DevState &s, const JoinTile<TILE> &C, double xi,
-                                             double yi, float fxi, float fyi, uint32_t jmin, uint32_t jbeg,
-                                             uint32_t jend, float t32_in, float t32_out, double t64,
-                                             uint32_t &n_staged, uint32_t &rej, uint32_t &band,
-                                             const JoinRows &rows, unsigned long long flag, bool has_prev)
+__device__ __forceinline__ JoinCounts join_columns(Stager &st, const DevState &s, const JoinTile<TILE> &C, double xi,
+                                                   double yi, float fxi, float fyi, float pfxi, float pfyi,
+                                                   uint32_t jmin, uint32_t jbeg, uint32_t jend, float t32_in,
+                                                   float t32_out, double t64, bool keep_all, const JoinRows &rows,
+                                                   unsigned long long flag, bool has_prev)
 {
+    uint32_t n_sure = 0, n_pass = 0, band = 0;
     const uint32_t tag = lane_id() << 16;
     // the stage's contacts as keys (kEmptyKey: a band entry that is not
-    // within reach in FP64, or a contact that is not stored)
-    auto key_of = [&](uint32_t e) {
+    // within reach in FP64, or a contact that is not stored); `counted`
+    // entries add to the recheck counters
+    auto key_of = [&](uint32_t e, bool counted) {
         const uint32_t rl = e >> 16, j = e & 0x7FFFu;
         if (USE32 && (e & kBandBit)) {
-            band++;
+            band += counted;
             const uint32_t pa = rows.p[rl];
             const double2 b = C.xy[j];
             const double ex = s.tp->xy[2 * (size_t)pa] - b.x;
             const double ey = s.tp->xy[2 * (size_t)pa + 1] - b.y;
-            if (!(ex * ex + ey * ey <= t64)) {
-                rej++;
-                return kEmptyKey;
-            }
+            if (!(ex * ex + ey * ey <= t64)) return kEmptyKey;
+            n_pass += counted;
         }
         unsigned long long k = pair_key(rows.p[rl], C.p[j]) | flag;
         if (has_prev) {
@@ -877,43 +953,51 @@ __device__ __forceinline__ void join_columns(Stager &st, const DevState &s, cons
         // a contact neither tagged nor from a border unit is not stored
         return (s.store_all || (k & (kTagPrev | kTagBorder))) ? k : kEmptyKey;
     };
+    const bool filt = !keep_all && has_prev;   // sure inside hits may be skipped
     for (uint32_t jb = jbeg; jb < jend; jb += kFlushRound) {
-        float2 cf[kFlushRound];
+        float2 cf[kFlushRound], cpf[kFlushRound];
         double2 cx[kFlushRound];
 #pragma unroll
         for (uint32_t jj = 0; jj < kFlushRound; jj++) {
             const uint32_t jc = min(jb + jj, jend - 1);
-            if constexpr (USE32)
+            if constexpr (USE32) {
                 cf[jj] = C.f[jc];
-            else
+                cpf[jj] = C.pf[jc];
+            } else {
                 cx[jj] = C.xy[jc];
+            }
         }
 #pragma unroll
         for (uint32_t jj = 0; jj < kFlushRound; jj++) {
             const uint32_t j = jb + jj;
             const bool cand = j >= jmin && j < jend;
-            bool hit;
+            bool sure, stage;
             uint32_t e = tag | j;
             if constexpr (USE32) {
                 const float dx = fxi - cf[jj].x;
                 const float dy = fyi - cf[jj].y;
                 const float s32 = dx * dx + dy * dy;
-                hit = s32 <= t32_out;
-                e |= s32 <= t32_in ? 0u : kBandBit;
+                sure = cand && s32 <= t32_in;
+                const bool hit = cand && s32 <= t32_out;
+                if (!sure) e |= kBandBit;
+                const float px = pfxi - cpf[jj].x;
+                const float py = pfyi - cpf[jj].y;
+                const float p32 = px * px + py * py;
+                // NaN (unbounded offsets) compares false: staged
+                stage = hit && (!sure || keep_all || (filt && !(p32 > t32_out)));
             } else {
                 const double ex = xi - cx[jj].x;
                 const double ey = yi - cx[jj].y;
-                hit = ex * ex + ey * ey <= t64;
+                sure = cand && ex * ex + ey * ey <= t64;
+                stage = sure && (keep_all || has_prev);
             }
-            stage_push(st, hit && cand, e);
+            n_sure += (uint32_t)__popcll(__ballot(sure));
+            stage_push(st, stage, e);
         }
-        if (st.n > kStage - 64 * kFlushRound) {
-            n_staged += st.n;
-            stage_flush(st, s, key_of);
-        }
+        if (st.n > kStage - 64 * kFlushRound) stage_flush(st, s, key_of);
     }
-    n_staged += st.n;
     stage_flush(st, s, key_of);
+    return JoinCounts{n_sure, n_pass, band};
 }
 
 // The refine stage for one leaf L per block (work queue over the leaf list):
@@ -939,8 +1023,8 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
     const uint32_t wave = threadIdx.x / 64, lane = lane_id();
     Stager st{contact_out(s), lds.stage[wave], 0, 0, kChunk, 0};
     JoinRows &rows = lds.rows[wave];
-    uint32_t n_in = 0, n_bd = 0;                 // staged entries (wave-uniform)
-    uint32_t rej_in = 0, rej_bd = 0, band = 0;   // per lane
+    uint32_t n_in = 0, n_bd = 0;                 // sure contacts (wave-uniform)
+    uint32_t pass_in = 0, pass_bd = 0, band = 0;   // band entries re-tested within reach, re-tested (per lane)
     for (;;) {
         if (threadIdx.x == 0) lds.item = atomicAdd(&s.ctr->work_inside, kInsideChunk);
         __syncthreads();
@@ -973,7 +1057,7 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
                     const uint32_t ntask = ntask_in + nuc;
                     for (uint32_t t = wave; t < ntask; t += kBlock / 64) {
                         double xi = 0.0, yi = 0.0, pxi = 0.0, pyi = 0.0;
-                        float fxi = 0.f, fyi = 0.f;
+                        float fxi = 0.f, fyi = 0.f, pfxi = 0.f, pfyi = 0.f;
                         uint32_t pi, jbeg, jend, jmin;
                         bool inside;
                         if (t < ntask_in) {
@@ -1004,6 +1088,11 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
                             }
                             fxi = af.x;
                             fyi = af.y;
+                            if (use32 && has_prev) {
+                                const float2 apf = A.pf[rr];
+                                pfxi = apf.x;
+                                pfyi = apf.y;
+                            }
                             pi = A.p[rr];
                             jbeg = q * 64;
                             jend = min(jbeg + 64, nj);
@@ -1035,14 +1124,17 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
                         rows.pxy[lane] = make_double2(pxi, pyi);
                         const JoinTile<TILE> &C = inside ? B : A;
                         const unsigned long long flag = inside ? 0ull : kTagBorder;
-                        uint32_t &n_staged = inside ? n_in : n_bd;
-                        uint32_t &rej = inside ? rej_in : rej_bd;
-                        if (use32)
-                            join_columns<true>(st, s, C, xi, yi, fxi, fyi, jmin, jbeg, jend, s.t32_in, s.t32_out,
-                                               t64, n_staged, rej, band, rows, flag, has_prev);
-                        else
-                            join_columns<false>(st, s, C, xi, yi, fxi, fyi, jmin, jbeg, jend, s.t32_in, s.t32_out,
-                                                t64, n_staged, rej, band, rows, flag, has_prev);
+                        const bool keep_all = !inside || s.store_all;
+                        const JoinCounts jc =
+                            use32 ? join_columns<true>(st, s, C, xi, yi, fxi, fyi, pfxi, pfyi, jmin, jbeg, jend,
+                                                       s.t32_in, s.t32_out, t64, keep_all, rows, flag, has_prev)
+                                  : join_columns<false>(st, s, C, xi, yi, fxi, fyi, pfxi, pfyi, jmin, jbeg, jend,
+                                                        s.t32_in, s.t32_out, t64, keep_all, rows, flag, has_prev);
+                        n_in += inside ? jc.sure : 0u;
+                        n_bd += inside ? 0u : jc.sure;
+                        pass_in += inside ? jc.pass : 0u;
+                        pass_bd += inside ? 0u : jc.pass;
+                        band += jc.band;
                     }
                     __syncthreads();
                 }
@@ -1051,14 +1143,14 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
     }
     stage_close(st, s);
     // one atomic per wave for each counter
-    uint32_t ri = rej_in, rb = rej_bd, bd = band;
+    uint32_t pi_ = pass_in, pb = pass_bd, bd = band;
     for (int d = 32; d >= 1; d >>= 1) {
-        ri += __shfl_xor(ri, d);
-        rb += __shfl_xor(rb, d);
+        pi_ += __shfl_xor(pi_, d);
+        pb += __shfl_xor(pb, d);
         bd += __shfl_xor(bd, d);
     }
     if (lane == 0) {
-        const uint32_t ci = n_in - ri, cb = n_bd - rb;
+        const uint32_t ci = n_in + pi_, cb = n_bd + pb;
         if (ci + cb) atomicAdd(&s.ctr->n_contacts, (unsigned long long)(ci + cb));
         if (cb) atomicAdd(&s.ctr->n_contacts_border, (unsigned long long)cb);
         if (bd) atomicAdd(&s.ctr->n_rechecks, (unsigned long long)bd);
@@ -1486,30 +1578,32 @@ __global__ __launch_bounds__(NT) void k_table(DevState s)
     glint_meeting *st = L.stage[threadIdx.x / 64];
     uint32_t nst = 0, dups = 0;
     const bool void_tick = (s.ctr->error & kErrContactCap) != 0;
-    // the previous generation's entries of the next region are loaded into
-    // registers while the current region is processed
+    // the previous generation's entries and the tick's contacts of the next
+    // region are loaded into registers while the current region is processed
     constexpr int kPer = kRegion / NT;
-    unsigned long long pk[kPer], nk[kPer];
+    unsigned long long pk[kPer], nk[kPer], nkn[kPer];
     uint32_t ps[kPer], ns[kPer];
-    uint32_t o0 = 0, o1 = 0;
-    if (blockIdx.x < nreg) {
-        o0 = told.rstart[blockIdx.x];
-        o1 = told.rstart[blockIdx.x + 1];
+    uint32_t o0 = 0, o1 = 0, c0n = 0, c1n = 0;
+    auto prefetch = [&](uint32_t rr) {
+        o0 = told.rstart[rr];
+        o1 = told.rstart[rr + 1];
+        c0n = void_tick ? 0 : tcur.rstart[rr];
+        c1n = void_tick ? 0 : tcur.rstart[rr + 1];
 #pragma unroll
         for (int q = 0; q < kPer; q++) {
             const uint32_t i = o0 + threadIdx.x + q * NT;
             pk[q] = i < o1 ? told.keys[i] : kEmptyKey;
             ps[q] = i < o1 ? told.starts[i] : kDupStart;
+            const uint32_t c = c0n + threadIdx.x + q * NT;
+            nkn[q] = c < c1n ? tcur.keys[c] : 0ull;
         }
-    }
+    };
+    if (blockIdx.x < nreg) prefetch(blockIdx.x);
     for (uint32_t r = blockIdx.x; r < nreg; r += gridDim.x) {
         for (uint32_t k = threadIdx.x; k < kRegion; k += NT) L.hk[k] = kEmptyKey;
-        const uint32_t c0 = void_tick ? 0 : tcur.rstart[r], c1 = void_tick ? 0 : tcur.rstart[r + 1];
+        const uint32_t c0 = c0n, c1 = c1n;
 #pragma unroll
-        for (int q = 0; q < kPer; q++) {
-            const uint32_t c = c0 + threadIdx.x + q * NT;
-            nk[q] = c < c1 ? tcur.keys[c] : 0ull;
-        }
+        for (int q = 0; q < kPer; q++) nk[q] = nkn[q];
         __syncthreads();
         // the tick's contacts that can continue a run or be a second copy
 #pragma unroll
@@ -1539,16 +1633,7 @@ __global__ __launch_bounds__(NT) void k_table(DevState s)
             table_old(L, s, v, v ? told.keys[i] : 0ull, v ? told.starts[i] : kDupStart, i, tick, last, st, nst);
         }
         const uint32_t rn = r + gridDim.x;
-        if (rn < nreg) {
-            o0 = told.rstart[rn];
-            o1 = told.rstart[rn + 1];
-#pragma unroll
-            for (int q = 0; q < kPer; q++) {
-                const uint32_t i = o0 + threadIdx.x + q * NT;
-                pk[q] = i < o1 ? told.keys[i] : kEmptyKey;
-                ps[q] = i < o1 ? told.starts[i] : kDupStart;
-            }
-        }
+        if (rn < nreg) prefetch(rn);
         __syncthreads();
         // starts (and boxes) of every contact of the region
 #pragma unroll
@@ -2104,6 +2189,7 @@ glint_status glint_create(const glint_config *cfg, glint_ctx **out)
         const double u = std::ldexp(1.0, -24);
         double w = (eps / 2 - (2 + std::sqrt(2.0)) * u * (r + eps)) / (2 * std::sqrt(2.0) * u * (1 + u)) - r;
         s.w32_max = w > 0 ? w : 0.0;
+        s.pf_lim = s.w32_max + r;
     }
     uint64_t node_cap = cfg->node_capacity ? cfg->node_capacity
                                            : std::max<uint64_t>(65536, 16 * (N / cfg->grid_capacity + 1) + 1024);
@@ -2154,7 +2240,6 @@ glint_status glint_create(const glint_config *cfg, glint_ctx **out)
     ALLOC(s.leaf_list, node_cap);
     ALLOC(s.leaf_of, N);
     ALLOC(s.prev_leaf, N);
-    ALLOC(s.rank_of, N);
     ALLOC(s.border, N);
     ALLOC(s.units, s.unit_cap);
     ALLOC(s.urank, s.unit_cap);
@@ -2168,9 +2253,14 @@ glint_status glint_create(const glint_config *cfg, glint_ctx **out)
     ALLOC(c->hist[0], 2 * N);
     ALLOC(c->hist[1], 2 * N);
     ALLOC(s.spid, N);
+    ALLOC(s.order, N);
+    ALLOC(s.ord_leaf, N);
+    ALLOC(s.ord_rank, N);
     if (cfg->precision == GLINT_PRECISION_FP32_GUARD) {
         ALLOC(s.fx, N);
         ALLOC(s.fy, N);
+        ALLOC(s.pfx, N);
+        ALLOC(s.pfy, N);
     }
     ALLOC(s.meet, s.meet_cap);
     ALLOC(s.ctr, 1);

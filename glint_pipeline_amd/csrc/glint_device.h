@@ -93,6 +93,8 @@ struct Counters {
     unsigned long long max_meet;
     unsigned long long acc_stored;
     unsigned long long max_stored;
+    uint32_t sorted_valid;          // spid holds a complete leaf order (after the first scatter)
+    uint32_t pad_;
     unsigned long long n_kvlog;     // meetings logged for the key-value output since its last flush
     unsigned long long n_mlog;      // meetings ever appended to the meeting log
 };
@@ -151,6 +153,7 @@ struct DevState {
     float t32_in;       // FP32 squared-distance bounds of the guard band
     float t32_out;
     double w32_max;     // widest leaf decided in FP32 (error bound inside the band)
+    double pf_lim;      // largest FP32 offset with the same error bound (w32_max + r)
     unsigned long long tab_mask;
     unsigned long long unit_cap;
     unsigned long long meet_cap;
@@ -167,7 +170,6 @@ struct DevState {
     uint32_t *leaf_list;
     uint32_t *leaf_of;
     uint32_t *prev_leaf;    // leaf of every object at the tick before (pid order)
-    uint32_t *rank_of;
     uint2 *border;
     uint2 *units;           // (object, candidate leaf) in lookup order
     uint32_t *urank;        // rank of the unit among the units of its leaf
@@ -179,8 +181,13 @@ struct DevState {
     double *sxp;            // positions at prev_tick, sorted like sx/sy
     double *syp;
     uint32_t *spid;
+    uint32_t *order;        // object handled by position k of the tick's assignment (spid of the tick before)
+    uint32_t *ord_leaf;     // its leaf and rank in that leaf, by k
+    uint32_t *ord_rank;
     float *fx;
     float *fy;
+    float *pfx;             // positions at the tick before as FP32 offsets from the same corner
+    float *pfy;             // (NaN beyond pf_lim: left to the FP64 test)
     glint_meeting *meet;
     unsigned long long *clist2;    // bucketing buffer (the join's output when there is one pass)
     uint32_t *bcount;              // coarse-bin counts per bucketing block
