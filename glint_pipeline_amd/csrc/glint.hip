@@ -52,6 +52,7 @@ constexpr int kBorderStage = 256;                   // staged border objects per
 constexpr int kLookupStack = 128;
 constexpr int kMaxCand = 32;                        // candidate leaves per border object
 constexpr uint32_t kInsideChunk = 4;                // leaves per work-queue grab
+constexpr uint32_t kHeavyLeaf = 256;                // leaves above this size are joined first
 
 // ------------------------------------------------------------ predicates
 
@@ -182,6 +183,7 @@ __global__ void k_reset_tick(DevState s)
     c->n_merge = 0;
     c->n_border = 0;
     c->n_leaf_list = 0;
+    c->n_heavy = 0;
     c->n_leaves = 0;
     c->max_leaf = 0;
     c->work_inside = 0;
@@ -625,7 +627,9 @@ __global__ __launch_bounds__(kBlock) void k_scan_objects(DevState s)
 }
 
 // Unit offsets of every leaf, and the join work list: leaves with >= 2
-// objects (inside pairs) or with border units aimed at them.
+// objects (inside pairs) or with border units aimed at them.  Leaves of more
+// than kHeavyLeaf objects are listed from the back, so that the join takes
+// them first and ends on small leaves.
 __global__ __launch_bounds__(kBlock) void k_scan_units(DevState s)
 {
     uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
@@ -642,9 +646,12 @@ __global__ __launch_bounds__(kBlock) void k_scan_units(DevState s)
         bool in = n < s.node_cap;
         if (in) s.ustart[n] = ex;
         ex += c[k];
-        bool work = in && (c[k] > 0 || s.count[n] >= 2);
-        uint32_t idx = wave_append_u32(&s.ctr->n_leaf_list, work);
-        if (work) s.leaf_list[idx] = n;
+        const uint32_t cnt = in ? s.count[n] : 0u;
+        const bool work = in && (c[k] > 0 || cnt >= 2);
+        const bool heavy = work && cnt > kHeavyLeaf;
+        const uint32_t idx = wave_append_u32(&s.ctr->n_leaf_list, work && !heavy);
+        const uint32_t hdx = wave_append_u32(&s.ctr->n_heavy, heavy);
+        if (work) s.leaf_list[heavy ? s.node_cap - 1 - hdx : idx] = n;
     }
 }
 
@@ -881,13 +888,6 @@ struct JoinRows {
     uint32_t p[64];
 };
 
-template <int TILE>
-struct JoinLds {
-    JoinTile<TILE> a, b;
-    JoinRows rows[kBlock / 64];
-    uint32_t stage[kBlock / 64][kStage + 64];
-    uint32_t item;
-};
 
 template <int TILE>
 __device__ __forceinline__ void load_tile(JoinTile<TILE> &t, const DevState &s, uint32_t from, uint32_t n, bool has_prev,
@@ -1000,17 +1000,55 @@ __device__ __forceinline__ JoinCounts join_columns(Stager &st, const DevState &s
     return JoinCounts{n_sure, n_pass, band};
 }
 
-// The refine stage for one leaf L per block (work queue over the leaf list):
+// A leaf of the join's current chunk of work.
+struct JoinLeaf {
+    uint32_t off, cnt;        // its objects in the sorted arrays
+    uint32_t uoff, ucnt;      // its border units
+    uint32_t base;            // first tile slot (leaves joined together)
+    uint32_t task0;           // first task among the group's tasks
+    uint32_t ntask_in;        // inside tasks (upper triangle of 64-row blocks)
+    uint32_t ntask;           // with the border tasks
+    uint32_t use32;
+    uint32_t pad;
+    double lx, ly;            // low corner (FP32 offsets of border units)
+};
+
+template <int TILE>
+struct JoinLds {
+    JoinTile<TILE> a, b;
+    JoinRows rows[kBlock / 64];
+    uint32_t stage[kBlock / 64][kStage + 64];
+    JoinLeaf leaf[kInsideChunk];
+    uint32_t gfirst[kInsideChunk + 1];   // groups of consecutive leaves of the chunk
+    uint32_t ngroup;
+    uint32_t item;
+};
+
+// Upper-triangle task t of n row blocks -> (r <= q), row-major.
+__device__ __forceinline__ void triangle_task(uint32_t t, uint32_t n, uint32_t &r, uint32_t &q)
+{
+    r = 0;
+    while (t >= n - r) {
+        t -= n - r;
+        r++;
+    }
+    q = r + t;
+}
+
+// The refine stage (work queue over chunks of the leaf list):
 //  - inside units (filter.cpp:107-109 with refinement.cpp:30, pid < q): every
-//    unordered pair of L's objects once;
+//    unordered pair of a leaf's objects once;
 //  - border units (pid, L, inside=false) (refinement.cpp:30 with !inside):
 //    each object whose lookup selected L against every object of L.
 // Work is split into 64x64 wave tasks (64 row lanes x 64 broadcast columns).
-// The column loop only tests distances and stages (row lane, column) of each
-// hit; when a task's stage is written out, one lane per contact forms the
-// pair key, adds kTagPrev when the pair was also within reach at prev_tick
-// (the only case that needs the previous table generation) and kTagBorder
-// when it came from a border unit (the only case that can find a pair twice).
+// Consecutive leaves of a chunk whose objects fit one tile are loaded
+// together and their tasks are shared by all waves at once; a leaf larger
+// than a tile is joined tile pair by tile pair.  The column loop only tests
+// distances and stages (row lane, column) of each hit; when a task's stage
+// is written out, one lane per contact forms the pair key, adds kTagPrev
+// when the pair was also within reach at prev_tick (the only case that
+// needs the previous table generation) and kTagBorder when it came from a
+// border unit (the only case that can find a pair twice).
 template <int TILE>
 __global__ __launch_bounds__(kBlock) void k_join(DevState s)
 {
@@ -1019,125 +1057,202 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
     const double *xy_prev = s.tp->xy_prev;
     const bool has_prev = s.tp->has_prev != 0;
     const double t64 = s.t64;
-    const uint32_t nl = (s.ctr->error & kErrUnitCap) ? 0 : s.ctr->n_leaf_list;
+    const uint32_t nh = s.ctr->n_heavy;
+    const uint32_t nl = (s.ctr->error & kErrUnitCap) ? 0 : s.ctr->n_leaf_list + nh;
     const uint32_t wave = threadIdx.x / 64, lane = lane_id();
     Stager st{contact_out(s), lds.stage[wave], 0, 0, kChunk, 0};
     JoinRows &rows = lds.rows[wave];
-    uint32_t n_in = 0, n_bd = 0;                 // sure contacts (wave-uniform)
-    uint32_t pass_in = 0, pass_bd = 0, band = 0;   // band entries re-tested within reach, re-tested (per lane)
+    uint32_t n_in = 0, n_bd = 0;                   // sure contacts (wave-uniform)
+    uint32_t pass_in = 0, pass_bd = 0, band = 0;   // band entries within reach / re-tested (per lane)
+
+    // One task: an inside task (row slot `row` of tile A) or a border task
+    // (unit object `pid`), against columns [jbeg, jend) of tile C from jmin.
+    auto run = [&](const JoinTile<TILE> &A, const JoinTile<TILE> &C, bool inside, uint32_t row, uint32_t pid,
+                   bool vi, uint32_t jbeg, uint32_t jend, uint32_t jmin, bool use32, double lx, double ly) {
+        double xi = 0.0, yi = 0.0, pxi = 0.0, pyi = 0.0;
+        float fxi = 0.f, fyi = 0.f, pfxi = 0.f, pfyi = 0.f;
+        uint32_t pi = 0;
+        if (inside) {
+            const uint32_t rr = vi ? row : jbeg;
+            const double2 a = A.xy[rr];
+            const float2 af = A.f[rr];
+            xi = a.x;
+            yi = a.y;
+            if (has_prev) {
+                const double2 ap = A.pxy[rr];
+                pxi = ap.x;
+                pyi = ap.y;
+            }
+            fxi = af.x;
+            fyi = af.y;
+            if (use32 && has_prev) {
+                const float2 apf = A.pf[rr];
+                pfxi = apf.x;
+                pfyi = apf.y;
+            }
+            pi = A.p[rr];
+        } else {
+            pi = pid;
+            if (vi) {
+                xi = xy[2 * (size_t)pi];
+                yi = xy[2 * (size_t)pi + 1];
+                if (has_prev) {
+                    pxi = xy_prev[2 * (size_t)pi];
+                    pyi = xy_prev[2 * (size_t)pi + 1];
+                }
+            }
+            if (use32) {
+                fxi = (float)(xi - lx);
+                fyi = (float)(yi - ly);
+            }
+        }
+        if (!vi) jmin = 0xFFFFFFFFu;   // an idle lane
+        rows.p[lane] = pi;
+        rows.pxy[lane] = make_double2(pxi, pyi);
+        const unsigned long long flag = inside ? 0ull : kTagBorder;
+        const bool keep_all = !inside || s.store_all;
+        const JoinCounts jc =
+            use32 ? join_columns<true>(st, s, C, xi, yi, fxi, fyi, pfxi, pfyi, jmin, jbeg, jend, s.t32_in, s.t32_out,
+                                       t64, keep_all, rows, flag, has_prev)
+                  : join_columns<false>(st, s, C, xi, yi, fxi, fyi, pfxi, pfyi, jmin, jbeg, jend, s.t32_in,
+                                        s.t32_out, t64, keep_all, rows, flag, has_prev);
+        n_in += inside ? jc.sure : 0u;
+        n_bd += inside ? 0u : jc.sure;
+        pass_in += inside ? jc.pass : 0u;
+        pass_bd += inside ? 0u : jc.pass;
+        band += jc.band;
+    };
+
     for (;;) {
         if (threadIdx.x == 0) lds.item = atomicAdd(&s.ctr->work_inside, kInsideChunk);
         __syncthreads();
         const uint32_t w0 = lds.item;
-        __syncthreads();
         if (w0 >= nl) break;
-        for (uint32_t w = w0; w < min(w0 + kInsideChunk, nl); w++) {
-            const uint32_t leaf = s.leaf_list[w];
-            const uint32_t off = s.offset[leaf], cnt = s.count[leaf];
-            const uint32_t uoff = s.ustart[leaf], ucnt = s.ucount[leaf];
-            const uint32_t ntile = (cnt + TILE - 1) / TILE;
-            const Node nd = s.nodes[leaf];
-            const bool use32 = s.precision == GLINT_PRECISION_FP32_GUARD && (nd.hx - nd.lx) <= s.w32_max &&
-                               (nd.hy - nd.ly) <= s.w32_max;
-            for (uint32_t I = 0; I < ntile; I++) {
-                const uint32_t ni = min((uint32_t)TILE, cnt - I * TILE);
-                load_tile(lds.a, s, off + I * TILE, ni, has_prev, use32);
-                for (uint32_t J = I; J < ntile; J++) {
-                    const uint32_t nj = min((uint32_t)TILE, cnt - J * TILE);
-                    const bool diag = (J == I);
-                    const JoinTile<TILE> &B = diag ? lds.a : lds.b;
-                    if (!diag) load_tile(lds.b, s, off + J * TILE, nj, has_prev, use32);
-                    __syncthreads();
-                    const JoinTile<TILE> &A = lds.a;
-                    // inside tasks: 64-row sub-tile r against 64-column sub-tile q
-                    const uint32_t nri = (ni + 63) / 64, nrj = (nj + 63) / 64;
-                    const uint32_t ntask_in = diag ? nri * (nri + 1) / 2 : nri * nrj;
-                    // border tasks (once per tile of L): 64 units against the tile
-                    const uint32_t nuc = diag ? (ucnt + 63) / 64 : 0;
-                    const uint32_t ntask = ntask_in + nuc;
-                    for (uint32_t t = wave; t < ntask; t += kBlock / 64) {
-                        double xi = 0.0, yi = 0.0, pxi = 0.0, pyi = 0.0;
-                        float fxi = 0.f, fyi = 0.f, pfxi = 0.f, pfyi = 0.f;
-                        uint32_t pi, jbeg, jend, jmin;
-                        bool inside;
-                        if (t < ntask_in) {
-                            uint32_t r, q;
-                            if (diag) {   // t -> (r <= q), row-major upper triangle
-                                r = 0;
-                                uint32_t rem = t;
-                                while (rem >= nri - r) {
-                                    rem -= nri - r;
-                                    r++;
-                                }
-                                q = r + rem;
-                            } else {
-                                r = t / nrj;
-                                q = t % nrj;
-                            }
-                            const uint32_t row = r * 64 + lane;
-                            const bool vi = row < ni;
-                            const uint32_t rr = vi ? row : 0;
-                            const double2 a = A.xy[rr];
-                            const float2 af = A.f[rr];
-                            xi = a.x;
-                            yi = a.y;
-                            if (has_prev) {
-                                const double2 ap = A.pxy[rr];
-                                pxi = ap.x;
-                                pyi = ap.y;
-                            }
-                            fxi = af.x;
-                            fyi = af.y;
-                            if (use32 && has_prev) {
-                                const float2 apf = A.pf[rr];
-                                pfxi = apf.x;
-                                pfyi = apf.y;
-                            }
-                            pi = A.p[rr];
-                            jbeg = q * 64;
-                            jend = min(jbeg + 64, nj);
-                            // only j > row in a diagonal task; ~0: an idle lane
-                            jmin = !vi ? 0xFFFFFFFFu : (diag ? row + 1 : 0u);
-                            inside = true;
-                        } else {
-                            const uint32_t u = (t - ntask_in) * 64 + lane;
-                            const bool vi = u < ucnt;
-                            pi = vi ? s.unit_pid[uoff + u] : 0;
-                            if (vi) {
-                                xi = xy[2 * (size_t)pi];
-                                yi = xy[2 * (size_t)pi + 1];
-                                if (has_prev) {
-                                    pxi = xy_prev[2 * (size_t)pi];
-                                    pyi = xy_prev[2 * (size_t)pi + 1];
-                                }
-                            }
-                            if (use32) {
-                                fxi = (float)(xi - nd.lx);
-                                fyi = (float)(yi - nd.ly);
-                            }
-                            jbeg = 0;
-                            jend = ni;
-                            jmin = vi ? 0u : 0xFFFFFFFFu;
-                            inside = false;
-                        }
-                        rows.p[lane] = pi;
-                        rows.pxy[lane] = make_double2(pxi, pyi);
-                        const JoinTile<TILE> &C = inside ? B : A;
-                        const unsigned long long flag = inside ? 0ull : kTagBorder;
-                        const bool keep_all = !inside || s.store_all;
-                        const JoinCounts jc =
-                            use32 ? join_columns<true>(st, s, C, xi, yi, fxi, fyi, pfxi, pfyi, jmin, jbeg, jend,
-                                                       s.t32_in, s.t32_out, t64, keep_all, rows, flag, has_prev)
-                                  : join_columns<false>(st, s, C, xi, yi, fxi, fyi, pfxi, pfyi, jmin, jbeg, jend,
-                                                        s.t32_in, s.t32_out, t64, keep_all, rows, flag, has_prev);
-                        n_in += inside ? jc.sure : 0u;
-                        n_bd += inside ? 0u : jc.sure;
-                        pass_in += inside ? jc.pass : 0u;
-                        pass_bd += inside ? 0u : jc.pass;
-                        band += jc.band;
-                    }
-                    __syncthreads();
+        const uint32_t nchunk = min((uint32_t)kInsideChunk, nl - w0);
+        if (threadIdx.x < nchunk) {
+            const uint32_t w = w0 + threadIdx.x;
+            const uint32_t leaf = s.leaf_list[w < nh ? s.node_cap - 1 - w : w - nh];
+            JoinLeaf L;
+            L.off = s.offset[leaf];
+            L.cnt = s.count[leaf];
+            L.uoff = s.ustart[leaf];
+            L.ucnt = s.ucount[leaf];
+            const double2 lo = reinterpret_cast<const double2 *>(s.nodes + leaf)[1];
+            const double2 hi = node_high(s.nodes, leaf);
+            L.lx = lo.x;
+            L.ly = lo.y;
+            L.use32 = s.precision == GLINT_PRECISION_FP32_GUARD && (hi.x - lo.x) <= s.w32_max &&
+                      (hi.y - lo.y) <= s.w32_max;
+            const uint32_t nri = (L.cnt + 63) / 64;
+            L.ntask_in = nri * (nri + 1) / 2;
+            L.ntask = L.ntask_in + (L.cnt ? (L.ucnt + 63) / 64 : 0u);
+            L.base = L.task0 = L.pad = 0;
+            lds.leaf[threadIdx.x] = L;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            // consecutive leaves that fit one tile form a group; a larger
+            // leaf is a group of its own
+            uint32_t ng = 0, sum = 0, tasks = 0;
+            lds.gfirst[0] = 0;
+            for (uint32_t k = 0; k < nchunk; k++) {
+                const uint32_t c = lds.leaf[k].cnt;
+                const bool open = k > lds.gfirst[ng];
+                if (open && (c > (uint32_t)TILE || sum + c > (uint32_t)TILE ||
+                             lds.leaf[lds.gfirst[ng]].cnt > (uint32_t)TILE)) {
+                    lds.gfirst[++ng] = k;
+                    sum = tasks = 0;
                 }
+                lds.leaf[k].base = sum;
+                lds.leaf[k].task0 = tasks;
+                sum += c;
+                tasks += lds.leaf[k].ntask;
+            }
+            lds.gfirst[++ng] = nchunk;
+            lds.ngroup = ng;
+        }
+        __syncthreads();
+        const uint32_t ng = lds.ngroup;
+        for (uint32_t g = 0; g < ng; g++) {
+            const uint32_t k0 = lds.gfirst[g], k1 = lds.gfirst[g + 1];
+            const JoinLeaf &F = lds.leaf[k0];
+            if (F.cnt > (uint32_t)TILE) {
+                // one leaf, tile pair by tile pair
+                const uint32_t ntile = (F.cnt + TILE - 1) / TILE;
+                for (uint32_t I = 0; I < ntile; I++) {
+                    const uint32_t ni = min((uint32_t)TILE, F.cnt - I * TILE);
+                    load_tile(lds.a, s, F.off + I * TILE, ni, has_prev, F.use32);
+                    for (uint32_t J = I; J < ntile; J++) {
+                        const uint32_t nj = min((uint32_t)TILE, F.cnt - J * TILE);
+                        const bool diag = (J == I);
+                        const JoinTile<TILE> &B = diag ? lds.a : lds.b;
+                        if (!diag) load_tile(lds.b, s, F.off + J * TILE, nj, has_prev, F.use32);
+                        __syncthreads();
+                        const uint32_t nri = (ni + 63) / 64, nrj = (nj + 63) / 64;
+                        const uint32_t ntask_in = diag ? nri * (nri + 1) / 2 : nri * nrj;
+                        // border tasks once per tile of the leaf: 64 units against the tile
+                        const uint32_t nuc = diag ? (F.ucnt + 63) / 64 : 0;
+                        for (uint32_t t = wave; t < ntask_in + nuc; t += kBlock / 64) {
+                            if (t < ntask_in) {
+                                uint32_t r, q;
+                                if (diag) {
+                                    triangle_task(t, nri, r, q);
+                                } else {
+                                    r = t / nrj;
+                                    q = t % nrj;
+                                }
+                                const uint32_t row = r * 64 + lane;
+                                const uint32_t jbeg = q * 64;
+                                run(lds.a, B, true, row, 0, row < ni, jbeg, min(jbeg + 64, nj), diag ? row + 1 : 0u,
+                                    F.use32, F.lx, F.ly);
+                            } else {
+                                const uint32_t u = (t - ntask_in) * 64 + lane;
+                                const bool vi = u < F.ucnt;
+                                run(lds.a, lds.a, false, 0, vi ? s.unit_pid[F.uoff + u] : 0u, vi, 0, ni, 0, F.use32,
+                                    F.lx, F.ly);
+                            }
+                        }
+                        __syncthreads();
+                    }
+                }
+            } else {
+                // leaves k0..k1 side by side in tile a, their tasks shared by all waves
+                const JoinLeaf &E = lds.leaf[k1 - 1];
+                const uint32_t total = E.base + E.cnt, ntask = E.task0 + E.ntask;
+                for (uint32_t i = threadIdx.x; i < total; i += kBlock) {
+                    uint32_t k = k0;
+                    while (k + 1 < k1 && lds.leaf[k + 1].base <= i) k++;
+                    const JoinLeaf &L = lds.leaf[k];
+                    const uint32_t from = L.off + (i - L.base);
+                    JoinTile<TILE> &T = lds.a;
+                    T.xy[i] = make_double2(s.sx[from], s.sy[from]);
+                    if (has_prev) T.pxy[i] = make_double2(s.sxp[from], s.syp[from]);
+                    T.f[i] = L.use32 ? make_float2(s.fx[from], s.fy[from]) : make_float2(0.f, 0.f);
+                    if (L.use32 && has_prev) T.pf[i] = make_float2(s.pfx[from], s.pfy[from]);
+                    T.p[i] = s.spid[from];
+                }
+                __syncthreads();
+                for (uint32_t t = wave; t < ntask; t += kBlock / 64) {
+                    uint32_t k = k0;
+                    while (k + 1 < k1 && lds.leaf[k + 1].task0 <= t) k++;
+                    const JoinLeaf &L = lds.leaf[k];
+                    const uint32_t lt = t - L.task0;
+                    if (lt < L.ntask_in) {
+                        uint32_t r, q;
+                        triangle_task(lt, (L.cnt + 63) / 64, r, q);
+                        const uint32_t row = L.base + r * 64 + lane;
+                        const uint32_t jbeg = L.base + q * 64;
+                        const uint32_t jend = L.base + min(q * 64 + 64, L.cnt);
+                        run(lds.a, lds.a, true, row, 0, r * 64 + lane < L.cnt, jbeg, jend,
+                            r == q ? row + 1 : 0u, L.use32, L.lx, L.ly);
+                    } else {
+                        const uint32_t u = (lt - L.ntask_in) * 64 + lane;
+                        const bool vi = u < L.ucnt;
+                        run(lds.a, lds.a, false, 0, vi ? s.unit_pid[L.uoff + u] : 0u, vi, L.base, L.base + L.cnt,
+                            0, L.use32, L.lx, L.ly);
+                    }
+                }
+                __syncthreads();
             }
         }
     }

@@ -71,7 +71,8 @@ struct Counters {
     uint32_t n_split;
     uint32_t n_merge;
     uint32_t n_border;
-    uint32_t n_leaf_list;
+    uint32_t n_leaf_list;   // join work list: light leaves from the front
+    uint32_t n_heavy;       // and leaves of more than kHeavyLeaf objects from the back
     uint32_t n_leaves;
     uint32_t max_leaf;
     uint32_t error;
