@@ -475,6 +475,10 @@ __global__ __launch_bounds__(kBlock) void k_assign(DevState s)
         if (active) {
             double x = xy[2 * (size_t)i], y = xy[2 * (size_t)i + 1];
             s.prev_leaf[i] = s.leaf_of[i];
+            if (!s.store_all) {
+                s.prev_cand[i] = s.cand[i];
+                s.cand[i] = make_uint4(kNone, kNone, kNone, 0u);
+            }
             if (!(x >= s.lx && x <= s.hx && y >= s.ly && y <= s.hy)) {
                 atomicOr(&s.ctr->error, kErrBadInput);
                 s.leaf_of[i] = kNone;
@@ -744,6 +748,8 @@ __global__ __launch_bounds__(kBlock) void k_lookup(DevState s)
                 }
             }
         }
+        if (k < nb && !s.store_all)
+            s.cand[pid] = make_uint4(nc > 0 ? cand[0] : kNone, nc > 1 ? cand[1] : kNone, nc > 2 ? cand[2] : kNone, nc);
         // one atomic per wave reserves the wave's units
         uint32_t wtotal;
         uint32_t wex = wave_exclusive_scan(nc, &wtotal);
@@ -886,6 +892,7 @@ struct JoinTile {
 struct JoinRows {
     double2 pxy[64];
     uint32_t p[64];
+    uint32_t pl[64];   // border rows: the object's leaf, bit 31 set when its candidates overflowed
 };
 
 
@@ -918,6 +925,7 @@ struct JoinCounts {
     uint32_t sure;   // sure contacts (wave-uniform)
     uint32_t pass;   // band entries within reach in FP64 (per lane)
     uint32_t band;   // band entries re-tested (per lane)
+    uint32_t dup;    // pairs found twice and not stored, counted once (per lane)
 };
 
 template <bool USE32, int TILE>
@@ -925,9 +933,10 @@ __device__ __forceinline__ JoinCounts join_columns(Stager &st, const DevState &s
                                                    double yi, float fxi, float fyi, float pfxi, float pfyi,
                                                    uint32_t jmin, uint32_t jbeg, uint32_t jend, float t32_in,
                                                    float t32_out, double t64, bool keep_all, const JoinRows &rows,
-                                                   unsigned long long flag, bool has_prev)
+                                                   unsigned long long flag, bool has_prev, const uint4 *ccand,
+                                                   uint32_t leaf)
 {
-    uint32_t n_sure = 0, n_pass = 0, band = 0;
+    uint32_t n_sure = 0, n_pass = 0, band = 0, n_dup = 0;
     const uint32_t tag = lane_id() << 16;
     // the stage's contacts as keys (kEmptyKey: a band entry that is not
     // within reach in FP64, or a contact that is not stored); `counted`
@@ -950,8 +959,19 @@ __device__ __forceinline__ JoinCounts join_columns(Stager &st, const DevState &s
             const double dy = a.y - b.y;
             if (dx * dx + dy * dy <= t64) k |= kTagPrev;
         }
-        // a contact neither tagged nor from a border unit is not stored
-        return (s.store_all || (k & (kTagPrev | kTagBorder))) ? k : kEmptyKey;
+        if (s.store_all || (k & kTagPrev)) return k;
+        if (!(k & kTagBorder)) return kEmptyKey;   // untagged inside contact: not stored
+        // Untagged border contact (row object p's lookup selected this leaf):
+        // not stored unless a candidate list overflowed.  It is found twice
+        // iff q's lookup selected p's leaf as well; that pair is counted as
+        // a second copy once, by the side whose row leaf is the smaller.
+        const uint4 qc = ccand[j];
+        const uint32_t pl = rows.pl[rl];
+        if ((pl >> 31) || qc.w > 3) return k;
+        const uint32_t lp = pl & 0x7FFFFFFFu;
+        const bool twice = qc.x == lp || qc.y == lp || qc.z == lp;
+        n_dup += counted && twice && lp < leaf;
+        return kEmptyKey;
     };
     const bool filt = !keep_all && has_prev;   // sure inside hits may be skipped
     for (uint32_t jb = jbeg; jb < jend; jb += kFlushRound) {
@@ -997,7 +1017,7 @@ __device__ __forceinline__ JoinCounts join_columns(Stager &st, const DevState &s
         if (st.n > kStage - 64 * kFlushRound) stage_flush(st, s, key_of);
     }
     stage_flush(st, s, key_of);
-    return JoinCounts{n_sure, n_pass, band};
+    return JoinCounts{n_sure, n_pass, band, n_dup};
 }
 
 // A leaf of the join's current chunk of work.
@@ -1009,7 +1029,7 @@ struct JoinLeaf {
     uint32_t ntask_in;        // inside tasks (upper triangle of 64-row blocks)
     uint32_t ntask;           // with the border tasks
     uint32_t use32;
-    uint32_t pad;
+    uint32_t node;            // the leaf
     double lx, ly;            // low corner (FP32 offsets of border units)
 };
 
@@ -1018,6 +1038,7 @@ struct JoinLds {
     JoinTile<TILE> a, b;
     JoinRows rows[kBlock / 64];
     uint32_t stage[kBlock / 64][kStage + 64];
+    uint4 ccand[TILE];                   // candidate leaves of the objects of tile a (see DevState::cand)
     JoinLeaf leaf[kInsideChunk];
     uint32_t gfirst[kInsideChunk + 1];   // groups of consecutive leaves of the chunk
     uint32_t ngroup;
@@ -1064,11 +1085,14 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
     JoinRows &rows = lds.rows[wave];
     uint32_t n_in = 0, n_bd = 0;                   // sure contacts (wave-uniform)
     uint32_t pass_in = 0, pass_bd = 0, band = 0;   // band entries within reach / re-tested (per lane)
+    uint32_t dups = 0;                             // unstored pairs found twice (per lane)
 
     // One task: an inside task (row slot `row` of tile A) or a border task
-    // (unit object `pid`), against columns [jbeg, jend) of tile C from jmin.
+    // (unit object `pid`, columns in tile a), against columns [jbeg, jend)
+    // of tile C from jmin; `leaf` is the leaf being joined.
     auto run = [&](const JoinTile<TILE> &A, const JoinTile<TILE> &C, bool inside, uint32_t row, uint32_t pid,
-                   bool vi, uint32_t jbeg, uint32_t jend, uint32_t jmin, bool use32, double lx, double ly) {
+                   bool vi, uint32_t jbeg, uint32_t jend, uint32_t jmin, bool use32, double lx, double ly,
+                   uint32_t leaf) {
         double xi = 0.0, yi = 0.0, pxi = 0.0, pyi = 0.0;
         float fxi = 0.f, fyi = 0.f, pfxi = 0.f, pfyi = 0.f;
         uint32_t pi = 0;
@@ -1109,18 +1133,21 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
         if (!vi) jmin = 0xFFFFFFFFu;   // an idle lane
         rows.p[lane] = pi;
         rows.pxy[lane] = make_double2(pxi, pyi);
+        if (!inside && !s.store_all && vi)
+            rows.pl[lane] = s.leaf_of[pi] | (s.cand[pi].w > 3 ? 0x80000000u : 0u);
         const unsigned long long flag = inside ? 0ull : kTagBorder;
         const bool keep_all = !inside || s.store_all;
         const JoinCounts jc =
             use32 ? join_columns<true>(st, s, C, xi, yi, fxi, fyi, pfxi, pfyi, jmin, jbeg, jend, s.t32_in, s.t32_out,
-                                       t64, keep_all, rows, flag, has_prev)
+                                       t64, keep_all, rows, flag, has_prev, lds.ccand, leaf)
                   : join_columns<false>(st, s, C, xi, yi, fxi, fyi, pfxi, pfyi, jmin, jbeg, jend, s.t32_in,
-                                        s.t32_out, t64, keep_all, rows, flag, has_prev);
+                                        s.t32_out, t64, keep_all, rows, flag, has_prev, lds.ccand, leaf);
         n_in += inside ? jc.sure : 0u;
         n_bd += inside ? 0u : jc.sure;
         pass_in += inside ? jc.pass : 0u;
         pass_bd += inside ? 0u : jc.pass;
         band += jc.band;
+        dups += jc.dup;
     };
 
     for (;;) {
@@ -1146,7 +1173,8 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
             const uint32_t nri = (L.cnt + 63) / 64;
             L.ntask_in = nri * (nri + 1) / 2;
             L.ntask = L.ntask_in + (L.cnt ? (L.ucnt + 63) / 64 : 0u);
-            L.base = L.task0 = L.pad = 0;
+            L.base = L.task0 = 0;
+            L.node = leaf;
             lds.leaf[threadIdx.x] = L;
         }
         __syncthreads();
@@ -1182,6 +1210,9 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
                 for (uint32_t I = 0; I < ntile; I++) {
                     const uint32_t ni = min((uint32_t)TILE, F.cnt - I * TILE);
                     load_tile(lds.a, s, F.off + I * TILE, ni, has_prev, F.use32);
+                    if (!s.store_all)
+                        for (uint32_t k = threadIdx.x; k < ni; k += kBlock)
+                            lds.ccand[k] = s.cand[s.spid[F.off + I * TILE + k]];
                     for (uint32_t J = I; J < ntile; J++) {
                         const uint32_t nj = min((uint32_t)TILE, F.cnt - J * TILE);
                         const bool diag = (J == I);
@@ -1204,12 +1235,12 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
                                 const uint32_t row = r * 64 + lane;
                                 const uint32_t jbeg = q * 64;
                                 run(lds.a, B, true, row, 0, row < ni, jbeg, min(jbeg + 64, nj), diag ? row + 1 : 0u,
-                                    F.use32, F.lx, F.ly);
+                                    F.use32, F.lx, F.ly, F.node);
                             } else {
                                 const uint32_t u = (t - ntask_in) * 64 + lane;
                                 const bool vi = u < F.ucnt;
                                 run(lds.a, lds.a, false, 0, vi ? s.unit_pid[F.uoff + u] : 0u, vi, 0, ni, 0, F.use32,
-                                    F.lx, F.ly);
+                                    F.lx, F.ly, F.node);
                             }
                         }
                         __syncthreads();
@@ -1229,7 +1260,9 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
                     if (has_prev) T.pxy[i] = make_double2(s.sxp[from], s.syp[from]);
                     T.f[i] = L.use32 ? make_float2(s.fx[from], s.fy[from]) : make_float2(0.f, 0.f);
                     if (L.use32 && has_prev) T.pf[i] = make_float2(s.pfx[from], s.pfy[from]);
-                    T.p[i] = s.spid[from];
+                    const uint32_t pid = s.spid[from];
+                    T.p[i] = pid;
+                    if (!s.store_all) lds.ccand[i] = s.cand[pid];
                 }
                 __syncthreads();
                 for (uint32_t t = wave; t < ntask; t += kBlock / 64) {
@@ -1244,12 +1277,12 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
                         const uint32_t jbeg = L.base + q * 64;
                         const uint32_t jend = L.base + min(q * 64 + 64, L.cnt);
                         run(lds.a, lds.a, true, row, 0, r * 64 + lane < L.cnt, jbeg, jend,
-                            r == q ? row + 1 : 0u, L.use32, L.lx, L.ly);
+                            r == q ? row + 1 : 0u, L.use32, L.lx, L.ly, L.node);
                     } else {
                         const uint32_t u = (lt - L.ntask_in) * 64 + lane;
                         const bool vi = u < L.ucnt;
                         run(lds.a, lds.a, false, 0, vi ? s.unit_pid[L.uoff + u] : 0u, vi, L.base, L.base + L.cnt,
-                            0, L.use32, L.lx, L.ly);
+                            0, L.use32, L.lx, L.ly, L.node);
                     }
                 }
                 __syncthreads();
@@ -1258,11 +1291,12 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
     }
     stage_close(st, s);
     // one atomic per wave for each counter
-    uint32_t pi_ = pass_in, pb = pass_bd, bd = band;
+    uint32_t pi_ = pass_in, pb = pass_bd, bd = band, du = dups;
     for (int d = 32; d >= 1; d >>= 1) {
         pi_ += __shfl_xor(pi_, d);
         pb += __shfl_xor(pb, d);
         bd += __shfl_xor(bd, d);
+        du += __shfl_xor(du, d);
     }
     if (lane == 0) {
         const uint32_t ci = n_in + pi_, cb = n_bd + pb;
@@ -1270,6 +1304,7 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
         if (cb) atomicAdd(&s.ctr->n_contacts_border, (unsigned long long)cb);
         if (bd) atomicAdd(&s.ctr->n_rechecks, (unsigned long long)bd);
         if (st.kept) atomicAdd(&s.ctr->n_stored, (unsigned long long)st.kept);
+        if (du) atomicAdd(&s.ctr->n_dup, (unsigned long long)du);
     }
 }
 
@@ -1642,9 +1677,12 @@ __device__ __forceinline__ void table_old(TableLds<KV, NT> &L, const DevState &s
 
 // Start (and, with the key-value output, box) of contact c, whose hash slot
 // is q (kNoSlot: a contact that was not hashed, new at this tick).  A tagged
-// contact with no entry in the previous generation either is new or was a
-// contact of the tick before that was not stored (found by the inside join
-// of its leaf, untagged); the objects' leaves at the tick before tell which.
+// contact (within reach at the tick before) with no entry in the previous
+// generation either is new or was a contact of the tick before that was not
+// stored: one found there untagged, by the inside join of a leaf holding
+// both objects or through a border unit of either object (when neither
+// object's candidate list overflowed).  The objects' leaves and candidate
+// leaves at the tick before tell which.
 template <bool KV, int NT>
 __device__ __forceinline__ void table_write(const TableLds<KV, NT> &L, const DevState &s, const TableGen &tcur,
                                             unsigned long long tagged, uint32_t c, uint32_t q, uint32_t tick,
@@ -1654,9 +1692,18 @@ __device__ __forceinline__ void table_write(const TableLds<KV, NT> &L, const Dev
     const uint32_t a = (uint32_t)(key >> 32), b = (uint32_t)key;
     uint32_t start = q == kNoSlot ? tick : L.hs[q];
     bool was_unstored = false;
-    if (start == tick && (tagged & kTagPrev) && !s.store_all && s.prev_leaf[a] == s.prev_leaf[b]) {
-        start = last;
-        was_unstored = true;
+    if (start == tick && (tagged & kTagPrev) && !s.store_all) {
+        const uint32_t la = s.prev_leaf[a], lb = s.prev_leaf[b];
+        bool tested = la == lb;
+        if (!tested) {
+            const uint4 ca = s.prev_cand[a], cb = s.prev_cand[b];
+            tested = ca.w <= 3 && cb.w <= 3 &&
+                     (ca.x == lb || ca.y == lb || ca.z == lb || cb.x == la || cb.y == la || cb.z == la);
+        }
+        if (tested) {
+            start = last;
+            was_unstored = true;
+        }
     }
     tcur.starts[c] = start;
     if constexpr (KV) {
@@ -2355,6 +2402,10 @@ glint_status glint_create(const glint_config *cfg, glint_ctx **out)
     ALLOC(s.leaf_list, node_cap);
     ALLOC(s.leaf_of, N);
     ALLOC(s.prev_leaf, N);
+    if (!s.store_all) {
+        ALLOC(s.cand, N);
+        ALLOC(s.prev_cand, N);
+    }
     ALLOC(s.border, N);
     ALLOC(s.units, s.unit_cap);
     ALLOC(s.urank, s.unit_cap);

@@ -171,6 +171,8 @@ struct DevState {
     uint32_t *leaf_list;
     uint32_t *leaf_of;
     uint32_t *prev_leaf;    // leaf of every object at the tick before (pid order)
+    uint4 *cand;            // candidate leaves of every object's lookup at this tick: up to 3, and their count
+    uint4 *prev_cand;       // the same at the tick before
     uint2 *border;
     uint2 *units;           // (object, candidate leaf) in lookup order
     uint32_t *urank;        // rank of the unit among the units of its leaf
