@@ -6,8 +6,8 @@ reference's candidate rule makes part of the result):
   join_tile      points per LDS tile of the distance join (256, 512)
   table_threads  workgroup size of the table update (256, 512, 1024)
   load_factor    table slots = next_pow2(distinct pairs per tick / load_factor); only the
-                 stored pairs (about a third on dense workloads) occupy slots, so values
-                 above 1 are valid
+                 stored pairs (those within reach at the tick before, about an eighth on
+                 dense workloads) occupy slots, so values above 1 are valid
   precision      FP64 test, or FP32 guard band with FP64 recheck (both exact)
   grid_capacity  leaf capacity (64..1024); tuned only when the caller allows
                  it, and results are then compared with the oracle run at the
@@ -29,7 +29,7 @@ from . import gen, runtime
 TUNED_PATH = Path(__file__).resolve().parent / "tuned.json"
 
 DEFAULTS = {"join_tile": 512, "table_threads": 512, "load_factor": 0.75, "precision": "fp32guard"}
-SPACE = {"join_tile": [256, 512], "table_threads": [256, 512, 1024], "load_factor": [0.375, 0.75, 1.5],
+SPACE = {"join_tile": [256, 512], "table_threads": [256, 512, 1024], "load_factor": [0.375, 0.75, 1.5, 3.0, 6.0],
          "precision": ["fp64", "fp32guard"]}
 LEAF_CAPACITIES = [64, 128, 256, 512, 1024]
 

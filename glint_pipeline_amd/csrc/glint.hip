@@ -881,7 +881,7 @@ __device__ __forceinline__ unsigned long long pair_key(uint32_t a, uint32_t b)
 template <int TILE>
 struct JoinTile {
     double2 xy[TILE];              // positions at tick
-    double2 pxy[TILE];             // positions at prev_tick
+    uint32_t g[TILE];              // index in the sorted arrays (positions at prev_tick: sxp, syp)
     float2 f[TILE];                // FP32 offsets from the leaf's low corner (guard mode)
     float2 pf[TILE];               // the same for prev_tick (NaN: not bounded)
     uint32_t p[TILE];              // object ids
@@ -902,7 +902,7 @@ __device__ __forceinline__ void load_tile(JoinTile<TILE> &t, const DevState &s, 
 {
     for (uint32_t k = threadIdx.x; k < n; k += kBlock) {
         t.xy[k] = make_double2(s.sx[from + k], s.sy[from + k]);
-        if (has_prev) t.pxy[k] = make_double2(s.sxp[from + k], s.syp[from + k]);
+        t.g[k] = from + k;
         t.f[k] = use32 ? make_float2(s.fx[from + k], s.fy[from + k]) : make_float2(0.f, 0.f);
         if (use32 && has_prev) t.pf[k] = make_float2(s.pfx[from + k], s.pfy[from + k]);
         t.p[k] = s.spid[from + k];
@@ -954,9 +954,10 @@ __device__ __forceinline__ JoinCounts join_columns(Stager &st, const DevState &s
         }
         unsigned long long k = pair_key(rows.p[rl], C.p[j]) | flag;
         if (has_prev) {
-            const double2 a = rows.pxy[rl], b = C.pxy[j];
-            const double dx = a.x - b.x;
-            const double dy = a.y - b.y;
+            const double2 a = rows.pxy[rl];
+            const uint32_t gj = C.g[j];
+            const double dx = a.x - s.sxp[gj];
+            const double dy = a.y - s.syp[gj];
             if (dx * dx + dy * dy <= t64) k |= kTagPrev;
         }
         if (s.store_all || (k & kTagPrev)) return k;
@@ -1103,9 +1104,8 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
             xi = a.x;
             yi = a.y;
             if (has_prev) {
-                const double2 ap = A.pxy[rr];
-                pxi = ap.x;
-                pyi = ap.y;
+                pxi = s.sxp[A.g[rr]];
+                pyi = s.syp[A.g[rr]];
             }
             fxi = af.x;
             fyi = af.y;
@@ -1257,7 +1257,7 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
                     const uint32_t from = L.off + (i - L.base);
                     JoinTile<TILE> &T = lds.a;
                     T.xy[i] = make_double2(s.sx[from], s.sy[from]);
-                    if (has_prev) T.pxy[i] = make_double2(s.sxp[from], s.syp[from]);
+                    T.g[i] = from;
                     T.f[i] = L.use32 ? make_float2(s.fx[from], s.fy[from]) : make_float2(0.f, 0.f);
                     if (L.use32 && has_prev) T.pf[i] = make_float2(s.pfx[from], s.pfy[from]);
                     const uint32_t pid = s.spid[from];
