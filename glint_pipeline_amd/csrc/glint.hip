@@ -77,6 +77,16 @@ __device__ __forceinline__ bool point_within64(double x1, double y1, double x2, 
     return s <= t64;
 }
 
+// Workgroup barrier for LDS data only: global loads and stores stay in
+// flight across it (__syncthreads also waits for every outstanding global
+// access, which drains loads issued ahead for the next step).
+__device__ __forceinline__ void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // Orders LDS accesses of the lanes of one wave (staging buffers).
 __device__ __forceinline__ void wave_sync()
 {
@@ -1766,7 +1776,7 @@ __global__ __launch_bounds__(NT) void k_table(DevState s)
         const uint32_t c0 = c0n, c1 = c1n;
 #pragma unroll
         for (int q = 0; q < kPer; q++) nk[q] = nkn[q];
-        __syncthreads();
+        lds_barrier();
         // the tick's contacts that can continue a run or be a second copy
 #pragma unroll
         for (int q = 0; q < kPer; q++) {
@@ -1782,7 +1792,7 @@ __global__ __launch_bounds__(NT) void k_table(DevState s)
             const unsigned long long k = tcur.keys[c];
             if (k & (kTagPrev | kTagBorder)) dups += table_put(L, s, tcur, k & kKeyBits, c, tick) == kNoSlot;
         }
-        __syncthreads();
+        lds_barrier();
         // the previous generation's entries of the region
 #pragma unroll
         for (int q = 0; q < kPer; q++) {
@@ -1796,7 +1806,7 @@ __global__ __launch_bounds__(NT) void k_table(DevState s)
         }
         const uint32_t rn = r + gridDim.x;
         if (rn < nreg) prefetch(rn);
-        __syncthreads();
+        lds_barrier();
         // starts (and boxes) of every contact of the region
 #pragma unroll
         for (int q = 0; q < kPer; q++) {
@@ -1812,7 +1822,7 @@ __global__ __launch_bounds__(NT) void k_table(DevState s)
             }
             table_write(L, s, tcur, k, c, q, tick, last);
         }
-        __syncthreads();
+        lds_barrier();
     }
     if (nst) table_flush_stage<KV, NT>(s, st, nst);
     // distinct pairs of the tick: contacts less second copies
