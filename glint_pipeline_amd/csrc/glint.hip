@@ -38,7 +38,7 @@ constexpr int kBlock = 256;
 constexpr int kScanItems = 4;                       // per thread
 constexpr int kScanTile = kBlock * kScanItems;      // elements per scan block
 constexpr int kTile = 512;                          // points per LDS tile of the join (default)
-constexpr int kStage = 384;                         // staged contacts per wave
+constexpr int kStage = 368;                         // staged contacts per wave
 constexpr int kFlushRound = 4;                      // join columns between stage checks
 constexpr int kMeetStage = 128;                     // staged meetings per wave (table update)
 constexpr uint32_t kChunk = 4096;                   // contact-list entries reserved per atomic
@@ -481,10 +481,11 @@ __global__ __launch_bounds__(kBlock) void k_assign(DevState s)
             if (i >= s.n) i = k;   // not a permutation: only after an earlier error
         }
         bool border = false;
-        uint32_t lv = 0, cur = kNone;
+        uint32_t lv = 0, cur = kNone, pleaf = kNone;
         if (active) {
             double x = xy[2 * (size_t)i], y = xy[2 * (size_t)i + 1];
-            s.prev_leaf[i] = s.leaf_of[i];
+            pleaf = s.leaf_of[i];
+            s.prev_leaf[i] = pleaf;
             if (!s.store_all) {
                 s.prev_cand[i] = s.cand[i];
                 s.cand[i] = make_uint4(kNone, kNone, kNone, 0u);
@@ -531,6 +532,7 @@ __global__ __launch_bounds__(kBlock) void k_assign(DevState s)
             s.order[k] = i;
             s.ord_leaf[k] = active ? cur : kNone;
             s.ord_rank[k] = rank;
+            s.ord_pleaf[k] = pleaf;
         }
         // border objects are staged per wave and appended one flush at a time
         unsigned long long m = __ballot(border);
@@ -687,6 +689,7 @@ __global__ __launch_bounds__(kBlock) void k_scatter(DevState s)
         s.sx[pos] = x;
         s.sy[pos] = y;
         s.spid[pos] = i;
+        s.splf[pos] = s.ord_pleaf[k];
         if (has_prev) {
             s.sxp[pos] = xy_prev[2 * (size_t)i];
             s.syp[pos] = xy_prev[2 * (size_t)i + 1];
@@ -892,6 +895,7 @@ template <int TILE>
 struct JoinTile {
     double2 xy[TILE];              // positions at tick
     uint32_t g[TILE];              // index in the sorted arrays (positions at prev_tick: sxp, syp)
+    uint32_t pl[TILE];             // leaf at prev_tick
     float2 f[TILE];                // FP32 offsets from the leaf's low corner (guard mode)
     float2 pf[TILE];               // the same for prev_tick (NaN: not bounded)
     uint32_t p[TILE];              // object ids
@@ -902,7 +906,8 @@ struct JoinTile {
 struct JoinRows {
     double2 pxy[64];
     uint32_t p[64];
-    uint32_t pl[64];   // border rows: the object's leaf, bit 31 set when its candidates overflowed
+    uint32_t pl[64];    // border rows: the object's leaf, bit 31 set when its candidates overflowed
+    uint32_t ppl[64];   // the object's leaf at prev_tick
 };
 
 
@@ -913,6 +918,7 @@ __device__ __forceinline__ void load_tile(JoinTile<TILE> &t, const DevState &s, 
     for (uint32_t k = threadIdx.x; k < n; k += kBlock) {
         t.xy[k] = make_double2(s.sx[from + k], s.sy[from + k]);
         t.g[k] = from + k;
+        t.pl[k] = s.splf[from + k];
         t.f[k] = use32 ? make_float2(s.fx[from + k], s.fy[from + k]) : make_float2(0.f, 0.f);
         if (use32 && has_prev) t.pf[k] = make_float2(s.pfx[from + k], s.pfy[from + k]);
         t.p[k] = s.spid[from + k];
@@ -969,6 +975,19 @@ __device__ __forceinline__ JoinCounts join_columns(Stager &st, const DevState &s
             const double dx = a.x - s.sxp[gj];
             const double dy = a.y - s.syp[gj];
             if (dx * dx + dy * dy <= t64) k |= kTagPrev;
+        }
+        if ((k & kTagPrev) && !s.store_all) {
+            // tested by the join of the tick before: both objects in one leaf
+            // then, or one's lookup selected the other's leaf (candidate
+            // lists that overflowed stored the pair, so they do not count)
+            const uint32_t la = rows.ppl[rl], lb = C.pl[j];
+            bool tested = la == lb;
+            if (!tested) {
+                const uint4 ca = s.prev_cand[rows.p[rl]], cb = s.prev_cand[C.p[j]];
+                tested = ca.w <= 3 && cb.w <= 3 &&
+                         (ca.x == lb || ca.y == lb || ca.z == lb || cb.x == la || cb.y == la || cb.z == la);
+            }
+            if (tested) k |= kTagTested;
         }
         if (s.store_all || (k & kTagPrev)) return k;
         if (!(k & kTagBorder)) return kEmptyKey;   // untagged inside contact: not stored
@@ -1143,6 +1162,8 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
         if (!vi) jmin = 0xFFFFFFFFu;   // an idle lane
         rows.p[lane] = pi;
         rows.pxy[lane] = make_double2(pxi, pyi);
+        if (has_prev && !s.store_all)
+            rows.ppl[lane] = inside ? A.pl[vi ? row : jbeg] : (vi ? s.prev_leaf[pi] : kNone);
         if (!inside && !s.store_all && vi)
             rows.pl[lane] = s.leaf_of[pi] | (s.cand[pi].w > 3 ? 0x80000000u : 0u);
         const unsigned long long flag = inside ? 0ull : kTagBorder;
@@ -1268,6 +1289,7 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
                     JoinTile<TILE> &T = lds.a;
                     T.xy[i] = make_double2(s.sx[from], s.sy[from]);
                     T.g[i] = from;
+                    T.pl[i] = s.splf[from];
                     T.f[i] = L.use32 ? make_float2(s.fx[from], s.fy[from]) : make_float2(0.f, 0.f);
                     if (L.use32 && has_prev) T.pf[i] = make_float2(s.pfx[from], s.pfy[from]);
                     const uint32_t pid = s.spid[from];
@@ -1689,10 +1711,8 @@ __device__ __forceinline__ void table_old(TableLds<KV, NT> &L, const DevState &s
 // is q (kNoSlot: a contact that was not hashed, new at this tick).  A tagged
 // contact (within reach at the tick before) with no entry in the previous
 // generation either is new or was a contact of the tick before that was not
-// stored: one found there untagged, by the inside join of a leaf holding
-// both objects or through a border unit of either object (when neither
-// object's candidate list overflowed).  The objects' leaves and candidate
-// leaves at the tick before tell which.
+// stored; the join set kTagTested when the join of the tick before tested
+// the pair, which tells which.
 template <bool KV, int NT>
 __device__ __forceinline__ void table_write(const TableLds<KV, NT> &L, const DevState &s, const TableGen &tcur,
                                             unsigned long long tagged, uint32_t c, uint32_t q, uint32_t tick,
@@ -1702,18 +1722,9 @@ __device__ __forceinline__ void table_write(const TableLds<KV, NT> &L, const Dev
     const uint32_t a = (uint32_t)(key >> 32), b = (uint32_t)key;
     uint32_t start = q == kNoSlot ? tick : L.hs[q];
     bool was_unstored = false;
-    if (start == tick && (tagged & kTagPrev) && !s.store_all) {
-        const uint32_t la = s.prev_leaf[a], lb = s.prev_leaf[b];
-        bool tested = la == lb;
-        if (!tested) {
-            const uint4 ca = s.prev_cand[a], cb = s.prev_cand[b];
-            tested = ca.w <= 3 && cb.w <= 3 &&
-                     (ca.x == lb || ca.y == lb || ca.z == lb || cb.x == la || cb.y == la || cb.z == la);
-        }
-        if (tested) {
-            start = last;
-            was_unstored = true;
-        }
+    if (start == tick && (tagged & kTagPrev) && (tagged & kTagTested) && !s.store_all) {
+        start = last;
+        was_unstored = true;
     }
     tcur.starts[c] = start;
     if constexpr (KV) {
@@ -2432,6 +2443,8 @@ glint_status glint_create(const glint_config *cfg, glint_ctx **out)
     ALLOC(s.order, N);
     ALLOC(s.ord_leaf, N);
     ALLOC(s.ord_rank, N);
+    ALLOC(s.ord_pleaf, N);
+    ALLOC(s.splf, N);
     if (cfg->precision == GLINT_PRECISION_FP32_GUARD) {
         ALLOC(s.fx, N);
         ALLOC(s.fy, N);

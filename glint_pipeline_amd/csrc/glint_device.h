@@ -25,10 +25,11 @@ constexpr uint32_t kLeafTag = 0xFFFFFFFFu;      // child[0] of a leaf
 constexpr uint32_t kNone = 0xFFFFFFFFu;
 constexpr unsigned long long kEmptyKey = ~0ull; // (a<<32)|b with a<b is never all ones
 constexpr uint32_t kDupStart = 0xFFFFFFFFu;     // start of the second copy of a pair found twice
-// Flags carried in the high bits of a contact's key (pids are below 2^30):
+// Flags carried in bits of a contact's key that pids (below 2^30) leave free:
 constexpr unsigned long long kTagPrev = 1ull << 63;     // the pair was within reach at the tick before
 constexpr unsigned long long kTagBorder = 1ull << 62;   // found through a border unit (may be found twice)
-constexpr unsigned long long kKeyBits = kTagBorder - 1;
+constexpr unsigned long long kTagTested = 1ull << 31;   // with kTagPrev: the join of the tick before tested it
+constexpr unsigned long long kKeyBits = 0x3FFFFFFF3FFFFFFFull;   // the pair itself
 
 enum NodeType : int32_t { kInvalid = 0, kBranch = 1, kLeaf = 2 };
 
@@ -187,6 +188,8 @@ struct DevState {
     uint32_t *order;        // object handled by position k of the tick's assignment (spid of the tick before)
     uint32_t *ord_leaf;     // its leaf and rank in that leaf, by k
     uint32_t *ord_rank;
+    uint32_t *ord_pleaf;    // its leaf at the tick before, by k
+    uint32_t *splf;         // leaf at the tick before, sorted like sx/sy
     float *fx;
     float *fy;
     float *pfx;             // positions at the tick before as FP32 offsets from the same corner

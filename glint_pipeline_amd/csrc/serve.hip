@@ -43,7 +43,7 @@ constexpr uint32_t kB = 256;
 constexpr uint64_t kAlign = 256;
 constexpr unsigned long long kEmpty = ~0ull;
 constexpr uint32_t kDup = 0xFFFFFFFFu;      // start of a second copy of a pair in a generation
-constexpr unsigned long long kPairBits = (1ull << 62) - 1;   // a list key without its two flag bits
+constexpr unsigned long long kPairBits = 0x3FFFFFFF3FFFFFFFull;   // a list key without its flag bits
 
 struct QCounters {
     unsigned long long hits1, hits2;
