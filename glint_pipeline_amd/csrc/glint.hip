@@ -904,7 +904,7 @@ struct JoinTile {
 // The row objects of a wave's current task (read when its contacts are
 // written out).
 struct JoinRows {
-    double2 pxy[64];
+    float2 pf[64];      // FP32 offsets at prev_tick (guard mode; NaN: unbounded)
     uint32_t p[64];
     uint32_t pl[64];    // border rows: the object's leaf, bit 31 set when its candidates overflowed
     uint32_t ppl[64];   // the object's leaf at prev_tick
@@ -970,11 +970,24 @@ __device__ __forceinline__ JoinCounts join_columns(Stager &st, const DevState &s
         }
         unsigned long long k = pair_key(rows.p[rl], C.p[j]) | flag;
         if (has_prev) {
-            const double2 a = rows.pxy[rl];
-            const uint32_t gj = C.g[j];
-            const double dx = a.x - s.sxp[gj];
-            const double dy = a.y - s.syp[gj];
-            if (dx * dx + dy * dy <= t64) k |= kTagPrev;
+            // within reach at prev_tick: decided on the FP32 offsets outside
+            // the guard band, else by the FP64 test
+            int dec = -1;
+            if (USE32) {
+                const float2 a = rows.pf[rl], b = C.pf[j];
+                const float px = a.x - b.x;
+                const float py = a.y - b.y;
+                const float p32 = px * px + py * py;
+                dec = p32 <= t32_in ? 1 : (p32 > t32_out ? 0 : -1);   // NaN: -1
+            }
+            if (dec < 0) {
+                const uint32_t pa = rows.p[rl], gj = C.g[j];
+                const double *xp = s.tp->xy_prev;
+                const double dx = xp[2 * (size_t)pa] - s.sxp[gj];
+                const double dy = xp[2 * (size_t)pa + 1] - s.syp[gj];
+                dec = dx * dx + dy * dy <= t64;
+            }
+            if (dec) k |= kTagPrev;
         }
         if ((k & kTagPrev) && !s.store_all) {
             // tested by the join of the tick before: both objects in one leaf
@@ -1157,11 +1170,16 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
             if (use32) {
                 fxi = (float)(xi - lx);
                 fyi = (float)(yi - ly);
+                if (has_prev) {
+                    const double ox = pxi - lx, oy = pyi - ly;
+                    pfxi = fabs(ox) <= s.pf_lim ? (float)ox : __builtin_nanf("");
+                    pfyi = fabs(oy) <= s.pf_lim ? (float)oy : __builtin_nanf("");
+                }
             }
         }
         if (!vi) jmin = 0xFFFFFFFFu;   // an idle lane
         rows.p[lane] = pi;
-        rows.pxy[lane] = make_double2(pxi, pyi);
+        rows.pf[lane] = make_float2(pfxi, pfyi);
         if (has_prev && !s.store_all)
             rows.ppl[lane] = inside ? A.pl[vi ? row : jbeg] : (vi ? s.prev_leaf[pi] : kNone);
         if (!inside && !s.store_all && vi)
