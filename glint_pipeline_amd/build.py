@@ -20,7 +20,7 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 INCLUDE = ROOT / "include"
 
-HIP_SOURCES = ["glint.hip", "walk.hip", "kv.hip", "serve.hip"]
+HIP_SOURCES = ["glint.hip", "walk.hip", "kv.hip", "serve.hip", "trace_io.hip"]
 HIP_HEADERS = ["glint_device.h", "kv_engine.h", "ctx_view.h"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -133,6 +133,16 @@ KV_KEYS, KV_BOXES, KV_OVERSIZE_KEYS, KV_OVERSIZE_BOXES, KV_SIDS, KV_CTFS, KV_BIT
     KV_PACKED = range(9)
 
 # (name, restype, argtypes) of every entry point of include/glint.h
+class TrHeader(ctypes.Structure):
+    _fields_ = [("num_objects", ctypes.c_uint32), ("duration", ctypes.c_uint32), ("low_x", ctypes.c_double),
+                ("low_y", ctypes.c_double), ("high_x", ctypes.c_double), ("high_y", ctypes.c_double)]
+
+
+class Projection(ctypes.Structure):
+    _fields_ = [("lon0", ctypes.c_double), ("lat0", ctypes.c_double), ("deg_per_m_lon", ctypes.c_double),
+                ("deg_per_m_lat", ctypes.c_double)]
+
+
 _VP = ctypes.c_void_p
 _U32, _U64, _I = ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int
 SIGNATURES = {
@@ -173,6 +183,11 @@ SIGNATURES = {
     "glint_batch_destroy": (_I, [_VP]),
     "glint_batch_context": (_I, [_VP, _U32, ctypes.POINTER(_VP)]),
     "glint_batch_last_error": (ctypes.c_char_p, [_VP]),
+    "glint_tr_header_read": (_I, [ctypes.c_char_p, ctypes.POINTER(TrHeader)]),
+    "glint_tr_read": (_I, [ctypes.c_char_p, _U32, _U32, _U32, _VP]),
+    "glint_tr_read_chunk": (_I, [ctypes.c_char_p, _U32, _U32, _VP]),
+    "glint_projection_for_box": (_I, [ctypes.c_double, ctypes.c_double, ctypes.POINTER(Projection)]),
+    "glint_project_device": (_I, [ctypes.POINTER(Projection), _VP, _VP, _U64, _U32]),
     "glint_batch_run_ticks": (_I, [_VP, _VP, _U32, _U32, _VP]),
     "glint_meeting_log_info": (_I, [_VP, ctypes.POINTER(_U64), ctypes.POINTER(_U64)]),
     "glint_meeting_log_read": (_I, [_VP, _U64, _VP, _U64, ctypes.POINTER(_U64)]),
@@ -226,6 +241,39 @@ def _check(st: int, where: str, ctx=None) -> None:
     if st != GLINT_OK:
         detail = lib().glint_last_error_message(ctx)
         raise GlintError(st, where, detail.decode() if detail else "")
+
+
+def tr_header(path) -> dict:
+    """Header of a reference trace file (objects, ticks, box)."""
+    h = TrHeader()
+    _check(lib().glint_tr_header_read(str(path).encode(), ctypes.byref(h)), "glint_tr_header_read")
+    return {f: getattr(h, f) for f, _ in TrHeader._fields_}
+
+
+def tr_read(path, num_objects: int, st: int, duration: int) -> np.ndarray:
+    """Ticks st..st+duration-1 of a reference trace file as (duration, N, 2)
+    FP64 points, loaded as the reference loads them."""
+    out = np.zeros((duration, num_objects, 2), dtype=np.float64)
+    _check(lib().glint_tr_read(str(path).encode(), num_objects, st, duration, out.ctypes.data), "glint_tr_read")
+    return out
+
+
+def tr_read_chunk(prefix, st: int, num_objects: int) -> np.ndarray:
+    """The 100 ticks of a reference chunk file prefix + st + "_" + N + ".tr"."""
+    out = np.zeros((100, num_objects, 2), dtype=np.float64)
+    _check(lib().glint_tr_read_chunk(str(prefix).encode(), st, num_objects, out.ctypes.data), "glint_tr_read_chunk")
+    return out
+
+
+def projection_for_box(lon_lo: float, lat_lo: float) -> Projection:
+    p = Projection()
+    _check(lib().glint_projection_for_box(lon_lo, lat_lo, ctypes.byref(p)), "glint_projection_for_box")
+    return p
+
+
+def project_device(proj: Projection, d_lonlat: int, d_xy: int, n: int, device: int = 0) -> None:
+    """lon/lat degrees (FP64 AoS, device memory) to planar metres on the GPU."""
+    _check(lib().glint_project_device(ctypes.byref(proj), d_lonlat, d_xy, n, device), "glint_project_device")
 
 
 class DeviceBuffer:

@@ -428,6 +428,49 @@ glint_status glint_gen_walk_device(const glint_gen_config *cfg, glint_walkers *d
 glint_status glint_gen_walk_updates_device(const glint_gen_config *cfg, glint_walkers *d_walkers, uint32_t tick,
                                            uint32_t *d_pids, double *d_xy, uint32_t *d_count, uint32_t device);
 
+/* ------------------------------------------------------------------ reference traces
+ * The reference's trace files hold FP64 (lon, lat) points tick-major, one
+ * Point (16 B) per object and tick:
+ *  - a chunk file "<prefix><st>_<N>.tr" holds ticks st..st+99 of N objects
+ *    and nothing else (datagen.cpp:18-26, read by process.cpp:128-144);
+ *  - a trace file starts with the number of objects (u32), the number of
+ *    ticks (u32) and the bounding box (4 doubles: low x, low y, high x,
+ *    high y), then every tick's points (process.cpp:146-187). */
+typedef struct glint_tr_header {
+    uint32_t num_objects;
+    uint32_t duration;
+    double low_x, low_y, high_x, high_y;
+} glint_tr_header;
+
+/* Header of a trace file. */
+glint_status glint_tr_header_read(const char *path, glint_tr_header *out);
+/* Ticks st..st+duration-1 of num_objects objects of a trace file into out
+ * (duration*num_objects points, tick-major), as the reference loads them
+ * (process.cpp:146-187): with more objects than the file holds, the next
+ * num_objects-loaded objects are read from the file's later ticks.  Errors:
+ * the file cannot be opened or is short (GLINT_ERR_BAD_INPUT), or the range
+ * exceeds the file (GLINT_ERR_INVALID_ARGUMENT; the reference asserts). */
+glint_status glint_tr_read(const char *path, uint32_t num_objects, uint32_t st, uint32_t duration, double *out);
+/* The 100 ticks of a chunk file: prefix + st + "_" + num_objects + ".tr"
+ * into out (100*num_objects points). */
+glint_status glint_tr_read_chunk(const char *prefix, uint32_t st, uint32_t num_objects, double *out);
+
+/* Planar metres from lon/lat degrees with the reference's conversion
+ * (util.h:38-39,57-66): metres = degrees / degree_per_meter, the longitude
+ * factor fixed at the box's low latitude, as the filter's x_buffer is
+ * (filter.cpp:30-31); origin at the box's low corner. */
+typedef struct glint_projection {
+    double lon0, lat0;        /* degrees mapped to (0, 0) */
+    double deg_per_m_lon;     /* 360 / (sin((90 - |lat0|) * PI / 180) * 40076 * 1000), PI = 3.14159265 */
+    double deg_per_m_lat;     /* 360 / (40076 * 1000) */
+} glint_projection;
+glint_status glint_projection_for_box(double lon_lo, double lat_lo, glint_projection *out);
+/* x = (lon - lon0) / deg_per_m_lon, y = (lat - lat0) / deg_per_m_lat for n
+ * points (FP64 AoS) in device memory, on the GPU; d_xy may equal
+ * d_lonlat.  Synchronous. */
+glint_status glint_project_device(const glint_projection *proj, const double *d_lonlat, double *d_xy, uint64_t n,
+                                  uint32_t device);
+
 #ifdef __cplusplus
 }
 #endif
