@@ -104,14 +104,16 @@ def profiled_traffic(kernel: str, workload: str, precision: int, objects: int) -
     return None
 
 
-def cpu_baseline(w, ticks: int, threads: int, min_seconds: float = 10.0, max_ticks: int = 5000) -> dict:
+def cpu_baseline(w, ticks: int, threads: int, min_seconds: float = 10.0, max_ticks: int = 5000,
+                 schema: str = "dynamic") -> dict:
     """The CPU oracle (C + OpenMP) on the first ticks of the same workload:
     at least `ticks` ticks, and more (the walk continues) until about
     `min_seconds` of CPU work is timed."""
-    from glint_pipeline_amd import gen
+    from glint_pipeline_amd import autotune, gen
     from oracle.oracle import Oracle
     g = gen.Generator(w)
-    o = Oracle(w.num_objects, w.box(), w.reach_distance, w.grid_capacity, w.min_meet_time, num_threads=threads)
+    o = Oracle(w.num_objects, autotune.box_for(w, schema), w.reach_distance, w.grid_capacity, w.min_meet_time,
+               num_threads=threads, dynamic_schema=autotune.SCHEMAS[schema])
     t_build0 = time.perf_counter()
     o.tick(g.positions(), 0)          # tick 0 builds the tree (not timed, as on the GPU side)
     t_build = time.perf_counter() - t_build0
@@ -162,6 +164,9 @@ def main() -> int:
     ap.add_argument("--table-threads", type=int, default=None, choices=[256, 512, 1024],
                     help="workgroup size of the table update (default: tuned or 256)")
     ap.add_argument("--load-factor", type=float, default=None)
+    ap.add_argument("--schema", default=None, choices=["dynamic", "grid", "static"],
+                    help="tree maintenance: split/merge (default), rebuilt from the 16384^2 grid every tick, "
+                         "or the first tick's tree kept")
     ap.add_argument("--autotune", action="store_true", help="tune leaf capacity, join tile, load factor and "
                                                              "precision on this workload before timing")
     ap.add_argument("--save-tuned", action="store_true", help="store the autotuned parameters in tuned.json")
@@ -226,7 +231,7 @@ def main() -> int:
                                     "workload": desc, "trials": res.trials}
             autotune.save_table(table)
     for k, v in (("precision", args.precision), ("grid_capacity", args.grid_capacity), ("join_tile", args.join_tile),
-                 ("load_factor", args.load_factor), ("table_threads", args.table_threads)):
+                 ("load_factor", args.load_factor), ("table_threads", args.table_threads), ("schema", args.schema)):
         if v is not None:
             params[k] = v
     precision = runtime.PRECISION_FP32_GUARD if params["precision"] == "fp32guard" else runtime.PRECISION_FP64
@@ -370,7 +375,7 @@ def main() -> int:
     if not args.no_cpu_baseline:
         threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
         t0 = time.perf_counter()
-        cb = cpu_baseline(w, args.cpu_ticks, threads)
+        cb = cpu_baseline(w, args.cpu_ticks, threads, schema=params.get("schema", "dynamic"))
         log(f"[bench] cpu baseline {cb} (wall {time.perf_counter() - t0:.1f} s)")
         cpu = {"value": round(N * cb["ticks"] / cb["seconds"], 1), "unit": "object-updates/s", "cores": threads,
                "kind": "port",

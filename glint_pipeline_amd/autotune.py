@@ -43,12 +43,25 @@ def table_slots(distinct: float, load_factor: float) -> int:
     return 1 << max(11, math.ceil(math.log2(max(distinct, 1.0) / load_factor)))
 
 
+SCHEMAS = {"static": runtime.SCHEMA_STATIC, "dynamic": runtime.SCHEMA_DYNAMIC, "grid": runtime.SCHEMA_GRID_REBUILD}
+
+
+def box_for(w: gen.Workload, schema: str = "dynamic") -> tuple:
+    """The root box: the workload's, or one metre wider on every side for the
+    grid rebuild, whose 16384^2 cell grid has no cell on the high y edge that
+    the walkers reach."""
+    lx, ly, hx, hy = w.box()
+    return (lx - 1.0, ly - 1.0, hx + 1.0, hy + 1.0) if schema == "grid" else (lx, ly, hx, hy)
+
+
 def config_for(w: gen.Workload, params: dict, distinct: float, graph_ticks: int = 8, **extra) -> runtime.Config:
     prec = runtime.PRECISION_FP32_GUARD if params.get("precision", "fp32guard") == "fp32guard" \
         else runtime.PRECISION_FP64
     n = w.num_objects
+    schema = params.get("schema", "dynamic")
     return runtime.make_config(
-        n, w.box(), reach_distance=w.reach_distance, grid_capacity=params.get("grid_capacity", w.grid_capacity),
+        n, box_for(w, schema), reach_distance=w.reach_distance,
+        grid_capacity=params.get("grid_capacity", w.grid_capacity), dynamic_schema=SCHEMAS[schema],
         min_meet_time=w.min_meet_time, precision=prec,
         num_meeting_buckets=table_slots(distinct, params.get("load_factor", 0.75)),
         unit_capacity=8 * n + 4096, meeting_capacity=max(1 << 20, 6 * n),

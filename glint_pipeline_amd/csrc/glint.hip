@@ -197,6 +197,7 @@ __global__ void k_reset_tick(DevState s)
     c->n_leaves = 0;
     c->max_leaf = 0;
     c->work_inside = 0;
+    c->g_heavy = 0;
 }
 
 // ------------------------------------------------------------ tree build
@@ -452,6 +453,315 @@ __global__ __launch_bounds__(kBlock) void k_log_append(DevState s)
     for (unsigned long long i = blockIdx.x * (unsigned long long)kBlock + threadIdx.x; i < n;
          i += (unsigned long long)gridDim.x * kBlock)
         s.mlog[(base + i) % s.mlog_cap] = s.meet[i];
+}
+
+// ------------------------------------------------------------ grid rebuild
+//
+// GLINT_SCHEMA_GRID_REBUILD: the reference's GPU path without a dynamic
+// schema builds the tree from scratch every tick (geometry.cu:1629-1645):
+// points are counted in a 16384 x 16384 cell grid (cuda_build_qtree,
+// geometry.cu:1270-1279) and merged bottom-up (cuda_merge_qtree,
+// geometry.cu:1295-1393), a block of cells becoming a branch iff it holds
+// more than grid_capacity points, a branch cell getting four leaf children.
+// The branch set is closed upwards, so the same tree is built here
+// top-down from block counts, without the reference's two 1 GiB cell arrays:
+//   - k_grid_cells: each object's cell; counts of the 1024^2 coarse blocks
+//     (depth 10, 16x16 cells each, 4 MiB);
+//   - k_grid_pyramid, k_grid_top: block counts of depths 9..0 from them, and
+//     a 16x16 fine histogram for every coarse block over capacity (only those
+//     can hold branches below depth 10);
+//   - k_grid_fine: the fine histograms;
+//   - k_grid_level(d), d = 0..14: every branch block of depth d writes its
+//     node (grid-formula box) and four children: branch blocks, listed for
+//     depth d+1, or leaves (the xhalf box formula, geometry.cu:1311-1326).
+// A fixed launch sequence, so a tick with the rebuild can be captured in a
+// graph.
+constexpr uint32_t kGridBits = 14;
+constexpr uint32_t kGridDim = 1u << kGridBits;             // one_dim (geometry.cu:1266)
+constexpr uint32_t kCoarseDepth = 10;
+constexpr uint32_t kCoarseDim = 1u << kCoarseDepth;
+constexpr uint32_t kFineDim = kGridDim / kCoarseDim;       // cells per coarse block side
+constexpr uint32_t kFineCells = kFineDim * kFineDim;
+
+__host__ __device__ constexpr uint32_t pyr_off(uint32_t d) { return ((1u << (2 * d)) - 1u) / 3u; }
+
+// One count per lane added to p[key] (kNone: nothing), one atomic per run of
+// equal keys in the wave.  Every lane of the wave must call it.
+__device__ __forceinline__ void wave_count(uint32_t *p, uint32_t key)
+{
+    bool need = key != kNone;
+    for (;;) {
+        const unsigned long long pend = __ballot(need);
+        if (!pend) break;
+        const uint32_t leader = __ffsll((long long)pend) - 1;
+        const uint32_t K = __shfl(key, leader);
+        const bool mine = need && key == K;
+        const unsigned long long mm = __ballot(mine);
+        if (lane_id() == leader) atomicAdd(&p[K], (uint32_t)__popcll(mm));
+        if (mine) need = false;
+    }
+}
+
+// Object k of the pass: the leaf order of the tick before when there is one.
+__device__ __forceinline__ uint32_t grid_object(const DevState &s, uint32_t k, bool ordered)
+{
+    if (!ordered) return k;
+    const uint32_t i = s.spid[k];
+    return i < s.n ? i : k;
+}
+
+// The cell as the reference computes it (geometry.cu:1276-1278): x + y*16384
+// in 32-bit arithmetic, so x = 16384 (a point on the high x edge) is the
+// first cell of the next row.  y = 16384 has no cell (the reference writes
+// past its histogram there): flagged.
+__global__ __launch_bounds__(kBlock) void k_grid_cells(DevState s)
+{
+    const double *xy = s.tp->xy;
+    const bool ordered = s.ctr->sorted_valid != 0;
+    for (uint32_t base = blockIdx.x * kBlock; base < s.n; base += gridDim.x * kBlock) {
+        const uint32_t k = base + threadIdx.x;
+        uint32_t key = kNone;
+        if (k < s.n) {
+            const uint32_t i = grid_object(s, k, ordered);
+            const double x = xy[2 * (size_t)i], y = xy[2 * (size_t)i + 1];
+            uint32_t lin = kNone;
+            if (x >= s.lx && x <= s.hx && y >= s.ly && y <= s.hy) {
+                const uint32_t cx = (uint32_t)((x - s.lx) / (s.hx - s.lx) * (double)kGridDim);
+                const uint32_t cy = (uint32_t)((y - s.ly) / (s.hy - s.ly) * (double)kGridDim);
+                lin = cx + cy * kGridDim;
+                if (lin >= kGridDim * kGridDim) {
+                    atomicOr(&s.ctr->error, kErrGridEdge);
+                    lin = kNone;
+                }
+            } else {
+                atomicOr(&s.ctr->error, kErrBadInput);
+            }
+            s.gcell[i] = lin;
+            if (lin != kNone) {
+                const uint32_t gx = lin & (kGridDim - 1), gy = lin >> kGridBits;
+                key = (gy / kFineDim) * kCoarseDim + gx / kFineDim;
+            }
+        }
+        wave_count(s.g10, key);
+    }
+}
+
+// One workgroup per 32x32 patch of coarse blocks (1024 workgroups of 256):
+// depths 9..5 of the pyramid, and a fine histogram for each coarse block
+// over capacity.
+__global__ __launch_bounds__(kBlock) void k_grid_pyramid(DevState s)
+{
+    __shared__ uint32_t lv[256];
+    const uint32_t t = threadIdx.x;
+    const uint32_t px = blockIdx.x & 31, py = blockIdx.x >> 5;
+    const uint32_t gx = px * 16 + (t & 15), gy = py * 16 + (t >> 4);   // depth-9 block
+    uint32_t sum = 0;
+    for (uint32_t i = 0; i < 4; i++) {
+        const uint32_t b = (2 * gy + (i >> 1)) * kCoarseDim + 2 * gx + (i & 1);
+        const uint32_t v = s.g10[b];
+        sum += v;
+        const bool heavy = v > s.cap;
+        const uint32_t h = wave_append_u32(&s.ctr->g_heavy, heavy);
+        uint32_t id = kNone;
+        if (heavy) {
+            if (h < s.heavy_cap)
+                id = h;
+            else
+                atomicOr(&s.ctr->error, kErrNodeCap);   // not reached: blocks over capacity are disjoint
+        }
+        s.gheavy[b] = id;
+    }
+    lv[t] = sum;
+    s.gpyr[pyr_off(9) + gy * (1u << 9) + gx] = sum;
+    uint32_t side = 16;
+    for (uint32_t d = 8; d >= 5; d--) {
+        const uint32_t ns = side / 2;
+        const uint32_t x = t % ns, y = t / ns;
+        uint32_t v = 0;
+        __syncthreads();
+        if (t < ns * ns)
+            v = lv[2 * y * side + 2 * x] + lv[2 * y * side + 2 * x + 1] + lv[(2 * y + 1) * side + 2 * x] +
+                lv[(2 * y + 1) * side + 2 * x + 1];
+        __syncthreads();
+        if (t < ns * ns) {
+            lv[t] = v;
+            s.gpyr[pyr_off(d) + (py * ns + y) * (1u << d) + px * ns + x] = v;
+        }
+        side = ns;
+    }
+}
+
+// Depths 4..0 from the 32x32 blocks of depth 5, and the root: a branch when
+// the points exceed capacity (the list of depth 0), else one leaf.
+__global__ __launch_bounds__(1024) void k_grid_top(DevState s)
+{
+    __shared__ uint32_t lv[1024];
+    const uint32_t t = threadIdx.x;
+    lv[t] = s.gpyr[pyr_off(5) + t];
+    uint32_t side = 32;
+    for (int d = 4; d >= 0; d--) {
+        const uint32_t ns = side / 2;
+        const uint32_t x = t % ns, y = t / ns;
+        uint32_t v = 0;
+        __syncthreads();
+        if (t < ns * ns)
+            v = lv[2 * y * side + 2 * x] + lv[2 * y * side + 2 * x + 1] + lv[(2 * y + 1) * side + 2 * x] +
+                lv[(2 * y + 1) * side + 2 * x + 1];
+        __syncthreads();
+        if (t < ns * ns) {
+            lv[t] = v;
+            s.gpyr[pyr_off((uint32_t)d) + y * ns + x] = v;
+        }
+        side = ns;
+    }
+    if (t == 0) {
+        Counters *c = s.ctr;
+        c->g_nodes = 1;
+        for (int d = 0; d < 16; d++) c->g_list[d] = 0;
+        if (lv[0] > s.cap) {
+            s.glist[0][0] = make_uint2(0u, 0u);
+            c->g_list[0] = 1;
+        } else {
+            // no branch anywhere (the reference writes no node then): the root block as one leaf
+            Node r;
+            r.lx = s.lx + 0.0 * (s.hx - s.lx) / 1.0;
+            r.ly = s.ly + 0.0 * (s.hy - s.ly) / 1.0;
+            r.hx = s.lx + 1.0 * (s.hx - s.lx) / 1.0;
+            r.hy = s.ly + 1.0 * (s.hy - s.ly) / 1.0;
+            r.mx = (r.lx + r.hx) / 2;
+            r.my = (r.ly + r.hy) / 2;
+            r.child[0] = kLeafTag;
+            r.child[1] = r.child[2] = r.child[3] = 0;
+            s.nodes[0] = r;
+            NodeMeta m;
+            m.type = kLeaf;
+            m.level = 0;
+            m.overflow = 0;
+            m.underflow = 0;
+            s.meta[0] = m;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_grid_fine_zero(DevState s)
+{
+    const uint64_t n = (uint64_t)min(s.ctr->g_heavy, s.heavy_cap) * kFineCells;
+    for (uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock)
+        s.gfine[i] = 0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_grid_fine(DevState s)
+{
+    const bool ordered = s.ctr->sorted_valid != 0;
+    for (uint32_t base = blockIdx.x * kBlock; base < s.n; base += gridDim.x * kBlock) {
+        const uint32_t k = base + threadIdx.x;
+        uint32_t key = kNone;
+        if (k < s.n) {
+            const uint32_t lin = s.gcell[grid_object(s, k, ordered)];
+            if (lin != kNone) {
+                const uint32_t gx = lin & (kGridDim - 1), gy = lin >> kGridBits;
+                const uint32_t h = s.gheavy[(gy / kFineDim) * kCoarseDim + gx / kFineDim];
+                if (h != kNone) key = h * kFineCells + (gy % kFineDim) * kFineDim + gx % kFineDim;
+            }
+        }
+        wave_count(s.gfine, key);
+    }
+}
+
+// Points in block (bx, by) of depth e (2^e blocks per side).  Below the
+// coarse depth the block lies in a coarse block over capacity (its parent
+// is a branch), whose fine histogram holds its cells.
+__device__ uint32_t grid_count(const DevState &s, uint32_t e, uint32_t bx, uint32_t by)
+{
+    if (e < kCoarseDepth) return s.gpyr[pyr_off(e) + by * (1u << e) + bx];
+    if (e == kCoarseDepth) return s.g10[by * kCoarseDim + bx];
+    const uint32_t sh = e - kCoarseDepth;
+    const uint32_t h = s.gheavy[(by >> sh) * kCoarseDim + (bx >> sh)];
+    if (h == kNone) return 0;
+    const uint32_t sz = 1u << (kGridBits - e), m = (1u << sh) - 1u;
+    const uint32_t ox = (bx & m) * sz, oy = (by & m) * sz;
+    const uint32_t *f = s.gfine + (size_t)h * kFineCells;
+    uint32_t sum = 0;
+    for (uint32_t yy = 0; yy < sz; yy++)
+        for (uint32_t xx = 0; xx < sz; xx++) sum += f[(oy + yy) * kFineDim + ox + xx];
+    return sum;
+}
+
+// The branch blocks of depth d: node boxes by the grid formula
+// (geometry.cu:1304-1307,1335-1338), four children each, from one
+// wave-aggregated reservation; a child over capacity is listed for depth
+// d+1, any other is a leaf.  A failed reservation leaves the block a leaf
+// (and flags the error), so the tree stays well formed.
+__global__ __launch_bounds__(kBlock) void k_grid_level(DevState s, uint32_t d)
+{
+    const uint32_t nb = min(s.ctr->g_list[d], s.glist_cap);
+    const uint2 *in = s.glist[d & 1];
+    uint2 *out = s.glist[(d + 1) & 1];
+    const double xdim = (double)(1u << d);
+    const double wx = s.hx - s.lx, wy = s.hy - s.ly;
+    for (uint32_t base = blockIdx.x * kBlock; base < nb; base += gridDim.x * kBlock) {
+        const uint32_t j = base + threadIdx.x;
+        const bool act = j < nb;
+        const unsigned long long mask = __ballot(act);
+        const uint32_t leader = __ffsll((long long)mask) - 1;
+        uint32_t c0 = 0;
+        if (lane_id() == leader) c0 = atomicAdd(&s.ctr->g_nodes, 4u * (uint32_t)__popcll(mask));
+        c0 = __shfl(c0, leader) + 4u * (uint32_t)__popcll(mask & ((1ull << lane_id()) - 1ull));
+        if (!act) continue;
+        const uint2 e = in[j];
+        const uint32_t node = e.x, bx = e.y & 0xFFFFu, by = e.y >> 16;
+        Node b;
+        b.lx = s.lx + (double)bx * wx / xdim;
+        b.ly = s.ly + (double)by * wy / xdim;
+        b.hx = s.lx + (double)(bx + 1) * wx / xdim;
+        b.hy = s.ly + (double)(by + 1) * wy / xdim;
+        b.mx = (b.lx + b.hx) / 2;
+        b.my = (b.ly + b.hy) / 2;
+        NodeMeta bm;
+        bm.type = kBranch;
+        bm.level = (int32_t)d;
+        bm.overflow = 0;
+        bm.underflow = 0;
+        if ((uint64_t)c0 + 4 > s.node_cap) {
+            atomicOr(&s.ctr->error, kErrNodeCap);
+            b.child[0] = kLeafTag;
+            b.child[1] = b.child[2] = b.child[3] = 0;
+            bm.type = kLeaf;
+        } else {
+            const double xhalf = b.mx - b.lx, yhalf = b.my - b.ly;
+            for (uint32_t i = 0; i < 4; i++) {
+                const uint32_t c = c0 + i;
+                b.child[i] = c;
+                const uint32_t cbx = 2 * bx + (i & 1), cby = 2 * by + (i >> 1);
+                if (d < kGridBits && grid_count(s, d + 1, cbx, cby) > s.cap) {
+                    const uint32_t slot = atomicAdd(&s.ctr->g_list[d + 1], 1u);
+                    if (slot < s.glist_cap) {
+                        out[slot] = make_uint2(c, cbx | (cby << 16));
+                        continue;
+                    }
+                    atomicOr(&s.ctr->error, kErrNodeCap);   // not reached: branches of a depth are disjoint
+                }
+                Node l;
+                l.lx = b.lx + (double)(i % 2 == 1) * xhalf;
+                l.ly = b.ly + (double)(i / 2 == 1) * yhalf;
+                l.hx = b.mx + (double)(i % 2 == 1) * xhalf;
+                l.hy = b.my + (double)(i / 2 == 1) * yhalf;
+                l.mx = (l.lx + l.hx) / 2;
+                l.my = (l.ly + l.hy) / 2;
+                l.child[0] = kLeafTag;
+                l.child[1] = l.child[2] = l.child[3] = 0;
+                s.nodes[c] = l;
+                NodeMeta lm;
+                lm.type = kLeaf;
+                lm.level = (int32_t)d + 1;
+                lm.overflow = 0;
+                lm.underflow = 0;
+                s.meta[c] = lm;
+            }
+        }
+        s.nodes[node] = b;
+        s.meta[node] = bm;
+    }
 }
 
 // ------------------------------------------------------------ partition
@@ -717,9 +1027,28 @@ __device__ __forceinline__ uint32_t wave_exclusive_scan(uint32_t v, uint32_t *to
     return inc - v;
 }
 
+// Units (pid, leaf) of one object written at once: used when its candidate
+// list fills up (deep trees of the grid rebuild can put many leaves within
+// reach), the common case goes through the wave-aggregated path below.
+__device__ __noinline__ void emit_units(const DevState &s, uint32_t pid, const uint32_t *cand, uint32_t n)
+{
+    const unsigned long long u0 = atomicAdd(&s.ctr->n_units, (unsigned long long)n);
+    for (uint32_t j = 0; j < n; j++) {
+        const uint32_t rank = atomicAdd(&s.ucount[cand[j]], 1u);
+        if (u0 + j < s.unit_cap) {
+            s.units[u0 + j] = make_uint2(pid, cand[j]);
+            s.urank[u0 + j] = rank;
+        } else {
+            atomicOr(&s.ctr->error, kErrUnitCap);
+        }
+    }
+}
+
 // lookup_rec (filter.cpp:130-152) from last_valid for every border object.
 // In complete mode every leaf within r other than the object's own.  Each
 // unit (object, candidate leaf) takes a rank among the units of its leaf.
+// The object's first three candidates and their number are kept for the
+// join of the next tick.
 __global__ __launch_bounds__(kBlock) void k_lookup(DevState s)
 {
     const double *xy = s.tp->xy;
@@ -727,7 +1056,8 @@ __global__ __launch_bounds__(kBlock) void k_lookup(DevState s)
     for (uint32_t base = blockIdx.x * blockDim.x; base < nb; base += gridDim.x * blockDim.x) {
         uint32_t k = base + threadIdx.x;
         uint32_t cand[kMaxCand];
-        uint32_t nc = 0;
+        uint32_t nc = 0, total = 0;
+        uint32_t f0 = kNone, f1 = kNone, f2 = kNone;
         uint32_t pid = 0;
         if (k < nb) {
             uint2 b = s.border[k];
@@ -747,10 +1077,15 @@ __global__ __launch_bounds__(kBlock) void k_lookup(DevState s)
                     if (ch.child[0] == kLeafTag) {
                         bool take = s.complete ? (c != own) : (y < ch.ly || (y < ch.hy && x < ch.lx));
                         if (take) {
-                            if (nc < kMaxCand)
-                                cand[nc++] = c;
-                            else
-                                atomicOr(&s.ctr->error, kErrStack);
+                            if (nc == kMaxCand) {
+                                emit_units(s, pid, cand, nc);
+                                nc = 0;
+                            }
+                            cand[nc++] = c;
+                            f0 = total == 0 ? c : f0;
+                            f1 = total == 1 ? c : f1;
+                            f2 = total == 2 ? c : f2;
+                            total++;
                         }
                     } else {
                         if (sp < kLookupStack)
@@ -761,8 +1096,7 @@ __global__ __launch_bounds__(kBlock) void k_lookup(DevState s)
                 }
             }
         }
-        if (k < nb && !s.store_all)
-            s.cand[pid] = make_uint4(nc > 0 ? cand[0] : kNone, nc > 1 ? cand[1] : kNone, nc > 2 ? cand[2] : kNone, nc);
+        if (k < nb && !s.store_all) s.cand[pid] = make_uint4(f0, f1, f2, total);
         // one atomic per wave reserves the wave's units
         uint32_t wtotal;
         uint32_t wex = wave_exclusive_scan(nc, &wtotal);
@@ -1894,13 +2228,15 @@ double sqrt_threshold(double r)
 enum KernelId {
     K_FILL_COUNTS, K_RESET, K_ASSIGN, K_SCAN_REDUCE, K_SCAN_TOP, K_SCAN_OBJECTS, K_SCATTER, K_FILL_UCOUNT,
     K_LOOKUP, K_SCAN_UNITS, K_UNIT_SCATTER, K_JOIN, K_BUCKET_COUNT, K_SCAN_WRITE, K_BUCKET_SCATTER, K_BUCKET_FINE,
-    K_TABLE, K_COLLECT, K_SPLIT, K_MERGE, K_SCHEMA_DONE, K_TICK_PARAMS, K_POS32, K_LOG_APPEND, K_COUNT
+    K_TABLE, K_COLLECT, K_SPLIT, K_MERGE, K_SCHEMA_DONE, K_TICK_PARAMS, K_POS32, K_LOG_APPEND, K_GRID_CELLS,
+    K_GRID_PYRAMID, K_GRID_TOP, K_GRID_FINE_ZERO, K_GRID_FINE, K_GRID_LEVEL, K_COUNT
 };
 const char *const kKernelNames[K_COUNT] = {
     "k_fill_u32", "k_reset_tick", "k_assign", "k_scan_reduce", "k_scan_top", "k_scan_objects", "k_scatter",
     "k_fill_ucount", "k_lookup", "k_scan_units", "k_unit_scatter", "k_join", "k_bucket_count", "k_scan_write",
     "k_bucket_scatter", "k_bucket_fine", "k_table", "k_collect", "k_split", "k_merge", "k_schema_done",
-    "k_advance_tick", "k_pos32", "k_log_append"};
+    "k_advance_tick", "k_pos32", "k_log_append", "k_grid_cells", "k_grid_pyramid", "k_grid_top",
+    "k_grid_fine_zero", "k_grid_fine", "k_grid_level"};
 
 // Per-kernel HIP-event timing on the context's stream (optional).
 struct KernelProfiler {
@@ -2044,6 +2380,11 @@ glint_status status_from_flags(glint_ctx *c, uint32_t f)
         c->err = "a position is NaN or outside the root box";
         return GLINT_ERR_BAD_INPUT;
     }
+    if (f & kErrGridEdge) {
+        c->err = "a position lies on the high y edge of the root box, which has no cell in the 16384^2 grid "
+                 "of the grid rebuild";
+        return GLINT_ERR_BAD_INPUT;
+    }
     if (f & kErrNodeCap) {
         c->err = "quadtree node pool exhausted (raise node_capacity)";
         return GLINT_ERR_NODE_CAPACITY;
@@ -2103,6 +2444,11 @@ glint_status build_tree(glint_ctx *c)
     DevState &s = c->s;
     const uint32_t gn = grid_for(s.n), gnode = grid_for(s.node_cap);
     k_init_tree<<<gnode, kBlock, 0, c->stream>>>(s);
+    if (c->cfg.dynamic_schema == GLINT_SCHEMA_GRID_REBUILD) {   // every tick builds its own tree
+        HIP_TRY(c, hipGetLastError());
+        c->built = true;
+        return GLINT_OK;
+    }
     for (int level = 0; level < 64; level++) {
         k_fill_u32<<<gnode, kBlock, 0, c->stream>>>(s.count, 0u, s.node_cap);
         k_reset_tick<<<1, 1, 0, c->stream>>>(s);
@@ -2119,6 +2465,33 @@ glint_status build_tree(glint_ctx *c)
     return GLINT_OK;
 }
 
+// The tree of this tick's positions (GLINT_SCHEMA_GRID_REBUILD), before the
+// assignment: a fixed sequence of launches.
+void enqueue_grid_rebuild(glint_ctx *c)
+{
+    DevState &s = c->s;
+    hipStream_t st = c->stream;
+    const uint32_t gn = grid_for(s.n);
+    launch(c, K_FILL_COUNTS, [&] {
+        k_fill_u32<<<grid_for(kCoarseDim * kCoarseDim), kBlock, 0, st>>>(s.g10, 0u, kCoarseDim * kCoarseDim);
+    });
+    launch(c, K_FILL_COUNTS, [&] {
+        k_fill_u32<<<grid_for(4ull * s.node_cap), kBlock, 0, st>>>(reinterpret_cast<uint32_t *>(s.meta), 0u,
+                                                                   4ull * s.node_cap);
+    });
+    launch(c, K_GRID_CELLS, [&] { k_grid_cells<<<gn, kBlock, 0, st>>>(s); });
+    launch(c, K_GRID_PYRAMID, [&] { k_grid_pyramid<<<1024, kBlock, 0, st>>>(s); });
+    launch(c, K_GRID_TOP, [&] { k_grid_top<<<1, 1024, 0, st>>>(s); });
+    launch(c, K_GRID_FINE_ZERO, [&] {
+        k_grid_fine_zero<<<grid_for((uint64_t)s.heavy_cap * kFineCells, 2048), kBlock, 0, st>>>(s);
+    });
+    launch(c, K_GRID_FINE, [&] { k_grid_fine<<<gn, kBlock, 0, st>>>(s); });
+    for (uint32_t d = 0; d <= kGridBits; d++) {
+        const uint64_t most = std::min<uint64_t>(1ull << (2 * d), s.glist_cap);
+        launch(c, K_GRID_LEVEL, [&] { k_grid_level<<<grid_for(most, 1024), kBlock, 0, st>>>(s, d); });
+    }
+}
+
 void enqueue_partition(glint_ctx *c)
 {
     DevState &s = c->s;
@@ -2126,6 +2499,7 @@ void enqueue_partition(glint_ctx *c)
     const uint32_t gn = grid_for(s.n), gnode = grid_for(s.node_cap), nb = c->scan_blocks;
     launch(c, K_FILL_COUNTS, [&] { k_fill_u32<<<gnode, kBlock, 0, st>>>(s.count, 0u, s.node_cap); });
     launch(c, K_RESET, [&] { k_reset_tick<<<1, 1, 0, st>>>(s); });
+    if (c->cfg.dynamic_schema == GLINT_SCHEMA_GRID_REBUILD) enqueue_grid_rebuild(c);
     launch(c, K_ASSIGN, [&] { k_assign<<<gn, kBlock, 0, st>>>(s); });
     launch(c, K_SCAN_REDUCE, [&] { k_scan_reduce<<<nb, kBlock, 0, st>>>(s.count, s.node_cap, s.block_sums); });
     launch(c, K_SCAN_TOP, [&] { k_scan_top<<<1, kBlock, 0, st>>>(s.block_sums, nb); });
@@ -2222,7 +2596,7 @@ void enqueue_schema(glint_ctx *c)
 {
     DevState &s = c->s;
     hipStream_t st = c->stream;
-    if (c->cfg.dynamic_schema) {
+    if (c->cfg.dynamic_schema == GLINT_SCHEMA_DYNAMIC) {
         const uint32_t gnode = grid_for(s.node_cap);
         launch(c, K_COLLECT, [&] { k_collect<<<gnode, kBlock, 0, st>>>(s); });
         launch(c, K_SPLIT, [&] { k_split<<<gnode, kBlock, 0, st>>>(s, 0); });
@@ -2237,7 +2611,9 @@ void fill_stats(glint_ctx *c)
     glint_tick_stats &t = c->last;
     t.tick = c->tick;
     t.num_leaves = h.n_leaves;
-    t.num_nodes = (uint32_t)(c->s.node_cap - (uint32_t)std::max(h.free_top, 0));
+    t.num_nodes = c->cfg.dynamic_schema == GLINT_SCHEMA_GRID_REBUILD
+                      ? std::min(h.g_nodes, c->s.node_cap)
+                      : (uint32_t)(c->s.node_cap - (uint32_t)std::max(h.free_top, 0));
     t.num_border = h.n_border;
     t.max_leaf_count = h.max_leaf;
     t.splits = h.last_splits;
@@ -2308,7 +2684,7 @@ glint_status glint_create(const glint_config *cfg, glint_ctx **out)
     g_create_error.clear();
     if (cfg->num_objects == 0 || cfg->grid_capacity == 0 || !(cfg->reach_distance > 0) ||
         !(cfg->high_x > cfg->low_x) || !(cfg->high_y > cfg->low_y) || cfg->candidate_mode > 1 ||
-        cfg->precision > 1 || cfg->num_objects >= 0x40000000u ||
+        cfg->precision > 1 || cfg->num_objects >= 0x40000000u || cfg->dynamic_schema > GLINT_SCHEMA_GRID_REBUILD ||
         (cfg->join_tile != 0 && cfg->join_tile != 256 && cfg->join_tile != 512) ||
         (cfg->table_threads != 0 && cfg->table_threads != 256 && cfg->table_threads != 512 &&
          cfg->table_threads != 1024)) {
@@ -2451,6 +2827,18 @@ glint_status glint_create(const glint_config *cfg, glint_ctx **out)
     ALLOC(s.unit_pid, s.unit_cap);
     ALLOC(s.ucount, node_cap);
     ALLOC(s.ustart, node_cap);
+    if (cfg->dynamic_schema == GLINT_SCHEMA_GRID_REBUILD) {
+        // blocks over capacity at one depth are disjoint: at most N/(cap+1)
+        s.heavy_cap = (uint32_t)(N / ((uint64_t)cfg->grid_capacity + 1) + 1);
+        s.glist_cap = s.heavy_cap;
+        ALLOC(s.gcell, N);
+        ALLOC(s.g10, (uint64_t)kCoarseDim * kCoarseDim);
+        ALLOC(s.gpyr, (uint64_t)pyr_off(kCoarseDepth));
+        ALLOC(s.gheavy, (uint64_t)kCoarseDim * kCoarseDim);
+        ALLOC(s.gfine, (uint64_t)s.heavy_cap * kFineCells);
+        ALLOC(s.glist[0], s.glist_cap);
+        ALLOC(s.glist[1], s.glist_cap);
+    }
     ALLOC(s.sx, N);
     ALLOC(s.sy, N);
     ALLOC(s.sxp, N);

@@ -9,6 +9,9 @@
 //   fx, fy               the same positions as FP32 offsets from the leaf's
 //                        low corner (guard-band mode)
 //   units                border check units (object id, candidate leaf)
+//   gcell, g10, gpyr,    grid rebuild only: object cells, the coarse 1024^2
+//   gheavy, gfine        histogram with its pyramid, and 16x16 cell histograms
+//                        of the coarse blocks over capacity
 //   gen[2]               the pairs of the last two ticks: each tick's contact
 //                        list grouped by table region (8 B keys) with a start
 //                        tick per entry (4 B) and the first entry per region
@@ -42,6 +45,7 @@ enum ErrorBits : uint32_t {
     kErrContactCap = 1u << 5,
     kErrStack = 1u << 6,
     kErrKvCap = 1u << 7,
+    kErrGridEdge = 1u << 8,   // grid rebuild: a position with no cell (on the high y edge)
 };
 
 struct alignas(64) Node {
@@ -99,6 +103,10 @@ struct Counters {
     uint32_t pad_;
     unsigned long long n_kvlog;     // meetings logged for the key-value output since its last flush
     unsigned long long n_mlog;      // meetings ever appended to the meeting log
+    // grid rebuild (GLINT_SCHEMA_GRID_REBUILD)
+    uint32_t g_nodes;               // nodes handed out this tick
+    uint32_t g_heavy;               // coarse blocks over capacity this tick
+    uint32_t g_list[16];            // branch blocks per depth
 };
 
 // Per-tick values, kept in device memory so a tick's launches can be
@@ -207,6 +215,15 @@ struct DevState {
     glint_meeting *mlog;           // meeting log ring (null when off)
     unsigned long long mlog_cap;
     unsigned long long kvlog_cap;
+    // grid rebuild (null unless GLINT_SCHEMA_GRID_REBUILD)
+    uint32_t *gcell;               // cell x + 16384*y of every object (pid order)
+    uint32_t *g10;                 // points per block of the 1024^2 coarse grid (depth 10)
+    uint32_t *gpyr;                // points per block at depths 0..9, depth d from (4^d-1)/3
+    uint32_t *gheavy;              // per coarse block: its fine histogram, or kNone
+    uint32_t *gfine;               // 16x16 cell counts of every coarse block over capacity
+    uint2 *glist[2];               // branch blocks of one depth (node, x | y << 16), ping-pong
+    uint32_t heavy_cap;            // fine histograms held
+    uint32_t glist_cap;            // branch blocks held per depth
     Counters *ctr;
 };
 

@@ -35,6 +35,7 @@ STATUS_NAMES = {
 globals().update({name: code for code, name in STATUS_NAMES.items()})
 CANDIDATES_REFERENCE, CANDIDATES_COMPLETE = 0, 1
 PRECISION_FP64, PRECISION_FP32_GUARD = 0, 1
+SCHEMA_STATIC, SCHEMA_DYNAMIC, SCHEMA_GRID_REBUILD = 0, 1, 2
 
 
 class GlintError(RuntimeError):
@@ -312,7 +313,7 @@ class DeviceBuffer:
 
 
 def make_config(num_objects: int, box, *, reach_distance: float = 2.0, grid_capacity: int = 50,
-                min_meet_time: int = 5, schema_update_delay: int = 1, dynamic_schema: bool = True,
+                min_meet_time: int = 5, schema_update_delay: int = 1, dynamic_schema: int = SCHEMA_DYNAMIC,
                 start_time: int = 0, complete: bool = False, precision: int = PRECISION_FP64,
                 guard_epsilon: float = 1e-3, num_meeting_buckets: int = 0, unit_capacity: int = 0,
                 meeting_capacity: int = 0, node_capacity: int = 0, record_contacts: bool = False,

@@ -60,6 +60,18 @@ enum {
     GLINT_PRECISION_FP32_GUARD = 1  /* FP32 test with a guard band, FP64 recheck inside it */
 };
 
+/* glint_config.dynamic_schema (config.h:35): how the tree follows the objects.
+ * A binding that replaces the reference's GPU path maps its flag to
+ * DYNAMIC when set and GRID_REBUILD when not (geometry.cu:1629,2160). */
+enum {
+    GLINT_SCHEMA_STATIC = 0,        /* the tree built at the first tick is kept (the reference's
+                                       CPU path with dynamic_schema off, process.cpp:608) */
+    GLINT_SCHEMA_DYNAMIC = 1,       /* leaves split and merge after every tick (geometry.cu:1114-1234) */
+    GLINT_SCHEMA_GRID_REBUILD = 2   /* the tree rebuilt every tick from a 16384^2 cell grid
+                                       (geometry.cu:1270-1393,1629-1645); positions on the high
+                                       y edge of the box have no cell there and are refused */
+};
+
 /* Mirrors the hot-path fields of the reference's configuration
  * (src/util/config.h:28-48, defaults at config.h:173-187). */
 typedef struct glint_config {
@@ -68,7 +80,7 @@ typedef struct glint_config {
     uint32_t min_meet_time;        /* config.h:39 */
     uint32_t max_meet_time;        /* config.h:40: longer meetings are flagged oversize */
     uint32_t schema_update_delay;  /* config.h:38 */
-    uint32_t dynamic_schema;       /* config.h:35 */
+    uint32_t dynamic_schema;       /* config.h:35: GLINT_SCHEMA_* */
     uint32_t start_time;           /* config.h:28 */
     uint32_t candidate_mode;       /* GLINT_CANDIDATES_* */
     uint32_t precision;            /* GLINT_PRECISION_* */

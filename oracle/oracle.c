@@ -28,6 +28,10 @@
  *    level (child boxes lo + (i%2)*half, mid + (i%2)*half); a branch whose
  *    four children are leaves with total count < cap merges; decisions use
  *    this tick's counts before any change.
+ *  - Grid rebuild (geometry.cu:1270-1393,1629-1645), the reference's GPU
+ *    path with dynamic_schema off: every tick, the tree of the 16384^2 cell
+ *    histogram in which a block is a branch iff it holds more than
+ *    grid_capacity points (see grid_build below).
  */
 #include "oracle.h"
 #include "oracle_kv_internal.h"
@@ -197,6 +201,138 @@ int oracle_build(void *h, const double *xy)
     build_rec(c, root, idx, n, tmp, xy);
     free(idx);
     free(tmp);
+    memset(c->count, 0, (size_t)c->cap_nodes * sizeof(uint32_t));
+    c->built = 1;
+    return 0;
+}
+
+/* ------------------------------------------------------ grid rebuild (GPU, static) */
+
+/* The reference's GPU path without a dynamic schema rebuilds the tree from
+ * its points every tick (geometry.cu:1629-1645): a 16384 x 16384 histogram of
+ * cells (cuda_build_qtree, geometry.cu:1270-1279), merged bottom-up one level
+ * at a time (cuda_merge_qtree, geometry.cu:1295-1393).  A block of cells is a
+ * branch iff it holds more than grid_capacity points; a branch cell gets four
+ * leaf children; node 0 is the root.  Restated here top-down: a branch's
+ * children are branches iff their own cell count exceeds the capacity (the
+ * branch set is closed upwards, so both orders give one tree).  Boxes:
+ * a branch block (bx, by) at depth d (xdim = 2^d blocks per side) spans
+ * low + bx*(high-low)/xdim .. low + (bx+1)*(high-low)/xdim (geometry.cu:
+ * 1304-1307,1335-1338); a leaf child i spans lo + (i%2)*xhalf .. mid +
+ * (i%2)*xhalf with xhalf = mid - lo (geometry.cu:1311-1326).  When no block
+ * exceeds the capacity the reference writes no node at all and keeps the
+ * node 0 of the tick before; here the root is then one leaf with the
+ * depth-0 block box. */
+#define GRID_BITS 14
+#define GRID_DIM (1u << GRID_BITS)
+
+static void grid_block_box(const octx *c, int d, uint32_t bx, uint32_t by, onode *n)
+{
+    const double xdim = (double)(1u << d);
+    const double lx = c->cfg.low_x, ly = c->cfg.low_y, hx = c->cfg.high_x, hy = c->cfg.high_y;
+    n->lo[0] = lx + (double)bx * (hx - lx) / xdim;
+    n->lo[1] = ly + (double)by * (hy - ly) / xdim;
+    n->hi[0] = lx + (double)(bx + 1) * (hx - lx) / xdim;
+    n->hi[1] = ly + (double)(by + 1) * (hy - ly) / xdim;
+    n->mid[0] = (n->lo[0] + n->hi[0]) / 2;
+    n->mid[1] = (n->lo[1] + n->hi[1]) / 2;
+}
+
+static void grid_rec(octx *c, uint32_t node, int d, uint32_t bx, uint32_t by, uint32_t *idx, uint32_t n,
+                     uint32_t *tmp, const uint32_t *cell)
+{
+    onode b;
+    memset(&b, 0, sizeof(b));
+    grid_block_box(c, d, bx, by, &b);
+    b.type = T_BRANCH;
+    b.level = d;
+    const double xhalf = b.mid[0] - b.lo[0], yhalf = b.mid[1] - b.lo[1];
+    /* the points of each child block: cell bit (13 - d) of x and y */
+    uint32_t cnt[4] = {0, 0, 0, 0}, off[4], pos[4];
+    if (d < GRID_BITS) {
+        const int sh = GRID_BITS - 1 - d;
+        for (uint32_t k = 0; k < n; k++) {
+            const uint32_t lin = cell[idx[k]];
+            cnt[(((lin >> GRID_BITS) >> sh) & 1) * 2 + (((lin & (GRID_DIM - 1)) >> sh) & 1)]++;
+        }
+        off[0] = 0;
+        for (int i = 1; i < 4; i++) off[i] = off[i - 1] + cnt[i - 1];
+        for (int i = 0; i < 4; i++) pos[i] = off[i];
+        for (uint32_t k = 0; k < n; k++) {
+            const uint32_t lin = cell[idx[k]];
+            tmp[pos[(((lin >> GRID_BITS) >> sh) & 1) * 2 + (((lin & (GRID_DIM - 1)) >> sh) & 1)]++] = idx[k];
+        }
+        memcpy(idx, tmp, (size_t)n * sizeof(uint32_t));
+    }
+    uint32_t ch[4];
+    for (int i = 0; i < 4; i++) ch[i] = alloc_node(c);
+    for (int i = 0; i < 4; i++) b.child[i] = ch[i];
+    c->nodes[node] = b;
+    for (int i = 0; i < 4; i++) {
+        const uint32_t cbx = 2 * bx + (uint32_t)(i & 1), cby = 2 * by + (uint32_t)(i >> 1);
+        if (d < GRID_BITS && cnt[i] > c->cfg.grid_capacity) {
+            grid_rec(c, ch[i], d + 1, cbx, cby, idx + off[i], cnt[i], tmp + off[i], cell);
+            continue;
+        }
+        onode l;
+        memset(&l, 0, sizeof(l));
+        l.lo[0] = b.lo[0] + (double)(i % 2 == 1) * xhalf;
+        l.lo[1] = b.lo[1] + (double)(i / 2 == 1) * yhalf;
+        l.hi[0] = b.mid[0] + (double)(i % 2 == 1) * xhalf;
+        l.hi[1] = b.mid[1] + (double)(i / 2 == 1) * yhalf;
+        l.mid[0] = (l.lo[0] + l.hi[0]) / 2;
+        l.mid[1] = (l.lo[1] + l.hi[1]) / 2;
+        l.type = T_LEAF;
+        l.level = d + 1;
+        c->nodes[ch[i]] = l;
+    }
+}
+
+/* The cell of every point as the reference computes it (geometry.cu:1276-
+ * 1278), x + y*16384 in 32-bit arithmetic: a point on the high x edge lands
+ * in the first cell of the next row, as there; one on the high y edge has
+ * no cell (the reference writes past its histogram) and is refused. */
+int oracle_grid_cells(const oracle_config *cfg, const double *xy, uint32_t *cell)
+{
+    int bad = 0;
+    const double lx = cfg->low_x, ly = cfg->low_y, hx = cfg->high_x, hy = cfg->high_y;
+#pragma omp parallel for reduction(| : bad)
+    for (int64_t i = 0; i < (int64_t)cfg->num_objects; i++) {
+        const uint32_t x = (uint32_t)((xy[2 * i] - lx) / (hx - lx) * GRID_DIM);
+        const uint32_t y = (uint32_t)((xy[2 * i + 1] - ly) / (hy - ly) * GRID_DIM);
+        const uint32_t lin = x + y * GRID_DIM;
+        cell[i] = lin;
+        if (lin >= GRID_DIM * GRID_DIM) bad = 1;
+    }
+    return bad ? -2 : 0;
+}
+
+static int grid_build(octx *c, const double *xy)
+{
+    const uint32_t n = c->cfg.num_objects;
+    uint32_t *cell = (uint32_t *)malloc((size_t)n * sizeof(uint32_t));
+    if (oracle_grid_cells(&c->cfg, xy, cell)) {
+        free(cell);
+        return -2;
+    }
+    c->nnodes = 0;
+    c->nfree = 0;
+    const uint32_t root = alloc_node(c);
+    if (n > c->cfg.grid_capacity) {
+        uint32_t *idx = (uint32_t *)malloc((size_t)n * sizeof(uint32_t));
+        uint32_t *tmp = (uint32_t *)malloc((size_t)n * sizeof(uint32_t));
+        for (uint32_t i = 0; i < n; i++) idx[i] = i;
+        grid_rec(c, root, 0, 0, 0, idx, n, tmp, cell);
+        free(idx);
+        free(tmp);
+    } else {
+        onode *r = &c->nodes[root];
+        memset(r, 0, sizeof(*r));
+        grid_block_box(c, 0, 0, 0, r);
+        r->type = T_LEAF;
+        r->level = 0;
+    }
+    free(cell);
     memset(c->count, 0, (size_t)c->cap_nodes * sizeof(uint32_t));
     c->built = 1;
     return 0;
@@ -564,7 +700,11 @@ int oracle_tick(void *h, const double *xy, uint32_t tick)
     if (check_inputs(c, xy)) return -2;
     memset(&c->st, 0, sizeof(c->st));
     double t0 = omp_get_wtime();
-    if (!c->built) oracle_build(c, xy);
+    if (c->cfg.dynamic_schema == ORACLE_SCHEMA_GRID_REBUILD) {
+        if (grid_build(c, xy)) return -2;
+    } else if (!c->built) {
+        oracle_build(c, xy);
+    }
 
     /* ---- filter: descent, last_valid, leaf membership (filter.cpp:222-261) */
     memset(c->count, 0, (size_t)c->cap_nodes * sizeof(uint32_t));
@@ -678,7 +818,7 @@ int oracle_tick(void *h, const double *xy, uint32_t tick)
     c->st.num_meetings = c->nmeet;
     c->st.max_leaf_count = maxc;
     c->st.ref_drop_leaves = drops;
-    if (c->cfg.dynamic_schema) update_schema(c);
+    if (c->cfg.dynamic_schema == ORACLE_SCHEMA_DYNAMIC) update_schema(c);
     double t4 = omp_get_wtime();
     c->st.t_filter = t1 - t0;
     c->st.t_refine = t2 - t1;

@@ -13,6 +13,8 @@
  *   refine          src/tracing/refinement.cpp:10-64 (reachability_unit)
  *   evaluate        src/tracing/meeting.cpp:32-76 (update_meetings_unit)
  *   schema update   src/cuda/geometry.cu:1114-1234 (collect, split, merge)
+ *   grid rebuild    src/cuda/geometry.cu:1270-1393,1629-1645 (build_qtree,
+ *                   merge_qtree: dynamic_schema off on the GPU)
  *   predicates      src/geometry/geometry.h:40-48,140-145,174-185
  *   tick order      src/tracing/process.cpp:431-612
  * with coordinates in planar metres (geography=false predicates) and the
@@ -29,12 +31,21 @@
 extern "C" {
 #endif
 
+/* Tree maintenance between ticks (config.h:35 dynamic_schema):
+ *   STATIC       the tree built from the first tick is kept (the reference's
+ *                CPU path with dynamic_schema off, process.cpp:608)
+ *   DYNAMIC      split and merge after every tick (geometry.cu:1114-1234)
+ *   GRID_REBUILD the tree rebuilt from a 16384^2 cell histogram every tick
+ *                (the reference's GPU path with dynamic_schema off,
+ *                geometry.cu:1629-1645) */
+enum { ORACLE_SCHEMA_STATIC = 0, ORACLE_SCHEMA_DYNAMIC = 1, ORACLE_SCHEMA_GRID_REBUILD = 2 };
+
 typedef struct oracle_config {
     uint32_t num_objects;
     uint32_t grid_capacity;        /* config.h:29 */
     uint32_t min_meet_time;        /* config.h:39 */
     uint32_t schema_update_delay;  /* config.h:38 */
-    uint32_t dynamic_schema;       /* config.h:35 */
+    uint32_t dynamic_schema;       /* config.h:35: ORACLE_SCHEMA_* */
     uint32_t complete_mode;        /* 1: every pair within r, no top/right rule */
     uint32_t num_threads;          /* 0: OpenMP default */
     uint32_t start_time;           /* first tick number, config.h:28 */
@@ -90,6 +101,10 @@ uint64_t oracle_get_leaves(void *h, double *boxes, uint32_t *levels, uint32_t *c
 
 /* Build the tree from xy without running a tick (tests). */
 int oracle_build(void *h, const double *xy);
+
+/* Cell x + 16384*y of every point for the grid rebuild (geometry.cu:1276-
+ * 1278); returns -2 if a point has no cell (on the high y edge). */
+int oracle_grid_cells(const oracle_config *cfg, const double *xy, uint32_t *cell);
 
 /* Descend one point: returns the depth-first index of its leaf, writes the
  * range [lo, hi) of depth-first leaf indices under last_valid and the

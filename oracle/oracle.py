@@ -13,6 +13,9 @@ import numpy as np
 
 HERE = Path(__file__).resolve().parent
 
+# tree maintenance between ticks (oracle.h ORACLE_SCHEMA_*)
+SCHEMA_STATIC, SCHEMA_DYNAMIC, SCHEMA_GRID_REBUILD = 0, 1, 2
+
 
 class OracleConfig(ctypes.Structure):
     _fields_ = [
@@ -80,6 +83,8 @@ def lib() -> ctypes.CDLL:
         L.oracle_destroy.argtypes = [vp]
         L.oracle_tick.argtypes = [vp, ctypes.c_void_p, ctypes.c_uint32]
         L.oracle_build.argtypes = [vp, ctypes.c_void_p]
+        L.oracle_grid_cells.argtypes = [ctypes.POINTER(OracleConfig), ctypes.c_void_p, ctypes.c_void_p]
+        L.oracle_grid_cells.restype = ctypes.c_int
         for f in ("oracle_num_contacts", "oracle_num_meetings", "oracle_num_active", "oracle_num_leaves"):
             getattr(L, f).argtypes = [vp]
             getattr(L, f).restype = ctypes.c_uint64
@@ -121,7 +126,7 @@ class Oracle:
     """One deterministic reference-semantics run (partition -> refine -> evaluate)."""
 
     def __init__(self, num_objects: int, box, reach_distance: float = 2.0, grid_capacity: int = 200,
-                 min_meet_time: int = 5, schema_update_delay: int = 1, dynamic_schema: bool = True,
+                 min_meet_time: int = 5, schema_update_delay: int = 1, dynamic_schema: int = 1,
                  complete: bool = False, num_threads: int = 0, start_time: int = 0):
         lx, ly, hx, hy = box
         self.cfg = OracleConfig(num_objects, grid_capacity, min_meet_time, schema_update_delay,
@@ -142,6 +147,14 @@ class Oracle:
             self.close()
         except Exception:
             pass
+
+    def grid_cells(self, xy: np.ndarray) -> np.ndarray:
+        """Cell x + 16384*y of every point, as the grid rebuild takes them."""
+        xy = np.ascontiguousarray(xy, dtype=np.float64)
+        cell = np.zeros(self.n, dtype=np.uint32)
+        if lib().oracle_grid_cells(ctypes.byref(self.cfg), xy.ctypes.data, cell.ctypes.data):
+            raise ValueError("a point has no cell")
+        return cell
 
     def tick(self, xy: np.ndarray, t: int) -> None:
         xy = np.ascontiguousarray(xy, dtype=np.float64)

@@ -13,15 +13,19 @@ def keys_of(meetings: np.ndarray) -> np.ndarray:
 
 
 def run_parity(w: gen.Workload, ticks: int, *, check_contacts: bool = True, check_leaves_every: int = 10,
-               complete: bool = False, precision: int = runtime.PRECISION_FP64, **cfg_kw) -> dict:
+               complete: bool = False, precision: int = runtime.PRECISION_FP64,
+               dynamic_schema: int = runtime.SCHEMA_DYNAMIC, box=None, **cfg_kw) -> dict:
     """Runs both paths for `ticks` ticks and asserts identical per-tick contact
     sets, meeting sets and leaf boxes.  Returns totals for the caller."""
     g = gen.Generator(w)
-    o = Oracle(w.num_objects, w.box(), w.reach_distance, w.grid_capacity, w.min_meet_time, complete=complete)
+    box = box or w.box()
+    o = Oracle(w.num_objects, box, w.reach_distance, w.grid_capacity, w.min_meet_time, complete=complete,
+               dynamic_schema=dynamic_schema)
     kw = dict(reach_distance=w.reach_distance, grid_capacity=w.grid_capacity, min_meet_time=w.min_meet_time,
-              complete=complete, precision=precision, record_contacts=check_contacts)
+              complete=complete, precision=precision, record_contacts=check_contacts,
+              dynamic_schema=dynamic_schema)
     kw.update(cfg_kw)
-    cfg = runtime.make_config(w.num_objects, w.box(), **kw)
+    cfg = runtime.make_config(w.num_objects, box, **kw)
     totals = {"meetings": 0, "contacts": 0}
     with runtime.Workbench(cfg) as wb:
         for t in range(ticks):

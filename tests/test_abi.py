@@ -47,7 +47,8 @@ def test_config_defaults_follow_the_reference():
 
 
 @pytest.mark.parametrize("field,value", [("num_objects", 0), ("grid_capacity", 0), ("reach_distance", -1.0),
-                                         ("high_x", -5.0), ("candidate_mode", 7), ("precision", 9)])
+                                         ("high_x", -5.0), ("candidate_mode", 7), ("precision", 9),
+                                         ("dynamic_schema", 3)])
 def test_invalid_configuration_is_rejected(field, value):
     c = runtime.make_config(100, (0, 0, 100, 100))
     setattr(c, field, value)
