@@ -105,9 +105,11 @@ __global__ void k_fill_u32(uint32_t *p, uint32_t v, uint64_t n)
 }
 
 __global__ void k_set_tick(TickParams *tp, const double *xy, TableGen tab_old, TableGen tab_cur, float4 *mbr_old,
-                           float4 *mbr_cur, double *xy_prev, double *xy_save, uint32_t tick, uint32_t prev_tick,
-                           uint32_t has_prev)
+                           float4 *mbr_cur, double *xy_prev, double *xy_save, uint4 *cand_prev, uint4 *cand,
+                           uint32_t tick, uint32_t prev_tick, uint32_t has_prev)
 {
+    tp->cand = cand;
+    tp->cand_prev = cand_prev;
     tp->xy = xy;
     tp->tab_old = tab_old;
     tp->tab_cur = tab_cur;
@@ -144,14 +146,20 @@ __global__ void k_advance_tick(TickParams *tp, uint64_t stride)
     double *h = tp->xy_prev;
     tp->xy_prev = tp->xy_save;
     tp->xy_save = h;
+    uint4 *cb = tp->cand_prev;
+    tp->cand_prev = tp->cand;
+    tp->cand = cb;
 }
 
 // State before the first tick of a run, one k_advance_tick short of it; the
 // positions already point at the first snapshot for the initial build.
 __global__ void k_begin_run(TickParams *tp, const double *base, TableGen tab_old, TableGen tab_cur, float4 *mbr_old,
-                            float4 *mbr_cur, double *xy_prev, double *xy_save, uint32_t tick, uint32_t next_tick,
-                            uint32_t has_prev_next, unsigned long long *stamps, uint32_t nstamps)
+                            float4 *mbr_cur, double *xy_prev, double *xy_save, uint4 *cand_prev, uint4 *cand,
+                            uint32_t tick, uint32_t next_tick, uint32_t has_prev_next, unsigned long long *stamps,
+                            uint32_t nstamps)
 {
+    tp->cand = cand;
+    tp->cand_prev = cand_prev;
     tp->trace_base = base;
     tp->trace_index = 0xFFFFFFFFu;
     tp->xy = base;
@@ -796,10 +804,6 @@ __global__ __launch_bounds__(kBlock) void k_assign(DevState s)
             double x = xy[2 * (size_t)i], y = xy[2 * (size_t)i + 1];
             pleaf = s.leaf_of[i];
             s.prev_leaf[i] = pleaf;
-            if (!s.store_all) {
-                s.prev_cand[i] = s.cand[i];
-                s.cand[i] = make_uint4(kNone, kNone, kNone, 0u);
-            }
             if (!(x >= s.lx && x <= s.hx && y >= s.ly && y <= s.hy)) {
                 atomicOr(&s.ctr->error, kErrBadInput);
                 s.leaf_of[i] = kNone;
@@ -1030,7 +1034,7 @@ __device__ __forceinline__ uint32_t wave_exclusive_scan(uint32_t v, uint32_t *to
 // Units (pid, leaf) of one object written at once: used when its candidate
 // list fills up (deep trees of the grid rebuild can put many leaves within
 // reach), the common case goes through the wave-aggregated path below.
-__device__ __noinline__ void emit_units(const DevState &s, uint32_t pid, const uint32_t *cand, uint32_t n)
+__device__ __forceinline__ void emit_units(const DevState &s, uint32_t pid, const uint32_t *cand, uint32_t n)
 {
     const unsigned long long u0 = atomicAdd(&s.ctr->n_units, (unsigned long long)n);
     for (uint32_t j = 0; j < n; j++) {
@@ -1096,7 +1100,7 @@ __global__ __launch_bounds__(kBlock) void k_lookup(DevState s)
                 }
             }
         }
-        if (k < nb && !s.store_all) s.cand[pid] = make_uint4(f0, f1, f2, total);
+        if (k < nb && !s.store_all) s.tp->cand[pid] = make_uint4(f0, f1, total > 3 ? kCandOverflow : f2, s.tp->tick);
         // one atomic per wave reserves the wave's units
         uint32_t wtotal;
         uint32_t wex = wave_exclusive_scan(nc, &wtotal);
@@ -1174,8 +1178,9 @@ __device__ __forceinline__ void stage_push(Stager &st, bool hit, uint32_t e)
     st.n += (uint32_t)__popcll(m);
 }
 
-// Writes the staged entries out: key(e) gives the contact-list key of entry
-// e, or kEmptyKey for an entry that is not kept; kept keys are packed.
+// Writes the staged entries out: key(e, valid) gives the contact-list key of
+// entry e, or kEmptyKey for an entry that is not kept (and for !valid);
+// kept keys are packed.
 template <class Key>
 __device__ __forceinline__ void stage_flush(Stager &st, const DevState &s, Key key)
 {
@@ -1184,7 +1189,8 @@ __device__ __forceinline__ void stage_flush(Stager &st, const DevState &s, Key k
     const uint32_t lane = lane_id();
     for (uint32_t done = 0; done < st.n; done += 64) {
         const uint32_t i = done + lane;
-        const unsigned long long k = i < st.n ? key(st.buf[i], true) : kEmptyKey;
+        // every lane calls key (it may exchange registers across the wave)
+        const unsigned long long k = key(i < st.n ? st.buf[i] : 0u, i < st.n);
         const bool keep = k != kEmptyKey;
         const unsigned long long m = __ballot(keep);
         const uint32_t cnt = (uint32_t)__popcll(m);
@@ -1225,14 +1231,34 @@ __device__ __forceinline__ unsigned long long pair_key(uint32_t a, uint32_t b)
     return ((unsigned long long)lo << 32) | hi;
 }
 
+// Candidate list of object pid as the join tests it: up to three leaves and
+// w = 0, or w = 4 when the lookup had more (an entry not written at `stamp`
+// is empty).
+__device__ __forceinline__ uint4 cand_read(const uint4 *buf, uint32_t pid, uint32_t stamp)
+{
+    const uint4 c = buf[pid];
+    if (c.w != stamp) return make_uint4(kNone, kNone, kNone, 0u);
+    return c.z == kCandOverflow ? make_uint4(kNone, kNone, kNone, 4u) : make_uint4(c.x, c.y, c.z, 0u);
+}
+
+// Positions of a tile object: FP64 for a leaf joined in FP64, else the FP32
+// offsets from the leaf's low corner at tick and at prev_tick (NaN: not
+// bounded).  The FP64 positions of a guard-band leaf stay in HBM (sx, sy),
+// read only for the rare band re-test.
+struct TileF {
+    float2 f, pf;
+};
+
 template <int TILE>
 struct JoinTile {
-    double2 xy[TILE];              // positions at tick
+    union {
+        double2 xy[TILE];
+        TileF fp[TILE];
+    };
     uint32_t g[TILE];              // index in the sorted arrays (positions at prev_tick: sxp, syp)
     uint32_t pl[TILE];             // leaf at prev_tick
-    float2 f[TILE];                // FP32 offsets from the leaf's low corner (guard mode)
-    float2 pf[TILE];               // the same for prev_tick (NaN: not bounded)
     uint32_t p[TILE];              // object ids
+    uint4 pc[TILE];                // candidate leaves at prev_tick (DevState::prev_cand)
 };
 
 // The row objects of a wave's current task (read when its contacts are
@@ -1244,19 +1270,29 @@ struct JoinRows {
     uint32_t ppl[64];   // the object's leaf at prev_tick
 };
 
+// Tile slot k <- sorted object `from` of a leaf joined in FP32 (use32) or FP64.
+template <int TILE>
+__device__ __forceinline__ void tile_put(JoinTile<TILE> &t, uint32_t k, const DevState &s, uint32_t from,
+                                         bool has_prev, bool use32, bool want_pc)
+{
+    if (use32)
+        t.fp[k] = TileF{make_float2(s.fx[from], s.fy[from]),
+                        has_prev ? make_float2(s.pfx[from], s.pfy[from]) : make_float2(0.f, 0.f)};
+    else
+        t.xy[k] = make_double2(s.sx[from], s.sy[from]);
+    t.g[k] = from;
+    t.pl[k] = s.splf[from];
+    const uint32_t pid = s.spid[from];
+    t.p[k] = pid;
+    if (want_pc) t.pc[k] = cand_read(s.tp->cand_prev, pid, s.tp->prev_tick);
+}
 
 template <int TILE>
 __device__ __forceinline__ void load_tile(JoinTile<TILE> &t, const DevState &s, uint32_t from, uint32_t n, bool has_prev,
                                           bool use32)
 {
-    for (uint32_t k = threadIdx.x; k < n; k += kBlock) {
-        t.xy[k] = make_double2(s.sx[from + k], s.sy[from + k]);
-        t.g[k] = from + k;
-        t.pl[k] = s.splf[from + k];
-        t.f[k] = use32 ? make_float2(s.fx[from + k], s.fy[from + k]) : make_float2(0.f, 0.f);
-        if (use32 && has_prev) t.pf[k] = make_float2(s.pfx[from + k], s.pfy[from + k]);
-        t.p[k] = s.spid[from + k];
-    }
+    const bool want_pc = has_prev && !s.store_all;
+    for (uint32_t k = threadIdx.x; k < n; k += kBlock) tile_put(t, k, s, from + k, has_prev, use32, want_pc);
 }
 
 // Stage entries: row lane << 16 | band << 15 | column.  A band entry is a
@@ -1269,8 +1305,11 @@ constexpr uint32_t kBandBit = 1u << 15;
 // In guard mode (USE32) pairs are decided on the FP32 offsets: inside
 // r - eps a hit, past r + eps a miss, in between a band entry.  Sure hits
 // are counted here; a sure inside hit is staged only if it may be stored:
-// when `keep_all`, or when the FP32 offsets at prev_tick do not place the
-// pair surely out of reach then (the exact test follows at write-out).
+// when every contact is kept, or when the FP32 offsets at prev_tick do not
+// place the pair surely out of reach then (the exact test follows at
+// write-out).  Every border hit is staged: untagged ones are needed to count
+// the pairs that both lookups find (settling them in the loop instead costs
+// a second LDS read per column and measured slower).
 struct JoinCounts {
     uint32_t sure;   // sure contacts (wave-uniform)
     uint32_t pass;   // band entries within reach in FP64 (per lane)
@@ -1278,29 +1317,38 @@ struct JoinCounts {
     uint32_t dup;    // pairs found twice and not stored, counted once (per lane)
 };
 
-template <bool USE32, int TILE>
+// Register x of lane `src` (every lane of the wave must take part).
+__device__ __forceinline__ uint4 lane_read4(uint4 x, uint32_t src)
+{
+    return make_uint4((uint32_t)__shfl((int)x.x, (int)src), (uint32_t)__shfl((int)x.y, (int)src),
+                      (uint32_t)__shfl((int)x.z, (int)src), (uint32_t)__shfl((int)x.w, (int)src));
+}
+
+template <bool USE32, bool BORDER, int TILE>
 __device__ __forceinline__ JoinCounts join_columns(Stager &st, const DevState &s, const JoinTile<TILE> &C, double xi,
                                                    double yi, float fxi, float fyi, float pfxi, float pfyi,
                                                    uint32_t jmin, uint32_t jbeg, uint32_t jend, float t32_in,
-                                                   float t32_out, double t64, bool keep_all, const JoinRows &rows,
-                                                   unsigned long long flag, bool has_prev, const uint4 *ccand,
-                                                   uint32_t leaf)
+                                                   float t32_out, double t64, const JoinRows &rows,
+                                                   bool has_prev, const uint4 *ccand, uint32_t leaf, uint4 rpc)
 {
     uint32_t n_sure = 0, n_pass = 0, band = 0, n_dup = 0;
     const uint32_t tag = lane_id() << 16;
+    const unsigned long long flag = BORDER ? kTagBorder : 0ull;
+    const bool tested_check = has_prev && !s.store_all;   // wave-uniform
     // the stage's contacts as keys (kEmptyKey: a band entry that is not
-    // within reach in FP64, or a contact that is not stored); `counted`
-    // entries add to the recheck counters
-    auto key_of = [&](uint32_t e, bool counted) {
+    // within reach in FP64, or a contact that is not stored)
+    auto key_of = [&](uint32_t e, bool valid) {
         const uint32_t rl = e >> 16, j = e & 0x7FFFu;
+        uint4 ca = make_uint4(kNone, kNone, kNone, 0u);
+        if (tested_check) ca = lane_read4(rpc, rl);   // the row object's candidates at prev_tick
+        if (!valid) return kEmptyKey;
         if (USE32 && (e & kBandBit)) {
-            band += counted;
-            const uint32_t pa = rows.p[rl];
-            const double2 b = C.xy[j];
-            const double ex = s.tp->xy[2 * (size_t)pa] - b.x;
-            const double ey = s.tp->xy[2 * (size_t)pa + 1] - b.y;
+            band++;
+            const uint32_t pa = rows.p[rl], gj = C.g[j];
+            const double ex = s.tp->xy[2 * (size_t)pa] - s.sx[gj];
+            const double ey = s.tp->xy[2 * (size_t)pa + 1] - s.sy[gj];
             if (!(ex * ex + ey * ey <= t64)) return kEmptyKey;
-            n_pass += counted;
+            n_pass++;
         }
         unsigned long long k = pair_key(rows.p[rl], C.p[j]) | flag;
         if (has_prev) {
@@ -1308,7 +1356,7 @@ __device__ __forceinline__ JoinCounts join_columns(Stager &st, const DevState &s
             // the guard band, else by the FP64 test
             int dec = -1;
             if (USE32) {
-                const float2 a = rows.pf[rl], b = C.pf[j];
+                const float2 a = rows.pf[rl], b = C.fp[j].pf;
                 const float px = a.x - b.x;
                 const float py = a.y - b.y;
                 const float p32 = px * px + py * py;
@@ -1323,21 +1371,21 @@ __device__ __forceinline__ JoinCounts join_columns(Stager &st, const DevState &s
             }
             if (dec) k |= kTagPrev;
         }
-        if ((k & kTagPrev) && !s.store_all) {
+        if ((k & kTagPrev) && tested_check) {
             // tested by the join of the tick before: both objects in one leaf
             // then, or one's lookup selected the other's leaf (candidate
             // lists that overflowed stored the pair, so they do not count)
             const uint32_t la = rows.ppl[rl], lb = C.pl[j];
             bool tested = la == lb;
             if (!tested) {
-                const uint4 ca = s.prev_cand[rows.p[rl]], cb = s.prev_cand[C.p[j]];
+                const uint4 cb = C.pc[j];
                 tested = ca.w <= 3 && cb.w <= 3 &&
                          (ca.x == lb || ca.y == lb || ca.z == lb || cb.x == la || cb.y == la || cb.z == la);
             }
             if (tested) k |= kTagTested;
         }
         if (s.store_all || (k & kTagPrev)) return k;
-        if (!(k & kTagBorder)) return kEmptyKey;   // untagged inside contact: not stored
+        if (!BORDER) return kEmptyKey;   // untagged inside contact: not stored
         // Untagged border contact (row object p's lookup selected this leaf):
         // not stored unless a candidate list overflowed.  It is found twice
         // iff q's lookup selected p's leaf as well; that pair is counted as
@@ -1347,22 +1395,19 @@ __device__ __forceinline__ JoinCounts join_columns(Stager &st, const DevState &s
         if ((pl >> 31) || qc.w > 3) return k;
         const uint32_t lp = pl & 0x7FFFFFFFu;
         const bool twice = qc.x == lp || qc.y == lp || qc.z == lp;
-        n_dup += counted && twice && lp < leaf;
+        n_dup += twice && lp < leaf;
         return kEmptyKey;
     };
-    const bool filt = !keep_all && has_prev;   // sure inside hits may be skipped
     for (uint32_t jb = jbeg; jb < jend; jb += kFlushRound) {
-        float2 cf[kFlushRound], cpf[kFlushRound];
+        TileF cfp[kFlushRound];
         double2 cx[kFlushRound];
 #pragma unroll
         for (uint32_t jj = 0; jj < kFlushRound; jj++) {
             const uint32_t jc = min(jb + jj, jend - 1);
-            if constexpr (USE32) {
-                cf[jj] = C.f[jc];
-                cpf[jj] = C.pf[jc];
-            } else {
+            if constexpr (USE32)
+                cfp[jj] = C.fp[jc];
+            else
                 cx[jj] = C.xy[jc];
-            }
         }
 #pragma unroll
         for (uint32_t jj = 0; jj < kFlushRound; jj++) {
@@ -1371,22 +1416,23 @@ __device__ __forceinline__ JoinCounts join_columns(Stager &st, const DevState &s
             bool sure, stage;
             uint32_t e = tag | j;
             if constexpr (USE32) {
-                const float dx = fxi - cf[jj].x;
-                const float dy = fyi - cf[jj].y;
+                const float dx = fxi - cfp[jj].f.x;
+                const float dy = fyi - cfp[jj].f.y;
                 const float s32 = dx * dx + dy * dy;
                 sure = cand && s32 <= t32_in;
                 const bool hit = cand && s32 <= t32_out;
                 if (!sure) e |= kBandBit;
-                const float px = pfxi - cpf[jj].x;
-                const float py = pfyi - cpf[jj].y;
+                const float px = pfxi - cfp[jj].pf.x;
+                const float py = pfyi - cfp[jj].pf.y;
                 const float p32 = px * px + py * py;
-                // NaN (unbounded offsets) compares false: staged
-                stage = hit && (!sure || keep_all || (filt && !(p32 > t32_out)));
+                // NaN (unbounded offsets) compares false: may be tagged
+                const bool maybe_tagged = has_prev && !(p32 > t32_out);
+                stage = hit && (!sure || BORDER || s.store_all || maybe_tagged);
             } else {
                 const double ex = xi - cx[jj].x;
                 const double ey = yi - cx[jj].y;
                 sure = cand && ex * ex + ey * ey <= t64;
-                stage = sure && (keep_all || has_prev);
+                stage = sure && (BORDER || s.store_all || has_prev);
             }
             n_sure += (uint32_t)__popcll(__ballot(sure));
             stage_push(st, stage, e);
@@ -1470,29 +1516,29 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
     auto run = [&](const JoinTile<TILE> &A, const JoinTile<TILE> &C, bool inside, uint32_t row, uint32_t pid,
                    bool vi, uint32_t jbeg, uint32_t jend, uint32_t jmin, bool use32, double lx, double ly,
                    uint32_t leaf) {
-        double xi = 0.0, yi = 0.0, pxi = 0.0, pyi = 0.0;
+        double xi = 0.0, yi = 0.0;
         float fxi = 0.f, fyi = 0.f, pfxi = 0.f, pfyi = 0.f;
         uint32_t pi = 0;
+        uint4 rpc = make_uint4(kNone, kNone, kNone, 0u);   // the row's candidates at prev_tick
+        const bool want_pc = has_prev && !s.store_all;
         if (inside) {
             const uint32_t rr = vi ? row : jbeg;
-            const double2 a = A.xy[rr];
-            const float2 af = A.f[rr];
-            xi = a.x;
-            yi = a.y;
-            if (has_prev) {
-                pxi = s.sxp[A.g[rr]];
-                pyi = s.syp[A.g[rr]];
-            }
-            fxi = af.x;
-            fyi = af.y;
-            if (use32 && has_prev) {
-                const float2 apf = A.pf[rr];
-                pfxi = apf.x;
-                pfyi = apf.y;
+            if (use32) {
+                const TileF af = A.fp[rr];
+                fxi = af.f.x;
+                fyi = af.f.y;
+                pfxi = af.pf.x;
+                pfyi = af.pf.y;
+            } else {
+                const double2 a = A.xy[rr];
+                xi = a.x;
+                yi = a.y;
             }
             pi = A.p[rr];
+            if (want_pc) rpc = A.pc[rr];
         } else {
             pi = pid;
+            double pxi = 0.0, pyi = 0.0;
             if (vi) {
                 xi = xy[2 * (size_t)pi];
                 yi = xy[2 * (size_t)pi + 1];
@@ -1500,6 +1546,7 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
                     pxi = xy_prev[2 * (size_t)pi];
                     pyi = xy_prev[2 * (size_t)pi + 1];
                 }
+                if (want_pc) rpc = cand_read(s.tp->cand_prev, pi, s.tp->prev_tick);
             }
             if (use32) {
                 fxi = (float)(xi - lx);
@@ -1514,17 +1561,22 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
         if (!vi) jmin = 0xFFFFFFFFu;   // an idle lane
         rows.p[lane] = pi;
         rows.pf[lane] = make_float2(pfxi, pfyi);
-        if (has_prev && !s.store_all)
-            rows.ppl[lane] = inside ? A.pl[vi ? row : jbeg] : (vi ? s.prev_leaf[pi] : kNone);
+        if (want_pc) rows.ppl[lane] = inside ? A.pl[vi ? row : jbeg] : (vi ? s.prev_leaf[pi] : kNone);
         if (!inside && !s.store_all && vi)
-            rows.pl[lane] = s.leaf_of[pi] | (s.cand[pi].w > 3 ? 0x80000000u : 0u);
-        const unsigned long long flag = inside ? 0ull : kTagBorder;
-        const bool keep_all = !inside || s.store_all;
+            rows.pl[lane] = s.leaf_of[pi] | (cand_read(s.tp->cand, pi, s.tp->tick).w > 3 ? 0x80000000u : 0u);
         const JoinCounts jc =
-            use32 ? join_columns<true>(st, s, C, xi, yi, fxi, fyi, pfxi, pfyi, jmin, jbeg, jend, s.t32_in, s.t32_out,
-                                       t64, keep_all, rows, flag, has_prev, lds.ccand, leaf)
-                  : join_columns<false>(st, s, C, xi, yi, fxi, fyi, pfxi, pfyi, jmin, jbeg, jend, s.t32_in,
-                                        s.t32_out, t64, keep_all, rows, flag, has_prev, lds.ccand, leaf);
+            use32 ? (inside ? join_columns<true, false>(st, s, C, xi, yi, fxi, fyi, pfxi, pfyi, jmin, jbeg, jend,
+                                                        s.t32_in, s.t32_out, t64, rows, has_prev, lds.ccand, leaf,
+                                                        rpc)
+                            : join_columns<true, true>(st, s, C, xi, yi, fxi, fyi, pfxi, pfyi, jmin, jbeg, jend,
+                                                       s.t32_in, s.t32_out, t64, rows, has_prev, lds.ccand, leaf,
+                                                       rpc))
+                  : (inside ? join_columns<false, false>(st, s, C, xi, yi, fxi, fyi, pfxi, pfyi, jmin, jbeg, jend,
+                                                         s.t32_in, s.t32_out, t64, rows, has_prev, lds.ccand, leaf,
+                                                         rpc)
+                            : join_columns<false, true>(st, s, C, xi, yi, fxi, fyi, pfxi, pfyi, jmin, jbeg, jend,
+                                                        s.t32_in, s.t32_out, t64, rows, has_prev, lds.ccand, leaf,
+                                                        rpc));
         n_in += inside ? jc.sure : 0u;
         n_bd += inside ? 0u : jc.sure;
         pass_in += inside ? jc.pass : 0u;
@@ -1595,7 +1647,7 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
                     load_tile(lds.a, s, F.off + I * TILE, ni, has_prev, F.use32);
                     if (!s.store_all)
                         for (uint32_t k = threadIdx.x; k < ni; k += kBlock)
-                            lds.ccand[k] = s.cand[s.spid[F.off + I * TILE + k]];
+                            lds.ccand[k] = cand_read(s.tp->cand, s.spid[F.off + I * TILE + k], s.tp->tick);
                     for (uint32_t J = I; J < ntile; J++) {
                         const uint32_t nj = min((uint32_t)TILE, F.cnt - J * TILE);
                         const bool diag = (J == I);
@@ -1638,15 +1690,8 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
                     while (k + 1 < k1 && lds.leaf[k + 1].base <= i) k++;
                     const JoinLeaf &L = lds.leaf[k];
                     const uint32_t from = L.off + (i - L.base);
-                    JoinTile<TILE> &T = lds.a;
-                    T.xy[i] = make_double2(s.sx[from], s.sy[from]);
-                    T.g[i] = from;
-                    T.pl[i] = s.splf[from];
-                    T.f[i] = L.use32 ? make_float2(s.fx[from], s.fy[from]) : make_float2(0.f, 0.f);
-                    if (L.use32 && has_prev) T.pf[i] = make_float2(s.pfx[from], s.pfy[from]);
-                    const uint32_t pid = s.spid[from];
-                    T.p[i] = pid;
-                    if (!s.store_all) lds.ccand[i] = s.cand[pid];
+                    tile_put(lds.a, i, s, from, has_prev, L.use32, has_prev && !s.store_all);
+                    if (!s.store_all) lds.ccand[i] = cand_read(s.tp->cand, lds.a.p[i], s.tp->tick);
                 }
                 __syncthreads();
                 for (uint32_t t = wave; t < ntask; t += kBlock / 64) {
@@ -2294,6 +2339,7 @@ struct glint_ctx {
     KvEngine *kv = nullptr;                 // sorted key-value output (null when off)
     double *hist[2] = {nullptr, nullptr};   // positions of the last two ticks (generation parity)
     int cur_gen = 0;            // table generation filled at the current tick
+    uint4 *cand[2] = {};        // candidate-list buffers: cand[cur_gen] is written at the current tick
     bool built = false;
     bool has_positions = false;
     bool partitioned = false;
@@ -2416,8 +2462,8 @@ void set_tick_params(glint_ctx *c, const double *xy)
 {
     const int g = c->cur_gen;
     k_set_tick<<<1, 1, 0, c->stream>>>(c->s.tp, xy, c->tab[1 - g], c->tab[g], c->mbr[1 - g], c->mbr[g],
-                                       c->hist[1 - g], c->hist[g], c->tick, c->has_prev ? c->prev_tick : c->tick,
-                                       c->has_prev ? 1u : 0u);
+                                       c->hist[1 - g], c->hist[g], c->cand[1 - g], c->cand[g], c->tick,
+                                       c->has_prev ? c->prev_tick : c->tick, c->has_prev ? 1u : 0u);
 }
 
 // Launch through the profiler: HIP events around the kernel when enabled.
@@ -2820,6 +2866,12 @@ glint_status glint_create(const glint_config *cfg, glint_ctx **out)
     if (!s.store_all) {
         ALLOC(s.cand, N);
         ALLOC(s.prev_cand, N);
+        c->cand[0] = s.cand;
+        c->cand[1] = s.prev_cand;
+        // empty lists stamped with no tick
+        k_fill_u32<<<grid_for(4 * N), kBlock, 0, c->stream>>>(reinterpret_cast<uint32_t *>(s.cand), kNone, 4 * N);
+        k_fill_u32<<<grid_for(4 * N), kBlock, 0, c->stream>>>(reinterpret_cast<uint32_t *>(s.prev_cand), kNone,
+                                                             4 * N);
     }
     ALLOC(s.border, N);
     ALLOC(s.units, s.unit_cap);
@@ -3158,7 +3210,7 @@ static glint_status run_begin(glint_ctx *c, const double *d_trace, uint32_t firs
     const int g = c->cur_gen;
     // one advance short of the first tick: tables and histories pre-swapped
     k_begin_run<<<1, 1, 0, c->stream>>>(c->s.tp, d_trace, c->tab[g], c->tab[1 - g], c->mbr[g], c->mbr[1 - g],
-                                        c->hist[g], c->hist[1 - g],
+                                        c->hist[g], c->hist[1 - g], c->cand[g], c->cand[1 - g],
                                         c->prev_tick, first_tick, c->has_prev ? 1u : 0u, c->stamps, 2 * nticks);
     HIP_TRY(c, hipGetLastError());
     if (!c->built) {

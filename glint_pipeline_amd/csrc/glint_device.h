@@ -28,6 +28,7 @@ constexpr uint32_t kLeafTag = 0xFFFFFFFFu;      // child[0] of a leaf
 constexpr uint32_t kNone = 0xFFFFFFFFu;
 constexpr unsigned long long kEmptyKey = ~0ull; // (a<<32)|b with a<b is never all ones
 constexpr uint32_t kDupStart = 0xFFFFFFFFu;     // start of the second copy of a pair found twice
+constexpr uint32_t kCandOverflow = 0xFFFFFFFEu; // third candidate slot of a lookup with more than three
 // Flags carried in bits of a contact's key that pids (below 2^30) leave free:
 constexpr unsigned long long kTagPrev = 1ull << 63;     // the pair was within reach at the tick before
 constexpr unsigned long long kTagBorder = 1ull << 62;   // found through a border unit (may be found twice)
@@ -138,6 +139,8 @@ struct TickParams {
     float4 *mbr_old;            // boxes of the pairs in tab_old, by entry (key-value output only)
     float4 *mbr_cur;            // boxes of the pairs in tab_cur
     double *xy_save;            // receives this tick's positions for the next tick
+    uint4 *cand;                // candidate lists written at tick (see DevState::cand)
+    uint4 *cand_prev;           // those written at prev_tick
     unsigned long long *stamps; // run_ticks: device wall clock at the end of every tick
     uint32_t nstamps;
     uint32_t tick;
@@ -180,8 +183,12 @@ struct DevState {
     uint32_t *leaf_list;
     uint32_t *leaf_of;
     uint32_t *prev_leaf;    // leaf of every object at the tick before (pid order)
-    uint4 *cand;            // candidate leaves of every object's lookup at this tick: up to 3, and their count
-    uint4 *prev_cand;       // the same at the tick before
+    // Candidate leaves of the lookups, pid order, in two buffers taken in turn
+    // (TickParams::cand, cand_prev): up to three leaves (the third is
+    // kCandOverflow when there were more) and the tick that wrote them, so
+    // an entry not written at the tick read is empty and nothing is reset.
+    uint4 *cand;
+    uint4 *prev_cand;
     uint2 *border;
     uint2 *units;           // (object, candidate leaf) in lookup order
     uint32_t *urank;        // rank of the unit among the units of its leaf
