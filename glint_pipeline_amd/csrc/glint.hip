@@ -117,6 +117,8 @@ __global__ void k_set_tick(TickParams *tp, const double *xy, TableGen tab_old, T
     tp->mbr_cur = mbr_cur;
     tp->xy_prev = xy_prev;
     tp->xy_save = xy_save;
+    tp->hist_prev = xy_prev;
+    tp->hist_save = xy_save;
     tp->tick = tick;
     tp->prev_tick = prev_tick;
     tp->has_prev = has_prev;
@@ -125,7 +127,10 @@ __global__ void k_set_tick(TickParams *tp, const double *xy, TableGen tab_old, T
 // Next tick of a device-resident trace: positions move one snapshot on and
 // the generations swap (this tick's table and positions become the previous
 // ones).  Every tick of a run starts with it, so a run of ticks is a fixed
-// sequence of launches that can be captured in a graph and replayed.
+// sequence of launches that can be captured in a graph and replayed.  Inside
+// a run the positions at prev_tick are the trace's previous snapshot; only
+// the run's last tick copies its positions into the history buffer, for the
+// tick after the run.
 __global__ void k_advance_tick(TickParams *tp, uint64_t stride)
 {
     uint32_t k = tp->trace_index + 1;
@@ -143,9 +148,11 @@ __global__ void k_advance_tick(TickParams *tp, uint64_t stride)
     float4 *m = tp->mbr_old;
     tp->mbr_old = tp->mbr_cur;
     tp->mbr_cur = m;
-    double *h = tp->xy_prev;
-    tp->xy_prev = tp->xy_save;
-    tp->xy_save = h;
+    double *h = tp->hist_prev;
+    tp->hist_prev = tp->hist_save;
+    tp->hist_save = h;
+    tp->xy_prev = k == 0 ? tp->hist_prev : tp->trace_base + stride * (k - 1);
+    tp->xy_save = k + 1 == tp->nticks ? tp->hist_save : nullptr;
     uint4 *cb = tp->cand_prev;
     tp->cand_prev = tp->cand;
     tp->cand = cb;
@@ -156,8 +163,9 @@ __global__ void k_advance_tick(TickParams *tp, uint64_t stride)
 __global__ void k_begin_run(TickParams *tp, const double *base, TableGen tab_old, TableGen tab_cur, float4 *mbr_old,
                             float4 *mbr_cur, double *xy_prev, double *xy_save, uint4 *cand_prev, uint4 *cand,
                             uint32_t tick, uint32_t next_tick, uint32_t has_prev_next, unsigned long long *stamps,
-                            uint32_t nstamps)
+                            uint32_t nstamps, uint32_t nticks)
 {
+    tp->nticks = nticks;
     tp->cand = cand;
     tp->cand_prev = cand_prev;
     tp->trace_base = base;
@@ -169,6 +177,8 @@ __global__ void k_begin_run(TickParams *tp, const double *base, TableGen tab_old
     tp->mbr_cur = mbr_cur;
     tp->xy_prev = xy_prev;
     tp->xy_save = xy_save;
+    tp->hist_prev = xy_prev;
+    tp->hist_save = xy_save;
     tp->tick = tick;
     tp->next_tick = next_tick;
     tp->has_prev_next = has_prev_next;
@@ -1008,8 +1018,10 @@ __global__ __launch_bounds__(kBlock) void k_scatter(DevState s)
             s.sxp[pos] = xy_prev[2 * (size_t)i];
             s.syp[pos] = xy_prev[2 * (size_t)i + 1];
         }
-        xy_save[2 * (size_t)i] = x;
-        xy_save[2 * (size_t)i + 1] = y;
+        if (xy_save) {
+            xy_save[2 * (size_t)i] = x;
+            xy_save[2 * (size_t)i + 1] = y;
+        }
         if (s.precision == GLINT_PRECISION_FP32_GUARD) {
             Node nd = s.nodes[leaf];
             s.fx[pos] = (float)(x - nd.lx);
@@ -3211,7 +3223,8 @@ static glint_status run_begin(glint_ctx *c, const double *d_trace, uint32_t firs
     // one advance short of the first tick: tables and histories pre-swapped
     k_begin_run<<<1, 1, 0, c->stream>>>(c->s.tp, d_trace, c->tab[g], c->tab[1 - g], c->mbr[g], c->mbr[1 - g],
                                         c->hist[g], c->hist[1 - g], c->cand[g], c->cand[1 - g],
-                                        c->prev_tick, first_tick, c->has_prev ? 1u : 0u, c->stamps, 2 * nticks);
+                                        c->prev_tick, first_tick, c->has_prev ? 1u : 0u, c->stamps, 2 * nticks,
+                                        nticks);
     HIP_TRY(c, hipGetLastError());
     if (!c->built) {
         glint_status st = build_tree(c);

@@ -135,10 +135,12 @@ struct TickParams {
     const double *trace_base;   // run_ticks: tick k of the trace at trace_base + 2*N*k
     TableGen tab_old;           // pairs seen at prev_tick
     TableGen tab_cur;           // pairs seen at tick (written by the table update)
-    double *xy_prev;            // positions at prev_tick (pid order), valid if has_prev
+    const double *xy_prev;      // positions at prev_tick (pid order), valid if has_prev
     float4 *mbr_old;            // boxes of the pairs in tab_old, by entry (key-value output only)
     float4 *mbr_cur;            // boxes of the pairs in tab_cur
-    double *xy_save;            // receives this tick's positions for the next tick
+    double *xy_save;            // receives this tick's positions for the next tick (null: not needed)
+    double *hist_prev;          // the two position-history buffers, swapped every tick: the one
+    double *hist_save;          // holding prev_tick's positions and the one for this tick's
     uint4 *cand;                // candidate lists written at tick (see DevState::cand)
     uint4 *cand_prev;           // those written at prev_tick
     unsigned long long *stamps; // run_ticks: device wall clock at the end of every tick
@@ -149,6 +151,7 @@ struct TickParams {
     uint32_t has_prev;
     uint32_t next_tick;         // tick number given by the next k_advance_tick
     uint32_t has_prev_next;
+    uint32_t nticks;            // run_ticks: ticks of the run
 };
 
 struct DevState {
