@@ -49,8 +49,6 @@ constexpr int kFineTile = 12288;                    // contacts per LDS tile of 
 constexpr uint32_t kMaxCoarse = 1024;               // coarse bins of pass 1
 constexpr uint32_t kMaxFine = 2048;                 // fine bins of pass 2
 constexpr int kBorderStage = 256;                   // staged border objects per wave (assign)
-constexpr int kLookupStack = 128;
-constexpr int kMaxCand = 32;                        // candidate leaves per border object
 constexpr uint32_t kInsideChunk = 1;                // leaves per work-queue grab (1 measured fastest of 1, 2, 4, 8)
 constexpr uint32_t kHeavyLeaf = 256;                // leaves above this size are joined first
 
@@ -245,6 +243,7 @@ __global__ void k_init_tree(DevState s)
         root.child[0] = kLeafTag;
         root.child[1] = root.child[2] = root.child[3] = 0;
         s.nodes[0] = root;
+        s.parent[0] = kNone;
         NodeMeta m;
         m.type = kLeaf;
         m.level = 0;
@@ -354,6 +353,7 @@ __global__ void k_split(DevState s, int build_boxes)
             ch.child[0] = kLeafTag;
             ch.child[1] = ch.child[2] = ch.child[3] = 0;
             s.nodes[c] = ch;
+            s.parent[c] = node;
             NodeMeta m;
             m.type = kLeaf;
             m.level = pm.level + 1;
@@ -638,6 +638,7 @@ __global__ __launch_bounds__(1024) void k_grid_top(DevState s)
         for (int d = 0; d < 16; d++) c->g_list[d] = 0;
         if (lv[0] > s.cap) {
             s.glist[0][0] = make_uint2(0u, 0u);
+            s.parent[0] = kNone;
             c->g_list[0] = 1;
         } else {
             // no branch anywhere (the reference writes no node then): the root block as one leaf
@@ -651,6 +652,7 @@ __global__ __launch_bounds__(1024) void k_grid_top(DevState s)
             r.child[0] = kLeafTag;
             r.child[1] = r.child[2] = r.child[3] = 0;
             s.nodes[0] = r;
+            s.parent[0] = kNone;
             NodeMeta m;
             m.type = kLeaf;
             m.level = 0;
@@ -750,6 +752,7 @@ __global__ __launch_bounds__(kBlock) void k_grid_level(DevState s, uint32_t d)
             for (uint32_t i = 0; i < 4; i++) {
                 const uint32_t c = c0 + i;
                 b.child[i] = c;
+                s.parent[c] = node;
                 const uint32_t cbx = 2 * bx + (i & 1), cby = 2 * by + (i >> 1);
                 if (d < kGridBits && grid_count(s, d + 1, cbx, cby) > s.cap) {
                     const uint32_t slot = atomicAdd(&s.ctr->g_list[d + 1], 1u);
@@ -1043,91 +1046,147 @@ __device__ __forceinline__ uint32_t wave_exclusive_scan(uint32_t v, uint32_t *to
     return inc - v;
 }
 
-// Units (pid, leaf) of one object written at once: used when its candidate
-// list fills up (deep trees of the grid rebuild can put many leaves within
-// reach), the common case goes through the wave-aggregated path below.
-__device__ __forceinline__ void emit_units(const DevState &s, uint32_t pid, const uint32_t *cand, uint32_t n)
+// Rank of this lane's count in p[key] (kNone: none): the lanes of the wave
+// that share a key take consecutive ranks with one atomic.  Every lane of the
+// wave must call it.
+__device__ __forceinline__ uint32_t wave_rank(uint32_t *p, uint32_t key)
 {
-    const unsigned long long u0 = atomicAdd(&s.ctr->n_units, (unsigned long long)n);
-    for (uint32_t j = 0; j < n; j++) {
-        const uint32_t rank = atomicAdd(&s.ucount[cand[j]], 1u);
-        if (u0 + j < s.unit_cap) {
-            s.units[u0 + j] = make_uint2(pid, cand[j]);
-            s.urank[u0 + j] = rank;
-        } else {
-            atomicOr(&s.ctr->error, kErrUnitCap);
+    uint32_t rank = 0;
+    bool need = key != kNone;
+    for (;;) {
+        const unsigned long long pend = __ballot(need);
+        if (!pend) break;
+        const uint32_t leader = __ffsll((long long)pend) - 1;
+        const uint32_t K = __shfl(key, leader);
+        const bool mine = need && key == K;
+        const unsigned long long mm = __ballot(mine);
+        uint32_t b = 0;
+        if (lane_id() == leader) b = atomicAdd(&p[K], (uint32_t)__popcll(mm));
+        b = __shfl(b, leader);
+        if (mine) {
+            rank = b + (uint32_t)__popcll(mm & ((1ull << lane_id()) - 1ull));
+            need = false;
         }
+    }
+    return rank;
+}
+
+// The children of a node as the lookup sees them: the four child nodes are
+// read together; accepted leaves are passed to take(leaf) when `takes`, and
+// the mask of branch children within r is returned.
+template <class Take>
+__device__ __forceinline__ uint32_t lookup_children(const DevState &s, uint4 ids, double x, double y, uint32_t own,
+                                                    bool takes, Take &&take)
+{
+    const Node n0 = s.nodes[ids.x], n1 = s.nodes[ids.y], n2 = s.nodes[ids.z], n3 = s.nodes[ids.w];
+    uint32_t desc = 0;
+    auto one = [&](const Node &cn, uint32_t c, uint32_t bit) {
+        if (!box_within(cn, x, y, s.t64)) return;
+        if (cn.child[0] != kLeafTag) {
+            desc |= bit;
+        } else if (takes && (s.complete ? (c != own) : (y < cn.ly || (y < cn.hy && x < cn.lx)))) {
+            take(c);
+        }
+    };
+    one(n0, ids.x, 1u);
+    one(n1, ids.y, 2u);
+    one(n2, ids.z, 4u);
+    one(n3, ids.w, 8u);
+    return desc;
+}
+
+// lookup_rec (filter.cpp:130-152) below node `from`: every leaf within r of
+// (x, y) that the top/right rule accepts (in complete mode every leaf within
+// r other than `own`) is passed to take(leaf).  A depth-first walk without a
+// stack: it climbs back through the parent links and resumes after the child
+// it came from (the branch mask of a node is recomputed then), so nothing
+// sits in scratch memory.
+template <class Take>
+__device__ __forceinline__ void lookup_walk(const DevState &s, double x, double y, uint32_t from, uint32_t own,
+                                            Take &&take)
+{
+    uint32_t node = from;
+    uint4 ids = node_children(s.nodes, from);
+    uint32_t desc = lookup_children(s, ids, x, y, own, true, take);
+    for (;;) {
+        if (desc) {
+            node = pick_child(ids, __builtin_ctz(desc));
+            ids = node_children(s.nodes, node);
+            desc = lookup_children(s, ids, x, y, own, true, take);
+            continue;
+        }
+        if (node == from) break;
+        const uint32_t p = s.parent[node];
+        if (p >= s.node_cap) {   // not reached: every node below `from` has its parent
+            atomicOr(&s.ctr->error, kErrStack);
+            break;
+        }
+        ids = node_children(s.nodes, p);
+        const uint32_t ci = ids.x == node ? 0u : (ids.y == node ? 1u : (ids.z == node ? 2u : 3u));
+        desc = lookup_children(s, ids, x, y, own, false, take) & ~((2u << ci) - 1u);
+        node = p;
     }
 }
 
-// lookup_rec (filter.cpp:130-152) from last_valid for every border object.
-// In complete mode every leaf within r other than the object's own.  Each
-// unit (object, candidate leaf) takes a rank among the units of its leaf.
-// The object's first three candidates and their number are kept for the
-// join of the next tick.
+// The lookup from last_valid for every border object.  Each unit (object,
+// candidate leaf) takes a rank among the units of its leaf.  The object's
+// first three candidates (kCandOverflow in the third when there are more)
+// are kept, stamped with the tick, for the joins of this tick and the next.
+// Objects with up to three candidates (nearly all) write their units from
+// registers after one atomic per wave; the others walk a second time.
 __global__ __launch_bounds__(kBlock) void k_lookup(DevState s)
 {
     const double *xy = s.tp->xy;
     const uint32_t nb = s.ctr->n_border;
     for (uint32_t base = blockIdx.x * blockDim.x; base < nb; base += gridDim.x * blockDim.x) {
-        uint32_t k = base + threadIdx.x;
-        uint32_t cand[kMaxCand];
-        uint32_t nc = 0, total = 0;
+        const uint32_t k = base + threadIdx.x;
+        uint32_t total = 0;
         uint32_t f0 = kNone, f1 = kNone, f2 = kNone;
-        uint32_t pid = 0;
+        uint32_t pid = 0, lv = 0, own = 0;
+        double x = 0.0, y = 0.0;
         if (k < nb) {
-            uint2 b = s.border[k];
+            const uint2 b = s.border[k];
             pid = b.x;
-            double x = xy[2 * (size_t)pid], y = xy[2 * (size_t)pid + 1];
-            uint32_t own = s.leaf_of[pid];
-            uint32_t stk[kLookupStack];
-            int sp = 0;
-            stk[sp++] = b.y;
-            while (sp > 0) {
-                uint32_t node = stk[--sp];
-                Node nd = s.nodes[node];
-                for (int i = 0; i < 4; i++) {
-                    uint32_t c = nd.child[i];
-                    Node ch = s.nodes[c];
-                    if (!box_within(ch, x, y, s.t64)) continue;
-                    if (ch.child[0] == kLeafTag) {
-                        bool take = s.complete ? (c != own) : (y < ch.ly || (y < ch.hy && x < ch.lx));
-                        if (take) {
-                            if (nc == kMaxCand) {
-                                emit_units(s, pid, cand, nc);
-                                nc = 0;
-                            }
-                            cand[nc++] = c;
-                            f0 = total == 0 ? c : f0;
-                            f1 = total == 1 ? c : f1;
-                            f2 = total == 2 ? c : f2;
-                            total++;
-                        }
-                    } else {
-                        if (sp < kLookupStack)
-                            stk[sp++] = c;
-                        else
-                            atomicOr(&s.ctr->error, kErrStack);
-                    }
-                }
-            }
+            lv = b.y;
+            x = xy[2 * (size_t)pid];
+            y = xy[2 * (size_t)pid + 1];
+            own = s.leaf_of[pid];
+            lookup_walk(s, x, y, lv, own, [&](uint32_t c) {
+                f0 = total == 0 ? c : f0;
+                f1 = total == 1 ? c : f1;
+                f2 = total == 2 ? c : f2;
+                total++;
+            });
+            if (!s.store_all) s.tp->cand[pid] = make_uint4(f0, f1, total > 3 ? kCandOverflow : f2, s.tp->tick);
         }
-        if (k < nb && !s.store_all) s.tp->cand[pid] = make_uint4(f0, f1, total > 3 ? kCandOverflow : f2, s.tp->tick);
-        // one atomic per wave reserves the wave's units
+        // one atomic per wave reserves the units of the lists of up to three;
+        // lanes that share a candidate leaf take consecutive ranks there with
+        // one atomic (objects come in leaf order, so neighbours share leaves)
+        const uint32_t nc = total <= 3 ? total : 0u;
         uint32_t wtotal;
-        uint32_t wex = wave_exclusive_scan(nc, &wtotal);
+        const uint32_t wex = wave_exclusive_scan(nc, &wtotal);
         unsigned long long wbase = 0;
         if (lane_id() == 0 && wtotal) wbase = atomicAdd(&s.ctr->n_units, (unsigned long long)wtotal);
         wbase = __shfl(wbase, 0);
-        for (uint32_t j = 0; j < nc; j++) {
-            unsigned long long u = wbase + wex + j;
-            uint32_t rank = atomicAdd(&s.ucount[cand[j]], 1u);
+        const uint32_t r0 = wave_rank(s.ucount, nc > 0 ? f0 : kNone);
+        const uint32_t r1 = wave_rank(s.ucount, nc > 1 ? f1 : kNone);
+        const uint32_t r2 = wave_rank(s.ucount, nc > 2 ? f2 : kNone);
+        unsigned long long u = wbase + wex;
+        auto write = [&](uint32_t c, uint32_t rank) {
             if (u < s.unit_cap) {
-                s.units[u] = make_uint2(pid, cand[j]);
+                s.units[u] = make_uint2(pid, c);
                 s.urank[u] = rank;
             } else {
                 atomicOr(&s.ctr->error, kErrUnitCap);
             }
+            u++;
+        };
+        if (nc > 0) write(f0, r0);
+        if (nc > 1) write(f1, r1);
+        if (nc > 2) write(f2, r2);
+        if (total > 3) {
+            u = atomicAdd(&s.ctr->n_units, (unsigned long long)total);
+            lookup_walk(s, x, y, lv, own, [&](uint32_t c) { write(c, atomicAdd(&s.ucount[c], 1u)); });
         }
     }
 }
@@ -2451,7 +2510,11 @@ glint_status status_from_flags(glint_ctx *c, uint32_t f)
         c->err = "meeting table full (raise num_meeting_buckets)";
         return GLINT_ERR_TABLE_FULL;
     }
-    if (f & (kErrUnitCap | kErrStack)) {
+    if (f & kErrStack) {
+        c->err = "inconsistent quadtree (a node without its parent link)";
+        return GLINT_ERR_HIP;
+    }
+    if (f & kErrUnitCap) {
         c->err = "border check units exceed unit_capacity";
         return GLINT_ERR_UNIT_CAPACITY;
     }
@@ -2874,6 +2937,7 @@ glint_status glint_create(const glint_config *cfg, glint_ctx **out)
     ALLOC(s.merge_list, node_cap);
     ALLOC(s.leaf_list, node_cap);
     ALLOC(s.leaf_of, N);
+    ALLOC(s.parent, node_cap);
     ALLOC(s.prev_leaf, N);
     if (!s.store_all) {
         ALLOC(s.cand, N);

@@ -185,6 +185,7 @@ struct DevState {
     uint32_t *merge_list;
     uint32_t *leaf_list;
     uint32_t *leaf_of;
+    uint32_t *parent;       // per node: its parent (kNone for the root), for stackless tree walks
     uint32_t *prev_leaf;    // leaf of every object at the tick before (pid order)
     // Candidate leaves of the lookups, pid order, in two buffers taken in turn
     // (TickParams::cand, cand_prev): up to three leaves (the third is
