@@ -1240,10 +1240,12 @@ struct Stager {
     uint32_t kept;              // entries written in all (uniform)
 };
 
-// Lanes without a hit write to their spare slot past the stage (no branch).
+// Lanes without a hit write to their spare slot past the stage (no branch);
+// a column step without any hit in the wave writes nothing.
 __device__ __forceinline__ void stage_push(Stager &st, bool hit, uint32_t e)
 {
     const unsigned long long m = __ballot(hit);
+    if (m == 0) return;
     const uint32_t lane = lane_id();
     st.buf[hit ? st.n + __popcll(m & ((1ull << lane) - 1ull)) : kStage + lane] = e;
     st.n += (uint32_t)__popcll(m);
@@ -1312,6 +1314,16 @@ __device__ __forceinline__ uint4 cand_read(const uint4 *buf, uint32_t pid, uint3
     return c.z == kCandOverflow ? make_uint4(kNone, kNone, kNone, 4u) : make_uint4(c.x, c.y, c.z, 0u);
 }
 
+// One bit per leaf id: a 32-bit summary of a candidate list whose clear bits
+// prove a leaf is not on it (all bits set for a list that overflowed).
+__device__ __forceinline__ uint32_t leaf_bit(uint32_t leaf) { return 1u << ((leaf * 0x9E3779B1u) >> 27); }
+__device__ __forceinline__ uint32_t cand_mask(uint4 c)
+{
+    if (c.w > 3) return 0xFFFFFFFFu;
+    return (c.x != kNone ? leaf_bit(c.x) : 0u) | (c.y != kNone ? leaf_bit(c.y) : 0u) |
+           (c.z != kNone ? leaf_bit(c.z) : 0u);
+}
+
 // Positions of a tile object: FP64 for a leaf joined in FP64, else the FP32
 // offsets from the leaf's low corner at tick and at prev_tick (NaN: not
 // bounded).  The FP64 positions of a guard-band leaf stay in HBM (sx, sy),
@@ -1375,12 +1387,13 @@ constexpr uint32_t kBandBit = 1u << 15;
 // [jbeg, jend) of tile C, from column jmin on (jmin = ~0 for an idle lane).
 // In guard mode (USE32) pairs are decided on the FP32 offsets: inside
 // r - eps a hit, past r + eps a miss, in between a band entry.  Sure hits
-// are counted here; a sure inside hit is staged only if it may be stored:
-// when every contact is kept, or when the FP32 offsets at prev_tick do not
-// place the pair surely out of reach then (the exact test follows at
-// write-out).  Every border hit is staged: untagged ones are needed to count
-// the pairs that both lookups find (settling them in the loop instead costs
-// a second LDS read per column and measured slower).
+// are counted here and staged only if they may be stored: when every
+// contact is kept, or when the FP32 offsets at prev_tick do not place the
+// pair surely out of reach then (the exact test follows at write-out).  A
+// sure border hit that is surely untagged is staged only when a candidate
+// list overflowed (it is stored then) or when the column object's candidate
+// summary does not rule out the row's leaf (the pair may be found twice,
+// which key_of counts); any other is neither stored nor a second copy.
 struct JoinCounts {
     uint32_t sure;   // sure contacts (wave-uniform)
     uint32_t pass;   // band entries within reach in FP64 (per lane)
@@ -1400,12 +1413,18 @@ __device__ __forceinline__ JoinCounts join_columns(Stager &st, const DevState &s
                                                    double yi, float fxi, float fyi, float pfxi, float pfyi,
                                                    uint32_t jmin, uint32_t jbeg, uint32_t jend, float t32_in,
                                                    float t32_out, double t64, const JoinRows &rows,
-                                                   bool has_prev, const uint4 *ccand, uint32_t leaf, uint4 rpc)
+                                                   bool has_prev, const uint4 *ccand, const uint32_t *cmask,
+                                                   uint32_t leaf, uint4 rpc, uint32_t rpl)
 {
     uint32_t n_sure = 0, n_pass = 0, band = 0, n_dup = 0;
     const uint32_t tag = lane_id() << 16;
     const unsigned long long flag = BORDER ? kTagBorder : 0ull;
     const bool tested_check = has_prev && !s.store_all;   // wave-uniform
+    // border rows in guard mode (pairs not all kept): the row leaf's summary
+    // bit, and whether the row's candidates overflowed
+    const bool settle = BORDER && USE32 && !s.store_all;
+    const uint32_t rbit = leaf_bit(rpl & 0x7FFFFFFFu);
+    const bool rovf = (rpl >> 31) != 0;
     // the stage's contacts as keys (kEmptyKey: a band entry that is not
     // within reach in FP64, or a contact that is not stored)
     auto key_of = [&](uint32_t e, bool valid) {
@@ -1472,6 +1491,7 @@ __device__ __forceinline__ JoinCounts join_columns(Stager &st, const DevState &s
     for (uint32_t jb = jbeg; jb < jend; jb += kFlushRound) {
         TileF cfp[kFlushRound];
         double2 cx[kFlushRound];
+        uint32_t cm[kFlushRound];
 #pragma unroll
         for (uint32_t jj = 0; jj < kFlushRound; jj++) {
             const uint32_t jc = min(jb + jj, jend - 1);
@@ -1479,6 +1499,7 @@ __device__ __forceinline__ JoinCounts join_columns(Stager &st, const DevState &s
                 cfp[jj] = C.fp[jc];
             else
                 cx[jj] = C.xy[jc];
+            if (settle) cm[jj] = cmask[jc];
         }
 #pragma unroll
         for (uint32_t jj = 0; jj < kFlushRound; jj++) {
@@ -1498,7 +1519,10 @@ __device__ __forceinline__ JoinCounts join_columns(Stager &st, const DevState &s
                 const float p32 = px * px + py * py;
                 // NaN (unbounded offsets) compares false: may be tagged
                 const bool maybe_tagged = has_prev && !(p32 > t32_out);
-                stage = hit && (!sure || BORDER || s.store_all || maybe_tagged);
+                if (settle)
+                    stage = hit && (!sure || maybe_tagged || rovf || (cm[jj] & rbit) != 0u);
+                else
+                    stage = hit && (!sure || BORDER || s.store_all || maybe_tagged);
             } else {
                 const double ex = xi - cx[jj].x;
                 const double ey = yi - cx[jj].y;
@@ -1533,6 +1557,7 @@ struct JoinLds {
     JoinRows rows[kBlock / 64];
     uint32_t stage[kBlock / 64][kStage + 64];
     uint4 ccand[TILE];                   // candidate leaves of the objects of tile a (see DevState::cand)
+    uint32_t cmask[TILE];                // and a 32-bit summary of them (cand_mask)
     JoinLeaf leaf[kInsideChunk];
     uint32_t gfirst[kInsideChunk + 1];   // groups of consecutive leaves of the chunk
     uint32_t ngroup;
@@ -1633,21 +1658,24 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
         rows.p[lane] = pi;
         rows.pf[lane] = make_float2(pfxi, pfyi);
         if (want_pc) rows.ppl[lane] = inside ? A.pl[vi ? row : jbeg] : (vi ? s.prev_leaf[pi] : kNone);
-        if (!inside && !s.store_all && vi)
-            rows.pl[lane] = s.leaf_of[pi] | (cand_read(s.tp->cand, pi, s.tp->tick).w > 3 ? 0x80000000u : 0u);
+        uint32_t rpl = 0;
+        if (!inside && !s.store_all && vi) {
+            rpl = s.leaf_of[pi] | (cand_read(s.tp->cand, pi, s.tp->tick).w > 3 ? 0x80000000u : 0u);
+            rows.pl[lane] = rpl;
+        }
         const JoinCounts jc =
             use32 ? (inside ? join_columns<true, false>(st, s, C, xi, yi, fxi, fyi, pfxi, pfyi, jmin, jbeg, jend,
-                                                        s.t32_in, s.t32_out, t64, rows, has_prev, lds.ccand, leaf,
-                                                        rpc)
+                                                        s.t32_in, s.t32_out, t64, rows, has_prev, lds.ccand, lds.cmask, leaf,
+                                                        rpc, rpl)
                             : join_columns<true, true>(st, s, C, xi, yi, fxi, fyi, pfxi, pfyi, jmin, jbeg, jend,
-                                                       s.t32_in, s.t32_out, t64, rows, has_prev, lds.ccand, leaf,
-                                                       rpc))
+                                                       s.t32_in, s.t32_out, t64, rows, has_prev, lds.ccand, lds.cmask, leaf,
+                                                       rpc, rpl))
                   : (inside ? join_columns<false, false>(st, s, C, xi, yi, fxi, fyi, pfxi, pfyi, jmin, jbeg, jend,
-                                                         s.t32_in, s.t32_out, t64, rows, has_prev, lds.ccand, leaf,
-                                                         rpc)
+                                                         s.t32_in, s.t32_out, t64, rows, has_prev, lds.ccand, lds.cmask, leaf,
+                                                         rpc, rpl)
                             : join_columns<false, true>(st, s, C, xi, yi, fxi, fyi, pfxi, pfyi, jmin, jbeg, jend,
-                                                        s.t32_in, s.t32_out, t64, rows, has_prev, lds.ccand, leaf,
-                                                        rpc));
+                                                        s.t32_in, s.t32_out, t64, rows, has_prev, lds.ccand, lds.cmask, leaf,
+                                                        rpc, rpl));
         n_in += inside ? jc.sure : 0u;
         n_bd += inside ? 0u : jc.sure;
         pass_in += inside ? jc.pass : 0u;
@@ -1718,7 +1746,8 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
                     load_tile(lds.a, s, F.off + I * TILE, ni, has_prev, F.use32);
                     if (!s.store_all)
                         for (uint32_t k = threadIdx.x; k < ni; k += kBlock)
-                            lds.ccand[k] = cand_read(s.tp->cand, s.spid[F.off + I * TILE + k], s.tp->tick);
+                            lds.cmask[k] = cand_mask(
+                                lds.ccand[k] = cand_read(s.tp->cand, s.spid[F.off + I * TILE + k], s.tp->tick));
                     for (uint32_t J = I; J < ntile; J++) {
                         const uint32_t nj = min((uint32_t)TILE, F.cnt - J * TILE);
                         const bool diag = (J == I);
@@ -1762,7 +1791,8 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
                     const JoinLeaf &L = lds.leaf[k];
                     const uint32_t from = L.off + (i - L.base);
                     tile_put(lds.a, i, s, from, has_prev, L.use32, has_prev && !s.store_all);
-                    if (!s.store_all) lds.ccand[i] = cand_read(s.tp->cand, lds.a.p[i], s.tp->tick);
+                    if (!s.store_all)
+                        lds.cmask[i] = cand_mask(lds.ccand[i] = cand_read(s.tp->cand, lds.a.p[i], s.tp->tick));
                 }
                 __syncthreads();
                 for (uint32_t t = wave; t < ntask; t += kBlock / 64) {

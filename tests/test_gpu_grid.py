@@ -35,12 +35,14 @@ def test_grid_rebuild_parity_hotspots_deep_trees(check_contacts):
                check_leaves_every=3, num_meeting_buckets=1 << 21, contact_capacity=4_000_000)
 
 
-def test_grid_rebuild_parity_cells_over_capacity_guard_band():
-    """Clusters of sigma 3 m: cells over capacity get four leaves of depth 15."""
+@pytest.mark.parametrize("check_contacts", [True, False])
+def test_grid_rebuild_parity_cells_over_capacity_guard_band(check_contacts):
+    """Clusters of sigma 3 m: cells over capacity get four leaves of depth 15,
+    and lookups there accept many leaves (candidate lists that overflow)."""
     w = gen.Workload("clusters", 30_000, 8, 30_000.0, seed=5, num_hotspots=4, hotspot_fraction=0.05,
                      hotspot_sigma=3.0, speed_min=0.1, speed_max=0.5, grid_capacity=8)
     run_parity(w, 8, dynamic_schema=GRID, box=margin_box(w), precision=runtime.PRECISION_FP32_GUARD,
-               check_leaves_every=2, join_tile=256, num_meeting_buckets=1 << 20)
+               check_leaves_every=2, join_tile=256, num_meeting_buckets=1 << 20, check_contacts=check_contacts)
 
 
 @pytest.mark.parametrize("graph_ticks", [0, 4])
