@@ -1395,7 +1395,7 @@ constexpr uint32_t kBandBit = 1u << 15;
 // summary does not rule out the row's leaf (the pair may be found twice,
 // which key_of counts); any other is neither stored nor a second copy.
 struct JoinCounts {
-    uint32_t sure;   // sure contacts (wave-uniform)
+    uint32_t sure;   // sure contacts (per lane)
     uint32_t pass;   // band entries within reach in FP64 (per lane)
     uint32_t band;   // band entries re-tested (per lane)
     uint32_t dup;    // pairs found twice and not stored, counted once (per lane)
@@ -1501,6 +1501,8 @@ __device__ __forceinline__ JoinCounts join_columns(Stager &st, const DevState &s
                 cx[jj] = C.xy[jc];
             if (settle) cm[jj] = cmask[jc];
         }
+        bool stg[kFlushRound];
+        uint32_t ent[kFlushRound];
 #pragma unroll
         for (uint32_t jj = 0; jj < kFlushRound; jj++) {
             const uint32_t j = jb + jj;
@@ -1529,9 +1531,13 @@ __device__ __forceinline__ JoinCounts join_columns(Stager &st, const DevState &s
                 sure = cand && ex * ex + ey * ey <= t64;
                 stage = sure && (BORDER || s.store_all || has_prev);
             }
-            n_sure += (uint32_t)__popcll(__ballot(sure));
-            stage_push(st, stage, e);
+            n_sure += sure;
+            stg[jj] = stage;
+            ent[jj] = e;
         }
+        // the round's stage writes together, after its distance tests
+#pragma unroll
+        for (uint32_t jj = 0; jj < kFlushRound; jj++) stage_push(st, stg[jj], ent[jj]);
         if (st.n > kStage - 64 * kFlushRound) stage_flush(st, s, key_of);
     }
     stage_flush(st, s, key_of);
@@ -1602,7 +1608,7 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
     const uint32_t wave = threadIdx.x / 64, lane = lane_id();
     Stager st{contact_out(s), lds.stage[wave], 0, 0, kChunk, 0};
     JoinRows &rows = lds.rows[wave];
-    uint32_t n_in = 0, n_bd = 0;                   // sure contacts (wave-uniform)
+    uint32_t n_in = 0, n_bd = 0;                   // sure contacts (per lane)
     uint32_t pass_in = 0, pass_bd = 0, band = 0;   // band entries within reach / re-tested (per lane)
     uint32_t dups = 0;                             // unstored pairs found twice (per lane)
 
@@ -1821,7 +1827,7 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
     }
     stage_close(st, s);
     // one atomic per wave for each counter
-    uint32_t pi_ = pass_in, pb = pass_bd, bd = band, du = dups;
+    uint32_t pi_ = pass_in + n_in, pb = pass_bd + n_bd, bd = band, du = dups;
     for (int d = 32; d >= 1; d >>= 1) {
         pi_ += __shfl_xor(pi_, d);
         pb += __shfl_xor(pb, d);
@@ -1829,7 +1835,7 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
         du += __shfl_xor(du, d);
     }
     if (lane == 0) {
-        const uint32_t ci = n_in + pi_, cb = n_bd + pb;
+        const uint32_t ci = pi_, cb = pb;
         if (ci + cb) atomicAdd(&s.ctr->n_contacts, (unsigned long long)(ci + cb));
         if (cb) atomicAdd(&s.ctr->n_contacts_border, (unsigned long long)cb);
         if (bd) atomicAdd(&s.ctr->n_rechecks, (unsigned long long)bd);
