@@ -104,8 +104,13 @@ __global__ void k_fill_u32(uint32_t *p, uint32_t v, uint64_t n)
 
 __global__ void k_set_tick(TickParams *tp, const double *xy, TableGen tab_old, TableGen tab_cur, float4 *mbr_old,
                            float4 *mbr_cur, double *xy_prev, double *xy_save, uint4 *cand_prev, uint4 *cand,
-                           uint32_t tick, uint32_t prev_tick, uint32_t has_prev)
+                           double *sx_old, double *sy_old, double *sx, double *sy, uint32_t tick, uint32_t prev_tick,
+                           uint32_t has_prev)
 {
+    tp->sx = sx;
+    tp->sy = sy;
+    tp->sx_old = sx_old;
+    tp->sy_old = sy_old;
     tp->cand = cand;
     tp->cand_prev = cand_prev;
     tp->xy = xy;
@@ -154,15 +159,26 @@ __global__ void k_advance_tick(TickParams *tp, uint64_t stride)
     uint4 *cb = tp->cand_prev;
     tp->cand_prev = tp->cand;
     tp->cand = cb;
+    double *q = tp->sx_old;
+    tp->sx_old = tp->sx;
+    tp->sx = q;
+    q = tp->sy_old;
+    tp->sy_old = tp->sy;
+    tp->sy = q;
 }
 
 // State before the first tick of a run, one k_advance_tick short of it; the
 // positions already point at the first snapshot for the initial build.
 __global__ void k_begin_run(TickParams *tp, const double *base, TableGen tab_old, TableGen tab_cur, float4 *mbr_old,
                             float4 *mbr_cur, double *xy_prev, double *xy_save, uint4 *cand_prev, uint4 *cand,
-                            uint32_t tick, uint32_t next_tick, uint32_t has_prev_next, unsigned long long *stamps,
-                            uint32_t nstamps, uint32_t nticks)
+                            double *sx_old, double *sy_old, double *sx, double *sy, uint32_t tick,
+                            uint32_t next_tick, uint32_t has_prev_next, unsigned long long *stamps, uint32_t nstamps,
+                            uint32_t nticks)
 {
+    tp->sx = sx;
+    tp->sy = sy;
+    tp->sx_old = sx_old;
+    tp->sy_old = sy_old;
     tp->nticks = nticks;
     tp->cand = cand;
     tp->cand_prev = cand_prev;
@@ -813,8 +829,10 @@ __global__ __launch_bounds__(kBlock) void k_assign(DevState s)
         }
         bool border = false;
         uint32_t lv = 0, cur = kNone, pleaf = kNone;
+        double x = 0.0, y = 0.0;
         if (active) {
-            double x = xy[2 * (size_t)i], y = xy[2 * (size_t)i + 1];
+            x = xy[2 * (size_t)i];
+            y = xy[2 * (size_t)i + 1];
             pleaf = s.leaf_of[i];
             s.prev_leaf[i] = pleaf;
             if (!(x >= s.lx && x <= s.hx && y >= s.ly && y <= s.hy)) {
@@ -857,6 +875,7 @@ __global__ __launch_bounds__(kBlock) void k_assign(DevState s)
         if (active) s.leaf_of[i] = cur;
         if (k < s.n) {
             s.order[k] = i;
+            s.ord_xy[k] = make_double2(x, y);
             s.ord_leaf[k] = active ? cur : kNone;
             s.ord_rank[k] = rank;
             s.ord_pleaf[k] = pleaf;
@@ -1003,23 +1022,30 @@ __global__ __launch_bounds__(kBlock) void k_scan_units(DevState s)
 // written in runs.
 __global__ __launch_bounds__(kBlock) void k_scatter(DevState s)
 {
-    const double *xy = s.tp->xy;
     const double *xy_prev = s.tp->xy_prev;
     double *xy_save = s.tp->xy_save;
+    double *sx = s.tp->sx, *sy = s.tp->sy;
+    const double *sx_old = s.tp->sx_old, *sy_old = s.tp->sy_old;
     const bool has_prev = s.tp->has_prev != 0;
+    // in the previous tick's leaf order, position k held the object now at k
+    const bool ordered = s.ctr->sorted_valid != 0;
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < s.n; k += gridDim.x * blockDim.x) {
         const uint32_t leaf = s.ord_leaf[k];
         if (leaf == kNone) continue;
         const uint32_t i = s.order[k];
         const uint32_t pos = s.offset[leaf] + s.ord_rank[k];
-        double x = xy[2 * (size_t)i], y = xy[2 * (size_t)i + 1];
-        s.sx[pos] = x;
-        s.sy[pos] = y;
+        const double2 p = s.ord_xy[k];
+        const double x = p.x, y = p.y;
+        sx[pos] = x;
+        sy[pos] = y;
         s.spid[pos] = i;
         s.splf[pos] = s.ord_pleaf[k];
+        double px = 0.0, py = 0.0;
         if (has_prev) {
-            s.sxp[pos] = xy_prev[2 * (size_t)i];
-            s.syp[pos] = xy_prev[2 * (size_t)i + 1];
+            px = ordered ? sx_old[k] : xy_prev[2 * (size_t)i];
+            py = ordered ? sy_old[k] : xy_prev[2 * (size_t)i + 1];
+            s.sxp[pos] = px;
+            s.syp[pos] = py;
         }
         if (xy_save) {
             xy_save[2 * (size_t)i] = x;
@@ -1030,13 +1056,12 @@ __global__ __launch_bounds__(kBlock) void k_scatter(DevState s)
             s.fx[pos] = (float)(x - nd.lx);
             s.fy[pos] = (float)(y - nd.ly);
             if (has_prev) {
-                const double ox = xy_prev[2 * (size_t)i] - nd.lx, oy = xy_prev[2 * (size_t)i + 1] - nd.ly;
+                const double ox = px - nd.lx, oy = py - nd.ly;
                 s.pfx[pos] = fabs(ox) <= s.pf_lim ? (float)ox : __builtin_nanf("");
                 s.pfy[pos] = fabs(oy) <= s.pf_lim ? (float)oy : __builtin_nanf("");
             }
         }
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) s.ctr->sorted_valid = 1;
 }
 
 __device__ __forceinline__ uint32_t wave_exclusive_scan(uint32_t v, uint32_t *total)
@@ -1138,6 +1163,8 @@ __global__ __launch_bounds__(kBlock) void k_lookup(DevState s)
 {
     const double *xy = s.tp->xy;
     const uint32_t nb = s.ctr->n_border;
+    // the scatter before has put the objects in leaf order (read by the next tick)
+    if (blockIdx.x == 0 && threadIdx.x == 0) s.ctr->sorted_valid = 1;
     for (uint32_t base = blockIdx.x * blockDim.x; base < nb; base += gridDim.x * blockDim.x) {
         const uint32_t k = base + threadIdx.x;
         uint32_t total = 0;
@@ -1362,7 +1389,7 @@ __device__ __forceinline__ void tile_put(JoinTile<TILE> &t, uint32_t k, const De
         t.fp[k] = TileF{make_float2(s.fx[from], s.fy[from]),
                         has_prev ? make_float2(s.pfx[from], s.pfy[from]) : make_float2(0.f, 0.f)};
     else
-        t.xy[k] = make_double2(s.sx[from], s.sy[from]);
+        t.xy[k] = make_double2(s.tp->sx[from], s.tp->sy[from]);
     t.g[k] = from;
     t.pl[k] = s.splf[from];
     const uint32_t pid = s.spid[from];
@@ -1435,8 +1462,8 @@ __device__ __forceinline__ JoinCounts join_columns(Stager &st, const DevState &s
         if (USE32 && (e & kBandBit)) {
             band++;
             const uint32_t pa = rows.p[rl], gj = C.g[j];
-            const double ex = s.tp->xy[2 * (size_t)pa] - s.sx[gj];
-            const double ey = s.tp->xy[2 * (size_t)pa + 1] - s.sy[gj];
+            const double ex = s.tp->xy[2 * (size_t)pa] - s.tp->sx[gj];
+            const double ey = s.tp->xy[2 * (size_t)pa + 1] - s.tp->sy[gj];
             if (!(ex * ex + ey * ey <= t64)) return kEmptyKey;
             n_pass++;
         }
@@ -2573,7 +2600,8 @@ void set_tick_params(glint_ctx *c, const double *xy)
 {
     const int g = c->cur_gen;
     k_set_tick<<<1, 1, 0, c->stream>>>(c->s.tp, xy, c->tab[1 - g], c->tab[g], c->mbr[1 - g], c->mbr[g],
-                                       c->hist[1 - g], c->hist[g], c->cand[1 - g], c->cand[g], c->tick,
+                                       c->hist[1 - g], c->hist[g], c->cand[1 - g], c->cand[g], c->s.sxb[1 - g],
+                                       c->s.syb[1 - g], c->s.sxb[g], c->s.syb[g], c->tick,
                                        c->has_prev ? c->prev_tick : c->tick, c->has_prev ? 1u : 0u);
 }
 
@@ -3003,8 +3031,11 @@ glint_status glint_create(const glint_config *cfg, glint_ctx **out)
         ALLOC(s.glist[0], s.glist_cap);
         ALLOC(s.glist[1], s.glist_cap);
     }
-    ALLOC(s.sx, N);
-    ALLOC(s.sy, N);
+    ALLOC(s.sxb[0], N);
+    ALLOC(s.syb[0], N);
+    ALLOC(s.sxb[1], N);
+    ALLOC(s.syb[1], N);
+    ALLOC(s.ord_xy, N);
     ALLOC(s.sxp, N);
     ALLOC(s.syp, N);
     ALLOC(c->hist[0], 2 * N);
@@ -3322,7 +3353,8 @@ static glint_status run_begin(glint_ctx *c, const double *d_trace, uint32_t firs
     const int g = c->cur_gen;
     // one advance short of the first tick: tables and histories pre-swapped
     k_begin_run<<<1, 1, 0, c->stream>>>(c->s.tp, d_trace, c->tab[g], c->tab[1 - g], c->mbr[g], c->mbr[1 - g],
-                                        c->hist[g], c->hist[1 - g], c->cand[g], c->cand[1 - g],
+                                        c->hist[g], c->hist[1 - g], c->cand[g], c->cand[1 - g], c->s.sxb[g],
+                                        c->s.syb[g], c->s.sxb[1 - g], c->s.syb[1 - g],
                                         c->prev_tick, first_tick, c->has_prev ? 1u : 0u, c->stamps, 2 * nticks,
                                         nticks);
     HIP_TRY(c, hipGetLastError());

@@ -141,6 +141,9 @@ struct TickParams {
     double *xy_save;            // receives this tick's positions for the next tick (null: not needed)
     double *hist_prev;          // the two position-history buffers, swapped every tick: the one
     double *hist_save;          // holding prev_tick's positions and the one for this tick's
+    double *sx, *sy;            // objects sorted by leaf at tick (written by the scatter)
+    double *sx_old, *sy_old;    // the same at prev_tick (read by the scatter: position k of the
+                                // tick's assignment is the object sorted at k then)
     uint4 *cand;                // candidate lists written at tick (see DevState::cand)
     uint4 *cand_prev;           // those written at prev_tick
     unsigned long long *stamps; // run_ticks: device wall clock at the end of every tick
@@ -199,12 +202,12 @@ struct DevState {
     uint32_t *unit_pid;     // units grouped by candidate leaf
     uint32_t *ucount;       // units per leaf
     uint32_t *ustart;       // first unit of each leaf in unit_pid
-    double *sx;
-    double *sy;
+    double *sxb[2], *syb[2]; // sorted positions of two ticks (TickParams::sx, sx_old)
     double *sxp;            // positions at prev_tick, sorted like sx/sy
     double *syp;
     uint32_t *spid;
     uint32_t *order;        // object handled by position k of the tick's assignment (spid of the tick before)
+    double2 *ord_xy;        // its position, read once by the assignment
     uint32_t *ord_leaf;     // its leaf and rank in that leaf, by k
     uint32_t *ord_rank;
     uint32_t *ord_pleaf;    // its leaf at the tick before, by k
