@@ -48,8 +48,8 @@ constexpr int kBucketTile = 16384;                  // contacts per LDS tile of 
 constexpr int kFineTile = 12288;                    // contacts per LDS tile of the fine pass (96 KB)
 constexpr uint32_t kMaxCoarse = 1024;               // coarse bins of pass 1
 constexpr uint32_t kMaxFine = 2048;                 // fine bins of pass 2
-constexpr int kBorderStage = 256;                   // staged border objects per wave (assign)
 constexpr uint32_t kInsideChunk = 1;                // leaves per work-queue grab (1 measured fastest of 1, 2, 4, 8)
+constexpr int kBorderStage = 128;                   // staged border objects per wave (assign)
 constexpr uint32_t kHeavyLeaf = 256;                // leaves above this size are joined first
 
 // ------------------------------------------------------------ predicates
@@ -811,8 +811,8 @@ __global__ __launch_bounds__(kBlock) void k_grid_level(DevState s, uint32_t d)
 // advances.  The order used is kept for the scatter.
 __global__ __launch_bounds__(kBlock) void k_assign(DevState s)
 {
-    __shared__ uint2 stage[kBlock / 64][kBorderStage];
-    uint2 *st = stage[threadIdx.x / 64];
+    __shared__ BorderObj stage[kBlock / 64][kBorderStage];
+    BorderObj *st = stage[threadIdx.x / 64];
     uint32_t nst = 0;
     const uint32_t lane = lane_id();
     const double *xy = s.tp->xy;
@@ -880,9 +880,10 @@ __global__ __launch_bounds__(kBlock) void k_assign(DevState s)
             s.ord_rank[k] = rank;
             s.ord_pleaf[k] = pleaf;
         }
-        // border objects are staged per wave and appended one flush at a time
-        unsigned long long m = __ballot(border);
-        if (border) st[nst + __popcll(m & ((1ull << lane) - 1ull))] = make_uint2(i, lv);
+        // border objects (with what the lookup needs) are staged per wave and
+        // appended one flush at a time
+        const unsigned long long m = __ballot(border);
+        if (border) st[nst + __popcll(m & ((1ull << lane) - 1ull))] = BorderObj{i, lv, cur, 0u, x, y};
         nst += (uint32_t)__popcll(m);
         if (nst > kBorderStage - 64) {
             wave_sync();
@@ -1161,7 +1162,6 @@ __device__ __forceinline__ void lookup_walk(const DevState &s, double x, double 
 // registers after one atomic per wave; the others walk a second time.
 __global__ __launch_bounds__(kBlock) void k_lookup(DevState s)
 {
-    const double *xy = s.tp->xy;
     const uint32_t nb = s.ctr->n_border;
     // the scatter before has put the objects in leaf order (read by the next tick)
     if (blockIdx.x == 0 && threadIdx.x == 0) s.ctr->sorted_valid = 1;
@@ -1172,12 +1172,12 @@ __global__ __launch_bounds__(kBlock) void k_lookup(DevState s)
         uint32_t pid = 0, lv = 0, own = 0;
         double x = 0.0, y = 0.0;
         if (k < nb) {
-            const uint2 b = s.border[k];
-            pid = b.x;
-            lv = b.y;
-            x = xy[2 * (size_t)pid];
-            y = xy[2 * (size_t)pid + 1];
-            own = s.leaf_of[pid];
+            const BorderObj b = s.border[k];
+            pid = b.pid;
+            lv = b.lv;
+            x = b.x;
+            y = b.y;
+            own = b.leaf;
             lookup_walk(s, x, y, lv, own, [&](uint32_t c) {
                 f0 = total == 0 ? c : f0;
                 f1 = total == 1 ? c : f1;

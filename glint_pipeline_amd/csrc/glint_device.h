@@ -56,6 +56,12 @@ struct alignas(64) Node {
 };
 static_assert(sizeof(Node) == 64, "one node per 64-byte line");
 
+// An object whose lookup runs this tick: id, last_valid, leaf and position.
+struct alignas(16) BorderObj {
+    uint32_t pid, lv, leaf, pad;
+    double x, y;
+};
+
 struct alignas(16) NodeMeta {
     int32_t type;
     int32_t level;
@@ -196,7 +202,7 @@ struct DevState {
     // an entry not written at the tick read is empty and nothing is reset.
     uint4 *cand;
     uint4 *prev_cand;
-    uint2 *border;
+    BorderObj *border;      // objects whose last_valid is not their leaf, for the lookup
     uint2 *units;           // (object, candidate leaf) in lookup order
     uint32_t *urank;        // rank of the unit among the units of its leaf
     uint32_t *unit_pid;     // units grouped by candidate leaf
