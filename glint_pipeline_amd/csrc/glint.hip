@@ -104,9 +104,11 @@ __global__ void k_fill_u32(uint32_t *p, uint32_t v, uint64_t n)
 
 __global__ void k_set_tick(TickParams *tp, const double *xy, TableGen tab_old, TableGen tab_cur, float4 *mbr_old,
                            float4 *mbr_cur, double *xy_prev, double *xy_save, uint4 *cand_prev, uint4 *cand,
-                           double *sx_old, double *sy_old, double *sx, double *sy, uint32_t tick, uint32_t prev_tick,
-                           uint32_t has_prev)
+                           double *sx_old, double *sy_old, double *sx, double *sy, uint32_t *sleaf_old,
+                           uint32_t *sleaf, uint32_t tick, uint32_t prev_tick, uint32_t has_prev)
 {
+    tp->sleaf = sleaf;
+    tp->sleaf_old = sleaf_old;
     tp->sx = sx;
     tp->sy = sy;
     tp->sx_old = sx_old;
@@ -159,6 +161,9 @@ __global__ void k_advance_tick(TickParams *tp, uint64_t stride)
     uint4 *cb = tp->cand_prev;
     tp->cand_prev = tp->cand;
     tp->cand = cb;
+    uint32_t *sl = tp->sleaf_old;
+    tp->sleaf_old = tp->sleaf;
+    tp->sleaf = sl;
     double *q = tp->sx_old;
     tp->sx_old = tp->sx;
     tp->sx = q;
@@ -171,10 +176,12 @@ __global__ void k_advance_tick(TickParams *tp, uint64_t stride)
 // positions already point at the first snapshot for the initial build.
 __global__ void k_begin_run(TickParams *tp, const double *base, TableGen tab_old, TableGen tab_cur, float4 *mbr_old,
                             float4 *mbr_cur, double *xy_prev, double *xy_save, uint4 *cand_prev, uint4 *cand,
-                            double *sx_old, double *sy_old, double *sx, double *sy, uint32_t tick,
-                            uint32_t next_tick, uint32_t has_prev_next, unsigned long long *stamps, uint32_t nstamps,
-                            uint32_t nticks)
+                            double *sx_old, double *sy_old, double *sx, double *sy, uint32_t *sleaf_old,
+                            uint32_t *sleaf, uint32_t tick, uint32_t next_tick, uint32_t has_prev_next,
+                            unsigned long long *stamps, uint32_t nstamps, uint32_t nticks)
 {
+    tp->sleaf = sleaf;
+    tp->sleaf_old = sleaf_old;
     tp->sx = sx;
     tp->sy = sy;
     tp->sx_old = sx_old;
@@ -833,11 +840,9 @@ __global__ __launch_bounds__(kBlock) void k_assign(DevState s)
         if (active) {
             x = xy[2 * (size_t)i];
             y = xy[2 * (size_t)i + 1];
-            pleaf = s.leaf_of[i];
-            s.prev_leaf[i] = pleaf;
+            pleaf = ordered ? s.tp->sleaf_old[k] : kNone;   // position k held the object then
             if (!(x >= s.lx && x <= s.hx && y >= s.ly && y <= s.hy)) {
                 atomicOr(&s.ctr->error, kErrBadInput);
-                s.leaf_of[i] = kNone;
                 active = false;
             } else {
                 cur = 0;
@@ -872,7 +877,6 @@ __global__ __launch_bounds__(kBlock) void k_assign(DevState s)
                 need = false;
             }
         }
-        if (active) s.leaf_of[i] = cur;
         if (k < s.n) {
             s.order[k] = i;
             s.ord_xy[k] = make_double2(x, y);
@@ -883,7 +887,7 @@ __global__ __launch_bounds__(kBlock) void k_assign(DevState s)
         // border objects (with what the lookup needs) are staged per wave and
         // appended one flush at a time
         const unsigned long long m = __ballot(border);
-        if (border) st[nst + __popcll(m & ((1ull << lane) - 1ull))] = BorderObj{i, lv, cur, 0u, x, y};
+        if (border) st[nst + __popcll(m & ((1ull << lane) - 1ull))] = BorderObj{i, lv, cur, pleaf, x, y};
         nst += (uint32_t)__popcll(m);
         if (nst > kBorderStage - 64) {
             wave_sync();
@@ -1040,6 +1044,7 @@ __global__ __launch_bounds__(kBlock) void k_scatter(DevState s)
         sx[pos] = x;
         sy[pos] = y;
         s.spid[pos] = i;
+        s.tp->sleaf[pos] = leaf;
         s.splf[pos] = s.ord_pleaf[k];
         double px = 0.0, py = 0.0;
         if (has_prev) {
@@ -1169,7 +1174,7 @@ __global__ __launch_bounds__(kBlock) void k_lookup(DevState s)
         const uint32_t k = base + threadIdx.x;
         uint32_t total = 0;
         uint32_t f0 = kNone, f1 = kNone, f2 = kNone;
-        uint32_t pid = 0, lv = 0, own = 0;
+        uint32_t pid = 0, lv = 0, own = 0, pleaf = kNone;
         double x = 0.0, y = 0.0;
         if (k < nb) {
             const BorderObj b = s.border[k];
@@ -1178,6 +1183,7 @@ __global__ __launch_bounds__(kBlock) void k_lookup(DevState s)
             x = b.x;
             y = b.y;
             own = b.leaf;
+            pleaf = b.pleaf;
             lookup_walk(s, x, y, lv, own, [&](uint32_t c) {
                 f0 = total == 0 ? c : f0;
                 f1 = total == 1 ? c : f1;
@@ -1199,9 +1205,12 @@ __global__ __launch_bounds__(kBlock) void k_lookup(DevState s)
         const uint32_t r1 = wave_rank(s.ucount, nc > 1 ? f1 : kNone);
         const uint32_t r2 = wave_rank(s.ucount, nc > 2 ? f2 : kNone);
         unsigned long long u = wbase + wex;
+        // what the join's border rows need of the object, beside each unit
+        const uint2 aux = make_uint2(own | (total > 3 ? 0x80000000u : 0u), pleaf);
         auto write = [&](uint32_t c, uint32_t rank) {
             if (u < s.unit_cap) {
                 s.units[u] = make_uint2(pid, c);
+                s.uaux[u] = aux;
                 s.urank[u] = rank;
             } else {
                 atomicOr(&s.ctr->error, kErrUnitCap);
@@ -1218,15 +1227,18 @@ __global__ __launch_bounds__(kBlock) void k_lookup(DevState s)
     }
 }
 
-// Units grouped by target leaf: unit_pid[ustart[leaf] + rank] = object.
+// Units grouped by target leaf: unit_pid[ustart[leaf] + rank] = object, and
+// its leaf and previous leaf beside it (unit_lf).
 __global__ __launch_bounds__(kBlock) void k_unit_scatter(DevState s)
 {
     if (s.ctr->error & kErrUnitCap) return;   // units were dropped: the tick is void
     const unsigned long long nu = s.ctr->n_units;
     for (unsigned long long u = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x; u < nu;
          u += (unsigned long long)gridDim.x * blockDim.x) {
-        uint2 un = s.units[u];
-        s.unit_pid[s.ustart[un.y] + s.urank[u]] = un.x;
+        const uint2 un = s.units[u];
+        const uint32_t q = s.ustart[un.y] + s.urank[u];
+        s.unit_pid[q] = un.x;
+        s.unit_lf[q] = s.uaux[u];
     }
 }
 
@@ -1643,8 +1655,8 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
     // (unit object `pid`, columns in tile a), against columns [jbeg, jend)
     // of tile C from jmin; `leaf` is the leaf being joined.
     auto run = [&](const JoinTile<TILE> &A, const JoinTile<TILE> &C, bool inside, uint32_t row, uint32_t pid,
-                   bool vi, uint32_t jbeg, uint32_t jend, uint32_t jmin, bool use32, double lx, double ly,
-                   uint32_t leaf) {
+                   uint2 ulf, bool vi, uint32_t jbeg, uint32_t jend, uint32_t jmin, bool use32, double lx,
+                   double ly, uint32_t leaf) {
         double xi = 0.0, yi = 0.0;
         float fxi = 0.f, fyi = 0.f, pfxi = 0.f, pfyi = 0.f;
         uint32_t pi = 0;
@@ -1690,10 +1702,10 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
         if (!vi) jmin = 0xFFFFFFFFu;   // an idle lane
         rows.p[lane] = pi;
         rows.pf[lane] = make_float2(pfxi, pfyi);
-        if (want_pc) rows.ppl[lane] = inside ? A.pl[vi ? row : jbeg] : (vi ? s.prev_leaf[pi] : kNone);
+        if (want_pc) rows.ppl[lane] = inside ? A.pl[vi ? row : jbeg] : (vi ? ulf.y : kNone);
         uint32_t rpl = 0;
         if (!inside && !s.store_all && vi) {
-            rpl = s.leaf_of[pi] | (cand_read(s.tp->cand, pi, s.tp->tick).w > 3 ? 0x80000000u : 0u);
+            rpl = ulf.x;   // the unit object's leaf, bit 31 when its candidates overflowed
             rows.pl[lane] = rpl;
         }
         const JoinCounts jc =
@@ -1802,12 +1814,13 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
                                 }
                                 const uint32_t row = r * 64 + lane;
                                 const uint32_t jbeg = q * 64;
-                                run(lds.a, B, true, row, 0, row < ni, jbeg, min(jbeg + 64, nj), diag ? row + 1 : 0u,
+                                run(lds.a, B, true, row, 0, make_uint2(0u, 0u), row < ni, jbeg, min(jbeg + 64, nj), diag ? row + 1 : 0u,
                                     F.use32, F.lx, F.ly, F.node);
                             } else {
                                 const uint32_t u = (t - ntask_in) * 64 + lane;
                                 const bool vi = u < F.ucnt;
-                                run(lds.a, lds.a, false, 0, vi ? s.unit_pid[F.uoff + u] : 0u, vi, 0, ni, 0, F.use32,
+                                run(lds.a, lds.a, false, 0, vi ? s.unit_pid[F.uoff + u] : 0u,
+                                    vi ? s.unit_lf[F.uoff + u] : make_uint2(0u, 0u), vi, 0, ni, 0, F.use32,
                                     F.lx, F.ly, F.node);
                             }
                         }
@@ -1839,12 +1852,13 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
                         const uint32_t row = L.base + r * 64 + lane;
                         const uint32_t jbeg = L.base + q * 64;
                         const uint32_t jend = L.base + min(q * 64 + 64, L.cnt);
-                        run(lds.a, lds.a, true, row, 0, r * 64 + lane < L.cnt, jbeg, jend,
+                        run(lds.a, lds.a, true, row, 0, make_uint2(0u, 0u), r * 64 + lane < L.cnt, jbeg, jend,
                             r == q ? row + 1 : 0u, L.use32, L.lx, L.ly, L.node);
                     } else {
                         const uint32_t u = (lt - L.ntask_in) * 64 + lane;
                         const bool vi = u < L.ucnt;
-                        run(lds.a, lds.a, false, 0, vi ? s.unit_pid[L.uoff + u] : 0u, vi, L.base, L.base + L.cnt,
+                        run(lds.a, lds.a, false, 0, vi ? s.unit_pid[L.uoff + u] : 0u,
+                            vi ? s.unit_lf[L.uoff + u] : make_uint2(0u, 0u), vi, L.base, L.base + L.cnt,
                             0, L.use32, L.lx, L.ly, L.node);
                     }
                 }
@@ -2601,7 +2615,8 @@ void set_tick_params(glint_ctx *c, const double *xy)
     const int g = c->cur_gen;
     k_set_tick<<<1, 1, 0, c->stream>>>(c->s.tp, xy, c->tab[1 - g], c->tab[g], c->mbr[1 - g], c->mbr[g],
                                        c->hist[1 - g], c->hist[g], c->cand[1 - g], c->cand[g], c->s.sxb[1 - g],
-                                       c->s.syb[1 - g], c->s.sxb[g], c->s.syb[g], c->tick,
+                                       c->s.syb[1 - g], c->s.sxb[g], c->s.syb[g], c->s.sleafb[1 - g],
+                                       c->s.sleafb[g], c->tick,
                                        c->has_prev ? c->prev_tick : c->tick, c->has_prev ? 1u : 0u);
 }
 
@@ -2953,6 +2968,7 @@ glint_status glint_create(const glint_config *cfg, glint_ctx **out)
         s.w32_max = w > 0 ? w : 0.0;
         s.pf_lim = s.w32_max + r;
     }
+
     uint64_t node_cap = cfg->node_capacity ? cfg->node_capacity
                                            : std::max<uint64_t>(65536, 16 * (N / cfg->grid_capacity + 1) + 1024);
     if (node_cap > 0x7FFFFFF0ull) node_cap = 0x7FFFFFF0ull;
@@ -3000,9 +3016,9 @@ glint_status glint_create(const glint_config *cfg, glint_ctx **out)
     ALLOC(s.split_list, node_cap);
     ALLOC(s.merge_list, node_cap);
     ALLOC(s.leaf_list, node_cap);
-    ALLOC(s.leaf_of, N);
+    ALLOC(s.sleafb[0], N);
+    ALLOC(s.sleafb[1], N);
     ALLOC(s.parent, node_cap);
-    ALLOC(s.prev_leaf, N);
     if (!s.store_all) {
         ALLOC(s.cand, N);
         ALLOC(s.prev_cand, N);
@@ -3017,6 +3033,8 @@ glint_status glint_create(const glint_config *cfg, glint_ctx **out)
     ALLOC(s.units, s.unit_cap);
     ALLOC(s.urank, s.unit_cap);
     ALLOC(s.unit_pid, s.unit_cap);
+    ALLOC(s.uaux, s.unit_cap);
+    ALLOC(s.unit_lf, s.unit_cap);
     ALLOC(s.ucount, node_cap);
     ALLOC(s.ustart, node_cap);
     if (cfg->dynamic_schema == GLINT_SCHEMA_GRID_REBUILD) {
@@ -3354,8 +3372,8 @@ static glint_status run_begin(glint_ctx *c, const double *d_trace, uint32_t firs
     // one advance short of the first tick: tables and histories pre-swapped
     k_begin_run<<<1, 1, 0, c->stream>>>(c->s.tp, d_trace, c->tab[g], c->tab[1 - g], c->mbr[g], c->mbr[1 - g],
                                         c->hist[g], c->hist[1 - g], c->cand[g], c->cand[1 - g], c->s.sxb[g],
-                                        c->s.syb[g], c->s.sxb[1 - g], c->s.syb[1 - g],
-                                        c->prev_tick, first_tick, c->has_prev ? 1u : 0u, c->stamps, 2 * nticks,
+                                        c->s.syb[g], c->s.sxb[1 - g], c->s.syb[1 - g], c->s.sleafb[g],
+                                        c->s.sleafb[1 - g], c->prev_tick, first_tick, c->has_prev ? 1u : 0u, c->stamps, 2 * nticks,
                                         nticks);
     HIP_TRY(c, hipGetLastError());
     if (!c->built) {

@@ -58,7 +58,7 @@ static_assert(sizeof(Node) == 64, "one node per 64-byte line");
 
 // An object whose lookup runs this tick: id, last_valid, leaf and position.
 struct alignas(16) BorderObj {
-    uint32_t pid, lv, leaf, pad;
+    uint32_t pid, lv, leaf, pleaf;   // pleaf: its leaf at the tick before
     double x, y;
 };
 
@@ -147,6 +147,8 @@ struct TickParams {
     double *xy_save;            // receives this tick's positions for the next tick (null: not needed)
     double *hist_prev;          // the two position-history buffers, swapped every tick: the one
     double *hist_save;          // holding prev_tick's positions and the one for this tick's
+    uint32_t *sleaf;            // leaf of each sorted object at tick (written by the scatter)
+    uint32_t *sleaf_old;        // the same at prev_tick
     double *sx, *sy;            // objects sorted by leaf at tick (written by the scatter)
     double *sx_old, *sy_old;    // the same at prev_tick (read by the scatter: position k of the
                                 // tick's assignment is the object sorted at k then)
@@ -193,9 +195,11 @@ struct DevState {
     uint32_t *split_list;
     uint32_t *merge_list;
     uint32_t *leaf_list;
-    uint32_t *leaf_of;
     uint32_t *parent;       // per node: its parent (kNone for the root), for stackless tree walks
-    uint32_t *prev_leaf;    // leaf of every object at the tick before (pid order)
+    uint32_t *sleafb[2];    // leaf of every sorted object, two ticks (TickParams::sleaf, sleaf_old)
+    uint2 *uaux;            // per unit (lookup order): the object's leaf (bit 31: its candidates
+                            // overflowed) and its leaf at the tick before
+    uint2 *unit_lf;         // the same grouped like unit_pid
     // Candidate leaves of the lookups, pid order, in two buffers taken in turn
     // (TickParams::cand, cand_prev): up to three leaves (the third is
     // kCandOverflow when there were more) and the tick that wrote them, so
