@@ -1129,9 +1129,10 @@ __device__ __forceinline__ uint32_t lookup_children(const DevState &s, uint4 ids
 // lookup_rec (filter.cpp:130-152) below node `from`: every leaf within r of
 // (x, y) that the top/right rule accepts (in complete mode every leaf within
 // r other than `own`) is passed to take(leaf).  A depth-first walk without a
-// stack: it climbs back through the parent links and resumes after the child
-// it came from (the branch mask of a node is recomputed then), so nothing
-// sits in scratch memory.
+// stack in memory: the branch children still to visit at each of the first
+// 16 levels below `from` are kept as 4-bit masks in one 64-bit register, and
+// the walk climbs back through the parent links (deeper levels recompute
+// their mask from the children), so nothing sits in scratch memory.
 template <class Take>
 __device__ __forceinline__ void lookup_walk(const DevState &s, double x, double y, uint32_t from, uint32_t own,
                                             Take &&take)
@@ -1139,9 +1140,17 @@ __device__ __forceinline__ void lookup_walk(const DevState &s, double x, double 
     uint32_t node = from;
     uint4 ids = node_children(s.nodes, from);
     uint32_t desc = lookup_children(s, ids, x, y, own, true, take);
+    unsigned long long masks = 0;   // level d (< 16): masks >> 4d
+    uint32_t depth = 0;             // of `node` below `from`
     for (;;) {
         if (desc) {
-            node = pick_child(ids, __builtin_ctz(desc));
+            const uint32_t ci = __builtin_ctz(desc);
+            if (depth < 16) {
+                const unsigned long long rest = desc & ~((2u << ci) - 1u);
+                masks = (masks & ~(0xFull << (4 * depth))) | (rest << (4 * depth));
+            }
+            node = pick_child(ids, (int)ci);
+            depth++;
             ids = node_children(s.nodes, node);
             desc = lookup_children(s, ids, x, y, own, true, take);
             continue;
@@ -1152,9 +1161,14 @@ __device__ __forceinline__ void lookup_walk(const DevState &s, double x, double 
             atomicOr(&s.ctr->error, kErrStack);
             break;
         }
+        depth--;
         ids = node_children(s.nodes, p);
-        const uint32_t ci = ids.x == node ? 0u : (ids.y == node ? 1u : (ids.z == node ? 2u : 3u));
-        desc = lookup_children(s, ids, x, y, own, false, take) & ~((2u << ci) - 1u);
+        if (depth < 16) {
+            desc = (uint32_t)(masks >> (4 * depth)) & 0xFu;
+        } else {
+            const uint32_t ci = ids.x == node ? 0u : (ids.y == node ? 1u : (ids.z == node ? 2u : 3u));
+            desc = lookup_children(s, ids, x, y, own, false, take) & ~((2u << ci) - 1u);
+        }
         node = p;
     }
 }
