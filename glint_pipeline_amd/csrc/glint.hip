@@ -1958,8 +1958,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_write(const uint32_t *in, uint3
 __global__ __launch_bounds__(kBucketThreads) void k_bucket_scatter(DevState s)
 {
     __shared__ unsigned long long tile[kBucketTile];
-    __shared__ uint32_t hist[kMaxCoarse + 1], start[kMaxCoarse + 1], cursor[kMaxCoarse + 1],
-        running[kMaxCoarse + 1];
+    __shared__ uint32_t hist[kMaxCoarse + 1], start[kMaxCoarse + 1], running[kMaxCoarse + 1];
     __shared__ uint32_t wsum[kBucketThreads / 64];
     const unsigned long long n = min(s.ctr->n_clist, s.clist_cap);
     const uint32_t nb = s.coarse_bins + 1;
@@ -1972,12 +1971,14 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_scatter(DevState s)
         const uint32_t tn = (uint32_t)min((unsigned long long)kBucketTile, b1 - t0);
         for (uint32_t k = threadIdx.x; k < nb; k += kBucketThreads) hist[k] = 0;
         __syncthreads();
+        // each key's rank in its bin is the count it found (no second pass of atomics)
         unsigned long long keys[kBucketTile / kBucketThreads];
+        uint32_t rk[kBucketTile / kBucketThreads];
 #pragma unroll
         for (int q = 0; q < kBucketTile / kBucketThreads; q++) {
             uint32_t i = threadIdx.x + q * kBucketThreads;
             keys[q] = i < tn ? in[t0 + i] : kEmptyKey;
-            if (i < tn) atomicAdd(&hist[coarse_bin(s, keys[q])], 1u);
+            rk[q] = i < tn ? atomicAdd(&hist[coarse_bin(s, keys[q])], 1u) : 0u;
         }
         __syncthreads();
         // exclusive scan of the tile histogram (nb <= 1025 bins, kPer per thread)
@@ -2000,10 +2001,7 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_scatter(DevState s)
             for (uint32_t w = 0; w < threadIdx.x / 64; w++) ex += wsum[w];
             for (int k = 0; k < kPer; k++) {
                 uint32_t b = threadIdx.x * kPer + k;
-                if (b < nb) {
-                    start[b] = ex;
-                    cursor[b] = ex;
-                }
+                if (b < nb) start[b] = ex;
                 ex += c[k];
             }
         }
@@ -2011,7 +2009,7 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_scatter(DevState s)
 #pragma unroll
         for (int q = 0; q < kBucketTile / kBucketThreads; q++) {
             uint32_t i = threadIdx.x + q * kBucketThreads;
-            if (i < tn) tile[atomicAdd(&cursor[coarse_bin(s, keys[q])], 1u)] = keys[q];
+            if (i < tn) tile[start[coarse_bin(s, keys[q])] + rk[q]] = keys[q];
         }
         __syncthreads();
         for (uint32_t p = threadIdx.x; p < tn; p += kBucketThreads) {
@@ -2061,7 +2059,7 @@ __device__ __forceinline__ uint32_t block_scan_bins(uint32_t *bins, uint32_t nbi
 __global__ __launch_bounds__(kBucketThreads) void k_bucket_fine(DevState s)
 {
     __shared__ unsigned long long tile[kFineTile];
-    __shared__ uint32_t thist[kMaxFine], tstart[kMaxFine], tcur[kMaxFine], gcur[kMaxFine];
+    __shared__ uint32_t thist[kMaxFine], tstart[kMaxFine], gcur[kMaxFine];
     __shared__ uint32_t wsum[kBucketThreads / 64];
     const uint32_t G1 = s.bucket_blocks;
     const uint32_t nf = 1u << s.lo_bits;
@@ -2082,22 +2080,21 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_fine(DevState s)
             for (uint32_t k = threadIdx.x; k < nf; k += kBucketThreads) thist[k] = 0;
             __syncthreads();
             unsigned long long keys[kFineTile / kBucketThreads];
-            uint32_t fb[kFineTile / kBucketThreads];
+            uint32_t fb[kFineTile / kBucketThreads], rk[kFineTile / kBucketThreads];
 #pragma unroll
             for (int q = 0; q < kFineTile / kBucketThreads; q++) {
                 const uint32_t i = threadIdx.x + q * kBucketThreads;
                 keys[q] = i < tn ? s.clist2[t0 + i] : 0ull;
                 fb[q] = region_of(s, keys[q] & kKeyBits) & (nf - 1);
-                if (i < tn) atomicAdd(&thist[fb[q]], 1u);
+                rk[q] = i < tn ? atomicAdd(&thist[fb[q]], 1u) : 0u;   // its rank in the region
             }
             __syncthreads();
             block_scan_bins(thist, nf, tstart, 0, wsum);
-            for (uint32_t k = threadIdx.x; k < nf; k += kBucketThreads) tcur[k] = tstart[k];
             __syncthreads();
 #pragma unroll
             for (int q = 0; q < kFineTile / kBucketThreads; q++) {
                 const uint32_t i = threadIdx.x + q * kBucketThreads;
-                if (i < tn) tile[atomicAdd(&tcur[fb[q]], 1u)] = keys[q];
+                if (i < tn) tile[tstart[fb[q]] + rk[q]] = keys[q];
             }
             __syncthreads();
             for (uint32_t p = threadIdx.x; p < tn; p += kBucketThreads) {
