@@ -1461,8 +1461,15 @@ __device__ __forceinline__ uint4 lane_read4(uint4 x, uint32_t src)
                       (uint32_t)__shfl((int)x.z, (int)src), (uint32_t)__shfl((int)x.w, (int)src));
 }
 
+// Positions of the tick read by the write-out, loaded once per kernel (the
+// tick parameters live in device memory and would be re-read after stores).
+struct TickView {
+    const double *xy, *xy_prev, *sx, *sy;
+};
+
 template <bool USE32, bool BORDER, int TILE>
-__device__ __forceinline__ JoinCounts join_columns(Stager &st, const DevState &s, const JoinTile<TILE> &C, double xi,
+__device__ __forceinline__ JoinCounts join_columns(Stager &st, const DevState &s, const TickView &tv,
+                                                   const JoinTile<TILE> &C, double xi,
                                                    double yi, float fxi, float fyi, float pfxi, float pfyi,
                                                    uint32_t jmin, uint32_t jbeg, uint32_t jend, float t32_in,
                                                    float t32_out, double t64, const JoinRows &rows,
@@ -1488,8 +1495,8 @@ __device__ __forceinline__ JoinCounts join_columns(Stager &st, const DevState &s
         if (USE32 && (e & kBandBit)) {
             band++;
             const uint32_t pa = rows.p[rl], gj = C.g[j];
-            const double ex = s.tp->xy[2 * (size_t)pa] - s.tp->sx[gj];
-            const double ey = s.tp->xy[2 * (size_t)pa + 1] - s.tp->sy[gj];
+            const double ex = tv.xy[2 * (size_t)pa] - tv.sx[gj];
+            const double ey = tv.xy[2 * (size_t)pa + 1] - tv.sy[gj];
             if (!(ex * ex + ey * ey <= t64)) return kEmptyKey;
             n_pass++;
         }
@@ -1507,7 +1514,7 @@ __device__ __forceinline__ JoinCounts join_columns(Stager &st, const DevState &s
             }
             if (dec < 0) {
                 const uint32_t pa = rows.p[rl], gj = C.g[j];
-                const double *xp = s.tp->xy_prev;
+                const double *xp = tv.xy_prev;
                 const double dx = xp[2 * (size_t)pa] - s.sxp[gj];
                 const double dy = xp[2 * (size_t)pa + 1] - s.syp[gj];
                 dec = dx * dx + dy * dy <= t64;
@@ -1655,6 +1662,7 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
     const double *xy = s.tp->xy;
     const double *xy_prev = s.tp->xy_prev;
     const bool has_prev = s.tp->has_prev != 0;
+    const TickView tv{xy, xy_prev, s.tp->sx, s.tp->sy};
     const double t64 = s.t64;
     const uint32_t nh = s.ctr->n_heavy;
     const uint32_t nl = (s.ctr->error & kErrUnitCap) ? 0 : s.ctr->n_leaf_list + nh;
@@ -1723,16 +1731,16 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
             rows.pl[lane] = rpl;
         }
         const JoinCounts jc =
-            use32 ? (inside ? join_columns<true, false>(st, s, C, xi, yi, fxi, fyi, pfxi, pfyi, jmin, jbeg, jend,
+            use32 ? (inside ? join_columns<true, false>(st, s, tv, C, xi, yi, fxi, fyi, pfxi, pfyi, jmin, jbeg, jend,
                                                         s.t32_in, s.t32_out, t64, rows, has_prev, lds.ccand, lds.cmask, leaf,
                                                         rpc, rpl)
-                            : join_columns<true, true>(st, s, C, xi, yi, fxi, fyi, pfxi, pfyi, jmin, jbeg, jend,
+                            : join_columns<true, true>(st, s, tv, C, xi, yi, fxi, fyi, pfxi, pfyi, jmin, jbeg, jend,
                                                        s.t32_in, s.t32_out, t64, rows, has_prev, lds.ccand, lds.cmask, leaf,
                                                        rpc, rpl))
-                  : (inside ? join_columns<false, false>(st, s, C, xi, yi, fxi, fyi, pfxi, pfyi, jmin, jbeg, jend,
+                  : (inside ? join_columns<false, false>(st, s, tv, C, xi, yi, fxi, fyi, pfxi, pfyi, jmin, jbeg, jend,
                                                          s.t32_in, s.t32_out, t64, rows, has_prev, lds.ccand, lds.cmask, leaf,
                                                          rpc, rpl)
-                            : join_columns<false, true>(st, s, C, xi, yi, fxi, fyi, pfxi, pfyi, jmin, jbeg, jend,
+                            : join_columns<false, true>(st, s, tv, C, xi, yi, fxi, fyi, pfxi, pfyi, jmin, jbeg, jend,
                                                         s.t32_in, s.t32_out, t64, rows, has_prev, lds.ccand, lds.cmask, leaf,
                                                         rpc, rpl));
         n_in += inside ? jc.sure : 0u;
