@@ -1526,7 +1526,7 @@ __device__ __forceinline__ JoinCounts join_columns(Stager &st, const DevState &s
             // then, or one's lookup selected the other's leaf (candidate
             // lists that overflowed stored the pair, so they do not count)
             const uint32_t la = rows.ppl[rl], lb = C.pl[j];
-            bool tested = la == lb;
+            bool tested = la == lb && la != kNone;   // kNone: no leaf known at the tick before
             if (!tested) {
                 const uint4 cb = C.pc[j];
                 tested = ca.w <= 3 && cb.w <= 3 &&
