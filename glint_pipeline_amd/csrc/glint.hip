@@ -1467,7 +1467,11 @@ struct TickView {
     const double *xy, *xy_prev, *sx, *sy;
 };
 
-template <bool USE32, bool BORDER, int TILE>
+// ROT: a diagonal inside task (rows and columns the same block of n objects
+// from jbeg): lane i tests the objects i+1 .. i+lim(i) (mod n), lim(i) =
+// (n-1)/2, plus one for i < n/2 when n is even, so each unordered pair is
+// tested once and no lane idles on a lower triangle.
+template <bool USE32, bool BORDER, int TILE, bool ROT = false>
 __device__ __forceinline__ JoinCounts join_columns(Stager &st, const DevState &s, const TickView &tv,
                                                    const JoinTile<TILE> &C, double xi,
                                                    double yi, float fxi, float fyi, float pfxi, float pfyi,
@@ -1548,13 +1552,28 @@ __device__ __forceinline__ JoinCounts join_columns(Stager &st, const DevState &s
         n_dup += twice && lp < leaf;
         return kEmptyKey;
     };
-    for (uint32_t jb = jbeg; jb < jend; jb += kFlushRound) {
+    const uint32_t nblk = jend - jbeg, lane = lane_id();
+    const uint32_t dmax = ROT ? (nblk - 1) / 2 + (nblk % 2 == 0 ? 1u : 0u) : 0u;
+    const uint32_t lim = ROT ? (lane < nblk ? (nblk - 1) / 2 + ((nblk % 2 == 0 && lane < nblk / 2) ? 1u : 0u) : 0u)
+                             : 0u;
+    const uint32_t lstart = ROT ? 1u : jbeg, lend = ROT ? dmax + 1 : jend;
+    for (uint32_t jb = lstart; jb < lend; jb += kFlushRound) {
         TileF cfp[kFlushRound];
         double2 cx[kFlushRound];
         uint32_t cm[kFlushRound];
+        uint32_t cj[kFlushRound];   // ROT: each lane's column
 #pragma unroll
         for (uint32_t jj = 0; jj < kFlushRound; jj++) {
-            const uint32_t jc = min(jb + jj, jend - 1);
+            uint32_t jc;
+            if constexpr (ROT) {
+                const uint32_t dd = min(jb + jj, dmax);
+                uint32_t jr = (lane < nblk ? lane : 0u) + dd;
+                jr = jr >= nblk ? jr - nblk : jr;
+                jc = jbeg + jr;
+                cj[jj] = jc;
+            } else {
+                jc = min(jb + jj, jend - 1);
+            }
             if constexpr (USE32)
                 cfp[jj] = C.fp[jc];
             else
@@ -1565,8 +1584,8 @@ __device__ __forceinline__ JoinCounts join_columns(Stager &st, const DevState &s
         uint32_t ent[kFlushRound];
 #pragma unroll
         for (uint32_t jj = 0; jj < kFlushRound; jj++) {
-            const uint32_t j = jb + jj;
-            const bool cand = j >= jmin && j < jend;
+            const uint32_t j = ROT ? cj[jj] : jb + jj;
+            const bool cand = ROT ? (jb + jj <= lim) : (j >= jmin && j < jend);
             bool sure, stage;
             uint32_t e = tag | j;
             if constexpr (USE32) {
@@ -1677,7 +1696,7 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
     // (unit object `pid`, columns in tile a), against columns [jbeg, jend)
     // of tile C from jmin; `leaf` is the leaf being joined.
     auto run = [&](const JoinTile<TILE> &A, const JoinTile<TILE> &C, bool inside, uint32_t row, uint32_t pid,
-                   uint2 ulf, bool vi, uint32_t jbeg, uint32_t jend, uint32_t jmin, bool use32, double lx,
+                   uint2 ulf, bool vi, uint32_t jbeg, uint32_t jend, uint32_t jmin, bool rot, bool use32, double lx,
                    double ly, uint32_t leaf) {
         double xi = 0.0, yi = 0.0;
         float fxi = 0.f, fyi = 0.f, pfxi = 0.f, pfyi = 0.f;
@@ -1731,6 +1750,13 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
             rows.pl[lane] = rpl;
         }
         const JoinCounts jc =
+            (inside && rot)
+                ? (use32 ? join_columns<true, false, TILE, true>(st, s, tv, C, xi, yi, fxi, fyi, pfxi, pfyi, jmin,
+                                                                jbeg, jend, s.t32_in, s.t32_out, t64, rows, has_prev,
+                                                                lds.ccand, lds.cmask, leaf, rpc, rpl)
+                         : join_columns<false, false, TILE, true>(st, s, tv, C, xi, yi, fxi, fyi, pfxi, pfyi, jmin,
+                                                                 jbeg, jend, s.t32_in, s.t32_out, t64, rows,
+                                                                 has_prev, lds.ccand, lds.cmask, leaf, rpc, rpl)) :
             use32 ? (inside ? join_columns<true, false>(st, s, tv, C, xi, yi, fxi, fyi, pfxi, pfyi, jmin, jbeg, jend,
                                                         s.t32_in, s.t32_out, t64, rows, has_prev, lds.ccand, lds.cmask, leaf,
                                                         rpc, rpl)
@@ -1837,12 +1863,13 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
                                 const uint32_t row = r * 64 + lane;
                                 const uint32_t jbeg = q * 64;
                                 run(lds.a, B, true, row, 0, make_uint2(0u, 0u), row < ni, jbeg, min(jbeg + 64, nj), diag ? row + 1 : 0u,
+                                    diag && r == q,
                                     F.use32, F.lx, F.ly, F.node);
                             } else {
                                 const uint32_t u = (t - ntask_in) * 64 + lane;
                                 const bool vi = u < F.ucnt;
                                 run(lds.a, lds.a, false, 0, vi ? s.unit_pid[F.uoff + u] : 0u,
-                                    vi ? s.unit_lf[F.uoff + u] : make_uint2(0u, 0u), vi, 0, ni, 0, F.use32,
+                                    vi ? s.unit_lf[F.uoff + u] : make_uint2(0u, 0u), vi, 0, ni, 0, false, F.use32,
                                     F.lx, F.ly, F.node);
                             }
                         }
@@ -1875,13 +1902,13 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
                         const uint32_t jbeg = L.base + q * 64;
                         const uint32_t jend = L.base + min(q * 64 + 64, L.cnt);
                         run(lds.a, lds.a, true, row, 0, make_uint2(0u, 0u), r * 64 + lane < L.cnt, jbeg, jend,
-                            r == q ? row + 1 : 0u, L.use32, L.lx, L.ly, L.node);
+                            r == q ? row + 1 : 0u, r == q, L.use32, L.lx, L.ly, L.node);
                     } else {
                         const uint32_t u = (lt - L.ntask_in) * 64 + lane;
                         const bool vi = u < L.ucnt;
                         run(lds.a, lds.a, false, 0, vi ? s.unit_pid[L.uoff + u] : 0u,
                             vi ? s.unit_lf[L.uoff + u] : make_uint2(0u, 0u), vi, L.base, L.base + L.cnt,
-                            0, L.use32, L.lx, L.ly, L.node);
+                            0, false, L.use32, L.lx, L.ly, L.node);
                     }
                 }
                 __syncthreads();
