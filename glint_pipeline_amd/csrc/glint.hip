@@ -38,7 +38,7 @@ constexpr int kBlock = 256;
 constexpr int kScanItems = 4;                       // per thread
 constexpr int kScanTile = kBlock * kScanItems;      // elements per scan block
 constexpr int kTile = 512;                          // points per LDS tile of the join (default)
-constexpr int kStage = 368;                         // staged contacts per wave
+constexpr int kStage = 432;                         // staged contacts per wave
 constexpr int kFlushRound = 4;                      // join columns between stage checks
 constexpr int kMeetStage = 128;                     // staged meetings per wave (table update)
 constexpr uint32_t kChunk = 4096;                   // contact-list entries reserved per atomic
@@ -1293,14 +1293,13 @@ struct Stager {
     uint32_t kept;              // entries written in all (uniform)
 };
 
-// Lanes without a hit write to their spare slot past the stage (no branch);
-// a column step without any hit in the wave writes nothing.
+// The lanes with a hit append e in lane order (mbcnt counts the hits of the
+// lanes below, from st.n); a column step without any hit writes nothing.
 __device__ __forceinline__ void stage_push(Stager &st, bool hit, uint32_t e)
 {
-    const unsigned long long m = __ballot(hit);
+    const unsigned long long m = __builtin_amdgcn_ballot_w64(hit);
     if (m == 0) return;
-    const uint32_t lane = lane_id();
-    st.buf[hit ? st.n + __popcll(m & ((1ull << lane) - 1ull)) : kStage + lane] = e;
+    if (hit) st.buf[__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, st.n))] = e;
     st.n += (uint32_t)__popcll(m);
 }
 
@@ -1431,13 +1430,14 @@ __device__ __forceinline__ void load_tile(JoinTile<TILE> &t, const DevState &s, 
     for (uint32_t k = threadIdx.x; k < n; k += kBlock) tile_put(t, k, s, from + k, has_prev, use32, want_pc);
 }
 
-// Stage entries: row lane << 16 | band << 15 | column.  A band entry is a
-// pair whose FP32 distance lies inside the guard band; it is re-tested in
-// FP64 when the stage is written out.
-constexpr uint32_t kBandBit = 1u << 15;
+// Stage entries: row lane << 16 | column.  In guard mode the write-out
+// repeats the FP32 distance of the entry (the row lane's offsets exchanged
+// across the wave, the same operations as the column loop, so the same
+// result): a pair inside the guard band is re-tested in FP64 there.
 
 // One wave task's column loop: the row lane's object against columns
-// [jbeg, jend) of tile C, from column jmin on (jmin = ~0 for an idle lane).
+// [jbeg, jend) of tile C.  An idle lane's position is infinite, which no
+// distance test accepts.
 // In guard mode (USE32) pairs are decided on the FP32 offsets: inside
 // r - eps a hit, past r + eps a miss, in between a band entry.  Sure hits
 // are counted here and staged only if they may be stored: when every
@@ -1475,12 +1475,15 @@ template <bool USE32, bool BORDER, int TILE, bool ROT = false>
 __device__ __forceinline__ JoinCounts join_columns(Stager &st, const DevState &s, const TickView &tv,
                                                    const JoinTile<TILE> &C, double xi,
                                                    double yi, float fxi, float fyi, float pfxi, float pfyi,
-                                                   uint32_t jmin, uint32_t jbeg, uint32_t jend, float t32_in,
+                                                   uint32_t jbeg, uint32_t jend, float t32_in,
                                                    float t32_out, double t64, const JoinRows &rows,
                                                    bool has_prev, const uint4 *ccand, const uint32_t *cmask,
                                                    uint32_t leaf, uint4 rpc, uint32_t rpl)
 {
     uint32_t n_sure = 0, n_pass = 0, band = 0, n_dup = 0;
+    // the task's columns are the same for every lane: scalar loop bounds
+    jbeg = __builtin_amdgcn_readfirstlane(jbeg);
+    jend = __builtin_amdgcn_readfirstlane(jend);
     const uint32_t tag = lane_id() << 16;
     const unsigned long long flag = BORDER ? kTagBorder : 0ull;
     const bool tested_check = has_prev && !s.store_all;   // wave-uniform
@@ -1492,11 +1495,24 @@ __device__ __forceinline__ JoinCounts join_columns(Stager &st, const DevState &s
     // the stage's contacts as keys (kEmptyKey: a band entry that is not
     // within reach in FP64, or a contact that is not stored)
     auto key_of = [&](uint32_t e, bool valid) {
-        const uint32_t rl = e >> 16, j = e & 0x7FFFu;
+        const uint32_t rl = e >> 16, j = e & 0xFFFFu;
         uint4 ca = make_uint4(kNone, kNone, kNone, 0u);
         if (tested_check) ca = lane_read4(rpc, rl);   // the row object's candidates at prev_tick
+        float rfx = 0.f, rfy = 0.f;
+        if (USE32) {
+            rfx = __shfl(fxi, (int)rl);
+            rfy = __shfl(fyi, (int)rl);
+        }
         if (!valid) return kEmptyKey;
-        if (USE32 && (e & kBandBit)) {
+        bool in_band = false;
+        if (USE32) {
+            const float2 cf = C.fp[j].f;
+            const float dx = rfx - cf.x;
+            const float dy = rfy - cf.y;
+            const float s32 = dx * dx + dy * dy;
+            in_band = !(s32 <= t32_in);
+        }
+        if (in_band) {
             band++;
             const uint32_t pa = rows.p[rl], gj = C.g[j];
             const double ex = tv.xy[2 * (size_t)pa] - tv.sx[gj];
@@ -1585,16 +1601,15 @@ __device__ __forceinline__ JoinCounts join_columns(Stager &st, const DevState &s
 #pragma unroll
         for (uint32_t jj = 0; jj < kFlushRound; jj++) {
             const uint32_t j = ROT ? cj[jj] : jb + jj;
-            const bool cand = ROT ? (jb + jj <= lim) : (j >= jmin && j < jend);
+            const bool cand = ROT ? (jb + jj <= lim) : (j < jend);   // uniform unless ROT
             bool sure, stage;
-            uint32_t e = tag | j;
+            const uint32_t e = tag | j;
             if constexpr (USE32) {
                 const float dx = fxi - cfp[jj].f.x;
                 const float dy = fyi - cfp[jj].f.y;
                 const float s32 = dx * dx + dy * dy;
                 sure = cand && s32 <= t32_in;
                 const bool hit = cand && s32 <= t32_out;
-                if (!sure) e |= kBandBit;
                 const float px = pfxi - cfp[jj].pf.x;
                 const float py = pfyi - cfp[jj].pf.y;
                 const float p32 = px * px + py * py;
@@ -1640,7 +1655,7 @@ template <int TILE>
 struct JoinLds {
     JoinTile<TILE> a, b;
     JoinRows rows[kBlock / 64];
-    uint32_t stage[kBlock / 64][kStage + 64];
+    uint32_t stage[kBlock / 64][kStage];
     uint4 ccand[TILE];                   // candidate leaves of the objects of tile a (see DevState::cand)
     uint32_t cmask[TILE];                // and a 32-bit summary of them (cand_mask)
     JoinLeaf leaf[kInsideChunk];
@@ -1694,9 +1709,10 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
 
     // One task: an inside task (row slot `row` of tile A) or a border task
     // (unit object `pid`, columns in tile a), against columns [jbeg, jend)
-    // of tile C from jmin; `leaf` is the leaf being joined.
+    // of tile C (rot: a diagonal task, each pair once); `leaf` is the leaf
+    // being joined; vi false for an idle lane.
     auto run = [&](const JoinTile<TILE> &A, const JoinTile<TILE> &C, bool inside, uint32_t row, uint32_t pid,
-                   uint2 ulf, bool vi, uint32_t jbeg, uint32_t jend, uint32_t jmin, bool rot, bool use32, double lx,
+                   uint2 ulf, bool vi, uint32_t jbeg, uint32_t jend, bool rot, bool use32, double lx,
                    double ly, uint32_t leaf) {
         double xi = 0.0, yi = 0.0;
         float fxi = 0.f, fyi = 0.f, pfxi = 0.f, pfyi = 0.f;
@@ -1740,7 +1756,10 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
                 }
             }
         }
-        if (!vi) jmin = 0xFFFFFFFFu;   // an idle lane
+        if (!vi) {   // an idle lane: no column is within reach
+            xi = yi = __builtin_inf();
+            fxi = fyi = __builtin_inff();
+        }
         rows.p[lane] = pi;
         rows.pf[lane] = make_float2(pfxi, pfyi);
         if (want_pc) rows.ppl[lane] = inside ? A.pl[vi ? row : jbeg] : (vi ? ulf.y : kNone);
@@ -1751,22 +1770,22 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
         }
         const JoinCounts jc =
             (inside && rot)
-                ? (use32 ? join_columns<true, false, TILE, true>(st, s, tv, C, xi, yi, fxi, fyi, pfxi, pfyi, jmin,
+                ? (use32 ? join_columns<true, false, TILE, true>(st, s, tv, C, xi, yi, fxi, fyi, pfxi, pfyi,
                                                                 jbeg, jend, s.t32_in, s.t32_out, t64, rows, has_prev,
                                                                 lds.ccand, lds.cmask, leaf, rpc, rpl)
-                         : join_columns<false, false, TILE, true>(st, s, tv, C, xi, yi, fxi, fyi, pfxi, pfyi, jmin,
+                         : join_columns<false, false, TILE, true>(st, s, tv, C, xi, yi, fxi, fyi, pfxi, pfyi,
                                                                  jbeg, jend, s.t32_in, s.t32_out, t64, rows,
                                                                  has_prev, lds.ccand, lds.cmask, leaf, rpc, rpl)) :
-            use32 ? (inside ? join_columns<true, false>(st, s, tv, C, xi, yi, fxi, fyi, pfxi, pfyi, jmin, jbeg, jend,
+            use32 ? (inside ? join_columns<true, false>(st, s, tv, C, xi, yi, fxi, fyi, pfxi, pfyi, jbeg, jend,
                                                         s.t32_in, s.t32_out, t64, rows, has_prev, lds.ccand, lds.cmask, leaf,
                                                         rpc, rpl)
-                            : join_columns<true, true>(st, s, tv, C, xi, yi, fxi, fyi, pfxi, pfyi, jmin, jbeg, jend,
+                            : join_columns<true, true>(st, s, tv, C, xi, yi, fxi, fyi, pfxi, pfyi, jbeg, jend,
                                                        s.t32_in, s.t32_out, t64, rows, has_prev, lds.ccand, lds.cmask, leaf,
                                                        rpc, rpl))
-                  : (inside ? join_columns<false, false>(st, s, tv, C, xi, yi, fxi, fyi, pfxi, pfyi, jmin, jbeg, jend,
+                  : (inside ? join_columns<false, false>(st, s, tv, C, xi, yi, fxi, fyi, pfxi, pfyi, jbeg, jend,
                                                          s.t32_in, s.t32_out, t64, rows, has_prev, lds.ccand, lds.cmask, leaf,
                                                          rpc, rpl)
-                            : join_columns<false, true>(st, s, tv, C, xi, yi, fxi, fyi, pfxi, pfyi, jmin, jbeg, jend,
+                            : join_columns<false, true>(st, s, tv, C, xi, yi, fxi, fyi, pfxi, pfyi, jbeg, jend,
                                                         s.t32_in, s.t32_out, t64, rows, has_prev, lds.ccand, lds.cmask, leaf,
                                                         rpc, rpl));
         n_in += inside ? jc.sure : 0u;
@@ -1862,14 +1881,13 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
                                 }
                                 const uint32_t row = r * 64 + lane;
                                 const uint32_t jbeg = q * 64;
-                                run(lds.a, B, true, row, 0, make_uint2(0u, 0u), row < ni, jbeg, min(jbeg + 64, nj), diag ? row + 1 : 0u,
-                                    diag && r == q,
+                                run(lds.a, B, true, row, 0, make_uint2(0u, 0u), row < ni, jbeg, min(jbeg + 64, nj), diag && r == q,
                                     F.use32, F.lx, F.ly, F.node);
                             } else {
                                 const uint32_t u = (t - ntask_in) * 64 + lane;
                                 const bool vi = u < F.ucnt;
                                 run(lds.a, lds.a, false, 0, vi ? s.unit_pid[F.uoff + u] : 0u,
-                                    vi ? s.unit_lf[F.uoff + u] : make_uint2(0u, 0u), vi, 0, ni, 0, false, F.use32,
+                                    vi ? s.unit_lf[F.uoff + u] : make_uint2(0u, 0u), vi, 0, ni, false, F.use32,
                                     F.lx, F.ly, F.node);
                             }
                         }
@@ -1902,13 +1920,13 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
                         const uint32_t jbeg = L.base + q * 64;
                         const uint32_t jend = L.base + min(q * 64 + 64, L.cnt);
                         run(lds.a, lds.a, true, row, 0, make_uint2(0u, 0u), r * 64 + lane < L.cnt, jbeg, jend,
-                            r == q ? row + 1 : 0u, r == q, L.use32, L.lx, L.ly, L.node);
+                            r == q, L.use32, L.lx, L.ly, L.node);
                     } else {
                         const uint32_t u = (lt - L.ntask_in) * 64 + lane;
                         const bool vi = u < L.ucnt;
                         run(lds.a, lds.a, false, 0, vi ? s.unit_pid[L.uoff + u] : 0u,
                             vi ? s.unit_lf[L.uoff + u] : make_uint2(0u, 0u), vi, L.base, L.base + L.cnt,
-                            0, false, L.use32, L.lx, L.ly, L.node);
+                            false, L.use32, L.lx, L.ly, L.node);
                     }
                 }
                 __syncthreads();
