@@ -1376,6 +1376,12 @@ __device__ __forceinline__ uint32_t cand_mask(uint4 c)
            (c.z != kNone ? leaf_bit(c.z) : 0u);
 }
 
+// Squared FP32 distance of a guard-band test: dy*dy rounded, then dx*dx added
+// in one fused operation (one rounding fewer than a product and a sum; the
+// band's error bound covers both).  The column loop and the write-out use it
+// alike, so they take the same decisions.
+__device__ __forceinline__ float dist2_f32(float dx, float dy) { return __builtin_fmaf(dx, dx, dy * dy); }
+
 // Positions of a tile object: FP64 for a leaf joined in FP64, else the FP32
 // offsets from the leaf's low corner at tick and at prev_tick (NaN: not
 // bounded).  The FP64 positions of a guard-band leaf stay in HBM (sx, sy),
@@ -1509,7 +1515,7 @@ __device__ __forceinline__ JoinCounts join_columns(Stager &st, const DevState &s
             const float2 cf = C.fp[j].f;
             const float dx = rfx - cf.x;
             const float dy = rfy - cf.y;
-            const float s32 = dx * dx + dy * dy;
+            const float s32 = dist2_f32(dx, dy);
             in_band = !(s32 <= t32_in);
         }
         if (in_band) {
@@ -1529,7 +1535,7 @@ __device__ __forceinline__ JoinCounts join_columns(Stager &st, const DevState &s
                 const float2 a = rows.pf[rl], b = C.fp[j].pf;
                 const float px = a.x - b.x;
                 const float py = a.y - b.y;
-                const float p32 = px * px + py * py;
+                const float p32 = dist2_f32(px, py);
                 dec = p32 <= t32_in ? 1 : (p32 > t32_out ? 0 : -1);   // NaN: -1
             }
             if (dec < 0) {
@@ -1607,12 +1613,12 @@ __device__ __forceinline__ JoinCounts join_columns(Stager &st, const DevState &s
             if constexpr (USE32) {
                 const float dx = fxi - cfp[jj].f.x;
                 const float dy = fyi - cfp[jj].f.y;
-                const float s32 = dx * dx + dy * dy;
+                const float s32 = dist2_f32(dx, dy);
                 sure = cand && s32 <= t32_in;
                 const bool hit = cand && s32 <= t32_out;
                 const float px = pfxi - cfp[jj].pf.x;
                 const float py = pfyi - cfp[jj].pf.y;
-                const float p32 = px * px + py * py;
+                const float p32 = dist2_f32(px, py);
                 // NaN (unbounded offsets) compares false: may be tagged
                 const bool maybe_tagged = has_prev && !(p32 > t32_out);
                 if (settle)
@@ -3025,7 +3031,8 @@ glint_status glint_create(const glint_config *cfg, glint_ctx **out)
         s.t32_out = fh;
         // Largest leaf side whose FP32 offsets keep |d32 - d64| well inside
         // the guard band: offsets carry <= u*(W + r) error each (u = 2^-24),
-        // the FP32 difference, squares and sum add <= (2+sqrt2)*u*(r+eps);
+        // the FP32 difference, squares and sum (the sum fused with one square)
+        // add <= (2+sqrt2)*u*(r+eps);
         // the bound is 2*sqrt2*u*(W+r) + (2+sqrt2)*u*(r+eps) < eps/2.
         const double u = std::ldexp(1.0, -24);
         double w = (eps / 2 - (2 + std::sqrt(2.0)) * u * (r + eps)) / (2 * std::sqrt(2.0) * u * (1 + u)) - r;
