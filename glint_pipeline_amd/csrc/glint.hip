@@ -2377,16 +2377,27 @@ __global__ __launch_bounds__(NT) void k_table(DevState s)
     uint32_t nst = 0, dups = 0;
     const bool void_tick = (s.ctr->error & kErrContactCap) != 0;
     // the previous generation's entries and the tick's contacts of the next
-    // region are loaded into registers while the current region is processed
+    // region are loaded into registers while the current region is processed;
+    // the bounds of the region after that are loaded one region earlier still,
+    // so that the entry loads do not wait for them
     constexpr int kPer = kRegion / NT;
     unsigned long long pk[kPer], nk[kPer], nkn[kPer];
     uint32_t ps[kPer], ns[kPer];
     uint32_t o0 = 0, o1 = 0, c0n = 0, c1n = 0;
+    uint32_t bo0 = 0, bo1 = 0, bc0 = 0, bc1 = 0;
+    auto bounds = [&](uint32_t rr) {
+        if (rr >= nreg) return;
+        bo0 = told.rstart[rr];
+        bo1 = told.rstart[rr + 1];
+        bc0 = void_tick ? 0 : tcur.rstart[rr];
+        bc1 = void_tick ? 0 : tcur.rstart[rr + 1];
+    };
     auto prefetch = [&](uint32_t rr) {
-        o0 = told.rstart[rr];
-        o1 = told.rstart[rr + 1];
-        c0n = void_tick ? 0 : tcur.rstart[rr];
-        c1n = void_tick ? 0 : tcur.rstart[rr + 1];
+        o0 = bo0;
+        o1 = bo1;
+        c0n = bc0;
+        c1n = bc1;
+        bounds(rr + gridDim.x);
 #pragma unroll
         for (int q = 0; q < kPer; q++) {
             const uint32_t i = o0 + threadIdx.x + q * NT;
@@ -2396,6 +2407,7 @@ __global__ __launch_bounds__(NT) void k_table(DevState s)
             nkn[q] = c < c1n ? tcur.keys[c] : 0ull;
         }
     };
+    bounds(blockIdx.x);
     if (blockIdx.x < nreg) prefetch(blockIdx.x);
     for (uint32_t r = blockIdx.x; r < nreg; r += gridDim.x) {
         for (uint32_t k = threadIdx.x; k < kRegion; k += NT) L.hk[k] = kEmptyKey;
