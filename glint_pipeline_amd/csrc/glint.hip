@@ -887,7 +887,8 @@ __global__ __launch_bounds__(kBlock) void k_assign(DevState s)
         // border objects (with what the lookup needs) are staged per wave and
         // appended one flush at a time
         const unsigned long long m = __ballot(border);
-        if (border) st[nst + __popcll(m & ((1ull << lane) - 1ull))] = BorderObj{i, lv, cur, pleaf, x, y};
+        if (border)
+            st[nst + __popcll(m & ((1ull << lane) - 1ull))] = BorderObj{i, lv, cur, pleaf, ordered ? k : kNone, rank, x, y};
         nst += (uint32_t)__popcll(m);
         if (nst > kBorderStage - 64) {
             wave_sync();
@@ -1044,6 +1045,7 @@ __global__ __launch_bounds__(kBlock) void k_scatter(DevState s)
         sx[pos] = x;
         sy[pos] = y;
         s.spid[pos] = i;
+        s.sprev[pos] = ordered ? k : kNone;
         s.tp->sleaf[pos] = leaf;
         s.splf[pos] = s.ord_pleaf[k];
         double px = 0.0, py = 0.0;
@@ -1188,7 +1190,7 @@ __global__ __launch_bounds__(kBlock) void k_lookup(DevState s)
         const uint32_t k = base + threadIdx.x;
         uint32_t total = 0;
         uint32_t f0 = kNone, f1 = kNone, f2 = kNone;
-        uint32_t pid = 0, lv = 0, own = 0, pleaf = kNone;
+        uint32_t pid = 0, lv = 0, own = 0, pleaf = kNone, prevk = kNone, pos = 0;
         double x = 0.0, y = 0.0;
         if (k < nb) {
             const BorderObj b = s.border[k];
@@ -1198,13 +1200,15 @@ __global__ __launch_bounds__(kBlock) void k_lookup(DevState s)
             y = b.y;
             own = b.leaf;
             pleaf = b.pleaf;
+            prevk = b.prevk;
+            pos = s.offset[own] + b.rank;
             lookup_walk(s, x, y, lv, own, [&](uint32_t c) {
                 f0 = total == 0 ? c : f0;
                 f1 = total == 1 ? c : f1;
                 f2 = total == 2 ? c : f2;
                 total++;
             });
-            if (!s.store_all) s.tp->cand[pid] = make_uint4(f0, f1, total > 3 ? kCandOverflow : f2, s.tp->tick);
+            if (!s.store_all) s.tp->cand[pos] = make_uint4(f0, f1, total > 3 ? kCandOverflow : f2, s.tp->tick);
         }
         // one atomic per wave reserves the units of the lists of up to three;
         // lanes that share a candidate leaf take consecutive ranks there with
@@ -1220,7 +1224,7 @@ __global__ __launch_bounds__(kBlock) void k_lookup(DevState s)
         const uint32_t r2 = wave_rank(s.ucount, nc > 2 ? f2 : kNone);
         unsigned long long u = wbase + wex;
         // what the join's border rows need of the object, beside each unit
-        const uint2 aux = make_uint2(own | (total > 3 ? 0x80000000u : 0u), pleaf);
+        const uint4 aux = make_uint4(own | (total > 3 ? 0x80000000u : 0u), pleaf, prevk, pos);
         auto write = [&](uint32_t c, uint32_t rank) {
             if (u < s.unit_cap) {
                 s.units[u] = make_uint2(pid, c);
@@ -1252,7 +1256,7 @@ __global__ __launch_bounds__(kBlock) void k_unit_scatter(DevState s)
         const uint2 un = s.units[u];
         const uint32_t q = s.ustart[un.y] + s.urank[u];
         s.unit_pid[q] = un.x;
-        s.unit_lf[q] = s.uaux[u];
+        s.unit_lf[q] = s.uaux[u];   // 16 bytes: leaf, previous leaf, previous and current sorted position
     }
 }
 
@@ -1356,14 +1360,22 @@ __device__ __forceinline__ unsigned long long pair_key(uint32_t a, uint32_t b)
     return ((unsigned long long)lo << 32) | hi;
 }
 
-// Candidate list of object pid as the join tests it: up to three leaves and
-// w = 0, or w = 4 when the lookup had more (an entry not written at `stamp`
-// is empty).
-__device__ __forceinline__ uint4 cand_read(const uint4 *buf, uint32_t pid, uint32_t stamp)
+// Candidate list of the object sorted at `at` in the tick that wrote buf, as
+// the join tests it: up to three leaves and w = 0, or w = 4 when the lookup
+// had more (an entry not written at `stamp` is empty).
+__device__ __forceinline__ uint4 cand_read(const uint4 *buf, uint32_t at, uint32_t stamp)
 {
-    const uint4 c = buf[pid];
+    const uint4 c = buf[at];
     if (c.w != stamp) return make_uint4(kNone, kNone, kNone, 0u);
     return c.z == kCandOverflow ? make_uint4(kNone, kNone, kNone, 4u) : make_uint4(c.x, c.y, c.z, 0u);
+}
+
+// The same at a previous sorted position that may be unknown (at >= n): an
+// empty list.  Positions are unknown only before the first lookup, when no
+// tick before is evaluated.
+__device__ __forceinline__ uint4 cand_at(const uint4 *buf, uint32_t at, uint32_t n, uint32_t stamp)
+{
+    return at < n ? cand_read(buf, at, stamp) : make_uint4(kNone, kNone, kNone, 0u);
 }
 
 // One bit per leaf id: a 32-bit summary of a candidate list whose clear bits
@@ -1425,7 +1437,7 @@ __device__ __forceinline__ void tile_put(JoinTile<TILE> &t, uint32_t k, const De
     t.pl[k] = s.splf[from];
     const uint32_t pid = s.spid[from];
     t.p[k] = pid;
-    if (want_pc) t.pc[k] = cand_read(s.tp->cand_prev, pid, s.tp->prev_tick);
+    if (want_pc) t.pc[k] = cand_at(s.tp->cand_prev, s.sprev[from], s.n, s.tp->prev_tick);
 }
 
 template <int TILE>
@@ -1718,7 +1730,7 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
     // of tile C (rot: a diagonal task, each pair once); `leaf` is the leaf
     // being joined; vi false for an idle lane.
     auto run = [&](const JoinTile<TILE> &A, const JoinTile<TILE> &C, bool inside, uint32_t row, uint32_t pid,
-                   uint2 ulf, bool vi, uint32_t jbeg, uint32_t jend, bool rot, bool use32, double lx,
+                   uint4 ulf, bool vi, uint32_t jbeg, uint32_t jend, bool rot, bool use32, double lx,
                    double ly, uint32_t leaf) {
         double xi = 0.0, yi = 0.0;
         float fxi = 0.f, fyi = 0.f, pfxi = 0.f, pfyi = 0.f;
@@ -1744,13 +1756,15 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
             pi = pid;
             double pxi = 0.0, pyi = 0.0;
             if (vi) {
-                xi = xy[2 * (size_t)pi];
-                yi = xy[2 * (size_t)pi + 1];
+                // read at the object's sorted positions (units of a leaf come
+                // from a few neighbouring leaves), not by id
+                xi = tv.sx[ulf.w];
+                yi = tv.sy[ulf.w];
                 if (has_prev) {
-                    pxi = xy_prev[2 * (size_t)pi];
-                    pyi = xy_prev[2 * (size_t)pi + 1];
+                    pxi = s.sxp[ulf.w];
+                    pyi = s.syp[ulf.w];
                 }
-                if (want_pc) rpc = cand_read(s.tp->cand_prev, pi, s.tp->prev_tick);
+                if (want_pc) rpc = cand_at(s.tp->cand_prev, ulf.z, s.n, s.tp->prev_tick);
             }
             if (use32) {
                 fxi = (float)(xi - lx);
@@ -1865,7 +1879,7 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
                     if (!s.store_all)
                         for (uint32_t k = threadIdx.x; k < ni; k += kBlock)
                             lds.cmask[k] = cand_mask(
-                                lds.ccand[k] = cand_read(s.tp->cand, s.spid[F.off + I * TILE + k], s.tp->tick));
+                                lds.ccand[k] = cand_read(s.tp->cand, F.off + I * TILE + k, s.tp->tick));
                     for (uint32_t J = I; J < ntile; J++) {
                         const uint32_t nj = min((uint32_t)TILE, F.cnt - J * TILE);
                         const bool diag = (J == I);
@@ -1887,13 +1901,13 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
                                 }
                                 const uint32_t row = r * 64 + lane;
                                 const uint32_t jbeg = q * 64;
-                                run(lds.a, B, true, row, 0, make_uint2(0u, 0u), row < ni, jbeg, min(jbeg + 64, nj), diag && r == q,
+                                run(lds.a, B, true, row, 0, make_uint4(0u, 0u, 0u, 0u), row < ni, jbeg, min(jbeg + 64, nj), diag && r == q,
                                     F.use32, F.lx, F.ly, F.node);
                             } else {
                                 const uint32_t u = (t - ntask_in) * 64 + lane;
                                 const bool vi = u < F.ucnt;
                                 run(lds.a, lds.a, false, 0, vi ? s.unit_pid[F.uoff + u] : 0u,
-                                    vi ? s.unit_lf[F.uoff + u] : make_uint2(0u, 0u), vi, 0, ni, false, F.use32,
+                                    vi ? s.unit_lf[F.uoff + u] : make_uint4(0u, 0u, 0u, 0u), vi, 0, ni, false, F.use32,
                                     F.lx, F.ly, F.node);
                             }
                         }
@@ -1911,7 +1925,7 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
                     const uint32_t from = L.off + (i - L.base);
                     tile_put(lds.a, i, s, from, has_prev, L.use32, has_prev && !s.store_all);
                     if (!s.store_all)
-                        lds.cmask[i] = cand_mask(lds.ccand[i] = cand_read(s.tp->cand, lds.a.p[i], s.tp->tick));
+                        lds.cmask[i] = cand_mask(lds.ccand[i] = cand_read(s.tp->cand, from, s.tp->tick));
                 }
                 __syncthreads();
                 for (uint32_t t = wave; t < ntask; t += kBlock / 64) {
@@ -1925,13 +1939,13 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
                         const uint32_t row = L.base + r * 64 + lane;
                         const uint32_t jbeg = L.base + q * 64;
                         const uint32_t jend = L.base + min(q * 64 + 64, L.cnt);
-                        run(lds.a, lds.a, true, row, 0, make_uint2(0u, 0u), r * 64 + lane < L.cnt, jbeg, jend,
+                        run(lds.a, lds.a, true, row, 0, make_uint4(0u, 0u, 0u, 0u), r * 64 + lane < L.cnt, jbeg, jend,
                             r == q, L.use32, L.lx, L.ly, L.node);
                     } else {
                         const uint32_t u = (lt - L.ntask_in) * 64 + lane;
                         const bool vi = u < L.ucnt;
                         run(lds.a, lds.a, false, 0, vi ? s.unit_pid[L.uoff + u] : 0u,
-                            vi ? s.unit_lf[L.uoff + u] : make_uint2(0u, 0u), vi, L.base, L.base + L.cnt,
+                            vi ? s.unit_lf[L.uoff + u] : make_uint4(0u, 0u, 0u, 0u), vi, L.base, L.base + L.cnt,
                             false, L.use32, L.lx, L.ly, L.node);
                     }
                 }
@@ -3142,6 +3156,7 @@ glint_status glint_create(const glint_config *cfg, glint_ctx **out)
     ALLOC(c->hist[0], 2 * N);
     ALLOC(c->hist[1], 2 * N);
     ALLOC(s.spid, N);
+    ALLOC(s.sprev, N);
     ALLOC(s.order, N);
     ALLOC(s.ord_leaf, N);
     ALLOC(s.ord_rank, N);

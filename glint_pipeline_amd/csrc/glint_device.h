@@ -57,8 +57,10 @@ struct alignas(64) Node {
 static_assert(sizeof(Node) == 64, "one node per 64-byte line");
 
 // An object whose lookup runs this tick: id, last_valid, leaf and position.
-struct alignas(16) BorderObj {
+struct alignas(8) BorderObj {
     uint32_t pid, lv, leaf, pleaf;   // pleaf: its leaf at the tick before
+    uint32_t prevk;                  // its sorted position at the tick before (kNone: none known)
+    uint32_t rank;                   // its rank in its leaf (sorted position offset[leaf] + rank)
     double x, y;
 };
 
@@ -197,13 +199,15 @@ struct DevState {
     uint32_t *leaf_list;
     uint32_t *parent;       // per node: its parent (kNone for the root), for stackless tree walks
     uint32_t *sleafb[2];    // leaf of every sorted object, two ticks (TickParams::sleaf, sleaf_old)
-    uint2 *uaux;            // per unit (lookup order): the object's leaf (bit 31: its candidates
-                            // overflowed) and its leaf at the tick before
-    uint2 *unit_lf;         // the same grouped like unit_pid
-    // Candidate leaves of the lookups, pid order, in two buffers taken in turn
-    // (TickParams::cand, cand_prev): up to three leaves (the third is
-    // kCandOverflow when there were more) and the tick that wrote them, so
-    // an entry not written at the tick read is empty and nothing is reset.
+    uint4 *uaux;            // per unit (lookup order): the object's leaf (bit 31: its candidates
+                            // overflowed), its leaf at the tick before, its sorted position at
+                            // the tick before (kNone: none known) and at the tick
+    uint4 *unit_lf;         // the same grouped like unit_pid
+    // Candidate leaves of the lookups, at the object's sorted position of the
+    // tick that wrote them, in two buffers taken in turn (TickParams::cand,
+    // cand_prev): up to three leaves (the third is kCandOverflow when there
+    // were more) and that tick, so an entry not written at the tick read is
+    // empty and nothing is reset.
     uint4 *cand;
     uint4 *prev_cand;
     BorderObj *border;      // objects whose last_valid is not their leaf, for the lookup
@@ -215,6 +219,7 @@ struct DevState {
     double *sxb[2], *syb[2]; // sorted positions of two ticks (TickParams::sx, sx_old)
     double *sxp;            // positions at prev_tick, sorted like sx/sy
     double *syp;
+    uint32_t *sprev;        // sorted position at prev_tick of each sorted object (kNone: none known)
     uint32_t *spid;
     uint32_t *order;        // object handled by position k of the tick's assignment (spid of the tick before)
     double2 *ord_xy;        // its position, read once by the assignment
