@@ -221,7 +221,7 @@ def main() -> int:
     if args.autotune:
         t0 = time.perf_counter()
         res = autotune.tune(w, tune_capacity=bool(spec["tuned"]), warmup=min(3, W), ticks=4, trace=trace,
-                            distinct_hint=distinct, fixed={"precision": spec["precision"]}, log=log)
+                            distinct_hint=distinct, fixed={"precision": spec["precision"]}, start=params, log=log)
         params.update(res.best)
         tuning = {"best": res.best, "best_ms_per_tick": round(res.best_ms, 3), "trials": res.trials,
                   "seconds": round(time.perf_counter() - t0, 1)}

@@ -31,7 +31,7 @@ TUNED_PATH = Path(__file__).resolve().parent / "tuned.json"
 DEFAULTS = {"join_tile": 512, "table_threads": 512, "load_factor": 0.75, "precision": "fp32guard"}
 SPACE = {"join_tile": [256, 512], "table_threads": [256, 512, 1024], "load_factor": [0.375, 0.75, 1.5, 3.0, 6.0],
          "precision": ["fp64", "fp32guard"]}
-LEAF_CAPACITIES = [64, 128, 256, 512, 1024]
+LEAF_CAPACITIES = [64, 128, 192, 256, 320, 384, 512, 1024]
 
 
 def size_class(n: int) -> str:
@@ -117,19 +117,21 @@ def device_trace(w: gen.Workload, ticks: int) -> runtime.DeviceBuffer:
 
 def tune(w: gen.Workload, tune_capacity: bool = False, warmup: int = 3, ticks: int = 5,
          trace: runtime.DeviceBuffer | None = None, distinct_hint: float | None = None,
-         fixed: dict | None = None, log=print) -> TuneResult:
+         fixed: dict | None = None, start: dict | None = None, log=print) -> TuneResult:
     """distinct_hint: expected distinct pairs per tick (sizes the probe run;
-    default 4 per object); fixed: knobs the workload prescribes (not searched)."""
+    default 4 per object); fixed: knobs the workload prescribes (not searched);
+    start: the point the search starts from (default: DEFAULTS), e.g. the
+    stored parameters, so that each knob is tried beside the others' best."""
     own = trace is None
     if own:
         trace = device_trace(w, warmup + ticks)
     try:
         hint = distinct_hint if distinct_hint else 4.0 * w.num_objects
         distinct = probe_distinct(w, trace, warmup + ticks, hint) * 1.25
-        best = {**DEFAULTS, **(fixed or {})}
+        best = {**DEFAULTS, **(start or {}), **(fixed or {})}
         knobs = {k: v for k, v in SPACE.items() if k not in (fixed or {})}
         if tune_capacity:
-            best["grid_capacity"] = w.grid_capacity
+            best["grid_capacity"] = (start or {}).get("grid_capacity", w.grid_capacity)
             knobs = {"grid_capacity": LEAF_CAPACITIES, **knobs}
         res = TuneResult(dict(best), float("inf"))
         seen = {}
