@@ -2004,7 +2004,16 @@ __global__ __launch_bounds__(kBlock) void k_bucket_count(DevState s)
     unsigned long long b0, b1;
     block_range(s, n, b0, b1);
     const unsigned long long *in = contact_out(s);
-    for (unsigned long long i = b0 + threadIdx.x; i < b1; i += kBlock) atomicAdd(&hist[coarse_bin(s, in[i])], 1u);
+    // four independent loads in flight per thread
+    constexpr int kU = 4;
+    for (unsigned long long i0 = b0 + threadIdx.x; i0 < b1; i0 += kU * kBlock) {
+        unsigned long long key[kU];
+#pragma unroll
+        for (int q = 0; q < kU; q++) key[q] = i0 + q * kBlock < b1 ? in[i0 + q * kBlock] : 0ull;
+#pragma unroll
+        for (int q = 0; q < kU; q++)
+            if (i0 + q * kBlock < b1) atomicAdd(&hist[coarse_bin(s, key[q])], 1u);
+    }
     __syncthreads();
     for (uint32_t k = threadIdx.x; k <= s.coarse_bins; k += kBlock) s.bcount[(size_t)k * gridDim.x + blockIdx.x] = hist[k];
 }
@@ -2142,8 +2151,14 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_fine(DevState s)
         const uint32_t seg0 = s.boff[(size_t)c * G1], seg1 = s.boff[(size_t)(c + 1) * G1];
         for (uint32_t k = threadIdx.x; k < nf; k += kBucketThreads) thist[k] = 0;
         __syncthreads();
-        for (uint32_t i = seg0 + threadIdx.x; i < seg1; i += kBucketThreads)
-            atomicAdd(&thist[region_of(s, s.clist2[i] & kKeyBits) & (nf - 1)], 1u);
+        for (uint32_t i0 = seg0 + threadIdx.x; i0 < seg1; i0 += 4 * kBucketThreads) {
+            unsigned long long key[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) key[q] = i0 + q * kBucketThreads < seg1 ? s.clist2[i0 + q * kBucketThreads] : 0ull;
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+                if (i0 + q * kBucketThreads < seg1) atomicAdd(&thist[region_of(s, key[q] & kKeyBits) & (nf - 1)], 1u);
+        }
         __syncthreads();
         block_scan_bins(thist, nf, gcur, seg0, wsum);
         for (uint32_t k = threadIdx.x; k < nf; k += kBucketThreads) rstart[(size_t)c * nf + k] = gcur[k];
