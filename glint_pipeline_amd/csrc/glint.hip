@@ -826,78 +826,112 @@ __global__ __launch_bounds__(kBlock) void k_assign(DevState s)
     const double r = s.r;
     const bool root_leaf = s.nodes[0].child[0] == kLeafTag;
     const bool ordered = s.ctr->sorted_valid != 0;
-    for (uint32_t base = blockIdx.x * blockDim.x; base < s.n; base += gridDim.x * blockDim.x) {
-        const uint32_t k = base + threadIdx.x;
-        bool active = k < s.n;
-        uint32_t i = k;
-        if (active && ordered) {
-            i = s.spid[k];
-            if (i >= s.n) i = k;   // not a permutation: only after an earlier error
-        }
-        bool border = false;
-        uint32_t lv = 0, cur = kNone, pleaf = kNone;
-        double x = 0.0, y = 0.0;
-        if (active) {
-            x = xy[2 * (size_t)i];
-            y = xy[2 * (size_t)i + 1];
-            pleaf = ordered ? s.tp->sleaf_old[k] : kNone;   // position k held the object then
-            if (!(x >= s.lx && x <= s.hx && y >= s.ly && y <= s.hy)) {
-                atomicOr(&s.ctr->error, kErrBadInput);
-                active = false;
-            } else {
-                cur = 0;
-                double2 mid = node_mid(s.nodes, 0);
-                uint4 ch = node_children(s.nodes, 0);
-                while (ch.x != kLeafTag) {
-                    cur = pick_child(ch, (y > mid.y) * 2 + (x > mid.x));
-                    mid = node_mid(s.nodes, cur);
-                    const double2 hi = node_high(s.nodes, cur);
-                    ch = node_children(s.nodes, cur);
-                    if (x + r < hi.x && y + r < hi.y) lv = cur;
+    // Each thread takes two objects per step (positions base and base +
+    // stride, both wave-uniform blocks) and descends for both at once, so
+    // that the node reads of one overlap those of the other.
+    constexpr uint32_t kU = 2;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t base0 = blockIdx.x * blockDim.x; base0 < s.n; base0 += kU * stride) {
+        uint32_t kk[kU], id[kU], lv[kU], cur[kU], pleaf[kU];
+        bool active[kU], border[kU];
+        double x[kU], y[kU];
+        uint4 ch[kU];
+        double2 mid[kU];
+#pragma unroll
+        for (uint32_t q = 0; q < kU; q++) {
+            const uint32_t k = base0 + q * stride + threadIdx.x;
+            kk[q] = k;
+            active[q] = k < s.n;
+            id[q] = k;
+            if (active[q] && ordered) {
+                id[q] = s.spid[k];
+                if (id[q] >= s.n) id[q] = k;   // not a permutation: only after an earlier error
+            }
+            border[q] = false;
+            lv[q] = 0;
+            cur[q] = kNone;
+            pleaf[q] = kNone;
+            x[q] = 0.0;
+            y[q] = 0.0;
+            ch[q] = make_uint4(kLeafTag, 0u, 0u, 0u);
+            mid[q] = make_double2(0.0, 0.0);
+            if (active[q]) {
+                x[q] = xy[2 * (size_t)id[q]];
+                y[q] = xy[2 * (size_t)id[q] + 1];
+                pleaf[q] = ordered ? s.tp->sleaf_old[k] : kNone;   // position k held the object then
+                if (!(x[q] >= s.lx && x[q] <= s.hx && y[q] >= s.ly && y[q] <= s.hy)) {
+                    atomicOr(&s.ctr->error, kErrBadInput);
+                    active[q] = false;
+                } else {
+                    cur[q] = 0;
+                    mid[q] = node_mid(s.nodes, 0);
+                    ch[q] = node_children(s.nodes, 0);
                 }
-                border = s.complete ? !root_leaf : (lv != cur);
-                if (s.complete) lv = 0;
             }
         }
-        // ranks: the lanes that share a leaf take consecutive ranks with one atomic
-        uint32_t rank = 0;
-        bool need = active;
+        // both descents, one level of each per step
         for (;;) {
-            const unsigned long long pend = __ballot(need);
-            if (!pend) break;
-            const uint32_t leader = __ffsll((long long)pend) - 1;
-            const uint32_t L = __shfl(cur, leader);
-            const bool mine = need && cur == L;
-            const unsigned long long mm = __ballot(mine);
-            uint32_t b = 0;
-            if (lane == leader) b = atomicAdd(&s.count[L], (uint32_t)__popcll(mm));
-            b = __shfl(b, leader);
-            if (mine) {
-                rank = b + (uint32_t)__popcll(mm & ((1ull << lane) - 1ull));
-                need = false;
+            bool more = false;
+#pragma unroll
+            for (uint32_t q = 0; q < kU; q++) {
+                if (ch[q].x == kLeafTag) continue;
+                cur[q] = pick_child(ch[q], (y[q] > mid[q].y) * 2 + (x[q] > mid[q].x));
+                mid[q] = node_mid(s.nodes, cur[q]);
+                const double2 hi = node_high(s.nodes, cur[q]);
+                ch[q] = node_children(s.nodes, cur[q]);
+                if (x[q] + r < hi.x && y[q] + r < hi.y) lv[q] = cur[q];
+                more = true;
             }
+            if (!more) break;
         }
-        if (k < s.n) {
-            s.order[k] = i;
-            s.ord_xy[k] = make_double2(x, y);
-            s.ord_leaf[k] = active ? cur : kNone;
-            s.ord_rank[k] = rank;
-            s.ord_pleaf[k] = pleaf;
-        }
-        // border objects (with what the lookup needs) are staged per wave and
-        // appended one flush at a time
-        const unsigned long long m = __ballot(border);
-        if (border)
-            st[nst + __popcll(m & ((1ull << lane) - 1ull))] = BorderObj{i, lv, cur, pleaf, ordered ? k : kNone, rank, x, y};
-        nst += (uint32_t)__popcll(m);
-        if (nst > kBorderStage - 64) {
-            wave_sync();
-            uint32_t b0 = 0;
-            if (lane == 0) b0 = atomicAdd(&s.ctr->n_border, nst);
-            b0 = __shfl(b0, 0);
-            for (uint32_t q = lane; q < nst; q += 64) s.border[b0 + q] = st[q];
-            wave_sync();
-            nst = 0;
+#pragma unroll
+        for (uint32_t q = 0; q < kU; q++) {
+            if (active[q]) {
+                border[q] = s.complete ? !root_leaf : (lv[q] != cur[q]);
+                if (s.complete) lv[q] = 0;
+            }
+            // ranks: the lanes that share a leaf take consecutive ranks with one atomic
+            uint32_t rank = 0;
+            bool need = active[q];
+            for (;;) {
+                const unsigned long long pend = __ballot(need);
+                if (!pend) break;
+                const uint32_t leader = __ffsll((long long)pend) - 1;
+                const uint32_t L = __shfl(cur[q], leader);
+                const bool mine = need && cur[q] == L;
+                const unsigned long long mm = __ballot(mine);
+                uint32_t b = 0;
+                if (lane == leader) b = atomicAdd(&s.count[L], (uint32_t)__popcll(mm));
+                b = __shfl(b, leader);
+                if (mine) {
+                    rank = b + (uint32_t)__popcll(mm & ((1ull << lane) - 1ull));
+                    need = false;
+                }
+            }
+            const uint32_t k = kk[q];
+            if (k < s.n) {
+                s.order[k] = id[q];
+                s.ord_xy[k] = make_double2(x[q], y[q]);
+                s.ord_leaf[k] = active[q] ? cur[q] : kNone;
+                s.ord_rank[k] = rank;
+                s.ord_pleaf[k] = pleaf[q];
+            }
+            // border objects (with what the lookup needs) are staged per wave and
+            // appended one flush at a time
+            const unsigned long long m = __ballot(border[q]);
+            if (border[q])
+                st[nst + __popcll(m & ((1ull << lane) - 1ull))] =
+                    BorderObj{id[q], lv[q], cur[q], pleaf[q], ordered ? k : kNone, rank, x[q], y[q]};
+            nst += (uint32_t)__popcll(m);
+            if (nst > kBorderStage - 64) {
+                wave_sync();
+                uint32_t b0 = 0;
+                if (lane == 0) b0 = atomicAdd(&s.ctr->n_border, nst);
+                b0 = __shfl(b0, 0);
+                for (uint32_t t = lane; t < nst; t += 64) s.border[b0 + t] = st[t];
+                wave_sync();
+                nst = 0;
+            }
         }
     }
     if (nst) {
