@@ -2091,9 +2091,13 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_scatter(DevState s)
         unsigned long long keys[kBucketTile / kBucketThreads];
         uint32_t rk[kBucketTile / kBucketThreads];
 #pragma unroll
-        for (int q = 0; q < kBucketTile / kBucketThreads; q++) {
-            uint32_t i = threadIdx.x + q * kBucketThreads;
+        for (int q = 0; q < kBucketTile / kBucketThreads; q++) {   // every load before the first atomic
+            const uint32_t i = threadIdx.x + q * kBucketThreads;
             keys[q] = i < tn ? in[t0 + i] : kEmptyKey;
+        }
+#pragma unroll
+        for (int q = 0; q < kBucketTile / kBucketThreads; q++) {
+            const uint32_t i = threadIdx.x + q * kBucketThreads;
             rk[q] = i < tn ? atomicAdd(&hist[coarse_bin(s, keys[q])], 1u) : 0u;
         }
         __syncthreads();
@@ -2204,9 +2208,13 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_fine(DevState s)
             unsigned long long keys[kFineTile / kBucketThreads];
             uint32_t fb[kFineTile / kBucketThreads], rk[kFineTile / kBucketThreads];
 #pragma unroll
-            for (int q = 0; q < kFineTile / kBucketThreads; q++) {
+            for (int q = 0; q < kFineTile / kBucketThreads; q++) {   // every load before the first atomic
                 const uint32_t i = threadIdx.x + q * kBucketThreads;
                 keys[q] = i < tn ? s.clist2[t0 + i] : 0ull;
+            }
+#pragma unroll
+            for (int q = 0; q < kFineTile / kBucketThreads; q++) {
+                const uint32_t i = threadIdx.x + q * kBucketThreads;
                 fb[q] = region_of(s, keys[q] & kKeyBits) & (nf - 1);
                 rk[q] = i < tn ? atomicAdd(&thist[fb[q]], 1u) : 0u;   // its rank in the region
             }
