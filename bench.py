@@ -89,19 +89,27 @@ def algorithmic_bytes(kernel: str, n: int, per_tick: dict, precision: int) -> fl
     return 0.0
 
 
-def profiled_traffic(kernel: str, workload: str, precision: int, objects: int) -> float | None:
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 summary
-    (FETCH_SIZE + WRITE_SIZE runs scaled by a plain copy's counters), when a
-    summary exists for this kernel, workload and precision."""
-    tag = f"{kernel}_{workload}_{objects // 1_000_000}m_{'fp32guard' if precision else 'fp64'}_traffic.txt"
+def profiled_traffic(kernel: str, workload: str, precision: int, objects: int) -> tuple[float | None, str]:
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary of this workload (FETCH_SIZE and WRITE_SIZE runs scaled by a
+    plain copy's counters), and where the figure comes from.  A summary made
+    from other kernel sources than the ones built now is not used."""
+    from glint_pipeline_amd.build import source_digest
+    tag = f"traffic_{workload}_{objects // 1_000_000}m_{'fp32guard' if precision else 'fp64'}.txt"
     f = ROOT / "profiles" / tag
     if not f.exists():
-        return None
+        return None, f"no counter summary profiles/{tag}"
+    found, sha = None, None
     for line in f.read_text().splitlines():
         parts = line.split()
         if len(parts) == 3 and parts[0] == "TRAFFIC_BYTES_PER_LAUNCH" and parts[1] == kernel:
-            return float(parts[2])
-    return None
+            found = float(parts[2])
+        if len(parts) == 2 and parts[0] == "SOURCE_SHA":
+            sha = parts[1]
+    cur = source_digest()
+    if sha != cur:
+        return None, f"profiles/{tag} measured kernel sources {sha}, built sources are {cur}"
+    return found, f"profiles/{tag} (FETCH_SIZE + WRITE_SIZE, copy-calibrated; sources {cur})"
 
 
 def cpu_baseline(w, ticks: int, threads: int, min_seconds: float = 10.0, max_ticks: int = 5000,
@@ -140,9 +148,8 @@ WORKLOAD_SPEC = {
 
 
 def distinct_estimate(workload: str, n: int) -> float:
-    # distinct pairs per tick grow with the square of the hotspot density:
-    # ~65 per object at 10M skewed (measured), ~0.06 per object for uniform walkers
-    return {"skewed": 70.0 * n / 1e7, "plain_gpu": 0.1, "plain_cpu": 0.1, "batched": 0.1}[workload] * n
+    from glint_pipeline_amd import autotune
+    return autotune.pairs_estimate(workload, n)
 
 
 def main() -> int:
@@ -171,11 +178,11 @@ def main() -> int:
                                                              "precision on this workload before timing")
     ap.add_argument("--save-tuned", action="store_true", help="store the autotuned parameters in tuned.json")
     ap.add_argument("--no-profile", action="store_true",
-                    help="skip the second run that times every kernel with HIP events over the timed ticks")
-    ap.add_argument("--cpu-ticks", type=int, default=1)
+                    help="run the second (verification) run without HIP events around every kernel")
+    ap.add_argument("--cpu-ticks", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--compare-forms", action="store_true",
-                    help="also time the plain per-tick FP64 form (host positions, one call sequence per tick) "
+    ap.add_argument("--no-forms", action="store_true",
+                    help="skip timing the plain per-tick FP64 form (host positions, one call sequence per tick) "
                          "against the batched guard-band form on the same inputs")
     ap.add_argument("--kv", action="store_true",
                     help="also time the ticks with the sorted key-value output on (pair boxes, meeting log) "
@@ -264,56 +271,99 @@ def main() -> int:
     totals = wb.totals()
     stats = wb.stats()
     log(f"[bench] timed {K} ticks in {elapsed:.3f} s; totals {totals}")
+    wb.close()
 
-    # ---- per-kernel times over the same ticks: a second run of the same
-    # inputs without graph capture (events cannot sit inside a replayed graph),
-    # HIP events around every launch on the context's stream during ticks W..W+K
+    # ---- verification and per-kernel times: a second run of the same inputs
+    # without graph capture (events cannot sit inside a replayed graph), HIP
+    # events around every launch on the context's stream during ticks
+    # W..W+K-1, the meeting log on.  Its totals over the timed ticks must equal
+    # the graph run's, and where the committed oracle fixture covers this
+    # workload, every fixture tick's meetings must equal the oracle's.
+    fixture = headline_fixture(args.workload, w, params)
+    fx_ticks = min(len(fixture["ticks"]), W + K) if fixture else 0
+    wv = runtime.Workbench(make_cfg(precision, 0, meeting_log_capacity=(64 << 20) if fixture else 0))
+    verify = {"graph_vs_plain": None, "fixture_ticks": 0}
+    mismatches = []
+    done = 0
+
+    def advance(upto: int) -> None:
+        nonlocal done
+        if upto <= done:
+            return
+        wv.run_ticks(trace.address(16 * N * done), done, upto - done, latencies=False)
+        done = upto
+
+    def start_timed() -> None:
+        wv.synchronize()
+        wv.totals(reset=True)
+        if not args.no_profile:
+            wv.set_kernel_profiling(True)
+            wv.kernel_times(reset=True)
+
     ktimes = {}
+    if W == 0:
+        start_timed()
+    for cut in sorted({W, fx_ticks, W + K} - {0}):
+        if done < W <= cut:
+            advance(W)
+            start_timed()
+        advance(cut)
+        if fixture and cut == fx_ticks:
+            wv.synchronize()
+            mismatches += check_fixture(wv, fixture, fx_ticks)
+            verify["fixture_ticks"] = fx_ticks
+    wv.synchronize()
     if not args.no_profile:
-        wb.close()
-        wb = runtime.Workbench(make_cfg(precision, 0))
-        wb.run_ticks(trace.address(0), 0, W, latencies=False)
-        wb.synchronize()
-        wb.set_kernel_profiling(True)
-        wb.kernel_times(reset=True)
-        wb.run_ticks(trace.address(16 * N * W), W, K, latencies=False)
-        wb.synchronize()
-        ktimes = wb.kernel_times()
-        wb.set_kernel_profiling(False)
+        ktimes = wv.kernel_times()
+        wv.set_kernel_profiling(False)
         ptick = sum(ms for ms, cnt in ktimes.values()) / K
         for name, (ms, cnt) in sorted(ktimes.items(), key=lambda kv: -kv[1][0]):
             if cnt:
                 log(f"[bench]   {name:16s} {ms / K:9.3f} ms/tick  {cnt // K:4d} launches/tick  {ms / cnt:9.3f} ms/launch")
         log(f"[bench]   kernels sum {ptick:.3f} ms/tick (events on every launch, no graph, ticks {W}..{W + K - 1})")
+    vt = wv.totals()
+    same = [k for k in ("ticks", "contacts", "distinct_pairs", "meetings", "stored", "units", "border_objects",
+                        "rechecks", "ref_drop_objects") if vt[k] != totals[k]]
+    vs = wv.stats()
+    if vs["num_active"] != stats["num_active"] or vs["num_leaves"] != stats["num_leaves"]:
+        same.append("last tick's active pairs or leaves")
+    verify["graph_vs_plain"] = not same
+    if same:
+        mismatches.append(f"graph run and plain run differ in {same}")
+    wv.close()
+    log(f"[bench] verification {verify}; {len(mismatches)} mismatches")
 
     value = N * K / elapsed
     per_tick = {k: totals[k] / max(1, totals["ticks"]) for k in
-                ("contacts", "contacts_border", "meetings", "units", "border_objects", "distinct_pairs", "stored")}
+                ("contacts", "contacts_border", "meetings", "units", "border_objects", "distinct_pairs", "stored",
+                 "ref_drop_leaves", "ref_drop_objects")}
     roofline = None
     if ktimes:
         dom, (dms, dcnt) = max(((k, v) for k, v in ktimes.items() if v[1]), key=lambda kv: kv[1][0])
         abytes = algorithmic_bytes(dom, N, per_tick, precision)
         avg_s = dms / dcnt / 1e3
         achieved = abytes / avg_s / 1e9 if abytes and avg_s > 0 else 0.0
-        traffic = profiled_traffic(dom, args.workload, precision, N)
+        traffic, tnote = profiled_traffic(dom, args.workload, precision, N)
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom,
-                    "kernel_ms_per_launch": round(dms / dcnt, 4), "algorithmic_bytes_per_launch": abytes,
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": tnote,
+                    "kernel": dom, "kernel_ms_per_launch": round(dms / dcnt, 4),
+                    "algorithmic_bytes_per_launch": abytes,
                     "kernel_share_of_tick": round(dms / K / (elapsed * 1e3 / K), 4),
                     "measured": f"HIP events on every launch over ticks {W}..{W + K - 1}, ungraphed run of the same "
                                 f"inputs"}
 
+    # ---- the plain FP64 per-tick form on the same inputs: positions copied
+    # from host memory every tick, the four stage calls, a host sync per tick
     forms = None
-    if args.compare_forms:
-        wb.close()
+    if not args.no_forms:
         forms = {}
-        # plain form: positions copied from the host every tick, one call sequence per tick, FP64
         host = np.zeros(2 * N, dtype=np.float64)
         wp = runtime.Workbench(make_cfg(runtime.PRECISION_FP64, 0))
         for t in range(W):
             trace.download(host, 16 * N * t)
             wp.tick(host, t)
         wp.synchronize()
+        wp.totals(reset=True)
         lat = []
         for t in range(W, W + K):
             trace.download(host, 16 * N * t)       # the caller's positions, in host memory (not timed)
@@ -325,26 +375,31 @@ def main() -> int:
             wp.update_schema()
             wp.synchronize()
             lat.append((time.perf_counter() - t1) * 1e3)
+        pt = wp.totals()
+        wp.close()
+        if pt["meetings"] != totals["meetings"] or pt["distinct_pairs"] != totals["distinct_pairs"]:
+            mismatches.append("plain FP64 form and guard-band form differ")
         forms["plain_fp64_per_tick_ms"] = round(float(np.sum(lat)) / K, 3)
         forms["plain_p99_ms"] = round(float(np.percentile(lat, 99)), 3)
-        wp.close()
-        # batched form: device-resident trace, 8 ticks per graph, FP32 guard band
-        wq = runtime.Workbench(make_cfg(runtime.PRECISION_FP32_GUARD, 8))
-        wq.run_ticks(trace.address(0), 0, W, latencies=False)
-        wq.synchronize()
-        t0 = time.perf_counter()
-        ms = wq.run_ticks(trace.address(16 * N * W), W, K, latencies=True)
-        wq.synchronize()
-        forms["batched_guard_graph8_ms"] = round((time.perf_counter() - t0) * 1e3 / K, 3)
-        forms["batched_p99_ms"] = round(float(np.percentile(ms, 99)), 3)
+        if precision == runtime.PRECISION_FP32_GUARD and graph_ticks == 8:
+            forms["batched_guard_graph8_ms"] = round(elapsed * 1e3 / K, 3)   # the timed run is that form
+            forms["batched_p99_ms"] = round(float(np.percentile(tick_ms, 99)), 3)
+        else:
+            wq = runtime.Workbench(make_cfg(runtime.PRECISION_FP32_GUARD, 8))
+            wq.run_ticks(trace.address(0), 0, W, latencies=False)
+            wq.synchronize()
+            t0 = time.perf_counter()
+            ms = wq.run_ticks(trace.address(16 * N * W), W, K, latencies=True)
+            wq.synchronize()
+            forms["batched_guard_graph8_ms"] = round((time.perf_counter() - t0) * 1e3 / K, 3)
+            forms["batched_p99_ms"] = round(float(np.percentile(ms, 99)), 3)
+            wq.close()
         forms["speedup"] = round(forms["plain_fp64_per_tick_ms"] / forms["batched_guard_graph8_ms"], 3)
-        wq.close()
+        forms["same_meetings"] = pt["meetings"] == totals["meetings"]
         log(f"[bench] forms {forms}")
 
     kv = None
     if args.kv:
-        if not args.compare_forms:
-            wb.close()
         restriction = 2 * 33554432                 # kv_restriction at G_bytes = 2 (config.h:53,80)
         wk = runtime.Workbench(make_cfg(precision, graph_ticks, kv_capacity=2 * restriction,
                                         kv_area_limit=1e30))   # keep every object in the files
@@ -378,11 +433,21 @@ def main() -> int:
         cb = cpu_baseline(w, args.cpu_ticks, threads, schema=params.get("schema", "dynamic"))
         log(f"[bench] cpu baseline {cb} (wall {time.perf_counter() - t0:.1f} s)")
         cpu = {"value": round(N * cb["ticks"] / cb["seconds"], 1), "unit": "object-updates/s", "cores": threads,
-               "kind": "port",
+               "kind": "port", "cpu": cpu_model(),
                "sample": f"oracle (C, OpenMP) ticks 1..{cb['ticks']} of the same workload after its tick-0 tree "
                          f"build; {cb['seconds']:.2f} s timed"}
 
     lat = np.asarray(tick_ms, dtype=np.float64)
+    worst = int(np.argmax(lat))
+    p99_note = {"tick": W + worst, "ms": round(float(lat[worst]), 3), "index_in_run": worst,
+                "index_in_graph": worst % graph_ticks if graph_ticks else None,
+                "ticks_over_1.25x_p50": int(np.count_nonzero(lat > 1.25 * np.percentile(lat, 50)))}
+    if mismatches:
+        print(json.dumps({"metric": METRIC, "value": None, "unit": "object-updates/s", "n_gpus": 1,
+                          "error": "results not verified: " + "; ".join(mismatches), "verification": verify}),
+              flush=True)
+        trace.free()
+        return 3
     out = {
         "metric": METRIC,
         "value": round(value, 1),
@@ -399,8 +464,14 @@ def main() -> int:
         "dtype": "f64" if precision == runtime.PRECISION_FP64 else "f32+f64",
         "data": "synthetic (seeded generator, positions resident in HBM before timing)",
         "config": {"workload": desc, "objects": N, "first_tick": W, "ticks": K, "graph_ticks": graph_ticks,
-                   "params": params, "table_slots": slots, "contacts_per_tick": round(per_tick["contacts"]),
-                   "stored_per_tick": round(per_tick["stored"]), "meetings_per_tick": round(per_tick["meetings"], 1), "leaves": stats["num_leaves"]},
+                   "params": params, "table_slots_last_tick": stats["table_slots"],
+                   "contacts_per_tick": round(per_tick["contacts"]),
+                   "stored_per_tick": round(per_tick["stored"]), "meetings_per_tick": round(per_tick["meetings"], 1),
+                   "leaves": stats["num_leaves"], "max_leaf_count": totals["max_leaf_count"],
+                   "ref_drop_leaves_per_tick": round(per_tick["ref_drop_leaves"], 1),
+                   "ref_drop_objects_per_tick": round(per_tick["ref_drop_objects"], 1)},
+        "worst_tick": p99_note,
+        "verification": verify,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
@@ -411,10 +482,50 @@ def main() -> int:
     if kv:
         out["kv_output"] = kv
     print(json.dumps(out), flush=True)
-    if not args.compare_forms and not args.kv:
-        wb.close()
     trace.free()
     return 0
+
+
+def cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+FIXTURE = ROOT / "tests" / "golden" / "skewed_10m_fixture.json"
+
+
+def headline_fixture(workload: str, w, params: dict) -> dict | None:
+    """The committed oracle fixture of this workload, if it is the one the
+    fixture was written for (10M skewed objects at its leaf capacity)."""
+    if workload != "skewed" or not FIXTURE.exists():
+        return None
+    fx = json.loads(FIXTURE.read_text())
+    c = fx["case"]
+    if (w.num_objects, w.grid_capacity, w.reach_distance, w.min_meet_time) != \
+            (c["objects"], c["grid_capacity"], c["reach"], c["min_meet"]) or params.get("schema", "dynamic") != "dynamic":
+        return None
+    return fx
+
+
+def check_fixture(wb, fx: dict, ticks: int) -> list[str]:
+    """Every meeting of ticks 0..ticks-1 from the meeting log against the
+    oracle's per-tick digests; returns the mismatches."""
+    import hashlib
+    bad = []
+    log_rows = wb.meeting_log(start=0, sort=False)
+    end = log_rows[:, 3] if len(log_rows) else np.zeros(0, dtype=np.uint32)
+    for t in fx["ticks"][:ticks]:
+        m = log_rows[end + 1 == t["tick"]]
+        m = m[np.lexsort((m[:, 2], m[:, 1], m[:, 0]))] if len(m) else m
+        if len(m) != t["meetings"] or \
+                hashlib.sha256(np.ascontiguousarray(m, dtype=np.uint32).tobytes()).hexdigest() != t["meetings_sha256"]:
+            bad.append(f"tick {t['tick']}: {len(m)} meetings, the oracle's fixture {t['meetings']}")
+    return bad
 
 
 def bench_serving(args) -> int:

@@ -27,6 +27,18 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("GLINT_OFFLOAD_ARCH", "gfx950")
 
 
+def source_digest() -> str:
+    """First 16 hex digits of the SHA-256 of the kernel sources and headers
+    libglint.so is built from: stamps measurements (rocprof counter
+    summaries) with the code they measured."""
+    import hashlib
+    h = hashlib.sha256()
+    for f in [CSRC / s for s in HIP_SOURCES + HIP_HEADERS] + sorted(INCLUDE.glob("*.h")):
+        h.update(f.name.encode())
+        h.update(f.read_bytes())
+    return h.hexdigest()[:16]
+
+
 def _run(cmd: list[str]) -> None:
     print("+", " ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

@@ -43,12 +43,13 @@ constexpr int kFlushRound = 4;                      // join columns between stag
 constexpr int kMeetStage = 128;                     // staged meetings per wave (table update)
 constexpr uint32_t kChunk = 4096;                   // contact-list entries reserved per atomic
 constexpr int kRegion = 2048;                       // meeting-table slots per region (LDS-resident)
+constexpr uint32_t kRegionFill = 1024;              // stored contacts per region the table shape aims at
+constexpr uint32_t kMaxRegionBits = 21;             // at most 2^21 regions (coarse 2^10 x fine 2^11)
 constexpr int kBucketThreads = 1024;                // workgroup of the two bucketing passes
 constexpr int kBucketTile = 16384;                  // contacts per LDS tile of the coarse pass (128 KB)
 constexpr int kFineTile = 12288;                    // contacts per LDS tile of the fine pass (96 KB)
 constexpr uint32_t kMaxCoarse = 1024;               // coarse bins of pass 1
 constexpr uint32_t kMaxFine = 2048;                 // fine bins of pass 2
-constexpr uint32_t kInsideChunk = 1;                // leaves per work-queue grab (1 measured fastest of 1, 2, 4, 8)
 constexpr int kBorderStage = 128;                   // staged border objects per wave (assign)
 constexpr uint32_t kHeavyLeaf = 256;                // leaves above this size are joined first
 
@@ -235,6 +236,10 @@ __global__ void k_reset_tick(DevState s)
     c->n_heavy = 0;
     c->n_leaves = 0;
     c->max_leaf = 0;
+    c->n_drop_leaves = 0;
+    c->n_drop_objs = 0;
+    c->n_tests = 0;
+    c->n_evals = 0;
     c->work_inside = 0;
     c->g_heavy = 0;
 }
@@ -483,6 +488,12 @@ __global__ void k_schema_done(DevState s)
     c->acc_stored += c->n_stored;
     if (c->n_stored > c->max_stored) c->max_stored = c->n_stored;
     if (c->n_meet > c->max_meet) c->max_meet = c->n_meet;
+    c->acc_drop_leaves += c->n_drop_leaves;
+    c->acc_drop_objs += c->n_drop_objs;
+    c->acc_tests += c->n_tests;
+    c->acc_evals += c->n_evals;
+    if (c->n_drop_objs > c->max_drop_objs) c->max_drop_objs = c->n_drop_objs;
+    if (c->max_leaf > c->max_leaf_run) c->max_leaf_run = c->max_leaf;
     if (s.mlog) c->n_mlog += c->n_meet < s.meet_cap ? c->n_meet : s.meet_cap;
 }
 
@@ -1008,7 +1019,11 @@ __global__ __launch_bounds__(kBlock) void k_scan_objects(DevState s)
     }
     uint32_t total;
     uint32_t ex = block_exclusive_scan(v, &total) + s.block_sums[blockIdx.x];
-    uint32_t mx = 0, nleaf = 0;
+    // leaves beyond the reference's leaf storage of 2*cap objects
+    // (grid_amplify, workbench.cpp:36), where it would drop objects
+    // (filter.cpp:100-104): only leaves hold objects
+    const uint32_t store = 2 * s.cap;
+    uint32_t mx = 0, nleaf = 0, over = 0, drop = 0;
     for (int k = 0; k < kScanItems; k++) {
         uint32_t n = base + k;
         if (n < s.node_cap) {
@@ -1017,14 +1032,22 @@ __global__ __launch_bounds__(kBlock) void k_scan_objects(DevState s)
         }
         ex += c[k];
         mx = max(mx, c[k]);
+        over += c[k] > store ? 1u : 0u;
+        drop += c[k] > store ? c[k] - store : 0u;
     }
     for (int d = 32; d >= 1; d >>= 1) {
         mx = max(mx, (uint32_t)__shfl_xor(mx, d));
         nleaf += (uint32_t)__shfl_xor(nleaf, d);
+        over += (uint32_t)__shfl_xor(over, d);
+        drop += (uint32_t)__shfl_xor(drop, d);
     }
     if (lane_id() == 0) {
         atomicMax(&s.ctr->max_leaf, mx);
         atomicAdd(&s.ctr->n_leaves, nleaf);
+        if (over) {
+            atomicAdd(&s.ctr->n_drop_leaves, over);
+            atomicAdd(&s.ctr->n_drop_objs, (unsigned long long)drop);
+        }
     }
 }
 
@@ -1309,13 +1332,37 @@ __global__ __launch_bounds__(kBlock) void k_unit_scatter(DevState s)
 // buffer (the one pass writes the grouped list into the generation).
 __device__ __forceinline__ unsigned long long *contact_out(const DevState &s)
 {
-    return s.lo_bits ? s.tp->tab_cur.keys : s.clist2;
+    return s.tp->tab_cur.keys;
 }
 
-__device__ __forceinline__ uint32_t region_of(const DevState &s, unsigned long long key)
+// Region of a pair in a table of 2^bits regions: the high bits of its hash
+// (the low bits pick its slot in the region's LDS hash).
+__device__ __forceinline__ uint32_t region_of(unsigned long long key, uint32_t bits)
 {
     // a one-region table has no region bits (a shift by 64 is undefined)
-    return s.region_bits ? (uint32_t)(mix64(key) >> (64 - s.region_bits)) : 0u;
+    return bits ? (uint32_t)(mix64(key) >> (64 - bits)) : 0u;
+}
+
+// The table's shape at this tick, from the contacts the join stored: the
+// fewest regions (at least 2^rbits_min) that keep kRegionFill contacts per
+// region on average, half a region's LDS slots, so that a tick denser than
+// the configuration expected still fits.  Every kernel after the join
+// computes the same shape from the same count.  Regions split into
+// 2^(bits - lo_bits) <= kMaxCoarse coarse bins of 2^lo_bits each.
+struct TableShape {
+    uint32_t bits, lo_bits, coarse;
+};
+
+__device__ __forceinline__ TableShape table_shape(const DevState &s)
+{
+    const unsigned long long n = s.ctr->n_stored;
+    uint32_t b = s.rbits_min;
+    while (b < s.rbits_max && ((unsigned long long)kRegionFill << b) < n) b++;
+    TableShape t;
+    t.bits = b;
+    t.lo_bits = b > 10 ? b - 10 : 0;
+    t.coarse = 1u << (b - t.lo_bits);
+    return t;
 }
 
 // Contacts found by one wave are staged in LDS as (row lane, column) index
@@ -1436,6 +1483,40 @@ struct TileF {
     float2 f, pf;
 };
 
+// Bounding box of a group of tile objects in the tile's frame, in FP32:
+// exact for FP32 offsets, rounded outward for FP64 positions (so it holds
+// them); an empty group's box is (+inf, +inf, -inf, -inf).
+struct GBox {
+    float lx, ly, hx, hy;
+};
+
+// The float next to finite f towards -inf (+inf): one step of its bit
+// pattern (zero steps to the smallest subnormal of the other sign).
+__device__ __forceinline__ float f32_prev(float f)
+{
+    const uint32_t b = __builtin_bit_cast(uint32_t, f);
+    return f > 0.f ? __builtin_bit_cast(float, b - 1u)
+                   : (f < 0.f ? __builtin_bit_cast(float, b + 1u) : __builtin_bit_cast(float, 0x80000001u));
+}
+
+__device__ __forceinline__ float f32_next(float f) { return -f32_prev(-f); }
+
+// The largest float <= v and the smallest float >= v.
+__device__ __forceinline__ float f32_down(double v)
+{
+    const float f = (float)v;
+    return (double)f > v ? f32_prev(f) : f;
+}
+
+__device__ __forceinline__ float f32_up(double v)
+{
+    const float f = (float)v;
+    return (double)f < v ? f32_next(f) : f;
+}
+
+constexpr uint32_t kGroup = 16;      // tile columns per box
+constexpr uint32_t kSubCells = 64;   // 8 x 8 sub-cells of a leaf, in Morton order
+
 template <int TILE>
 struct JoinTile {
     union {
@@ -1446,6 +1527,7 @@ struct JoinTile {
     uint32_t pl[TILE];             // leaf at prev_tick
     uint32_t p[TILE];              // object ids
     uint4 pc[TILE];                // candidate leaves at prev_tick (DevState::prev_cand)
+    GBox box[TILE / kGroup];       // box of every kGroup consecutive slots
 };
 
 // The row objects of a wave's current task (read when its contacts are
@@ -1474,12 +1556,101 @@ __device__ __forceinline__ void tile_put(JoinTile<TILE> &t, uint32_t k, const De
     if (want_pc) t.pc[k] = cand_at(s.tp->cand_prev, s.sprev[from], s.n, s.tp->prev_tick);
 }
 
+__device__ __forceinline__ uint32_t spread3(uint32_t v) { return (v & 1u) | ((v & 2u) << 1) | ((v & 4u) << 2); }
+
+// Sorted objects from .. from+n-1 of one leaf into tile slots 0..n-1, in the
+// Morton order of the leaf's 8 x 8 sub-cells (a counting sort in LDS), and
+// the box of every kGroup consecutive slots: a wave task then skips the
+// column groups whose box is out of reach of its rows' box.  With cand, the
+// candidate lists written at this tick go to ccand/cmask in the same order.
+// Every thread of the block calls it; it ends with the tile complete.
 template <int TILE>
-__device__ __forceinline__ void load_tile(JoinTile<TILE> &t, const DevState &s, uint32_t from, uint32_t n, bool has_prev,
-                                          bool use32)
+__device__ __forceinline__ void load_tile_sorted(JoinTile<TILE> &t, const DevState &s, uint32_t from, uint32_t n,
+                                                 bool has_prev, bool use32, double lx, double ly, double w,
+                                                 bool cand, uint4 *ccand, uint32_t *cmask, uint32_t *hist)
 {
+    constexpr int kPer = (TILE + kBlock - 1) / kBlock;
     const bool want_pc = has_prev && !s.store_all;
-    for (uint32_t k = threadIdx.x; k < n; k += kBlock) tile_put(t, k, s, from + k, has_prev, use32, want_pc);
+    const double *sx = s.tp->sx, *sy = s.tp->sy;
+    const double scale = w > 0.0 ? 8.0 / w : 0.0;
+    if (threadIdx.x < kSubCells) hist[threadIdx.x] = 0;
+    __syncthreads();
+    uint32_t cell[kPer], rank[kPer];
+#pragma unroll
+    for (int q = 0; q < kPer; q++) {
+        const uint32_t k = threadIdx.x + q * kBlock;
+        cell[q] = 0;
+        if (k < n) {
+            // the cell only orders the tile; any value is correct
+            const double ox = use32 ? (double)s.fx[from + k] : sx[from + k] - lx;
+            const double oy = use32 ? (double)s.fy[from + k] : sy[from + k] - ly;
+            const uint32_t cx = (uint32_t)fmin(fmax(ox * scale, 0.0), 7.0);
+            const uint32_t cy = (uint32_t)fmin(fmax(oy * scale, 0.0), 7.0);
+            cell[q] = spread3(cx) | (spread3(cy) << 1);
+            rank[q] = atomicAdd(&hist[cell[q]], 1u);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) {   // exclusive offsets of the 64 cells, in place
+        const uint32_t v = hist[threadIdx.x];
+        hist[threadIdx.x] = wave_inclusive_scan(v) - v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kPer; q++) {
+        const uint32_t k = threadIdx.x + q * kBlock;
+        if (k < n) {
+            const uint32_t d = hist[cell[q]] + rank[q];
+            tile_put(t, d, s, from + k, has_prev, use32, want_pc);
+            if (cand) cmask[d] = cand_mask(ccand[d] = cand_read(s.tp->cand, from + k, s.tp->tick));
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kPer; q++) {
+        const uint32_t k = threadIdx.x + q * kBlock;
+        float x0 = __builtin_inff(), y0 = __builtin_inff(), x1 = -__builtin_inff(), y1 = -__builtin_inff();
+        if (k < n) {
+            if (use32) {
+                x0 = x1 = t.fp[k].f.x;
+                y0 = y1 = t.fp[k].f.y;
+            } else {
+                x0 = f32_down(t.xy[k].x);
+                x1 = f32_up(t.xy[k].x);
+                y0 = f32_down(t.xy[k].y);
+                y1 = f32_up(t.xy[k].y);
+            }
+        }
+        for (int d = kGroup / 2; d >= 1; d >>= 1) {
+            x0 = fminf(x0, __shfl_xor(x0, d));
+            y0 = fminf(y0, __shfl_xor(y0, d));
+            x1 = fmaxf(x1, __shfl_xor(x1, d));
+            y1 = fmaxf(y1, __shfl_xor(y1, d));
+        }
+        if (k < (uint32_t)TILE && k % kGroup == 0) t.box[k / kGroup] = GBox{x0, y0, x1, y1};
+    }
+    __syncthreads();
+}
+
+// True when no pair of a row in box a and a column in box b can pass the
+// distance test of the column loop: the squared gap between the boxes,
+// computed with the loop's own operations (FP32 fused, or FP64), is past
+// the threshold.  Rounding is monotone, so every pair's computed squared
+// distance is at least the gap's.
+// For FP64 positions the boxes hold them (rounded outward), and the gap
+// computed in FP64 from their corners is at most the pair's own difference.
+template <bool USE32>
+__device__ __forceinline__ bool box_out_of_reach(const GBox &a, const GBox &b, float t32_out, double t64)
+{
+    if constexpr (USE32) {
+        const float gx = fmaxf(fmaxf(b.lx - a.hx, a.lx - b.hx), 0.f);
+        const float gy = fmaxf(fmaxf(b.ly - a.hy, a.ly - b.hy), 0.f);
+        return dist2_f32(gx, gy) > t32_out;
+    } else {
+        const double gx = fmax(fmax((double)b.lx - (double)a.hx, (double)a.lx - (double)b.hx), 0.0);
+        const double gy = fmax(fmax((double)b.ly - (double)a.hy, (double)a.ly - (double)b.hy), 0.0);
+        return gx * gx + gy * gy > t64;
+    }
 }
 
 // Stage entries: row lane << 16 | column.  In guard mode the write-out
@@ -1504,6 +1675,7 @@ struct JoinCounts {
     uint32_t pass;   // band entries within reach in FP64 (per lane)
     uint32_t band;   // band entries re-tested (per lane)
     uint32_t dup;    // pairs found twice and not stored, counted once (per lane)
+    uint32_t cols;   // columns the wave evaluated (uniform; the rest were skipped by box)
 };
 
 // Register x of lane `src` (every lane of the wave must take part).
@@ -1530,9 +1702,9 @@ __device__ __forceinline__ JoinCounts join_columns(Stager &st, const DevState &s
                                                    uint32_t jbeg, uint32_t jend, float t32_in,
                                                    float t32_out, double t64, const JoinRows &rows,
                                                    bool has_prev, const uint4 *ccand, const uint32_t *cmask,
-                                                   uint32_t leaf, uint4 rpc, uint32_t rpl)
+                                                   uint32_t leaf, uint4 rpc, uint32_t rpl, const GBox &rbox)
 {
-    uint32_t n_sure = 0, n_pass = 0, band = 0, n_dup = 0;
+    uint32_t n_sure = 0, n_pass = 0, band = 0, n_dup = 0, cols = 0;
     // the task's columns are the same for every lane: scalar loop bounds
     jbeg = __builtin_amdgcn_readfirstlane(jbeg);
     jend = __builtin_amdgcn_readfirstlane(jend);
@@ -1624,8 +1796,8 @@ __device__ __forceinline__ JoinCounts join_columns(Stager &st, const DevState &s
     const uint32_t dmax = ROT ? (nblk - 1) / 2 + (nblk % 2 == 0 ? 1u : 0u) : 0u;
     const uint32_t lim = ROT ? (lane < nblk ? (nblk - 1) / 2 + ((nblk % 2 == 0 && lane < nblk / 2) ? 1u : 0u) : 0u)
                              : 0u;
-    const uint32_t lstart = ROT ? 1u : jbeg, lend = ROT ? dmax + 1 : jend;
-    for (uint32_t jb = lstart; jb < lend; jb += kFlushRound) {
+    // one round: columns jb .. jb+kFlushRound-1 (below jlim; ROT: rotation steps)
+    auto round = [&](uint32_t jb, uint32_t jlim) {
         TileF cfp[kFlushRound];
         double2 cx[kFlushRound];
         uint32_t cm[kFlushRound];
@@ -1640,7 +1812,7 @@ __device__ __forceinline__ JoinCounts join_columns(Stager &st, const DevState &s
                 jc = jbeg + jr;
                 cj[jj] = jc;
             } else {
-                jc = min(jb + jj, jend - 1);
+                jc = min(jb + jj, jlim - 1);
             }
             if constexpr (USE32)
                 cfp[jj] = C.fp[jc];
@@ -1653,7 +1825,7 @@ __device__ __forceinline__ JoinCounts join_columns(Stager &st, const DevState &s
 #pragma unroll
         for (uint32_t jj = 0; jj < kFlushRound; jj++) {
             const uint32_t j = ROT ? cj[jj] : jb + jj;
-            const bool cand = ROT ? (jb + jj <= lim) : (j < jend);   // uniform unless ROT
+            const bool cand = ROT ? (jb + jj <= lim) : (j < jlim);   // uniform unless ROT
             bool sure, stage;
             const uint32_t e = tag | j;
             if constexpr (USE32) {
@@ -1685,22 +1857,35 @@ __device__ __forceinline__ JoinCounts join_columns(Stager &st, const DevState &s
 #pragma unroll
         for (uint32_t jj = 0; jj < kFlushRound; jj++) stage_push(st, stg[jj], ent[jj]);
         if (st.n > kStage - 64 * kFlushRound) stage_flush(st, s, key_of);
+    };
+    if constexpr (ROT) {
+        for (uint32_t jb = 1; jb <= dmax; jb += kFlushRound) round(jb, 0u);
+        cols = dmax;
+    } else {
+        // column groups of kGroup (jbeg is a multiple of kGroup); a group whose
+        // box is out of reach of the rows' box holds no contact
+        for (uint32_t g0 = jbeg; g0 < jend; g0 += kGroup) {
+            const uint32_t g1 = min(g0 + kGroup, jend);
+            // every lane holds the same boxes: the decision is taken as a scalar
+            if (__builtin_amdgcn_readfirstlane(
+                    (uint32_t)box_out_of_reach<USE32>(rbox, C.box[g0 / kGroup], t32_out, t64)))
+                continue;
+            cols += g1 - g0;
+            for (uint32_t jb = g0; jb < g1; jb += kFlushRound) round(jb, g1);
+        }
     }
     stage_flush(st, s, key_of);
-    return JoinCounts{n_sure, n_pass, band, n_dup};
+    return JoinCounts{n_sure, n_pass, band, n_dup, cols};
 }
 
 // A leaf of the join's current chunk of work.
 struct JoinLeaf {
     uint32_t off, cnt;        // its objects in the sorted arrays
     uint32_t uoff, ucnt;      // its border units
-    uint32_t base;            // first tile slot (leaves joined together)
-    uint32_t task0;           // first task among the group's tasks
-    uint32_t ntask_in;        // inside tasks (upper triangle of 64-row blocks)
-    uint32_t ntask;           // with the border tasks
     uint32_t use32;
     uint32_t node;            // the leaf
     double lx, ly;            // low corner (FP32 offsets of border units)
+    double w;                 // side (sub-cells of the tile sort)
 };
 
 template <int TILE>
@@ -1710,11 +1895,11 @@ struct JoinLds {
     uint32_t stage[kBlock / 64][kStage];
     uint4 ccand[TILE];                   // candidate leaves of the objects of tile a (see DevState::cand)
     uint32_t cmask[TILE];                // and a 32-bit summary of them (cand_mask)
-    JoinLeaf leaf[kInsideChunk];
-    uint32_t gfirst[kInsideChunk + 1];   // groups of consecutive leaves of the chunk
-    uint32_t ngroup;
+    JoinLeaf leaf;                       // the leaf being joined
+    uint32_t hist[kSubCells];            // sub-cell counts of a tile being loaded
     uint32_t item;
 };
+
 
 // Upper-triangle task t of n row blocks -> (r <= q), row-major.
 __device__ __forceinline__ void triangle_task(uint32_t t, uint32_t n, uint32_t &r, uint32_t &q)
@@ -1758,6 +1943,8 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
     uint32_t n_in = 0, n_bd = 0;                   // sure contacts (per lane)
     uint32_t pass_in = 0, pass_bd = 0, band = 0;   // band entries within reach / re-tested (per lane)
     uint32_t dups = 0;                             // unstored pairs found twice (per lane)
+    unsigned long long tests = 0;                  // candidate pairs of the lane's tasks
+    unsigned long long evals = 0;                  // columns the wave evaluated (uniform)
 
     // One task: an inside task (row slot `row` of tile A) or a border task
     // (unit object `pid`, columns in tile a), against columns [jbeg, jend)
@@ -1814,6 +2001,37 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
             xi = yi = __builtin_inf();
             fxi = fyi = __builtin_inff();
         }
+        // the rows' box: the boxes of the row block's column groups for an
+        // inside task, else the units' positions (idle lanes left out)
+        GBox rbox;
+        if (inside) {
+            const uint32_t g0 = (row - lane) / kGroup;   // the row block's first group, for every lane
+            rbox = A.box[g0];
+            for (uint32_t g = g0 + 1; g < g0 + 64 / kGroup && g < (uint32_t)TILE / kGroup; g++) {
+                const GBox &b = A.box[g];
+                rbox = GBox{fminf(rbox.lx, b.lx), fminf(rbox.ly, b.ly), fmaxf(rbox.hx, b.hx), fmaxf(rbox.hy, b.hy)};
+            }
+        } else {
+            float x0 = __builtin_inff(), y0 = __builtin_inff(), x1 = -__builtin_inff(), y1 = -__builtin_inff();
+            if (vi) {
+                x0 = use32 ? fxi : f32_down(xi);
+                x1 = use32 ? fxi : f32_up(xi);
+                y0 = use32 ? fyi : f32_down(yi);
+                y1 = use32 ? fyi : f32_up(yi);
+            }
+            for (int d = 32; d >= 1; d >>= 1) {
+                x0 = fminf(x0, __shfl_xor(x0, d));
+                y0 = fminf(y0, __shfl_xor(y0, d));
+                x1 = fmaxf(x1, __shfl_xor(x1, d));
+                y1 = fmaxf(y1, __shfl_xor(y1, d));
+            }
+            rbox = GBox{x0, y0, x1, y1};
+        }
+        // the same on every lane: held in scalar registers
+        rbox.lx = __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(uint32_t, rbox.lx)));
+        rbox.ly = __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(uint32_t, rbox.ly)));
+        rbox.hx = __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(uint32_t, rbox.hx)));
+        rbox.hy = __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(uint32_t, rbox.hy)));
         rows.p[lane] = pi;
         rows.pf[lane] = make_float2(pfxi, pfyi);
         if (want_pc) rows.ppl[lane] = inside ? A.pl[vi ? row : jbeg] : (vi ? ulf.y : kNone);
@@ -1826,22 +2044,28 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
             (inside && rot)
                 ? (use32 ? join_columns<true, false, TILE, true>(st, s, tv, C, xi, yi, fxi, fyi, pfxi, pfyi,
                                                                 jbeg, jend, s.t32_in, s.t32_out, t64, rows, has_prev,
-                                                                lds.ccand, lds.cmask, leaf, rpc, rpl)
+                                                                lds.ccand, lds.cmask, leaf, rpc, rpl, rbox)
                          : join_columns<false, false, TILE, true>(st, s, tv, C, xi, yi, fxi, fyi, pfxi, pfyi,
                                                                  jbeg, jend, s.t32_in, s.t32_out, t64, rows,
-                                                                 has_prev, lds.ccand, lds.cmask, leaf, rpc, rpl)) :
+                                                                 has_prev, lds.ccand, lds.cmask, leaf, rpc, rpl, rbox)) :
             use32 ? (inside ? join_columns<true, false>(st, s, tv, C, xi, yi, fxi, fyi, pfxi, pfyi, jbeg, jend,
                                                         s.t32_in, s.t32_out, t64, rows, has_prev, lds.ccand, lds.cmask, leaf,
-                                                        rpc, rpl)
+                                                        rpc, rpl, rbox)
                             : join_columns<true, true>(st, s, tv, C, xi, yi, fxi, fyi, pfxi, pfyi, jbeg, jend,
                                                        s.t32_in, s.t32_out, t64, rows, has_prev, lds.ccand, lds.cmask, leaf,
-                                                       rpc, rpl))
+                                                       rpc, rpl, rbox))
                   : (inside ? join_columns<false, false>(st, s, tv, C, xi, yi, fxi, fyi, pfxi, pfyi, jbeg, jend,
                                                          s.t32_in, s.t32_out, t64, rows, has_prev, lds.ccand, lds.cmask, leaf,
-                                                         rpc, rpl)
+                                                         rpc, rpl, rbox)
                             : join_columns<false, true>(st, s, tv, C, xi, yi, fxi, fyi, pfxi, pfyi, jbeg, jend,
                                                         s.t32_in, s.t32_out, t64, rows, has_prev, lds.ccand, lds.cmask, leaf,
-                                                        rpc, rpl));
+                                                        rpc, rpl, rbox));
+        // the lane's candidate pairs: its columns, or on a diagonal task the
+        // lim(i) columns of join_columns' rotation (each pair once)
+        const uint32_t ncol = jend - jbeg;
+        if (vi)
+            tests += (inside && rot) ? (ncol - 1) / 2 + ((ncol % 2 == 0 && lane < ncol / 2) ? 1u : 0u) : ncol;
+        evals += jc.cols;
         n_in += inside ? jc.sure : 0u;
         n_bd += inside ? 0u : jc.sure;
         pass_in += inside ? jc.pass : 0u;
@@ -1851,136 +2075,68 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
     };
 
     for (;;) {
-        if (threadIdx.x == 0) lds.item = atomicAdd(&s.ctr->work_inside, kInsideChunk);
-        __syncthreads();
-        const uint32_t w0 = lds.item;
-        if (w0 >= nl) break;
-        const uint32_t nchunk = min((uint32_t)kInsideChunk, nl - w0);
-        if (threadIdx.x < nchunk) {
-            const uint32_t w = w0 + threadIdx.x;
-            const uint32_t leaf = s.leaf_list[w < nh ? s.node_cap - 1 - w : w - nh];
-            JoinLeaf L;
-            L.off = s.offset[leaf];
-            L.cnt = s.count[leaf];
-            L.uoff = s.ustart[leaf];
-            L.ucnt = s.ucount[leaf];
-            const double2 lo = reinterpret_cast<const double2 *>(s.nodes + leaf)[1];
-            const double2 hi = node_high(s.nodes, leaf);
-            L.lx = lo.x;
-            L.ly = lo.y;
-            L.use32 = s.precision == GLINT_PRECISION_FP32_GUARD && (hi.x - lo.x) <= s.w32_max &&
-                      (hi.y - lo.y) <= s.w32_max;
-            const uint32_t nri = (L.cnt + 63) / 64;
-            L.ntask_in = nri * (nri + 1) / 2;
-            L.ntask = L.ntask_in + (L.cnt ? (L.ucnt + 63) / 64 : 0u);
-            L.base = L.task0 = 0;
-            L.node = leaf;
-            lds.leaf[threadIdx.x] = L;
-        }
-        __syncthreads();
         if (threadIdx.x == 0) {
-            // consecutive leaves that fit one tile form a group; a larger
-            // leaf is a group of its own
-            uint32_t ng = 0, sum = 0, tasks = 0;
-            lds.gfirst[0] = 0;
-            for (uint32_t k = 0; k < nchunk; k++) {
-                const uint32_t c = lds.leaf[k].cnt;
-                const bool open = k > lds.gfirst[ng];
-                if (open && (c > (uint32_t)TILE || sum + c > (uint32_t)TILE ||
-                             lds.leaf[lds.gfirst[ng]].cnt > (uint32_t)TILE)) {
-                    lds.gfirst[++ng] = k;
-                    sum = tasks = 0;
-                }
-                lds.leaf[k].base = sum;
-                lds.leaf[k].task0 = tasks;
-                sum += c;
-                tasks += lds.leaf[k].ntask;
+            const uint32_t w = atomicAdd(&s.ctr->work_inside, 1u);
+            lds.item = w;
+            if (w < nl) {
+                const uint32_t leaf = s.leaf_list[w < nh ? s.node_cap - 1 - w : w - nh];
+                JoinLeaf L;
+                L.off = s.offset[leaf];
+                L.cnt = s.count[leaf];
+                L.uoff = s.ustart[leaf];
+                L.ucnt = s.ucount[leaf];
+                const double2 lo = reinterpret_cast<const double2 *>(s.nodes + leaf)[1];
+                const double2 hi = node_high(s.nodes, leaf);
+                L.lx = lo.x;
+                L.ly = lo.y;
+                L.w = fmax(hi.x - lo.x, hi.y - lo.y);
+                L.use32 = s.precision == GLINT_PRECISION_FP32_GUARD && (hi.x - lo.x) <= s.w32_max &&
+                          (hi.y - lo.y) <= s.w32_max;
+                L.node = leaf;
+                lds.leaf = L;
             }
-            lds.gfirst[++ng] = nchunk;
-            lds.ngroup = ng;
         }
         __syncthreads();
-        const uint32_t ng = lds.ngroup;
-        for (uint32_t g = 0; g < ng; g++) {
-            const uint32_t k0 = lds.gfirst[g], k1 = lds.gfirst[g + 1];
-            const JoinLeaf &F = lds.leaf[k0];
-            if (F.cnt > (uint32_t)TILE) {
-                // one leaf, tile pair by tile pair
-                const uint32_t ntile = (F.cnt + TILE - 1) / TILE;
-                for (uint32_t I = 0; I < ntile; I++) {
-                    const uint32_t ni = min((uint32_t)TILE, F.cnt - I * TILE);
-                    load_tile(lds.a, s, F.off + I * TILE, ni, has_prev, F.use32);
-                    if (!s.store_all)
-                        for (uint32_t k = threadIdx.x; k < ni; k += kBlock)
-                            lds.cmask[k] = cand_mask(
-                                lds.ccand[k] = cand_read(s.tp->cand, F.off + I * TILE + k, s.tp->tick));
-                    for (uint32_t J = I; J < ntile; J++) {
-                        const uint32_t nj = min((uint32_t)TILE, F.cnt - J * TILE);
-                        const bool diag = (J == I);
-                        const JoinTile<TILE> &B = diag ? lds.a : lds.b;
-                        if (!diag) load_tile(lds.b, s, F.off + J * TILE, nj, has_prev, F.use32);
-                        __syncthreads();
-                        const uint32_t nri = (ni + 63) / 64, nrj = (nj + 63) / 64;
-                        const uint32_t ntask_in = diag ? nri * (nri + 1) / 2 : nri * nrj;
-                        // border tasks once per tile of the leaf: 64 units against the tile
-                        const uint32_t nuc = diag ? (F.ucnt + 63) / 64 : 0;
-                        for (uint32_t t = wave; t < ntask_in + nuc; t += kBlock / 64) {
-                            if (t < ntask_in) {
-                                uint32_t r, q;
-                                if (diag) {
-                                    triangle_task(t, nri, r, q);
-                                } else {
-                                    r = t / nrj;
-                                    q = t % nrj;
-                                }
-                                const uint32_t row = r * 64 + lane;
-                                const uint32_t jbeg = q * 64;
-                                run(lds.a, B, true, row, 0, make_uint4(0u, 0u, 0u, 0u), row < ni, jbeg, min(jbeg + 64, nj), diag && r == q,
-                                    F.use32, F.lx, F.ly, F.node);
-                            } else {
-                                const uint32_t u = (t - ntask_in) * 64 + lane;
-                                const bool vi = u < F.ucnt;
-                                run(lds.a, lds.a, false, 0, vi ? s.unit_pid[F.uoff + u] : 0u,
-                                    vi ? s.unit_lf[F.uoff + u] : make_uint4(0u, 0u, 0u, 0u), vi, 0, ni, false, F.use32,
-                                    F.lx, F.ly, F.node);
-                            }
-                        }
-                        __syncthreads();
-                    }
-                }
-            } else {
-                // leaves k0..k1 side by side in tile a, their tasks shared by all waves
-                const JoinLeaf &E = lds.leaf[k1 - 1];
-                const uint32_t total = E.base + E.cnt, ntask = E.task0 + E.ntask;
-                for (uint32_t i = threadIdx.x; i < total; i += kBlock) {
-                    uint32_t k = k0;
-                    while (k + 1 < k1 && lds.leaf[k + 1].base <= i) k++;
-                    const JoinLeaf &L = lds.leaf[k];
-                    const uint32_t from = L.off + (i - L.base);
-                    tile_put(lds.a, i, s, from, has_prev, L.use32, has_prev && !s.store_all);
-                    if (!s.store_all)
-                        lds.cmask[i] = cand_mask(lds.ccand[i] = cand_read(s.tp->cand, from, s.tp->tick));
-                }
-                __syncthreads();
-                for (uint32_t t = wave; t < ntask; t += kBlock / 64) {
-                    uint32_t k = k0;
-                    while (k + 1 < k1 && lds.leaf[k + 1].task0 <= t) k++;
-                    const JoinLeaf &L = lds.leaf[k];
-                    const uint32_t lt = t - L.task0;
-                    if (lt < L.ntask_in) {
+        const uint32_t item = lds.item;
+        const JoinLeaf F = lds.leaf;
+        __syncthreads();   // read by every thread before thread 0 takes the next leaf
+        if (item >= nl) break;
+        // the leaf tile pair by tile pair (one tile when it fits); border
+        // units once per tile of the leaf, against that tile
+        const uint32_t ntile = (F.cnt + TILE - 1) / TILE;
+        for (uint32_t I = 0; I < ntile; I++) {
+            const uint32_t ni = min((uint32_t)TILE, F.cnt - I * TILE);
+            load_tile_sorted(lds.a, s, F.off + I * TILE, ni, has_prev, F.use32, F.lx, F.ly, F.w, !s.store_all,
+                             lds.ccand, lds.cmask, lds.hist);
+            for (uint32_t J = I; J < ntile; J++) {
+                const uint32_t nj = min((uint32_t)TILE, F.cnt - J * TILE);
+                const bool diag = (J == I);
+                const JoinTile<TILE> &B = diag ? lds.a : lds.b;
+                if (!diag)
+                    load_tile_sorted(lds.b, s, F.off + J * TILE, nj, has_prev, F.use32, F.lx, F.ly, F.w, false,
+                                     nullptr, nullptr, lds.hist);
+                const uint32_t nri = (ni + 63) / 64, nrj = (nj + 63) / 64;
+                const uint32_t ntask_in = diag ? nri * (nri + 1) / 2 : nri * nrj;
+                const uint32_t nuc = diag ? (F.ucnt + 63) / 64 : 0;
+                for (uint32_t t = wave; t < ntask_in + nuc; t += kBlock / 64) {
+                    if (t < ntask_in) {
                         uint32_t r, q;
-                        triangle_task(lt, (L.cnt + 63) / 64, r, q);
-                        const uint32_t row = L.base + r * 64 + lane;
-                        const uint32_t jbeg = L.base + q * 64;
-                        const uint32_t jend = L.base + min(q * 64 + 64, L.cnt);
-                        run(lds.a, lds.a, true, row, 0, make_uint4(0u, 0u, 0u, 0u), r * 64 + lane < L.cnt, jbeg, jend,
-                            r == q, L.use32, L.lx, L.ly, L.node);
+                        if (diag) {
+                            triangle_task(t, nri, r, q);
+                        } else {
+                            r = t / nrj;
+                            q = t % nrj;
+                        }
+                        const uint32_t row = r * 64 + lane;
+                        const uint32_t jbeg = q * 64;
+                        run(lds.a, B, true, row, 0, make_uint4(0u, 0u, 0u, 0u), row < ni, jbeg, min(jbeg + 64, nj),
+                            diag && r == q, F.use32, F.lx, F.ly, F.node);
                     } else {
-                        const uint32_t u = (lt - L.ntask_in) * 64 + lane;
-                        const bool vi = u < L.ucnt;
-                        run(lds.a, lds.a, false, 0, vi ? s.unit_pid[L.uoff + u] : 0u,
-                            vi ? s.unit_lf[L.uoff + u] : make_uint4(0u, 0u, 0u, 0u), vi, L.base, L.base + L.cnt,
-                            false, L.use32, L.lx, L.ly, L.node);
+                        const uint32_t u = (t - ntask_in) * 64 + lane;
+                        const bool vi = u < F.ucnt;
+                        run(lds.a, lds.a, false, 0, vi ? s.unit_pid[F.uoff + u] : 0u,
+                            vi ? s.unit_lf[F.uoff + u] : make_uint4(0u, 0u, 0u, 0u), vi, 0, ni, false, F.use32,
+                            F.lx, F.ly, F.node);
                     }
                 }
                 __syncthreads();
@@ -1996,8 +2152,11 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
         bd += __shfl_xor(bd, d);
         du += __shfl_xor(du, d);
     }
+    tests = wave_sum_u64(tests);
     if (lane == 0) {
         const uint32_t ci = pi_, cb = pb;
+        if (tests) atomicAdd(&s.ctr->n_tests, tests);
+        if (evals) atomicAdd(&s.ctr->n_evals, 64ull * evals);
         if (ci + cb) atomicAdd(&s.ctr->n_contacts, (unsigned long long)(ci + cb));
         if (cb) atomicAdd(&s.ctr->n_contacts_border, (unsigned long long)cb);
         if (bd) atomicAdd(&s.ctr->n_rechecks, (unsigned long long)bd);
@@ -2015,10 +2174,10 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
 // runs.  Pass 2 (k_bucket_fine) splits every coarse bin by the low bits with
 // one workgroup per bin and records the first contact of every region.
 
-__device__ __forceinline__ uint32_t coarse_bin(const DevState &s, unsigned long long tagged)
+__device__ __forceinline__ uint32_t coarse_bin(const TableShape &sh, unsigned long long tagged)
 {
-    if (tagged == kEmptyKey) return s.coarse_bins;
-    return region_of(s, tagged & kKeyBits) >> s.lo_bits;
+    if (tagged == kEmptyKey) return sh.coarse;
+    return region_of(tagged & kKeyBits, sh.bits) >> sh.lo_bits;
 }
 
 __device__ __forceinline__ void block_range(const DevState &s, unsigned long long n, unsigned long long &b0,
@@ -2033,7 +2192,8 @@ __global__ __launch_bounds__(kBlock) void k_bucket_count(DevState s)
 {
     __shared__ uint32_t hist[kMaxCoarse + 1];
     const unsigned long long n = min(s.ctr->n_clist, s.clist_cap);
-    for (uint32_t k = threadIdx.x; k <= s.coarse_bins; k += kBlock) hist[k] = 0;
+    const TableShape sh = table_shape(s);
+    for (uint32_t k = threadIdx.x; k <= sh.coarse; k += kBlock) hist[k] = 0;
     __syncthreads();
     unsigned long long b0, b1;
     block_range(s, n, b0, b1);
@@ -2046,10 +2206,12 @@ __global__ __launch_bounds__(kBlock) void k_bucket_count(DevState s)
         for (int q = 0; q < kU; q++) key[q] = i0 + q * kBlock < b1 ? in[i0 + q * kBlock] : 0ull;
 #pragma unroll
         for (int q = 0; q < kU; q++)
-            if (i0 + q * kBlock < b1) atomicAdd(&hist[coarse_bin(s, key[q])], 1u);
+            if (i0 + q * kBlock < b1) atomicAdd(&hist[coarse_bin(sh, key[q])], 1u);
     }
     __syncthreads();
-    for (uint32_t k = threadIdx.x; k <= s.coarse_bins; k += kBlock) s.bcount[(size_t)k * gridDim.x + blockIdx.x] = hist[k];
+    // bins past this tick's (the empty-slot bin is sh.coarse) count nothing
+    for (uint32_t k = threadIdx.x; k <= s.coarse_max; k += kBlock)
+        s.bcount[(size_t)k * gridDim.x + blockIdx.x] = k <= sh.coarse ? hist[k] : 0u;
 }
 
 // Exclusive offsets of a scanned counter array (third kernel of the scan).
@@ -2077,9 +2239,10 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_scatter(DevState s)
     __shared__ uint32_t hist[kMaxCoarse + 1], start[kMaxCoarse + 1], running[kMaxCoarse + 1];
     __shared__ uint32_t wsum[kBucketThreads / 64];
     const unsigned long long n = min(s.ctr->n_clist, s.clist_cap);
-    const uint32_t nb = s.coarse_bins + 1;
+    const TableShape sh = table_shape(s);
+    const uint32_t nb = sh.coarse + 1;
     const unsigned long long *in = contact_out(s);
-    unsigned long long *out = s.lo_bits ? s.clist2 : s.tp->tab_cur.keys;
+    unsigned long long *out = s.clist2;
     unsigned long long b0, b1;
     block_range(s, n, b0, b1);
     for (uint32_t k = threadIdx.x; k < nb; k += kBucketThreads) running[k] = s.boff[(size_t)k * gridDim.x + blockIdx.x];
@@ -2098,7 +2261,7 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_scatter(DevState s)
 #pragma unroll
         for (int q = 0; q < kBucketTile / kBucketThreads; q++) {
             const uint32_t i = threadIdx.x + q * kBucketThreads;
-            rk[q] = i < tn ? atomicAdd(&hist[coarse_bin(s, keys[q])], 1u) : 0u;
+            rk[q] = i < tn ? atomicAdd(&hist[coarse_bin(sh, keys[q])], 1u) : 0u;
         }
         __syncthreads();
         // exclusive scan of the tile histogram (nb <= 1025 bins, kPer per thread)
@@ -2129,13 +2292,13 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_scatter(DevState s)
 #pragma unroll
         for (int q = 0; q < kBucketTile / kBucketThreads; q++) {
             uint32_t i = threadIdx.x + q * kBucketThreads;
-            if (i < tn) tile[start[coarse_bin(s, keys[q])] + rk[q]] = keys[q];
+            if (i < tn) tile[start[coarse_bin(sh, keys[q])] + rk[q]] = keys[q];
         }
         __syncthreads();
         for (uint32_t p = threadIdx.x; p < tn; p += kBucketThreads) {
             unsigned long long key = tile[p];
-            uint32_t b = coarse_bin(s, key);
-            if (b < s.coarse_bins) out[running[b] + (p - start[b])] = key;
+            uint32_t b = coarse_bin(sh, key);
+            if (b < sh.coarse) out[running[b] + (p - start[b])] = key;
         }
         __syncthreads();
         for (uint32_t k = threadIdx.x; k < nb; k += kBucketThreads) running[k] += hist[k];
@@ -2182,10 +2345,11 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_fine(DevState s)
     __shared__ uint32_t thist[kMaxFine], tstart[kMaxFine], gcur[kMaxFine];
     __shared__ uint32_t wsum[kBucketThreads / 64];
     const uint32_t G1 = s.bucket_blocks;
-    const uint32_t nf = 1u << s.lo_bits;
+    const TableShape sh = table_shape(s);
+    const uint32_t nf = 1u << sh.lo_bits;
     unsigned long long *out = s.tp->tab_cur.keys;
     uint32_t *rstart = s.tp->tab_cur.rstart;
-    for (uint32_t c = blockIdx.x; c < s.coarse_bins; c += gridDim.x) {
+    for (uint32_t c = blockIdx.x; c < sh.coarse; c += gridDim.x) {
         const uint32_t seg0 = s.boff[(size_t)c * G1], seg1 = s.boff[(size_t)(c + 1) * G1];
         for (uint32_t k = threadIdx.x; k < nf; k += kBucketThreads) thist[k] = 0;
         __syncthreads();
@@ -2195,7 +2359,7 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_fine(DevState s)
             for (int q = 0; q < 4; q++) key[q] = i0 + q * kBucketThreads < seg1 ? s.clist2[i0 + q * kBucketThreads] : 0ull;
 #pragma unroll
             for (int q = 0; q < 4; q++)
-                if (i0 + q * kBucketThreads < seg1) atomicAdd(&thist[region_of(s, key[q] & kKeyBits) & (nf - 1)], 1u);
+                if (i0 + q * kBucketThreads < seg1) atomicAdd(&thist[region_of(key[q] & kKeyBits, sh.bits) & (nf - 1)], 1u);
         }
         __syncthreads();
         block_scan_bins(thist, nf, gcur, seg0, wsum);
@@ -2215,7 +2379,7 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_fine(DevState s)
 #pragma unroll
             for (int q = 0; q < kFineTile / kBucketThreads; q++) {
                 const uint32_t i = threadIdx.x + q * kBucketThreads;
-                fb[q] = region_of(s, keys[q] & kKeyBits) & (nf - 1);
+                fb[q] = region_of(keys[q] & kKeyBits, sh.bits) & (nf - 1);
                 rk[q] = i < tn ? atomicAdd(&thist[fb[q]], 1u) : 0u;   // its rank in the region
             }
             __syncthreads();
@@ -2229,7 +2393,7 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_fine(DevState s)
             __syncthreads();
             for (uint32_t p = threadIdx.x; p < tn; p += kBucketThreads) {
                 const unsigned long long key = tile[p];
-                const uint32_t f = region_of(s, key & kKeyBits) & (nf - 1);
+                const uint32_t f = region_of(key & kKeyBits, sh.bits) & (nf - 1);
                 out[gcur[f] + (p - tstart[f])] = key;
             }
             __syncthreads();
@@ -2237,16 +2401,15 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_fine(DevState s)
             __syncthreads();
         }
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) rstart[(size_t)s.coarse_bins * nf] = s.boff[(size_t)s.coarse_bins * G1];
-}
-
-// Without a second pass the coarse bins are the regions.
-__global__ void k_bucket_single(DevState s)
-{
-    const uint32_t G1 = s.bucket_blocks;
-    uint32_t *rstart = s.tp->tab_cur.rstart;
-    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c <= s.coarse_bins; c += gridDim.x * blockDim.x)
-        rstart[c] = s.boff[(size_t)c * G1];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        // the generation's entry count and region bits, where the next tick
+        // (and the readers of a generation) find them whatever its shape
+        const uint32_t total = s.boff[(size_t)sh.coarse * G1];
+        rstart[(size_t)sh.coarse * nf] = total;
+        rstart[s.rs_total] = total;
+        rstart[s.rs_total + 1] = sh.bits;
+        s.ctr->tab_bits = sh.bits;
+    }
 }
 
 // ------------------------------------------------------------ table update
@@ -2437,13 +2600,28 @@ __device__ __forceinline__ void table_write(const TableLds<KV, NT> &L, const Dev
 // does not depend on the order of the updates.  Boxes stay in HBM beside the
 // list entries; the points come from a float copy of the positions that
 // fits the last-level cache.
-template <bool KV, int NT>
+// RESHAPED: the variant for a tick whose table shape differs from the
+// previous generation's; the host launches both variants and the one that
+// does not apply returns at once (the common variant then needs no registers
+// for the mapping between shapes).
+template <bool KV, int NT, bool RESHAPED>
 __global__ __launch_bounds__(NT) void k_table(DevState s)
 {
     __shared__ TableLds<KV, NT> L;
     const TableGen told = s.tp->tab_old, tcur = s.tp->tab_cur;
     const uint32_t tick = s.tp->tick, last = s.tp->prev_tick;
-    const uint32_t nreg = 1u << s.region_bits;
+    // regions of this tick, and of the previous generation (its shape came
+    // from its own tick's count): region r holds the pairs whose hash has r
+    // in its high `bits` bits, so an old region of fewer bits holds the
+    // pairs of 2^(bits - pbits) regions (filtered here by the hash), and a
+    // region of this tick holds those of 2^(pbits - bits) consecutive old ones
+    const uint32_t bits = table_shape(s).bits, pbits_gen = told.rstart[s.rs_total + 1];
+    if ((pbits_gen != bits) != RESHAPED) return;
+    const uint32_t pbits = RESHAPED ? pbits_gen : bits;
+    const uint32_t nreg = 1u << bits;
+    const bool filter_old = pbits < bits;
+    // old region range of region rr: rstart[(rr >> dn) << up] .. rstart[((rr >> dn) + 1) << up]
+    const uint32_t up = pbits > bits ? pbits - bits : 0u, dn = bits > pbits ? bits - pbits : 0u;
     glint_meeting *st = L.stage[threadIdx.x / 64];
     uint32_t nst = 0, dups = 0;
     const bool void_tick = (s.ctr->error & kErrContactCap) != 0;
@@ -2458,8 +2636,8 @@ __global__ __launch_bounds__(NT) void k_table(DevState s)
     uint32_t bo0 = 0, bo1 = 0, bc0 = 0, bc1 = 0;
     auto bounds = [&](uint32_t rr) {
         if (rr >= nreg) return;
-        bo0 = told.rstart[rr];
-        bo1 = told.rstart[rr + 1];
+        bo0 = told.rstart[(rr >> dn) << up];
+        bo1 = told.rstart[((rr >> dn) + 1) << up];
         bc0 = void_tick ? 0 : tcur.rstart[rr];
         bc1 = void_tick ? 0 : tcur.rstart[rr + 1];
     };
@@ -2503,15 +2681,25 @@ __global__ __launch_bounds__(NT) void k_table(DevState s)
         }
         lds_barrier();
         // the previous generation's entries of the region
+        if (filter_old) {   // a table that grew since the previous tick: only this region's pairs
 #pragma unroll
-        for (int q = 0; q < kPer; q++) {
-            const uint32_t i = o0 + threadIdx.x + q * NT;
-            table_old(L, s, i < o1, pk[q], ps[q], i, tick, last, st, nst);
+            for (int q = 0; q < kPer; q++) {
+                const uint32_t i = o0 + threadIdx.x + q * NT;
+                const bool mine = region_of(pk[q] & kKeyBits, bits) == r;
+                table_old(L, s, i < o1 && mine, pk[q], ps[q], i, tick, last, st, nst);
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < kPer; q++) {
+                const uint32_t i = o0 + threadIdx.x + q * NT;
+                table_old(L, s, i < o1, pk[q], ps[q], i, tick, last, st, nst);
+            }
         }
         for (uint32_t b = o0 + kRegion; b < o1; b += NT) {   // regions of more than kRegion entries
             const uint32_t i = b + threadIdx.x;
-            const bool v = i < o1;
-            table_old(L, s, v, v ? told.keys[i] : 0ull, v ? told.starts[i] : kDupStart, i, tick, last, st, nst);
+            const unsigned long long k = i < o1 ? told.keys[i] : 0ull;
+            const bool v = i < o1 && (!filter_old || region_of(k & kKeyBits, bits) == r);
+            table_old(L, s, v, k, v ? told.starts[i] : kDupStart, i, tick, last, st, nst);
         }
         const uint32_t rn = r + gridDim.x;
         if (rn < nreg) prefetch(rn);
@@ -2574,13 +2762,13 @@ double sqrt_threshold(double r)
 enum KernelId {
     K_FILL_COUNTS, K_RESET, K_ASSIGN, K_SCAN_REDUCE, K_SCAN_TOP, K_SCAN_OBJECTS, K_SCATTER, K_FILL_UCOUNT,
     K_LOOKUP, K_SCAN_UNITS, K_UNIT_SCATTER, K_JOIN, K_BUCKET_COUNT, K_SCAN_WRITE, K_BUCKET_SCATTER, K_BUCKET_FINE,
-    K_TABLE, K_COLLECT, K_SPLIT, K_MERGE, K_SCHEMA_DONE, K_TICK_PARAMS, K_POS32, K_LOG_APPEND, K_GRID_CELLS,
+    K_TABLE, K_TABLE_RESHAPED, K_COLLECT, K_SPLIT, K_MERGE, K_SCHEMA_DONE, K_TICK_PARAMS, K_POS32, K_LOG_APPEND, K_GRID_CELLS,
     K_GRID_PYRAMID, K_GRID_TOP, K_GRID_FINE_ZERO, K_GRID_FINE, K_GRID_LEVEL, K_COUNT
 };
 const char *const kKernelNames[K_COUNT] = {
     "k_fill_u32", "k_reset_tick", "k_assign", "k_scan_reduce", "k_scan_top", "k_scan_objects", "k_scatter",
     "k_fill_ucount", "k_lookup", "k_scan_units", "k_unit_scatter", "k_join", "k_bucket_count", "k_scan_write",
-    "k_bucket_scatter", "k_bucket_fine", "k_table", "k_collect", "k_split", "k_merge", "k_schema_done",
+    "k_bucket_scatter", "k_bucket_fine", "k_table", "k_table_reshaped", "k_collect", "k_split", "k_merge", "k_schema_done",
     "k_advance_tick", "k_pos32", "k_log_append", "k_grid_cells", "k_grid_pyramid", "k_grid_top",
     "k_grid_fine_zero", "k_grid_fine", "k_grid_level"};
 
@@ -2882,17 +3070,16 @@ void enqueue_refine(glint_ctx *c)
     else
         launch(c, K_JOIN, [&] { k_join<kTile><<<c->join_blocks, kBlock, 0, st>>>(s); });
     // group the tick's contacts by table region
+    // (the table's shape is decided on the device from the tick's stored
+    // contacts; launch sizes cover the largest shape)
     const uint32_t G1 = s.bucket_blocks;
-    const uint32_t nc = (s.coarse_bins + 1) * G1, nb = (nc + kScanTile - 1) / kScanTile;
+    const uint32_t nc = (s.coarse_max + 1) * G1, nb = (nc + kScanTile - 1) / kScanTile;
     launch(c, K_BUCKET_COUNT, [&] { k_bucket_count<<<G1, kBlock, 0, st>>>(s); });
     launch(c, K_SCAN_REDUCE, [&] { k_scan_reduce<<<nb, kBlock, 0, st>>>(s.bcount, nc, s.block_sums); });
     launch(c, K_SCAN_TOP, [&] { k_scan_top<<<1, kBlock, 0, st>>>(s.block_sums, nb); });
     launch(c, K_SCAN_WRITE, [&] { k_scan_write<<<nb, kBlock, 0, st>>>(s.bcount, nc, s.block_sums, s.boff); });
     launch(c, K_BUCKET_SCATTER, [&] { k_bucket_scatter<<<G1, kBucketThreads, 0, st>>>(s); });
-    if (s.lo_bits)
-        launch(c, K_BUCKET_FINE, [&] { k_bucket_fine<<<s.coarse_bins, kBucketThreads, 0, st>>>(s); });
-    else
-        launch(c, K_BUCKET_FINE, [&] { k_bucket_single<<<grid_for(s.coarse_bins + 1), kBlock, 0, st>>>(s); });
+    launch(c, K_BUCKET_FINE, [&] { k_bucket_fine<<<s.coarse_max, kBucketThreads, 0, st>>>(s); });
 }
 
 // Location updates scattered into the resident positions; pids out of
@@ -2922,26 +3109,34 @@ __global__ __launch_bounds__(kBlock) void k_pos32(DevState s)
 void enqueue_evaluate(glint_ctx *c)
 {
     if (c->kv) launch(c, K_POS32, [&] { k_pos32<<<grid_for(c->s.n), kBlock, 0, c->stream>>>(c->s); });
-    const uint32_t nreg = 1u << c->s.region_bits;
-    const uint32_t g = std::min<uint32_t>(nreg, 4096);
+    const uint32_t g = std::min<uint32_t>(1u << c->s.rbits_max, 4096);   // persistent over the regions
     const uint32_t nt = c->cfg.table_threads ? c->cfg.table_threads : 512;
-    launch(c, K_TABLE, [&] {
-        if (c->kv) {
-            if (nt == 1024)
-                k_table<true, 1024><<<g, 1024, 0, c->stream>>>(c->s);
-            else if (nt == 512)
-                k_table<true, 512><<<g, 512, 0, c->stream>>>(c->s);
-            else
-                k_table<true, 256><<<g, 256, 0, c->stream>>>(c->s);
-        } else {
-            if (nt == 1024)
-                k_table<false, 1024><<<g, 1024, 0, c->stream>>>(c->s);
-            else if (nt == 512)
-                k_table<false, 512><<<g, 512, 0, c->stream>>>(c->s);
-            else
-                k_table<false, 256><<<g, 256, 0, c->stream>>>(c->s);
-        }
-    });
+    // both shape variants; the one that does not apply to the tick returns at once
+    for (int reshaped = 0; reshaped < 2; reshaped++)
+        launch(c, reshaped ? K_TABLE_RESHAPED : K_TABLE, [&] {
+            hipStream_t st = c->stream;
+            if (c->kv) {
+                if (nt == 1024)
+                    reshaped ? k_table<true, 1024, true><<<g, 1024, 0, st>>>(c->s)
+                             : k_table<true, 1024, false><<<g, 1024, 0, st>>>(c->s);
+                else if (nt == 512)
+                    reshaped ? k_table<true, 512, true><<<g, 512, 0, st>>>(c->s)
+                             : k_table<true, 512, false><<<g, 512, 0, st>>>(c->s);
+                else
+                    reshaped ? k_table<true, 256, true><<<g, 256, 0, st>>>(c->s)
+                             : k_table<true, 256, false><<<g, 256, 0, st>>>(c->s);
+            } else {
+                if (nt == 1024)
+                    reshaped ? k_table<false, 1024, true><<<g, 1024, 0, st>>>(c->s)
+                             : k_table<false, 1024, false><<<g, 1024, 0, st>>>(c->s);
+                else if (nt == 512)
+                    reshaped ? k_table<false, 512, true><<<g, 512, 0, st>>>(c->s)
+                             : k_table<false, 512, false><<<g, 512, 0, st>>>(c->s);
+                else
+                    reshaped ? k_table<false, 256, true><<<g, 256, 0, st>>>(c->s)
+                             : k_table<false, 256, false><<<g, 256, 0, st>>>(c->s);
+            }
+        });
     if (c->s.mlog) launch(c, K_LOG_APPEND, [&] { k_log_append<<<1024, kBlock, 0, c->stream>>>(c->s); });
 }
 
@@ -2978,7 +3173,11 @@ void fill_stats(glint_ctx *c)
     t.num_stored = h.n_stored;
     t.num_meetings = h.n_meet;
     t.num_rechecks = h.n_rechecks;
-    t.table_slots = c->s.tab_mask + 1;
+    t.table_slots = (uint64_t)kRegion << h.tab_bits;
+    t.ref_drop_leaves = h.n_drop_leaves;
+    t.ref_drop_objects = h.n_drop_objs;
+    t.num_tests = h.n_tests;
+    t.num_evaluated = h.n_evals;
 }
 
 }  // namespace
@@ -3134,22 +3333,28 @@ glint_status glint_create(const glint_config *cfg, glint_ctx **out)
         c->err = "num_meeting_buckets above 2^32";
         return fail(GLINT_ERR_INVALID_ARGUMENT);
     }
-    s.tab_mask = slots - 1;
-    {
-        uint32_t bits = 0;
-        while ((1ull << bits) < slots / kRegion) bits++;
-        s.region_bits = bits;
-        s.lo_bits = bits > 10 ? bits - 10 : 0;
-        s.coarse_bins = 1u << (bits - s.lo_bits);
-        s.bucket_blocks = 1024;
-    }
     s.unit_cap = cfg->unit_capacity ? cfg->unit_capacity : std::max<uint64_t>(4 * N, 1024);
     s.meet_cap = cfg->meeting_capacity ? cfg->meeting_capacity : std::max<uint64_t>(2 * N, 1u << 20);
     // contacts per tick plus the unused tails of the join's per-wave chunks
     s.clist_cap = (cfg->contact_capacity ? cfg->contact_capacity : 16 * N + (1u << 20)) +
                   (uint64_t)c->join_blocks * (kBlock / 64) * kChunk;
+    {
+        // the configured slots fix the fewest regions a tick uses; a tick
+        // that stores more contacts than they hold at kRegionFill per region
+        // uses more, up to the most its contact list can fill
+        uint32_t bits = 0;
+        while ((1ull << bits) < slots / kRegion) bits++;
+        uint32_t most = 0;
+        while (most < kMaxRegionBits && ((uint64_t)kRegionFill << most) < s.clist_cap) most++;
+        s.rbits_min = std::min<uint32_t>(bits, kMaxRegionBits);
+        s.rbits_max = std::max(s.rbits_min, most);
+        const uint32_t lo = s.rbits_max > 10 ? s.rbits_max - 10 : 0;
+        s.coarse_max = 1u << (s.rbits_max - lo);
+        s.rs_total = 1u << s.rbits_max;
+        s.bucket_blocks = 1024;
+    }
     c->scan_blocks = (uint32_t)((node_cap + kScanTile - 1) / kScanTile);
-    const uint64_t nbc = (uint64_t)(s.coarse_bins + 1) * s.bucket_blocks;
+    const uint64_t nbc = (uint64_t)(s.coarse_max + 1) * s.bucket_blocks;
     const uint32_t sum_blocks = std::max<uint32_t>(c->scan_blocks, (uint32_t)((nbc + kScanTile - 1) / kScanTile));
 
     glint_status st = GLINT_OK;
@@ -3230,7 +3435,7 @@ glint_status glint_create(const glint_config *cfg, glint_ctx **out)
     for (int g = 0; g < 2; g++) {
         ALLOC(c->tab[g].keys, s.clist_cap);
         ALLOC(c->tab[g].starts, s.clist_cap);
-        ALLOC(c->tab[g].rstart, (1ull << s.region_bits) + 1);
+        ALLOC(c->tab[g].rstart, (uint64_t)s.rs_total + 2);
     }
     ALLOC(c->own_xy, 2 * N);
     if (cfg->meeting_log_capacity) {
@@ -3264,8 +3469,8 @@ glint_status glint_create(const glint_config *cfg, glint_ctx **out)
     }
     // both generations start empty
     for (int g = 0; g < 2; g++)
-        k_fill_u32<<<grid_for((1ull << s.region_bits) + 1), kBlock, 0, c->stream>>>(c->tab[g].rstart, 0u,
-                                                                                 (1ull << s.region_bits) + 1);
+        k_fill_u32<<<grid_for((uint64_t)s.rs_total + 2), kBlock, 0, c->stream>>>(c->tab[g].rstart, 0u,
+                                                                               (uint64_t)s.rs_total + 2);
     e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) {
         c->err = std::string("initialisation: ") + hipGetErrorString(e);
@@ -3366,7 +3571,7 @@ glint_status glint_ctx_view(glint_ctx *c, CtxView *v)
     const TableGen &tab = c->tab[c->evaluated ? 1 - c->cur_gen : c->cur_gen];
     v->keys = tab.keys;
     v->starts = tab.starts;
-    v->count = tab.rstart + (1ull << c->s.region_bits);
+    v->count = tab.rstart + c->s.rs_total;
     v->cap = c->s.clist_cap;
     v->mlog = c->s.mlog;
     v->mlog_cap = c->s.mlog_cap;
@@ -3576,14 +3781,20 @@ glint_status glint_run_ticks(glint_ctx *c, const double *d_trace, uint32_t first
     if (G && !c->prof.on && nticks >= G) {
         if (!c->graph_exec) {
             // G consecutive ticks captured once; every tick reads its parameters
-            // from device memory, so the graph is replayed unchanged
+            // from device memory, so the graph is replayed unchanged.  Every
+            // exit after the capture began ends it, so the stream stays usable.
             hipGraph_t graph = nullptr;
             HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
             for (uint32_t k = 0; k < G; k++) enqueue_tick_body(c, stride);
-            HIP_TRY(c, hipStreamEndCapture(c->stream, &graph));
-            hipError_t e = hipGraphInstantiate(&c->graph_exec, graph, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(graph);
-            if (e != hipSuccess) return hip_fail(c, e, "hipGraphInstantiate");
+            hipError_t e = hipGetLastError();
+            const hipError_t ee = hipStreamEndCapture(c->stream, &graph);
+            if (e == hipSuccess) e = ee;
+            if (e == hipSuccess) e = hipGraphInstantiate(&c->graph_exec, graph, nullptr, nullptr, 0);
+            if (graph) (void)hipGraphDestroy(graph);
+            if (e != hipSuccess) {
+                c->graph_exec = nullptr;
+                return hip_fail(c, e, "graph capture of the tick sequence");
+            }
         }
         for (; done + G <= nticks; done += G) HIP_TRY(c, hipGraphLaunch(c->graph_exec, c->stream));
     }
@@ -3696,7 +3907,7 @@ glint_status glint_active_pairs(glint_ctx *c, uint64_t *keys, uint32_t *starts, 
     // after evaluate the current tick's pairs are the previous generation
     const TableGen &tab = c->tab[c->evaluated ? 1 - c->cur_gen : c->cur_gen];
     uint32_t n = 0;
-    HIP_TRY(c, hipMemcpy(&n, tab.rstart + (1ull << c->s.region_bits), 4, hipMemcpyDeviceToHost));
+    HIP_TRY(c, hipMemcpy(&n, tab.rstart + c->s.rs_total, 4, hipMemcpyDeviceToHost));
     std::vector<unsigned long long> hk(n);
     std::vector<uint32_t> hs(n);
     if (n) {
@@ -3779,6 +3990,12 @@ glint_status glint_totals_get(glint_ctx *c, glint_totals *out, int reset)
     out->stored = h.acc_stored;
     out->max_stored_tick = h.max_stored;
     out->max_meetings_tick = h.max_meet;
+    out->ref_drop_leaves = h.acc_drop_leaves;
+    out->ref_drop_objects = h.acc_drop_objs;
+    out->max_ref_drop_objects_tick = h.max_drop_objs;
+    out->max_leaf_count = h.max_leaf_run;
+    out->tests = h.acc_tests;
+    out->evaluated = h.acc_evals;
     if (reset) {
         size_t off = offsetof(Counters, acc_ticks);
         HIP_TRY(c, hipMemsetAsync((char *)c->s.ctr + off, 0, sizeof(Counters) - off, c->stream));
@@ -3868,7 +4085,13 @@ glint_status glint_batch_run_ticks(glint_batch *b, const double *const *d_traces
     } while (0)
     for (uint32_t i = 0; i < n; i++) {
         if (!d_traces[i]) return GLINT_ERR_INVALID_ARGUMENT;
-        if (b->ctx[i]->prof.on) b->ctx[i]->prof.on = false;
+        if (b->ctx[i]->prof.on) {
+            // per-kernel events of the scenarios would serialise their streams
+            b->err = "scenario " + std::to_string(i) + ": kernel profiling is on; a batch runs without it";
+            return GLINT_ERR_STATE;
+        }
+    }
+    for (uint32_t i = 0; i < n; i++) {
         glint_status st = run_begin(b->ctx[i], d_traces[i], first_tick, nticks);
         if (st) return ctx_fail(i, st);
     }
@@ -3877,21 +4100,32 @@ glint_status glint_batch_run_ticks(glint_batch *b, const double *const *d_traces
     uint32_t done = 0;
     if (G && nticks >= G) {
         if (!b->exec) {
+            // every exit after the capture began ends it (the forked streams
+            // join the origin before the capture ends)
             hipGraph_t graph = nullptr;
             BATCH_TRY(hipStreamBeginCapture(b->origin, hipStreamCaptureModeThreadLocal));
-            BATCH_TRY(hipEventRecord(b->fork, b->origin));
-            for (uint32_t i = 0; i < n; i++) {
+            hipError_t e = hipEventRecord(b->fork, b->origin);
+            uint32_t forked = 0;
+            for (uint32_t i = 0; i < n && e == hipSuccess; i++) {
                 glint_ctx *c = b->ctx[i];
-                BATCH_TRY(hipStreamWaitEvent(c->stream, b->fork, 0));
+                e = hipStreamWaitEvent(c->stream, b->fork, 0);
+                if (e != hipSuccess) break;
+                forked = i + 1;
                 for (uint32_t k = 0; k < G; k++) enqueue_tick_body(c, 2ull * c->s.n);
-                BATCH_TRY(hipEventRecord(b->join[i], c->stream));
-                BATCH_TRY(hipStreamWaitEvent(b->origin, b->join[i], 0));
+                e = hipGetLastError();
             }
-            BATCH_TRY(hipStreamEndCapture(b->origin, &graph));
-            hipError_t e = hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(graph);
+            for (uint32_t i = 0; i < forked; i++) {
+                const hipError_t ej = hipEventRecord(b->join[i], b->ctx[i]->stream);
+                const hipError_t ew = ej == hipSuccess ? hipStreamWaitEvent(b->origin, b->join[i], 0) : ej;
+                if (e == hipSuccess) e = ew;
+            }
+            const hipError_t ee = hipStreamEndCapture(b->origin, &graph);
+            if (e == hipSuccess) e = ee;
+            if (e == hipSuccess) e = hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0);
+            if (graph) (void)hipGraphDestroy(graph);
             if (e != hipSuccess) {
-                b->err = std::string("hipGraphInstantiate: ") + hipGetErrorString(e);
+                b->exec = nullptr;
+                b->err = std::string("graph capture of the batch: ") + hipGetErrorString(e);
                 return GLINT_ERR_HIP;
             }
         }

@@ -94,7 +94,21 @@ struct Counters {
     uint32_t n_nodes;
     uint32_t last_splits;
     uint32_t last_merges;
-    // running totals over ticks (glint_totals)
+    uint32_t n_drop_leaves;         // leaves over 2*cap this tick (the reference's leaf storage, filter.cpp:100-104)
+    uint32_t sorted_valid;          // spid holds a complete leaf order (after the first scatter)
+    uint32_t tab_bits;              // log2(regions) of the last tick's table
+    uint32_t pad0_;
+    unsigned long long n_drop_objs; // objects beyond 2*cap in those leaves
+    unsigned long long n_tests;     // candidate pairs the join tested (the reference's distance tests)
+    unsigned long long n_evals;     // pair slots the join evaluated (64 per column of a wave task)
+    unsigned long long n_kvlog;     // meetings logged for the key-value output since its last flush
+    unsigned long long n_mlog;      // meetings ever appended to the meeting log
+    // grid rebuild (GLINT_SCHEMA_GRID_REBUILD)
+    uint32_t g_nodes;               // nodes handed out this tick
+    uint32_t g_heavy;               // coarse blocks over capacity this tick
+    uint32_t g_list[16];            // branch blocks per depth
+    // running totals over ticks (glint_totals); glint_totals_get resets from
+    // acc_ticks to the end of the struct, so nothing else may follow them
     unsigned long long acc_ticks;
     unsigned long long acc_contacts;
     unsigned long long acc_contacts_border;
@@ -108,14 +122,13 @@ struct Counters {
     unsigned long long max_meet;
     unsigned long long acc_stored;
     unsigned long long max_stored;
-    uint32_t sorted_valid;          // spid holds a complete leaf order (after the first scatter)
+    unsigned long long acc_drop_leaves;
+    unsigned long long acc_drop_objs;
+    unsigned long long max_drop_objs;
+    unsigned long long acc_tests;
+    unsigned long long acc_evals;
+    uint32_t max_leaf_run;          // largest leaf over the ticks
     uint32_t pad_;
-    unsigned long long n_kvlog;     // meetings logged for the key-value output since its last flush
-    unsigned long long n_mlog;      // meetings ever appended to the meeting log
-    // grid rebuild (GLINT_SCHEMA_GRID_REBUILD)
-    uint32_t g_nodes;               // nodes handed out this tick
-    uint32_t g_heavy;               // coarse blocks over capacity this tick
-    uint32_t g_list[16];            // branch blocks per depth
 };
 
 // Per-tick values, kept in device memory so a tick's launches can be
@@ -131,7 +144,9 @@ struct KvLogEntry {
 // One generation of the meeting table: the contacts of one tick grouped by
 // table region, keys[i] = (pid_a << 32) | pid_b with the kTag flags;
 // starts[i] = first tick of the pair's run (kDupStart: a second copy,
-// ignored); region r holds entries rstart[r] .. rstart[r+1].
+// ignored); with 2^b regions at that tick, region r holds entries
+// rstart[r] .. rstart[r+1]; rstart[rs_total] is the entry count and
+// rstart[rs_total + 1] is b.
 struct TableGen {
     unsigned long long *keys;
     uint32_t *starts;
@@ -183,7 +198,6 @@ struct DevState {
     float t32_out;
     double w32_max;     // widest leaf decided in FP32 (error bound inside the band)
     double pf_lim;      // largest FP32 offset with the same error bound (w32_max + r)
-    unsigned long long tab_mask;
     unsigned long long unit_cap;
     unsigned long long meet_cap;
     unsigned long long clist_cap;
@@ -235,9 +249,11 @@ struct DevState {
     unsigned long long *clist2;    // bucketing buffer (the join's output when there is one pass)
     uint32_t *bcount;              // coarse-bin counts per bucketing block
     uint32_t *boff;                // their exclusive offsets
-    uint32_t region_bits;          // log2(regions)
-    uint32_t lo_bits;              // region bits split by the second pass
-    uint32_t coarse_bins;          // bins of the first pass
+    uint32_t rbits_min;            // log2(regions) of the configured table: the fewest a tick uses
+    uint32_t rbits_max;            // the most a tick can need (its stored contacts are below clist_cap)
+    uint32_t coarse_max;           // first-pass bins at rbits_max
+    uint32_t rs_total;             // 2^rbits_max: rstart index of a generation's entry count (its
+                                   // region bits follow at rs_total + 1)
     uint32_t bucket_blocks;        // blocks of the first pass
     KvLogEntry *kvlog;             // key-value output log (null when off)
     float2 *pos32;                 // float positions of the tick, pid order (key-value output)
@@ -279,6 +295,17 @@ __device__ __forceinline__ unsigned long long wave_append_u64(unsigned long long
     if (lane == leader) base = atomicAdd(counter, (unsigned long long)__popcll(mask));
     base = __shfl(base, leader);
     return base + __popcll(mask & ((1ull << lane) - 1ull));
+}
+
+// Sum over the wave's lanes (every lane gets it).
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
+{
+    for (int d = 32; d >= 1; d >>= 1) {
+        const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)v, d);
+        const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(v >> 32), d);
+        v += ((unsigned long long)hi << 32) | lo;
+    }
+    return v;
 }
 
 __device__ __forceinline__ uint32_t wave_append_u32(uint32_t *counter, bool pred)

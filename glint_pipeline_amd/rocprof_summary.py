@@ -88,14 +88,60 @@ def summarize(d: str, kernel: str | None = None, skip: int = 0, take: int | None
     return "\n".join(out)
 
 
+def _counter_avgs(d: str, counter: str, skip: int, take: int | None) -> dict:
+    """{kernel: average of `counter` per dispatch} over the selected dispatches."""
+    acc = defaultdict(list)
+    for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+        with open(f) as fh:
+            rows = list(csv.DictReader(fh))
+        if rows and "Dispatch_Id" in rows[0]:
+            rows.sort(key=lambda r: int(r["Dispatch_Id"]))
+        for r in rows:
+            if r.get("Counter_Name") == counter:
+                acc[_short(r.get("Kernel_Name", "?"))].append(float(r.get("Counter_Value", 0)))
+    return {k: sum(v) / len(v) for k, v in ((k, _select(v, skip, take)) for k, v in acc.items()) if v}
+
+
+def traffic(fetch_dir: str, write_dir: str, copy_fetch_dir: str, copy_write_dir: str, copy_bytes: int,
+            skip: int, take: int | None) -> str:
+    """HBM bytes per launch of every kernel: FETCH_SIZE and WRITE_SIZE (KiB
+    per dispatch) scaled by what the same counters read for a plain copy of
+    copy_bytes bytes (which reads and writes copy_bytes each).  The last
+    lines are machine-readable: TRAFFIC_BYTES_PER_LAUNCH <kernel> <bytes>
+    and SOURCE_SHA <digest of the kernel sources measured>."""
+    from .build import source_digest
+    fetch = _counter_avgs(fetch_dir, "FETCH_SIZE", skip, take)
+    write = _counter_avgs(write_dir, "WRITE_SIZE", skip, take)
+    cf = _counter_avgs(copy_fetch_dir, "FETCH_SIZE", 0, None).get("k_copy")
+    cw = _counter_avgs(copy_write_dir, "WRITE_SIZE", 0, None).get("k_copy")
+    if not cf or not cw:
+        raise SystemExit("no k_copy calibration counters found")
+    ff, fw = copy_bytes / 1024.0 / cf, copy_bytes / 1024.0 / cw
+    out = [f"# calibration: a {copy_bytes} B copy reads FETCH_SIZE {cf:.1f} KiB and WRITE_SIZE {cw:.1f} KiB:",
+           f"# bytes = FETCH_SIZE * {ff:.3f} * 1024 + WRITE_SIZE * {fw:.3f} * 1024",
+           f"{'kernel':28s} {'FETCH_KiB':>14s} {'WRITE_KiB':>14s} {'read_GB':>10s} {'written_GB':>10s} {'total_GB':>10s}"]
+    lines = []
+    for k in sorted(set(fetch) | set(write), key=lambda k: -(fetch.get(k, 0) * ff + write.get(k, 0) * fw)):
+        rb, wb = fetch.get(k, 0.0) * ff * 1024, write.get(k, 0.0) * fw * 1024
+        out.append(f"{k:28s} {fetch.get(k, 0):14.1f} {write.get(k, 0):14.1f} {rb / 1e9:10.4f} {wb / 1e9:10.4f} "
+                   f"{(rb + wb) / 1e9:10.4f}")
+        lines.append(f"TRAFFIC_BYTES_PER_LAUNCH {k.split('<')[0]} {rb + wb:.0f}")
+    return "\n".join(out + [""] + lines + [f"SOURCE_SHA {source_digest()}"])
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("dir")
     ap.add_argument("--kernel", default=None)
     ap.add_argument("--skip", type=int, default=0, help="dispatches of each kernel to leave out first")
     ap.add_argument("--take", type=int, default=None, help="dispatches of each kernel to average after those")
+    ap.add_argument("--traffic", nargs=4, metavar=("WRITE_DIR", "COPY_FETCH_DIR", "COPY_WRITE_DIR", "COPY_BYTES"),
+                    help="dir holds a FETCH_SIZE run: per-kernel HBM bytes per launch, calibrated by a copy")
     a = ap.parse_args()
-    print(summarize(a.dir, a.kernel, a.skip, a.take))
+    if a.traffic:
+        print(traffic(a.dir, a.traffic[0], a.traffic[1], a.traffic[2], int(a.traffic[3]), a.skip, a.take))
+    else:
+        print(summarize(a.dir, a.kernel, a.skip, a.take))
 
 
 if __name__ == "__main__":

@@ -92,13 +92,16 @@ class TickStats(ctypes.Structure):
         ("num_meetings", ctypes.c_uint64), ("num_rechecks", ctypes.c_uint64), ("table_slots", ctypes.c_uint64),
         ("ms_partition", ctypes.c_float), ("ms_lookup", ctypes.c_float), ("ms_refine", ctypes.c_float),
         ("ms_evaluate", ctypes.c_float), ("ms_schema", ctypes.c_float), ("num_stored", ctypes.c_uint64),
+        ("ref_drop_objects", ctypes.c_uint64), ("ref_drop_leaves", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
+        ("num_tests", ctypes.c_uint64), ("num_evaluated", ctypes.c_uint64),
     ]
 
 
 class Totals(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint64) for n in (
         "ticks", "contacts", "contacts_border", "distinct_pairs", "meetings", "units", "border_objects",
-        "rechecks", "max_contacts_tick", "max_distinct_tick", "max_meetings_tick", "stored", "max_stored_tick")]
+        "rechecks", "max_contacts_tick", "max_distinct_tick", "max_meetings_tick", "stored", "max_stored_tick",
+        "ref_drop_leaves", "ref_drop_objects", "max_ref_drop_objects_tick", "max_leaf_count", "tests", "evaluated")]
 
 
 class ServerConfig(ctypes.Structure):

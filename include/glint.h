@@ -124,6 +124,16 @@ typedef struct glint_tick_stats {
     uint64_t table_slots;
     float ms_partition, ms_lookup, ms_refine, ms_evaluate, ms_schema;  /* HIP event times */
     uint64_t num_stored;           /* contacts kept in the meeting table's generation */
+    uint64_t ref_drop_objects;     /* objects beyond 2*grid_capacity in their leaf: the reference
+                                      stores at most that many per leaf and drops the rest in
+                                      thread order (workbench.cpp:36, filter.cpp:100-104) */
+    uint32_t ref_drop_leaves;      /* leaves holding more than 2*grid_capacity objects */
+    uint32_t reserved;
+    uint64_t num_tests;            /* candidate pairs tested: every pair of a leaf once and every
+                                      border unit against its leaf's objects (refinement.cpp:28-33) */
+    uint64_t num_evaluated;        /* pair slots the join computed a distance for: 64 per column of a
+                                      64-row task, after skipping column groups whose bounding box is
+                                      out of reach of the rows' box (those pairs are misses) */
 } glint_tick_stats;
 
 typedef struct glint_ctx glint_ctx;
@@ -237,6 +247,12 @@ typedef struct glint_totals {
     uint64_t max_meetings_tick;
     uint64_t stored;              /* contacts kept in the meeting table, summed over ticks */
     uint64_t max_stored_tick;
+    uint64_t ref_drop_leaves;     /* leaves over 2*grid_capacity objects, summed over ticks */
+    uint64_t ref_drop_objects;    /* objects beyond 2*grid_capacity in those leaves, summed */
+    uint64_t max_ref_drop_objects_tick;
+    uint64_t max_leaf_count;      /* largest leaf population of any tick */
+    uint64_t tests;               /* candidate pairs tested, summed over ticks */
+    uint64_t evaluated;           /* pair slots computed, summed over ticks */
 } glint_totals;
 glint_status glint_totals_get(glint_ctx *ctx, glint_totals *out, int reset);
 
@@ -275,7 +291,8 @@ const char *glint_batch_last_error(const glint_batch *batch);
 /* nticks ticks of every scenario: scenario i reads tick k at
  * d_traces[i] + 2*N_i*k (device memory, FP64 AoS).  tick_ms (optional,
  * n*nticks floats): tick_ms[i*nticks + k] is the device wall time of
- * scenario i's tick k inside the batch. */
+ * scenario i's tick k inside the batch.  GLINT_ERR_STATE if kernel
+ * profiling is on for a scenario (its events would serialise the batch). */
 glint_status glint_batch_run_ticks(glint_batch *batch, const double *const *d_traces, uint32_t first_tick,
                                    uint32_t nticks, float *tick_ms);
 

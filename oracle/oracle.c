@@ -801,12 +801,15 @@ int oracle_tick(void *h, const double *xy, uint32_t tick)
     double t3 = omp_get_wtime();
 
     /* ---- stats, then schema update (process.cpp:608-612) */
-    uint64_t nleaves = 0, maxc = 0, drops = 0;
+    uint64_t nleaves = 0, maxc = 0, drops = 0, drop_objs = 0;
     for (uint32_t n = 0; n < c->nnodes; n++) {
         if (c->nodes[n].type != T_LEAF) continue;
         nleaves++;
         if (c->count[n] > maxc) maxc = c->count[n];
-        if (c->count[n] > 2ull * c->cfg.grid_capacity) drops++;
+        if (c->count[n] > 2ull * c->cfg.grid_capacity) {
+            drops++;
+            drop_objs += c->count[n] - 2ull * c->cfg.grid_capacity;
+        }
     }
     c->st.num_leaves = nleaves;
     c->st.num_nodes = c->nnodes - c->nfree;
@@ -818,6 +821,7 @@ int oracle_tick(void *h, const double *xy, uint32_t tick)
     c->st.num_meetings = c->nmeet;
     c->st.max_leaf_count = maxc;
     c->st.ref_drop_leaves = drops;
+    c->st.ref_drop_objects = drop_objs;
     if (c->cfg.dynamic_schema == ORACLE_SCHEMA_DYNAMIC) update_schema(c);
     double t4 = omp_get_wtime();
     c->st.t_filter = t1 - t0;

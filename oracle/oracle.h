@@ -65,6 +65,7 @@ typedef struct oracle_stats {
     uint64_t max_leaf_count;      /* largest leaf population this tick */
     uint64_t ref_drop_leaves;     /* leaves over 2*grid_capacity (reference would drop) */
     uint64_t splits, merges;      /* schema changes applied after this tick */
+    uint64_t ref_drop_objects;    /* objects beyond 2*grid_capacity in those leaves */
     double t_filter, t_refine, t_evaluate, t_schema;  /* seconds */
 } oracle_stats;
 

@@ -38,7 +38,8 @@ class OracleConfig(ctypes.Structure):
 class OracleStats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint64) for n in (
         "num_leaves", "num_nodes", "num_border", "num_units", "num_tests", "num_contacts",
-        "num_active", "num_meetings", "max_leaf_count", "ref_drop_leaves", "splits", "merges")] + [
+        "num_active", "num_meetings", "max_leaf_count", "ref_drop_leaves", "splits", "merges",
+        "ref_drop_objects")] + [
         (n, ctypes.c_double) for n in ("t_filter", "t_refine", "t_evaluate", "t_schema")]
 
 

@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 from glint_pipeline_amd import gen, runtime
-from tests.parity import run_parity
+from tests.parity import run_parity, run_ticks_parity
 
 pytestmark = pytest.mark.gpu
 
@@ -93,49 +93,50 @@ def test_parity_without_recorded_contacts(min_meet):
     assert tot["meetings"] > 100
 
 
-def test_guard_band_rechecks_only_inside_band():
-    w = gen.uniform_city(5000, 10, 600.0, seed=43, grid_capacity=32)
-    g = gen.Generator(w)
-    cfg = runtime.make_config(w.num_objects, w.box(), grid_capacity=w.grid_capacity,
-                              precision=runtime.PRECISION_FP32_GUARD, guard_epsilon=1e-3)
+def test_guard_band_rechecks_exactly_the_band():
+    """One leaf (N below capacity) so every pair is tested once, inside its
+    leaf.  Pairs planted at r + delta, |delta| <= 2 eps, and random ones.
+    The FP32 distance of a pair differs from its FP64 distance by less than
+    eps/2 on leaves this small, so every pair within eps/2 of r is re-tested
+    in FP64 and none farther than 3 eps/2 from it: the GPU's recheck count
+    lies between the two brute-force counts.  The contacts equal the
+    oracle's."""
+    from oracle.oracle import Oracle
+    rng = np.random.default_rng(47)
+    r, eps, side = 2.0, 1e-3, 100.0
+    npairs, nrand = 1500, 1000
+    a = rng.uniform(5.0, side - 5.0, (npairs, 2))
+    th = rng.uniform(0.0, 2 * np.pi, npairs)
+    d = r + rng.uniform(-2 * eps, 2 * eps, npairs)
+    b = a + np.stack([d * np.cos(th), d * np.sin(th)], axis=1)
+    xy = np.concatenate([a, b, rng.uniform(0.0, side, (nrand, 2))])
+    n = len(xy)
+    dx = xy[:, None, 0] - xy[None, :, 0]
+    dy = xy[:, None, 1] - xy[None, :, 1]
+    dist = np.sqrt(dx * dx + dy * dy)[np.triu_indices(n, 1)]
+    must = int(np.count_nonzero(np.abs(dist - r) <= eps / 2))
+    may = int(np.count_nonzero(np.abs(dist - r) < 1.5 * eps))
+    assert must > 300
+    box = (0.0, 0.0, side, side)
+    o = Oracle(n, box, r, 10 * n, 5)
+    o.tick(xy.reshape(-1), 0)
+    cfg = runtime.make_config(n, box, reach_distance=r, grid_capacity=10 * n, record_contacts=True,
+                              precision=runtime.PRECISION_FP32_GUARD, guard_epsilon=eps)
     with runtime.Workbench(cfg) as wb:
-        for t in range(w.ticks):
-            xy = g.positions() if t == 0 else g.step()
-            wb.tick(xy, t)
-        tot = wb.totals()
-    # rechecks are pairs whose FP32 distance lies within 1 mm of the 2 m reach
-    assert tot["rechecks"] < tot["contacts"]
+        wb.tick(xy.reshape(-1), 0)
+        st = wb.stats()
+        assert st["num_leaves"] == 1
+        assert must <= st["num_rechecks"] <= may, (must, st["num_rechecks"], may)
+        assert np.array_equal(wb.contacts(), o.contacts())
+    o.close()
 
 
 @pytest.mark.parametrize("graph_ticks", [0, 4])
 def test_run_ticks_device_trace_matches_oracle(graph_ticks):
     """Many ticks from a device-resident trace in one call (no host sync per
-    tick; with graph_ticks, replays of one captured hipGraph) give the same
-    active pairs and meeting totals as the oracle."""
-    from oracle.oracle import Oracle
+    tick; with graph_ticks, replays of one captured hipGraph): every tick's
+    meetings (through the meeting log), the active pairs and the leaves equal
+    the oracle's."""
     w = gen.uniform_city(20_000, 40, 1000.0, seed=51, grid_capacity=32)
-    g = gen.Generator(w)
-    dw = runtime.DeviceWalkers(g)
-    trace = runtime.DeviceBuffer(16 * w.num_objects * w.ticks)
-    trace.upload(g.positions(), 0)
-    dw.walk(w.ticks - 1, trace, 16 * w.num_objects)
-    o = Oracle(w.num_objects, w.box(), w.reach_distance, w.grid_capacity, w.min_meet_time)
-    om = 0
-    for t in range(w.ticks):
-        o.tick(g.positions() if t == 0 else g.step(), t)
-        om += len(o.meetings())
-    cfg = runtime.make_config(w.num_objects, w.box(), grid_capacity=w.grid_capacity, graph_ticks=graph_ticks)
-    with runtime.Workbench(cfg) as wb:
-        ms = wb.run_ticks(trace.address(0), 0, 15)
-        ms2 = wb.run_ticks(trace.address(16 * w.num_objects * 15), 15, w.ticks - 15)
-        assert len(ms) == 15 and len(ms2) == w.ticks - 15 and (ms > 0).all()
-        tot = wb.totals()
-        assert tot["ticks"] == w.ticks and tot["meetings"] == om
-        k, s = wb.active_pairs()
-        ok, os_ = o.active()
-        keep = os_ != w.ticks - 1   # pairs first seen at the last tick are not kept
-        assert len(k) > 100 and np.array_equal(k, ok[keep]) and np.array_equal(s, os_[keep])
-        assert wb.stats()["num_active"] == len(ok)
-        b, lv, _ = wb.leaves()
-        ob, olv, _ = o.leaves()
-        assert np.array_equal(b, ob) and np.array_equal(lv, olv)
+    res = run_ticks_parity(w, [15, 25], precision=runtime.PRECISION_FP64, graph_ticks=graph_ticks)
+    assert res["meetings"] > 100

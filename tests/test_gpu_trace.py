@@ -6,6 +6,7 @@ import numpy as np
 import pytest
 
 from glint_pipeline_amd import gen, runtime
+from tests.parity import run_ticks_parity
 
 pytestmark = pytest.mark.gpu
 
@@ -27,7 +28,6 @@ def test_device_projection_is_bit_exact():
 
 
 def test_reference_trace_file_end_to_end(tmp_path):
-    from oracle.oracle import Oracle
     w = gen.uniform_city(6000, 30, 600.0, seed=71, grid_capacity=32)
     g = gen.Generator(w)
     # positions() and step() return the generator's own buffer: copy each tick
@@ -47,22 +47,12 @@ def test_reference_trace_file_end_to_end(tmp_path):
     assert np.array_equal(tr, lonlat)
     metres = np.stack([(tr[..., 0] - p.lon0) / p.deg_per_m_lon, (tr[..., 1] - p.lat0) / p.deg_per_m_lat], axis=-1)
     box = (-1.0, -1.0, w.side + 1.0, w.side + 1.0)
-    o = Oracle(w.num_objects, box, w.reach_distance, w.grid_capacity, w.min_meet_time)
-    om = 0
-    for t in range(w.ticks):
-        o.tick(np.ascontiguousarray(metres[t]).reshape(-1), t)
-        om += len(o.meetings())
     buf = runtime.DeviceBuffer(tr.nbytes)
     buf.upload(tr)
     runtime.project_device(p, buf.address(), buf.address(), w.num_objects * w.ticks)
-    cfg = runtime.make_config(w.num_objects, box, grid_capacity=w.grid_capacity, min_meet_time=w.min_meet_time)
-    with runtime.Workbench(cfg) as wb:
-        wb.run_ticks(buf.address(), 0, w.ticks)
-        tot = wb.totals()
-        k, s = wb.active_pairs()
+    # every tick's meetings, the active pairs and the leaves of the projected
+    # file equal the oracle's on the host projection of the same file
+    res = run_ticks_parity(w, [w.ticks], box=box, precision=runtime.PRECISION_FP64,
+                           trace=(buf, lambda t: np.ascontiguousarray(metres[t]).reshape(-1)))
     buf.free()
-    ok, os_ = o.active()
-    o.close()
-    keep = os_ != w.ticks - 1
-    assert om > 20 and tot["meetings"] == om
-    assert np.array_equal(k, ok[keep]) and np.array_equal(s, os_[keep])
+    assert res["meetings"] > 20
