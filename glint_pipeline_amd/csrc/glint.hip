@@ -21,6 +21,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -3182,6 +3183,59 @@ void fill_stats(glint_ctx *c)
 
 }  // namespace
 
+// ============================================== allocation-free tick runs
+//
+// glint_run_ticks_noalloc keeps its per-tick clock stamps and their
+// conversion to milliseconds in this file-scope device array instead of a
+// buffer the library allocates.  Every GPU holds its own copy until the
+// process ends; all calls on one GPU share it, so calls that overlap on one
+// GPU race (each call writes every entry it reads, nothing needs clearing).
+// Host code never passes the array's name to a kernel or copy: its device
+// address on each GPU is resolved once with hipGetSymbolAddress (at context
+// creation, with that device current) and that pointer is used.
+constexpr uint32_t kNoAllocMaxTicks = 4096;
+constexpr uint32_t kNoAllocWords = 2 * kNoAllocMaxTicks + kNoAllocMaxTicks / 2;   // stamps, then floats
+static __device__ __align__(256) unsigned long long scratch[kNoAllocWords];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+unsigned long long *g_scratch[kMaxDevices] = {};   // scratch's address on each device (null: not resolved)
+std::mutex g_scratch_mu;
+
+// Resolves scratch's address on `device`, which must be current.
+glint_status resolve_scratch(glint_ctx *c, int device)
+{
+    if (device < 0 || device >= kMaxDevices) {
+        c->err = "device ordinal beyond the scratch table";
+        return GLINT_ERR_INVALID_ARGUMENT;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch[device]) return GLINT_OK;
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+        return hip_fail(c, e != hipSuccess ? e : hipErrorInvalidSymbol, "hipGetSymbolAddress(scratch)");
+    g_scratch[device] = static_cast<unsigned long long *>(ptr);
+    return GLINT_OK;
+}
+
+unsigned long long *scratch_on(int device)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return device >= 0 && device < kMaxDevices ? g_scratch[device] : nullptr;
+}
+
+// Tick k's latency in milliseconds from its two stamps.
+__global__ __launch_bounds__(kBlock) void k_stamps_to_ms(const unsigned long long *stamps, float *ms, uint32_t n,
+                                                         double khz)
+{
+    for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < n; k += gridDim.x * kBlock)
+        ms[k] = (float)((double)(stamps[2 * k + 1] - stamps[2 * k]) / khz);
+}
+
+}  // namespace
+
 // ====================================================================== C ABI
 
 extern "C" {
@@ -3271,6 +3325,10 @@ glint_status glint_create(const glint_config *cfg, glint_ctx **out)
     if (e != hipSuccess) {
         c->err = std::string("hipStreamCreate: ") + hipGetErrorString(e);
         return fail(GLINT_ERR_HIP);
+    }
+    {
+        const glint_status sst = resolve_scratch(c, c->device);
+        if (sst) return fail(sst);
     }
     for (auto &ev : c->ev) {
         if (hipEventCreate(&ev) != hipSuccess) {
@@ -3709,7 +3767,10 @@ static void enqueue_tick_body(glint_ctx *c, uint64_t stride)
 
 // Checks, stamp buffer, the pre-run state (k_begin_run) and, on the first
 // run, the tree build from the first snapshot.
-static glint_status run_begin(glint_ctx *c, const double *d_trace, uint32_t first_tick, uint32_t nticks)
+// stamps: device buffer of 2*nticks stamps, or null for the context's own
+// (allocated, or grown, here).
+static glint_status run_begin(glint_ctx *c, const double *d_trace, uint32_t first_tick, uint32_t nticks,
+                              unsigned long long *stamps = nullptr)
 {
     if (c->has_positions && !c->evaluated) {
         c->err = "previous tick was not evaluated";
@@ -3720,20 +3781,21 @@ static glint_status run_begin(glint_ctx *c, const double *d_trace, uint32_t firs
         return GLINT_ERR_STATE;
     }
     HIP_TRY(c, hipSetDevice(c->device));
-    if (c->stamp_cap < 2 * nticks) {
+    if (!stamps && c->stamp_cap < 2 * nticks) {
         if (c->stamps) HIP_TRY(c, hipFree(c->stamps));
         c->stamps = nullptr;
         c->stamp_cap = 0;
         HIP_TRY(c, hipMalloc((void **)&c->stamps, (size_t)2 * nticks * sizeof(unsigned long long)));
         c->stamp_cap = 2 * nticks;
     }
+    if (!stamps) stamps = c->stamps;
     c->tick = first_tick;
     const int g = c->cur_gen;
     // one advance short of the first tick: tables and histories pre-swapped
     k_begin_run<<<1, 1, 0, c->stream>>>(c->s.tp, d_trace, c->tab[g], c->tab[1 - g], c->mbr[g], c->mbr[1 - g],
                                         c->hist[g], c->hist[1 - g], c->cand[g], c->cand[1 - g], c->s.sxb[g],
                                         c->s.syb[g], c->s.sxb[1 - g], c->s.syb[1 - g], c->s.sleafb[g],
-                                        c->s.sleafb[1 - g], c->prev_tick, first_tick, c->has_prev ? 1u : 0u, c->stamps, 2 * nticks,
+                                        c->s.sleafb[1 - g], c->prev_tick, first_tick, c->has_prev ? 1u : 0u, stamps, 2 * nticks,
                                         nticks);
     HIP_TRY(c, hipGetLastError());
     if (!c->built) {
@@ -3802,6 +3864,43 @@ glint_status glint_run_ticks(glint_ctx *c, const double *d_trace, uint32_t first
     HIP_TRY(c, hipGetLastError());
     run_end_state(c, first_tick, nticks);
     return run_finish(c, nticks, tick_ms);
+}
+
+glint_status glint_run_ticks_noalloc(glint_ctx *c, const double *d_trace, uint32_t first_tick, uint32_t nticks,
+                                     float *tick_ms)
+{
+    if (!c || !d_trace || nticks == 0) return GLINT_ERR_INVALID_ARGUMENT;
+    if (nticks > kNoAllocMaxTicks) {
+        c->err = "glint_run_ticks_noalloc runs at most " + std::to_string(kNoAllocMaxTicks) +
+                 " ticks per call (its stamps live in a fixed device array)";
+        return GLINT_ERR_INVALID_ARGUMENT;
+    }
+    if (c->prof.on) {
+        c->err = "glint_run_ticks_noalloc: kernel profiling creates events; turn it off first";
+        return GLINT_ERR_STATE;
+    }
+    unsigned long long *stamps = scratch_on(c->device);   // resolved when the context was created
+    if (!stamps) {
+        c->err = "scratch array not resolved on this device";
+        return GLINT_ERR_STATE;
+    }
+    float *ms = reinterpret_cast<float *>(stamps + 2 * kNoAllocMaxTicks);
+    glint_status st = run_begin(c, d_trace, first_tick, nticks, stamps);
+    if (st) return st;
+    const uint64_t stride = 2ull * c->s.n;
+    for (uint32_t k = 0; k < nticks; k++) enqueue_tick_body(c, stride);   // plain launches, no graph
+    HIP_TRY(c, hipGetLastError());
+    run_end_state(c, first_tick, nticks);
+    if (tick_ms) {
+        const double khz = c->wall_khz > 0 ? c->wall_khz : 100000.0;
+        k_stamps_to_ms<<<grid_for(nticks), kBlock, 0, c->stream>>>(stamps, ms, nticks, khz);
+        HIP_TRY(c, hipGetLastError());
+        HIP_TRY(c, hipMemcpyAsync(tick_ms, ms, (size_t)nticks * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    }
+    st = read_counters(c);   // synchronises the stream
+    if (st) return st;
+    fill_stats(c);
+    return status_from_flags(c, c->h_ctr->error);
 }
 
 glint_status glint_synchronize(glint_ctx *c)

@@ -164,6 +164,7 @@ SIGNATURES = {
     "glint_update_schema": (_I, [_VP]),
     "glint_process_tick": (_I, [_VP, _VP, _U32]),
     "glint_run_ticks": (_I, [_VP, _VP, _U32, _U32, _VP]),
+    "glint_run_ticks_noalloc": (_I, [_VP, _VP, _U32, _U32, _VP]),
     "glint_synchronize": (_I, [_VP]),
     "glint_stats": (_I, [_VP, ctypes.POINTER(TickStats)]),
     "glint_contacts": (_I, [_VP, _VP, _U64, ctypes.POINTER(_U64)]),

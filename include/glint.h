@@ -133,7 +133,9 @@ typedef struct glint_tick_stats {
                                       border unit against its leaf's objects (refinement.cpp:28-33) */
     uint64_t num_evaluated;        /* pair slots the join computed a distance for: 64 per column of a
                                       64-row task, after skipping column groups whose bounding box is
-                                      out of reach of the rows' box (those pairs are misses) */
+                                      out of reach of the rows' box (those pairs are misses); a work
+                                      count that varies with the order objects land in the join's
+                                      tiles, unlike every result */
 } glint_tick_stats;
 
 typedef struct glint_ctx glint_ctx;
@@ -206,6 +208,17 @@ glint_status glint_process_tick(glint_ctx *ctx, const double *xy, uint32_t tick)
  * per call (process.cpp:431). */
 glint_status glint_run_ticks(glint_ctx *ctx, const double *d_trace, uint32_t first_tick,
                              uint32_t nticks, float *tick_ms);
+
+/* glint_run_ticks for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * allocation): it has no counterpart in the reference.  The same ticks and
+ * results as glint_run_ticks, with plain launches (no graph is captured
+ * or instantiated), and its per-tick stamps kept in a fixed device array
+ * of the library, so at most 4096 ticks per call (GLINT_ERR_INVALID_ARGUMENT
+ * beyond).  That array is shared by every call on one GPU: calls that
+ * overlap on one GPU race.  GLINT_ERR_STATE while kernel profiling is on. */
+glint_status glint_run_ticks_noalloc(glint_ctx *ctx, const double *d_trace, uint32_t first_tick,
+                                     uint32_t nticks, float *tick_ms);
 
 /* Waits for the context's stream and reports device-side errors. */
 glint_status glint_synchronize(glint_ctx *ctx);
