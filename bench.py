@@ -112,6 +112,31 @@ def profiled_traffic(kernel: str, workload: str, precision: int, objects: int) -
     return found, f"profiles/{tag} (FETCH_SIZE + WRITE_SIZE, copy-calibrated; sources {cur})"
 
 
+VALU_PEAK = 256 * 2.4e9   # wave64 VALU instructions per second: one per CU and cycle, 256 CUs at 2.4 GHz
+
+
+def profiled_valu(kernel: str, workload: str, precision: int, objects: int) -> tuple[float | None, str]:
+    """VALU wave instructions per launch of `kernel` from the committed
+    rocprofv3 counter summary (SQ_INSTS_VALU), if it measured the kernel
+    sources built now."""
+    from glint_pipeline_amd.build import source_digest
+    tag = f"valu_{workload}_{objects // 1_000_000}m_{'fp32guard' if precision else 'fp64'}.txt"
+    f = ROOT / "profiles" / tag
+    if not f.exists():
+        return None, f"no counter summary profiles/{tag}"
+    found, sha = None, None
+    for line in f.read_text().splitlines():
+        parts = line.split()
+        if len(parts) == 3 and parts[0] == "VALU_INSTS_PER_LAUNCH" and parts[1] == kernel:
+            found = float(parts[2])
+        if len(parts) == 2 and parts[0] == "SOURCE_SHA":
+            sha = parts[1]
+    cur = source_digest()
+    if sha != cur:
+        return None, f"profiles/{tag} measured kernel sources {sha}, built sources are {cur}"
+    return found, f"profiles/{tag} (SQ_INSTS_VALU; sources {cur})"
+
+
 def cpu_baseline(w, ticks: int, threads: int, min_seconds: float = 10.0, max_ticks: int = 5000,
                  schema: str = "dynamic") -> dict:
     """The CPU oracle (C + OpenMP) on the first ticks of the same workload:
@@ -351,6 +376,17 @@ def main() -> int:
                     "kernel_share_of_tick": round(dms / K / (elapsed * 1e3 / K), 4),
                     "measured": f"HIP events on every launch over ticks {W}..{W + K - 1}, ungraphed run of the same "
                                 f"inputs"}
+        if dom == "k_join":
+            # the join's own bound is instruction issue: VALU wave instructions
+            # per launch (committed counter summary) against the issue peak,
+            # and the candidate pairs it tests per second
+            vi, vnote = profiled_valu(dom, args.workload, precision, N)
+            roofline["compute"] = {
+                "bound": "valu issue", "unit": "wave instructions/s", "peak": VALU_PEAK,
+                "insts_per_launch": vi, "achieved": round(vi / avg_s, 1) if vi else None,
+                "frac": round(vi / avg_s / VALU_PEAK, 4) if vi else None, "source": vnote,
+                "pair_tests_per_s": round(totals["tests"] / max(1, totals["ticks"]) / avg_s, 1),
+                "pair_slots_evaluated_per_s": round(totals["evaluated"] / max(1, totals["ticks"]) / avg_s, 1)}
 
     # ---- the plain FP64 per-tick form on the same inputs: positions copied
     # from host memory every tick, the four stage calls, a host sync per tick

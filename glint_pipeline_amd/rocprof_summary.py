@@ -120,12 +120,34 @@ def traffic(fetch_dir: str, write_dir: str, copy_fetch_dir: str, copy_write_dir:
     out = [f"# calibration: a {copy_bytes} B copy reads FETCH_SIZE {cf:.1f} KiB and WRITE_SIZE {cw:.1f} KiB:",
            f"# bytes = FETCH_SIZE * {ff:.3f} * 1024 + WRITE_SIZE * {fw:.3f} * 1024",
            f"{'kernel':28s} {'FETCH_KiB':>14s} {'WRITE_KiB':>14s} {'read_GB':>10s} {'written_GB':>10s} {'total_GB':>10s}"]
-    lines = []
+    base = defaultdict(float)   # template variants of one kernel summed (each runs once a tick)
     for k in sorted(set(fetch) | set(write), key=lambda k: -(fetch.get(k, 0) * ff + write.get(k, 0) * fw)):
         rb, wb = fetch.get(k, 0.0) * ff * 1024, write.get(k, 0.0) * fw * 1024
         out.append(f"{k:28s} {fetch.get(k, 0):14.1f} {write.get(k, 0):14.1f} {rb / 1e9:10.4f} {wb / 1e9:10.4f} "
                    f"{(rb + wb) / 1e9:10.4f}")
-        lines.append(f"TRAFFIC_BYTES_PER_LAUNCH {k.split('<')[0]} {rb + wb:.0f}")
+        base[k.split('<')[0]] += rb + wb
+    lines = [f"TRAFFIC_BYTES_PER_LAUNCH {k} {v:.0f}" for k, v in sorted(base.items(), key=lambda kv: -kv[1])]
+    return "\n".join(out + [""] + lines + [f"SOURCE_SHA {source_digest()}"])
+
+
+def valu(d: str, skip: int, take: int | None) -> str:
+    """The counter table of a run with SQ_INSTS_VALU among its counters, then
+    machine-readable lines: VALU_INSTS_PER_LAUNCH <kernel> <wave instructions>
+    (template variants summed) and SOURCE_SHA <digest of the kernel sources>."""
+    from .build import source_digest
+    insts = _counter_avgs(d, "SQ_INSTS_VALU", skip, take)
+    names = ["SQ_INSTS_VALU", "SQ_ACTIVE_INST_VALU", "SQ_BUSY_CYCLES", "GRBM_GUI_ACTIVE", "SQ_WAVES"]
+    cols = {n: _counter_avgs(d, n, skip, take) for n in names}
+    sel = f"dispatches {skip}..{skip + take - 1 if take else 'end'} of each kernel"
+    out = [f"# counters per dispatch, averages over {sel} (kernels of at least 10^5 VALU instructions)",
+           f"{'kernel':28s} " + " ".join(f"{n:>20s}" for n in names)]
+    for k, v in sorted(insts.items(), key=lambda kv: -kv[1]):
+        if v >= 1e5:
+            out.append(f"{k:28s} " + " ".join(f"{cols[n].get(k, 0.0):20.1f}" for n in names))
+    base = defaultdict(float)
+    for k, v in insts.items():
+        base[k.split('<')[0]] += v
+    lines = [f"VALU_INSTS_PER_LAUNCH {k} {v:.0f}" for k, v in sorted(base.items(), key=lambda kv: -kv[1]) if v >= 1e5]
     return "\n".join(out + [""] + lines + [f"SOURCE_SHA {source_digest()}"])
 
 
@@ -137,8 +159,11 @@ def main() -> None:
     ap.add_argument("--take", type=int, default=None, help="dispatches of each kernel to average after those")
     ap.add_argument("--traffic", nargs=4, metavar=("WRITE_DIR", "COPY_FETCH_DIR", "COPY_WRITE_DIR", "COPY_BYTES"),
                     help="dir holds a FETCH_SIZE run: per-kernel HBM bytes per launch, calibrated by a copy")
+    ap.add_argument("--valu", action="store_true", help="dir holds an SQ_INSTS_VALU run: VALU instructions per launch")
     a = ap.parse_args()
-    if a.traffic:
+    if a.valu:
+        print(valu(a.dir, a.skip, a.take))
+    elif a.traffic:
         print(traffic(a.dir, a.traffic[0], a.traffic[1], a.traffic[2], int(a.traffic[3]), a.skip, a.take))
     else:
         print(summarize(a.dir, a.kernel, a.skip, a.take))
