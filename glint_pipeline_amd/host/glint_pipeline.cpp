@@ -61,9 +61,15 @@ int main(int argc, char **argv)
         tr.process(ticks, [&](uint32_t tick, const std::vector<glint_meeting> &m) {
             total += m.size();
             glint_tick_stats st = tr.bench().stats();
-            std::printf("tick %u meetings %zu contacts %llu leaves %u partition %.3f refine %.3f evaluate %.3f ms\n",
-                        tick, m.size(), (unsigned long long)st.num_contacts, st.num_leaves, st.ms_partition,
-                        st.ms_refine, st.ms_evaluate);
+            // digest of the tick's meetings in their sorted order: h = (h ^ v) * FNV prime
+            // over every field of every meeting
+            uint64_t h = 1469598103934665603ull;
+            for (const glint_meeting &x : m)
+                for (uint32_t v : {x.pid_a, x.pid_b, x.start, x.end}) h = (h ^ v) * 1099511628211ull;
+            std::printf("tick %u meetings %zu digest %016llx contacts %llu leaves %u partition %.3f refine %.3f "
+                        "evaluate %.3f ms\n",
+                        tick, m.size(), (unsigned long long)h, (unsigned long long)st.num_contacts, st.num_leaves,
+                        st.ms_partition, st.ms_refine, st.ms_evaluate);
         });
         double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         std::printf("%u objects x %u ticks: %llu meetings, %.3f s (%.1f object-updates/s incl. host generator)\n", n,

@@ -14,9 +14,10 @@ def keys_of(meetings: np.ndarray) -> np.ndarray:
 
 def run_parity(w: gen.Workload, ticks: int, *, check_contacts: bool = True, check_leaves_every: int = 10,
                complete: bool = False, precision: int = runtime.PRECISION_FP64,
-               dynamic_schema: int = runtime.SCHEMA_DYNAMIC, box=None, **cfg_kw) -> dict:
+               dynamic_schema: int = runtime.SCHEMA_DYNAMIC, box=None, cfg=None, **cfg_kw) -> dict:
     """Runs both paths for `ticks` ticks and asserts identical per-tick contact
-    sets, meeting sets and leaf boxes.  Returns totals for the caller."""
+    sets, meeting sets and leaf boxes (cfg: a complete configuration to use
+    instead of one made from the arguments).  Returns totals for the caller."""
     g = gen.Generator(w)
     box = box or w.box()
     o = Oracle(w.num_objects, box, w.reach_distance, w.grid_capacity, w.min_meet_time, complete=complete,
@@ -25,7 +26,9 @@ def run_parity(w: gen.Workload, ticks: int, *, check_contacts: bool = True, chec
               complete=complete, precision=precision, record_contacts=check_contacts,
               dynamic_schema=dynamic_schema)
     kw.update(cfg_kw)
-    cfg = runtime.make_config(w.num_objects, box, **kw)
+    if cfg is None:
+        cfg = runtime.make_config(w.num_objects, box, **kw)
+    check_contacts = check_contacts and bool(cfg.record_contacts)
     totals = {"meetings": 0, "contacts": 0}
     with runtime.Workbench(cfg) as wb:
         for t in range(ticks):
@@ -52,7 +55,7 @@ def run_parity(w: gen.Workload, ticks: int, *, check_contacts: bool = True, chec
                 assert np.array_equal(b, ob) and np.array_equal(lv, olv), f"tick {t}: leaf boxes differ"
         k, s = wb.active_pairs()
         ok, os_ = o.active()
-        if not check_contacts and w.min_meet_time != 0:
+        if not cfg.record_contacts and w.min_meet_time != 0:
             # pairs first seen at the last tick are not kept without recorded contacts
             keep = os_ != ticks - 1
             ok, os_ = ok[keep], os_[keep]

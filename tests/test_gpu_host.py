@@ -1,0 +1,49 @@
+"""The C++ host path: glint_pipeline (the reference's pipeline-style driver
+over glint::tracer and glint::workbench, glint_tracer.hpp) run as a child
+process on the GPU; every tick's meetings, as a digest printed by the
+driver, equal the oracle's on the same seeded walk."""
+import re
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from glint_pipeline_amd import gen
+
+pytestmark = pytest.mark.gpu
+
+DRIVER = Path(__file__).resolve().parents[1] / "glint_pipeline_amd" / "host" / "glint_pipeline"
+
+
+def digest(m: np.ndarray) -> int:
+    h = 1469598103934665603
+    for v in m.astype(np.uint64).reshape(-1):
+        h = ((h ^ int(v)) * 1099511628211) & 0xFFFFFFFFFFFFFFFF
+    return h
+
+
+@pytest.mark.parametrize("n,ticks,side,spots,cap,min_meet,reach", [(20_000, 30, 1000.0, 0, 32, 5, 2.0),
+                                                                    (30_000, 20, 3000.0, 8, 64, 3, 3.0)])
+def test_cpp_pipeline_driver_matches_oracle(n, ticks, side, spots, cap, min_meet, reach):
+    from oracle.oracle import Oracle
+    assert DRIVER.exists(), "glint_pipeline is not built; run __graft_entry__.build()"
+    out = subprocess.run([str(DRIVER), str(n), str(ticks), str(side), str(spots), str(cap), str(min_meet), str(reach)],
+                         capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stderr
+    got = {int(t): (int(m), int(d, 16)) for t, m, d in
+           re.findall(r"^tick (\d+) meetings (\d+) digest ([0-9a-f]+)", out.stdout, re.M)}
+    assert sorted(got) == list(range(ticks))
+    # the driver's generator: seed 1, hotspots at 80% with sigma 200 m when asked
+    w = gen.Workload("driver", n, ticks, side, seed=1, num_hotspots=spots, hotspot_fraction=0.8 if spots else 0.0,
+                     hotspot_sigma=200.0, grid_capacity=cap, min_meet_time=min_meet, reach_distance=reach)
+    g = gen.Generator(w)
+    o = Oracle(n, w.box(), reach, cap, min_meet)
+    total = 0
+    for t in range(ticks):
+        o.tick(g.positions() if t == 0 else g.step(), t)
+        m = o.meetings()
+        total += len(m)
+        assert got[t] == (len(m), digest(m)), f"tick {t}: the driver's meetings differ from the oracle's"
+    o.close()
+    assert total > 20
