@@ -183,7 +183,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=96)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="skewed", choices=["skewed", "plain_gpu", "plain_cpu", "batched", "serving"])
-    ap.add_argument("--history", type=int, default=300, help="serving: ticks of history before the timed ticks")
+    ap.add_argument("--history", type=int, default=3600, help="serving: ticks of history before the timed ticks")
     ap.add_argument("--no-pace", action="store_true",
                     help="serving: run ticks back to back with 50 queries per tick (a stress test) instead of "
                          "one tick per wall-clock second with Poisson arrivals at 50/s")
@@ -579,7 +579,7 @@ def bench_serving(args) -> int:
     w.move_fraction = 0.1
     params = autotune.params_for(autotune.size_class(N))
     cfg = autotune.config_for(w, params, distinct_estimate("batched", N), graph_ticks=0,
-                              meeting_log_capacity=1 << 27)
+                              meeting_log_capacity=1 << 29)
     g = gen.Generator(w)
     dw = runtime.DeviceWalkers(g)
     pids, upd, cnt = runtime.DeviceBuffer(4 * N), runtime.DeviceBuffer(16 * N), runtime.DeviceBuffer(4)
@@ -596,8 +596,11 @@ def bench_serving(args) -> int:
         else:
             k = dw.step_updates(pids, upd, cnt)
             sv.tick(t, d_pids=pids.address(), d_xy=upd.address(), k=k)
+        if t % 500 == 499:
+            log(f"[bench] serving: history tick {t + 1}/{H}, {time.perf_counter() - t0:.0f} s, "
+                f"log {wb.meeting_log_info()}, index {sv.index_info()}")
     log(f"[bench] serving: {N} objects, {H} ticks of history in {time.perf_counter() - t0:.1f} s; "
-        f"log {wb.meeting_log_info()}")
+        f"log {wb.meeting_log_info()}; index {sv.index_info()}")
     tick_ms, ids = [], []
     pace = not args.no_pace
 
@@ -636,6 +639,8 @@ def bench_serving(args) -> int:
         retries += info["retries"]
         h1, h2 = sv.result(qid)
         hits += len(h1) + len(h2)
+    index_info = sv.index_info()
+    wb_log = wb.meeting_log_info()
     sv.close()
     wb.close()
     for b in (pids, upd, cnt, full):
@@ -653,8 +658,10 @@ def bench_serving(args) -> int:
         "data": "synthetic (seeded walk of resident objects, 10% moving per tick; Poisson queries)",
         "config": {"workload": f"serving: {N} resident objects, side {w.side:.0f} m, 10% moving per tick, "
                                f"queries Poisson 50/tick, seeds U{{1..10^4}}, windows U[60,3600] ticks "
-                               f"(clipped to {H + K} ticks of history), 20% two-hop",
-                   "params": params, "arena_bytes": 8 << 30, "max_running": 16},
+                               f"({H} ticks of history before the first query), 20% two-hop",
+                   "params": params, "arena_bytes": 8 << 30, "max_running": 16,
+                   "meeting_log": {"capacity": 1 << 29, "logged": wb_log[0], "first_retained": wb_log[1]},
+                   "meeting_index": index_info},
         "queries": {"count": len(ids), "p50_ms": round(float(np.percentile(ql, 50)), 3) if len(ql) else None,
                     "p99_ms": round(float(np.percentile(ql, 99)), 3) if len(ql) else None,
                     "admitted_later": int(waited), "retries": int(retries), "hits": int(hits),

@@ -4305,6 +4305,181 @@ glint_status glint_meeting_log_read(glint_ctx *c, uint64_t from, glint_meeting *
     return GLINT_OK;
 }
 
+}  // extern "C"
+
+// ---------------------------------------------------------- snapshot
+//
+// The state the next tick reads, at a tick boundary: the tree (nodes, their
+// metadata, the free-node stack, parent links, leaf counts), the last tick's
+// objects in leaf order (ids, leaves, positions of the last two ticks), the
+// candidate lists of the last two ticks, the position histories, the last
+// tick's generation of the meeting table (its
+// stored contacts, start ticks and region offsets; with the key-value output
+// their boxes), the counters, the meeting log and the key-value log.  The
+// reference dumps its meeting buckets and metadata to disk and loads them
+// back into the table (workbench.cpp:446-470,574-595; geometry.cu:543-565).
+
+namespace {
+
+struct SnapHeader {
+    uint64_t magic, bytes;
+    uint32_t n, node_cap, rs_total, precision, store_all, kv, dynamic_schema, cur_gen;
+    uint32_t built, has_prev, tick, prev_tick, kv_end_time_min, nsections;
+    uint64_t clist_cap, mlog_cap, kvlog_cap, gen_count, mlog_count, kvlog_count;
+};
+constexpr uint64_t kSnapMagic = 0x31746e696c67ull;   // "glint1"
+
+struct Section {
+    void *dev;
+    uint64_t bytes;
+};
+
+// The device ranges of a snapshot, in order (gen, mlog and kvlog counts as given).
+std::vector<Section> state_sections(glint_ctx *c, uint64_t gen_count, uint64_t mlog_count, uint64_t kvlog_count)
+{
+    DevState &s = c->s;
+    const uint64_t N = s.n, NC = s.node_cap;
+    const int g = 1 - c->cur_gen;   // after evaluate the last tick's pairs are in the other generation
+    std::vector<Section> v = {
+        {s.nodes, NC * sizeof(Node)},      {s.meta, NC * sizeof(NodeMeta)}, {s.free_stack, NC * 4},
+        {s.parent, NC * 4},                {s.count, NC * 4},               {s.spid, N * 4},
+        {s.sleafb[0], N * 4},              {s.sleafb[1], N * 4},            {s.sxb[0], N * 8},
+        {s.sxb[1], N * 8},                 {s.syb[0], N * 8},               {s.syb[1], N * 8},
+        {c->hist[0], N * 16},              {c->hist[1], N * 16},            {s.ctr, sizeof(Counters)},
+        {c->tab[g].keys, gen_count * 8},   {c->tab[g].starts, gen_count * 4},
+        {c->tab[g].rstart, ((uint64_t)s.rs_total + 2) * 4},
+    };
+    if (s.cand) {
+        v.push_back({s.cand, N * sizeof(uint4)});
+        v.push_back({s.prev_cand, N * sizeof(uint4)});
+    }
+    if (c->kv) {
+        v.push_back({c->mbr[g], gen_count * sizeof(float4)});
+        v.push_back({s.kvlog, kvlog_count * sizeof(KvLogEntry)});
+    }
+    if (s.mlog) v.push_back({s.mlog, mlog_count * sizeof(glint_meeting)});
+    return v;
+}
+
+glint_status snapshot_header(glint_ctx *c, SnapHeader &h)
+{
+    if (!c->evaluated || !c->has_prev) {
+        c->err = "a snapshot is taken after an evaluated tick (at a tick boundary)";
+        return GLINT_ERR_STATE;
+    }
+    glint_status st = read_counters(c);
+    if (st) return st;
+    DevState &s = c->s;
+    const int g = 1 - c->cur_gen;
+    uint32_t cnt = 0;
+    HIP_TRY(c, hipMemcpy(&cnt, c->tab[g].rstart + s.rs_total, 4, hipMemcpyDeviceToHost));
+    h = SnapHeader{};
+    h.magic = kSnapMagic;
+    h.n = s.n;
+    h.node_cap = s.node_cap;
+    h.rs_total = s.rs_total;
+    h.precision = s.precision;
+    h.store_all = s.store_all;
+    h.kv = c->kv != nullptr;
+    h.dynamic_schema = c->cfg.dynamic_schema;
+    h.cur_gen = (uint32_t)c->cur_gen;
+    h.built = c->built;
+    h.has_prev = c->has_prev;
+    h.tick = c->tick;
+    h.prev_tick = c->prev_tick;
+    h.kv_end_time_min = c->kv ? kv_end_time_min(c->kv) : 0;
+    h.clist_cap = s.clist_cap;
+    h.mlog_cap = s.mlog_cap;
+    h.kvlog_cap = s.kvlog_cap;
+    h.gen_count = cnt;
+    h.mlog_count = s.mlog ? std::min<uint64_t>(c->h_ctr->n_mlog, s.mlog_cap) : 0;
+    h.kvlog_count = c->kv ? std::min<uint64_t>(c->h_ctr->n_kvlog, s.kvlog_cap) : 0;
+    const std::vector<Section> v = state_sections(c, h.gen_count, h.mlog_count, h.kvlog_count);
+    h.nsections = (uint32_t)v.size();
+    h.bytes = sizeof(SnapHeader);
+    for (const Section &x : v) h.bytes += x.bytes;
+    return GLINT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+glint_status glint_snapshot_size(glint_ctx *c, uint64_t *bytes)
+{
+    if (!c || !bytes) return GLINT_ERR_INVALID_ARGUMENT;
+    HIP_TRY(c, hipSetDevice(c->device));
+    SnapHeader h;
+    glint_status st = snapshot_header(c, h);
+    if (st) return st;
+    *bytes = h.bytes;
+    return GLINT_OK;
+}
+
+glint_status glint_snapshot(glint_ctx *c, void *buf, uint64_t bytes)
+{
+    if (!c || !buf) return GLINT_ERR_INVALID_ARGUMENT;
+    HIP_TRY(c, hipSetDevice(c->device));
+    SnapHeader h;
+    glint_status st = snapshot_header(c, h);
+    if (st) return st;
+    if (bytes < h.bytes) {
+        c->err = "snapshot buffer too small (glint_snapshot_size)";
+        return GLINT_ERR_OUTPUT_CAPACITY;
+    }
+    uint8_t *p = static_cast<uint8_t *>(buf);
+    std::memcpy(p, &h, sizeof h);
+    p += sizeof h;
+    for (const Section &x : state_sections(c, h.gen_count, h.mlog_count, h.kvlog_count)) {
+        if (x.bytes) HIP_TRY(c, hipMemcpy(p, x.dev, x.bytes, hipMemcpyDeviceToHost));
+        p += x.bytes;
+    }
+    return GLINT_OK;
+}
+
+glint_status glint_restore(glint_ctx *c, const void *buf, uint64_t bytes)
+{
+    if (!c || !buf || bytes < sizeof(SnapHeader)) return GLINT_ERR_INVALID_ARGUMENT;
+    SnapHeader h;
+    std::memcpy(&h, buf, sizeof h);
+    DevState &s = c->s;
+    if (h.magic != kSnapMagic || h.bytes != bytes || h.n != s.n || h.node_cap != s.node_cap ||
+        h.rs_total != s.rs_total || h.precision != s.precision || h.store_all != s.store_all ||
+        h.kv != (c->kv != nullptr) || h.dynamic_schema != c->cfg.dynamic_schema || h.clist_cap != s.clist_cap ||
+        h.mlog_cap != s.mlog_cap || h.kvlog_cap != s.kvlog_cap || h.gen_count > s.clist_cap || h.cur_gen > 1) {
+        c->err = "the snapshot does not belong to a context of this configuration";
+        return GLINT_ERR_INVALID_ARGUMENT;
+    }
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->cur_gen = (int)h.cur_gen;   // the sections follow the generation parity of the snapshot
+    const uint8_t *p = static_cast<const uint8_t *>(buf) + sizeof h;
+    const std::vector<Section> v = state_sections(c, h.gen_count, h.mlog_count, h.kvlog_count);
+    if (v.size() != h.nsections) {
+        c->err = "the snapshot's sections do not match this context";
+        return GLINT_ERR_INVALID_ARGUMENT;
+    }
+    for (const Section &x : v) {
+        if (x.bytes) HIP_TRY(c, hipMemcpy(x.dev, p, x.bytes, hipMemcpyHostToDevice));
+        p += x.bytes;
+    }
+    c->built = h.built != 0;
+    c->has_prev = h.has_prev != 0;
+    c->tick = h.tick;
+    c->prev_tick = h.prev_tick;
+    c->has_positions = c->partitioned = c->refined = c->evaluated = true;
+    // the last tick's positions (the history the next tick reads) become the
+    // resident ones, from which update ingestion continues
+    HIP_TRY(c, hipMemcpy(c->own_xy, c->hist[1 - c->cur_gen], (size_t)s.n * 16, hipMemcpyDeviceToDevice));
+    c->last_xy = c->own_xy;
+    c->resident = true;
+    if (c->kv) kv_set_end_time_min(c->kv, h.kv_end_time_min);
+    glint_status st = read_counters(c);
+    if (st) return st;
+    fill_stats(c);
+    return status_from_flags(c, c->h_ctr->error);
+}
+
 glint_status glint_kv_pending(glint_ctx *c, uint64_t *keys)
 {
     if (!c || !keys) return GLINT_ERR_INVALID_ARGUMENT;

@@ -854,6 +854,8 @@ void kv_destroy(KvEngine *e)
 }
 
 KvLogEntry *kv_log(KvEngine *e) { return e->log; }
+uint32_t kv_end_time_min(const KvEngine *e) { return e->end_time_min; }
+void kv_set_end_time_min(KvEngine *e, uint32_t t) { e->end_time_min = t; }
 uint64_t kv_log_capacity(const KvEngine *e) { return e->key_cap / 2; }
 
 glint_status kv_flush(KvEngine *e, uint64_t nlog, uint32_t cur_time, glint_kv_summary *out, std::string &err)

@@ -31,4 +31,7 @@ uint64_t kv_log_capacity(const KvEngine *e);
 // next run's end offsets count from cur_time.
 glint_status kv_flush(KvEngine *e, uint64_t nlog, uint32_t cur_time, glint_kv_summary *out, std::string &err);
 uint64_t kv_part_bytes(const KvEngine *e, uint32_t part);
+// The tick end offsets of the next flush count from (snapshot and restore).
+uint32_t kv_end_time_min(const KvEngine *e);
+void kv_set_end_time_min(KvEngine *e, uint32_t t);
 glint_status kv_copy(KvEngine *e, uint32_t part, void *dst, uint64_t bytes, std::string &err);

@@ -17,12 +17,27 @@
 // The answer therefore depends only on T, not on the admission delay, so a
 // query gives the same result alone and in any mix.
 //
+// Meeting index: every index_block_ticks ticks the meetings logged since the
+// last block are sealed into a block of directed entries (object, target,
+// start, end), two per meeting, sorted by (object, target, start) with a
+// stable radix sort, as the reference keeps its flushed meetings sorted by
+// object for id searches (workbench.cpp:397-444, step_merge.cpp:501-568).  A
+// query binary-searches every block that can hold meetings of its window for
+// each seed (hop 1) or hop-1 contact (hop 2), keeping the meetings that ended
+// by T (end < T) and overlap the window; only the log after the last block
+// sealed is scanned.  Blocks and the scanned tail split the log at that
+// block's end, so every meeting is read once.
+//
 // Admission (continuous batching): queries wait in FIFO order; one is
 // admitted when a stream slot is free and its footprint (seed bitmap, seeds,
 // hit buffers) fits a first-fit allocator over a preallocated device arena.
 // A query whose hits overflow its estimate is re-queued with the exact size.
 
 #include <hip/hip_runtime.h>
+
+#include <cstring>   // before rocPRIM, whose iterators use memset
+
+#include <rocprim/device/device_radix_sort.hpp>
 
 #include <algorithm>
 #include <chrono>
@@ -172,6 +187,58 @@ __global__ __launch_bounds__(kB) void k_log_hop2(const glint_meeting *log, uint6
     }
 }
 
+// Directed index entries of log entries [l0, l1) (ring positions): two per
+// meeting, key (object << 32) | target, value (start << 32) | end.
+__global__ __launch_bounds__(kB) void k_index_expand(const glint_meeting *log, uint64_t log_cap, uint64_t l0,
+                                                     uint64_t l1, unsigned long long *keys, unsigned long long *vals)
+{
+    for (uint64_t i = l0 + blockIdx.x * (uint64_t)kB + threadIdx.x; i < l1; i += (uint64_t)gridDim.x * kB) {
+        const glint_meeting m = log[i % log_cap];
+        const uint64_t j = 2 * (i - l0);
+        const unsigned long long v = ((unsigned long long)m.start << 32) | m.end;
+        keys[j] = ((unsigned long long)m.pid_a << 32) | m.pid_b;
+        keys[j + 1] = ((unsigned long long)m.pid_b << 32) | m.pid_a;
+        vals[j] = v;
+        vals[j + 1] = v;
+    }
+}
+
+// One (object, block) pair per thread: the block's entries of the object
+// (a binary search for its first), kept if they ended by T, overlap the
+// window and (hop 2) end at or after the object's first exposure.
+__global__ __launch_bounds__(kB) void k_index_hits(const unsigned long long *keys, const unsigned long long *vals,
+                                                   const ulonglong2 *blocks, uint32_t nblocks, const uint32_t *ids,
+                                                   const uint32_t *exposure, uint32_t nids, uint32_t T,
+                                                   uint32_t t_start, uint32_t t_end, glint_hit *out,
+                                                   unsigned long long *count, uint64_t cap)
+{
+    const uint64_t total = (uint64_t)nids * nblocks;
+    for (uint64_t w = blockIdx.x * (uint64_t)kB + threadIdx.x; w < total; w += (uint64_t)gridDim.x * kB) {
+        const uint32_t id = ids[w / nblocks];
+        const ulonglong2 b = blocks[w % nblocks];   // (first entry, entries)
+        const unsigned long long want = (unsigned long long)id << 32;
+        uint64_t lo = b.x, hi = b.x + b.y;
+        while (lo < hi) {
+            const uint64_t mid = (lo + hi) / 2;
+            if (keys[mid] < want)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        const uint32_t from = exposure ? exposure[w / nblocks] : 0u;
+        for (uint64_t j = lo; j < b.x + b.y; j++) {
+            const unsigned long long k = keys[j];
+            if ((uint32_t)(k >> 32) != id) break;
+            const unsigned long long v = vals[j];
+            const uint32_t start = (uint32_t)(v >> 32), end = (uint32_t)v;
+            if (end < T && start <= t_end && end >= t_start && end >= from) {
+                const unsigned long long at = atomicAdd(count, 1ull);
+                if (at < cap) out[at] = glint_hit{id, (uint32_t)k, start, end};
+            }
+        }
+    }
+}
+
 inline uint32_t grid_of(uint64_t n, uint32_t cap = 4096)
 {
     return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n + kB - 1) / kB, cap));
@@ -251,6 +318,32 @@ struct Query {
     glint_hit *h1 = nullptr, *h2 = nullptr;
     QCounters *ctr = nullptr;
     uint32_t nids = 0;
+    ulonglong2 *blocks_d = nullptr;   // index blocks searched (first entry, entries), in the arena
+    std::vector<ulonglong2> blocks_h;
+    uint32_t nblocks = 0, room = 0;
+    uint64_t tail = 0;                // log entries from here on are scanned
+};
+
+// One sealed block of the meeting index.
+struct IndexBlock {
+    uint64_t off, n;       // its entries in the index storage
+    uint64_t l0, l1;       // the log entries it holds
+    uint32_t t0, t1;       // the ticks that logged them
+};
+
+struct MeetingIndex {
+    unsigned long long *keys = nullptr, *vals = nullptr;     // block storage (a ring of blocks)
+    uint64_t cap = 0;
+    unsigned long long *skeys = nullptr, *svals = nullptr;   // a block before its sort
+    uint64_t block_max = 0;                                  // entries a block holds at most
+    void *tmp = nullptr;
+    size_t tmp_bytes = 0;
+    std::deque<IndexBlock> blocks;
+    uint64_t next = 0;          // where the next block goes
+    uint64_t sealed = 0;        // log entries below this are in blocks
+    uint32_t first_tick = 0;    // the first tick not in a block
+    uint32_t block_ticks = 64;
+    uint64_t entries = 0;       // entries in the blocks held
 };
 
 struct Slot_ {
@@ -271,6 +364,7 @@ struct glint_server {
     std::deque<uint64_t> pending;
     std::map<uint64_t, Query> queries;
     std::map<uint32_t, uint64_t> log_after;   // tick -> meetings logged after it
+    MeetingIndex index;
     uint64_t next_id = 1;
     bool have_tick = false;
     double meetings_per_tick = 0;             // running average, for hit estimates
@@ -296,12 +390,32 @@ namespace {
 
 uint64_t bitmap_bytes(uint32_t n) { return up(((uint64_t)n + 31) / 32 * 4); }
 
+constexpr uint32_t kBlockSlack = 64;   // index blocks sealed between admission and hop 2 that a query may use
+
+// The index blocks a query searches for meetings ended by T within the
+// window (at most `room`, oldest first), and where the scan of the log
+// takes over: the end of the last block used, within [lo, hi_sub].
+void index_blocks(const glint_server *s, const Query &q, uint32_t room, std::vector<ulonglong2> &out,
+                  uint64_t &tail)
+{
+    out.clear();
+    uint64_t covered = q.lo;   // log entries [lo, covered) are in the blocks taken
+    for (const IndexBlock &b : s->index.blocks) {
+        if (b.l1 <= q.lo) continue;                          // ended before the window
+        if (out.size() >= room || b.l0 >= q.hi_sub || b.l0 > covered) break;   // full, after T, or a gap
+        covered = b.l1;
+        if (b.t1 > q.t_start) out.push_back(make_ulonglong2(b.off, b.n));   // else all ended before the window
+    }
+    tail = std::min(covered, q.hi_sub);
+}
+
 uint64_t footprint(const glint_server *s, const Query &q)
 {
     uint64_t b = up(sizeof(QCounters)) + bitmap_bytes(s->view.num_objects) + up(4ull * q.seeds.size()) +
                  up(q.cap1 * sizeof(glint_hit));
     if (q.hops >= 2)
         b += bitmap_bytes(s->view.num_objects) + 2 * up(4ull * q.cap1) + up(q.cap2 * sizeof(glint_hit));
+    b += up(sizeof(ulonglong2) * (s->index.blocks.size() + kBlockSlack));   // the index blocks searched
     return b;
 }
 
@@ -356,6 +470,8 @@ glint_status start_query(glint_server *s, Query &q, int slot)
         q.starts_d = (uint32_t *)take(4ull * q.cap1);
         q.h2 = (glint_hit *)take(q.cap2 * sizeof(glint_hit));
     }
+    q.room = (uint32_t)(s->index.blocks.size() + kBlockSlack);
+    q.blocks_d = (ulonglong2 *)take(sizeof(ulonglong2) * q.room);
     Slot_ &sl = s->slots[slot];
     sl.query = q.id;
     q.slot = slot;
@@ -381,11 +497,22 @@ glint_status start_query(glint_server *s, Query &q, int slot)
     // while the scan does
     if (q.lo < first + v.mlog_cap / 8 && first > 0)
         return fail_query(s, q, "window no longer retained by the meeting log");
-    if (total > q.lo)
-        k_log_hop1<<<grid_of(total - q.lo, 8192), kB, 0, st>>>(v.mlog, v.mlog_cap, q.lo, q.hi_sub, total, q.T,
-                                                               v.min_meet, q.t_start, q.t_end, q.bm1, q.h1,
-                                                               &q.ctr->hits1, q.cap1);
+    index_blocks(s, q, q.room, q.blocks_h, q.tail);
+    if (total > q.tail)
+        k_log_hop1<<<grid_of(total - q.tail, 8192), kB, 0, st>>>(v.mlog, v.mlog_cap, q.tail, q.hi_sub, total, q.T,
+                                                                 v.min_meet, q.t_start, q.t_end, q.bm1, q.h1,
+                                                                 &q.ctr->hits1, q.cap1);
+    // the blocks were sealed on the context's stream before the table hits
     SV_TRY(s, hipStreamWaitEvent(st, sl.snap, 0));
+    q.nblocks = (uint32_t)q.blocks_h.size();
+    if (q.nblocks) {
+        SV_TRY(s, hipMemcpyAsync(q.blocks_d, q.blocks_h.data(), sizeof(ulonglong2) * q.nblocks,
+                                 hipMemcpyHostToDevice, st));
+        const uint64_t work = (uint64_t)q.seeds.size() * q.nblocks;
+        k_index_hits<<<grid_of(work, 8192), kB, 0, st>>>(s->index.keys, s->index.vals, q.blocks_d, q.nblocks,
+                                                         q.seeds_d, nullptr, (uint32_t)q.seeds.size(), q.T,
+                                                         q.t_start, q.t_end, q.h1, &q.ctr->hits1, q.cap1);
+    }
     SV_TRY(s, hipEventRecord(sl.done, st));
     SV_TRY(s, hipGetLastError());
     q.state = kRunning;
@@ -477,10 +604,26 @@ glint_status advance(glint_server *s)
                 k_set_bits<<<grid_of(q.nids), kB, 0, st>>>(q.ids_d, q.nids, q.bm2);
                 CtxView v{};
                 glint_ctx_view(s->ctx, &v);
-                if (q.hi_sub > q.lo)
-                    k_log_hop2<<<grid_of(q.hi_sub - q.lo, 8192), kB, 0, st>>>(
-                        v.mlog, v.mlog_cap, q.lo, q.hi_sub, q.t_start, q.t_end, q.bm2, q.ids_d, q.starts_d, q.nids,
+                uint64_t tail = 0;
+                index_blocks(s, q, q.room, q.blocks_h, tail);
+                if (q.hi_sub > tail)
+                    k_log_hop2<<<grid_of(q.hi_sub - tail, 8192), kB, 0, st>>>(
+                        v.mlog, v.mlog_cap, tail, q.hi_sub, q.t_start, q.t_end, q.bm2, q.ids_d, q.starts_d, q.nids,
                         q.h2, &q.ctr->hits2, q.cap2);
+                q.nblocks = (uint32_t)q.blocks_h.size();
+                if (q.nblocks) {
+                    // blocks sealed since admission: their sort is ordered before
+                    // this point of the context's stream
+                    SV_TRY(s, hipEventRecord(sl.snap, v.stream));
+                    SV_TRY(s, hipStreamWaitEvent(st, sl.snap, 0));
+                    SV_TRY(s, hipMemcpyAsync(q.blocks_d, q.blocks_h.data(), sizeof(ulonglong2) * q.nblocks,
+                                             hipMemcpyHostToDevice, st));
+                    const uint64_t work = (uint64_t)q.nids * q.nblocks;
+                    k_index_hits<<<grid_of(work, 8192), kB, 0, st>>>(s->index.keys, s->index.vals, q.blocks_d,
+                                                                     q.nblocks, q.ids_d, q.starts_d, q.nids, q.T,
+                                                                     q.t_start, q.t_end, q.h2, &q.ctr->hits2,
+                                                                     q.cap2);
+                }
             }
             SV_TRY(s, hipEventRecord(sl.done, st));
             q.phase = 2;
@@ -501,6 +644,53 @@ glint_status advance(glint_server *s)
         q.state = kDone;
         q.t_done = s->now_ms();
     }
+    return GLINT_OK;
+}
+
+// Seals log entries [l0, l1) into one index block, on the context's stream.
+glint_status seal_block(glint_server *s, uint64_t l0, uint64_t l1, uint32_t tick)
+{
+    MeetingIndex &ix = s->index;
+    const uint64_t n = 2 * (l1 - l0);
+    if (ix.next + n > ix.cap) ix.next = 0;   // blocks form a ring in the storage
+    const uint64_t a = ix.next, b = a + n;
+    for (auto it = ix.blocks.begin(); it != ix.blocks.end();) {
+        if (it->off < b && a < it->off + it->n) {   // overwritten by the new block
+            ix.entries -= it->n;
+            it = ix.blocks.erase(it);
+        } else {
+            ++it;
+        }
+    }
+    hipStream_t st = s->view.stream;
+    k_index_expand<<<grid_of(l1 - l0, 4096), kB, 0, st>>>(s->view.mlog, s->view.mlog_cap, l0, l1, ix.skeys,
+                                                           ix.svals);
+    size_t bytes = ix.tmp_bytes;
+    // stable: a pair's meetings stay in log order, which is the order of their starts
+    SV_TRY(s, rocprim::radix_sort_pairs(ix.tmp, bytes, ix.skeys, ix.keys + a, ix.svals, ix.vals + a, (size_t)n, 0,
+                                        62, st));
+    SV_TRY(s, hipGetLastError());
+    ix.blocks.push_back(IndexBlock{a, n, l0, l1, ix.first_tick, tick});
+    ix.entries += n;
+    ix.next = b;
+    ix.sealed = l1;
+    ix.first_tick = tick + 1;
+    return GLINT_OK;
+}
+
+// After a tick: seal the log since the last block when index_block_ticks
+// ticks have passed, or in pieces of block_max entries when it grows faster.
+glint_status index_tick(glint_server *s, uint64_t total, uint64_t first, uint32_t tick)
+{
+    MeetingIndex &ix = s->index;
+    if (!ix.keys) return GLINT_OK;
+    if (ix.sealed < first) ix.sealed = first;   // entries the ring lost before they were sealed
+    const uint64_t most = ix.block_max / 2;     // meetings per block
+    while (total - ix.sealed >= most) {
+        glint_status st = seal_block(s, ix.sealed, ix.sealed + most, tick);
+        if (st) return st;
+    }
+    if (total > ix.sealed && tick + 1 - ix.first_tick >= ix.block_ticks) return seal_block(s, ix.sealed, total, tick);
     return GLINT_OK;
 }
 
@@ -528,6 +718,9 @@ glint_status glint_server_destroy(glint_server *s)
     }
     if (s->view.stream) (void)hipStreamSynchronize(s->view.stream);
     if (s->arena_d) (void)hipFree(s->arena_d);
+    for (void *p : {(void *)s->index.keys, (void *)s->index.vals, (void *)s->index.skeys, (void *)s->index.svals,
+                    s->index.tmp})
+        if (p) (void)hipFree(p);
     delete s;
     return GLINT_OK;
 }
@@ -552,6 +745,29 @@ glint_status glint_server_create(glint_ctx *ctx, const glint_server_config *cfg,
         return GLINT_ERR_OUT_OF_MEMORY;
     }
     s->arena.init(cfg->arena_bytes);
+    MeetingIndex &ix = s->index;
+    ix.block_ticks = cfg->index_block_ticks ? cfg->index_block_ticks : 64;
+    if (ix.block_ticks != 0xFFFFFFFFu) {
+        // blocks of at most 2^22 meetings; storage for twice the log's meetings
+        // (two entries each) and two blocks more, so blocks outlive the log
+        const uint64_t L = s->view.mlog_cap;
+        ix.block_max = 2 * std::min<uint64_t>(L, 1ull << 22);
+        ix.cap = 2 * L + 2 * ix.block_max;
+        bool ok = hipMalloc((void **)&ix.keys, 8 * ix.cap) == hipSuccess &&
+                  hipMalloc((void **)&ix.vals, 8 * ix.cap) == hipSuccess &&
+                  hipMalloc((void **)&ix.skeys, 8 * ix.block_max) == hipSuccess &&
+                  hipMalloc((void **)&ix.svals, 8 * ix.block_max) == hipSuccess;
+        if (ok) {
+            ok = rocprim::radix_sort_pairs(nullptr, ix.tmp_bytes, ix.skeys, ix.keys, ix.svals, ix.vals,
+                                           (size_t)ix.block_max, 0, 62, s->view.stream) == hipSuccess &&
+                 hipMalloc(&ix.tmp, std::max<size_t>(ix.tmp_bytes, 256)) == hipSuccess;
+        }
+        if (!ok) {
+            (void)hipGetLastError();
+            glint_server_destroy(s);
+            return GLINT_ERR_OUT_OF_MEMORY;
+        }
+    }
     s->slots.resize(cfg->max_running);
     for (Slot_ &sl : s->slots) {
         if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess ||
@@ -591,6 +807,7 @@ glint_status glint_server_tick(glint_server *s, const double *d_full_xy, const u
     s->log_after[tick] = total;
     while (s->log_after.size() > 1 && s->log_after.begin()->second < first) s->log_after.erase(s->log_after.begin());
     glint_ctx_view(c, &s->view);
+    if ((st = index_tick(s, total, first, tick))) return st;
     if ((st = advance(s))) return st;
     return admit(s);
 }
@@ -625,6 +842,15 @@ glint_status glint_server_submit(glint_server *s, const uint32_t *seeds, uint32_
     s->pending.push_back(q.id);
     *id = q.id;
     return admit(s);
+}
+
+glint_status glint_server_index_info(glint_server *s, uint64_t *blocks, uint64_t *entries, uint64_t *sealed)
+{
+    if (!s) return GLINT_ERR_INVALID_ARGUMENT;
+    if (blocks) *blocks = s->index.blocks.size();
+    if (entries) *entries = s->index.entries;
+    if (sealed) *sealed = s->index.sealed;
+    return GLINT_OK;
 }
 
 glint_status glint_server_poll(glint_server *s, uint32_t *running, uint32_t *pending)

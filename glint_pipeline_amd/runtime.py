@@ -106,7 +106,8 @@ class Totals(ctypes.Structure):
 
 class ServerConfig(ctypes.Structure):
     _fields_ = [("arena_bytes", ctypes.c_uint64), ("max_running", ctypes.c_uint32),
-                ("estimate_percent", ctypes.c_uint32), ("min_hits", ctypes.c_uint64)]
+                ("estimate_percent", ctypes.c_uint32), ("min_hits", ctypes.c_uint64),
+                ("index_block_ticks", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
 
 
 class QueryInfo(ctypes.Structure):
@@ -179,6 +180,7 @@ SIGNATURES = {
     "glint_server_tick": (_I, [_VP, _VP, _VP, _VP, _U32, _U32]),
     "glint_server_submit": (_I, [_VP, _VP, _U32, _U32, _U32, _U32, ctypes.POINTER(_U64)]),
     "glint_server_poll": (_I, [_VP, ctypes.POINTER(_U32), ctypes.POINTER(_U32)]),
+    "glint_server_index_info": (_I, [_VP, ctypes.POINTER(_U64), ctypes.POINTER(_U64), ctypes.POINTER(_U64)]),
     "glint_server_drain": (_I, [_VP]),
     "glint_server_query_info": (_I, [_VP, _U64, ctypes.POINTER(QueryInfo)]),
     "glint_server_result": (_I, [_VP, _U64, _VP, _U64, ctypes.POINTER(_U64)]),
@@ -196,6 +198,9 @@ SIGNATURES = {
     "glint_batch_run_ticks": (_I, [_VP, _VP, _U32, _U32, _VP]),
     "glint_meeting_log_info": (_I, [_VP, ctypes.POINTER(_U64), ctypes.POINTER(_U64)]),
     "glint_meeting_log_read": (_I, [_VP, _U64, _VP, _U64, ctypes.POINTER(_U64)]),
+    "glint_snapshot_size": (_I, [_VP, ctypes.POINTER(_U64)]),
+    "glint_snapshot": (_I, [_VP, _VP, _U64]),
+    "glint_restore": (_I, [_VP, _VP, _U64]),
     "glint_kv_pending": (_I, [_VP, ctypes.POINTER(_U64)]),
     "glint_kv_flush": (_I, [_VP, ctypes.POINTER(KvSummary)]),
     "glint_kv_part_size": (_I, [_VP, _U32, ctypes.POINTER(_U64)]),
@@ -569,6 +574,19 @@ class Workbench:
             "packed": part(KV_PACKED, np.uint8),
         }
 
+    def snapshot(self) -> np.ndarray:
+        """The state the next tick reads, as bytes in host memory (at a tick boundary)."""
+        n = ctypes.c_uint64()
+        self._chk(lib().glint_snapshot_size(self.h, ctypes.byref(n)), "glint_snapshot_size")
+        buf = np.zeros(n.value, dtype=np.uint8)
+        self._chk(lib().glint_snapshot(self.h, buf.ctypes.data, buf.size), "glint_snapshot")
+        return buf
+
+    def restore(self, buf: np.ndarray) -> None:
+        """Continue from a snapshot of a context with the same configuration."""
+        buf = np.ascontiguousarray(buf, dtype=np.uint8)
+        self._chk(lib().glint_restore(self.h, buf.ctypes.data, buf.size), "glint_restore")
+
     def inject_failure(self, stage: str) -> None:
         self._chk(lib().glint_debug_inject_failure(self.h, stage.encode()), "glint_debug_inject_failure")
 
@@ -703,9 +721,9 @@ class Server:
     updates, and tracing queries admitted by HBM footprint."""
 
     def __init__(self, wb: Workbench, arena_bytes: int = 1 << 30, max_running: int = 8, min_hits: int = 4096,
-                 estimate_percent: int = 100):
+                 estimate_percent: int = 100, index_block_ticks: int = 0):
         self.wb = wb
-        sc = ServerConfig(arena_bytes, max_running, estimate_percent, min_hits)
+        sc = ServerConfig(arena_bytes, max_running, estimate_percent, min_hits, index_block_ticks, 0)
         h = ctypes.c_void_p()
         st = lib().glint_server_create(wb.h, ctypes.byref(sc), ctypes.byref(h))
         if st != GLINT_OK:
@@ -737,6 +755,14 @@ class Server:
 
     def drain(self) -> None:
         self._chk(lib().glint_server_drain(self.h), "glint_server_drain")
+
+    def index_info(self) -> dict:
+        """Blocks of the per-object meeting index, their entries, and the log
+        entries below which every meeting is in a block."""
+        b, e, sl = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        self._chk(lib().glint_server_index_info(self.h, ctypes.byref(b), ctypes.byref(e), ctypes.byref(sl)),
+                  "glint_server_index_info")
+        return {"blocks": b.value, "entries": e.value, "sealed": sl.value}
 
     def info(self, qid: int) -> dict:
         qi = QueryInfo()

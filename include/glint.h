@@ -330,6 +330,9 @@ typedef struct glint_server_config {
     uint32_t max_running;      /* queries on the device at once (one stream each) */
     uint32_t estimate_percent; /* scale of the hit-buffer estimates (0: 100); an overflow re-queues */
     uint64_t min_hits;         /* smallest hit buffer per hop (entries; 0: 4096) */
+    uint32_t index_block_ticks;/* ticks of meetings per block of the per-object meeting index (0: 64;
+                                  0xFFFFFFFF: no index, queries scan the log) */
+    uint32_t reserved;
 } glint_server_config;
 
 /* One contact found by a query: source (a seed at hop 1, a hop-1 contact at
@@ -368,6 +371,14 @@ glint_status glint_server_tick(glint_server *server, const double *d_full_xy, co
  * tick.  It runs when a slot and arena memory are free. */
 glint_status glint_server_submit(glint_server *server, const uint32_t *seeds, uint32_t nseeds, uint32_t t_start,
                                  uint32_t t_end, uint32_t hops, uint64_t *id);
+/* The per-object meeting index (the role of the reference's id search over
+ * meetings sorted by object, workbench.cpp:397-444, step_merge.cpp:501-568):
+ * blocks of directed meetings sorted by (object, target, start), sealed from
+ * the meeting log every index_block_ticks ticks; queries binary-search them
+ * for each seed and scan only the log since the last block.  Reports the
+ * blocks held, their entries (two per meeting) and the log entries below
+ * which every meeting is in a block. */
+glint_status glint_server_index_info(glint_server *server, uint64_t *blocks, uint64_t *entries, uint64_t *sealed);
 /* Advance finished work and admit pending queries; counts are optional. */
 glint_status glint_server_poll(glint_server *server, uint32_t *running, uint32_t *pending);
 /* Poll until no query is running or pending. */
@@ -388,6 +399,23 @@ glint_status glint_meeting_log_info(glint_ctx *ctx, uint64_t *total, uint64_t *f
 /* Copy entries [from, total) (at most cap) to host memory; from must be retained. */
 glint_status glint_meeting_log_read(glint_ctx *ctx, uint64_t from, glint_meeting *out, uint64_t cap,
                                     uint64_t *n_out);
+
+/* --------------------------------------------------------------- snapshots
+ * The reference dumps its meeting buckets and metadata and loads them back
+ * into the table (dump_meetings / load_meetings / dump_meta,
+ * workbench.cpp:446-470,574-595; insert_meetings_to_buckets,
+ * geometry.cu:543-565).  Here a snapshot is everything the next tick reads,
+ * taken at a tick boundary (after an evaluated tick): the quadtree, the last
+ * tick's objects in leaf order with their candidate lists, the position
+ * histories, the last generation of the meeting table, the counters, the
+ * meeting log and the key-value log.  A context created with the same
+ * configuration that restores it continues the run exactly; its resident
+ * positions (for glint_ingest_updates) are the last tick's. */
+glint_status glint_snapshot_size(glint_ctx *ctx, uint64_t *bytes);
+/* Copies the snapshot into host memory (bytes >= glint_snapshot_size). */
+glint_status glint_snapshot(glint_ctx *ctx, void *buf, uint64_t bytes);
+/* GLINT_ERR_INVALID_ARGUMENT if the snapshot belongs to another configuration. */
+glint_status glint_restore(glint_ctx *ctx, const void *buf, uint64_t bytes);
 
 /* ---------------------------------------------------- sorted key-value output
  * The flush branch of the reference's GPU path (geometry.cu:1752-2138).  With

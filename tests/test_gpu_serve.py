@@ -37,7 +37,8 @@ def plan(seed=5):
     return out
 
 
-def serve(queries, arena, max_running, min_hits=4096, alone=False, poll=True, estimate_percent=100):
+def serve(queries, arena, max_running, min_hits=4096, alone=False, poll=True, estimate_percent=100,
+          index_block_ticks=4):
     """Runs the walk under the server; returns per-query (hop1, hop2, info),
     the final log, the log totals after each tick, and active pairs at every
     submission tick."""
@@ -53,7 +54,8 @@ def serve(queries, arena, max_running, min_hits=4096, alone=False, poll=True, es
     full.upload(g.positions())
     results, totals, actives = {}, [], {}
     with runtime.Workbench(cfg) as wb, runtime.Server(wb, arena_bytes=arena, max_running=max_running,
-                                                      min_hits=min_hits, estimate_percent=estimate_percent) as sv:
+                                                      min_hits=min_hits, estimate_percent=estimate_percent,
+                                                      index_block_ticks=index_block_ticks) as sv:
         ids = {}
         for t in range(TICKS):
             if t == 0:
@@ -72,6 +74,11 @@ def serve(queries, arena, max_running, min_hits=4096, alone=False, poll=True, es
             if poll:
                 sv.poll()
         sv.drain()
+        info = sv.index_info()
+        if index_block_ticks != 0xFFFFFFFF:
+            assert info["blocks"] >= TICKS // index_block_ticks - 1 and info["entries"] == 2 * info["sealed"]
+        else:
+            assert info["blocks"] == 0
         for i, qid in ids.items():
             info = sv.info(qid)
             h1, h2 = sv.result(qid)
@@ -134,13 +141,18 @@ def test_served_log_matches_oracle():
     assert len(want) > 100 and np.array_equal(got, want)
 
 
-def test_queries_alone_and_in_a_mix():
+@pytest.mark.parametrize("index_block_ticks", [4, 7, 0xFFFFFFFF])
+def test_queries_alone_and_in_a_mix(index_block_ticks):
+    """Blocks of 4 or 7 ticks of the per-object meeting index searched with
+    the log tail scanned, or the whole log scanned (no index): the same
+    answers, each its definition."""
     queries = plan()
-    print(f"[serve] {len(queries)} queries", flush=True)
-    alone, log, totals, actives = serve(queries, 1 << 28, 1, alone=True)
+    print(f"[serve] {len(queries)} queries, index blocks of {index_block_ticks} ticks", flush=True)
+    alone, log, totals, actives = serve(queries, 1 << 28, 1, alone=True, index_block_ticks=index_block_ticks)
     # a 256 KB arena and two slots, admissions only at ticks: queries wait
     # for later ticks, and tiny hit buffers force re-admission after overflow
-    mix, log2, _, _ = serve(queries, 256 << 10, 2, min_hits=16, poll=False, estimate_percent=5)
+    mix, log2, _, _ = serve(queries, 256 << 10, 2, min_hits=16, poll=False, estimate_percent=5,
+                            index_block_ticks=index_block_ticks)
     srt = lambda v: v[np.lexsort((v[:, 3], v[:, 2], v[:, 1], v[:, 0]))]
     assert np.array_equal(srt(log), srt(log2))
     waited = retried = 0
