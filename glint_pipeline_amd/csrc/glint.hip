@@ -56,6 +56,126 @@ constexpr uint32_t kHeavyLeaf = 256;                // leaves above this size ar
 
 // ------------------------------------------------------------ predicates
 
+// Degrees per metre (util.h:38,57-66), each operation in the reference's
+// order: PI is the reference's 3.14159265, 90 degrees is taken as 89.
+constexpr double kRefPi = 3.14159265;
+constexpr double kDegPerMeterLat = 360.0 / (40076.0 * 1000.0);
+
+// Double-double arithmetic (Dekker, Knuth): error-free sums and products.
+// Contraction is off in each so that no product is fused into a later sum,
+// which would break the error-free property.
+struct DD {
+    double hi, lo;
+};
+
+__device__ inline DD dd_quick(double a, double b)   // |a| >= |b|
+{
+#pragma clang fp contract(off)
+    const double s = a + b;
+    return DD{s, b - (s - a)};
+}
+
+__device__ inline DD dd_two_sum(double a, double b)
+{
+#pragma clang fp contract(off)
+    const double s = a + b, bb = s - a;
+    return DD{s, (a - (s - bb)) + (b - bb)};
+}
+
+__device__ inline DD dd_add(DD a, DD b)
+{
+#pragma clang fp contract(off)
+    DD s = dd_two_sum(a.hi, b.hi);
+    const DD t = dd_two_sum(a.lo, b.lo);
+    s.lo += t.hi;
+    s = dd_quick(s.hi, s.lo);
+    s.lo += t.lo;
+    return dd_quick(s.hi, s.lo);
+}
+
+__device__ inline DD dd_mul(DD a, DD b)
+{
+#pragma clang fp contract(off)
+    const double p = a.hi * b.hi;
+    double e = fma(a.hi, b.hi, -p);
+    e = fma(a.hi, b.lo, e);
+    e = fma(a.lo, b.hi, e);
+    return dd_quick(p, e);
+}
+
+// (-1)^k / (2k+1)! as double-double (head correctly rounded, tail the
+// rounded remainder), k = 0..16.
+constexpr DD kSinTaylor[17] = {
+    {0x1.0000000000000p+0, 0x0.0p+0},
+    {-0x1.5555555555555p-3, -0x1.5555555555555p-57},
+    {0x1.1111111111111p-7, 0x1.1111111111111p-63},
+    {-0x1.a01a01a01a01ap-13, -0x1.a01a01a01a01ap-73},
+    {0x1.71de3a556c734p-19, -0x1.c154f8ddc6c00p-73},
+    {-0x1.ae64567f544e4p-26, 0x1.c062e06d1f209p-80},
+    {0x1.6124613a86d09p-33, 0x1.f28e0cc748ebep-87},
+    {-0x1.ae7f3e733b81fp-41, -0x1.1d8656b0ee8cbp-97},
+    {0x1.952c77030ad4ap-49, 0x1.ac981465ddc6cp-103},
+    {-0x1.2f49b46814157p-57, -0x1.2650f61dbdcb4p-112},
+    {0x1.71b8ef6dcf572p-66, -0x1.d043ae40c4647p-120},
+    {-0x1.761b41316381ap-75, 0x1.3423c7d91404fp-130},
+    {0x1.3f3ccdd165fa9p-84, -0x1.58ddadf344487p-139},
+    {-0x1.d1ab1c2dccea3p-94, -0x1.054d0c78aea14p-149},
+    {0x1.259f98b4358adp-103, 0x1.eaf8c39dd9bc5p-157},
+    {-0x1.434d2e783f5bcp-113, -0x1.0b87b91be9affp-167},
+    {0x1.3981254dd0d52p-123, -0x1.2b1f4c8015a2fp-177},
+};
+
+// sin(x) correctly rounded for |x| <= 1.6 (every angle a latitude inside
+// (-90, 90) gives): the Taylor series to x^33 in double-double, whose
+// truncation (x^35/35! < 2^-110 at pi/2) and rounding errors stay below
+// 2^-100 relative, then one rounding to double.  Outside that range (not
+// reached by positions inside a valid box) the device libm's sin.
+__device__ inline double sin_correctly_rounded(double x)
+{
+    if (!(fabs(x) <= 1.6)) return sin(x);
+    const DD y = dd_mul(DD{x, 0.0}, DD{x, 0.0});
+    DD p = kSinTaylor[16];
+#pragma unroll
+    for (int k = 15; k >= 0; --k) p = dd_add(dd_mul(p, y), kSinTaylor[k]);
+    return dd_mul(DD{x, 0.0}, p).hi;
+}
+
+// The device evaluates the sine correctly rounded.  The host libm the
+// reference calls (glibc 2.35 here) returns the other neighbour of the
+// correctly rounded value for about 0.1% of latitudes, those whose exact
+// sine lies within about 0.005 ulp of a rounding midpoint; the factors then
+// differ by one ulp.
+__device__ inline double deg_per_meter_lon(double latitude)
+{
+    double absla = fabs(latitude);
+    if (absla == 90) absla = 89;
+    return 360.0 / (sin_correctly_rounded((90 - absla) * kRefPi / 180) * 40076.0 * 1000.0);
+}
+
+// The same expression on the host, with the host libm's sin as the
+// reference computes it (x_buffer, filter.cpp:30).
+inline double deg_per_meter_lon_host(double latitude)
+{
+    double absla = std::fabs(latitude);
+    if (absla == 90) absla = 89;
+    return 360.0 / (std::sin((90 - absla) * kRefPi / 180) * 40076.0 * 1000.0);
+}
+
+// box::distance(p, true) <= r (geometry.h:174-185): the clamped offsets in
+// degrees divided by the degrees per metre, longitude at the point's
+// latitude (f = deg_per_meter_lon(py)).
+__device__ __forceinline__ bool box_within_geo(const Node &b, double px, double py, double t64, double f)
+{
+    if (px >= b.lx && px <= b.hx && py >= b.ly && py <= b.hy) return true;
+    double dx = fabs(px - (b.lx + b.hx) / 2) - (b.hx - b.lx) / 2;
+    double dy = fabs(py - (b.ly + b.hy) / 2) - (b.hy - b.ly) / 2;
+    dx = dx < 0.0 ? 0.0 : dx;
+    dy = dy < 0.0 ? 0.0 : dy;
+    dy = dy / kDegPerMeterLat;
+    dx = dx / f;
+    return dx * dx + dy * dy <= t64;
+}
+
 __device__ __forceinline__ bool box_within(const Node &b, double px, double py, double t64)
 {
     // box::distance (geometry.h:174-185) <= r, with contain() first.
@@ -337,7 +457,9 @@ __global__ void k_build_collect(DevState s)
         bool want = false;
         if (s.meta[n].type == kLeaf) {
             Node nd = s.nodes[n];
-            want = s.count[n] >= s.cap && (nd.hx - nd.lx) / 2 > s.r;
+            // QTNode::split (QTree.h:158): mbr.width(geography) / 2 > r
+            const double wd = s.geography ? (nd.hx - nd.lx) / deg_per_meter_lon(nd.ly) : nd.hx - nd.lx;
+            want = s.count[n] >= s.cap && wd / 2 > s.r;
         }
         uint32_t idx = wave_append_u32(&s.ctr->n_split, want);
         if (want) s.split_list[idx] = n;
@@ -432,7 +554,9 @@ __global__ void k_collect(DevState s)
         bool split = false, merge = false;
         if (m.type == kLeaf) {
             Node nd = s.nodes[n];
-            if (nd.hy - nd.ly > 2 * s.r && s.count[n] > s.cap) {
+            // height(mbr) > 2r (geometry.cu:1200), height in metres (geometry.cu:30) with geography
+            const double ht = s.geography ? (nd.hy - nd.ly) / kDegPerMeterLat : nd.hy - nd.ly;
+            if (ht > 2 * s.r && s.count[n] > s.cap) {
                 if (++m.overflow >= s.delay) {
                     split = true;
                     m.overflow = 0;
@@ -823,7 +947,8 @@ __global__ __launch_bounds__(kBlock) void k_grid_level(DevState s, uint32_t d)
 // ------------------------------------------------------------ partition
 
 // Descent with last_valid (filter.cpp:229-243): last_valid is the deepest
-// node on the path, below the root, with x + r < high.x && y + r < high.y.
+// node on the path, below the root, with x + x_buffer < high.x &&
+// y + y_buffer < high.y (both r in planar metres).
 // Objects are taken in the leaf order of the tick before (spid), so the
 // lanes of a wave mostly descend the same path (one node line per wave and
 // level) and share a leaf, whose count one atomic per leaf and wave
@@ -835,7 +960,7 @@ __global__ __launch_bounds__(kBlock) void k_assign(DevState s)
     uint32_t nst = 0;
     const uint32_t lane = lane_id();
     const double *xy = s.tp->xy;
-    const double r = s.r;
+    const double xbuf = s.xbuf, ybuf = s.ybuf;   // x_buffer, y_buffer (filter.cpp:30-31)
     const bool root_leaf = s.nodes[0].child[0] == kLeafTag;
     const bool ordered = s.ctr->sorted_valid != 0;
     // Each thread takes two objects per step (positions base and base +
@@ -891,7 +1016,7 @@ __global__ __launch_bounds__(kBlock) void k_assign(DevState s)
                 mid[q] = node_mid(s.nodes, cur[q]);
                 const double2 hi = node_high(s.nodes, cur[q]);
                 ch[q] = node_children(s.nodes, cur[q]);
-                if (x[q] + r < hi.x && y[q] + r < hi.y) lv[q] = cur[q];
+                if (x[q] + xbuf < hi.x && y[q] + ybuf < hi.y) lv[q] = cur[q];
                 more = true;
             }
             if (!more) break;
@@ -1084,6 +1209,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_units(DevState s)
 // Counting sort by leaf, in the order of the assignment: objects that share
 // a leaf were given consecutive ranks there, so the sorted arrays are
 // written in runs.
+template <bool GEO>
 __global__ __launch_bounds__(kBlock) void k_scatter(DevState s)
 {
     const double *xy_prev = s.tp->xy_prev;
@@ -1106,6 +1232,7 @@ __global__ __launch_bounds__(kBlock) void k_scatter(DevState s)
         s.sprev[pos] = ordered ? k : kNone;
         s.tp->sleaf[pos] = leaf;
         s.splf[pos] = s.ord_pleaf[k];
+        if (GEO) s.sgeo[pos] = deg_per_meter_lon(y);   // geography: the object's longitude scale
         double px = 0.0, py = 0.0;
         if (has_prev) {
             px = ordered ? sx_old[k] : xy_prev[2 * (size_t)i];
@@ -1165,14 +1292,14 @@ __device__ __forceinline__ uint32_t wave_rank(uint32_t *p, uint32_t key)
 // The children of a node as the lookup sees them: the four child nodes are
 // read together; accepted leaves are passed to take(leaf) when `takes`, and
 // the mask of branch children within r is returned.
-template <class Take>
-__device__ __forceinline__ uint32_t lookup_children(const DevState &s, uint4 ids, double x, double y, uint32_t own,
-                                                    bool takes, Take &&take)
+template <bool GEO, class Take>
+__device__ __forceinline__ uint32_t lookup_children(const DevState &s, uint4 ids, double x, double y, double f,
+                                                    uint32_t own, bool takes, Take &&take)
 {
     const Node n0 = s.nodes[ids.x], n1 = s.nodes[ids.y], n2 = s.nodes[ids.z], n3 = s.nodes[ids.w];
     uint32_t desc = 0;
     auto one = [&](const Node &cn, uint32_t c, uint32_t bit) {
-        if (!box_within(cn, x, y, s.t64)) return;
+        if (!(GEO ? box_within_geo(cn, x, y, s.t64, f) : box_within(cn, x, y, s.t64))) return;
         if (cn.child[0] != kLeafTag) {
             desc |= bit;
         } else if (takes && (s.complete ? (c != own) : (y < cn.ly || (y < cn.hy && x < cn.lx)))) {
@@ -1193,13 +1320,14 @@ __device__ __forceinline__ uint32_t lookup_children(const DevState &s, uint4 ids
 // 16 levels below `from` are kept as 4-bit masks in one 64-bit register, and
 // the walk climbs back through the parent links (deeper levels recompute
 // their mask from the children), so nothing sits in scratch memory.
-template <class Take>
-__device__ __forceinline__ void lookup_walk(const DevState &s, double x, double y, uint32_t from, uint32_t own,
-                                            Take &&take)
+// GEO: geography, f the degrees of longitude per metre at y.
+template <bool GEO, class Take>
+__device__ __forceinline__ void lookup_walk(const DevState &s, double x, double y, double f, uint32_t from,
+                                            uint32_t own, Take &&take)
 {
     uint32_t node = from;
     uint4 ids = node_children(s.nodes, from);
-    uint32_t desc = lookup_children(s, ids, x, y, own, true, take);
+    uint32_t desc = lookup_children<GEO>(s, ids, x, y, f, own, true, take);
     unsigned long long masks = 0;   // level d (< 16): masks >> 4d
     uint32_t depth = 0;             // of `node` below `from`
     for (;;) {
@@ -1212,7 +1340,7 @@ __device__ __forceinline__ void lookup_walk(const DevState &s, double x, double 
             node = pick_child(ids, (int)ci);
             depth++;
             ids = node_children(s.nodes, node);
-            desc = lookup_children(s, ids, x, y, own, true, take);
+            desc = lookup_children<GEO>(s, ids, x, y, f, own, true, take);
             continue;
         }
         if (node == from) break;
@@ -1227,7 +1355,7 @@ __device__ __forceinline__ void lookup_walk(const DevState &s, double x, double 
             desc = (uint32_t)(masks >> (4 * depth)) & 0xFu;
         } else {
             const uint32_t ci = ids.x == node ? 0u : (ids.y == node ? 1u : (ids.z == node ? 2u : 3u));
-            desc = lookup_children(s, ids, x, y, own, false, take) & ~((2u << ci) - 1u);
+            desc = lookup_children<GEO>(s, ids, x, y, f, own, false, take) & ~((2u << ci) - 1u);
         }
         node = p;
     }
@@ -1239,6 +1367,7 @@ __device__ __forceinline__ void lookup_walk(const DevState &s, double x, double 
 // are kept, stamped with the tick, for the joins of this tick and the next.
 // Objects with up to three candidates (nearly all) write their units from
 // registers after one atomic per wave; the others walk a second time.
+template <bool GEO>
 __global__ __launch_bounds__(kBlock) void k_lookup(DevState s)
 {
     const uint32_t nb = s.ctr->n_border;
@@ -1249,18 +1378,19 @@ __global__ __launch_bounds__(kBlock) void k_lookup(DevState s)
         uint32_t total = 0;
         uint32_t f0 = kNone, f1 = kNone, f2 = kNone;
         uint32_t pid = 0, lv = 0, own = 0, pleaf = kNone, prevk = kNone, pos = 0;
-        double x = 0.0, y = 0.0;
+        double x = 0.0, y = 0.0, f = 0.0;
         if (k < nb) {
             const BorderObj b = s.border[k];
             pid = b.pid;
             lv = b.lv;
             x = b.x;
             y = b.y;
+            if (GEO) f = deg_per_meter_lon(y);
             own = b.leaf;
             pleaf = b.pleaf;
             prevk = b.prevk;
             pos = s.offset[own] + b.rank;
-            lookup_walk(s, x, y, lv, own, [&](uint32_t c) {
+            lookup_walk<GEO>(s, x, y, f, lv, own, [&](uint32_t c) {
                 f0 = total == 0 ? c : f0;
                 f1 = total == 1 ? c : f1;
                 f2 = total == 2 ? c : f2;
@@ -1298,7 +1428,7 @@ __global__ __launch_bounds__(kBlock) void k_lookup(DevState s)
         if (nc > 2) write(f2, r2);
         if (total > 3) {
             u = atomicAdd(&s.ctr->n_units, (unsigned long long)total);
-            lookup_walk(s, x, y, lv, own, [&](uint32_t c) { write(c, atomicAdd(&s.ucount[c], 1u)); });
+            lookup_walk<GEO>(s, x, y, f, lv, own, [&](uint32_t c) { write(c, atomicAdd(&s.ucount[c], 1u)); });
         }
     }
 }
@@ -1555,6 +1685,15 @@ __device__ __forceinline__ void tile_put(JoinTile<TILE> &t, uint32_t k, const De
     const uint32_t pid = s.spid[from];
     t.p[k] = pid;
     if (want_pc) t.pc[k] = cand_at(s.tp->cand_prev, s.sprev[from], s.n, s.tp->prev_tick);
+    if (s.sgeo) {   // geography keeps every contact (no candidate lists): the slot holds the longitude scale
+        const unsigned long long f = __double_as_longlong(s.sgeo[from]);
+        t.pc[k] = make_uint4((uint32_t)f, (uint32_t)(f >> 32), 0u, 0u);
+    }
+}
+
+__device__ __forceinline__ double tile_geo(const uint4 &pc)
+{
+    return __longlong_as_double((long long)(((unsigned long long)pc.y << 32) | pc.x));
 }
 
 __device__ __forceinline__ uint32_t spread3(uint32_t v) { return (v & 1u) | ((v & 2u) << 1) | ((v & 4u) << 2); }
@@ -1696,14 +1835,20 @@ struct TickView {
 // from jbeg): lane i tests the objects i+1 .. i+lim(i) (mod n), lim(i) =
 // (n-1)/2, plus one for i < n/2 when n is even, so each unordered pair is
 // tested once and no lane idles on a lower triangle.
-template <bool USE32, bool BORDER, int TILE, bool ROT = false>
+// GEO (geography, FP64 only): the distance of the reference's test
+// p1->distance(p2, true) (refinement.cpp:33, geometry.h:40-48): dy divided
+// by the degrees of latitude per metre and dx by the degrees of longitude
+// per metre at p1's latitude, p1 the unit's object (border) or the smaller
+// id (inside, where pid < q is tested); fi and pi are the row's scale and id.
+template <bool USE32, bool BORDER, int TILE, bool ROT = false, bool GEO = false>
 __device__ __forceinline__ JoinCounts join_columns(Stager &st, const DevState &s, const TickView &tv,
                                                    const JoinTile<TILE> &C, double xi,
                                                    double yi, float fxi, float fyi, float pfxi, float pfyi,
                                                    uint32_t jbeg, uint32_t jend, float t32_in,
                                                    float t32_out, double t64, const JoinRows &rows,
                                                    bool has_prev, const uint4 *ccand, const uint32_t *cmask,
-                                                   uint32_t leaf, uint4 rpc, uint32_t rpl, const GBox &rbox)
+                                                   uint32_t leaf, uint4 rpc, uint32_t rpl, const GBox &rbox,
+                                                   double fi = 0.0, uint32_t pi = 0)
 {
     uint32_t n_sure = 0, n_pass = 0, band = 0, n_dup = 0, cols = 0;
     // the task's columns are the same for every lane: scalar loop bounds
@@ -1757,7 +1902,9 @@ __device__ __forceinline__ JoinCounts join_columns(Stager &st, const DevState &s
                 const float p32 = dist2_f32(px, py);
                 dec = p32 <= t32_in ? 1 : (p32 > t32_out ? 0 : -1);   // NaN: -1
             }
-            if (dec < 0) {
+            if (GEO) {
+                dec = 1;   // every contact is kept and looked up (the scaled test is not symmetric)
+            } else if (dec < 0) {
                 const uint32_t pa = rows.p[rl], gj = C.g[j];
                 const double *xp = tv.xy_prev;
                 const double dx = xp[2 * (size_t)pa] - s.sxp[gj];
@@ -1845,8 +1992,14 @@ __device__ __forceinline__ JoinCounts join_columns(Stager &st, const DevState &s
                 else
                     stage = hit && (!sure || BORDER || s.store_all || maybe_tagged);
             } else {
-                const double ex = xi - cx[jj].x;
-                const double ey = yi - cx[jj].y;
+                double ex = xi - cx[jj].x;
+                double ey = yi - cx[jj].y;
+                if constexpr (GEO) {
+                    const uint32_t jc = ROT ? cj[jj] : min(jb + jj, jlim - 1);
+                    const double f = BORDER ? fi : (pi < C.p[jc] ? fi : tile_geo(C.pc[jc]));
+                    ey = ey / kDegPerMeterLat;
+                    ex = ex / f;
+                }
                 sure = cand && ex * ex + ey * ey <= t64;
                 stage = sure && (BORDER || s.store_all || has_prev);
             }
@@ -1868,8 +2021,9 @@ __device__ __forceinline__ JoinCounts join_columns(Stager &st, const DevState &s
         for (uint32_t g0 = jbeg; g0 < jend; g0 += kGroup) {
             const uint32_t g1 = min(g0 + kGroup, jend);
             // every lane holds the same boxes: the decision is taken as a scalar
-            if (__builtin_amdgcn_readfirstlane(
-                    (uint32_t)box_out_of_reach<USE32>(rbox, C.box[g0 / kGroup], t32_out, t64)))
+            // (the boxes are in degrees without the scale: no skip with geography)
+            if (!GEO && __builtin_amdgcn_readfirstlane(
+                            (uint32_t)box_out_of_reach<USE32>(rbox, C.box[g0 / kGroup], t32_out, t64)))
                 continue;
             cols += g1 - g0;
             for (uint32_t jb = g0; jb < g1; jb += kFlushRound) round(jb, g1);
@@ -1927,7 +2081,7 @@ __device__ __forceinline__ void triangle_task(uint32_t t, uint32_t n, uint32_t &
 // when the pair was also within reach at prev_tick (the only case that
 // needs the previous table generation) and kTagBorder when it came from a
 // border unit (the only case that can find a pair twice).
-template <int TILE>
+template <int TILE, bool GEO = false>
 __global__ __launch_bounds__(kBlock) void k_join(DevState s)
 {
     __shared__ JoinLds<TILE> lds;
@@ -2041,7 +2195,23 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
             rpl = ulf.x;   // the unit object's leaf, bit 31 when its candidates overflowed
             rows.pl[lane] = rpl;
         }
-        const JoinCounts jc =
+        double fi = 0.0;   // geography: the row object's longitude scale
+        if constexpr (GEO) {
+            if (inside)
+                fi = tile_geo(A.pc[vi ? row : jbeg]);
+            else if (vi)
+                fi = s.sgeo[ulf.w];
+        }
+        const JoinCounts jc = GEO ?
+            ((inside && rot) ? join_columns<false, false, TILE, true, GEO>(
+                                   st, s, tv, C, xi, yi, fxi, fyi, pfxi, pfyi, jbeg, jend, s.t32_in, s.t32_out, t64,
+                                   rows, has_prev, lds.ccand, lds.cmask, leaf, rpc, rpl, rbox, fi, pi)
+             : inside ? join_columns<false, false, TILE, false, GEO>(
+                            st, s, tv, C, xi, yi, fxi, fyi, pfxi, pfyi, jbeg, jend, s.t32_in, s.t32_out, t64, rows,
+                            has_prev, lds.ccand, lds.cmask, leaf, rpc, rpl, rbox, fi, pi)
+                      : join_columns<false, true, TILE, false, GEO>(
+                            st, s, tv, C, xi, yi, fxi, fyi, pfxi, pfyi, jbeg, jend, s.t32_in, s.t32_out, t64, rows,
+                            has_prev, lds.ccand, lds.cmask, leaf, rpc, rpl, rbox, fi, pi)) :
             (inside && rot)
                 ? (use32 ? join_columns<true, false, TILE, true>(st, s, tv, C, xi, yi, fxi, fyi, pfxi, pfyi,
                                                                 jbeg, jend, s.t32_in, s.t32_out, t64, rows, has_prev,
@@ -3046,7 +3216,10 @@ void enqueue_partition(glint_ctx *c)
     launch(c, K_SCAN_REDUCE, [&] { k_scan_reduce<<<nb, kBlock, 0, st>>>(s.count, s.node_cap, s.block_sums); });
     launch(c, K_SCAN_TOP, [&] { k_scan_top<<<1, kBlock, 0, st>>>(s.block_sums, nb); });
     launch(c, K_SCAN_OBJECTS, [&] { k_scan_objects<<<nb, kBlock, 0, st>>>(s); });
-    launch(c, K_SCATTER, [&] { k_scatter<<<gn, kBlock, 0, st>>>(s); });
+    if (s.geography)
+        launch(c, K_SCATTER, [&] { k_scatter<true><<<gn, kBlock, 0, st>>>(s); });
+    else
+        launch(c, K_SCATTER, [&] { k_scatter<false><<<gn, kBlock, 0, st>>>(s); });
 }
 
 void enqueue_lookup(glint_ctx *c)
@@ -3055,7 +3228,10 @@ void enqueue_lookup(glint_ctx *c)
     hipStream_t st = c->stream;
     const uint32_t gnode = grid_for(s.node_cap), nb = c->scan_blocks;
     launch(c, K_FILL_UCOUNT, [&] { k_fill_u32<<<gnode, kBlock, 0, st>>>(s.ucount, 0u, s.node_cap); });
-    launch(c, K_LOOKUP, [&] { k_lookup<<<grid_for(s.n), kBlock, 0, st>>>(s); });
+    if (s.geography)
+        launch(c, K_LOOKUP, [&] { k_lookup<true><<<grid_for(s.n), kBlock, 0, st>>>(s); });
+    else
+        launch(c, K_LOOKUP, [&] { k_lookup<false><<<grid_for(s.n), kBlock, 0, st>>>(s); });
     launch(c, K_SCAN_REDUCE, [&] { k_scan_reduce<<<nb, kBlock, 0, st>>>(s.ucount, s.node_cap, s.block_sums); });
     launch(c, K_SCAN_TOP, [&] { k_scan_top<<<1, kBlock, 0, st>>>(s.block_sums, nb); });
     launch(c, K_SCAN_UNITS, [&] { k_scan_units<<<nb, kBlock, 0, st>>>(s); });
@@ -3066,7 +3242,9 @@ void enqueue_refine(glint_ctx *c)
 {
     DevState &s = c->s;
     hipStream_t st = c->stream;
-    if (c->cfg.join_tile == 256)
+    if (s.geography)
+        launch(c, K_JOIN, [&] { k_join<256, true><<<c->join_blocks, kBlock, 0, st>>>(s); });
+    else if (c->cfg.join_tile == 256)
         launch(c, K_JOIN, [&] { k_join<256><<<c->join_blocks, kBlock, 0, st>>>(s); });
     else
         launch(c, K_JOIN, [&] { k_join<kTile><<<c->join_blocks, kBlock, 0, st>>>(s); });
@@ -3234,6 +3412,12 @@ __global__ __launch_bounds__(kBlock) void k_stamps_to_ms(const unsigned long lon
         ms[k] = (float)((double)(stamps[2 * k + 1] - stamps[2 * k]) / khz);
 }
 
+__global__ __launch_bounds__(kBlock) void k_longitude_scale(const double *lat, double *out, uint64_t n)
+{
+    for (uint64_t k = (uint64_t)blockIdx.x * kBlock + threadIdx.x; k < n; k += (uint64_t)gridDim.x * kBlock)
+        out[k] = deg_per_meter_lon(lat[k]);
+}
+
 }  // namespace
 
 // ====================================================================== C ABI
@@ -3268,6 +3452,16 @@ double glint_reach_threshold(double r)
     return r > 0 ? sqrt_threshold(r) : 0.0;
 }
 
+glint_status glint_longitude_scale_device(const double *d_latitudes, double *d_scale, uint64_t n, uint32_t device)
+{
+    if (n && (!d_latitudes || !d_scale)) return GLINT_ERR_INVALID_ARGUMENT;
+    if (n == 0) return GLINT_OK;
+    if (hipSetDevice((int)device) != hipSuccess) return GLINT_ERR_HIP;
+    k_longitude_scale<<<grid_for(n), kBlock>>>(d_latitudes, d_scale, n);
+    if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) return GLINT_ERR_HIP;
+    return GLINT_OK;
+}
+
 void glint_config_default(glint_config *cfg)
 {
     if (!cfg) return;
@@ -3299,6 +3493,14 @@ glint_status glint_create(const glint_config *cfg, glint_ctx **out)
     }
     if (cfg->precision == GLINT_PRECISION_FP32_GUARD && !(cfg->guard_epsilon > 0)) {
         g_create_error = "guard_epsilon must be positive";
+        return GLINT_ERR_INVALID_ARGUMENT;
+    }
+    if (cfg->geography > 1 ||
+        (cfg->geography && (cfg->precision != GLINT_PRECISION_FP64 ||
+                            cfg->dynamic_schema == GLINT_SCHEMA_GRID_REBUILD || cfg->low_y <= -90.0 ||
+                            cfg->high_y >= 90.0))) {
+        g_create_error = "geography needs FP64 precision, a schema other than the grid rebuild and latitudes "
+                         "inside (-90, 90)";
         return GLINT_ERR_INVALID_ARGUMENT;
     }
     glint_ctx *c = new (std::nothrow) glint_ctx();
@@ -3353,7 +3555,13 @@ glint_status glint_create(const glint_config *cfg, glint_ctx **out)
     // at the tick before needs no entry: it can only continue a run (found
     // again from the leaves of the tick before) or end unemitted, unless
     // min_meet_time is 0 or every contact is kept for glint_contacts
-    s.store_all = (cfg->record_contacts || cfg->min_meet_time == 0) ? 1u : 0u;
+    s.store_all = (cfg->record_contacts || cfg->min_meet_time == 0 || cfg->geography) ? 1u : 0u;
+    // with geography the scaled pair distance depends on which object is p1,
+    // so a pair's state at the tick before cannot be told from positions
+    // alone: every contact is stored and looked up
+    s.geography = cfg->geography;
+    s.xbuf = cfg->geography ? cfg->reach_distance * deg_per_meter_lon_host(cfg->low_y) : cfg->reach_distance;
+    s.ybuf = cfg->geography ? cfg->reach_distance * kDegPerMeterLat : cfg->reach_distance;
     s.delay = c->cfg.schema_update_delay;
     s.lx = cfg->low_x;
     s.ly = cfg->low_y;
@@ -3477,6 +3685,7 @@ glint_status glint_create(const glint_config *cfg, glint_ctx **out)
     ALLOC(c->hist[1], 2 * N);
     ALLOC(s.spid, N);
     ALLOC(s.sprev, N);
+    if (cfg->geography) ALLOC(s.sgeo, N);
     ALLOC(s.order, N);
     ALLOC(s.ord_leaf, N);
     ALLOC(s.ord_rank, N);

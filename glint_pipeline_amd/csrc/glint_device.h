@@ -196,6 +196,9 @@ struct DevState {
     double t64;         // largest s with sqrt(s) <= r (host-computed)
     float t32_in;       // FP32 squared-distance bounds of the guard band
     float t32_out;
+    uint32_t geography; // lon/lat degrees with the reference's scaled predicates (geometry.h:40-48,174-185)
+    double xbuf, ybuf;  // last_valid margins (filter.cpp:30-31): r, or r in degrees at the root's low latitude
+    double *sgeo;       // geography: degrees of longitude per metre at each sorted object's latitude
     double w32_max;     // widest leaf decided in FP32 (error bound inside the band)
     double pf_lim;      // largest FP32 offset with the same error bound (w32_max + r)
     unsigned long long unit_cap;

@@ -75,6 +75,8 @@ class Config(ctypes.Structure):
         ("meeting_log_capacity", ctypes.c_uint64),
         ("join_tile", ctypes.c_uint32),
         ("table_threads", ctypes.c_uint32),
+        ("geography", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
     ]
 
 
@@ -195,6 +197,7 @@ SIGNATURES = {
     "glint_tr_read_chunk": (_I, [ctypes.c_char_p, _U32, _U32, _VP]),
     "glint_projection_for_box": (_I, [ctypes.c_double, ctypes.c_double, ctypes.POINTER(Projection)]),
     "glint_project_device": (_I, [ctypes.POINTER(Projection), _VP, _VP, _U64, _U32]),
+    "glint_longitude_scale_device": (_I, [_VP, _VP, _U64, _U32]),
     "glint_batch_run_ticks": (_I, [_VP, _VP, _U32, _U32, _VP]),
     "glint_meeting_log_info": (_I, [_VP, ctypes.POINTER(_U64), ctypes.POINTER(_U64)]),
     "glint_meeting_log_read": (_I, [_VP, _U64, _VP, _U64, ctypes.POINTER(_U64)]),
@@ -286,6 +289,12 @@ def project_device(proj: Projection, d_lonlat: int, d_xy: int, n: int, device: i
     _check(lib().glint_project_device(ctypes.byref(proj), d_lonlat, d_xy, n, device), "glint_project_device")
 
 
+def longitude_scale_device(d_latitudes: int, d_scale: int, n: int, device: int = 0) -> None:
+    """Degrees per metre of longitude at n latitudes (FP64, device memory),
+    computed by the device function the geography predicates use."""
+    _check(lib().glint_longitude_scale_device(d_latitudes, d_scale, n, device), "glint_longitude_scale_device")
+
+
 class DeviceBuffer:
     """Raw device allocation owned by Python (no framework needed)."""
 
@@ -329,7 +338,7 @@ def make_config(num_objects: int, box, *, reach_distance: float = 2.0, grid_capa
                 contact_capacity: int = 0, device: int = 0, max_meet_time: int = 4096,
                 graph_ticks: int = 0, kv_capacity: int = 0, kv_split_num: int = 10,
                 kv_area_limit: float = 100.0, meeting_log_capacity: int = 0, join_tile: int = 0,
-                table_threads: int = 0) -> Config:
+                table_threads: int = 0, geography: bool = False) -> Config:
     c = Config()
     lib().glint_config_default(ctypes.byref(c))
     c.num_objects = num_objects
@@ -358,6 +367,7 @@ def make_config(num_objects: int, box, *, reach_distance: float = 2.0, grid_capa
     c.meeting_log_capacity = meeting_log_capacity
     c.join_tile = join_tile
     c.table_threads = table_threads
+    c.geography = int(geography)
     return c
 
 

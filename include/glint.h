@@ -101,6 +101,13 @@ typedef struct glint_config {
     uint64_t meeting_log_capacity; /* meetings kept in the device meeting log ring; 0: log off */
     uint32_t join_tile;            /* points per LDS tile of the distance join: 256 or 512 (0: 512) */
     uint32_t table_threads;        /* workgroup size of the table update: 256, 512 or 1024 (0: 512) */
+    uint32_t geography;            /* 1: coordinates are lon/lat degrees and the predicates are the
+                                      reference's geography=true forms (geometry.h:40-48,159-185,
+                                      filter.cpp:30-31): the pair distance scales dx at the latitude
+                                      of the unit's object, box distances at the point's latitude,
+                                      splits use widths and heights in metres.  FP64 only, not with
+                                      GLINT_SCHEMA_GRID_REBUILD; every contact is kept */
+    uint32_t reserved;
 } glint_config;
 
 /* One emitted meeting: pid_a < pid_b, first and last tick of contact. */
@@ -149,6 +156,14 @@ const char *glint_last_error_message(const glint_ctx *ctx);
  * s <= glint_reach_threshold(r) exactly when sqrt(s) <= r (the reference's
  * test, geometry.h:47 with refinement.cpp:33).  Host-only. */
 double glint_reach_threshold(double r);
+
+/* Degrees per metre of longitude at each of n latitudes (device memory in
+ * and out), computed on `device` by the same function the geography
+ * predicates use: util.h:57-66's expression, PI = 3.14159265, 90 degrees
+ * taken as 89.  The predicates are exact only where these equal the host's
+ * libm results bit for bit, which this lets a caller check on its own data. */
+glint_status glint_longitude_scale_device(const double *d_latitudes, double *d_scale, uint64_t n,
+                                          uint32_t device);
 
 /* Defaults of the reference (config.h:29-41): capacity 50, min 5, r = 2 m,
  * delay 1, dynamic schema on; the caller sets num_objects and the box. */

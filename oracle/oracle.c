@@ -98,6 +98,28 @@ int oracle_point_within(double x1, double y1, double x2, double y2, double r)
     return sqrt(dx * dx + dy * dy) <= r;
 }
 
+/* util.h:38,57: the reference's PI and degrees of latitude per metre */
+#define ORACLE_REF_PI 3.14159265
+static const double kDegPerMeterLat = 360.0 / (40076.0 * 1000.0);
+
+double oracle_degree_per_meter_longitude(double latitude)
+{
+    /* degree_per_meter_longitude (util.h:59-66); 90 is taken as 89 */
+    double absla = fabs(latitude);
+    if (absla == 90) absla = 89;
+    return 360.0 / (sin((90 - absla) * ORACLE_REF_PI / 180) * 40076.0 * 1000.0);
+}
+
+int oracle_point_within_geo(double x1, double y1, double x2, double y2, double r)
+{
+    /* Point::distance with geography=true (geometry.h:40-48): p1 = (x1, y1) */
+    double dx = x1 - x2;
+    double dy = y1 - y2;
+    dy = dy / kDegPerMeterLat;
+    dx = dx / oracle_degree_per_meter_longitude(y1);
+    return sqrt(dx * dx + dy * dy) <= r;
+}
+
 static int box_contains(const onode *n, double px, double py)
 {
     /* box::contain(Point&) (geometry.h:140-145) */
@@ -115,10 +137,28 @@ double oracle_box_distance(double lx, double ly, double hx, double hy, double px
     return sqrt(dx * dx + dy * dy);
 }
 
-static double node_distance(const onode *n, double px, double py)
+double oracle_box_distance_geo(double lx, double ly, double hx, double hy, double px, double py)
+{
+    /* box::distance(Point&, true) (geometry.h:174-185) */
+    if (px >= lx && px <= hx && py >= ly && py <= hy) return 0;
+    double dx = maxd(fabs(px - (lx + hx) / 2) - (hx - lx) / 2, 0.0);
+    double dy = maxd(fabs(py - (ly + hy) / 2) - (hy - ly) / 2, 0.0);
+    dy = dy / kDegPerMeterLat;
+    dx = dx / oracle_degree_per_meter_longitude(py);
+    return sqrt(dx * dx + dy * dy);
+}
+
+static double node_distance(const octx *c, const onode *n, double px, double py)
 {
     if (box_contains(n, px, py)) return 0;
+    if (c->cfg.geography) return oracle_box_distance_geo(n->lo[0], n->lo[1], n->hi[0], n->hi[1], px, py);
     return oracle_box_distance(n->lo[0], n->lo[1], n->hi[0], n->hi[1], px, py);
+}
+
+/* The pair test of reachability_unit (refinement.cpp:33): p1 is the unit's object. */
+static int pair_within(const octx *c, double x1, double y1, double x2, double y2, double r)
+{
+    return c->cfg.geography ? oracle_point_within_geo(x1, y1, x2, y2, r) : oracle_point_within(x1, y1, x2, y2, r);
 }
 
 static uint32_t alloc_node(octx *c)
@@ -156,6 +196,8 @@ static void build_rec(octx *c, uint32_t node, uint32_t *idx, uint32_t n, uint32_
 {
     onode nd = c->nodes[node];
     double width = nd.hi[0] - nd.lo[0];
+    /* box::width(geography) (geometry.h:166-172): metres at the low latitude */
+    if (c->cfg.geography) width = width / oracle_degree_per_meter_longitude(nd.lo[1]);
     /* QTNode::split (QTree.h:158-159): object_index >= cap && width/2 > r */
     if (!(n >= c->cfg.grid_capacity && width / 2 > c->cfg.reach_distance)) {
         c->nodes[node].type = T_LEAF;
@@ -343,7 +385,10 @@ static int grid_build(octx *c, const double *xy)
 static uint32_t descend(const octx *c, double x, double y, uint32_t *last_valid)
 {
     uint32_t cur = 0, lv = 0;
+    /* x_buffer, y_buffer (filter.cpp:30-31): r, or r in degrees at the root's low latitude */
     const double r = c->cfg.reach_distance;
+    const double xb = c->cfg.geography ? r * oracle_degree_per_meter_longitude(c->cfg.low_y) : r;
+    const double yb = c->cfg.geography ? r * kDegPerMeterLat : r;
     if (c->nodes[0].type == T_LEAF) {   /* single-leaf tree: nothing to look up */
         *last_valid = 0;
         return 0;
@@ -353,7 +398,7 @@ static uint32_t descend(const octx *c, double x, double y, uint32_t *last_valid)
         int ch = (y > n->mid[1]) * 2 + (x > n->mid[0]);
         cur = n->child[ch];
         const onode *m = &c->nodes[cur];
-        if (x + r < m->hi[0] && y + r < m->hi[1]) lv = cur;   /* filter.cpp:235-238 */
+        if (x + xb < m->hi[0] && y + yb < m->hi[1]) lv = cur;   /* filter.cpp:235-238 */
         if (m->type == T_LEAF) break;
     }
     *last_valid = lv;
@@ -382,7 +427,7 @@ static void lookup(const octx *c, double x, double y, uint32_t node, uint32_t ow
     for (int i = 0; i < 4; i++) {
         uint32_t ch = c->nodes[node].child[i];
         const onode *m = &c->nodes[ch];
-        double dist = node_distance(m, x, y);
+        double dist = node_distance(c, m, x, y);
         if (dist <= r) {
             if (m->type == T_LEAF) {
                 if (c->cfg.complete_mode) {
@@ -603,8 +648,10 @@ static void update_schema(octx *c)
     for (uint32_t n = 0; n < c->nnodes; n++) {
         onode *nd = &c->nodes[n];
         if (nd->type == T_LEAF) {
-            /* geometry.cu:1200-1210 (height(), which equals width() on square boxes) */
-            if (nd->hi[1] - nd->lo[1] > 2 * r && c->count[n] > cap) {
+            /* geometry.cu:1200-1210 (height(), which equals width() on square boxes;
+             * in metres with geography, geometry.cu:30) */
+            const double ht = c->cfg.geography ? (nd->hi[1] - nd->lo[1]) / kDegPerMeterLat : nd->hi[1] - nd->lo[1];
+            if (ht > 2 * r && c->count[n] > cap) {
                 if (++nd->overflow >= delay) {
                     splits[ns++] = n;
                     nd->overflow = 0;
@@ -646,6 +693,7 @@ void *oracle_create(const oracle_config *cfg)
 {
     if (!cfg || cfg->num_objects == 0 || cfg->grid_capacity == 0) return NULL;
     if (!(cfg->reach_distance > 0)) return NULL;
+    if (cfg->geography && cfg->dynamic_schema == ORACLE_SCHEMA_GRID_REBUILD) return NULL;
     if (!(cfg->high_x > cfg->low_x) || !(cfg->high_y > cfg->low_y)) return NULL;
     octx *c = (octx *)calloc(1, sizeof(octx));
     c->cfg = *cfg;
@@ -752,7 +800,7 @@ int oracle_tick(void *h, const double *xy, uint32_t tick)
                 uint32_t q = c->members[k];
                 if (pid < q) {
                     ntests++;
-                    if (oracle_point_within(px, py, xy[2 * (size_t)q], xy[2 * (size_t)q + 1], r))
+                    if (pair_within(c, px, py, xy[2 * (size_t)q], xy[2 * (size_t)q + 1], r))
                         ubuf_push(mine, ((uint64_t)pid << 32) | q);
                 }
             }
@@ -769,7 +817,7 @@ int oracle_tick(void *h, const double *xy, uint32_t tick)
                 for (uint32_t k = c->mstart[l2]; k < c->mstart[l2] + c->count[l2]; k++) {
                     uint32_t q = c->members[k];
                     ntests++;
-                    if (oracle_point_within(px, py, xy[2 * (size_t)q], xy[2 * (size_t)q + 1], r)) {
+                    if (pair_within(c, px, py, xy[2 * (size_t)q], xy[2 * (size_t)q + 1], r)) {
                         uint32_t a = pid < q ? pid : q, b = pid < q ? q : pid;
                         ubuf_push(mine, ((uint64_t)a << 32) | b);
                     }

@@ -17,7 +17,8 @@
  *                   merge_qtree: dynamic_schema off on the GPU)
  *   predicates      src/geometry/geometry.h:40-48,140-145,174-185
  *   tick order      src/tracing/process.cpp:431-612
- * with coordinates in planar metres (geography=false predicates) and the
+ * with coordinates in planar metres (geography=false predicates), or lon/lat
+ * degrees with the geography=true predicates (oracle_config.geography), and the
  * deterministic semantics the reference leaves undefined: no leaf-overflow
  * drops, one table entry per unordered pair, unbounded probing, schema
  * decisions taken from the counts of the tick before any split or merge.
@@ -51,6 +52,8 @@ typedef struct oracle_config {
     uint32_t start_time;           /* first tick number, config.h:28 */
     double low_x, low_y, high_x, high_y;   /* root box */
     double reach_distance;         /* config.h:41 */
+    uint32_t geography;            /* 1: lon/lat degrees with the geography=true predicates */
+    uint32_t pad_;
 } oracle_config;
 
 typedef struct oracle_stats {
@@ -68,6 +71,14 @@ typedef struct oracle_stats {
     uint64_t ref_drop_objects;    /* objects beyond 2*grid_capacity in those leaves */
     double t_filter, t_refine, t_evaluate, t_schema;  /* seconds */
 } oracle_stats;
+
+/* Geography (geometry.h:40-48,159-185; util.h:38,57-66): degrees of
+ * longitude per metre at a latitude, the reference's Point::distance with
+ * geography (p1's latitude), and box::distance with geography (the point's
+ * latitude). */
+double oracle_degree_per_meter_longitude(double latitude);
+int oracle_point_within_geo(double x1, double y1, double x2, double y2, double r);
+double oracle_box_distance_geo(double lx, double ly, double hx, double hy, double px, double py);
 
 void *oracle_create(const oracle_config *cfg);
 void oracle_destroy(void *h);

@@ -32,6 +32,8 @@ class OracleConfig(ctypes.Structure):
         ("high_x", ctypes.c_double),
         ("high_y", ctypes.c_double),
         ("reach_distance", ctypes.c_double),
+        ("geography", ctypes.c_uint32),
+        ("pad_", ctypes.c_uint32),
     ]
 
 
@@ -109,6 +111,12 @@ def lib() -> ctypes.CDLL:
         L.oracle_point_within.restype = ctypes.c_int
         L.oracle_box_distance.argtypes = [ctypes.c_double] * 6
         L.oracle_box_distance.restype = ctypes.c_double
+        L.oracle_degree_per_meter_longitude.argtypes = [ctypes.c_double]
+        L.oracle_degree_per_meter_longitude.restype = ctypes.c_double
+        L.oracle_point_within_geo.argtypes = [ctypes.c_double] * 5
+        L.oracle_point_within_geo.restype = ctypes.c_int
+        L.oracle_box_distance_geo.argtypes = [ctypes.c_double] * 6
+        L.oracle_box_distance_geo.restype = ctypes.c_double
         L.oracle_kv_enable.argtypes = [vp, ctypes.POINTER(OracleKvConfig)]
         L.oracle_kv_pending.argtypes = [vp]
         L.oracle_kv_pending.restype = ctypes.c_uint64
@@ -128,11 +136,11 @@ class Oracle:
 
     def __init__(self, num_objects: int, box, reach_distance: float = 2.0, grid_capacity: int = 200,
                  min_meet_time: int = 5, schema_update_delay: int = 1, dynamic_schema: int = 1,
-                 complete: bool = False, num_threads: int = 0, start_time: int = 0):
+                 complete: bool = False, num_threads: int = 0, start_time: int = 0, geography: bool = False):
         lx, ly, hx, hy = box
         self.cfg = OracleConfig(num_objects, grid_capacity, min_meet_time, schema_update_delay,
                                 int(dynamic_schema), int(complete), num_threads, start_time,
-                                lx, ly, hx, hy, reach_distance)
+                                lx, ly, hx, hy, reach_distance, int(geography), 0)
         self.h = lib().oracle_create(ctypes.byref(self.cfg))
         if not self.h:
             raise ValueError("invalid oracle configuration")
@@ -268,6 +276,18 @@ class Oracle:
 
 def point_within(x1, y1, x2, y2, r) -> bool:
     return bool(lib().oracle_point_within(x1, y1, x2, y2, r))
+
+
+def degree_per_meter_longitude(latitude: float) -> float:
+    return lib().oracle_degree_per_meter_longitude(latitude)
+
+
+def point_within_geo(x1, y1, x2, y2, r) -> bool:
+    return bool(lib().oracle_point_within_geo(x1, y1, x2, y2, r))
+
+
+def box_distance_geo(lx, ly, hx, hy, px, py) -> float:
+    return lib().oracle_box_distance_geo(lx, ly, hx, hy, px, py)
 
 
 def box_distance(lx, ly, hx, hy, px, py) -> float:

@@ -14,17 +14,19 @@ def keys_of(meetings: np.ndarray) -> np.ndarray:
 
 def run_parity(w: gen.Workload, ticks: int, *, check_contacts: bool = True, check_leaves_every: int = 10,
                complete: bool = False, precision: int = runtime.PRECISION_FP64,
-               dynamic_schema: int = runtime.SCHEMA_DYNAMIC, box=None, cfg=None, **cfg_kw) -> dict:
+               dynamic_schema: int = runtime.SCHEMA_DYNAMIC, box=None, cfg=None, positions=None,
+               geography: bool = False, **cfg_kw) -> dict:
     """Runs both paths for `ticks` ticks and asserts identical per-tick contact
     sets, meeting sets and leaf boxes (cfg: a complete configuration to use
-    instead of one made from the arguments).  Returns totals for the caller."""
+    instead of one made from the arguments; positions(t): tick t's positions
+    instead of the workload's walk).  Returns totals for the caller."""
     g = gen.Generator(w)
     box = box or w.box()
     o = Oracle(w.num_objects, box, w.reach_distance, w.grid_capacity, w.min_meet_time, complete=complete,
-               dynamic_schema=dynamic_schema)
+               dynamic_schema=dynamic_schema, geography=geography)
     kw = dict(reach_distance=w.reach_distance, grid_capacity=w.grid_capacity, min_meet_time=w.min_meet_time,
               complete=complete, precision=precision, record_contacts=check_contacts,
-              dynamic_schema=dynamic_schema)
+              dynamic_schema=dynamic_schema, geography=geography)
     kw.update(cfg_kw)
     if cfg is None:
         cfg = runtime.make_config(w.num_objects, box, **kw)
@@ -32,7 +34,7 @@ def run_parity(w: gen.Workload, ticks: int, *, check_contacts: bool = True, chec
     totals = {"meetings": 0, "contacts": 0}
     with runtime.Workbench(cfg) as wb:
         for t in range(ticks):
-            xy = g.positions() if t == 0 else g.step()
+            xy = positions(t) if positions else g.positions() if t == 0 else g.step()
             o.tick(xy, t)
             m = wb.tick(xy, t)
             om = o.meetings()
@@ -55,7 +57,7 @@ def run_parity(w: gen.Workload, ticks: int, *, check_contacts: bool = True, chec
                 assert np.array_equal(b, ob) and np.array_equal(lv, olv), f"tick {t}: leaf boxes differ"
         k, s = wb.active_pairs()
         ok, os_ = o.active()
-        if not cfg.record_contacts and w.min_meet_time != 0:
+        if not (cfg.record_contacts or cfg.geography) and w.min_meet_time != 0:
             # pairs first seen at the last tick are not kept without recorded contacts
             keep = os_ != ticks - 1
             ok, os_ = ok[keep], os_[keep]
@@ -93,7 +95,7 @@ def device_trace(w: gen.Workload, ticks: int):
 
 def run_ticks_parity(w: gen.Workload, runs: list[int], *, precision: int = runtime.PRECISION_FP32_GUARD,
                      dynamic_schema: int = runtime.SCHEMA_DYNAMIC, box=None, trace=None, extra=None,
-                     **cfg_kw) -> dict:
+                     geography: bool = False, **cfg_kw) -> dict:
     """Ticks through glint_run_ticks in calls of `runs` ticks each (a call of
     at least graph_ticks ticks replays the captured graph), without recorded
     contacts unless asked: the path the benchmark times.  Asserts that the
@@ -114,7 +116,7 @@ def run_ticks_parity(w: gen.Workload, runs: list[int], *, precision: int = runti
     else:
         trace, host_tick = trace
     o = Oracle(w.num_objects, box, w.reach_distance, w.grid_capacity, w.min_meet_time,
-               dynamic_schema=dynamic_schema)
+               dynamic_schema=dynamic_schema, geography=geography)
     per_tick, meetings = [], {}
     for t in range(ticks):
         o.tick(host_tick(t), t)
@@ -124,7 +126,8 @@ def run_ticks_parity(w: gen.Workload, runs: list[int], *, precision: int = runti
         if len(m):
             meetings[t] = m.copy()
     kw = dict(reach_distance=w.reach_distance, grid_capacity=w.grid_capacity, min_meet_time=w.min_meet_time,
-              precision=precision, dynamic_schema=dynamic_schema, meeting_log_capacity=1 << 22)
+              precision=precision, dynamic_schema=dynamic_schema, meeting_log_capacity=1 << 22,
+              geography=geography)
     kw.update(cfg_kw)
     cfg = runtime.make_config(w.num_objects, box, **kw)
     with runtime.Workbench(cfg) as wb:
@@ -153,7 +156,7 @@ def run_ticks_parity(w: gen.Workload, runs: list[int], *, precision: int = runti
         assert st["ref_drop_leaves"] == per_tick[-1]["ref_drop_leaves"]
         k, s = wb.active_pairs()
         ok, os_ = o.active()
-        if not cfg.record_contacts and w.min_meet_time != 0:
+        if not (cfg.record_contacts or cfg.geography) and w.min_meet_time != 0:
             keep = os_ != ticks - 1
             ok, os_ = ok[keep], os_[keep]
         assert np.array_equal(k, ok) and np.array_equal(s, os_), "active pairs after the last tick differ"
@@ -192,7 +195,7 @@ def run_positions_parity(xys: list, box, *, grid_capacity: int, reach: float = 2
             out.append(st)
         k, s = wb.active_pairs()
         ok, os_ = o.active()
-        if not cfg.record_contacts and min_meet != 0:
+        if not (cfg.record_contacts or cfg.geography) and min_meet != 0:
             keep = os_ != len(xys) - 1
             ok, os_ = ok[keep], os_[keep]
         assert np.array_equal(k, ok) and np.array_equal(s, os_), "active pairs after the last tick differ"
