@@ -42,7 +42,7 @@ constexpr int kTile = 512;                          // points per LDS tile of th
 constexpr int kStage = 432;                         // staged contacts per wave
 constexpr int kFlushRound = 4;                      // join columns between stage checks
 constexpr int kMeetStage = 128;                     // staged meetings per wave (table update)
-constexpr uint32_t kChunk = 4096;                   // contact-list entries reserved per atomic
+constexpr uint32_t kChunk = 4096;                   // most contact-list entries reserved per atomic
 constexpr int kRegion = 2048;                       // meeting-table slots per region (LDS-resident)
 constexpr uint32_t kRegionFill = 1024;              // stored contacts per region the table shape aims at
 constexpr uint32_t kMaxRegionBits = 21;             // at most 2^21 regions (coarse 2^10 x fine 2^11)
@@ -215,14 +215,72 @@ __device__ __forceinline__ void wave_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// ------------------------------------------------------------ fused launches
+//
+// Every kernel of the tick sequence is a body run by one workgroup for one
+// context: `blk` is the workgroup's index among that context's workgroups
+// and their count.  k_<name> runs one context, its state in the kernel
+// arguments; kf_<name> runs many contexts in one launch (independent
+// scenarios of a batch): workgroups [start[i], start[i+1]) run member i,
+// whose state is read from device memory.  The members' shares are the
+// grids their own launches would have, so a fused launch does the same work
+// as the members' launches one after another, without the gaps between them.
+
+// The workgroup's place in a fused launch.
+struct Blk {
+    uint32_t id_, count_;
+    __device__ __forceinline__ uint32_t id() const { return id_; }
+    __device__ __forceinline__ uint32_t count() const { return count_; }
+};
+
+// The workgroup's place in a launch of its own: the hardware's indices, read
+// where they are used.
+struct ThisGrid {
+    __device__ __forceinline__ uint32_t id() const { return blockIdx.x; }
+    __device__ __forceinline__ uint32_t count() const { return gridDim.x; }
+};
+
+__device__ __forceinline__ ThisGrid this_blk() { return ThisGrid{}; }
+
+constexpr uint32_t kMaxFused = 64;   // contexts per fused launch
+
+struct Many {
+    const DevState *__restrict__ ss;   // member states (device memory)
+    uint32_t n;                        // members
+    uint32_t start[kMaxFused + 1];     // first workgroup of each member; start[n] = the grid
+};
+
+struct Member {
+    uint32_t i;
+    Blk blk;
+};
+
+// The member running this workgroup: the last i with start[i] <= blockIdx.x
+// (members with no workgroups are skipped over).
+__device__ __forceinline__ Member member_of(const Many &m)
+{
+    const uint32_t b = blockIdx.x;
+    uint32_t lo = 0, hi = m.n;   // start[lo] <= b < start[hi]
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (m.start[mid] <= b)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return Member{lo, Blk{b - m.start[lo], m.start[lo + 1] - m.start[lo]}};
+}
+
 // ------------------------------------------------------------ small kernels
 
-__global__ void k_fill_u32(uint32_t *p, uint32_t v, uint64_t n)
+template <class P>
+__device__ __forceinline__ void fill_body(uint32_t *p, uint32_t v, uint64_t n, const P blk)
 {
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
-         i += (uint64_t)gridDim.x * blockDim.x)
+    for (uint64_t i = blk.id() * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)blk.count() * blockDim.x)
         p[i] = v;
 }
+
+__global__ void k_fill_u32(uint32_t *p, uint32_t v, uint64_t n) { fill_body(p, v, n, this_blk()); }
 
 __global__ void k_set_tick(TickParams *tp, const double *xy, TableGen tab_old, TableGen tab_cur, float4 *mbr_old,
                            float4 *mbr_cur, double *xy_prev, double *xy_save, uint4 *cand_prev, uint4 *cand,
@@ -258,7 +316,7 @@ __global__ void k_set_tick(TickParams *tp, const double *xy, TableGen tab_old, T
 // a run the positions at prev_tick are the trace's previous snapshot; only
 // the run's last tick copies its positions into the history buffer, for the
 // tick after the run.
-__global__ void k_advance_tick(TickParams *tp, uint64_t stride)
+__device__ __forceinline__ void advance_tick(TickParams *tp, uint64_t stride)
 {
     uint32_t k = tp->trace_index + 1;
     tp->trace_index = k;
@@ -292,6 +350,15 @@ __global__ void k_advance_tick(TickParams *tp, uint64_t stride)
     q = tp->sy_old;
     tp->sy_old = tp->sy;
     tp->sy = q;
+}
+
+__global__ void k_advance_tick(TickParams *tp, uint64_t stride) { advance_tick(tp, stride); }
+
+__global__ void kf_advance_tick(const Many m)
+{
+    const Member w = member_of(m);
+    const DevState &s = m.ss[w.i];
+    if (threadIdx.x == 0) advance_tick(s.tp, 2ull * s.n);
 }
 
 // State before the first tick of a run, one k_advance_tick short of it; the
@@ -332,13 +399,22 @@ __global__ void k_begin_run(TickParams *tp, const double *base, TableGen tab_old
 // Tick k of a run is stamped at its start (k_advance_tick, entry 2k) and at
 // its end (entry 2k+1), so a tick's latency excludes time spent waiting for
 // other work between ticks.
-__global__ void k_stamp(TickParams *tp)
+__device__ __forceinline__ void stamp_end(TickParams *tp)
 {
     uint32_t k = tp->trace_index;
     if (tp->stamps && 2 * k + 1 < tp->nstamps) tp->stamps[2 * k + 1] = wall_clock64();
 }
 
-__global__ void k_reset_tick(DevState s)
+__global__ void k_stamp(TickParams *tp) { stamp_end(tp); }
+
+__global__ void kf_stamp(const Many m)
+{
+    const Member w = member_of(m);
+    if (threadIdx.x == 0) stamp_end(m.ss[w.i].tp);
+}
+
+template <class P>
+__device__ __forceinline__ void reset_tick_body(const DevState s, const P blk)
 {
     Counters *c = s.ctr;
     c->n_units = 0;
@@ -363,6 +439,13 @@ __global__ void k_reset_tick(DevState s)
     c->n_evals = 0;
     c->work_inside = 0;
     c->g_heavy = 0;
+}
+
+__global__ void k_reset_tick(DevState s) { reset_tick_body(s, this_blk()); }
+__global__ void kf_reset_tick(const Many m)
+{
+    const Member w = member_of(m);
+    reset_tick_body(m.ss[w.i], w.blk);
 }
 
 // ------------------------------------------------------------ tree build
@@ -469,10 +552,11 @@ __global__ void k_build_collect(DevState s)
 // Split the listed leaves one level.  build_boxes selects the child boxes of
 // the initial build, (lo, mid) and (mid, hi) (QTree.h:163-166); otherwise the
 // elastic update's lo + half and mid + half (geometry.cu:1151-1156).
-__global__ void k_split(DevState s, int build_boxes)
+template <class P>
+__device__ __forceinline__ void split_body(const DevState s, const P blk, int build_boxes)
 {
     uint32_t ns = s.ctr->n_split;
-    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < ns; k += gridDim.x * blockDim.x) {
+    for (uint32_t k = blk.id() * blockDim.x + threadIdx.x; k < ns; k += blk.count() * blockDim.x) {
         uint32_t node = s.split_list[k];
         int32_t top = atomicSub(&s.ctr->free_top, 4);
         if (top < 4) {
@@ -519,11 +603,19 @@ __global__ void k_split(DevState s, int build_boxes)
     }
 }
 
+__global__ void k_split(DevState s, int build_boxes) { split_body(s, this_blk(), build_boxes); }
+__global__ void kf_split(const Many m, int build_boxes)
+{
+    const Member w = member_of(m);
+    split_body(m.ss[w.i], w.blk, build_boxes);
+}
+
 // geometry.cu:1166-1190: the four leaf children return to the pool.
-__global__ void k_merge(DevState s)
+template <class P>
+__device__ __forceinline__ void merge_body(const DevState s, const P blk)
 {
     uint32_t nm = s.ctr->n_merge;
-    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nm; k += gridDim.x * blockDim.x) {
+    for (uint32_t k = blk.id() * blockDim.x + threadIdx.x; k < nm; k += blk.count() * blockDim.x) {
         uint32_t node = s.merge_list[k];
         Node p = s.nodes[node];
         int32_t top = atomicAdd(&s.ctr->free_top, 4);
@@ -546,10 +638,18 @@ __global__ void k_merge(DevState s)
     }
 }
 
-// geometry.cu:1193-1234: decisions from this tick's counts, before any change.
-__global__ void k_collect(DevState s)
+__global__ void k_merge(DevState s) { merge_body(s, this_blk()); }
+__global__ void kf_merge(const Many m)
 {
-    for (uint32_t n = blockIdx.x * blockDim.x + threadIdx.x; n < s.node_cap; n += gridDim.x * blockDim.x) {
+    const Member w = member_of(m);
+    merge_body(m.ss[w.i], w.blk);
+}
+
+// geometry.cu:1193-1234: decisions from this tick's counts, before any change.
+template <class P>
+__device__ __forceinline__ void collect_body(const DevState s, const P blk)
+{
+    for (uint32_t n = blk.id() * blockDim.x + threadIdx.x; n < s.node_cap; n += blk.count() * blockDim.x) {
         NodeMeta m = s.meta[n];
         bool split = false, merge = false;
         if (m.type == kLeaf) {
@@ -593,8 +693,16 @@ __global__ void k_collect(DevState s)
     }
 }
 
+__global__ void k_collect(DevState s) { collect_body(s, this_blk()); }
+__global__ void kf_collect(const Many m)
+{
+    const Member w = member_of(m);
+    collect_body(m.ss[w.i], w.blk);
+}
+
 // End of a tick: schema change counts and the running totals.
-__global__ void k_schema_done(DevState s)
+template <class P>
+__device__ __forceinline__ void schema_done_body(const DevState s, const P blk)
 {
     Counters *c = s.ctr;
     c->last_splits = c->n_split;
@@ -622,14 +730,29 @@ __global__ void k_schema_done(DevState s)
     if (s.mlog) c->n_mlog += c->n_meet < s.meet_cap ? c->n_meet : s.meet_cap;
 }
 
+__global__ void k_schema_done(DevState s) { schema_done_body(s, this_blk()); }
+__global__ void kf_schema_done(const Many m)
+{
+    const Member w = member_of(m);
+    schema_done_body(m.ss[w.i], w.blk);
+}
+
 // The tick's meetings appended to the meeting log ring (k_schema_done
 // advances its total afterwards).
-__global__ __launch_bounds__(kBlock) void k_log_append(DevState s)
+template <class P>
+__device__ __forceinline__ void log_append_body(const DevState s, const P blk)
 {
     const unsigned long long n = min(s.ctr->n_meet, s.meet_cap), base = s.ctr->n_mlog;
-    for (unsigned long long i = blockIdx.x * (unsigned long long)kBlock + threadIdx.x; i < n;
-         i += (unsigned long long)gridDim.x * kBlock)
+    for (unsigned long long i = blk.id() * (unsigned long long)kBlock + threadIdx.x; i < n;
+         i += (unsigned long long)blk.count() * kBlock)
         s.mlog[(base + i) % s.mlog_cap] = s.meet[i];
+}
+
+__global__ __launch_bounds__(kBlock) void k_log_append(DevState s) { log_append_body(s, this_blk()); }
+__global__ __launch_bounds__(kBlock) void kf_log_append(const Many m)
+{
+    const Member w = member_of(m);
+    log_append_body(m.ss[w.i], w.blk);
 }
 
 // ------------------------------------------------------------ grid rebuild
@@ -659,6 +782,20 @@ constexpr uint32_t kCoarseDepth = 10;
 constexpr uint32_t kCoarseDim = 1u << kCoarseDepth;
 constexpr uint32_t kFineDim = kGridDim / kCoarseDim;       // cells per coarse block side
 constexpr uint32_t kFineCells = kFineDim * kFineDim;
+
+// Per-context arrays cleared by the tick sequence.
+enum FillArray : int { kFillCount, kFillUcount, kFillMeta, kFillCoarse };
+
+template <int F>
+__global__ void kf_fill(const Many m)
+{
+    const Member w = member_of(m);
+    const DevState &s = m.ss[w.i];
+    if (F == kFillCount) fill_body(s.count, 0u, s.node_cap, w.blk);
+    if (F == kFillUcount) fill_body(s.ucount, 0u, s.node_cap, w.blk);
+    if (F == kFillMeta) fill_body(reinterpret_cast<uint32_t *>(s.meta), 0u, 4ull * s.node_cap, w.blk);
+    if (F == kFillCoarse) fill_body(s.g10, 0u, kCoarseDim * kCoarseDim, w.blk);
+}
 
 __host__ __device__ constexpr uint32_t pyr_off(uint32_t d) { return ((1u << (2 * d)) - 1u) / 3u; }
 
@@ -691,11 +828,12 @@ __device__ __forceinline__ uint32_t grid_object(const DevState &s, uint32_t k, b
 // in 32-bit arithmetic, so x = 16384 (a point on the high x edge) is the
 // first cell of the next row.  y = 16384 has no cell (the reference writes
 // past its histogram there): flagged.
-__global__ __launch_bounds__(kBlock) void k_grid_cells(DevState s)
+template <class P>
+__device__ __forceinline__ void grid_cells_body(const DevState s, const P blk)
 {
     const double *xy = s.tp->xy;
     const bool ordered = s.ctr->sorted_valid != 0;
-    for (uint32_t base = blockIdx.x * kBlock; base < s.n; base += gridDim.x * kBlock) {
+    for (uint32_t base = blk.id() * kBlock; base < s.n; base += blk.count() * kBlock) {
         const uint32_t k = base + threadIdx.x;
         uint32_t key = kNone;
         if (k < s.n) {
@@ -723,14 +861,22 @@ __global__ __launch_bounds__(kBlock) void k_grid_cells(DevState s)
     }
 }
 
+__global__ __launch_bounds__(kBlock) void k_grid_cells(DevState s) { grid_cells_body(s, this_blk()); }
+__global__ __launch_bounds__(kBlock) void kf_grid_cells(const Many m)
+{
+    const Member w = member_of(m);
+    grid_cells_body(m.ss[w.i], w.blk);
+}
+
 // One workgroup per 32x32 patch of coarse blocks (1024 workgroups of 256):
 // depths 9..5 of the pyramid, and a fine histogram for each coarse block
 // over capacity.
-__global__ __launch_bounds__(kBlock) void k_grid_pyramid(DevState s)
+template <class P>
+__device__ __forceinline__ void grid_pyramid_body(const DevState s, const P blk)
 {
     __shared__ uint32_t lv[256];
     const uint32_t t = threadIdx.x;
-    const uint32_t px = blockIdx.x & 31, py = blockIdx.x >> 5;
+    const uint32_t px = blk.id() & 31, py = blk.id() >> 5;
     const uint32_t gx = px * 16 + (t & 15), gy = py * 16 + (t >> 4);   // depth-9 block
     uint32_t sum = 0;
     for (uint32_t i = 0; i < 4; i++) {
@@ -768,9 +914,17 @@ __global__ __launch_bounds__(kBlock) void k_grid_pyramid(DevState s)
     }
 }
 
+__global__ __launch_bounds__(kBlock) void k_grid_pyramid(DevState s) { grid_pyramid_body(s, this_blk()); }
+__global__ __launch_bounds__(kBlock) void kf_grid_pyramid(const Many m)
+{
+    const Member w = member_of(m);
+    grid_pyramid_body(m.ss[w.i], w.blk);
+}
+
 // Depths 4..0 from the 32x32 blocks of depth 5, and the root: a branch when
 // the points exceed capacity (the list of depth 0), else one leaf.
-__global__ __launch_bounds__(1024) void k_grid_top(DevState s)
+template <class P>
+__device__ __forceinline__ void grid_top_body(const DevState s, const P blk)
 {
     __shared__ uint32_t lv[1024];
     const uint32_t t = threadIdx.x;
@@ -822,17 +976,33 @@ __global__ __launch_bounds__(1024) void k_grid_top(DevState s)
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_grid_fine_zero(DevState s)
+__global__ __launch_bounds__(1024) void k_grid_top(DevState s) { grid_top_body(s, this_blk()); }
+__global__ __launch_bounds__(1024) void kf_grid_top(const Many m)
+{
+    const Member w = member_of(m);
+    grid_top_body(m.ss[w.i], w.blk);
+}
+
+template <class P>
+__device__ __forceinline__ void grid_fine_zero_body(const DevState s, const P blk)
 {
     const uint64_t n = (uint64_t)min(s.ctr->g_heavy, s.heavy_cap) * kFineCells;
-    for (uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock)
+    for (uint64_t i = blk.id() * (uint64_t)kBlock + threadIdx.x; i < n; i += (uint64_t)blk.count() * kBlock)
         s.gfine[i] = 0;
 }
 
-__global__ __launch_bounds__(kBlock) void k_grid_fine(DevState s)
+__global__ __launch_bounds__(kBlock) void k_grid_fine_zero(DevState s) { grid_fine_zero_body(s, this_blk()); }
+__global__ __launch_bounds__(kBlock) void kf_grid_fine_zero(const Many m)
+{
+    const Member w = member_of(m);
+    grid_fine_zero_body(m.ss[w.i], w.blk);
+}
+
+template <class P>
+__device__ __forceinline__ void grid_fine_body(const DevState s, const P blk)
 {
     const bool ordered = s.ctr->sorted_valid != 0;
-    for (uint32_t base = blockIdx.x * kBlock; base < s.n; base += gridDim.x * kBlock) {
+    for (uint32_t base = blk.id() * kBlock; base < s.n; base += blk.count() * kBlock) {
         const uint32_t k = base + threadIdx.x;
         uint32_t key = kNone;
         if (k < s.n) {
@@ -845,6 +1015,13 @@ __global__ __launch_bounds__(kBlock) void k_grid_fine(DevState s)
         }
         wave_count(s.gfine, key);
     }
+}
+
+__global__ __launch_bounds__(kBlock) void k_grid_fine(DevState s) { grid_fine_body(s, this_blk()); }
+__global__ __launch_bounds__(kBlock) void kf_grid_fine(const Many m)
+{
+    const Member w = member_of(m);
+    grid_fine_body(m.ss[w.i], w.blk);
 }
 
 // Points in block (bx, by) of depth e (2^e blocks per side).  Below the
@@ -871,14 +1048,16 @@ __device__ uint32_t grid_count(const DevState &s, uint32_t e, uint32_t bx, uint3
 // wave-aggregated reservation; a child over capacity is listed for depth
 // d+1, any other is a leaf.  A failed reservation leaves the block a leaf
 // (and flags the error), so the tree stays well formed.
-__global__ __launch_bounds__(kBlock) void k_grid_level(DevState s, uint32_t d)
+template <class P>
+__device__ __forceinline__ void grid_level_body(const DevState s, const P blk, uint32_t d)
 {
     const uint32_t nb = min(s.ctr->g_list[d], s.glist_cap);
-    const uint2 *in = s.glist[d & 1];
-    uint2 *out = s.glist[(d + 1) & 1];
+    // selects, not an index into the state (which would place it in scratch memory)
+    const uint2 *in = (d & 1) ? s.glist[1] : s.glist[0];
+    uint2 *out = (d & 1) ? s.glist[0] : s.glist[1];
     const double xdim = (double)(1u << d);
     const double wx = s.hx - s.lx, wy = s.hy - s.ly;
-    for (uint32_t base = blockIdx.x * kBlock; base < nb; base += gridDim.x * kBlock) {
+    for (uint32_t base = blk.id() * kBlock; base < nb; base += blk.count() * kBlock) {
         const uint32_t j = base + threadIdx.x;
         const bool act = j < nb;
         const unsigned long long mask = __ballot(act);
@@ -944,6 +1123,13 @@ __global__ __launch_bounds__(kBlock) void k_grid_level(DevState s, uint32_t d)
     }
 }
 
+__global__ __launch_bounds__(kBlock) void k_grid_level(DevState s, uint32_t d) { grid_level_body(s, this_blk(), d); }
+__global__ __launch_bounds__(kBlock) void kf_grid_level(const Many m, uint32_t d)
+{
+    const Member w = member_of(m);
+    grid_level_body(m.ss[w.i], w.blk, d);
+}
+
 // ------------------------------------------------------------ partition
 
 // Descent with last_valid (filter.cpp:229-243): last_valid is the deepest
@@ -953,7 +1139,8 @@ __global__ __launch_bounds__(kBlock) void k_grid_level(DevState s, uint32_t d)
 // lanes of a wave mostly descend the same path (one node line per wave and
 // level) and share a leaf, whose count one atomic per leaf and wave
 // advances.  The order used is kept for the scatter.
-__global__ __launch_bounds__(kBlock) void k_assign(DevState s)
+template <class P>
+__device__ __forceinline__ void assign_body(const DevState s, const P blk)
 {
     __shared__ BorderObj stage[kBlock / 64][kBorderStage];
     BorderObj *st = stage[threadIdx.x / 64];
@@ -967,8 +1154,8 @@ __global__ __launch_bounds__(kBlock) void k_assign(DevState s)
     // stride, both wave-uniform blocks) and descends for both at once, so
     // that the node reads of one overlap those of the other.
     constexpr uint32_t kU = 2;
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t base0 = blockIdx.x * blockDim.x; base0 < s.n; base0 += kU * stride) {
+    const uint32_t stride = blk.count() * blockDim.x;
+    for (uint32_t base0 = blk.id() * blockDim.x; base0 < s.n; base0 += kU * stride) {
         uint32_t kk[kU], id[kU], lv[kU], cur[kU], pleaf[kU];
         bool active[kU], border[kU];
         double x[kU], y[kU];
@@ -1080,6 +1267,13 @@ __global__ __launch_bounds__(kBlock) void k_assign(DevState s)
     }
 }
 
+__global__ __launch_bounds__(kBlock) void k_assign(DevState s) { assign_body(s, this_blk()); }
+__global__ __launch_bounds__(kBlock) void kf_assign(const Many m)
+{
+    const Member w = member_of(m);
+    assign_body(m.ss[w.i], w.blk);
+}
+
 __device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t x)
 {
     uint32_t lane = lane_id();
@@ -1109,18 +1303,44 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *t
 }
 
 // Three-kernel exclusive scan over per-node counters (node_cap entries).
-__global__ __launch_bounds__(kBlock) void k_scan_reduce(const uint32_t *in, uint32_t n, uint32_t *block_sums)
+template <class P>
+__device__ __forceinline__ void scan_reduce_body(const uint32_t *in, uint32_t n, uint32_t *block_sums, const P blk)
 {
-    uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    uint32_t base = blk.id() * kScanTile + threadIdx.x * kScanItems;
     uint32_t v = 0;
     for (int k = 0; k < kScanItems; k++)
         if (base + k < n) v += in[base + k];
     uint32_t total;
     block_exclusive_scan(v, &total);
-    if (threadIdx.x == 0) block_sums[blockIdx.x] = total;
+    if (threadIdx.x == 0) block_sums[blk.id()] = total;
 }
 
-__global__ __launch_bounds__(kBlock) void k_scan_top(uint32_t *block_sums, uint32_t nblocks)
+__global__ __launch_bounds__(kBlock) void k_scan_reduce(const uint32_t *in, uint32_t n, uint32_t *block_sums)
+{
+    scan_reduce_body(in, n, block_sums, this_blk());
+}
+
+// The counter arrays the tick sequence scans: leaf object counts, leaf unit
+// counts, and the region bins of the contact grouping.
+enum ScanArray : int { kScanCount, kScanUcount, kScanBins };
+
+__host__ __device__ inline uint32_t scan_length(const DevState &s, int a)
+{
+    return a == kScanBins ? (s.coarse_max + 1) * s.bucket_blocks : s.node_cap;
+}
+
+__host__ __device__ inline uint32_t scan_blocks_for(uint32_t n) { return (n + kScanTile - 1) / kScanTile; }
+
+template <int A>
+__global__ __launch_bounds__(kBlock) void kf_scan_reduce(const Many m)
+{
+    const Member w = member_of(m);
+    const DevState &s = m.ss[w.i];
+    const uint32_t *in = A == kScanCount ? s.count : (A == kScanUcount ? s.ucount : s.bcount);
+    scan_reduce_body(in, scan_length(s, A), s.block_sums, w.blk);
+}
+
+__device__ __forceinline__ void scan_top_body(uint32_t *block_sums, uint32_t nblocks)
 {
     uint32_t carry = 0;
     for (uint32_t b0 = 0; b0 < nblocks; b0 += kBlock) {
@@ -1133,10 +1353,24 @@ __global__ __launch_bounds__(kBlock) void k_scan_top(uint32_t *block_sums, uint3
     }
 }
 
-// Object offsets of every leaf (node-index order) and the leaf statistics.
-__global__ __launch_bounds__(kBlock) void k_scan_objects(DevState s)
+__global__ __launch_bounds__(kBlock) void k_scan_top(uint32_t *block_sums, uint32_t nblocks)
 {
-    uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    scan_top_body(block_sums, nblocks);
+}
+
+template <int A>
+__global__ __launch_bounds__(kBlock) void kf_scan_top(const Many m)
+{
+    const Member w = member_of(m);
+    const DevState &s = m.ss[w.i];
+    scan_top_body(s.block_sums, scan_blocks_for(scan_length(s, A)));
+}
+
+// Object offsets of every leaf (node-index order) and the leaf statistics.
+template <class P>
+__device__ __forceinline__ void scan_objects_body(const DevState s, const P blk)
+{
+    uint32_t base = blk.id() * kScanTile + threadIdx.x * kScanItems;
     uint32_t c[kScanItems];
     uint32_t v = 0;
     for (int k = 0; k < kScanItems; k++) {
@@ -1144,7 +1378,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_objects(DevState s)
         v += c[k];
     }
     uint32_t total;
-    uint32_t ex = block_exclusive_scan(v, &total) + s.block_sums[blockIdx.x];
+    uint32_t ex = block_exclusive_scan(v, &total) + s.block_sums[blk.id()];
     // leaves beyond the reference's leaf storage of 2*cap objects
     // (grid_amplify, workbench.cpp:36), where it would drop objects
     // (filter.cpp:100-104): only leaves hold objects
@@ -1177,13 +1411,21 @@ __global__ __launch_bounds__(kBlock) void k_scan_objects(DevState s)
     }
 }
 
+__global__ __launch_bounds__(kBlock) void k_scan_objects(DevState s) { scan_objects_body(s, this_blk()); }
+__global__ __launch_bounds__(kBlock) void kf_scan_objects(const Many m)
+{
+    const Member w = member_of(m);
+    scan_objects_body(m.ss[w.i], w.blk);
+}
+
 // Unit offsets of every leaf, and the join work list: leaves with >= 2
 // objects (inside pairs) or with border units aimed at them.  Leaves of more
 // than kHeavyLeaf objects are listed from the back, so that the join takes
 // them first and ends on small leaves.
-__global__ __launch_bounds__(kBlock) void k_scan_units(DevState s)
+template <class P>
+__device__ __forceinline__ void scan_units_body(const DevState s, const P blk)
 {
-    uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    uint32_t base = blk.id() * kScanTile + threadIdx.x * kScanItems;
     uint32_t c[kScanItems];
     uint32_t v = 0;
     for (int k = 0; k < kScanItems; k++) {
@@ -1191,7 +1433,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_units(DevState s)
         v += c[k];
     }
     uint32_t total;
-    uint32_t ex = block_exclusive_scan(v, &total) + s.block_sums[blockIdx.x];
+    uint32_t ex = block_exclusive_scan(v, &total) + s.block_sums[blk.id()];
     for (int k = 0; k < kScanItems; k++) {
         uint32_t n = base + k;
         bool in = n < s.node_cap;
@@ -1206,11 +1448,18 @@ __global__ __launch_bounds__(kBlock) void k_scan_units(DevState s)
     }
 }
 
+__global__ __launch_bounds__(kBlock) void k_scan_units(DevState s) { scan_units_body(s, this_blk()); }
+__global__ __launch_bounds__(kBlock) void kf_scan_units(const Many m)
+{
+    const Member w = member_of(m);
+    scan_units_body(m.ss[w.i], w.blk);
+}
+
 // Counting sort by leaf, in the order of the assignment: objects that share
 // a leaf were given consecutive ranks there, so the sorted arrays are
 // written in runs.
-template <bool GEO>
-__global__ __launch_bounds__(kBlock) void k_scatter(DevState s)
+template <bool GEO, class P>
+__device__ __forceinline__ void scatter_body(const DevState s, const P blk)
 {
     const double *xy_prev = s.tp->xy_prev;
     double *xy_save = s.tp->xy_save;
@@ -1219,7 +1468,7 @@ __global__ __launch_bounds__(kBlock) void k_scatter(DevState s)
     const bool has_prev = s.tp->has_prev != 0;
     // in the previous tick's leaf order, position k held the object now at k
     const bool ordered = s.ctr->sorted_valid != 0;
-    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < s.n; k += gridDim.x * blockDim.x) {
+    for (uint32_t k = blk.id() * blockDim.x + threadIdx.x; k < s.n; k += blk.count() * blockDim.x) {
         const uint32_t leaf = s.ord_leaf[k];
         if (leaf == kNone) continue;
         const uint32_t i = s.order[k];
@@ -1255,6 +1504,15 @@ __global__ __launch_bounds__(kBlock) void k_scatter(DevState s)
             }
         }
     }
+}
+
+template <bool GEO>
+__global__ __launch_bounds__(kBlock) void k_scatter(DevState s) { scatter_body<GEO>(s, this_blk()); }
+template <bool GEO>
+__global__ __launch_bounds__(kBlock) void kf_scatter(const Many m)
+{
+    const Member w = member_of(m);
+    scatter_body<GEO>(m.ss[w.i], w.blk);
 }
 
 __device__ __forceinline__ uint32_t wave_exclusive_scan(uint32_t v, uint32_t *total)
@@ -1367,13 +1625,13 @@ __device__ __forceinline__ void lookup_walk(const DevState &s, double x, double 
 // are kept, stamped with the tick, for the joins of this tick and the next.
 // Objects with up to three candidates (nearly all) write their units from
 // registers after one atomic per wave; the others walk a second time.
-template <bool GEO>
-__global__ __launch_bounds__(kBlock) void k_lookup(DevState s)
+template <bool GEO, class P>
+__device__ __forceinline__ void lookup_body(const DevState s, const P blk)
 {
     const uint32_t nb = s.ctr->n_border;
     // the scatter before has put the objects in leaf order (read by the next tick)
-    if (blockIdx.x == 0 && threadIdx.x == 0) s.ctr->sorted_valid = 1;
-    for (uint32_t base = blockIdx.x * blockDim.x; base < nb; base += gridDim.x * blockDim.x) {
+    if (blk.id() == 0 && threadIdx.x == 0) s.ctr->sorted_valid = 1;
+    for (uint32_t base = blk.id() * blockDim.x; base < nb; base += blk.count() * blockDim.x) {
         const uint32_t k = base + threadIdx.x;
         uint32_t total = 0;
         uint32_t f0 = kNone, f1 = kNone, f2 = kNone;
@@ -1433,19 +1691,36 @@ __global__ __launch_bounds__(kBlock) void k_lookup(DevState s)
     }
 }
 
+template <bool GEO>
+__global__ __launch_bounds__(kBlock) void k_lookup(DevState s) { lookup_body<GEO>(s, this_blk()); }
+template <bool GEO>
+__global__ __launch_bounds__(kBlock) void kf_lookup(const Many m)
+{
+    const Member w = member_of(m);
+    lookup_body<GEO>(m.ss[w.i], w.blk);
+}
+
 // Units grouped by target leaf: unit_pid[ustart[leaf] + rank] = object, and
 // its leaf and previous leaf beside it (unit_lf).
-__global__ __launch_bounds__(kBlock) void k_unit_scatter(DevState s)
+template <class P>
+__device__ __forceinline__ void unit_scatter_body(const DevState s, const P blk)
 {
     if (s.ctr->error & kErrUnitCap) return;   // units were dropped: the tick is void
     const unsigned long long nu = s.ctr->n_units;
-    for (unsigned long long u = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x; u < nu;
-         u += (unsigned long long)gridDim.x * blockDim.x) {
+    for (unsigned long long u = blk.id() * (unsigned long long)blockDim.x + threadIdx.x; u < nu;
+         u += (unsigned long long)blk.count() * blockDim.x) {
         const uint2 un = s.units[u];
         const uint32_t q = s.ustart[un.y] + s.urank[u];
         s.unit_pid[q] = un.x;
         s.unit_lf[q] = s.uaux[u];   // 16 bytes: leaf, previous leaf, previous and current sorted position
     }
+}
+
+__global__ __launch_bounds__(kBlock) void k_unit_scatter(DevState s) { unit_scatter_body(s, this_blk()); }
+__global__ __launch_bounds__(kBlock) void kf_unit_scatter(const Many m)
+{
+    const Member w = member_of(m);
+    unit_scatter_body(m.ss[w.i], w.blk);
 }
 
 // ------------------------------------------------------------ meeting table
@@ -1537,16 +1812,17 @@ __device__ __forceinline__ void stage_flush(Stager &st, const DevState &s, Key k
         const uint32_t cnt = (uint32_t)__popcll(m);
         if (cnt == 0) continue;
         st.kept += cnt;
-        if (st.used + cnt > kChunk) {
+        const uint32_t C = s.chunk;
+        if (st.used + cnt > C) {
             // close the open chunk (its tail holds kEmptyKey) and reserve the next
-            if (st.chunk + kChunk <= s.clist_cap)
-                for (uint32_t t = st.used + lane; t < kChunk; t += 64) st.out[st.chunk + t] = kEmptyKey;
+            if (st.chunk + C <= s.clist_cap)
+                for (uint32_t t = st.used + lane; t < C; t += 64) st.out[st.chunk + t] = kEmptyKey;
             unsigned long long c = 0;
-            if (lane == 0) c = atomicAdd(&s.ctr->n_clist, (unsigned long long)kChunk);
+            if (lane == 0) c = atomicAdd(&s.ctr->n_clist, (unsigned long long)C);
             st.chunk = __shfl(c, 0);
             st.used = 0;
         }
-        if (st.chunk + kChunk <= s.clist_cap) {
+        if (st.chunk + C <= s.clist_cap) {
             if (keep) st.out[st.chunk + st.used + __popcll(m & ((1ull << lane) - 1ull))] = k;
         } else if (lane == 0) {
             atomicOr(&s.ctr->error, kErrContactCap);
@@ -1560,8 +1836,8 @@ __device__ __forceinline__ void stage_flush(Stager &st, const DevState &s, Key k
 // The unused tail of a wave's last chunk holds kEmptyKey (skipped later).
 __device__ __forceinline__ void stage_close(Stager &st, const DevState &s)
 {
-    if (st.used < kChunk && st.chunk + kChunk <= s.clist_cap)
-        for (uint32_t k = st.used + lane_id(); k < kChunk; k += 64) st.out[st.chunk + k] = kEmptyKey;
+    if (st.used < s.chunk && st.chunk + s.chunk <= s.clist_cap)
+        for (uint32_t k = st.used + lane_id(); k < s.chunk; k += 64) st.out[st.chunk + k] = kEmptyKey;
 }
 
 // ------------------------------------------------------------ refine: join
@@ -2081,8 +2357,8 @@ __device__ __forceinline__ void triangle_task(uint32_t t, uint32_t n, uint32_t &
 // when the pair was also within reach at prev_tick (the only case that
 // needs the previous table generation) and kTagBorder when it came from a
 // border unit (the only case that can find a pair twice).
-template <int TILE, bool GEO = false>
-__global__ __launch_bounds__(kBlock) void k_join(DevState s)
+template <int TILE, bool GEO = false, class P>
+__device__ __forceinline__ void join_body(const DevState s, const P blk)
 {
     __shared__ JoinLds<TILE> lds;
     const double *xy = s.tp->xy;
@@ -2093,7 +2369,7 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
     const uint32_t nh = s.ctr->n_heavy;
     const uint32_t nl = (s.ctr->error & kErrUnitCap) ? 0 : s.ctr->n_leaf_list + nh;
     const uint32_t wave = threadIdx.x / 64, lane = lane_id();
-    Stager st{contact_out(s), lds.stage[wave], 0, 0, kChunk, 0};
+    Stager st{contact_out(s), lds.stage[wave], 0, 0, s.chunk, 0};
     JoinRows &rows = lds.rows[wave];
     uint32_t n_in = 0, n_bd = 0;                   // sure contacts (per lane)
     uint32_t pass_in = 0, pass_bd = 0, band = 0;   // band entries within reach / re-tested (per lane)
@@ -2336,6 +2612,15 @@ __global__ __launch_bounds__(kBlock) void k_join(DevState s)
     }
 }
 
+template <int TILE, bool GEO = false>
+__global__ __launch_bounds__(kBlock) void k_join(DevState s) { join_body<TILE, GEO>(s, this_blk()); }
+template <int TILE, bool GEO = false>
+__global__ __launch_bounds__(kBlock) void kf_join(const Many m)
+{
+    const Member w = member_of(m);
+    join_body<TILE, GEO>(m.ss[w.i], w.blk);
+}
+
 // ------------------------------------------------------------ bucketing
 //
 // Two passes group the contact list by region.  Pass 1 (k_bucket_count, a
@@ -2351,15 +2636,17 @@ __device__ __forceinline__ uint32_t coarse_bin(const TableShape &sh, unsigned lo
     return region_of(tagged & kKeyBits, sh.bits) >> sh.lo_bits;
 }
 
-__device__ __forceinline__ void block_range(const DevState &s, unsigned long long n, unsigned long long &b0,
+template <class P>
+__device__ __forceinline__ void block_range(const P blk, unsigned long long n, unsigned long long &b0,
                                             unsigned long long &b1)
 {
-    const unsigned long long per = ((n + gridDim.x - 1) / gridDim.x + kBucketTile - 1) / kBucketTile * kBucketTile;
-    b0 = min(n, per * blockIdx.x);
+    const unsigned long long per = ((n + blk.count() - 1) / blk.count() + kBucketTile - 1) / kBucketTile * kBucketTile;
+    b0 = min(n, per * blk.id());
     b1 = min(n, b0 + per);
 }
 
-__global__ __launch_bounds__(kBlock) void k_bucket_count(DevState s)
+template <class P>
+__device__ __forceinline__ void bucket_count_body(const DevState s, const P blk)
 {
     __shared__ uint32_t hist[kMaxCoarse + 1];
     const unsigned long long n = min(s.ctr->n_clist, s.clist_cap);
@@ -2367,7 +2654,7 @@ __global__ __launch_bounds__(kBlock) void k_bucket_count(DevState s)
     for (uint32_t k = threadIdx.x; k <= sh.coarse; k += kBlock) hist[k] = 0;
     __syncthreads();
     unsigned long long b0, b1;
-    block_range(s, n, b0, b1);
+    block_range(blk, n, b0, b1);
     const unsigned long long *in = contact_out(s);
     // four independent loads in flight per thread
     constexpr int kU = 4;
@@ -2382,14 +2669,22 @@ __global__ __launch_bounds__(kBlock) void k_bucket_count(DevState s)
     __syncthreads();
     // bins past this tick's (the empty-slot bin is sh.coarse) count nothing
     for (uint32_t k = threadIdx.x; k <= s.coarse_max; k += kBlock)
-        s.bcount[(size_t)k * gridDim.x + blockIdx.x] = k <= sh.coarse ? hist[k] : 0u;
+        s.bcount[(size_t)k * blk.count() + blk.id()] = k <= sh.coarse ? hist[k] : 0u;
+}
+
+__global__ __launch_bounds__(kBlock) void k_bucket_count(DevState s) { bucket_count_body(s, this_blk()); }
+__global__ __launch_bounds__(kBlock) void kf_bucket_count(const Many m)
+{
+    const Member w = member_of(m);
+    bucket_count_body(m.ss[w.i], w.blk);
 }
 
 // Exclusive offsets of a scanned counter array (third kernel of the scan).
-__global__ __launch_bounds__(kBlock) void k_scan_write(const uint32_t *in, uint32_t n, const uint32_t *block_sums,
-                                                       uint32_t *out)
+template <class P>
+__device__ __forceinline__ void scan_write_body(const uint32_t *in, uint32_t n, const uint32_t *block_sums,
+                                                uint32_t *out, const P blk)
 {
-    uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    uint32_t base = blk.id() * kScanTile + threadIdx.x * kScanItems;
     uint32_t c[kScanItems];
     uint32_t v = 0;
     for (int k = 0; k < kScanItems; k++) {
@@ -2397,14 +2692,28 @@ __global__ __launch_bounds__(kBlock) void k_scan_write(const uint32_t *in, uint3
         v += c[k];
     }
     uint32_t total;
-    uint32_t ex = block_exclusive_scan(v, &total) + block_sums[blockIdx.x];
+    uint32_t ex = block_exclusive_scan(v, &total) + block_sums[blk.id()];
     for (int k = 0; k < kScanItems; k++) {
         if (base + k < n) out[base + k] = ex;
         ex += c[k];
     }
 }
 
-__global__ __launch_bounds__(kBucketThreads) void k_bucket_scatter(DevState s)
+__global__ __launch_bounds__(kBlock) void k_scan_write(const uint32_t *in, uint32_t n, const uint32_t *block_sums,
+                                                       uint32_t *out)
+{
+    scan_write_body(in, n, block_sums, out, this_blk());
+}
+
+__global__ __launch_bounds__(kBlock) void kf_scan_write_bins(const Many m)
+{
+    const Member w = member_of(m);
+    const DevState &s = m.ss[w.i];
+    scan_write_body(s.bcount, scan_length(s, kScanBins), s.block_sums, s.boff, w.blk);
+}
+
+template <class P>
+__device__ __forceinline__ void bucket_scatter_body(const DevState s, const P blk)
 {
     __shared__ unsigned long long tile[kBucketTile];
     __shared__ uint32_t hist[kMaxCoarse + 1], start[kMaxCoarse + 1], running[kMaxCoarse + 1];
@@ -2415,8 +2724,8 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_scatter(DevState s)
     const unsigned long long *in = contact_out(s);
     unsigned long long *out = s.clist2;
     unsigned long long b0, b1;
-    block_range(s, n, b0, b1);
-    for (uint32_t k = threadIdx.x; k < nb; k += kBucketThreads) running[k] = s.boff[(size_t)k * gridDim.x + blockIdx.x];
+    block_range(blk, n, b0, b1);
+    for (uint32_t k = threadIdx.x; k < nb; k += kBucketThreads) running[k] = s.boff[(size_t)k * blk.count() + blk.id()];
     for (unsigned long long t0 = b0; t0 < b1; t0 += kBucketTile) {
         const uint32_t tn = (uint32_t)min((unsigned long long)kBucketTile, b1 - t0);
         for (uint32_t k = threadIdx.x; k < nb; k += kBucketThreads) hist[k] = 0;
@@ -2477,6 +2786,13 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_scatter(DevState s)
     }
 }
 
+__global__ __launch_bounds__(kBucketThreads) void k_bucket_scatter(DevState s) { bucket_scatter_body(s, this_blk()); }
+__global__ __launch_bounds__(kBucketThreads) void kf_bucket_scatter(const Many m)
+{
+    const Member w = member_of(m);
+    bucket_scatter_body(m.ss[w.i], w.blk);
+}
+
 // Pass 2: one workgroup per coarse bin c splits its segment of clist2 by the
 // low region bits into the generation's list and writes the first contact of
 // each region.
@@ -2510,7 +2826,8 @@ __device__ __forceinline__ uint32_t block_scan_bins(uint32_t *bins, uint32_t nbi
     return total;
 }
 
-__global__ __launch_bounds__(kBucketThreads) void k_bucket_fine(DevState s)
+template <class P>
+__device__ __forceinline__ void bucket_fine_body(const DevState s, const P blk)
 {
     __shared__ unsigned long long tile[kFineTile];
     __shared__ uint32_t thist[kMaxFine], tstart[kMaxFine], gcur[kMaxFine];
@@ -2520,7 +2837,7 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_fine(DevState s)
     const uint32_t nf = 1u << sh.lo_bits;
     unsigned long long *out = s.tp->tab_cur.keys;
     uint32_t *rstart = s.tp->tab_cur.rstart;
-    for (uint32_t c = blockIdx.x; c < sh.coarse; c += gridDim.x) {
+    for (uint32_t c = blk.id(); c < sh.coarse; c += blk.count()) {
         const uint32_t seg0 = s.boff[(size_t)c * G1], seg1 = s.boff[(size_t)(c + 1) * G1];
         for (uint32_t k = threadIdx.x; k < nf; k += kBucketThreads) thist[k] = 0;
         __syncthreads();
@@ -2572,7 +2889,7 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_fine(DevState s)
             __syncthreads();
         }
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (blk.id() == 0 && threadIdx.x == 0) {
         // the generation's entry count and region bits, where the next tick
         // (and the readers of a generation) find them whatever its shape
         const uint32_t total = s.boff[(size_t)sh.coarse * G1];
@@ -2581,6 +2898,13 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_fine(DevState s)
         rstart[s.rs_total + 1] = sh.bits;
         s.ctr->tab_bits = sh.bits;
     }
+}
+
+__global__ __launch_bounds__(kBucketThreads) void k_bucket_fine(DevState s) { bucket_fine_body(s, this_blk()); }
+__global__ __launch_bounds__(kBucketThreads) void kf_bucket_fine(const Many m)
+{
+    const Member w = member_of(m);
+    bucket_fine_body(m.ss[w.i], w.blk);
 }
 
 // ------------------------------------------------------------ table update
@@ -2775,8 +3099,8 @@ __device__ __forceinline__ void table_write(const TableLds<KV, NT> &L, const Dev
 // previous generation's; the host launches both variants and the one that
 // does not apply returns at once (the common variant then needs no registers
 // for the mapping between shapes).
-template <bool KV, int NT, bool RESHAPED>
-__global__ __launch_bounds__(NT) void k_table(DevState s)
+template <bool KV, int NT, bool RESHAPED, class P>
+__device__ __forceinline__ void table_body(const DevState s, const P blk)
 {
     __shared__ TableLds<KV, NT> L;
     const TableGen told = s.tp->tab_old, tcur = s.tp->tab_cur;
@@ -2817,7 +3141,7 @@ __global__ __launch_bounds__(NT) void k_table(DevState s)
         o1 = bo1;
         c0n = bc0;
         c1n = bc1;
-        bounds(rr + gridDim.x);
+        bounds(rr + blk.count());
 #pragma unroll
         for (int q = 0; q < kPer; q++) {
             const uint32_t i = o0 + threadIdx.x + q * NT;
@@ -2827,9 +3151,9 @@ __global__ __launch_bounds__(NT) void k_table(DevState s)
             nkn[q] = c < c1n ? tcur.keys[c] : 0ull;
         }
     };
-    bounds(blockIdx.x);
-    if (blockIdx.x < nreg) prefetch(blockIdx.x);
-    for (uint32_t r = blockIdx.x; r < nreg; r += gridDim.x) {
+    bounds(blk.id());
+    if (blk.id() < nreg) prefetch(blk.id());
+    for (uint32_t r = blk.id(); r < nreg; r += blk.count()) {
         for (uint32_t k = threadIdx.x; k < kRegion; k += NT) L.hk[k] = kEmptyKey;
         const uint32_t c0 = c0n, c1 = c1n;
 #pragma unroll
@@ -2872,7 +3196,7 @@ __global__ __launch_bounds__(NT) void k_table(DevState s)
             const bool v = i < o1 && (!filter_old || region_of(k & kKeyBits, bits) == r);
             table_old(L, s, v, k, v ? told.starts[i] : kDupStart, i, tick, last, st, nst);
         }
-        const uint32_t rn = r + gridDim.x;
+        const uint32_t rn = r + blk.count();
         if (rn < nreg) prefetch(rn);
         lds_barrier();
         // starts (and boxes) of every contact of the region
@@ -2896,6 +3220,15 @@ __global__ __launch_bounds__(NT) void k_table(DevState s)
     // distinct pairs of the tick: contacts less second copies
     for (int d = 32; d >= 1; d >>= 1) dups += __shfl_xor(dups, d);
     if (lane_id() == 0 && dups) atomicAdd(&s.ctr->n_dup, (unsigned long long)dups);
+}
+
+template <bool KV, int NT, bool RESHAPED>
+__global__ __launch_bounds__(NT) void k_table(DevState s) { table_body<KV, NT, RESHAPED>(s, this_blk()); }
+template <bool KV, int NT, bool RESHAPED>
+__global__ __launch_bounds__(NT) void kf_table(const Many m)
+{
+    const Member w = member_of(m);
+    table_body<KV, NT, RESHAPED>(m.ss[w.i], w.blk);
 }
 
 // ------------------------------------------------------------ host side
@@ -3009,7 +3342,10 @@ struct glint_ctx {
     uint32_t tick = 0;
     uint32_t prev_tick = 0;
     uint32_t scan_blocks = 0;
-    uint32_t join_blocks = 1024;
+    uint32_t join_blocks = 1024;   // workgroups of the join
+    uint32_t fused_join_blocks = 1024;   // its share of a fused join: about one per 1024 objects, 64 to 1024
+    uint32_t table_grid = 4096;    // workgroups of the table update (persistent over the regions)
+    uint32_t fine_grid = 1024;     // workgroups of the fine bucketing pass (persistent over the bins)
     double *own_xy = nullptr;
     const double *last_xy = nullptr;   // positions of the current tick (caller's or own_xy)
     bool resident = false;             // own_xy holds the resident positions (update ingestion)
@@ -3026,14 +3362,16 @@ struct glint_ctx {
     double wall_khz = 0;
 };
 
-// Independent scenarios ticked together: one context (and stream) each,
-// optionally one graph holding graph_ticks ticks of every scenario as
-// parallel branches forked from and joined into an origin stream.
+// Independent scenarios ticked together: one context each, grouped by
+// kernel variant; every step of a tick is one fused launch per group on the
+// batch's stream, optionally captured for graph_ticks ticks in one graph.
 struct glint_batch {
     std::vector<glint_ctx *> ctx;
+    std::vector<std::vector<uint32_t>> groups;   // contexts fused into one launch per step
+    std::vector<DevState *> group_states;        // each group's member states (device memory)
     hipStream_t origin = nullptr;
-    hipEvent_t fork = nullptr, after = nullptr;
-    std::vector<hipEvent_t> join, ready;
+    hipEvent_t after = nullptr;
+    std::vector<hipEvent_t> ready;
     hipGraphExec_t exec = nullptr;
     uint32_t graph_ticks = 0;
     std::string err;
@@ -3177,88 +3515,179 @@ glint_status build_tree(glint_ctx *c)
     return GLINT_OK;
 }
 
+// Where a tick sequence is enqueued: one context on its own stream (each
+// kernel timed when profiling is on), or a fused group of contexts on the
+// group's stream, one launch per step for all of them.  The members of a
+// group share every setting that picks a kernel variant (join tile, table
+// workgroup, key-value output, geography, schema mode), so the sequence is
+// decided by any one of them.
+struct FusedGroup {
+    std::vector<glint_ctx *> ctx;      // at most kMaxFused
+    DevState *d_states = nullptr;      // the members' states in device memory
+    hipStream_t stream = nullptr;
+};
+
+struct Seq {
+    glint_ctx *one = nullptr;
+    const FusedGroup *group = nullptr;
+    const glint_ctx &rep() const { return one ? *one : *group->ctx[0]; }
+};
+
+// One step of the sequence: blocks(c) workgroups for context c (none: c
+// takes no part); single(stream, grid, c) launches the one-context kernel,
+// fused(stream, grid, many) the fused one.
+template <class B, class S, class F>
+void step(const Seq &q, int id, B blocks, S single, F fused)
+{
+    if (q.one) {
+        glint_ctx *c = q.one;
+        const uint32_t g = blocks(*c);
+        if (g) launch(c, id, [&] { single(c->stream, g, *c); });
+        return;
+    }
+    Many m{};
+    m.ss = q.group->d_states;
+    m.n = (uint32_t)q.group->ctx.size();
+    uint32_t total = 0;
+    for (uint32_t i = 0; i < m.n; i++) {
+        m.start[i] = total;
+        total += blocks(*q.group->ctx[i]);
+    }
+    m.start[m.n] = total;
+    if (total) fused(q.group->stream, total, m);
+}
+
+// A step of a kernel whose only argument is the state: k_<name> and kf_<name>.
+#define GLINT_STEP(q, id, threads, blocks_expr, name)                                                     \
+    step(                                                                                                \
+        q, id, [&](const glint_ctx &c) -> uint32_t { return (blocks_expr); },                            \
+        [&](hipStream_t st_, uint32_t g_, glint_ctx &c) { k_##name<<<g_, threads, 0, st_>>>(c.s); },     \
+        [&](hipStream_t st_, uint32_t g_, const Many &m_) { kf_##name<<<g_, threads, 0, st_>>>(m_); })
+
+// Clearing of a per-node counter array, and the three kernels of its scan.
+void step_fill(const Seq &q, int id, int which)
+{
+    step(
+        q, id,
+        [&](const glint_ctx &c) -> uint32_t {
+            return which == kFillCoarse ? grid_for(kCoarseDim * kCoarseDim)
+                                        : grid_for((which == kFillMeta ? 4ull : 1ull) * c.s.node_cap);
+        },
+        [&](hipStream_t st, uint32_t g, glint_ctx &c) {
+            const DevState &s = c.s;
+            if (which == kFillCount) k_fill_u32<<<g, kBlock, 0, st>>>(s.count, 0u, s.node_cap);
+            if (which == kFillUcount) k_fill_u32<<<g, kBlock, 0, st>>>(s.ucount, 0u, s.node_cap);
+            if (which == kFillMeta)
+                k_fill_u32<<<g, kBlock, 0, st>>>(reinterpret_cast<uint32_t *>(s.meta), 0u, 4ull * s.node_cap);
+            if (which == kFillCoarse) k_fill_u32<<<g, kBlock, 0, st>>>(s.g10, 0u, kCoarseDim * kCoarseDim);
+        },
+        [&](hipStream_t st, uint32_t g, const Many &m) {
+            if (which == kFillCount) kf_fill<kFillCount><<<g, kBlock, 0, st>>>(m);
+            if (which == kFillUcount) kf_fill<kFillUcount><<<g, kBlock, 0, st>>>(m);
+            if (which == kFillMeta) kf_fill<kFillMeta><<<g, kBlock, 0, st>>>(m);
+            if (which == kFillCoarse) kf_fill<kFillCoarse><<<g, kBlock, 0, st>>>(m);
+        });
+}
+
+template <int A>
+const uint32_t *scan_input(const DevState &s)
+{
+    return A == kScanCount ? s.count : (A == kScanUcount ? s.ucount : s.bcount);
+}
+
+template <int A>
+void step_scan_totals(const Seq &q)
+{
+    auto nb = [](const glint_ctx &c) -> uint32_t { return scan_blocks_for(scan_length(c.s, A)); };
+    step(
+        q, K_SCAN_REDUCE, nb,
+        [&](hipStream_t st, uint32_t g, glint_ctx &c) {
+            k_scan_reduce<<<g, kBlock, 0, st>>>(scan_input<A>(c.s), scan_length(c.s, A), c.s.block_sums);
+        },
+        [&](hipStream_t st, uint32_t g, const Many &m) { kf_scan_reduce<A><<<g, kBlock, 0, st>>>(m); });
+    step(
+        q, K_SCAN_TOP, [](const glint_ctx &) -> uint32_t { return 1; },
+        [&](hipStream_t st, uint32_t, glint_ctx &c) { k_scan_top<<<1, kBlock, 0, st>>>(c.s.block_sums, nb(c)); },
+        [&](hipStream_t st, uint32_t g, const Many &m) { kf_scan_top<A><<<g, kBlock, 0, st>>>(m); });
+}
+
 // The tree of this tick's positions (GLINT_SCHEMA_GRID_REBUILD), before the
 // assignment: a fixed sequence of launches.
-void enqueue_grid_rebuild(glint_ctx *c)
+void enqueue_grid_rebuild(const Seq &q)
 {
-    DevState &s = c->s;
-    hipStream_t st = c->stream;
-    const uint32_t gn = grid_for(s.n);
-    launch(c, K_FILL_COUNTS, [&] {
-        k_fill_u32<<<grid_for(kCoarseDim * kCoarseDim), kBlock, 0, st>>>(s.g10, 0u, kCoarseDim * kCoarseDim);
-    });
-    launch(c, K_FILL_COUNTS, [&] {
-        k_fill_u32<<<grid_for(4ull * s.node_cap), kBlock, 0, st>>>(reinterpret_cast<uint32_t *>(s.meta), 0u,
-                                                                   4ull * s.node_cap);
-    });
-    launch(c, K_GRID_CELLS, [&] { k_grid_cells<<<gn, kBlock, 0, st>>>(s); });
-    launch(c, K_GRID_PYRAMID, [&] { k_grid_pyramid<<<1024, kBlock, 0, st>>>(s); });
-    launch(c, K_GRID_TOP, [&] { k_grid_top<<<1, 1024, 0, st>>>(s); });
-    launch(c, K_GRID_FINE_ZERO, [&] {
-        k_grid_fine_zero<<<grid_for((uint64_t)s.heavy_cap * kFineCells, 2048), kBlock, 0, st>>>(s);
-    });
-    launch(c, K_GRID_FINE, [&] { k_grid_fine<<<gn, kBlock, 0, st>>>(s); });
-    for (uint32_t d = 0; d <= kGridBits; d++) {
-        const uint64_t most = std::min<uint64_t>(1ull << (2 * d), s.glist_cap);
-        launch(c, K_GRID_LEVEL, [&] { k_grid_level<<<grid_for(most, 1024), kBlock, 0, st>>>(s, d); });
-    }
+    step_fill(q, K_FILL_COUNTS, kFillCoarse);
+    step_fill(q, K_FILL_COUNTS, kFillMeta);
+    GLINT_STEP(q, K_GRID_CELLS, kBlock, grid_for(c.s.n), grid_cells);
+    GLINT_STEP(q, K_GRID_PYRAMID, kBlock, 1024u, grid_pyramid);
+    GLINT_STEP(q, K_GRID_TOP, 1024, 1u, grid_top);
+    GLINT_STEP(q, K_GRID_FINE_ZERO, kBlock, grid_for((uint64_t)c.s.heavy_cap * kFineCells, 2048), grid_fine_zero);
+    GLINT_STEP(q, K_GRID_FINE, kBlock, grid_for(c.s.n), grid_fine);
+    for (uint32_t d = 0; d <= kGridBits; d++)
+        step(
+            q, K_GRID_LEVEL,
+            [&](const glint_ctx &c) -> uint32_t {
+                return grid_for(std::min<uint64_t>(1ull << (2 * d), c.s.glist_cap), 1024);
+            },
+            [&](hipStream_t st, uint32_t g, glint_ctx &c) { k_grid_level<<<g, kBlock, 0, st>>>(c.s, d); },
+            [&](hipStream_t st, uint32_t g, const Many &m) { kf_grid_level<<<g, kBlock, 0, st>>>(m, d); });
 }
 
-void enqueue_partition(glint_ctx *c)
+void enqueue_partition(const Seq &q)
 {
-    DevState &s = c->s;
-    hipStream_t st = c->stream;
-    const uint32_t gn = grid_for(s.n), gnode = grid_for(s.node_cap), nb = c->scan_blocks;
-    launch(c, K_FILL_COUNTS, [&] { k_fill_u32<<<gnode, kBlock, 0, st>>>(s.count, 0u, s.node_cap); });
-    launch(c, K_RESET, [&] { k_reset_tick<<<1, 1, 0, st>>>(s); });
-    if (c->cfg.dynamic_schema == GLINT_SCHEMA_GRID_REBUILD) enqueue_grid_rebuild(c);
-    launch(c, K_ASSIGN, [&] { k_assign<<<gn, kBlock, 0, st>>>(s); });
-    launch(c, K_SCAN_REDUCE, [&] { k_scan_reduce<<<nb, kBlock, 0, st>>>(s.count, s.node_cap, s.block_sums); });
-    launch(c, K_SCAN_TOP, [&] { k_scan_top<<<1, kBlock, 0, st>>>(s.block_sums, nb); });
-    launch(c, K_SCAN_OBJECTS, [&] { k_scan_objects<<<nb, kBlock, 0, st>>>(s); });
-    if (s.geography)
-        launch(c, K_SCATTER, [&] { k_scatter<true><<<gn, kBlock, 0, st>>>(s); });
+    step_fill(q, K_FILL_COUNTS, kFillCount);
+    GLINT_STEP(q, K_RESET, 1, 1u, reset_tick);
+    if (q.rep().cfg.dynamic_schema == GLINT_SCHEMA_GRID_REBUILD) enqueue_grid_rebuild(q);
+    GLINT_STEP(q, K_ASSIGN, kBlock, grid_for(c.s.n), assign);
+    step_scan_totals<kScanCount>(q);
+    GLINT_STEP(q, K_SCAN_OBJECTS, kBlock, c.scan_blocks, scan_objects);
+    if (q.rep().s.geography)
+        GLINT_STEP(q, K_SCATTER, kBlock, grid_for(c.s.n), scatter<true>);
     else
-        launch(c, K_SCATTER, [&] { k_scatter<false><<<gn, kBlock, 0, st>>>(s); });
+        GLINT_STEP(q, K_SCATTER, kBlock, grid_for(c.s.n), scatter<false>);
 }
 
-void enqueue_lookup(glint_ctx *c)
+void enqueue_lookup(const Seq &q)
 {
-    DevState &s = c->s;
-    hipStream_t st = c->stream;
-    const uint32_t gnode = grid_for(s.node_cap), nb = c->scan_blocks;
-    launch(c, K_FILL_UCOUNT, [&] { k_fill_u32<<<gnode, kBlock, 0, st>>>(s.ucount, 0u, s.node_cap); });
-    if (s.geography)
-        launch(c, K_LOOKUP, [&] { k_lookup<true><<<grid_for(s.n), kBlock, 0, st>>>(s); });
+    step_fill(q, K_FILL_UCOUNT, kFillUcount);
+    if (q.rep().s.geography)
+        GLINT_STEP(q, K_LOOKUP, kBlock, grid_for(c.s.n), lookup<true>);
     else
-        launch(c, K_LOOKUP, [&] { k_lookup<false><<<grid_for(s.n), kBlock, 0, st>>>(s); });
-    launch(c, K_SCAN_REDUCE, [&] { k_scan_reduce<<<nb, kBlock, 0, st>>>(s.ucount, s.node_cap, s.block_sums); });
-    launch(c, K_SCAN_TOP, [&] { k_scan_top<<<1, kBlock, 0, st>>>(s.block_sums, nb); });
-    launch(c, K_SCAN_UNITS, [&] { k_scan_units<<<nb, kBlock, 0, st>>>(s); });
-    launch(c, K_UNIT_SCATTER, [&] { k_unit_scatter<<<grid_for(s.unit_cap), kBlock, 0, st>>>(s); });
+        GLINT_STEP(q, K_LOOKUP, kBlock, grid_for(c.s.n), lookup<false>);
+    step_scan_totals<kScanUcount>(q);
+    GLINT_STEP(q, K_SCAN_UNITS, kBlock, c.scan_blocks, scan_units);
+    GLINT_STEP(q, K_UNIT_SCATTER, kBlock, grid_for(c.s.unit_cap), unit_scatter);
 }
 
-void enqueue_refine(glint_ctx *c)
+void enqueue_refine(const Seq &q)
 {
-    DevState &s = c->s;
-    hipStream_t st = c->stream;
-    if (s.geography)
-        launch(c, K_JOIN, [&] { k_join<256, true><<<c->join_blocks, kBlock, 0, st>>>(s); });
-    else if (c->cfg.join_tile == 256)
-        launch(c, K_JOIN, [&] { k_join<256><<<c->join_blocks, kBlock, 0, st>>>(s); });
+    const glint_ctx &r = q.rep();
+    // a context alone gets the whole device; in a fused group, a share by size
+    auto jb = [&](const glint_ctx &c) -> uint32_t { return q.one ? c.join_blocks : c.fused_join_blocks; };
+    if (r.s.geography)
+        step(
+            q, K_JOIN, jb, [&](hipStream_t st, uint32_t g, glint_ctx &c) { k_join<256, true><<<g, kBlock, 0, st>>>(c.s); },
+            [&](hipStream_t st, uint32_t g, const Many &m) { kf_join<256, true><<<g, kBlock, 0, st>>>(m); });
+    else if (r.cfg.join_tile == 256)
+        step(
+            q, K_JOIN, jb, [&](hipStream_t st, uint32_t g, glint_ctx &c) { k_join<256><<<g, kBlock, 0, st>>>(c.s); },
+            [&](hipStream_t st, uint32_t g, const Many &m) { kf_join<256><<<g, kBlock, 0, st>>>(m); });
     else
-        launch(c, K_JOIN, [&] { k_join<kTile><<<c->join_blocks, kBlock, 0, st>>>(s); });
+        step(
+            q, K_JOIN, jb, [&](hipStream_t st, uint32_t g, glint_ctx &c) { k_join<kTile><<<g, kBlock, 0, st>>>(c.s); },
+            [&](hipStream_t st, uint32_t g, const Many &m) { kf_join<kTile><<<g, kBlock, 0, st>>>(m); });
     // group the tick's contacts by table region
     // (the table's shape is decided on the device from the tick's stored
     // contacts; launch sizes cover the largest shape)
-    const uint32_t G1 = s.bucket_blocks;
-    const uint32_t nc = (s.coarse_max + 1) * G1, nb = (nc + kScanTile - 1) / kScanTile;
-    launch(c, K_BUCKET_COUNT, [&] { k_bucket_count<<<G1, kBlock, 0, st>>>(s); });
-    launch(c, K_SCAN_REDUCE, [&] { k_scan_reduce<<<nb, kBlock, 0, st>>>(s.bcount, nc, s.block_sums); });
-    launch(c, K_SCAN_TOP, [&] { k_scan_top<<<1, kBlock, 0, st>>>(s.block_sums, nb); });
-    launch(c, K_SCAN_WRITE, [&] { k_scan_write<<<nb, kBlock, 0, st>>>(s.bcount, nc, s.block_sums, s.boff); });
-    launch(c, K_BUCKET_SCATTER, [&] { k_bucket_scatter<<<G1, kBucketThreads, 0, st>>>(s); });
-    launch(c, K_BUCKET_FINE, [&] { k_bucket_fine<<<s.coarse_max, kBucketThreads, 0, st>>>(s); });
+    GLINT_STEP(q, K_BUCKET_COUNT, kBlock, c.s.bucket_blocks, bucket_count);
+    step_scan_totals<kScanBins>(q);
+    step(
+        q, K_SCAN_WRITE, [](const glint_ctx &c) -> uint32_t { return scan_blocks_for(scan_length(c.s, kScanBins)); },
+        [&](hipStream_t st, uint32_t g, glint_ctx &c) {
+            k_scan_write<<<g, kBlock, 0, st>>>(c.s.bcount, scan_length(c.s, kScanBins), c.s.block_sums, c.s.boff);
+        },
+        [&](hipStream_t st, uint32_t g, const Many &m) { kf_scan_write_bins<<<g, kBlock, 0, st>>>(m); });
+    GLINT_STEP(q, K_BUCKET_SCATTER, kBucketThreads, c.s.bucket_blocks, bucket_scatter);
+    GLINT_STEP(q, K_BUCKET_FINE, kBucketThreads, c.fine_grid, bucket_fine);
 }
 
 // Location updates scattered into the resident positions; pids out of
@@ -3278,59 +3707,73 @@ __global__ __launch_bounds__(kBlock) void k_apply_updates(DevState s, double *xy
 }
 
 // Float copy of this tick's positions (pid order) for the pair boxes.
-__global__ __launch_bounds__(kBlock) void k_pos32(DevState s)
+template <class P>
+__device__ __forceinline__ void pos32_body(const DevState s, const P blk)
 {
     const double *xy = s.tp->xy;
-    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < s.n; i += gridDim.x * kBlock)
+    for (uint32_t i = blk.id() * kBlock + threadIdx.x; i < s.n; i += blk.count() * kBlock)
         s.pos32[i] = make_float2((float)xy[2 * (size_t)i], (float)xy[2 * (size_t)i + 1]);
 }
 
-void enqueue_evaluate(glint_ctx *c)
+__global__ __launch_bounds__(kBlock) void k_pos32(DevState s) { pos32_body(s, this_blk()); }
+__global__ __launch_bounds__(kBlock) void kf_pos32(const Many m)
 {
-    if (c->kv) launch(c, K_POS32, [&] { k_pos32<<<grid_for(c->s.n), kBlock, 0, c->stream>>>(c->s); });
-    const uint32_t g = std::min<uint32_t>(1u << c->s.rbits_max, 4096);   // persistent over the regions
-    const uint32_t nt = c->cfg.table_threads ? c->cfg.table_threads : 512;
-    // both shape variants; the one that does not apply to the tick returns at once
-    for (int reshaped = 0; reshaped < 2; reshaped++)
-        launch(c, reshaped ? K_TABLE_RESHAPED : K_TABLE, [&] {
-            hipStream_t st = c->stream;
-            if (c->kv) {
-                if (nt == 1024)
-                    reshaped ? k_table<true, 1024, true><<<g, 1024, 0, st>>>(c->s)
-                             : k_table<true, 1024, false><<<g, 1024, 0, st>>>(c->s);
-                else if (nt == 512)
-                    reshaped ? k_table<true, 512, true><<<g, 512, 0, st>>>(c->s)
-                             : k_table<true, 512, false><<<g, 512, 0, st>>>(c->s);
-                else
-                    reshaped ? k_table<true, 256, true><<<g, 256, 0, st>>>(c->s)
-                             : k_table<true, 256, false><<<g, 256, 0, st>>>(c->s);
-            } else {
-                if (nt == 1024)
-                    reshaped ? k_table<false, 1024, true><<<g, 1024, 0, st>>>(c->s)
-                             : k_table<false, 1024, false><<<g, 1024, 0, st>>>(c->s);
-                else if (nt == 512)
-                    reshaped ? k_table<false, 512, true><<<g, 512, 0, st>>>(c->s)
-                             : k_table<false, 512, false><<<g, 512, 0, st>>>(c->s);
-                else
-                    reshaped ? k_table<false, 256, true><<<g, 256, 0, st>>>(c->s)
-                             : k_table<false, 256, false><<<g, 256, 0, st>>>(c->s);
-            }
-        });
-    if (c->s.mlog) launch(c, K_LOG_APPEND, [&] { k_log_append<<<1024, kBlock, 0, c->stream>>>(c->s); });
+    const Member w = member_of(m);
+    pos32_body(m.ss[w.i], w.blk);
 }
 
-void enqueue_schema(glint_ctx *c)
+template <bool KV, int NT>
+void step_table(const Seq &q, bool reshaped)
 {
-    DevState &s = c->s;
-    hipStream_t st = c->stream;
-    if (c->cfg.dynamic_schema == GLINT_SCHEMA_DYNAMIC) {
-        const uint32_t gnode = grid_for(s.node_cap);
-        launch(c, K_COLLECT, [&] { k_collect<<<gnode, kBlock, 0, st>>>(s); });
-        launch(c, K_SPLIT, [&] { k_split<<<gnode, kBlock, 0, st>>>(s, 0); });
-        launch(c, K_MERGE, [&] { k_merge<<<gnode, kBlock, 0, st>>>(s); });
-    }
-    launch(c, K_SCHEMA_DONE, [&] { k_schema_done<<<1, 1, 0, st>>>(s); });
+    auto g = [](const glint_ctx &c) -> uint32_t { return c.table_grid; };
+    if (reshaped)
+        step(
+            q, K_TABLE_RESHAPED, g,
+            [&](hipStream_t st, uint32_t n, glint_ctx &c) { k_table<KV, NT, true><<<n, NT, 0, st>>>(c.s); },
+            [&](hipStream_t st, uint32_t n, const Many &m) { kf_table<KV, NT, true><<<n, NT, 0, st>>>(m); });
+    else
+        step(
+            q, K_TABLE, g,
+            [&](hipStream_t st, uint32_t n, glint_ctx &c) { k_table<KV, NT, false><<<n, NT, 0, st>>>(c.s); },
+            [&](hipStream_t st, uint32_t n, const Many &m) { kf_table<KV, NT, false><<<n, NT, 0, st>>>(m); });
 }
+
+void enqueue_evaluate(const Seq &q)
+{
+    const glint_ctx &r = q.rep();
+    if (r.kv) GLINT_STEP(q, K_POS32, kBlock, grid_for(c.s.n), pos32);
+    // persistent over the regions
+    const uint32_t nt = r.cfg.table_threads ? r.cfg.table_threads : 512;
+    // both shape variants; the one that does not apply to the tick returns at once
+    for (int reshaped = 0; reshaped < 2; reshaped++) {
+        if (r.kv)
+            nt == 1024 ? step_table<true, 1024>(q, reshaped) : nt == 512 ? step_table<true, 512>(q, reshaped)
+                                                                         : step_table<true, 256>(q, reshaped);
+        else
+            nt == 1024 ? step_table<false, 1024>(q, reshaped) : nt == 512 ? step_table<false, 512>(q, reshaped)
+                                                                          : step_table<false, 256>(q, reshaped);
+    }
+    GLINT_STEP(q, K_LOG_APPEND, kBlock, c.s.mlog ? 1024u : 0u, log_append);
+}
+
+void enqueue_schema(const Seq &q)
+{
+    if (q.rep().cfg.dynamic_schema == GLINT_SCHEMA_DYNAMIC) {
+        GLINT_STEP(q, K_COLLECT, kBlock, grid_for(c.s.node_cap), collect);
+        step(
+            q, K_SPLIT, [](const glint_ctx &c) -> uint32_t { return grid_for(c.s.node_cap); },
+            [&](hipStream_t st, uint32_t g, glint_ctx &c) { k_split<<<g, kBlock, 0, st>>>(c.s, 0); },
+            [&](hipStream_t st, uint32_t g, const Many &m) { kf_split<<<g, kBlock, 0, st>>>(m, 0); });
+        GLINT_STEP(q, K_MERGE, kBlock, grid_for(c.s.node_cap), merge);
+    }
+    GLINT_STEP(q, K_SCHEMA_DONE, 1, 1u, schema_done);
+}
+
+void enqueue_partition(glint_ctx *c) { enqueue_partition(Seq{c, nullptr}); }
+void enqueue_lookup(glint_ctx *c) { enqueue_lookup(Seq{c, nullptr}); }
+void enqueue_refine(glint_ctx *c) { enqueue_refine(Seq{c, nullptr}); }
+void enqueue_evaluate(glint_ctx *c) { enqueue_evaluate(Seq{c, nullptr}); }
+void enqueue_schema(glint_ctx *c) { enqueue_schema(Seq{c, nullptr}); }
 
 void fill_stats(glint_ctx *c)
 {
@@ -3601,9 +4044,16 @@ glint_status glint_create(const glint_config *cfg, glint_ctx **out)
     }
     s.unit_cap = cfg->unit_capacity ? cfg->unit_capacity : std::max<uint64_t>(4 * N, 1024);
     s.meet_cap = cfg->meeting_capacity ? cfg->meeting_capacity : std::max<uint64_t>(2 * N, 1u << 20);
-    // contacts per tick plus the unused tails of the join's per-wave chunks
-    s.clist_cap = (cfg->contact_capacity ? cfg->contact_capacity : 16 * N + (1u << 20)) +
-                  (uint64_t)c->join_blocks * (kBlock / 64) * kChunk;
+    c->fused_join_blocks =
+        (uint32_t)std::min<uint64_t>(1024, std::max<uint64_t>(64, ((N + 1023) / 1024 + 63) / 64 * 64));
+    // contacts per tick plus the unused tails of the join's per-wave chunks;
+    // the chunk is about a quarter of the contacts a wave can expect, so that
+    // sparse scenes do not fill the list (and every pass over it) with tails
+    const uint64_t contacts = cfg->contact_capacity ? cfg->contact_capacity : 16 * N + (1u << 20);
+    const uint64_t waves = (uint64_t)c->join_blocks * (kBlock / 64);
+    s.chunk = 64;
+    while (s.chunk < kChunk && (uint64_t)s.chunk * 8 <= contacts / waves) s.chunk *= 2;
+    s.clist_cap = contacts + waves * kChunk;
     {
         // the configured slots fix the fewest regions a tick uses; a tick
         // that stores more contacts than they hold at kRegionFill per region
@@ -3617,7 +4067,19 @@ glint_status glint_create(const glint_config *cfg, glint_ctx **out)
         const uint32_t lo = s.rbits_max > 10 ? s.rbits_max - 10 : 0;
         s.coarse_max = 1u << (s.rbits_max - lo);
         s.rs_total = 1u << s.rbits_max;
-        s.bucket_blocks = 1024;
+        // coarse-pass workgroups: one per 4 tiles of the configured
+        // contacts, a power of two from 16 to 1024
+        uint32_t bb = 16;
+        while (bb < 1024 && (uint64_t)bb * 4 * kBucketTile < contacts) bb *= 2;
+        s.bucket_blocks = bb;
+        // the persistent passes get the workgroups of the configured shape
+        // (at least 256 table regions and 64 bins when the shape can grow
+        // that far); a tick whose table grows past it loops over the rest
+        c->table_grid = std::min<uint32_t>(
+            std::max<uint32_t>(1u << s.rbits_min, std::min<uint32_t>(1u << s.rbits_max, 256)), 4096);
+        // (a shape of b bits has min(2^b, 1024) coarse bins)
+        const uint32_t fine_min = 1u << std::min<uint32_t>(s.rbits_min, 10);
+        c->fine_grid = std::min<uint32_t>(s.coarse_max, std::max<uint32_t>(fine_min, 64));
     }
     c->scan_blocks = (uint32_t)((node_cap + kScanTile - 1) / kScanTile);
     const uint64_t nbc = (uint64_t)(s.coarse_max + 1) * s.bucket_blocks;
@@ -3974,6 +4436,33 @@ static void enqueue_tick_body(glint_ctx *c, uint64_t stride)
     k_stamp<<<1, 1, 0, c->stream>>>(c->s.tp);
 }
 
+// One tick of every member of a fused group: one launch per step.
+static void enqueue_tick_body(const FusedGroup &g)
+{
+    const Seq q{nullptr, &g};
+    const uint32_t n = (uint32_t)g.ctx.size();
+    Many m{};
+    m.ss = g.d_states;
+    m.n = n;
+    for (uint32_t i = 0; i <= n; i++) m.start[i] = i;   // one workgroup each
+    kf_advance_tick<<<n, 1, 0, g.stream>>>(m);
+    enqueue_partition(q);
+    enqueue_lookup(q);
+    enqueue_refine(q);
+    enqueue_evaluate(q);
+    enqueue_schema(q);
+    kf_stamp<<<n, 1, 0, g.stream>>>(m);
+}
+
+// The settings that pick kernel variants; contexts alike in all of them can
+// share fused launches.
+static std::vector<uint64_t> variant_key(const glint_ctx *c)
+{
+    const uint64_t tile = c->s.geography ? 256 : (c->cfg.join_tile == 256 ? 256 : kTile);
+    const uint64_t nt = c->cfg.table_threads ? c->cfg.table_threads : 512;
+    return {tile, nt, c->kv ? 1ull : 0ull, c->s.geography, c->cfg.dynamic_schema};
+}
+
 // Checks, stamp buffer, the pre-run state (k_begin_run) and, on the first
 // run, the tree build from the first snapshot.
 // stamps: device buffer of 2*nticks stamps, or null for the context's own
@@ -4317,12 +4806,11 @@ glint_status glint_batch_destroy(glint_batch *b)
     if (!b) return GLINT_ERR_INVALID_ARGUMENT;
     if (b->origin) (void)hipStreamSynchronize(b->origin);
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
-    for (hipEvent_t e : b->join)
-        if (e) (void)hipEventDestroy(e);
     for (hipEvent_t e : b->ready)
         if (e) (void)hipEventDestroy(e);
-    if (b->fork) (void)hipEventDestroy(b->fork);
     if (b->after) (void)hipEventDestroy(b->after);
+    for (DevState *d : b->group_states)
+        if (d) (void)hipFree(d);
     for (glint_ctx *c : b->ctx) glint_destroy(c);
     if (b->origin) (void)hipStreamDestroy(b->origin);
     delete b;
@@ -4350,16 +4838,35 @@ glint_status glint_batch_create(const glint_config *cfgs, uint32_t n, uint32_t g
         if (st) return fail(st, "scenario " + std::to_string(i) + ": " + g_create_error);
         b->ctx.push_back(ctx);
     }
+    // contexts alike in every kernel variant share fused launches, at most
+    // kMaxFused to a group, in the order given
+    std::vector<std::vector<uint64_t>> keys;
+    for (uint32_t i = 0; i < n; i++) {
+        const std::vector<uint64_t> k = variant_key(b->ctx[i]);
+        size_t gi = 0;
+        while (gi < keys.size() && (keys[gi] != k || b->groups[gi].size() == kMaxFused)) gi++;
+        if (gi == keys.size()) {
+            keys.push_back(k);
+            b->groups.emplace_back();
+        }
+        b->groups[gi].push_back(i);
+    }
     if (hipSetDevice((int)cfgs[0].device) != hipSuccess ||
         hipStreamCreateWithFlags(&b->origin, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&b->after, hipEventDisableTiming) != hipSuccess)
         return fail(GLINT_ERR_HIP, "batch stream or event creation failed");
-    b->join.assign(n, nullptr);
+    b->group_states.assign(b->groups.size(), nullptr);
+    for (size_t gi = 0; gi < b->groups.size(); gi++) {
+        std::vector<DevState> h;
+        for (uint32_t i : b->groups[gi]) h.push_back(b->ctx[i]->s);
+        if (hipMalloc((void **)&b->group_states[gi], h.size() * sizeof(DevState)) != hipSuccess ||
+            hipMemcpy(b->group_states[gi], h.data(), h.size() * sizeof(DevState), hipMemcpyHostToDevice) !=
+                hipSuccess)
+            return fail(GLINT_ERR_HIP, "batch state upload failed");
+    }
     b->ready.assign(n, nullptr);
     for (uint32_t i = 0; i < n; i++)
-        if (hipEventCreateWithFlags(&b->join[i], hipEventDisableTiming) != hipSuccess ||
-            hipEventCreateWithFlags(&b->ready[i], hipEventDisableTiming) != hipSuccess)
+        if (hipEventCreateWithFlags(&b->ready[i], hipEventDisableTiming) != hipSuccess)
             return fail(GLINT_ERR_HIP, "batch event creation failed");
     *out = b;
     return GLINT_OK;
@@ -4373,6 +4880,13 @@ glint_status glint_batch_context(glint_batch *b, uint32_t i, glint_ctx **out)
 }
 
 const char *glint_batch_last_error(const glint_batch *b) { return b ? b->err.c_str() : g_create_error.c_str(); }
+
+glint_status glint_batch_groups(const glint_batch *b, uint32_t *groups)
+{
+    if (!b || !groups) return GLINT_ERR_INVALID_ARGUMENT;
+    *groups = (uint32_t)b->groups.size();
+    return GLINT_OK;
+}
 
 glint_status glint_batch_run_ticks(glint_batch *b, const double *const *d_traces, uint32_t first_tick,
                                    uint32_t nticks, float *tick_ms)
@@ -4394,7 +4908,7 @@ glint_status glint_batch_run_ticks(glint_batch *b, const double *const *d_traces
     for (uint32_t i = 0; i < n; i++) {
         if (!d_traces[i]) return GLINT_ERR_INVALID_ARGUMENT;
         if (b->ctx[i]->prof.on) {
-            // per-kernel events of the scenarios would serialise their streams
+            // fused launches cannot be timed per scenario
             b->err = "scenario " + std::to_string(i) + ": kernel profiling is on; a batch runs without it";
             return GLINT_ERR_STATE;
         }
@@ -4404,29 +4918,32 @@ glint_status glint_batch_run_ticks(glint_batch *b, const double *const *d_traces
         if (st) return ctx_fail(i, st);
     }
     BATCH_TRY(hipSetDevice(b->ctx[0]->device));
+    std::vector<FusedGroup> groups(b->groups.size());
+    for (size_t gi = 0; gi < groups.size(); gi++) {
+        for (uint32_t i : b->groups[gi]) groups[gi].ctx.push_back(b->ctx[i]);
+        groups[gi].d_states = b->group_states[gi];
+        groups[gi].stream = b->origin;
+    }
+    auto enqueue_ticks = [&](uint32_t k) {
+        for (; k > 0; k--)
+            for (const FusedGroup &g : groups) enqueue_tick_body(g);
+    };
+    // the batch's stream runs after every scenario's pre-run launches, and
+    // the scenarios' streams continue after it
+    for (uint32_t i = 0; i < n; i++) {
+        BATCH_TRY(hipEventRecord(b->ready[i], b->ctx[i]->stream));
+        BATCH_TRY(hipStreamWaitEvent(b->origin, b->ready[i], 0));
+    }
     const uint32_t G = b->graph_ticks;
     uint32_t done = 0;
     if (G && nticks >= G) {
         if (!b->exec) {
-            // every exit after the capture began ends it (the forked streams
-            // join the origin before the capture ends)
+            // G ticks of every group captured once; every exit after the
+            // capture began ends it
             hipGraph_t graph = nullptr;
             BATCH_TRY(hipStreamBeginCapture(b->origin, hipStreamCaptureModeThreadLocal));
-            hipError_t e = hipEventRecord(b->fork, b->origin);
-            uint32_t forked = 0;
-            for (uint32_t i = 0; i < n && e == hipSuccess; i++) {
-                glint_ctx *c = b->ctx[i];
-                e = hipStreamWaitEvent(c->stream, b->fork, 0);
-                if (e != hipSuccess) break;
-                forked = i + 1;
-                for (uint32_t k = 0; k < G; k++) enqueue_tick_body(c, 2ull * c->s.n);
-                e = hipGetLastError();
-            }
-            for (uint32_t i = 0; i < forked; i++) {
-                const hipError_t ej = hipEventRecord(b->join[i], b->ctx[i]->stream);
-                const hipError_t ew = ej == hipSuccess ? hipStreamWaitEvent(b->origin, b->join[i], 0) : ej;
-                if (e == hipSuccess) e = ew;
-            }
+            enqueue_ticks(G);
+            hipError_t e = hipGetLastError();
             const hipError_t ee = hipStreamEndCapture(b->origin, &graph);
             if (e == hipSuccess) e = ee;
             if (e == hipSuccess) e = hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0);
@@ -4437,19 +4954,12 @@ glint_status glint_batch_run_ticks(glint_batch *b, const double *const *d_traces
                 return GLINT_ERR_HIP;
             }
         }
-        // the graph runs on the origin stream after every scenario's pre-run
-        // launches, and the scenarios' streams continue after it
-        for (uint32_t i = 0; i < n; i++) {
-            BATCH_TRY(hipEventRecord(b->ready[i], b->ctx[i]->stream));
-            BATCH_TRY(hipStreamWaitEvent(b->origin, b->ready[i], 0));
-        }
         for (; done + G <= nticks; done += G) BATCH_TRY(hipGraphLaunch(b->exec, b->origin));
-        BATCH_TRY(hipEventRecord(b->after, b->origin));
-        for (uint32_t i = 0; i < n; i++) BATCH_TRY(hipStreamWaitEvent(b->ctx[i]->stream, b->after, 0));
     }
-    for (; done < nticks; done++)
-        for (uint32_t i = 0; i < n; i++) enqueue_tick_body(b->ctx[i], 2ull * b->ctx[i]->s.n);
+    enqueue_ticks(nticks - done);
     BATCH_TRY(hipGetLastError());
+    BATCH_TRY(hipEventRecord(b->after, b->origin));
+    for (uint32_t i = 0; i < n; i++) BATCH_TRY(hipStreamWaitEvent(b->ctx[i]->stream, b->after, 0));
 #undef BATCH_TRY
     glint_status first_err = GLINT_OK;
     std::vector<std::vector<unsigned long long>> stamps(n);
@@ -4468,8 +4978,8 @@ glint_status glint_batch_run_ticks(glint_batch *b, const double *const *d_traces
         }
     }
     if (tick_ms && !first_err) {
-        // each scenario's own tick times (device wall clock from its tick's
-        // start to its end, while the other scenarios share the device)
+        // each scenario's tick times: device wall clock from the start of its
+        // group's tick to its end
         const double khz = b->ctx[0]->wall_khz > 0 ? b->ctx[0]->wall_khz : 100000.0;
         for (uint32_t i = 0; i < n; i++)
             for (uint32_t k = 0; k < nticks; k++)

@@ -204,6 +204,8 @@ struct DevState {
     unsigned long long unit_cap;
     unsigned long long meet_cap;
     unsigned long long clist_cap;
+    uint32_t chunk;     // contact-list entries a wave of the join reserves at a time (64..4096)
+    uint32_t pad_chunk_;
     TickParams *tp;
     Node *nodes;
     NodeMeta *meta;

@@ -192,6 +192,7 @@ SIGNATURES = {
     "glint_batch_destroy": (_I, [_VP]),
     "glint_batch_context": (_I, [_VP, _U32, ctypes.POINTER(_VP)]),
     "glint_batch_last_error": (ctypes.c_char_p, [_VP]),
+    "glint_batch_groups": (_I, [_VP, ctypes.POINTER(_U32)]),
     "glint_tr_header_read": (_I, [ctypes.c_char_p, ctypes.POINTER(TrHeader)]),
     "glint_tr_read": (_I, [ctypes.c_char_p, _U32, _U32, _U32, _VP]),
     "glint_tr_read_chunk": (_I, [ctypes.c_char_p, _U32, _U32, _VP]),
@@ -695,6 +696,12 @@ class Batch:
             ch = ctypes.c_void_p()
             _check(lib().glint_batch_context(self.h, i, ctypes.byref(ch)), "glint_batch_context")
             self.contexts.append(Workbench(c, handle=ch))
+
+    def groups(self) -> int:
+        """Groups of scenarios that share fused launches."""
+        g = ctypes.c_uint32()
+        _check(lib().glint_batch_groups(self.h, ctypes.byref(g)), "glint_batch_groups")
+        return g.value
 
     def run_ticks(self, d_traces: list, first_tick: int, nticks: int, latencies: bool = True):
         ptrs = (ctypes.c_void_p * len(d_traces))(*[ctypes.c_void_p(p) for p in d_traces])

@@ -91,7 +91,9 @@ typedef struct glint_config {
     uint64_t num_meeting_buckets;  /* slots per table generation; 0: automatic (power of 2) */
     uint64_t unit_capacity;        /* border check units per tick; 0: automatic */
     uint64_t meeting_capacity;     /* meetings per tick; 0: automatic */
-    uint64_t contact_capacity;     /* recorded contacts per tick (record_contacts) */
+    uint64_t contact_capacity;     /* contacts the join may store per tick (all of them with
+                                      record_contacts, geography or min_meet_time 0, else those
+                                      the meeting table needs); 0: 16*N + 2^20 */
     uint32_t node_capacity;        /* quadtree nodes; 0: automatic */
     uint32_t record_contacts;      /* keep each tick's contact pairs for glint_contacts() */
     uint32_t graph_ticks;          /* glint_run_ticks: ticks per captured hipGraph (0: plain launches) */
@@ -303,11 +305,16 @@ glint_status glint_debug_inject_failure(glint_ctx *ctx, const char *stage);
 /* ----------------------------------------------------------------- batches
  * Many independent scenarios (cities of uneven size, each with its own
  * configuration) ticked together on one device: each scenario is a context
- * of its own, with its own stream and state, so its results are those of
- * the same context run alone.  With graph_ticks > 0, graph_ticks ticks of
- * every scenario are captured once in one hipGraph (the scenarios as
- * parallel branches) and replayed.  The reference runs one scenario per
- * process (process.cpp:378-634). */
+ * of its own, with its own state, so its results are those of the same
+ * context run alone.  Every step of the tick sequence is one launch over
+ * all scenarios: each scenario gets the workgroups its own launch would
+ * have, and every workgroup finds its scenario from its index.  Scenarios
+ * that differ in a setting that picks a kernel variant (join tile, table
+ * workgroup, key-value output, geography, schema mode) go to separate
+ * groups, one launch per step and group; a group holds at most 64.  With
+ * graph_ticks > 0, graph_ticks ticks of every group are captured once in
+ * one hipGraph and replayed.  The reference runs one scenario per process
+ * (process.cpp:378-634). */
 typedef struct glint_batch glint_batch;
 
 glint_status glint_batch_create(const glint_config *cfgs, uint32_t n, uint32_t graph_ticks, glint_batch **out);
@@ -316,6 +323,8 @@ glint_status glint_batch_destroy(glint_batch *batch);
  * (glint_stats, glint_active_pairs, glint_meeting_log_read, ...) works on it. */
 glint_status glint_batch_context(glint_batch *batch, uint32_t i, glint_ctx **out);
 const char *glint_batch_last_error(const glint_batch *batch);
+/* Number of groups of scenarios sharing fused launches. */
+glint_status glint_batch_groups(const glint_batch *batch, uint32_t *groups);
 /* nticks ticks of every scenario: scenario i reads tick k at
  * d_traces[i] + 2*N_i*k (device memory, FP64 AoS).  tick_ms (optional,
  * n*nticks floats): tick_ms[i*nticks + k] is the device wall time of

@@ -121,7 +121,8 @@ def test_hotspots_tick_by_tick_with_contacts(lon0, lat0):
     assert tot["contacts"] > 10_000_000 and tot["meetings"] > 100_000
     # leaves narrower than 4 m do not split: some hold more than a join tile
     with runtime.Workbench(runtime.make_config(w.num_objects, box, grid_capacity=w.grid_capacity,
-                                               geography=True, precision=runtime.PRECISION_FP64)) as wb:
+                                               geography=True, precision=runtime.PRECISION_FP64,
+                                               contact_capacity=1 << 24)) as wb:
         wb.tick(positions(0), 0)
         stats.append(wb.stats())
     assert stats[0]["max_leaf_count"] > 512
