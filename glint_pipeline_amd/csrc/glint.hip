@@ -520,16 +520,39 @@ __device__ __forceinline__ uint32_t descend_to_leaf(const Node *nodes, double x,
     return cur;
 }
 
+// One count per lane added to p[key] (kNone: nothing), one atomic per run of
+// equal keys in the wave.  Every lane of the wave must call it.
+__device__ __forceinline__ void wave_count(uint32_t *p, uint32_t key)
+{
+    bool need = key != kNone;
+    for (;;) {
+        const unsigned long long pend = __ballot(need);
+        if (!pend) break;
+        const uint32_t leader = __ffsll((long long)pend) - 1;
+        const uint32_t K = __shfl(key, leader);
+        const bool mine = need && key == K;
+        const unsigned long long mm = __ballot(mine);
+        if (lane_id() == leader) atomicAdd(&p[K], (uint32_t)__popcll(mm));
+        if (mine) need = false;
+    }
+}
+
+// Leaf populations of the tree built so far.  The first levels put most
+// objects in a few leaves, so the counts are aggregated per wave.
 __global__ void k_build_count(DevState s)
 {
     const double *xy = s.tp->xy;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < s.n; i += gridDim.x * blockDim.x) {
-        double x = xy[2 * (size_t)i], y = xy[2 * (size_t)i + 1];
-        if (!(x >= s.lx && x <= s.hx && y >= s.ly && y <= s.hy)) {
-            atomicOr(&s.ctr->error, kErrBadInput);
-            continue;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < s.n; base += gridDim.x * blockDim.x) {
+        const uint32_t i = base + threadIdx.x;
+        uint32_t leaf = kNone;
+        if (i < s.n) {
+            const double x = xy[2 * (size_t)i], y = xy[2 * (size_t)i + 1];
+            if (x >= s.lx && x <= s.hx && y >= s.ly && y <= s.hy)
+                leaf = descend_to_leaf(s.nodes, x, y);
+            else
+                atomicOr(&s.ctr->error, kErrBadInput);
         }
-        atomicAdd(&s.count[descend_to_leaf(s.nodes, x, y)], 1u);
+        wave_count(s.count, leaf);
     }
 }
 
@@ -798,23 +821,6 @@ __global__ void kf_fill(const Many m)
 }
 
 __host__ __device__ constexpr uint32_t pyr_off(uint32_t d) { return ((1u << (2 * d)) - 1u) / 3u; }
-
-// One count per lane added to p[key] (kNone: nothing), one atomic per run of
-// equal keys in the wave.  Every lane of the wave must call it.
-__device__ __forceinline__ void wave_count(uint32_t *p, uint32_t key)
-{
-    bool need = key != kNone;
-    for (;;) {
-        const unsigned long long pend = __ballot(need);
-        if (!pend) break;
-        const uint32_t leader = __ffsll((long long)pend) - 1;
-        const uint32_t K = __shfl(key, leader);
-        const bool mine = need && key == K;
-        const unsigned long long mm = __ballot(mine);
-        if (lane_id() == leader) atomicAdd(&p[K], (uint32_t)__popcll(mm));
-        if (mine) need = false;
-    }
-}
 
 // Object k of the pass: the leaf order of the tick before when there is one.
 __device__ __forceinline__ uint32_t grid_object(const DevState &s, uint32_t k, bool ordered)
@@ -2934,20 +2940,61 @@ struct TableLds {
     uint32_t hs[kRegion];             // their start tick
     uint32_t ho[KV ? kRegion : 1];    // list index of the old entry they continue (kNoSlot: none)
     glint_meeting stage[NT / 64][kMeetStage];
+    float4 sbox[KV ? NT / 64 : 1][KV ? kMeetStage : 1];   // the staged meetings' boxes
 };
 
+// Writes a wave's staged meetings out with one reservation; with the
+// key-value output, those whose box lies in the map box (geometry.cu:665-670,
+// box_contain geometry.cu:53-58) are also logged with their boxes, with one
+// more reservation.
 template <bool KV, int NT>
-__device__ __forceinline__ void table_flush_stage(const DevState &s, glint_meeting *st, uint32_t &nst)
+__device__ __forceinline__ void table_flush_stage(const DevState &s, glint_meeting *st, const float4 *sb,
+                                                  uint32_t &nst)
 {
     wave_sync();
+    const uint32_t lane = lane_id();
     unsigned long long b = 0;
-    if (lane_id() == 0) b = atomicAdd(&s.ctr->n_meet, (unsigned long long)nst);
+    if (lane == 0) b = atomicAdd(&s.ctr->n_meet, (unsigned long long)nst);
     b = __shfl(b, 0);
-    for (uint32_t j = lane_id(); j < nst; j += 64) {
+    for (uint32_t j = lane; j < nst; j += 64) {
         if (b + j < s.meet_cap)
             s.meet[b + j] = st[j];
         else
             atomicOr(&s.ctr->error, kErrMeetCap);
+    }
+    if constexpr (KV) {
+        auto in_map = [&](uint32_t j) {
+            if (j >= nst) return false;
+            const float4 bx = sb[j];
+            return bx.x >= s.lx && bx.z <= s.hx && bx.y >= s.ly && bx.w <= s.hy;
+        };
+        uint32_t total = 0;
+        for (uint32_t j0 = 0; j0 < nst; j0 += 64) total += (uint32_t)__popcll(__ballot(in_map(j0 + lane)));
+        if (total) {
+            unsigned long long k = 0;
+            if (lane == 0) k = atomicAdd(&s.ctr->n_kvlog, (unsigned long long)total);
+            k = __shfl(k, 0);
+            for (uint32_t j0 = 0; j0 < nst; j0 += 64) {
+                const uint32_t j = j0 + lane;
+                const bool put = in_map(j);
+                const unsigned long long m = __ballot(put);
+                if (put) {
+                    const unsigned long long at = k + __popcll(m & ((1ull << lane) - 1ull));
+                    if (at < s.kvlog_cap) {
+                        KvLogEntry e;
+                        e.pid_a = st[j].pid_a;
+                        e.pid_b = st[j].pid_b;
+                        e.start = st[j].start;
+                        e.end = st[j].end;
+                        e.box = sb[j];
+                        s.kvlog[at] = e;
+                    } else {
+                        atomicOr(&s.ctr->error, kErrKvCap);
+                    }
+                }
+                k += (unsigned long long)__popcll(m);
+            }
+        }
     }
     wave_sync();
     nst = 0;
@@ -2996,7 +3043,7 @@ __device__ __forceinline__ uint32_t table_find(const TableLds<KV, NT> &L, unsign
 template <bool KV, int NT>
 __device__ __forceinline__ void table_old(TableLds<KV, NT> &L, const DevState &s, bool valid,
                                           unsigned long long tagged, uint32_t start, uint32_t i, uint32_t tick,
-                                          uint32_t last, glint_meeting *st, uint32_t &nst)
+                                          uint32_t last, glint_meeting *st, float4 *sb, uint32_t &nst)
 {
     valid = valid && start != kDupStart;
     const unsigned long long key = tagged & kKeyBits;
@@ -3018,37 +3065,12 @@ __device__ __forceinline__ void table_old(TableLds<KV, NT> &L, const DevState &s
         mt.pid_b = (uint32_t)key;
         mt.start = start;
         mt.end = last;
-        st[nst + __popcll(m & ((1ull << lane) - 1ull))] = mt;
+        const uint32_t at = nst + __popcll(m & ((1ull << lane) - 1ull));
+        st[at] = mt;
+        if constexpr (KV) sb[at] = s.tp->mbr_old[i];   // the run's box, logged when the stage is written
     }
     nst += (uint32_t)__popcll(m);
-    if constexpr (KV) {
-        // meetings whose box lies in the map box are logged
-        // (geometry.cu:665-670, box_contain geometry.cu:53-58)
-        float4 bx = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (emit) bx = s.tp->mbr_old[i];
-        const bool put = emit && bx.x >= s.lx && bx.z <= s.hx && bx.y >= s.ly && bx.w <= s.hy;
-        const unsigned long long mk = __ballot(put);
-        if (mk) {
-            unsigned long long b = 0;
-            if (lane == 0) b = atomicAdd(&s.ctr->n_kvlog, (unsigned long long)__popcll(mk));
-            b = __shfl(b, 0);
-            if (put) {
-                const unsigned long long j = b + __popcll(mk & ((1ull << lane) - 1ull));
-                if (j < s.kvlog_cap) {
-                    KvLogEntry e;
-                    e.pid_a = (uint32_t)(key >> 32);
-                    e.pid_b = (uint32_t)key;
-                    e.start = start;
-                    e.end = last;
-                    e.box = bx;
-                    s.kvlog[j] = e;
-                } else {
-                    atomicOr(&s.ctr->error, kErrKvCap);
-                }
-            }
-        }
-    }
-    if (nst > kMeetStage - 64) table_flush_stage<KV, NT>(s, st, nst);
+    if (nst > kMeetStage - 64) table_flush_stage<KV, NT>(s, st, sb, nst);
 }
 
 // Start (and, with the key-value output, box) of contact c, whose hash slot
@@ -3118,6 +3140,7 @@ __device__ __forceinline__ void table_body(const DevState s, const P blk)
     // old region range of region rr: rstart[(rr >> dn) << up] .. rstart[((rr >> dn) + 1) << up]
     const uint32_t up = pbits > bits ? pbits - bits : 0u, dn = bits > pbits ? bits - pbits : 0u;
     glint_meeting *st = L.stage[threadIdx.x / 64];
+    float4 *sb = KV ? L.sbox[threadIdx.x / 64] : nullptr;
     uint32_t nst = 0, dups = 0;
     const bool void_tick = (s.ctr->error & kErrContactCap) != 0;
     // the previous generation's entries and the tick's contacts of the next
@@ -3181,20 +3204,20 @@ __device__ __forceinline__ void table_body(const DevState s, const P blk)
             for (int q = 0; q < kPer; q++) {
                 const uint32_t i = o0 + threadIdx.x + q * NT;
                 const bool mine = region_of(pk[q] & kKeyBits, bits) == r;
-                table_old(L, s, i < o1 && mine, pk[q], ps[q], i, tick, last, st, nst);
+                table_old(L, s, i < o1 && mine, pk[q], ps[q], i, tick, last, st, sb, nst);
             }
         } else {
 #pragma unroll
             for (int q = 0; q < kPer; q++) {
                 const uint32_t i = o0 + threadIdx.x + q * NT;
-                table_old(L, s, i < o1, pk[q], ps[q], i, tick, last, st, nst);
+                table_old(L, s, i < o1, pk[q], ps[q], i, tick, last, st, sb, nst);
             }
         }
         for (uint32_t b = o0 + kRegion; b < o1; b += NT) {   // regions of more than kRegion entries
             const uint32_t i = b + threadIdx.x;
             const unsigned long long k = i < o1 ? told.keys[i] : 0ull;
             const bool v = i < o1 && (!filter_old || region_of(k & kKeyBits, bits) == r);
-            table_old(L, s, v, k, v ? told.starts[i] : kDupStart, i, tick, last, st, nst);
+            table_old(L, s, v, k, v ? told.starts[i] : kDupStart, i, tick, last, st, sb, nst);
         }
         const uint32_t rn = r + blk.count();
         if (rn < nreg) prefetch(rn);
@@ -3216,7 +3239,7 @@ __device__ __forceinline__ void table_body(const DevState s, const P blk)
         }
         lds_barrier();
     }
-    if (nst) table_flush_stage<KV, NT>(s, st, nst);
+    if (nst) table_flush_stage<KV, NT>(s, st, sb, nst);
     // distinct pairs of the tick: contacts less second copies
     for (int d = 32; d >= 1; d >>= 1) dups += __shfl_xor(dups, d);
     if (lane_id() == 0 && dups) atomicAdd(&s.ctr->n_dup, (unsigned long long)dups);

@@ -7,10 +7,11 @@
 //    aggregates the reference updates with atomics during evaluation (key
 //    counts, uint boxes, duration-oversize flags) are rebuilt from the log at
 //    flush time, which keeps them out of the per-tick path;
-//  - every sort is one hand-written stable LSD radix sort (4-bit digits,
-//    256-thread blocks ranking 2048-item tiles in LDS) that visits only the
-//    digits whose bits differ across the keys; the STR slab sort is one sort
-//    of (slab + 1, y) keys;
+//  - every sort is a stable LSD radix sort over only the bits that differ
+//    across the keys (rocPRIM's onesweep sort of 64-bit keys with 32-bit
+//    values; a 128-bit key is sorted by its low word, then stably by its high
+//    word, the rows following through index permutations); the STR slab sort
+//    is one sort of (slab + 1, y) keys;
 //  - per-file limits, bitmaps (in LDS) and packed keys run one workgroup per
 //    file.
 // Results are the reference's: the final key order is the total order of the
@@ -27,6 +28,8 @@
 #include <new>
 #include <vector>
 
+#include <rocprim/device/device_radix_sort.hpp>
+
 #include "kv_engine.h"
 
 using glint::KvLogEntry;
@@ -34,9 +37,6 @@ using glint::KvLogEntry;
 namespace {
 
 constexpr uint32_t kB = 256;
-constexpr uint32_t kRItems = 8;                   // consecutive items per thread in a radix tile
-constexpr uint32_t kRTile = kB * kRItems;         // 2048 items per tile
-constexpr uint32_t kRMaxBlocks = 1024;
 constexpr uint32_t kBitmapEdge = 256;
 constexpr uint32_t kBitmapBytes = kBitmapEdge * kBitmapEdge / 8;   // bench->bit_count / 8 per file
 constexpr uint64_t kLow52 = (1ull << 52) - 1;     // key bits below the sid (bit 116 = hi bit 52)
@@ -92,127 +92,20 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *t
 
 // --------------------------------------------------------------- radix sort
 
-__device__ __forceinline__ void tile_range(uint64_t n, uint64_t per, uint64_t &b0, uint64_t &b1)
+__global__ __launch_bounds__(kB) void k_iota(uint32_t *p, uint64_t n)
 {
-    b0 = min(n, per * blockIdx.x);
-    b1 = min(n, b0 + per);
+    for (uint64_t i = blockIdx.x * (uint64_t)kB + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kB)
+        p[i] = (uint32_t)i;
 }
 
-template <bool WIDE>
-__device__ __forceinline__ uint32_t digit_of(const uint64_t *hi, const uint64_t *lo, uint64_t i, uint32_t shift)
+// dst64[i] = src64[perm[i]], dst32[i] = src32[perm[i]].
+__global__ __launch_bounds__(kB) void k_permute(const uint32_t *perm, const uint64_t *src64, uint64_t *dst64,
+                                                const uint32_t *src32, uint32_t *dst32, uint64_t n)
 {
-    if (WIDE && shift >= 64) return (uint32_t)(hi[i] >> (shift - 64)) & 15u;
-    return (uint32_t)(lo[i] >> shift) & 15u;
-}
-
-// Digit histogram of each block's range: counts[d * gridDim.x + block].
-template <bool WIDE>
-__global__ __launch_bounds__(kB) void k_rcount(const uint64_t *hi, const uint64_t *lo, uint64_t n, uint64_t per,
-                                               uint32_t shift, uint32_t *counts)
-{
-    __shared__ uint32_t h[16];
-    if (threadIdx.x < 16) h[threadIdx.x] = 0;
-    __syncthreads();
-    uint64_t b0, b1;
-    tile_range(n, per, b0, b1);
-    for (uint64_t i0 = b0; i0 < b1; i0 += kB) {
-        const uint64_t i = i0 + threadIdx.x;
-        const uint32_t d = i < b1 ? digit_of<WIDE>(hi, lo, i, shift) : 16u;
-        for (uint32_t k = 0; k < 16; k++) {
-            const unsigned long long m = __ballot(d == k);
-            if (lane() == 0 && m) atomicAdd(&h[k], (uint32_t)__popcll(m));
-        }
-    }
-    __syncthreads();
-    if (threadIdx.x < 16) counts[threadIdx.x * gridDim.x + blockIdx.x] = h[threadIdx.x];
-}
-
-// Exclusive scan of a small array in place, one workgroup.
-__global__ __launch_bounds__(kB) void k_scan_small(uint32_t *a, uint32_t n)
-{
-    __shared__ uint32_t wsum[kB / 64];
-    uint32_t carry = 0;
-    for (uint32_t b = 0; b < n; b += kB) {
-        const uint32_t i = b + threadIdx.x;
-        const uint32_t v = i < n ? a[i] : 0;
-        uint32_t total;
-        const uint32_t ex = block_exclusive_scan(v, &total, wsum);
-        if (i < n) a[i] = carry + ex;
-        carry += total;
-    }
-}
-
-// Stable scatter by one digit.  Each thread ranks its 8 consecutive items;
-// the (digit, thread) counts of a tile are scanned in digit-major order in
-// LDS, so items keep their order within every digit.
-template <bool WIDE>
-__global__ __launch_bounds__(kB) void k_rscatter(SortView src, SortView dst, uint64_t n, uint64_t per,
-                                                 uint32_t shift, const uint32_t *offs)
-{
-    __shared__ uint32_t cnt[16 * kB];
-    __shared__ uint32_t base[16];
-    __shared__ uint32_t wsum[kB / 64];
-    uint64_t b0, b1;
-    tile_range(n, per, b0, b1);
-    if (threadIdx.x < 16) base[threadIdx.x] = offs[threadIdx.x * gridDim.x + blockIdx.x];
-    for (uint64_t t0 = b0; t0 < b1; t0 += kRTile) {
-        uint64_t kh[kRItems], kl[kRItems];
-        uint32_t kv[kRItems];
-        uint32_t digits = 0;
-        uint64_t counts = 0;   // 16 nibbles
-#pragma unroll
-        for (uint32_t q = 0; q < kRItems; q++) {
-            const uint64_t i = t0 + threadIdx.x * kRItems + q;
-            kh[q] = 0;
-            kl[q] = 0;
-            kv[q] = 0;
-            if (i < b1) {
-                if (WIDE) kh[q] = src.hi[i];
-                kl[q] = src.lo[i];
-                kv[q] = src.val[i];
-                const uint32_t d = (WIDE && shift >= 64) ? (uint32_t)(kh[q] >> (shift - 64)) & 15u
-                                                         : (uint32_t)(kl[q] >> shift) & 15u;
-                digits |= d << (4 * q);
-                counts += 1ull << (4 * d);
-            }
-        }
-#pragma unroll
-        for (uint32_t d = 0; d < 16; d++) cnt[d * kB + threadIdx.x] = (uint32_t)(counts >> (4 * d)) & 15u;
-        __syncthreads();
-        uint32_t v[16], sum = 0;
-#pragma unroll
-        for (uint32_t j = 0; j < 16; j++) {
-            v[j] = cnt[16 * threadIdx.x + j];
-            sum += v[j];
-        }
-        uint32_t total;
-        uint32_t ex = block_exclusive_scan(sum, &total, wsum);
-#pragma unroll
-        for (uint32_t j = 0; j < 16; j++) {
-            cnt[16 * threadIdx.x + j] = ex;
-            ex += v[j];
-        }
-        __syncthreads();
-        uint64_t run = 0;
-#pragma unroll
-        for (uint32_t q = 0; q < kRItems; q++) {
-            const uint64_t i = t0 + threadIdx.x * kRItems + q;
-            if (i < b1) {
-                const uint32_t d = (digits >> (4 * q)) & 15u;
-                const uint32_t r = (uint32_t)(run >> (4 * d)) & 15u;
-                const uint64_t pos = (uint64_t)base[d] + (cnt[d * kB + threadIdx.x] - cnt[d * kB]) + r;
-                if (WIDE) dst.hi[pos] = kh[q];
-                dst.lo[pos] = kl[q];
-                dst.val[pos] = kv[q];
-                run += 1ull << (4 * d);
-            }
-        }
-        __syncthreads();
-        if (threadIdx.x < 16) {
-            const uint32_t d = threadIdx.x;
-            base[d] += (d < 15 ? cnt[(d + 1) * kB] : total) - cnt[d * kB];
-        }
-        __syncthreads();
+    for (uint64_t i = blockIdx.x * (uint64_t)kB + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kB) {
+        const uint32_t j = perm[i];
+        dst64[i] = src64[j];
+        dst32[i] = src32[j];
     }
 }
 
@@ -295,10 +188,13 @@ __global__ __launch_bounds__(kB) void k_expand(const KvLogEntry *log, uint64_t n
             out.val[j] = (uint32_t)j;
             boxes[j] = e.box;
             atomicAdd(&cnt[pid], 1u);
-            atomicMin(&obox[pid].x, bx0);
-            atomicMin(&obox[pid].y, by0);
-            atomicMax(&obox[pid].z, bx1);
-            atomicMax(&obox[pid].w, by1);
+            // the box only shrinks towards its final value: an update that
+            // the current value already covers needs no atomic
+            const uint4 cur = obox[pid];   // an older value only costs an atomic more
+            if (bx0 < cur.x) atomicMin(&obox[pid].x, bx0);
+            if (by0 < cur.y) atomicMin(&obox[pid].y, by0);
+            if (bx1 > cur.z) atomicMax(&obox[pid].z, bx1);
+            if (by1 > cur.w) atomicMax(&obox[pid].w, by1);
         }
     }
     for (int d = 32; d >= 1; d >>= 1) ovs += __shfl_xor(ovs, d);
@@ -340,36 +236,75 @@ __global__ __launch_bounds__(kB) void k_objects(uint32_t n, const uint32_t *cnt,
 }
 
 // Keys of sid-1 objects to the tail, the rest to the head (geometry.cu:1776-1797).
+// Every workgroup takes 4096 keys at a time and reserves its places with one
+// atomic per side (the order within each side is free: both are sorted
+// afterwards).
+constexpr uint32_t kPartItems = 16;
 __global__ __launch_bounds__(kB) void k_partition(SortView in, SortView out, uint64_t n, const uint32_t *sids,
                                                   KvCtr *ctr)
 {
-    for (uint64_t i0 = blockIdx.x * (uint64_t)kB; i0 < n; i0 += (uint64_t)gridDim.x * kB) {
-        const uint64_t i = i0 + threadIdx.x;
-        const bool valid = i < n;
-        uint64_t h = 0, l = 0;
-        uint32_t v = 0;
-        bool keep = false;
-        if (valid) {
-            h = in.hi[i];
-            l = in.lo[i];
-            v = in.val[i];
-            keep = sids[(uint32_t)(h >> 26) & kOidMask] != 1;
+    __shared__ uint32_t wk[kB / 64], wo[kB / 64];
+    __shared__ unsigned long long base_k, base_o;
+    const uint32_t w = threadIdx.x / 64;
+    const unsigned long long below = (1ull << lane()) - 1ull;
+    for (uint64_t t0 = blockIdx.x * (uint64_t)kB * kPartItems; t0 < n; t0 += (uint64_t)gridDim.x * kB * kPartItems) {
+        uint64_t h[kPartItems], l[kPartItems];
+        uint32_t v[kPartItems];
+        uint32_t keepbits = 0, validbits = 0;
+#pragma unroll
+        for (uint32_t q = 0; q < kPartItems; q++) {
+            const uint64_t i = t0 + (uint64_t)q * kB + threadIdx.x;   // coalesced rows
+            h[q] = 0;
+            l[q] = 0;
+            v[q] = 0;
+            if (i < n) {
+                h[q] = in.hi[i];
+                l[q] = in.lo[i];
+                v[q] = in.val[i];
+                validbits |= 1u << q;
+                if (sids[(uint32_t)(h[q] >> 26) & kOidMask] != 1) keepbits |= 1u << q;
+            }
         }
-        const unsigned long long mk = __ballot(valid && keep), mo = __ballot(valid && !keep);
-        unsigned long long bk = 0, bo = 0;
+        // this wave's counts, then the workgroup's, then one reservation per side
+        uint32_t ck = 0, co = 0;
+#pragma unroll
+        for (uint32_t q = 0; q < kPartItems; q++) {
+            ck += (uint32_t)__popcll(__ballot((keepbits >> q) & 1u));
+            co += (uint32_t)__popcll(__ballot(((validbits & ~keepbits) >> q) & 1u));
+        }
         if (lane() == 0) {
-            if (mk) bk = atomicAdd(&ctr->nk, (unsigned long long)__popcll(mk));
-            if (mo) bo = atomicAdd(&ctr->no, (unsigned long long)__popcll(mo));
+            wk[w] = ck;
+            wo[w] = co;
         }
-        bk = __shfl(bk, 0);
-        bo = __shfl(bo, 0);
-        if (valid) {
-            const unsigned long long below = (1ull << lane()) - 1ull;
-            const uint64_t pos = keep ? bk + __popcll(mk & below) : n - 1 - (bo + __popcll(mo & below));
-            out.hi[pos] = h;
-            out.lo[pos] = l;
-            out.val[pos] = v;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t tk = 0, to = 0;
+            for (uint32_t k = 0; k < kB / 64; k++) {
+                const uint32_t a = wk[k], b = wo[k];
+                wk[k] = tk;
+                wo[k] = to;
+                tk += a;
+                to += b;
+            }
+            base_k = tk ? atomicAdd(&ctr->nk, (unsigned long long)tk) : 0;
+            base_o = to ? atomicAdd(&ctr->no, (unsigned long long)to) : 0;
         }
+        __syncthreads();
+        unsigned long long pk = base_k + wk[w], po = base_o + wo[w];
+#pragma unroll
+        for (uint32_t q = 0; q < kPartItems; q++) {
+            const bool keep = (keepbits >> q) & 1u, other = ((validbits & ~keepbits) >> q) & 1u;
+            const unsigned long long mk = __ballot(keep), mo = __ballot(other);
+            if (keep || other) {
+                const uint64_t pos = keep ? pk + __popcll(mk & below) : n - 1 - (po + __popcll(mo & below));
+                out.hi[pos] = h[q];
+                out.lo[pos] = l[q];
+                out.val[pos] = v[q];
+            }
+            pk += (unsigned long long)__popcll(mk);
+            po += (unsigned long long)__popcll(mo);
+        }
+        __syncthreads();
     }
 }
 
@@ -695,7 +630,10 @@ struct KvEngine {
     KvLogEntry *log = nullptr;
     SortView a{}, b{};
     float4 *box_in = nullptr, *box_out = nullptr;
-    uint32_t *cnt = nullptr, *sids = nullptr, *cap = nullptr, *rcounts = nullptr;
+    uint32_t *cnt = nullptr, *sids = nullptr, *cap = nullptr;
+    uint32_t *idx0 = nullptr, *idx1 = nullptr;   // row indices of the two-word sorts
+    void *sort_tmp = nullptr;                    // the sort's temporary storage
+    size_t sort_tmp_bytes = 0;
     uint4 *obox = nullptr;
     SortView oa{}, ob{};   // (mid, oid) pairs; hi unused
     glint_ctf *ctfs = nullptr;
@@ -741,23 +679,25 @@ glint_status read_ctr(KvEngine *e, std::string &err)
     return GLINT_OK;
 }
 
-// Digits (shifts) whose bits differ across the n keys of v, ascending.
-glint_status varying_digits(KvEngine *e, const SortView &v, uint64_t n, bool wide, std::vector<uint32_t> &shifts,
-                            std::string &err)
+// The bits that differ across the n keys of v: [*_lo, *_hi) of the low and
+// the high word (empty ranges: lo >= hi).
+struct VaryingBits {
+    uint32_t lo_begin, lo_end, hi_begin, hi_end;
+};
+
+glint_status varying_bits(KvEngine *e, const SortView &v, uint64_t n, bool wide, VaryingBits &vb,
+                          std::string &err)
 {
-    shifts.clear();
+    vb = VaryingBits{0, 0, 0, 0};
     if (n < 2) return GLINT_OK;
     k_ctr_reset_masks<<<1, 1, 0, e->st>>>(e->ctr);
     k_andor<<<grid_of(n, 1024), kB, 0, e->st>>>(wide ? v.hi : nullptr, v.lo, n, e->ctr);
     KV_TRY(hipGetLastError());
     glint_status s = read_ctr(e, err);
     if (s) return s;
-    const uint64_t dh = e->h_ctr->or_hi ^ e->h_ctr->and_hi, dl = e->h_ctr->or_lo ^ e->h_ctr->and_lo;
-    for (uint32_t sh = 0; sh < 64; sh += 4)
-        if ((dl >> sh) & 15u) shifts.push_back(sh);
-    if (wide)
-        for (uint32_t sh = 0; sh < 64; sh += 4)
-            if ((dh >> sh) & 15u) shifts.push_back(64 + sh);
+    const uint64_t dh = wide ? e->h_ctr->or_hi ^ e->h_ctr->and_hi : 0, dl = e->h_ctr->or_lo ^ e->h_ctr->and_lo;
+    if (dl) vb.lo_begin = (uint32_t)__builtin_ctzll(dl), vb.lo_end = 64 - (uint32_t)__builtin_clzll(dl);
+    if (dh) vb.hi_begin = (uint32_t)__builtin_ctzll(dh), vb.hi_end = 64 - (uint32_t)__builtin_clzll(dh);
     return GLINT_OK;
 }
 
@@ -766,34 +706,64 @@ SortView offset(const SortView &v, uint64_t k)
     return SortView{v.hi ? v.hi + k : nullptr, v.lo + k, v.val + k};
 }
 
-// Stable LSD sort of n items of `a` by the varying digits, `b` as scratch;
-// the result is left in `a`.
+// Stable sort of n (key, value) pairs by key bits [b0, b1): the pairs start
+// in (k0, v0), with (k1, v1) as scratch; returns whether they ended in the
+// scratch pair.
+glint_status sort_pairs(KvEngine *e, uint64_t *k0, uint64_t *k1, uint32_t *v0, uint32_t *v1, uint64_t n, uint32_t b0,
+                        uint32_t b1, bool &in_scratch, std::string &err)
+{
+    rocprim::double_buffer<uint64_t> keys(k0, k1);
+    rocprim::double_buffer<uint32_t> vals(v0, v1);
+    size_t bytes = e->sort_tmp_bytes;
+    KV_TRY(rocprim::radix_sort_pairs(e->sort_tmp, bytes, keys, vals, (size_t)n, b0, b1, e->st));
+    in_scratch = keys.current() != k0;
+    return GLINT_OK;
+}
+
+// Stable sort of n items of `a` by their keys (the high word, if wide, then
+// the low word), `b` as scratch; the result is left in `a`.  A wide key is
+// sorted by its low word carrying the row index, the rows permuted, then by
+// its high word carrying the row index again, and the rows permuted; only
+// the bits that differ across the keys are sorted.
 glint_status radix_sort(KvEngine *e, SortView a, SortView b, uint64_t n, bool wide, std::string &err)
 {
-    std::vector<uint32_t> shifts;
-    glint_status s = varying_digits(e, a, n, wide, shifts, err);
-    if (s || shifts.empty()) return s;
-    const uint32_t G = (uint32_t)std::min<uint64_t>(kRMaxBlocks, (n + kRTile - 1) / kRTile);
-    const uint64_t per = ((n + G - 1) / G + kRTile - 1) / kRTile * kRTile;
-    SortView src = a, dst = b;
-    for (uint32_t sh : shifts) {
-        if (wide) {
-            k_rcount<true><<<G, kB, 0, e->st>>>(src.hi, src.lo, n, per, sh, e->rcounts);
-            k_scan_small<<<1, kB, 0, e->st>>>(e->rcounts, 16 * G);
-            k_rscatter<true><<<G, kB, 0, e->st>>>(src, dst, n, per, sh, e->rcounts);
-        } else {
-            k_rcount<false><<<G, kB, 0, e->st>>>(nullptr, src.lo, n, per, sh, e->rcounts);
-            k_scan_small<<<1, kB, 0, e->st>>>(e->rcounts, 16 * G);
-            k_rscatter<false><<<G, kB, 0, e->st>>>(src, dst, n, per, sh, e->rcounts);
+    VaryingBits vb;
+    glint_status s = varying_bits(e, a, n, wide, vb, err);
+    if (s) return s;
+    const uint32_t g = grid_of(n);
+    bool scratch = false;
+    if (!wide) {
+        if (vb.lo_begin >= vb.lo_end) return GLINT_OK;
+        s = sort_pairs(e, a.lo, b.lo, a.val, b.val, n, vb.lo_begin, vb.lo_end, scratch, err);
+        if (s) return s;
+        if (scratch) {
+            KV_TRY(hipMemcpyAsync(a.lo, b.lo, n * 8, hipMemcpyDeviceToDevice, e->st));
+            KV_TRY(hipMemcpyAsync(a.val, b.val, n * 4, hipMemcpyDeviceToDevice, e->st));
         }
-        std::swap(src, dst);
+        return GLINT_OK;
+    }
+    // rows (hi, lo, val) live in a; b and the two index buffers are scratch
+    if (vb.lo_begin < vb.lo_end) {
+        k_iota<<<g, kB, 0, e->st>>>(e->idx0, n);
+        s = sort_pairs(e, a.lo, b.lo, e->idx0, e->idx1, n, vb.lo_begin, vb.lo_end, scratch, err);
+        if (s) return s;
+        const uint32_t *perm = scratch ? e->idx1 : e->idx0;
+        if (scratch) KV_TRY(hipMemcpyAsync(a.lo, b.lo, n * 8, hipMemcpyDeviceToDevice, e->st));
+        k_permute<<<g, kB, 0, e->st>>>(perm, a.hi, b.hi, a.val, b.val, n);
+        KV_TRY(hipMemcpyAsync(a.hi, b.hi, n * 8, hipMemcpyDeviceToDevice, e->st));
+        KV_TRY(hipMemcpyAsync(a.val, b.val, n * 4, hipMemcpyDeviceToDevice, e->st));
+    }
+    if (vb.hi_begin < vb.hi_end) {
+        k_iota<<<g, kB, 0, e->st>>>(e->idx0, n);
+        s = sort_pairs(e, a.hi, b.hi, e->idx0, e->idx1, n, vb.hi_begin, vb.hi_end, scratch, err);
+        if (s) return s;
+        const uint32_t *perm = scratch ? e->idx1 : e->idx0;
+        if (scratch) KV_TRY(hipMemcpyAsync(a.hi, b.hi, n * 8, hipMemcpyDeviceToDevice, e->st));
+        k_permute<<<g, kB, 0, e->st>>>(perm, a.lo, b.lo, a.val, b.val, n);
+        KV_TRY(hipMemcpyAsync(a.lo, b.lo, n * 8, hipMemcpyDeviceToDevice, e->st));
+        KV_TRY(hipMemcpyAsync(a.val, b.val, n * 4, hipMemcpyDeviceToDevice, e->st));
     }
     KV_TRY(hipGetLastError());
-    if (src.lo != a.lo) {
-        if (wide) KV_TRY(hipMemcpyAsync(a.hi, src.hi, n * 8, hipMemcpyDeviceToDevice, e->st));
-        KV_TRY(hipMemcpyAsync(a.lo, src.lo, n * 8, hipMemcpyDeviceToDevice, e->st));
-        KV_TRY(hipMemcpyAsync(a.val, src.val, n * 4, hipMemcpyDeviceToDevice, e->st));
-    }
     return GLINT_OK;
 }
 
@@ -827,7 +797,7 @@ glint_status kv_create(const KvParams &p, hipStream_t stream, KvEngine **out, st
               kalloc(e, &e->cnt, N, err) && kalloc(e, &e->sids, N, err) && kalloc(e, &e->obox, N, err) &&
               kalloc(e, &e->oa.lo, N, err) && kalloc(e, &e->oa.val, N, err) && kalloc(e, &e->ob.lo, N, err) &&
               kalloc(e, &e->ob.val, N, err) && kalloc(e, &e->cap, e->nctf, err) &&
-              kalloc(e, &e->rcounts, 16 * kRMaxBlocks, err) && kalloc(e, &e->ctfs, e->nctf, err) &&
+              kalloc(e, &e->idx0, K, err) && kalloc(e, &e->idx1, K, err) && kalloc(e, &e->ctfs, e->nctf, err) &&
               kalloc(e, &e->bitmaps, (uint64_t)e->nctf * (kBitmapBytes / 4), err) &&
               kalloc(e, &e->obitmap, kBitmapBytes / 4, err) && kalloc(e, &e->packed, 16 * K + 16, err) &&
               kalloc(e, &e->ctr, 1, err);
@@ -838,6 +808,20 @@ glint_status kv_create(const KvParams &p, hipStream_t stream, KvEngine **out, st
     if (!ok) {
         kv_destroy(e);
         return GLINT_ERR_OUT_OF_MEMORY;
+    }
+    {
+        // temporary storage of the largest sort (every bit of a 64-bit key)
+        rocprim::double_buffer<uint64_t> keys(e->a.lo, e->b.lo);
+        rocprim::double_buffer<uint32_t> vals(e->idx0, e->idx1);
+        size_t bytes = 0;
+        const uint64_t most = std::max<uint64_t>(K, N);
+        if (rocprim::radix_sort_pairs(nullptr, bytes, keys, vals, (size_t)most, 0, 64, stream) != hipSuccess ||
+            !kalloc(e, reinterpret_cast<uint8_t **>(&e->sort_tmp), bytes, err)) {
+            if (err.empty()) err = "key-value output: sort storage query failed";
+            kv_destroy(e);
+            return GLINT_ERR_OUT_OF_MEMORY;
+        }
+        e->sort_tmp_bytes = bytes;
     }
     std::memset(e->h_ctr, 0, sizeof(KvCtr));
     *out = e;
