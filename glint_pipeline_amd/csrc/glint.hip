@@ -3079,13 +3079,28 @@ __device__ __forceinline__ void table_old(TableLds<KV, NT> &L, const DevState &s
 // generation either is new or was a contact of the tick before that was not
 // stored; the join set kTagTested when the join of the tick before tested
 // the pair, which tells which.
+// The pair's two positions at this tick and at the tick before (key-value
+// output only; both in one 16-byte gather per object).
+struct PairPos {
+    float4 a, b;
+};
+
+template <bool KV>
+__device__ __forceinline__ PairPos pair_pos(const DevState &s, unsigned long long tagged)
+{
+    PairPos p{};
+    if constexpr (KV) {
+        p.a = s.pos32[(uint32_t)((tagged & kKeyBits) >> 32)];
+        p.b = s.pos32[(uint32_t)(tagged & kKeyBits)];
+    }
+    return p;
+}
+
 template <bool KV, int NT>
 __device__ __forceinline__ void table_write(const TableLds<KV, NT> &L, const DevState &s, const TableGen &tcur,
                                             unsigned long long tagged, uint32_t c, uint32_t q, uint32_t tick,
-                                            uint32_t last)
+                                            uint32_t last, const PairPos pp)
 {
-    const unsigned long long key = tagged & kKeyBits;
-    const uint32_t a = (uint32_t)(key >> 32), b = (uint32_t)key;
     uint32_t start = q == kNoSlot ? tick : L.hs[q];
     bool was_unstored = false;
     if (start == tick && (tagged & kTagPrev) && (tagged & kTagTested) && !s.store_all) {
@@ -3094,17 +3109,14 @@ __device__ __forceinline__ void table_write(const TableLds<KV, NT> &L, const Dev
     }
     tcur.starts[c] = start;
     if constexpr (KV) {
-        const float2 pa = s.pos32[a], pb = s.pos32[b];
+        const float4 pa = pp.a, pb = pp.b;
         float4 m = make_float4(fminf(pa.x, pb.x), fminf(pa.y, pb.y), fmaxf(pa.x, pb.x), fmaxf(pa.y, pb.y));
         if (q != kNoSlot && L.ho[q] != kNoSlot) {
             const float4 o = s.tp->mbr_old[L.ho[q]];
             m = make_float4(fminf(m.x, o.x), fminf(m.y, o.y), fmaxf(m.z, o.z), fmaxf(m.w, o.w));
         } else if (was_unstored) {   // the run's box so far: both points at the tick before
-            const double *xp = s.tp->xy_prev;
-            const float ax = (float)xp[2 * (size_t)a], ay = (float)xp[2 * (size_t)a + 1];
-            const float bx = (float)xp[2 * (size_t)b], by = (float)xp[2 * (size_t)b + 1];
-            m = make_float4(fminf(m.x, fminf(ax, bx)), fminf(m.y, fminf(ay, by)), fmaxf(m.z, fmaxf(ax, bx)),
-                            fmaxf(m.w, fmaxf(ay, by)));
+            m = make_float4(fminf(m.x, fminf(pa.z, pb.z)), fminf(m.y, fminf(pa.w, pb.w)),
+                            fmaxf(m.z, fmaxf(pa.z, pb.z)), fmaxf(m.w, fmaxf(pa.w, pb.w)));
         }
         s.tp->mbr_cur[c] = m;
     }
@@ -3181,6 +3193,11 @@ __device__ __forceinline__ void table_body(const DevState s, const P blk)
         const uint32_t c0 = c0n, c1 = c1n;
 #pragma unroll
         for (int q = 0; q < kPer; q++) nk[q] = nkn[q];
+        // the region's pair positions (key-value output), in flight while
+        // the region is hashed and the previous generation is read
+        PairPos pp[kPer];
+#pragma unroll
+        for (int q = 0; q < kPer; q++) pp[q] = c0 + threadIdx.x + q * NT < c1 ? pair_pos<KV>(s, nk[q]) : PairPos{};
         lds_barrier();
         // the tick's contacts that can continue a run or be a second copy
 #pragma unroll
@@ -3226,7 +3243,7 @@ __device__ __forceinline__ void table_body(const DevState s, const P blk)
 #pragma unroll
         for (int q = 0; q < kPer; q++) {
             const uint32_t c = c0 + threadIdx.x + q * NT;
-            if (c < c1 && ns[q] != kNoSlot - 1) table_write(L, s, tcur, nk[q], c, ns[q], tick, last);
+            if (c < c1 && ns[q] != kNoSlot - 1) table_write(L, s, tcur, nk[q], c, ns[q], tick, last, pp[q]);
         }
         for (uint32_t c = c0 + kRegion + threadIdx.x; c < c1; c += NT) {
             const unsigned long long k = tcur.keys[c];
@@ -3235,7 +3252,7 @@ __device__ __forceinline__ void table_body(const DevState s, const P blk)
                 q = table_find(L, k & kKeyBits);
                 if (q == kNoSlot || L.hc[q] != c) continue;   // a second copy (already marked)
             }
-            table_write(L, s, tcur, k, c, q, tick, last);
+            table_write(L, s, tcur, k, c, q, tick, last, pair_pos<KV>(s, k));
         }
         lds_barrier();
     }
@@ -3729,13 +3746,16 @@ __global__ __launch_bounds__(kBlock) void k_apply_updates(DevState s, double *xy
     }
 }
 
-// Float copy of this tick's positions (pid order) for the pair boxes.
+// Float copies of this tick's positions and of the tick before's (pid
+// order) for the pair boxes.
 template <class P>
 __device__ __forceinline__ void pos32_body(const DevState s, const P blk)
 {
     const double *xy = s.tp->xy;
+    const double *xp = s.tp->has_prev ? s.tp->xy_prev : xy;
     for (uint32_t i = blk.id() * kBlock + threadIdx.x; i < s.n; i += blk.count() * kBlock)
-        s.pos32[i] = make_float2((float)xy[2 * (size_t)i], (float)xy[2 * (size_t)i + 1]);
+        s.pos32[i] = make_float4((float)xy[2 * (size_t)i], (float)xy[2 * (size_t)i + 1], (float)xp[2 * (size_t)i],
+                                 (float)xp[2 * (size_t)i + 1]);
 }
 
 __global__ __launch_bounds__(kBlock) void k_pos32(DevState s) { pos32_body(s, this_blk()); }

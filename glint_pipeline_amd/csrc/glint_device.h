@@ -261,7 +261,8 @@ struct DevState {
                                    // region bits follow at rs_total + 1)
     uint32_t bucket_blocks;        // blocks of the first pass
     KvLogEntry *kvlog;             // key-value output log (null when off)
-    float2 *pos32;                 // float positions of the tick, pid order (key-value output)
+    float4 *pos32;                 // float positions of the tick and of the tick before (x, y, px, py),
+                                   // pid order (key-value output)
     glint_meeting *mlog;           // meeting log ring (null when off)
     unsigned long long mlog_cap;
     unsigned long long kvlog_cap;
