@@ -1138,13 +1138,64 @@ __global__ __launch_bounds__(kBlock) void kf_grid_level(const Many m, uint32_t d
 
 // ------------------------------------------------------------ partition
 
+// The region of every node of the tree: the points the descent (child =
+// (y > my)*2 + (x > mx) at every branch) sends into it, the intersection of
+// its ancestors' half-planes, as (x.lo, x.hi] x (y.lo, y.hi] (infinite where
+// no ancestor bounds it).
+template <class P>
+__device__ __forceinline__ void regions_body(const DevState s, const P blk)
+{
+    for (uint32_t n = blk.id() * blockDim.x + threadIdx.x; n < s.node_cap; n += blk.count() * blockDim.x) {
+        const int32_t t = s.meta[n].type;
+        if (t != kLeaf && t != kBranch) continue;
+        double xlo = -INFINITY, xhi = INFINITY, ylo = -INFINITY, yhi = INFINITY;
+        for (uint32_t c = n; c != 0;) {
+            const uint32_t a = s.parent[c];
+            const double2 mid = node_mid(s.nodes, a);
+            const uint4 ch = node_children(s.nodes, a);
+            const uint32_t i = ch.x == c ? 0u : ch.y == c ? 1u : ch.z == c ? 2u : 3u;
+            if (i & 1u)
+                xlo = fmax(xlo, mid.x);
+            else
+                xhi = fmin(xhi, mid.x);
+            if (i & 2u)
+                ylo = fmax(ylo, mid.y);
+            else
+                yhi = fmin(yhi, mid.y);
+            c = a;
+        }
+        s.region[n] = make_double4(xlo, xhi, ylo, yhi);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_regions(DevState s) { regions_body(s, this_blk()); }
+__global__ __launch_bounds__(kBlock) void kf_regions(const Many m)
+{
+    const Member w = member_of(m);
+    regions_body(m.ss[w.i], w.blk);
+}
+
+__device__ __forceinline__ bool in_region(const double4 &r, double x, double y)
+{
+    return x > r.x && x <= r.y && y > r.z && y <= r.w;
+}
+
 // Descent with last_valid (filter.cpp:229-243): last_valid is the deepest
 // node on the path, below the root, with x + x_buffer < high.x &&
 // y + y_buffer < high.y (both r in planar metres).
 // Objects are taken in the leaf order of the tick before (spid), so the
-// lanes of a wave mostly descend the same path (one node line per wave and
-// level) and share a leaf, whose count one atomic per leaf and wave
-// advances.  The order used is kept for the scatter.
+// lanes of a wave mostly share a leaf, whose count one atomic per leaf and
+// wave advances.  The order used is kept for the scatter.
+// An object starts from its leaf of the tick before (incremental mode): the
+// schema update since then left that node a leaf, split it (a branch) or
+// merged it (freed; its parent is now a leaf).  The object climbs from there
+// to the first node whose region holds it and descends from that node; the
+// leaf is the one the descent from the root would reach, since every node's
+// region is exactly the set of points the descent sends into it.
+// last_valid is then the first node below the root, climbing from the leaf,
+// that meets the condition (the deepest one on the path).  Without a leaf of
+// the tick before (the first tick, the grid rebuild) the descent starts at
+// the root.
 template <class P>
 __device__ __forceinline__ void assign_body(const DevState s, const P blk)
 {
@@ -1199,6 +1250,20 @@ __device__ __forceinline__ void assign_body(const DevState s, const P blk)
                 }
             }
         }
+        // incremental mode: start from the leaf of the tick before (see above)
+        const bool incr = ordered && s.incremental;
+        if (incr) {
+#pragma unroll
+            for (uint32_t q = 0; q < kU; q++) {
+                if (!active[q] || pleaf[q] >= s.node_cap) continue;
+                uint32_t p = pleaf[q];
+                if (s.meta[p].type == kInvalid) p = s.parent[p];   // merged: the parent is the leaf now
+                while (p != 0 && !in_region(s.region[p], x[q], y[q])) p = s.parent[p];
+                cur[q] = p;
+                mid[q] = node_mid(s.nodes, p);
+                ch[q] = node_children(s.nodes, p);
+            }
+        }
         // both descents, one level of each per step
         for (;;) {
             bool more = false;
@@ -1209,10 +1274,23 @@ __device__ __forceinline__ void assign_body(const DevState s, const P blk)
                 mid[q] = node_mid(s.nodes, cur[q]);
                 const double2 hi = node_high(s.nodes, cur[q]);
                 ch[q] = node_children(s.nodes, cur[q]);
-                if (x[q] + xbuf < hi.x && y[q] + ybuf < hi.y) lv[q] = cur[q];
+                if (!incr && x[q] + xbuf < hi.x && y[q] + ybuf < hi.y) lv[q] = cur[q];
                 more = true;
             }
             if (!more) break;
+        }
+        if (incr) {
+#pragma unroll
+            for (uint32_t q = 0; q < kU; q++) {
+                if (!active[q]) continue;
+                for (uint32_t n = cur[q]; n != 0; n = s.parent[n]) {
+                    const double2 hi = node_high(s.nodes, n);
+                    if (x[q] + xbuf < hi.x && y[q] + ybuf < hi.y) {
+                        lv[q] = n;
+                        break;
+                    }
+                }
+            }
         }
 #pragma unroll
         for (uint32_t q = 0; q < kU; q++) {
@@ -3307,14 +3385,14 @@ enum KernelId {
     K_FILL_COUNTS, K_RESET, K_ASSIGN, K_SCAN_REDUCE, K_SCAN_TOP, K_SCAN_OBJECTS, K_SCATTER, K_FILL_UCOUNT,
     K_LOOKUP, K_SCAN_UNITS, K_UNIT_SCATTER, K_JOIN, K_BUCKET_COUNT, K_SCAN_WRITE, K_BUCKET_SCATTER, K_BUCKET_FINE,
     K_TABLE, K_TABLE_RESHAPED, K_COLLECT, K_SPLIT, K_MERGE, K_SCHEMA_DONE, K_TICK_PARAMS, K_POS32, K_LOG_APPEND, K_GRID_CELLS,
-    K_GRID_PYRAMID, K_GRID_TOP, K_GRID_FINE_ZERO, K_GRID_FINE, K_GRID_LEVEL, K_COUNT
+    K_GRID_PYRAMID, K_GRID_TOP, K_GRID_FINE_ZERO, K_GRID_FINE, K_GRID_LEVEL, K_REGIONS, K_COUNT
 };
 const char *const kKernelNames[K_COUNT] = {
     "k_fill_u32", "k_reset_tick", "k_assign", "k_scan_reduce", "k_scan_top", "k_scan_objects", "k_scatter",
     "k_fill_ucount", "k_lookup", "k_scan_units", "k_unit_scatter", "k_join", "k_bucket_count", "k_scan_write",
     "k_bucket_scatter", "k_bucket_fine", "k_table", "k_table_reshaped", "k_collect", "k_split", "k_merge", "k_schema_done",
     "k_advance_tick", "k_pos32", "k_log_append", "k_grid_cells", "k_grid_pyramid", "k_grid_top",
-    "k_grid_fine_zero", "k_grid_fine", "k_grid_level"};
+    "k_grid_fine_zero", "k_grid_fine", "k_grid_level", "k_regions"};
 
 // Per-kernel HIP-event timing on the context's stream (optional).
 struct KernelProfiler {
@@ -3677,6 +3755,7 @@ void enqueue_partition(const Seq &q)
     step_fill(q, K_FILL_COUNTS, kFillCount);
     GLINT_STEP(q, K_RESET, 1, 1u, reset_tick);
     if (q.rep().cfg.dynamic_schema == GLINT_SCHEMA_GRID_REBUILD) enqueue_grid_rebuild(q);
+    if (q.rep().s.incremental) GLINT_STEP(q, K_REGIONS, kBlock, grid_for(c.s.node_cap), regions);
     GLINT_STEP(q, K_ASSIGN, kBlock, grid_for(c.s.n), assign);
     step_scan_totals<kScanCount>(q);
     GLINT_STEP(q, K_SCAN_OBJECTS, kBlock, c.scan_blocks, scan_objects);
@@ -4046,6 +4125,7 @@ glint_status glint_create(const glint_config *cfg, glint_ctx **out)
     // so a pair's state at the tick before cannot be told from positions
     // alone: every contact is stored and looked up
     s.geography = cfg->geography;
+    s.incremental = cfg->dynamic_schema != GLINT_SCHEMA_GRID_REBUILD;
     s.xbuf = cfg->geography ? cfg->reach_distance * deg_per_meter_lon_host(cfg->low_y) : cfg->reach_distance;
     s.ybuf = cfg->geography ? cfg->reach_distance * kDegPerMeterLat : cfg->reach_distance;
     s.delay = c->cfg.schema_update_delay;
@@ -4149,6 +4229,7 @@ glint_status glint_create(const glint_config *cfg, glint_ctx **out)
     ALLOC(s.sleafb[0], N);
     ALLOC(s.sleafb[1], N);
     ALLOC(s.parent, node_cap);
+    ALLOC(s.region, node_cap);
     if (!s.store_all) {
         ALLOC(s.cand, N);
         ALLOC(s.prev_cand, N);

@@ -217,6 +217,9 @@ struct DevState {
     uint32_t *merge_list;
     uint32_t *leaf_list;
     uint32_t *parent;       // per node: its parent (kNone for the root), for stackless tree walks
+    double4 *region;        // per node: the points the descent sends into it, (x.lo, x.hi] x (y.lo, y.hi]
+    uint32_t incremental;   // objects are placed from their leaf of the tick before (not with the grid rebuild)
+    uint32_t pad_inc_;
     uint32_t *sleafb[2];    // leaf of every sorted object, two ticks (TickParams::sleaf, sleaf_old)
     uint4 *uaux;            // per unit (lookup order): the object's leaf (bit 31: its candidates
                             // overflowed), its leaf at the tick before, its sorted position at
