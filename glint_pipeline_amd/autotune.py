@@ -28,7 +28,7 @@ from . import gen, runtime
 
 TUNED_PATH = Path(__file__).resolve().parent / "tuned.json"
 
-DEFAULTS = {"join_tile": 512, "table_threads": 512, "load_factor": 0.75, "precision": "fp32guard"}
+DEFAULTS = {"join_tile": 256, "table_threads": 512, "load_factor": 0.75, "precision": "fp32guard"}
 SPACE = {"join_tile": [256, 512], "table_threads": [256, 512, 1024], "load_factor": [0.375, 0.75, 1.5, 3.0, 6.0],
          "precision": ["fp64", "fp32guard"]}
 LEAF_CAPACITIES = [64, 128, 192, 256, 320, 384, 512, 1024]
@@ -73,7 +73,7 @@ def config_for(w: gen.Workload, params: dict, distinct: float, graph_ticks: int 
         num_meeting_buckets=table_slots(distinct, params.get("load_factor", 0.75)),
         unit_capacity=8 * n + 4096, meeting_capacity=max(1 << 20, 6 * n),
         contact_capacity=int(1.5 * distinct) + (1 << 20), graph_ticks=graph_ticks,
-        join_tile=params.get("join_tile", 512), table_threads=params.get("table_threads", 0), **extra)
+        join_tile=params.get("join_tile", 256), table_threads=params.get("table_threads", 0), **extra)
 
 
 @dataclass

@@ -38,7 +38,7 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kScanItems = 4;                       // per thread
 constexpr int kScanTile = kBlock * kScanItems;      // elements per scan block
-constexpr int kTile = 512;                          // points per LDS tile of the join (default)
+constexpr int kTile = 256;                          // points per LDS tile of the join (default)
 constexpr int kStage = 432;                         // staged contacts per wave
 constexpr int kFlushRound = 4;                      // join columns between stage checks
 constexpr int kMeetStage = 128;                     // staged meetings per wave (table update)
@@ -3786,10 +3786,10 @@ void enqueue_refine(const Seq &q)
         step(
             q, K_JOIN, jb, [&](hipStream_t st, uint32_t g, glint_ctx &c) { k_join<256, true><<<g, kBlock, 0, st>>>(c.s); },
             [&](hipStream_t st, uint32_t g, const Many &m) { kf_join<256, true><<<g, kBlock, 0, st>>>(m); });
-    else if (r.cfg.join_tile == 256)
+    else if (r.cfg.join_tile == 512)
         step(
-            q, K_JOIN, jb, [&](hipStream_t st, uint32_t g, glint_ctx &c) { k_join<256><<<g, kBlock, 0, st>>>(c.s); },
-            [&](hipStream_t st, uint32_t g, const Many &m) { kf_join<256><<<g, kBlock, 0, st>>>(m); });
+            q, K_JOIN, jb, [&](hipStream_t st, uint32_t g, glint_ctx &c) { k_join<512><<<g, kBlock, 0, st>>>(c.s); },
+            [&](hipStream_t st, uint32_t g, const Many &m) { kf_join<512><<<g, kBlock, 0, st>>>(m); });
     else
         step(
             q, K_JOIN, jb, [&](hipStream_t st, uint32_t g, glint_ctx &c) { k_join<kTile><<<g, kBlock, 0, st>>>(c.s); },
@@ -4582,7 +4582,7 @@ static void enqueue_tick_body(const FusedGroup &g)
 // share fused launches.
 static std::vector<uint64_t> variant_key(const glint_ctx *c)
 {
-    const uint64_t tile = c->s.geography ? 256 : (c->cfg.join_tile == 256 ? 256 : kTile);
+    const uint64_t tile = c->s.geography ? 256 : (c->cfg.join_tile == 512 ? 512 : kTile);
     const uint64_t nt = c->cfg.table_threads ? c->cfg.table_threads : 512;
     return {tile, nt, c->kv ? 1ull : 0ull, c->s.geography, c->cfg.dynamic_schema};
 }

@@ -101,7 +101,7 @@ typedef struct glint_config {
     uint64_t kv_capacity;          /* directed keys held between flushes (2 per meeting); 0: output off */
     double kv_area_limit;          /* dProbFalse (process.cpp:408-411): object box area, squared input units */
     uint64_t meeting_log_capacity; /* meetings kept in the device meeting log ring; 0: log off */
-    uint32_t join_tile;            /* points per LDS tile of the distance join: 256 or 512 (0: 512) */
+    uint32_t join_tile;            /* points per LDS tile of the distance join: 256 or 512 (0: 256) */
     uint32_t table_threads;        /* workgroup size of the table update: 256, 512 or 1024 (0: 512) */
     uint32_t geography;            /* 1: coordinates are lon/lat degrees and the predicates are the
                                       reference's geography=true forms (geometry.h:40-48,159-185,

@@ -126,7 +126,7 @@ def variant_cfg(w, v):
               meeting_log_capacity=1 << 20)
     box = w.box()
     if v == 1:
-        kw["join_tile"] = 256
+        kw["join_tile"] = 512
     elif v == 2:
         kw["dynamic_schema"] = runtime.SCHEMA_GRID_REBUILD
         box = (-1.0, -1.0, w.side + 1.0, w.side + 1.0)   # no positions on the grid's high y edge
