@@ -365,6 +365,11 @@ struct glint_server {
     std::map<uint64_t, Query> queries;
     std::map<uint32_t, uint64_t> log_after;   // tick -> meetings logged after it
     MeetingIndex index;
+    // the index is sealed on a stream of its own after each tick, so that a
+    // sort does not hold up the next tick; readers wait for `sealed`
+    hipStream_t index_stream = nullptr;
+    hipEvent_t tick_done = nullptr, sealed = nullptr;
+    bool any_sealed = false;
     uint64_t next_id = 1;
     bool have_tick = false;
     double meetings_per_tick = 0;             // running average, for hit estimates
@@ -502,8 +507,9 @@ glint_status start_query(glint_server *s, Query &q, int slot)
         k_log_hop1<<<grid_of(total - q.tail, 8192), kB, 0, st>>>(v.mlog, v.mlog_cap, q.tail, q.hi_sub, total, q.T,
                                                                  v.min_meet, q.t_start, q.t_end, q.bm1, q.h1,
                                                                  &q.ctr->hits1, q.cap1);
-    // the blocks were sealed on the context's stream before the table hits
+    // the table hits, and the sorts of the blocks listed
     SV_TRY(s, hipStreamWaitEvent(st, sl.snap, 0));
+    if (s->any_sealed) SV_TRY(s, hipStreamWaitEvent(st, s->sealed, 0));
     q.nblocks = (uint32_t)q.blocks_h.size();
     if (q.nblocks) {
         SV_TRY(s, hipMemcpyAsync(q.blocks_d, q.blocks_h.data(), sizeof(ulonglong2) * q.nblocks,
@@ -612,10 +618,9 @@ glint_status advance(glint_server *s)
                         q.h2, &q.ctr->hits2, q.cap2);
                 q.nblocks = (uint32_t)q.blocks_h.size();
                 if (q.nblocks) {
-                    // blocks sealed since admission: their sort is ordered before
-                    // this point of the context's stream
-                    SV_TRY(s, hipEventRecord(sl.snap, v.stream));
-                    SV_TRY(s, hipStreamWaitEvent(st, sl.snap, 0));
+                    // blocks sealed since admission: their sorts are ordered
+                    // before this point of the index stream
+                    if (s->any_sealed) SV_TRY(s, hipStreamWaitEvent(st, s->sealed, 0));
                     SV_TRY(s, hipMemcpyAsync(q.blocks_d, q.blocks_h.data(), sizeof(ulonglong2) * q.nblocks,
                                              hipMemcpyHostToDevice, st));
                     const uint64_t work = (uint64_t)q.nids * q.nblocks;
@@ -647,7 +652,8 @@ glint_status advance(glint_server *s)
     return GLINT_OK;
 }
 
-// Seals log entries [l0, l1) into one index block, on the context's stream.
+// Seals log entries [l0, l1) into one index block, on the index stream
+// (after the tick that logged them, which index_tick orders).
 glint_status seal_block(glint_server *s, uint64_t l0, uint64_t l1, uint32_t tick)
 {
     MeetingIndex &ix = s->index;
@@ -662,7 +668,7 @@ glint_status seal_block(glint_server *s, uint64_t l0, uint64_t l1, uint32_t tick
             ++it;
         }
     }
-    hipStream_t st = s->view.stream;
+    hipStream_t st = s->index_stream;
     k_index_expand<<<grid_of(l1 - l0, 4096), kB, 0, st>>>(s->view.mlog, s->view.mlog_cap, l0, l1, ix.skeys,
                                                            ix.svals);
     size_t bytes = ix.tmp_bytes;
@@ -670,6 +676,8 @@ glint_status seal_block(glint_server *s, uint64_t l0, uint64_t l1, uint32_t tick
     SV_TRY(s, rocprim::radix_sort_pairs(ix.tmp, bytes, ix.skeys, ix.keys + a, ix.svals, ix.vals + a, (size_t)n, 0,
                                         62, st));
     SV_TRY(s, hipGetLastError());
+    SV_TRY(s, hipEventRecord(s->sealed, st));
+    s->any_sealed = true;
     ix.blocks.push_back(IndexBlock{a, n, l0, l1, ix.first_tick, tick});
     ix.entries += n;
     ix.next = b;
@@ -686,6 +694,11 @@ glint_status index_tick(glint_server *s, uint64_t total, uint64_t first, uint32_
     if (!ix.keys) return GLINT_OK;
     if (ix.sealed < first) ix.sealed = first;   // entries the ring lost before they were sealed
     const uint64_t most = ix.block_max / 2;     // meetings per block
+    if (total - ix.sealed >= most || (total > ix.sealed && tick + 1 - ix.first_tick >= ix.block_ticks)) {
+        // the sorts read what this tick logged
+        SV_TRY(s, hipEventRecord(s->tick_done, s->view.stream));
+        SV_TRY(s, hipStreamWaitEvent(s->index_stream, s->tick_done, 0));
+    }
     while (total - ix.sealed >= most) {
         glint_status st = seal_block(s, ix.sealed, ix.sealed + most, tick);
         if (st) return st;
@@ -717,6 +730,10 @@ glint_status glint_server_destroy(glint_server *s)
         if (sl.stream) (void)hipStreamDestroy(sl.stream);
     }
     if (s->view.stream) (void)hipStreamSynchronize(s->view.stream);
+    if (s->index_stream) (void)hipStreamSynchronize(s->index_stream);
+    if (s->tick_done) (void)hipEventDestroy(s->tick_done);
+    if (s->sealed) (void)hipEventDestroy(s->sealed);
+    if (s->index_stream) (void)hipStreamDestroy(s->index_stream);
     if (s->arena_d) (void)hipFree(s->arena_d);
     for (void *p : {(void *)s->index.keys, (void *)s->index.vals, (void *)s->index.skeys, (void *)s->index.svals,
                     s->index.tmp})
@@ -745,6 +762,12 @@ glint_status glint_server_create(glint_ctx *ctx, const glint_server_config *cfg,
         return GLINT_ERR_OUT_OF_MEMORY;
     }
     s->arena.init(cfg->arena_bytes);
+    if (hipStreamCreateWithFlags(&s->index_stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&s->tick_done, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&s->sealed, hipEventDisableTiming) != hipSuccess) {
+        glint_server_destroy(s);
+        return GLINT_ERR_HIP;
+    }
     MeetingIndex &ix = s->index;
     ix.block_ticks = cfg->index_block_ticks ? cfg->index_block_ticks : 64;
     if (ix.block_ticks != 0xFFFFFFFFu) {
@@ -759,7 +782,7 @@ glint_status glint_server_create(glint_ctx *ctx, const glint_server_config *cfg,
                   hipMalloc((void **)&ix.svals, 8 * ix.block_max) == hipSuccess;
         if (ok) {
             ok = rocprim::radix_sort_pairs(nullptr, ix.tmp_bytes, ix.skeys, ix.keys, ix.svals, ix.vals,
-                                           (size_t)ix.block_max, 0, 62, s->view.stream) == hipSuccess &&
+                                           (size_t)ix.block_max, 0, 62, s->index_stream) == hipSuccess &&
                  hipMalloc(&ix.tmp, std::max<size_t>(ix.tmp_bytes, 256)) == hipSuccess;
         }
         if (!ok) {
@@ -808,7 +831,8 @@ glint_status glint_server_tick(glint_server *s, const double *d_full_xy, const u
     while (s->log_after.size() > 1 && s->log_after.begin()->second < first) s->log_after.erase(s->log_after.begin());
     glint_ctx_view(c, &s->view);
     if ((st = index_tick(s, total, first, tick))) return st;
-    if ((st = advance(s))) return st;
+    // pending queries start on this tick's state; running ones advance (and
+    // copy their hits to the host) in glint_server_poll, between ticks
     return admit(s);
 }
 

@@ -386,8 +386,10 @@ glint_status glint_server_create(glint_ctx *ctx, const glint_server_config *cfg,
 glint_status glint_server_destroy(glint_server *server);
 const char *glint_server_last_error(const glint_server *server);
 /* One tick: every position (d_full_xy, device) or k location updates, then
- * partition, refine, evaluate and schema update; then running queries
- * advance and pending ones are admitted. */
+ * partition, refine, evaluate and schema update; then pending queries are
+ * admitted on the new state.  Running queries advance (hop 2 starts, hits
+ * are copied to the host) in glint_server_poll, so that a tick never waits
+ * for a query's results. */
 glint_status glint_server_tick(glint_server *server, const double *d_full_xy, const uint32_t *d_pids,
                                const double *d_xy, uint32_t k, uint32_t tick);
 /* Queue a query: contacts of the seeds (host array) whose meetings overlap
