@@ -2413,6 +2413,7 @@ struct JoinLds {
     JoinLeaf leaf;                       // the leaf being joined
     uint32_t hist[kSubCells];            // sub-cell counts of a tile being loaded
     uint32_t item;
+    uint32_t next;                       // the next task of the tile pair being joined
 };
 
 
@@ -2455,6 +2456,7 @@ __device__ __forceinline__ void join_body(const DevState s, const P blk)
     const uint32_t wave = threadIdx.x / 64, lane = lane_id();
     Stager st{contact_out(s), lds.stage[wave], 0, 0, s.chunk, 0};
     JoinRows &rows = lds.rows[wave];
+    if (threadIdx.x == 0) lds.next = 0;   // (the first leaf's barriers come before any use)
     uint32_t n_in = 0, n_bd = 0;                   // sure contacts (per lane)
     uint32_t pass_in = 0, pass_bd = 0, band = 0;   // band entries within reach / re-tested (per lane)
     uint32_t dups = 0;                             // unstored pairs found twice (per lane)
@@ -2648,12 +2650,30 @@ __device__ __forceinline__ void join_body(const DevState s, const P blk)
                                      nullptr, nullptr, lds.hist);
                 const uint32_t nri = (ni + 63) / 64, nrj = (nj + 63) / 64;
                 const uint32_t ntask_in = diag ? nri * (nri + 1) / 2 : nri * nrj;
-                const uint32_t nuc = diag ? (F.ucnt + 63) / 64 : 0;
-                for (uint32_t t = wave; t < ntask_in + nuc; t += kBlock / 64) {
-                    if (t < ntask_in) {
+                // border tasks: 64 units against 64 columns of the tile, taken
+                // first (every one a full 64 x 64 block), then the inside tasks;
+                // each wave takes the next task when it is done with one, so
+                // the tile pair ends when the work does, not when the wave with
+                // the most column steps does
+                const uint32_t nut = diag ? (F.ucnt + 63) / 64 * nri : 0;
+                for (;;) {
+                    uint32_t t = 0;
+                    if (lane == 0) t = atomicAdd(&lds.next, 1u);
+                    t = __builtin_amdgcn_readfirstlane(t);
+                    if (t >= nut + ntask_in) break;
+                    if (t >= nut) {
+                        t -= nut;
                         uint32_t r, q;
                         if (diag) {
-                            triangle_task(t, nri, r, q);
+                            // the blocks against each other first, then each
+                            // block against itself (half the pairs) at the end
+                            const uint32_t noff = nri * (nri - 1) / 2;
+                            if (t < noff) {
+                                triangle_task(t, nri - 1, r, q);
+                                q++;
+                            } else {
+                                r = q = t - noff;
+                            }
                         } else {
                             r = t / nrj;
                             q = t % nrj;
@@ -2663,14 +2683,15 @@ __device__ __forceinline__ void join_body(const DevState s, const P blk)
                         run(lds.a, B, true, row, 0, make_uint4(0u, 0u, 0u, 0u), row < ni, jbeg, min(jbeg + 64, nj),
                             diag && r == q, F.use32, F.lx, F.ly, F.node);
                     } else {
-                        const uint32_t u = (t - ntask_in) * 64 + lane;
+                        const uint32_t u = t / nri * 64 + lane, jbeg = t % nri * 64;
                         const bool vi = u < F.ucnt;
                         run(lds.a, lds.a, false, 0, vi ? s.unit_pid[F.uoff + u] : 0u,
-                            vi ? s.unit_lf[F.uoff + u] : make_uint4(0u, 0u, 0u, 0u), vi, 0, ni, false, F.use32,
-                            F.lx, F.ly, F.node);
+                            vi ? s.unit_lf[F.uoff + u] : make_uint4(0u, 0u, 0u, 0u), vi, jbeg, min(jbeg + 64, ni),
+                            false, F.use32, F.lx, F.ly, F.node);
                     }
                 }
                 __syncthreads();
+                if (threadIdx.x == 0) lds.next = 0;   // read again only after the next tile's barriers
             }
         }
     }
