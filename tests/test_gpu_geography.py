@@ -137,3 +137,36 @@ def test_unsupported_configurations_refused():
         with pytest.raises(runtime.GlintError) as e:
             runtime.Workbench(runtime.make_config(100, b, geography=True, **kw))
         assert e.value.status == runtime.GLINT_ERR_INVALID_ARGUMENT
+
+
+def test_lonlat_cities_in_one_batch():
+    """Three lon/lat cities (north, south, far north) in one batch share the
+    geography kernels' fused launches; each equals itself run alone."""
+    T = 16
+    places = [(-87.94, 41.64), (151.10, -33.95), (18.90, 69.60)]
+    ws = [gen.uniform_city(n, T, 400.0, seed=180 + i, grid_capacity=24) for i, n in enumerate((3000, 5000, 8000))]
+    cfgs, bufs, boxes = [], [], []
+    for w, (lon0, lat0) in zip(ws, places):
+        ll, box = lonlat_walk(w, T, lon0, lat0)
+        b = runtime.DeviceBuffer(ll.nbytes)
+        b.upload(ll)
+        bufs.append(b)
+        cfgs.append(runtime.make_config(w.num_objects, box, grid_capacity=w.grid_capacity, geography=True,
+                                        precision=runtime.PRECISION_FP64, meeting_log_capacity=1 << 18))
+    with runtime.Batch(cfgs, graph_ticks=4) as bt:
+        assert bt.groups() == 1
+        bt.run_ticks([b.address(0) for b in bufs], 0, T)
+        got = [(c.meeting_log(), c.active_pairs(), c.leaves()) for c in bt.contexts]
+    total = 0
+    for cfg, b, (log, act, lvs) in zip(cfgs, bufs, got):
+        with runtime.Workbench(cfg) as wb:
+            wb.run_ticks(b.address(0), 0, T)
+            assert np.array_equal(log, wb.meeting_log())
+            for x, y in zip(act, wb.active_pairs()):
+                assert np.array_equal(x, y)
+            for x, y in zip(lvs, wb.leaves()):
+                assert np.array_equal(x, y)
+        total += len(log)
+    assert total > 1000
+    for b in bufs:
+        b.free()
