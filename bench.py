@@ -137,6 +137,30 @@ def profiled_valu(kernel: str, workload: str, precision: int, objects: int) -> t
     return found, f"profiles/{tag} (SQ_INSTS_VALU; sources {cur})"
 
 
+# the tick's kernels by the reference's stages (filter = partition and
+# lookup, reachability = join and the grouping of contacts by table region,
+# update_meetings = the table update, update_schema); the scans serve all three
+STAGE_OF_KERNEL = {
+    "k_fill_u32": "filter", "k_reset_tick": "filter", "k_regions": "filter", "k_assign": "filter",
+    "k_scan_objects": "filter", "k_scatter": "filter", "k_fill_ucount": "filter", "k_lookup": "filter",
+    "k_scan_units": "filter", "k_unit_scatter": "filter", "k_grid_cells": "filter", "k_grid_pyramid": "filter",
+    "k_grid_top": "filter", "k_grid_fine_zero": "filter", "k_grid_fine": "filter", "k_grid_level": "filter",
+    "k_join": "reachability", "k_bucket_count": "reachability", "k_scan_write": "reachability",
+    "k_bucket_scatter": "reachability", "k_bucket_fine": "reachability",
+    "k_table": "update_meetings", "k_table_reshaped": "update_meetings", "k_pos32": "update_meetings",
+    "k_log_append": "update_meetings",
+    "k_collect": "update_schema", "k_split": "update_schema", "k_merge": "update_schema",
+    "k_schema_done": "update_schema",
+}
+
+
+def gpu_stages(ktimes: dict, ticks: int) -> dict:
+    out = {"filter": 0.0, "reachability": 0.0, "update_meetings": 0.0, "update_schema": 0.0, "other": 0.0}
+    for name, (ms, _cnt) in ktimes.items():
+        out[STAGE_OF_KERNEL.get(name, "other")] += ms / ticks
+    return {k: round(v, 3) for k, v in out.items()}
+
+
 def cpu_baseline(w, ticks: int, threads: int, min_seconds: float = 10.0, max_ticks: int = 5000,
                  schema: str = "dynamic") -> dict:
     """The CPU oracle (C + OpenMP) on the first ticks of the same workload:
@@ -152,15 +176,21 @@ def cpu_baseline(w, ticks: int, threads: int, min_seconds: float = 10.0, max_tic
     t_build = time.perf_counter() - t_build0
     tt = 0.0
     t = 0
+    stages = {"filter": 0.0, "reachability": 0.0, "update_meetings": 0.0, "update_schema": 0.0}
     while t < max_ticks and (t < ticks or tt < min_seconds):
         t += 1
         xy = g.step()
         t0 = time.perf_counter()
         o.tick(xy, t)
         tt += time.perf_counter() - t0
+        st = o.stats()   # the oracle's own timers of its stages (the reference's profiler, workbench.h:19-53)
+        for k, f in (("filter", "t_filter"), ("reachability", "t_refine"), ("update_meetings", "t_evaluate"),
+                     ("update_schema", "t_schema")):
+            stages[k] += st[f]
     st = o.stats()
     o.close()
-    return {"seconds": tt, "ticks": t, "tick0_seconds": t_build, "contacts_last_tick": st["num_contacts"]}
+    return {"seconds": tt, "ticks": t, "tick0_seconds": t_build, "contacts_last_tick": st["num_contacts"],
+            "stages_ms_per_tick": {k: round(v * 1e3 / t, 3) for k, v in stages.items()}}
 
 
 WORKLOAD_SPEC = {
@@ -338,8 +368,10 @@ def main() -> int:
             mismatches += check_fixture(wv, fixture, fx_ticks)
             verify["fixture_ticks"] = fx_ticks
     wv.synchronize()
+    gpu_stage_ms = None
     if not args.no_profile:
         ktimes = wv.kernel_times()
+        gpu_stage_ms = gpu_stages(ktimes, K)
         wv.set_kernel_profiling(False)
         ptick = sum(ms for ms, cnt in ktimes.values()) / K
         for name, (ms, cnt) in sorted(ktimes.items(), key=lambda kv: -kv[1][0]):
@@ -469,7 +501,7 @@ def main() -> int:
         cb = cpu_baseline(w, args.cpu_ticks, threads, schema=params.get("schema", "dynamic"))
         log(f"[bench] cpu baseline {cb} (wall {time.perf_counter() - t0:.1f} s)")
         cpu = {"value": round(N * cb["ticks"] / cb["seconds"], 1), "unit": "object-updates/s", "cores": threads,
-               "kind": "port", "cpu": cpu_model(),
+               "kind": "port", "cpu": cpu_model(), "stages_ms_per_tick": cb["stages_ms_per_tick"],
                "sample": f"oracle (C, OpenMP) ticks 1..{cb['ticks']} of the same workload after its tick-0 tree "
                          f"build; {cb['seconds']:.2f} s timed"}
 
@@ -507,6 +539,7 @@ def main() -> int:
                    "ref_drop_leaves_per_tick": round(per_tick["ref_drop_leaves"], 1),
                    "ref_drop_objects_per_tick": round(per_tick["ref_drop_objects"], 1)},
         "worst_tick": p99_note,
+        "stages_ms_per_tick": gpu_stage_ms,   # kernel time by the reference's stages (ungraphed run, HIP events)
         "verification": verify,
         "roofline": roofline,
         "cpu_baseline": cpu,
