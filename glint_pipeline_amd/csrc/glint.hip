@@ -1878,6 +1878,15 @@ __device__ __forceinline__ void stage_push(Stager &st, bool hit, uint32_t e)
     st.n += (uint32_t)__popcll(m);
 }
 
+// The same for a mask of lanes already formed (bit i: lane i appends e).
+__device__ __forceinline__ void stage_push_mask(Stager &st, unsigned long long m, uint32_t e)
+{
+    if (m == 0) return;
+    if (__builtin_amdgcn_inverse_ballot_w64(m))
+        st.buf[st.n + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u))] = e;
+    st.n += (uint32_t)__popcll(m);
+}
+
 // Writes the staged entries out: key(e, valid) gives the contact-list key of
 // entry e, or kEmptyKey for an entry that is not kept (and for !valid);
 // kept keys are packed.
@@ -2214,6 +2223,7 @@ __device__ __forceinline__ JoinCounts join_columns(Stager &st, const DevState &s
     // the task's columns are the same for every lane: scalar loop bounds
     jbeg = __builtin_amdgcn_readfirstlane(jbeg);
     jend = __builtin_amdgcn_readfirstlane(jend);
+    has_prev = __builtin_amdgcn_readfirstlane((uint32_t)has_prev) != 0u;   // (uniform: a scalar branch)
     const uint32_t tag = lane_id() << 16;
     const unsigned long long flag = BORDER ? kTagBorder : 0ull;
     const bool tested_check = has_prev && !s.store_all;   // wave-uniform
@@ -2304,6 +2314,20 @@ __device__ __forceinline__ JoinCounts join_columns(Stager &st, const DevState &s
     const uint32_t dmax = ROT ? (nblk - 1) / 2 + (nblk % 2 == 0 ? 1u : 0u) : 0u;
     const uint32_t lim = ROT ? (lane < nblk ? (nblk - 1) / 2 + ((nblk % 2 == 0 && lane < nblk / 2) ? 1u : 0u) : 0u)
                              : 0u;
+    // ROT: every lim(i) is dmax or dmax - 1, so the lanes with a column at
+    // rotation step d are those of the block for d < dmax, those with
+    // lim(i) = dmax at d = dmax, and none past it
+    const unsigned long long mrot_all = ROT ? __builtin_amdgcn_ballot_w64(lane < nblk) : 0ull;
+    const unsigned long long mrot_top = ROT ? __builtin_amdgcn_ballot_w64(lane < nblk && lim == dmax) : 0ull;
+    // guard mode: every hit is staged when every contact is kept and for
+    // border rows that are not settled here; settled border rows stage their
+    // sure hits also when their candidates overflowed
+    // (uniform flags as all-or-nothing masks: the loop has no branch on them)
+    const bool stage_all_hits = !settle && (BORDER || s.store_all);
+    const unsigned long long mall = stage_all_hits ? ~0ull : 0ull;
+    const unsigned long long mprev = has_prev ? ~0ull : 0ull;
+    const unsigned long long msettle = settle ? ~0ull : 0ull;
+    const unsigned long long mfree = __builtin_amdgcn_ballot_w64(rovf) & msettle;
     // one round: columns jb .. jb+kFlushRound-1 (below jlim; ROT: rotation steps)
     auto round = [&](uint32_t jb, uint32_t jlim) {
         TileF cfp[kFlushRound];
@@ -2320,37 +2344,45 @@ __device__ __forceinline__ JoinCounts join_columns(Stager &st, const DevState &s
                 jc = jbeg + jr;
                 cj[jj] = jc;
             } else {
-                jc = min(jb + jj, jlim - 1);
+                // jb is in a group of kGroup slots that starts at a multiple
+                // of kGroup below TILE, so jb + jj is a slot of the tile; a
+                // slot past jlim is read and its lanes masked out (mcand)
+                jc = GEO ? min(jb + jj, jlim - 1) : jb + jj;
             }
             if constexpr (USE32)
                 cfp[jj] = C.fp[jc];
             else
                 cx[jj] = C.xy[jc];
-            if (settle) cm[jj] = cmask[jc];
+            if constexpr (BORDER && USE32) cm[jj] = cmask[jc];   // (used when settled)
         }
-        bool stg[kFlushRound];
-        uint32_t ent[kFlushRound];
+        // Every decision is a mask of lanes (a compare's own result), combined
+        // with scalar operations; the vector unit only forms the distances.
+        unsigned long long mst[kFlushRound];
 #pragma unroll
         for (uint32_t jj = 0; jj < kFlushRound; jj++) {
-            const uint32_t j = ROT ? cj[jj] : jb + jj;
-            const bool cand = ROT ? (jb + jj <= lim) : (j < jlim);   // uniform unless ROT
-            bool sure, stage;
-            const uint32_t e = tag | j;
+            // the lanes with a candidate pair at this column
+            unsigned long long mcand;
+            if constexpr (ROT) {
+                const uint32_t d = jb + jj;
+                mcand = d < dmax ? mrot_all : (d == dmax ? mrot_top : 0ull);
+            } else {
+                mcand = jb + jj < jlim ? ~0ull : 0ull;
+            }
+            unsigned long long msure, mstage;
             if constexpr (USE32) {
                 const float dx = fxi - cfp[jj].f.x;
                 const float dy = fyi - cfp[jj].f.y;
                 const float s32 = dist2_f32(dx, dy);
-                sure = cand && s32 <= t32_in;
-                const bool hit = cand && s32 <= t32_out;
+                msure = __builtin_amdgcn_ballot_w64(s32 <= t32_in) & mcand;
+                const unsigned long long mhit = __builtin_amdgcn_ballot_w64(s32 <= t32_out) & mcand;
+                // a sure hit is staged when it may be stored
                 const float px = pfxi - cfp[jj].pf.x;
                 const float py = pfyi - cfp[jj].pf.y;
                 const float p32 = dist2_f32(px, py);
                 // NaN (unbounded offsets) compares false: may be tagged
-                const bool maybe_tagged = has_prev && !(p32 > t32_out);
-                if (settle)
-                    stage = hit && (!sure || maybe_tagged || rovf || (cm[jj] & rbit) != 0u);
-                else
-                    stage = hit && (!sure || BORDER || s.store_all || maybe_tagged);
+                unsigned long long mkeep = mall | mfree | (__builtin_amdgcn_ballot_w64(!(p32 > t32_out)) & mprev);
+                if constexpr (BORDER) mkeep |= __builtin_amdgcn_ballot_w64((cm[jj] & rbit) != 0u) & msettle;
+                mstage = mhit & (~msure | mkeep);
             } else {
                 double ex = xi - cx[jj].x;
                 double ey = yi - cx[jj].y;
@@ -2360,16 +2392,15 @@ __device__ __forceinline__ JoinCounts join_columns(Stager &st, const DevState &s
                     ey = ey / kDegPerMeterLat;
                     ex = ex / f;
                 }
-                sure = cand && ex * ex + ey * ey <= t64;
-                stage = sure && (BORDER || s.store_all || has_prev);
+                msure = __builtin_amdgcn_ballot_w64(ex * ex + ey * ey <= t64) & mcand;
+                mstage = (BORDER || s.store_all || has_prev) ? msure : 0ull;
             }
-            n_sure += sure;
-            stg[jj] = stage;
-            ent[jj] = e;
+            n_sure += (uint32_t)__popcll(msure);
+            mst[jj] = mstage;
         }
         // the round's stage writes together, after its distance tests
 #pragma unroll
-        for (uint32_t jj = 0; jj < kFlushRound; jj++) stage_push(st, stg[jj], ent[jj]);
+        for (uint32_t jj = 0; jj < kFlushRound; jj++) stage_push_mask(st, mst[jj], tag | (ROT ? cj[jj] : jb + jj));
         if (st.n > kStage - 64 * kFlushRound) stage_flush(st, s, key_of);
     };
     if constexpr (ROT) {
@@ -2390,7 +2421,8 @@ __device__ __forceinline__ JoinCounts join_columns(Stager &st, const DevState &s
         }
     }
     stage_flush(st, s, key_of);
-    return JoinCounts{n_sure, n_pass, band, n_dup, cols};
+    // n_sure is the wave's count (the same on every lane): lane 0 reports it
+    return JoinCounts{lane == 0 ? n_sure : 0u, n_pass, band, n_dup, cols};
 }
 
 // A leaf of the join's current chunk of work.
@@ -2453,7 +2485,7 @@ __device__ __forceinline__ void join_body(const DevState s, const P blk)
     const double t64 = s.t64;
     const uint32_t nh = s.ctr->n_heavy;
     const uint32_t nl = (s.ctr->error & kErrUnitCap) ? 0 : s.ctr->n_leaf_list + nh;
-    const uint32_t wave = threadIdx.x / 64, lane = lane_id();
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64), lane = lane_id();
     Stager st{contact_out(s), lds.stage[wave], 0, 0, s.chunk, 0};
     JoinRows &rows = lds.rows[wave];
     if (threadIdx.x == 0) lds.next = 0;   // (the first leaf's barriers come before any use)
