@@ -2039,25 +2039,47 @@ struct JoinRows {
     uint32_t ppl[64];   // the object's leaf at prev_tick
 };
 
-// Tile slot k <- sorted object `from` of a leaf joined in FP32 (use32) or FP64.
-template <int TILE>
-__device__ __forceinline__ void tile_put(JoinTile<TILE> &t, uint32_t k, const DevState &s, uint32_t from,
-                                         bool has_prev, bool use32, bool want_pc)
+// One object of a tile as read from the sorted arrays: FP32 offsets of this
+// and the previous tick (use32) or FP64 coordinates, leaf at prev_tick, id,
+// candidate leaves at prev_tick (or the geography scale), and this tick's
+// candidate list.
+struct TileObj {
+    uint4 pos;   // the TileF (use32) or the double2, as bits
+    uint32_t pl, p;
+    uint4 pc, cc;
+    __device__ float fx() const { return __uint_as_float(pos.x); }
+    __device__ float fy() const { return __uint_as_float(pos.y); }
+    __device__ double x() const { return __hiloint2double((int)pos.y, (int)pos.x); }
+    __device__ double y() const { return __hiloint2double((int)pos.w, (int)pos.z); }
+};
+
+// Sorted object `from` of a leaf joined in FP32 (use32) or FP64; every load
+// is issued here, before any is used.
+__device__ __forceinline__ TileObj tile_fetch(const DevState &s, uint32_t from, bool has_prev, bool use32,
+                                              bool want_pc, bool cand)
 {
-    if (use32)
-        t.fp[k] = TileF{make_float2(s.fx[from], s.fy[from]),
-                        has_prev ? make_float2(s.pfx[from], s.pfy[from]) : make_float2(0.f, 0.f)};
-    else
-        t.xy[k] = make_double2(s.tp->sx[from], s.tp->sy[from]);
-    t.g[k] = from;
-    t.pl[k] = s.splf[from];
-    const uint32_t pid = s.spid[from];
-    t.p[k] = pid;
-    if (want_pc) t.pc[k] = cand_at(s.tp->cand_prev, s.sprev[from], s.n, s.tp->prev_tick);
+    TileObj o;
+    if (use32) {
+        o.pos = make_uint4(__float_as_uint(s.fx[from]), __float_as_uint(s.fy[from]), 0u, 0u);
+        if (has_prev) {
+            o.pos.z = __float_as_uint(s.pfx[from]);
+            o.pos.w = __float_as_uint(s.pfy[from]);
+        }
+    } else {
+        const double x = s.tp->sx[from], y = s.tp->sy[from];
+        o.pos = make_uint4((uint32_t)__double2loint(x), (uint32_t)__double2hiint(x), (uint32_t)__double2loint(y),
+                           (uint32_t)__double2hiint(y));
+    }
+    o.pl = s.splf[from];
+    o.p = s.spid[from];
+    o.pc = make_uint4(kNone, kNone, kNone, 0u);
+    if (want_pc) o.pc = cand_at(s.tp->cand_prev, s.sprev[from], s.n, s.tp->prev_tick);
     if (s.sgeo) {   // geography keeps every contact (no candidate lists): the slot holds the longitude scale
         const unsigned long long f = __double_as_longlong(s.sgeo[from]);
-        t.pc[k] = make_uint4((uint32_t)f, (uint32_t)(f >> 32), 0u, 0u);
+        o.pc = make_uint4((uint32_t)f, (uint32_t)(f >> 32), 0u, 0u);
     }
+    o.cc = cand ? cand_read(s.tp->cand, from, s.tp->tick) : make_uint4(kNone, kNone, kNone, 0u);
+    return o;
 }
 
 __device__ __forceinline__ double tile_geo(const uint4 &pc)
@@ -2072,7 +2094,10 @@ __device__ __forceinline__ uint32_t spread3(uint32_t v) { return (v & 1u) | ((v 
 // the box of every kGroup consecutive slots: a wave task then skips the
 // column groups whose box is out of reach of its rows' box.  With cand, the
 // candidate lists written at this tick go to ccand/cmask in the same order.
-// Every thread of the block calls it; it ends with the tile complete.
+// Each thread reads all of its objects' fields at once (one round of loads,
+// plus the candidate list at prev_tick behind its sorted index) and keeps
+// them in registers through the sort.  Every thread of the block calls it;
+// it ends with the tile complete.
 template <int TILE>
 __device__ __forceinline__ void load_tile_sorted(JoinTile<TILE> &t, const DevState &s, uint32_t from, uint32_t n,
                                                  bool has_prev, bool use32, double lx, double ly, double w,
@@ -2080,9 +2105,14 @@ __device__ __forceinline__ void load_tile_sorted(JoinTile<TILE> &t, const DevSta
 {
     constexpr int kPer = (TILE + kBlock - 1) / kBlock;
     const bool want_pc = has_prev && !s.store_all;
-    const double *sx = s.tp->sx, *sy = s.tp->sy;
     const double scale = w > 0.0 ? 8.0 / w : 0.0;
     if (threadIdx.x < kSubCells) hist[threadIdx.x] = 0;
+    TileObj obj[kPer];
+#pragma unroll
+    for (int q = 0; q < kPer; q++) {
+        const uint32_t k = threadIdx.x + q * kBlock;
+        if (k < n) obj[q] = tile_fetch(s, from + k, has_prev, use32, want_pc, cand);
+    }
     __syncthreads();
     uint32_t cell[kPer], rank[kPer];
 #pragma unroll
@@ -2091,8 +2121,8 @@ __device__ __forceinline__ void load_tile_sorted(JoinTile<TILE> &t, const DevSta
         cell[q] = 0;
         if (k < n) {
             // the cell only orders the tile; any value is correct
-            const double ox = use32 ? (double)s.fx[from + k] : sx[from + k] - lx;
-            const double oy = use32 ? (double)s.fy[from + k] : sy[from + k] - ly;
+            const double ox = use32 ? (double)obj[q].fx() : obj[q].x() - lx;
+            const double oy = use32 ? (double)obj[q].fy() : obj[q].y() - ly;
             const uint32_t cx = (uint32_t)fmin(fmax(ox * scale, 0.0), 7.0);
             const uint32_t cy = (uint32_t)fmin(fmax(oy * scale, 0.0), 7.0);
             cell[q] = spread3(cx) | (spread3(cy) << 1);
@@ -2110,8 +2140,13 @@ __device__ __forceinline__ void load_tile_sorted(JoinTile<TILE> &t, const DevSta
         const uint32_t k = threadIdx.x + q * kBlock;
         if (k < n) {
             const uint32_t d = hist[cell[q]] + rank[q];
-            tile_put(t, d, s, from + k, has_prev, use32, want_pc);
-            if (cand) cmask[d] = cand_mask(ccand[d] = cand_read(s.tp->cand, from + k, s.tp->tick));
+            // (the same 16 bytes either way)
+            reinterpret_cast<uint4 *>(t.fp)[d] = obj[q].pos;
+            t.g[d] = from + k;
+            t.pl[d] = obj[q].pl;
+            t.p[d] = obj[q].p;
+            if (want_pc || s.sgeo) t.pc[d] = obj[q].pc;
+            if (cand) cmask[d] = cand_mask(ccand[d] = obj[q].cc);
         }
     }
     __syncthreads();
@@ -2639,40 +2674,49 @@ __device__ __forceinline__ void join_body(const DevState s, const P blk)
         dups += jc.dup;
     };
 
-    for (;;) {
-        if (threadIdx.x == 0) {
-            const uint32_t w = atomicAdd(&s.ctr->work_inside, 1u);
-            lds.item = w;
-            if (w < nl) {
-                const uint32_t leaf = s.leaf_list[w < nh ? s.node_cap - 1 - w : w - nh];
-                JoinLeaf L;
-                L.off = s.offset[leaf];
-                L.cnt = s.count[leaf];
-                L.uoff = s.ustart[leaf];
-                L.ucnt = s.ucount[leaf];
-                const double2 lo = reinterpret_cast<const double2 *>(s.nodes + leaf)[1];
-                const double2 hi = node_high(s.nodes, leaf);
-                L.lx = lo.x;
-                L.ly = lo.y;
-                L.w = fmax(hi.x - lo.x, hi.y - lo.y);
-                L.use32 = s.precision == GLINT_PRECISION_FP32_GUARD && (hi.x - lo.x) <= s.w32_max &&
-                          (hi.y - lo.y) <= s.w32_max;
-                L.node = leaf;
-                lds.leaf = L;
-            }
+    // thread 0: takes the next leaf from the queue and leaves it in LDS
+    auto fetch_leaf = [&]() {
+        const uint32_t w = atomicAdd(&s.ctr->work_inside, 1u);
+        lds.item = w;
+        if (w < nl) {
+            const uint32_t leaf = s.leaf_list[w < nh ? s.node_cap - 1 - w : w - nh];
+            JoinLeaf L;
+            L.off = s.offset[leaf];
+            L.cnt = s.count[leaf];
+            L.uoff = s.ustart[leaf];
+            L.ucnt = s.ucount[leaf];
+            const double2 lo = reinterpret_cast<const double2 *>(s.nodes + leaf)[1];
+            const double2 hi = node_high(s.nodes, leaf);
+            L.lx = lo.x;
+            L.ly = lo.y;
+            L.w = fmax(hi.x - lo.x, hi.y - lo.y);
+            L.use32 = s.precision == GLINT_PRECISION_FP32_GUARD && (hi.x - lo.x) <= s.w32_max &&
+                      (hi.y - lo.y) <= s.w32_max;
+            L.node = leaf;
+            lds.leaf = L;
         }
-        __syncthreads();
+    };
+    if (threadIdx.x == 0) fetch_leaf();
+    __syncthreads();
+    for (;;) {
         const uint32_t item = lds.item;
         const JoinLeaf F = lds.leaf;
         __syncthreads();   // read by every thread before thread 0 takes the next leaf
         if (item >= nl) break;
-        // the leaf tile pair by tile pair (one tile when it fits); border
-        // units once per tile of the leaf, against that tile
+        // The next leaf is taken once this one's first tile is loaded: thread
+        // 0 waits on the queue and the leaf's record while the other waves
+        // start on the tasks (read after the last barrier of this leaf).
+        if (F.cnt == 0) {
+            if (threadIdx.x == 0) fetch_leaf();
+            __syncthreads();
+            continue;
+        }
         const uint32_t ntile = (F.cnt + TILE - 1) / TILE;
         for (uint32_t I = 0; I < ntile; I++) {
             const uint32_t ni = min((uint32_t)TILE, F.cnt - I * TILE);
             load_tile_sorted(lds.a, s, F.off + I * TILE, ni, has_prev, F.use32, F.lx, F.ly, F.w, !s.store_all,
                              lds.ccand, lds.cmask, lds.hist);
+            if (I == 0 && threadIdx.x == 0) fetch_leaf();
             for (uint32_t J = I; J < ntile; J++) {
                 const uint32_t nj = min((uint32_t)TILE, F.cnt - J * TILE);
                 const bool diag = (J == I);
@@ -2752,7 +2796,7 @@ __device__ __forceinline__ void join_body(const DevState s, const P blk)
 template <int TILE, bool GEO = false>
 __global__ __launch_bounds__(kBlock) void k_join(DevState s) { join_body<TILE, GEO>(s, this_blk()); }
 template <int TILE, bool GEO = false>
-__global__ __launch_bounds__(kBlock) void kf_join(const Many m)
+__global__ __launch_bounds__(kBlock, 4) void kf_join(const Many m)   // (4 waves per SIMD: at most 128 VGPRs)
 {
     const Member w = member_of(m);
     join_body<TILE, GEO>(m.ss[w.i], w.blk);
