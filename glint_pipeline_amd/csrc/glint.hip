@@ -2147,6 +2147,11 @@ __device__ __forceinline__ void load_tile_sorted(JoinTile<TILE> &t, const DevSta
             t.p[d] = obj[q].p;
             if (want_pc || s.sgeo) t.pc[d] = obj[q].pc;
             if (cand) cmask[d] = cand_mask(ccand[d] = obj[q].cc);
+        } else if (k < (n + kGroup - 1) / kGroup * kGroup) {
+            // the rest of the last group: positions no distance test accepts
+            reinterpret_cast<uint4 *>(t.fp)[k] =
+                use32 ? make_uint4(0x7F800000u, 0x7F800000u, 0x7F800000u, 0x7F800000u)
+                      : make_uint4(0u, 0x7FF00000u, 0u, 0x7FF00000u);
         }
     }
     __syncthreads();
@@ -2360,9 +2365,10 @@ __device__ __forceinline__ JoinCounts join_columns(Stager &st, const DevState &s
     // (uniform flags as all-or-nothing masks: the loop has no branch on them)
     const bool stage_all_hits = !settle && (BORDER || s.store_all);
     const unsigned long long mall = stage_all_hits ? ~0ull : 0ull;
-    const unsigned long long mprev = has_prev ? ~0ull : 0ull;
     const unsigned long long msettle = settle ? ~0ull : 0ull;
-    const unsigned long long mfree = __builtin_amdgcn_ballot_w64(rovf) & msettle;
+    // sure hits staged whatever their previous offsets (the rows' previous
+    // offsets are infinite without a tick before: never within reach)
+    const unsigned long long mfix = mall | (__builtin_amdgcn_ballot_w64(rovf) & msettle);
     // one round: columns jb .. jb+kFlushRound-1 (below jlim; ROT: rotation steps)
     auto round = [&](uint32_t jb, uint32_t jlim) {
         TileF cfp[kFlushRound];
@@ -2401,7 +2407,9 @@ __device__ __forceinline__ JoinCounts join_columns(Stager &st, const DevState &s
                 const uint32_t d = jb + jj;
                 mcand = d < dmax ? mrot_all : (d == dmax ? mrot_top : 0ull);
             } else {
-                mcand = jb + jj < jlim ? ~0ull : 0ull;
+                // slots past the tile's objects hold infinite positions (no
+                // distance test passes) up to the end of their group
+                mcand = GEO ? (jb + jj < jlim ? ~0ull : 0ull) : ~0ull;
             }
             unsigned long long msure, mstage;
             if constexpr (USE32) {
@@ -2415,7 +2423,7 @@ __device__ __forceinline__ JoinCounts join_columns(Stager &st, const DevState &s
                 const float py = pfyi - cfp[jj].pf.y;
                 const float p32 = dist2_f32(px, py);
                 // NaN (unbounded offsets) compares false: may be tagged
-                unsigned long long mkeep = mall | mfree | (__builtin_amdgcn_ballot_w64(!(p32 > t32_out)) & mprev);
+                unsigned long long mkeep = mfix | __builtin_amdgcn_ballot_w64(!(p32 > t32_out));
                 if constexpr (BORDER) mkeep |= __builtin_amdgcn_ballot_w64((cm[jj] & rbit) != 0u) & msettle;
                 mstage = mhit & (~msure | mkeep);
             } else {
@@ -2581,6 +2589,7 @@ __device__ __forceinline__ void join_body(const DevState s, const P blk)
                 }
             }
         }
+        if (!has_prev) pfxi = pfyi = __builtin_inff();   // (the tile's previous offsets are 0)
         if (!vi) {   // an idle lane: no column is within reach
             xi = yi = __builtin_inf();
             fxi = fyi = __builtin_inff();
