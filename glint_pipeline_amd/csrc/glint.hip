@@ -2284,38 +2284,57 @@ __device__ __forceinline__ JoinCounts join_columns(Stager &st, const DevState &s
             rfy = __shfl(fyi, (int)rl);
         }
         if (!valid) return kEmptyKey;
+        // every LDS read of the entry at once (one round trip)
+        const uint32_t pa = rows.p[rl], pb = C.p[j];
+        TileF cf = TileF{make_float2(0.f, 0.f), make_float2(0.f, 0.f)};
+        float2 rpf = make_float2(0.f, 0.f);
+        if (USE32) {
+            cf = C.fp[j];
+            if (has_prev) rpf = rows.pf[rl];
+        }
+        uint32_t la = kNone, lb = kNone;
+        uint4 cb = make_uint4(kNone, kNone, kNone, 0u);
+        if (tested_check) {
+            la = rows.ppl[rl];
+            lb = C.pl[j];
+            cb = C.pc[j];
+        }
+        uint4 qc = make_uint4(kNone, kNone, kNone, 0u);
+        uint32_t pl = 0;
+        if (BORDER && !s.store_all) {
+            qc = ccand[j];
+            pl = rows.pl[rl];
+        }
         bool in_band = false;
         if (USE32) {
-            const float2 cf = C.fp[j].f;
-            const float dx = rfx - cf.x;
-            const float dy = rfy - cf.y;
+            const float dx = rfx - cf.f.x;
+            const float dy = rfy - cf.f.y;
             const float s32 = dist2_f32(dx, dy);
             in_band = !(s32 <= t32_in);
         }
         if (in_band) {
             band++;
-            const uint32_t pa = rows.p[rl], gj = C.g[j];
+            const uint32_t gj = C.g[j];
             const double ex = tv.xy[2 * (size_t)pa] - tv.sx[gj];
             const double ey = tv.xy[2 * (size_t)pa + 1] - tv.sy[gj];
             if (!(ex * ex + ey * ey <= t64)) return kEmptyKey;
             n_pass++;
         }
-        unsigned long long k = pair_key(rows.p[rl], C.p[j]) | flag;
+        unsigned long long k = pair_key(pa, pb) | flag;
         if (has_prev) {
             // within reach at prev_tick: decided on the FP32 offsets outside
             // the guard band, else by the FP64 test
             int dec = -1;
             if (USE32) {
-                const float2 a = rows.pf[rl], b = C.fp[j].pf;
-                const float px = a.x - b.x;
-                const float py = a.y - b.y;
+                const float px = rpf.x - cf.pf.x;
+                const float py = rpf.y - cf.pf.y;
                 const float p32 = dist2_f32(px, py);
                 dec = p32 <= t32_in ? 1 : (p32 > t32_out ? 0 : -1);   // NaN: -1
             }
             if (GEO) {
                 dec = 1;   // every contact is kept and looked up (the scaled test is not symmetric)
             } else if (dec < 0) {
-                const uint32_t pa = rows.p[rl], gj = C.g[j];
+                const uint32_t gj = C.g[j];
                 const double *xp = tv.xy_prev;
                 const double dx = xp[2 * (size_t)pa] - s.sxp[gj];
                 const double dy = xp[2 * (size_t)pa + 1] - s.syp[gj];
@@ -2327,13 +2346,9 @@ __device__ __forceinline__ JoinCounts join_columns(Stager &st, const DevState &s
             // tested by the join of the tick before: both objects in one leaf
             // then, or one's lookup selected the other's leaf (candidate
             // lists that overflowed stored the pair, so they do not count)
-            const uint32_t la = rows.ppl[rl], lb = C.pl[j];
-            bool tested = la == lb && la != kNone;   // kNone: no leaf known at the tick before
-            if (!tested) {
-                const uint4 cb = C.pc[j];
-                tested = ca.w <= 3 && cb.w <= 3 &&
-                         (ca.x == lb || ca.y == lb || ca.z == lb || cb.x == la || cb.y == la || cb.z == la);
-            }
+            const bool tested = (la == lb && la != kNone) ||   // kNone: no leaf known at the tick before
+                                (ca.w <= 3 && cb.w <= 3 &&
+                                 (ca.x == lb || ca.y == lb || ca.z == lb || cb.x == la || cb.y == la || cb.z == la));
             if (tested) k |= kTagTested;
         }
         if (s.store_all || (k & kTagPrev)) return k;
@@ -2342,8 +2357,6 @@ __device__ __forceinline__ JoinCounts join_columns(Stager &st, const DevState &s
         // not stored unless a candidate list overflowed.  It is found twice
         // iff q's lookup selected p's leaf as well; that pair is counted as
         // a second copy once, by the side whose row leaf is the smaller.
-        const uint4 qc = ccand[j];
-        const uint32_t pl = rows.pl[rl];
         if ((pl >> 31) || qc.w > 3) return k;
         const uint32_t lp = pl & 0x7FFFFFFFu;
         const bool twice = qc.x == lp || qc.y == lp || qc.z == lp;
