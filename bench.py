@@ -115,26 +115,30 @@ def profiled_traffic(kernel: str, workload: str, precision: int, objects: int) -
 VALU_PEAK = 256 * 2.4e9   # wave64 VALU instructions per second: one per CU and cycle, 256 CUs at 2.4 GHz
 
 
-def profiled_valu(kernel: str, workload: str, precision: int, objects: int) -> tuple[float | None, str]:
+def profiled_valu(kernel: str, workload: str, precision: int,
+                  objects: int) -> tuple[float | None, float | None, str]:
     """VALU wave instructions per launch of `kernel` from the committed
     rocprofv3 counter summary (SQ_INSTS_VALU), if it measured the kernel
-    sources built now."""
+    sources built now, with the pair slots per launch of the ticks it
+    measured (None if not recorded)."""
     from glint_pipeline_amd.build import source_digest
     tag = f"valu_{workload}_{objects // 1_000_000}m_{'fp32guard' if precision else 'fp64'}.txt"
     f = ROOT / "profiles" / tag
     if not f.exists():
         return None, f"no counter summary profiles/{tag}"
-    found, sha = None, None
+    found, slots, sha = None, None, None
     for line in f.read_text().splitlines():
         parts = line.split()
         if len(parts) == 3 and parts[0] == "VALU_INSTS_PER_LAUNCH" and parts[1] == kernel:
             found = float(parts[2])
+        if len(parts) == 3 and parts[0] == "PAIR_SLOTS_PER_LAUNCH" and parts[1] == kernel:
+            slots = float(parts[2])
         if len(parts) == 2 and parts[0] == "SOURCE_SHA":
             sha = parts[1]
     cur = source_digest()
     if sha != cur:
-        return None, f"profiles/{tag} measured kernel sources {sha}, built sources are {cur}"
-    return found, f"profiles/{tag} (SQ_INSTS_VALU; sources {cur})"
+        return None, None, f"profiles/{tag} measured kernel sources {sha}, built sources are {cur}"
+    return found, slots, f"profiles/{tag} (SQ_INSTS_VALU; sources {cur})"
 
 
 # the tick's kernels by the reference's stages (filter = partition and
@@ -412,10 +416,16 @@ def main() -> int:
             # the join's own bound is instruction issue: VALU wave instructions
             # per launch (committed counter summary) against the issue peak,
             # and the candidate pairs it tests per second
-            vi, vnote = profiled_valu(dom, args.workload, precision, N)
+            vi, vslots, vnote = profiled_valu(dom, args.workload, precision, N)
+            slots = totals["evaluated"] / max(1, totals["ticks"])
+            if vi and vslots:
+                # the profile's ticks differ from this run's: the instruction
+                # count scales with the pair slots evaluated
+                vi = vi * slots / vslots
+                vnote += f"; scaled by this run's pair slots per launch ({slots:.4g} against {vslots:.4g})"
             roofline["compute"] = {
                 "bound": "valu issue", "unit": "wave instructions/s", "peak": VALU_PEAK,
-                "insts_per_launch": vi, "achieved": round(vi / avg_s, 1) if vi else None,
+                "insts_per_launch": round(vi) if vi else None, "achieved": round(vi / avg_s, 1) if vi else None,
                 "frac": round(vi / avg_s / VALU_PEAK, 4) if vi else None, "source": vnote,
                 "pair_tests_per_s": round(totals["tests"] / max(1, totals["ticks"]) / avg_s, 1),
                 "pair_slots_evaluated_per_s": round(totals["evaluated"] / max(1, totals["ticks"]) / avg_s, 1)}
