@@ -125,7 +125,7 @@ def profiled_valu(kernel: str, workload: str, precision: int,
     tag = f"valu_{workload}_{objects // 1_000_000}m_{'fp32guard' if precision else 'fp64'}.txt"
     f = ROOT / "profiles" / tag
     if not f.exists():
-        return None, f"no counter summary profiles/{tag}"
+        return None, None, f"no counter summary profiles/{tag}"
     found, slots, sha = None, None, None
     for line in f.read_text().splitlines():
         parts = line.split()
