@@ -34,7 +34,7 @@ def test_cpp_pipeline_driver_matches_oracle(n, ticks, side, spots, cap, min_meet
     got = {int(t): (int(m), int(d, 16)) for t, m, d in
            re.findall(r"^tick (\d+) meetings (\d+) digest ([0-9a-f]+)", out.stdout, re.M)}
     assert sorted(got) == list(range(ticks))
-    # the driver's generator: seed 1, hotspots at 80% with sigma 200 m when asked
+    # the C++ program's generator: seed 1, hotspots at 80% with sigma 200 m when asked
     w = gen.Workload("driver", n, ticks, side, seed=1, num_hotspots=spots, hotspot_fraction=0.8 if spots else 0.0,
                      hotspot_sigma=200.0, grid_capacity=cap, min_meet_time=min_meet, reach_distance=reach)
     g = gen.Generator(w)
@@ -44,6 +44,6 @@ def test_cpp_pipeline_driver_matches_oracle(n, ticks, side, spots, cap, min_meet
         o.tick(g.positions() if t == 0 else g.step(), t)
         m = o.meetings()
         total += len(m)
-        assert got[t] == (len(m), digest(m)), f"tick {t}: the driver's meetings differ from the oracle's"
+        assert got[t] == (len(m), digest(m)), f"tick {t}: the C++ program's meetings differ from the oracle's"
     o.close()
     assert total > 20
