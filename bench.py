@@ -620,7 +620,10 @@ def bench_serving(args) -> int:
     H, K = args.history, args.steps
     w = gen.uniform_city(N, H + K + 1, gen.city_side(N), seed=7, grid_capacity=200)
     w.move_fraction = 0.1
-    params = autotune.params_for(autotune.size_class(N))
+    # the tuned parameters of the size class, with the FP32 guard-band test
+    params = dict(autotune.params_for(autotune.size_class(N)), precision="fp32guard")
+    if args.precision:
+        params["precision"] = args.precision
     cfg = autotune.config_for(w, params, distinct_estimate("batched", N), graph_ticks=0,
                               meeting_log_capacity=1 << 29)
     g = gen.Generator(w)

@@ -83,7 +83,8 @@ def test_serving_20m_resident_matches_fixture():
     fx = json.loads(FIXTURE.read_text())["serving"]
     w = serving_workload()
     N = w.num_objects
-    params = autotune.params_for(autotune.size_class(N))
+    # the serving bench's parameters: the size class's, with the FP32 guard band
+    params = dict(autotune.params_for(autotune.size_class(N)), precision="fp32guard")
     cfg = autotune.config_for(w, params, autotune.pairs_estimate("batched", N), graph_ticks=0,
                               meeting_log_capacity=1 << 22)
     g = gen.Generator(w)
