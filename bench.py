@@ -166,24 +166,27 @@ def gpu_stages(ktimes: dict, ticks: int) -> dict:
 
 
 def cpu_baseline(w, ticks: int, threads: int, min_seconds: float = 10.0, max_ticks: int = 5000,
-                 schema: str = "dynamic") -> dict:
+                 schema: str = "dynamic", geo: "LonLat | None" = None) -> dict:
     """The CPU oracle (C + OpenMP) on the first ticks of the same workload:
     at least `ticks` ticks, and more (the walk continues) until about
-    `min_seconds` of CPU work is timed."""
+    `min_seconds` of CPU work is timed.  With geo, the positions are mapped
+    to degrees and the oracle runs the geography predicates."""
     from glint_pipeline_amd import autotune, gen
     from oracle.oracle import Oracle
     g = gen.Generator(w)
-    o = Oracle(w.num_objects, autotune.box_for(w, schema), w.reach_distance, w.grid_capacity, w.min_meet_time,
-               num_threads=threads, dynamic_schema=autotune.SCHEMAS[schema])
+    box = geo.box(w) if geo else autotune.box_for(w, schema)
+    o = Oracle(w.num_objects, box, w.reach_distance, w.grid_capacity, w.min_meet_time,
+               num_threads=threads, dynamic_schema=autotune.SCHEMAS[schema], geography=geo is not None)
+    conv = geo.to_degrees if geo else (lambda xy: xy)
     t_build0 = time.perf_counter()
-    o.tick(g.positions(), 0)          # tick 0 builds the tree (not timed, as on the GPU side)
+    o.tick(conv(g.positions()), 0)    # tick 0 builds the tree (not timed, as on the GPU side)
     t_build = time.perf_counter() - t_build0
     tt = 0.0
     t = 0
     stages = {"filter": 0.0, "reachability": 0.0, "update_meetings": 0.0, "update_schema": 0.0}
     while t < max_ticks and (t < ticks or tt < min_seconds):
         t += 1
-        xy = g.step()
+        xy = conv(g.step())
         t0 = time.perf_counter()
         o.tick(xy, t)
         tt += time.perf_counter() - t0
@@ -195,6 +198,29 @@ def cpu_baseline(w, ticks: int, threads: int, min_seconds: float = 10.0, max_tic
     o.close()
     return {"seconds": tt, "ticks": t, "tick0_seconds": t_build, "contacts_last_tick": st["num_contacts"],
             "stages_ms_per_tick": {k: round(v * 1e3 / t, 3) for k, v in stages.items()}}
+
+
+class LonLat:
+    """A walk in metres mapped to longitude/latitude degrees near (lon0,
+    lat0), the input of the reference's geography=true predicates; the reach
+    and the split rules stay in metres.  The degrees per metre are the
+    reference's expressions (util.h:57-66, PI = 3.14159265)."""
+
+    def __init__(self, lon0: float = -87.63, lat0: float = 41.88):
+        self.lon0, self.lat0 = lon0, lat0
+        self.fx = 360.0 / (math.sin((90.0 - abs(lat0)) * 3.14159265 / 180.0) * 40076.0 * 1000.0)
+        self.fy = 360.0 / (40076.0 * 1000.0)
+
+    def to_degrees(self, xy: np.ndarray) -> np.ndarray:
+        ll = np.array(xy, dtype=np.float64).reshape(-1, 2)
+        ll[:, 0] = self.lon0 + ll[:, 0] * self.fx
+        ll[:, 1] = self.lat0 + ll[:, 1] * self.fy
+        return ll.reshape(np.shape(xy))
+
+    def box(self, w) -> tuple:
+        # the walk's box with 1 m of margin, in degrees
+        return (self.lon0 - self.fx, self.lat0 - self.fy, self.lon0 + (w.side + 1) * self.fx,
+                self.lat0 + (w.side + 1) * self.fy)
 
 
 WORKLOAD_SPEC = {
@@ -222,6 +248,8 @@ def main() -> int:
                     help="serving: run ticks back to back with 50 queries per tick (a stress test) instead of "
                          "one tick per wall-clock second with Poisson arrivals at 50/s")
     ap.add_argument("--objects", type=int, default=None)
+    ap.add_argument("--geography", action="store_true",
+                    help="positions mapped to lon/lat degrees; the reference's geography=true predicates (FP64)")
     ap.add_argument("--precision", default=None, choices=["fp64", "fp32guard"],
                     help="default: the workload's configuration (skewed: fp32guard; plain forms: fp64)")
     ap.add_argument("--graph-ticks", type=int, default=None, help="ticks per captured hipGraph (default: per workload)")
@@ -282,6 +310,16 @@ def main() -> int:
     if total_ticks > 1:
         dw.walk(total_ticks - 1, trace, 16 * N)
     log(f"[bench] generated {total_ticks} ticks of positions on the device in {time.perf_counter() - t0:.1f} s")
+    geo = LonLat() if args.geography else None
+    if geo:
+        # every tick's positions mapped to degrees in place (FP64 predicates:
+        # the guard band is defined on planar offsets)
+        host = np.empty((N, 2), dtype=np.float64)
+        for t in range(total_ticks):
+            trace.download(host, 16 * N * t)
+            trace.upload(geo.to_degrees(host), 16 * N * t)
+        desc += f"; positions in lon/lat degrees near ({geo.lon0}, {geo.lat0}), the geography predicates"
+        args.no_forms = True
 
     tuning = None
     if args.autotune:
@@ -300,6 +338,8 @@ def main() -> int:
                  ("load_factor", args.load_factor), ("table_threads", args.table_threads), ("schema", args.schema)):
         if v is not None:
             params[k] = v
+    if geo:
+        params["precision"] = "fp64"
     precision = runtime.PRECISION_FP32_GUARD if params["precision"] == "fp32guard" else runtime.PRECISION_FP64
     w.grid_capacity = params.get("grid_capacity", w.grid_capacity)
     desc = desc.replace("leaf capacity 200", f"leaf capacity {w.grid_capacity}")
@@ -307,7 +347,10 @@ def main() -> int:
     def make_cfg(prec: int, graph: int, **extra):
         p = dict(params)
         p["precision"] = "fp32guard" if prec == runtime.PRECISION_FP32_GUARD else "fp64"
-        return autotune.config_for(w, p, distinct, graph_ticks=graph, **extra)
+        c = autotune.config_for(w, p, distinct, graph_ticks=graph, geography=geo is not None, **extra)
+        if geo:
+            c.low_x, c.low_y, c.high_x, c.high_y = geo.box(w)
+        return c
 
     cfg = make_cfg(precision, graph_ticks)
     slots = int(cfg.num_meeting_buckets)
@@ -338,7 +381,7 @@ def main() -> int:
     # W..W+K-1, the meeting log on.  Its totals over the timed ticks must equal
     # the graph run's, and where the committed oracle fixture covers this
     # workload, every fixture tick's meetings must equal the oracle's.
-    fixture = headline_fixture(args.workload, w, params)
+    fixture = None if geo else headline_fixture(args.workload, w, params)
     fx_ticks = min(len(fixture["ticks"]), W + K) if fixture else 0
     wv = runtime.Workbench(make_cfg(precision, 0, meeting_log_capacity=(64 << 20) if fixture else 0))
     verify = {"graph_vs_plain": None, "fixture_ticks": 0}
@@ -508,7 +551,7 @@ def main() -> int:
     if not args.no_cpu_baseline:
         threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
         t0 = time.perf_counter()
-        cb = cpu_baseline(w, args.cpu_ticks, threads, schema=params.get("schema", "dynamic"))
+        cb = cpu_baseline(w, args.cpu_ticks, threads, schema=params.get("schema", "dynamic"), geo=geo)
         log(f"[bench] cpu baseline {cb} (wall {time.perf_counter() - t0:.1f} s)")
         cpu = {"value": round(N * cb["ticks"] / cb["seconds"], 1), "unit": "object-updates/s", "cores": threads,
                "kind": "port", "cpu": cpu_model(), "stages_ms_per_tick": cb["stages_ms_per_tick"],
