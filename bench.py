@@ -248,6 +248,8 @@ def main() -> int:
                     help="serving: run ticks back to back with 50 queries per tick (a stress test) instead of "
                          "one tick per wall-clock second with Poisson arrivals at 50/s")
     ap.add_argument("--objects", type=int, default=None)
+    ap.add_argument("--snapshot", action="store_true",
+                    help="after the timed run, time a snapshot of the tick state to host memory and its restore")
     ap.add_argument("--geography", action="store_true",
                     help="positions mapped to lon/lat degrees; the reference's geography=true predicates (FP64)")
     ap.add_argument("--precision", default=None, choices=["fp64", "fp32guard"],
@@ -373,6 +375,24 @@ def main() -> int:
     totals = wb.totals()
     stats = wb.stats()
     log(f"[bench] timed {K} ticks in {elapsed:.3f} s; totals {totals}")
+    snap = None
+    if args.snapshot:
+        # the tick-boundary state to host memory and into a fresh context of
+        # the same configuration (the serving loop's dump and reload)
+        t0 = time.perf_counter()
+        buf = wb.snapshot()
+        t_snap = time.perf_counter() - t0
+        wr = runtime.Workbench(make_cfg(precision, graph_ticks))
+        t0 = time.perf_counter()
+        wr.restore(buf)
+        wr.synchronize()
+        t_rest = time.perf_counter() - t0
+        same = wr.active_pairs()[0].size == wb.active_pairs()[0].size
+        wr.close()
+        snap = {"bytes": int(buf.size), "snapshot_ms": round(t_snap * 1e3, 1), "restore_ms": round(t_rest * 1e3, 1),
+                "host_GBps": round(buf.size / max(t_snap, 1e-9) / 1e9, 2), "same_active_pairs": bool(same)}
+        log(f"[bench] snapshot {snap}")
+        del buf
     wb.close()
 
     # ---- verification and per-kernel times: a second run of the same inputs
@@ -603,6 +623,8 @@ def main() -> int:
         out["forms"] = forms
     if kv:
         out["kv_output"] = kv
+    if snap:
+        out["snapshot"] = snap
     print(json.dumps(out), flush=True)
     trace.free()
     return 0
