@@ -1810,9 +1810,9 @@ __global__ __launch_bounds__(kBlock) void kf_unit_scatter(const Many m)
 // ------------------------------------------------------------ meeting table
 //
 // The table is an open-addressed hash of the pair key (pid_lo << 32) | pid_hi
-// in two generations of B 16-byte slots.  h = mix64(key): the high bits pick
-// a region of kRegion slots, the low bits the home slot inside it, and linear
-// probing wraps inside the region.  A tick's contacts are bucketed by region
+// in two generations of B 16-byte slots.  h = pair_hash(key): the high bits pick
+// a region of kRegion slots, the 11 bits below them the home slot inside it,
+// and linear probing wraps inside the region.  A tick's contacts are bucketed by region
 // and every region is then updated by one workgroup in LDS: the region of the
 // previous generation is read once, the new region is written once.
 
@@ -1827,10 +1827,21 @@ __device__ __forceinline__ unsigned long long *contact_out(const DevState &s)
 
 // Region of a pair in a table of 2^bits regions: the high bits of its hash
 // (the low bits pick its slot in the region's LDS hash).
+__device__ __forceinline__ unsigned long long pair_hash(unsigned long long key)
+{
+    return key * 0x9E3779B97F4A7C15ull;   // multiplicative: every key bit reaches the high bits
+}
 __device__ __forceinline__ uint32_t region_of(unsigned long long key, uint32_t bits)
 {
     // a one-region table has no region bits (a shift by 64 is undefined)
-    return bits ? (uint32_t)(mix64(key) >> (64 - bits)) : 0u;
+    return bits ? (uint32_t)(pair_hash(key) >> (64 - bits)) : 0u;
+}
+// Home slot of a pair in its region's LDS hash: the kRegion-bit field of
+// the hash just below the region bits.
+static_assert(kRegion == 2048, "slot_of takes 11 bits");
+__device__ __forceinline__ uint32_t slot_of(unsigned long long key, uint32_t bits)
+{
+    return (uint32_t)(pair_hash(key) >> (64 - 11 - bits)) & (kRegion - 1);
 }
 
 // The table's shape at this tick, from the contacts the join stored: the
@@ -3201,9 +3212,9 @@ __device__ __forceinline__ void table_flush_stage(const DevState &s, glint_meeti
 // kNoSlot for a second copy of a pair (its start is marked) or a full region.
 template <bool KV, int NT>
 __device__ __forceinline__ uint32_t table_put(TableLds<KV, NT> &L, const DevState &s, const TableGen &tcur,
-                                              unsigned long long key, uint32_t c, uint32_t tick)
+                                              unsigned long long key, uint32_t c, uint32_t tick, uint32_t bits)
 {
-    const uint32_t h = (uint32_t)mix64(key) & (kRegion - 1);
+    const uint32_t h = slot_of(key, bits);
     for (uint32_t p = 0; p < kRegion; p++) {
         const uint32_t q = (h + p) & (kRegion - 1);
         const unsigned long long prev = atomicCAS(&L.hk[q], kEmptyKey, key);
@@ -3223,9 +3234,9 @@ __device__ __forceinline__ uint32_t table_put(TableLds<KV, NT> &L, const DevStat
 }
 
 template <bool KV, int NT>
-__device__ __forceinline__ uint32_t table_find(const TableLds<KV, NT> &L, unsigned long long key)
+__device__ __forceinline__ uint32_t table_find(const TableLds<KV, NT> &L, unsigned long long key, uint32_t bits)
 {
-    const uint32_t h = (uint32_t)mix64(key) & (kRegion - 1);
+    const uint32_t h = slot_of(key, bits);
     for (uint32_t p = 0; p < kRegion; p++) {
         const uint32_t q = (h + p) & (kRegion - 1);
         const unsigned long long k = L.hk[q];
@@ -3240,13 +3251,13 @@ __device__ __forceinline__ uint32_t table_find(const TableLds<KV, NT> &L, unsign
 template <bool KV, int NT>
 __device__ __forceinline__ void table_old(TableLds<KV, NT> &L, const DevState &s, bool valid,
                                           unsigned long long tagged, uint32_t start, uint32_t i, uint32_t tick,
-                                          uint32_t last, glint_meeting *st, float4 *sb, uint32_t &nst)
+                                          uint32_t last, glint_meeting *st, float4 *sb, uint32_t &nst, uint32_t bits)
 {
     valid = valid && start != kDupStart;
     const unsigned long long key = tagged & kKeyBits;
     bool emit = false;
     if (valid) {
-        const uint32_t q = table_find(L, key);
+        const uint32_t q = table_find(L, key, bits);
         if (q != kNoSlot) {
             L.hs[q] = start;
             if constexpr (KV) L.ho[q] = i;
@@ -3402,14 +3413,14 @@ __device__ __forceinline__ void table_body(const DevState s, const P blk)
             const uint32_t c = c0 + threadIdx.x + q * NT;
             ns[q] = kNoSlot;
             if (c < c1 && (nk[q] & (kTagPrev | kTagBorder))) {
-                ns[q] = table_put(L, s, tcur, nk[q] & kKeyBits, c, tick);
+                ns[q] = table_put(L, s, tcur, nk[q] & kKeyBits, c, tick, bits);
                 dups += ns[q] == kNoSlot;
                 if (ns[q] == kNoSlot) ns[q] = kNoSlot - 1;   // nothing left to write
             }
         }
         for (uint32_t c = c0 + kRegion + threadIdx.x; c < c1; c += NT) {   // regions of more than kRegion contacts
             const unsigned long long k = tcur.keys[c];
-            if (k & (kTagPrev | kTagBorder)) dups += table_put(L, s, tcur, k & kKeyBits, c, tick) == kNoSlot;
+            if (k & (kTagPrev | kTagBorder)) dups += table_put(L, s, tcur, k & kKeyBits, c, tick, bits) == kNoSlot;
         }
         lds_barrier();
         // the previous generation's entries of the region
@@ -3418,20 +3429,20 @@ __device__ __forceinline__ void table_body(const DevState s, const P blk)
             for (int q = 0; q < kPer; q++) {
                 const uint32_t i = o0 + threadIdx.x + q * NT;
                 const bool mine = region_of(pk[q] & kKeyBits, bits) == r;
-                table_old(L, s, i < o1 && mine, pk[q], ps[q], i, tick, last, st, sb, nst);
+                table_old(L, s, i < o1 && mine, pk[q], ps[q], i, tick, last, st, sb, nst, bits);
             }
         } else {
 #pragma unroll
             for (int q = 0; q < kPer; q++) {
                 const uint32_t i = o0 + threadIdx.x + q * NT;
-                table_old(L, s, i < o1, pk[q], ps[q], i, tick, last, st, sb, nst);
+                table_old(L, s, i < o1, pk[q], ps[q], i, tick, last, st, sb, nst, bits);
             }
         }
         for (uint32_t b = o0 + kRegion; b < o1; b += NT) {   // regions of more than kRegion entries
             const uint32_t i = b + threadIdx.x;
             const unsigned long long k = i < o1 ? told.keys[i] : 0ull;
             const bool v = i < o1 && (!filter_old || region_of(k & kKeyBits, bits) == r);
-            table_old(L, s, v, k, v ? told.starts[i] : kDupStart, i, tick, last, st, sb, nst);
+            table_old(L, s, v, k, v ? told.starts[i] : kDupStart, i, tick, last, st, sb, nst, bits);
         }
         const uint32_t rn = r + blk.count();
         if (rn < nreg) prefetch(rn);
@@ -3446,7 +3457,7 @@ __device__ __forceinline__ void table_body(const DevState s, const P blk)
             const unsigned long long k = tcur.keys[c];
             uint32_t q = kNoSlot;
             if (k & (kTagPrev | kTagBorder)) {
-                q = table_find(L, k & kKeyBits);
+                q = table_find(L, k & kKeyBits, bits);
                 if (q == kNoSlot || L.hc[q] != c) continue;   // a second copy (already marked)
             }
             table_write(L, s, tcur, k, c, q, tick, last, pair_pos<KV>(s, k));
