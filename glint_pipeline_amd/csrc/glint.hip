@@ -1570,8 +1570,6 @@ __device__ __forceinline__ void scatter_body(const DevState s, const P blk)
         if (has_prev) {
             px = ordered ? sx_old[k] : xy_prev[2 * (size_t)i];
             py = ordered ? sy_old[k] : xy_prev[2 * (size_t)i + 1];
-            s.sxp[pos] = px;
-            s.syp[pos] = py;
         }
         if (xy_save) {
             xy_save[2 * (size_t)i] = x;
@@ -2034,7 +2032,7 @@ struct JoinTile {
         double2 xy[TILE];
         TileF fp[TILE];
     };
-    uint32_t g[TILE];              // index in the sorted arrays (positions at prev_tick: sxp, syp)
+    uint32_t g[TILE];              // index in the sorted arrays
     uint32_t pl[TILE];             // leaf at prev_tick
     uint32_t p[TILE];              // object ids
     uint4 pc[TILE];                // candidate leaves at prev_tick (DevState::prev_cand)
@@ -2248,7 +2246,7 @@ __device__ __forceinline__ uint4 lane_read4(uint4 x, uint32_t src)
 // Positions of the tick read by the write-out, loaded once per kernel (the
 // tick parameters live in device memory and would be re-read after stores).
 struct TickView {
-    const double *xy, *xy_prev, *sx, *sy;
+    const double *xy, *xy_prev, *sx, *sy, *sx_old, *sy_old;
 };
 
 // ROT: a diagonal inside task (rows and columns the same block of n objects
@@ -2345,10 +2343,9 @@ __device__ __forceinline__ JoinCounts join_columns(Stager &st, const DevState &s
             if (GEO) {
                 dec = 1;   // every contact is kept and looked up (the scaled test is not symmetric)
             } else if (dec < 0) {
-                const uint32_t gj = C.g[j];
-                const double *xp = tv.xy_prev;
-                const double dx = xp[2 * (size_t)pa] - s.sxp[gj];
-                const double dy = xp[2 * (size_t)pa + 1] - s.syp[gj];
+                const double *xp = tv.xy_prev;   // (by id: this path is rare)
+                const double dx = xp[2 * (size_t)pa] - xp[2 * (size_t)pb];
+                const double dy = xp[2 * (size_t)pa + 1] - xp[2 * (size_t)pb + 1];
                 dec = dx * dx + dy * dy <= t64;
             }
             if (dec) k |= kTagPrev;
@@ -2548,7 +2545,7 @@ __device__ __forceinline__ void join_body(const DevState s, const P blk)
     const double *xy = s.tp->xy;
     const double *xy_prev = s.tp->xy_prev;
     const bool has_prev = s.tp->has_prev != 0;
-    const TickView tv{xy, xy_prev, s.tp->sx, s.tp->sy};
+    const TickView tv{xy, xy_prev, s.tp->sx, s.tp->sy, s.tp->sx_old, s.tp->sy_old};
     const double t64 = s.t64;
     const uint32_t nh = s.ctr->n_heavy;
     const uint32_t nl = (s.ctr->error & kErrUnitCap) ? 0 : s.ctr->n_leaf_list + nh;
@@ -2598,8 +2595,10 @@ __device__ __forceinline__ void join_body(const DevState s, const P blk)
                 xi = tv.sx[ulf.w];
                 yi = tv.sy[ulf.w];
                 if (has_prev) {
-                    pxi = s.sxp[ulf.w];
-                    pyi = s.syp[ulf.w];
+                    // the sorted positions of the tick before, at the object's
+                    // sorted position then (by id when that is not known)
+                    pxi = ulf.z < s.n ? tv.sx_old[ulf.z] : tv.xy_prev[2 * (size_t)pid];
+                    pyi = ulf.z < s.n ? tv.sy_old[ulf.z] : tv.xy_prev[2 * (size_t)pid + 1];
                 }
                 if (want_pc) rpc = cand_at(s.tp->cand_prev, ulf.z, s.n, s.tp->prev_tick);
             }
@@ -4395,8 +4394,6 @@ glint_status glint_create(const glint_config *cfg, glint_ctx **out)
     ALLOC(s.sxb[1], N);
     ALLOC(s.syb[1], N);
     ALLOC(s.ord_xy, N);
-    ALLOC(s.sxp, N);
-    ALLOC(s.syp, N);
     ALLOC(c->hist[0], 2 * N);
     ALLOC(c->hist[1], 2 * N);
     ALLOC(s.spid, N);

@@ -239,8 +239,6 @@ struct DevState {
     uint32_t *ucount;       // units per leaf
     uint32_t *ustart;       // first unit of each leaf in unit_pid
     double *sxb[2], *syb[2]; // sorted positions of two ticks (TickParams::sx, sx_old)
-    double *sxp;            // positions at prev_tick, sorted like sx/sy
-    double *syp;
     uint32_t *sprev;        // sorted position at prev_tick of each sorted object (kNone: none known)
     uint32_t *spid;
     uint32_t *order;        // object handled by position k of the tick's assignment (spid of the tick before)
