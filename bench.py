@@ -109,7 +109,8 @@ def profiled_traffic(kernel: str, workload: str, precision: int, objects: int) -
     cur = source_digest()
     if sha != cur:
         return None, f"profiles/{tag} measured kernel sources {sha}, built sources are {cur}"
-    return found, f"profiles/{tag} (FETCH_SIZE + WRITE_SIZE, copy-calibrated; sources {cur})"
+    return found, (f"profiles/{tag} (FETCH_SIZE + WRITE_SIZE per launch averaged over the ticks the summary "
+                   f"names, copy-calibrated; sources {cur})")
 
 
 VALU_PEAK = 256 * 2.4e9   # wave64 VALU instructions per second: one per CU and cycle, 256 CUs at 2.4 GHz
