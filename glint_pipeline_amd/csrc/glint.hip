@@ -1877,17 +1877,8 @@ struct Stager {
     uint32_t kept;              // entries written in all (uniform)
 };
 
-// The lanes with a hit append e in lane order (mbcnt counts the hits of the
-// lanes below, from st.n); a column step without any hit writes nothing.
-__device__ __forceinline__ void stage_push(Stager &st, bool hit, uint32_t e)
-{
-    const unsigned long long m = __builtin_amdgcn_ballot_w64(hit);
-    if (m == 0) return;
-    if (hit) st.buf[__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, st.n))] = e;
-    st.n += (uint32_t)__popcll(m);
-}
-
-// The same for a mask of lanes already formed (bit i: lane i appends e).
+// The lanes of mask m (bit i: lane i) append e in lane order (mbcnt counts the
+// lanes of m below each, from st.n); an empty mask writes nothing.
 __device__ __forceinline__ void stage_push_mask(Stager &st, unsigned long long m, uint32_t e)
 {
     if (m == 0) return;
@@ -2408,7 +2399,9 @@ __device__ __forceinline__ JoinCounts join_columns(Stager &st, const DevState &s
             } else {
                 // jb is in a group of kGroup slots that starts at a multiple
                 // of kGroup below TILE, so jb + jj is a slot of the tile; a
-                // slot past jlim is read and its lanes masked out (mcand)
+                // slot past the tile's objects holds an infinite position
+                // (geography, which reads a scale from the slot, clamps the
+                // column and masks its lanes out)
                 jc = GEO ? min(jb + jj, jlim - 1) : jb + jj;
             }
             if constexpr (USE32)
