@@ -38,3 +38,19 @@ def test_more_than_one_gpu_is_refused():
     assert r.returncode != 0
     line = json.loads(r.stdout.strip().splitlines()[-1])
     assert line["n_gpus"] == 2 and "one GPU" in json.dumps(line)
+
+
+def test_lonlat_mapping_uses_the_reference_scales():
+    from oracle.oracle import degree_per_meter_longitude
+    geo = bench.LonLat(-87.63, 41.88)
+    assert geo.fx == degree_per_meter_longitude(41.88)
+    assert geo.fy == 360.0 / (40076.0 * 1000.0)
+    ll = geo.to_degrees([[0.0, 0.0], [1000.0, 2000.0]])
+    assert ll[0].tolist() == [-87.63, 41.88]
+    assert ll[1].tolist() == [-87.63 + 1000.0 * geo.fx, 41.88 + 2000.0 * geo.fy]
+
+    class W:
+        side = 30000.0
+    lx, ly, hx, hy = geo.box(W)
+    assert lx < -87.63 < hx and ly < 41.88 < hy   # the walk's box [0, side] lies inside
+    assert hx > -87.63 + 30000.0 * geo.fx and hy > 41.88 + 30000.0 * geo.fy
