@@ -814,7 +814,9 @@ def bench_batched(args) -> int:
         runtime.DeviceWalkers(g).walk(W + K - 1, b, 16 * w.num_objects)
         bufs.append(b)
         cls = autotune.size_class(w.num_objects)
-        p = autotune.params_for(cls)
+        p = dict(autotune.params_for(cls))
+        if args.precision:
+            p["precision"] = args.precision
         classes.append(cls)
         cfgs.append(autotune.config_for(w, p, distinct_estimate("batched", w.num_objects), graph_ticks=G))
     log(f"[bench] {len(ws)} cities, {total} objects, classes { {c: classes.count(c) for c in set(classes)} }; "
